@@ -1,0 +1,195 @@
+"""Throughput of the force-and-step loop on one MI355X.
+
+Workload: the default brush of MD_Brushes (8x8 linear chains of 100 charged
+beads, counterions and trivalent salt: 21,396 particles, 21,332 charged),
+SPME mesh 64x64x256 order 4, Coulomb every 10 steps, dt = 0.005; one step =
+new_position + update_verlet_list (main.f90:92-94) with Philox noise, all
+state resident in HBM.  Headline: FP32-force steps/s, reported beside the
+plain FP64 path and the FP32-vs-FP64 relative force RMS on the same
+configuration.  The CPU baseline is the FP64 oracle (a serial C restatement
+of the reference) timed on this host for a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from md_brushes_amd import (MD_FP32, MD_FP64, MD_NOISE_PHILOX, Engine, config,  # noqa: E402
+                            initialize_position, initialize_velocity)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def kernel_bytes(p, name):
+    """Algorithmic (compulsory) bytes of one launch: each array the kernel
+    must touch, once.  Neighbour-list storage is excluded."""
+    NN, Nq = p.NN, p.Nq
+    K1, K2, K3 = p.ng
+    M, Mh = K1 * K2 * K3, (K1 // 2 + 1) * K2 * K3
+    f32 = p.precision == MD_FP32
+    r = 4 if f32 else 8
+    if name == "short_range":
+        # pos (32) [+ pos32 16], held Coulomb 3r, v read+write 48, acc 24,
+        # flags 4, list length 4, bond offsets 4, bonded partners 2*4*N_bond/NN
+        per = 32 + (16 if f32 else 0) + 3 * r + 48 + 24 + 12
+        return NN * per + 8 * p.N_bond
+    if name == "kick_drift":
+        return NN * (24 + 48 + 64 + (16 if f32 else 0))
+    if name == "real":
+        return Nq * (4 + 32 + (16 if f32 else 0) + 24 + 4 + 3 * r)
+    if name == "spread":
+        return Nq * (4 + 32) + 8 * M
+    if name == "gather":
+        return Nq * (4 + 32 + 24) + r * M
+    if name in ("fft_x_fwd",):
+        return 8 * M + 2 * r * Mh
+    if name in ("fft_x_inv",):
+        return 2 * r * Mh + r * M
+    if name in ("fft_y_fwd", "fft_y_inv"):
+        return 2 * 2 * r * Mh
+    if name == "fft_z_conv":
+        return 2 * 2 * r * Mh + r * Mh
+    return 0
+
+
+def prepare(seed, equil_steps):
+    """Seeded default brush after FP64 equilibration (FP64 engine)."""
+    p64 = config("default", MD_FP64, seed=seed)
+    pos = initialize_position(p64, seed)
+    vel = initialize_velocity(p64, seed + 1)
+    e = Engine(p64)
+    e.upload(pos, vel)
+    e.build_lists()
+    e.compute_force(0, MD_NOISE_PHILOX)
+    if equil_steps:
+        e.run(1, equil_steps)
+    pos, vel, acc, _ = e.download()
+    e.close()
+    return pos, vel, acc
+
+
+def time_engine(prec, state, steps, warmup, seed):
+    p = config("default", prec, seed=seed)
+    e = Engine(p)
+    pos, vel, acc = state
+    e.upload(pos, vel, acc)
+    e.build_lists()
+    e.compute_force(0, MD_NOISE_PHILOX)
+    e.run(1, warmup)
+    t0 = time.perf_counter()
+    e.run(1 + warmup, steps)  # ends with a stream synchronize
+    dt = time.perf_counter() - t0
+    return p, e, steps / dt, dt
+
+
+def cpu_baseline(state, seed, budget_s):
+    """The FP64 oracle (serial C) on this host: whole multistep cycles until
+    about budget_s seconds have passed."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc  # the oracle: baseline only
+
+    p = config("default", MD_FP64, seed=seed)
+    o = orc.Oracle(p)
+    pos, vel, acc = state
+    o.set_state(pos, vel, acc)
+    o.build_lists()
+    o.compute_force(0, MD_NOISE_PHILOX)
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        rc = o.run(1 + steps, p.multistep)
+        if rc:
+            raise RuntimeError(f"oracle run failed: {rc}")
+        steps += p.multistep
+        el = time.perf_counter() - t0
+        if el > budget_s:
+            break
+    return steps / el, steps, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--equil", type=int, default=1000)
+    ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (replicas shard "
+              "across GPUs with no communication; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    state = prepare(args.seed, args.equil)
+    p32, e32, sps32, dt32 = time_engine(MD_FP32, state, args.steps, args.warmup, args.seed)
+    out = {
+        "metric": "steps/s at 21,333 particles (FP32 vs FP64, force err<=1e-4); replica p50/p99",
+        "value": round(sps32, 2),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * dt32 / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 forces, fp64 state",
+        "data": "synthetic: seeded brush, 1000 FP64 equilibration steps",
+        "config": {"workload": "default brush 8x8 linear N=100 + trivalent salt, NN=21396, Nq=21332, "
+                               "SPME 64x64x256 p=4, multistep 10",
+                   "NN": p32.NN, "Nq": p32.Nq, "mesh": list(p32.ng), "order": p32.ordr[0],
+                   "alpha": round(p32.alpha, 5), "r_c": round(p32.rcc, 4), "multistep": p32.multistep},
+    }
+    # accuracy gate on one configuration: FP32 forces against FP64 forces
+    pos, vel, acc = state
+    e64 = Engine(config("default", MD_FP64, seed=args.seed))
+    e64.upload(pos, vel, acc)
+    e64.build_lists()
+    fs64, fc64 = e64.force_parts()
+    e32b = Engine(p32)
+    e32b.upload(pos, vel, acc)
+    e32b.build_lists()
+    fs32, fc32 = e32b.force_parts()
+    f64, f32 = fs64 + fc64, fs32 + fc32
+    out["fp32_force_rel_rms"] = float(np.sqrt(((f32 - f64) ** 2).sum() / (f64 ** 2).sum()))
+    e32b.close()
+    e64.close()
+    if not args.no_fp64:
+        n64 = min(args.steps, 4000)
+        _, e64t, sps64, _ = time_engine(MD_FP64, state, n64, args.warmup, args.seed)
+        out["fp64_steps_per_s"] = round(sps64, 2)
+        e64t.close()
+
+    # per-kernel durations (HIP events on the engine's stream)
+    ms = e32.time_kernels(20)
+    per_step = {k: v * (1.0 / p32.multistep if k in ("spread", "fft_x_fwd", "fft_y_fwd", "fft_z_conv",
+                                                    "fft_y_inv", "fft_x_inv", "gather", "real") else 1.0)
+                for k, v in ms.items() if k not in ("bin", "cellsort", "build")}
+    dom = max(per_step, key=per_step.get)
+    b = kernel_bytes(p32, dom)
+    achieved = b / (ms[dom] * 1e-3)
+    out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 2),
+                       "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 5),
+                       "traffic": None, "bytes_per_launch": b, "avg_launch_us": round(ms[dom] * 1e3, 3)}
+    out["kernel_us"] = {k: round(v * 1e3, 3) for k, v in ms.items()}
+    e32.close()
+
+    if not args.no_cpu:
+        sps_cpu, n, el = cpu_baseline(state, args.seed, args.cpu_seconds)
+        out["cpu_baseline"] = {"value": round(sps_cpu, 3), "unit": "steps/s", "cores": 1, "kind": "port",
+                               "sample": f"{n} steps of the same workload (FP64 oracle, serial C, -O2) "
+                                         f"in {el:.1f} s"}
+        out["speedup_vs_cpu"] = round(sps32 / sps_cpu, 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

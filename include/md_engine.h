@@ -1,0 +1,193 @@
+/*
+ * md_engine.h -- C-ABI of the MI355X force-and-step engine for Langevin
+ * dynamics of grafted polyelectrolyte brushes (linear, star and mixed).
+ *
+ * The reference program keeps all state in Fortran module variables and its
+ * hot-path routines take no arguments.  This header is the argument-passing
+ * form of those routines; each entry point names the reference routine it
+ * replaces (file:line in wangshaoyun/MD_Brushes).  A Fortran host binds it
+ * with iso_c_binding; Python binds it with ctypes.
+ *
+ * Conventions
+ *   - Arrays are the reference's Fortran arrays in column-major order:
+ *     pos(NN,4) = x(1:NN), y(1:NN), z(1:NN), q(1:NN); vel(NN,3); acc(NN,3).
+ *   - Particle indices in topology arrays are 1-based, as in the reference.
+ *   - Every int-returning call returns MD_OK (0) or an MD_E* status; the
+ *     message is available from md_last_error().
+ *   - All device state is owned by the engine and lives until md_destroy().
+ *     One host thread per engine; calls on one engine are not re-entrant.
+ */
+#ifndef MD_ENGINE_H
+#define MD_ENGINE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MD_MAX_ARM 16
+
+enum {
+  MD_OK = 0,
+  MD_EINVAL = 1,        /* bad argument or parameter                       */
+  MD_EBOND = 2,         /* FENE bond stretched past R0 (compt_acc.f90:879)   */
+  MD_ECELL = 3,         /* particle outside the cell grid (x==Lx/2, z<0, z>=Lz) */
+  MD_ENONFINITE = 4,    /* non-finite force                                 */
+  MD_EOVERFLOW = 5,     /* neighbour-list or cell capacity exceeded          */
+  MD_EHIP = 6,          /* HIP runtime error                                 */
+  MD_ENOMEM = 7,        /* device allocation failed                          */
+  MD_EIO = 8            /* parameter file could not be read                  */
+};
+
+enum { MD_FP64 = 0, MD_FP32 = 1 };
+
+/* Noise source for compute_force / new_position. */
+enum {
+  MD_NOISE_NONE = 0,    /* deterministic forces only                        */
+  MD_NOISE_ETA = 1,     /* injected standard normals eta(NN,3)              */
+  MD_NOISE_PHILOX = 2   /* counter-based Philox4x32-10 stream keyed by seed  */
+};
+
+/*
+ * Parameters.  The first two blocks are the values of system_data.txt
+ * (in_output.f90:189-213) and force_data.txt (compt_acc.f90:140-157) in file
+ * order; charged_arm is q_arm.txt (in_output.f90:241-247).  md_derive_params
+ * fills the derived block exactly as the reference derives it
+ * (in_output.f90:253-288, compt_acc.f90:159-162, compt_acc.f90:375-420).
+ * The override block lets a caller fix the SPME mesh and Ewald alpha instead
+ * of the reference's automatic rule; zero means "reference rule".
+ */
+typedef struct md_params {
+  /* system_data.txt */
+  int32_t uniform_or_random;
+  int32_t arm, Nma, Nga, Nml, Ngl, man_l, man_s;
+  int32_t StepNum0, StepNum, DeltaStep1, DeltaStep2, DeltaStep3, multistep;
+  double Lz, Z_empty, sigmag, Beta, qq, qqi, ion_ratio, R_bond, dt;
+  /* force_data.txt */
+  double epsilon, sigma, sigma_s, rcl, rvl, rsk, R0_2, kFENE;
+  int32_t ordr[3];
+  int32_t pad0;
+  double lb, xi, EF, tol, tau_rf;
+  /* q_arm.txt */
+  int32_t charged_arm[MD_MAX_ARM];
+  /* overrides (0 = reference rule) */
+  int32_t ovr_ng[3];
+  int32_t pad1;
+  double ovr_alpha;
+  double ovr_rcc;
+  /* derived */
+  int32_t Nta, Ntl, Npe, N_anchor, Nq_PE, Nq_salt_ions, Nq, NN, N_bond;
+  int32_t real_verlet;
+  int32_t ng[3];
+  int32_t pad2;
+  double Lx, Ly, Zfac;                 /* Zfac: Z_empty after in_output.f90:288 */
+  double sigma_sp, rcl_s, rcl_sp, rvl_eff;
+  double alpha, rcc, rvc;
+  double gdim[3];
+  /* engine options */
+  int32_t precision;                   /* MD_FP64 or MD_FP32                     */
+  int32_t pad3;
+  uint64_t seed;                       /* key of the Philox noise stream         */
+} md_params;
+
+typedef struct md_engine md_engine;
+
+/* Replaces read_data (in_output.f90:171-214), read_force_parameters
+ * (compt_acc.f90:138-157) and the q_arm.txt read (in_output.f90:241-247).
+ * q_arm may be NULL when Nga == 0.  Derived fields are left for
+ * md_derive_params. */
+int md_read_params(const char *system_data, const char *force_data,
+                   const char *q_arm, md_params *p);
+
+/* Replaces data_operation (in_output.f90:217-288), the derived radii of
+ * read_force_parameters (compt_acc.f90:159-162) and
+ * Initialize_ewald_parameters (compt_acc.f90:375-430). */
+int md_derive_params(md_params *p);
+
+/* Replaces build_fene_list (compt_acc.f90:317-372) and build_charge
+ * (compt_acc.f90:433-488).  Outputs are 1-based:
+ * charge(Nq), fene_list(N_bond,2) column-major, anchor_list(N_anchor). */
+int md_build_topology(const md_params *p, int32_t *charge, int32_t *fene_list,
+                      int32_t *anchor_list);
+
+/* Seeded restatement of Initialize_position (init_update.f90:6-66) with
+ * uniform or random grafting (init_update.f90:69-394) and ions
+ * (init_update.f90:397-458); pos(NN,4) column-major.  The reference seeds
+ * from the OS (main.f90:12); here the stream is fixed by `seed`. */
+int md_initialize_position(const md_params *p, uint64_t seed, double *pos);
+
+/* Replaces initialize_velocity (init_update.f90:462-495): Gaussian
+ * velocities with variance 1/Beta, anchors at rest, then rescale_velocity. */
+int md_initialize_velocity(const md_params *p, uint64_t seed, double *vel);
+
+/* Allocates device state for one system (initialize_force_parameters,
+ * compt_acc.f90:85-135: B-spline coefficients, PME_BC table, buffers).
+ * The topology is built from p with md_build_topology. */
+int md_create(md_engine **e, const md_params *p);
+
+/* Uploads pos(NN,4), vel(NN,3), acc(NN,3) (acc may be NULL = zero). */
+int md_upload_state(md_engine *e, const double *pos, const double *vel,
+                    const double *acc);
+
+/* Downloads state; any pointer may be NULL.  dr_max[2] = (dr_max1, dr_max2). */
+int md_download_state(md_engine *e, double *pos, double *vel, double *acc,
+                      double *dr_max);
+
+/* Held Coulomb force acc_c(NN,3) (compt_acc.f90:55). */
+int md_download_coulomb(md_engine *e, double *acc_c);
+
+/* Replaces lj_verlet_list (compt_acc.f90:185-314) and, when real_verlet
+ * is set, real_verlet_list (compt_acc.f90:491-626). */
+int md_build_lists(md_engine *e);
+
+/* Verlet lists as (i,j) pairs, 1-based, sorted by (i,j).  Returns the pair
+ * count in *npair; writes at most cap pairs into pairs(cap,2) column-major.
+ * which = 0: LJ list, 1: real-space list. */
+int md_download_list(md_engine *e, int32_t which, int32_t *pairs, int64_t cap,
+                     int64_t *npair);
+
+/* Replaces Compute_Force (compt_acc.f90:815-862) for step `step`.
+ * noise: MD_NOISE_*; eta(NN,3) standard normals when noise == MD_NOISE_ETA. */
+int md_compute_force(md_engine *e, int32_t step, int32_t noise,
+                     const double *eta);
+
+/* Replaces new_position (init_update.f90:555-607) including
+ * period_condition_pos (init_update.f90:610-643). */
+int md_new_position(md_engine *e, int32_t step, int32_t noise,
+                    const double *eta);
+
+/* Replaces update_verlet_list (compt_acc.f90:699-726). */
+int md_update_verlet_list(md_engine *e);
+
+/* Replaces rescale_velocity (init_update.f90:498-552). */
+int md_rescale_velocity(md_engine *e);
+
+/* Runs steps step0 .. step0+nsteps-1 of the loop at main.f90:67-94
+ * (new_position, then in pre-heat the rescale every 20 steps below 300000,
+ * then update_verlet_list) with Philox noise, on the device, without host
+ * round trips.  preheat selects the main.f90:67-75 form. */
+int md_run(md_engine *e, int32_t step0, int32_t nsteps, int32_t preheat);
+
+/* Deterministic force parts for accuracy checks: writes
+ * f_short(NN,3) = LJ + wall + FENE, f_coul(NN,3) = SPME + real space + slab,
+ * evaluated at the current positions (no noise, no field, no anchor mask). */
+int md_force_parts(md_engine *e, double *f_short, double *f_coul);
+
+/* Mean duration in milliseconds of each kernel of a step, timed with HIP
+ * events on the engine's stream over `iters` launches (no reference
+ * counterpart: measurement only).  ms[13]: kick_drift, short_range, spread,
+ * fft_x_fwd, fft_y_fwd, fft_z_conv, fft_y_inv, fft_x_inv, gather, real, bin,
+ * cellsort, build.  Leaves the state advanced by the timed kick-drifts. */
+int md_time_kernels(md_engine *e, int32_t iters, double *ms, int32_t n);
+
+/* Copies the last error message (NUL-terminated) into buf. */
+int md_last_error(md_engine *e, char *buf, int32_t len);
+
+void md_destroy(md_engine *e);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MD_ENGINE_H */

@@ -1,0 +1,134 @@
+// md_device.h -- device-side data layout of the MI355X brush engine.
+//
+// One engine holds R replicas ("systems") of possibly different sizes.  Every
+// kernel covers all replicas in one launch: its blocks are split into
+// per-replica ranges by a prefix table (blk[r] .. blk[r+1]) so that every
+// block belongs to exactly one replica and reads that replica's descriptor
+// with uniform (scalar) loads.  A single system is the case R = 1.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mdk {
+
+constexpr int kBlock = 256;
+constexpr int kMaxOrder = 8;
+
+// Per-replica control block, written by kernels.
+struct Ctrl {
+  unsigned long long dmax1_bits;  // max |dr| of this step, all particles
+  unsigned long long dmax2_bits;  // max |dr| of this step, charged particles
+  double dr_max1, dr_max2;        // accumulated (compt_acc.f90:16,41)
+  int flag1, flag2;               // rebuild LJ / real list after this step
+  int gen1, gen2;                 // rebuild generations (count-buffer parity)
+  int err;                        // first error code (MD_E*)
+  int err_info;                   // bond or particle index of the error
+  int err_step;
+  int ovf;                        // which capacity overflowed (bitmask)
+  int max_nb1, max_nb2;           // largest list length seen at last build
+  int max_cell1, max_cell2;       // largest cell occupancy seen at last build
+  int step;                       // step counter of graph-replayed steps
+  int pad;
+};
+
+// Per-replica descriptor (read-only in kernels).
+struct Rep {
+  // sizes
+  int NN, Nq, N_anchor, real_verlet;
+  // geometry
+  double Lx, Ly, Lz, hLx, hLy;
+  // integrator (init_update.f90:555-607)
+  double dt, c1m, c1p, sig;       // c1m = 1-xi*dt/2, c1p = 1+xi*dt/2
+  double Beta;
+  double EF;
+  uint64_t seed;
+  // short range (compt_acc.f90:895-977, 865-892)
+  double rcl2, rcl3, rcl4, sigma2, sigma3, sigma_s;
+  double wall_r, wall_rs, wall_c;  // rcl/5^(1/6), rcl_s/5^(1/6), 0.4 (default real)
+  double kR02, R0_2;               // kFENE*R0_2, R0_2
+  // Coulomb (compt_acc.f90:980-1049, 1213-1246)
+  double c1, alpha, alpha2, c2, rcc2, slab;  // lb/Beta, 2a/sqrt(pi), 4pi/V
+  int K[3], order;
+  double kg[3];                    // K_d / gdim_d
+  double fnorm[3];                 // K_d/gdim_d/(K1 K2 K3)
+  int Kh;                          // K1/2+1
+  int lK[3];                       // log2 K_d
+  int px_tile, py_tile, pz_tile;   // FFT tile sizes (row pairs, lines, lines)
+  // Verlet lists (compt_acc.f90:185-314, 491-626)
+  double rv1sq, rv2sq;
+  int nc1[3], nc2[3];
+  double rcel1[3], rcel2[3];
+  int ccap1, ccap2;                // cell capacity
+  int nbcap1, nbcap2;              // neighbour capacity
+  double skin1, skin2;             // (rvl-rcl_s)/2, rsk/2
+  // --- device pointers ---
+  double4 *pos;                    // (x,y,z,q) FP64, canonical
+  float4 *pos32;                   // FP32 copy for the FP32 force path
+  double *vx, *vy, *vz;
+  double *ax, *ay, *az;
+  void *ccx, *ccy, *ccz;           // held Coulomb force, real type of the path
+  int *flags;                      // bit0 anchor, bit1 charged
+  int *charge;                     // Nq: particle index of charge m
+  int *bstart;                     // NN+1 CSR of bonded partners
+  int *bpart;
+  int *cnt1[2], *mem1;             // LJ cells: count (double-buffered), members
+  int *cnt2[2], *mem2;             // real cells
+  int *nbn1, *nb1;                 // LJ list: length per particle, ELL slots
+  int *nbn2, *nb2;                 // real list: per charge slot m
+  unsigned long long *meshq;       // fixed-point spread mesh K3*K2*K1
+  void *spec;                      // half spectrum K3*K2*Kh complex
+  void *meshu;                     // real potential mesh K3*K2*K1
+  void *bc;                        // influence function K3*K2*Kh real
+  const void *tw[3];               // twiddles per axis (K_d/2 complex)
+  double *fsp;                     // Nq*3 reciprocal force before mean removal
+  double *part;                    // block partial sums: 4 per charge block
+  int nqblk;                       // number of charge blocks
+  Ctrl *ctrl;
+};
+
+// Work map: block ranges per replica.
+struct WMap {
+  const int *blk;  // nrep+1 prefix
+  int nrep;
+  int total;
+};
+
+__device__ __forceinline__ int find_rep(const WMap &m, int b) {
+  int lo = 0, hi = m.nrep;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (m.blk[mid] <= b) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Fixed-point scale of the spread mesh: 2^40.
+constexpr double kQScale = 1099511627776.0;
+constexpr double kQInv = 1.0 / 1099511627776.0;
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
+__host__ __device__ inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2,
+                                              uint32_t c3, uint32_t k0, uint32_t k1,
+                                              uint32_t out[4]) {
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    if (r > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    c0 = hi1 ^ c1 ^ k0;
+    c1 = lo1;
+    c2 = hi0 ^ c3 ^ k1;
+    c3 = lo0;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+constexpr uint32_t kNoiseTag = 0x4D444E53u;  // counter word 2 of the noise stream
+constexpr uint32_t kVelTag = 0x4D445645u;    // initial velocities
+constexpr uint32_t kInitTag = 0x4D44494Eu;   // initial configuration
+
+}  // namespace mdk

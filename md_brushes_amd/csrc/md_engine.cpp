@@ -1,0 +1,799 @@
+// md_engine.cpp -- host side of the brush engine: device allocation, launch
+// plans, graph replay of the step loop, and the C-ABI of include/md_engine.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/md_engine.h"
+#include "md_kernels.h"
+
+using mdk::Ctrl;
+using mdk::Plan;
+using mdk::Rep;
+using mdk::WMap;
+
+namespace {
+
+constexpr double kPi = 3.141592653589793;
+
+bool is_pow2(int n) { return n >= 4 && (n & (n - 1)) == 0; }
+int ilog2(int n) {
+  int l = 0;
+  while ((1 << l) < n) l++;
+  return l;
+}
+int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+// Cardinal B-spline M_p(x) by the Cox-de Boor recursion.
+double bspline(double x, int p) {
+  if (p == 2) return (x >= 0 && x <= 2) ? 1 - std::fabs(x - 1) : 0;
+  return (x * bspline(x, p - 1) + (p - x) * bspline(x - 1, p - 1)) / (p - 1);
+}
+
+// |b(k)|^2 of the Euler exponential spline, pmeOrthoTabBC
+// (compt_acc.f90:1636-1647), with its 1e15 guard for vanishing sums.
+std::vector<double> bmod2(int K, int p) {
+  std::vector<double> out(K);
+  for (int k = 0; k < K; k++) {
+    double re = 0, im = 0;
+    for (int j = 0; j <= p - 2; j++) {
+      const double ang = 2 * kPi * k * j / K;
+      const double m = bspline(j + 1.0, p);
+      re += m * std::cos(ang);
+      im += m * std::sin(ang);
+    }
+    double mag2 = re * re + im * im;
+    if (std::sqrt(mag2) < (double)1e-10f) mag2 = (double)1e15f * (double)1e15f;
+    out[k] = 1.0 / mag2;
+  }
+  return out;
+}
+
+struct HostTopo {
+  std::vector<int> charge;   // 0-based particle of charge m
+  std::vector<int> flags;    // bit0 anchor, bit1 charged
+  std::vector<int> bstart, bpart;
+};
+
+}  // namespace
+
+struct md_engine {
+  md_params p{};
+  bool fp32 = false;
+  bool coulomb = false;
+  hipStream_t st = nullptr;
+  Plan plan;
+  Rep h{};                      // host copy of the single replica's descriptor
+  std::vector<void *> allocs;
+  std::vector<int> maps;
+  int *d_maps = nullptr;
+  double *d_eta = nullptr, *d_out1 = nullptr, *d_out2 = nullptr;
+  int *h_step = nullptr;        // pinned
+  HostTopo topo;
+  std::string err;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  int ncell1 = 0, ncell2 = 0;
+  bool uploaded = false;
+
+  ~md_engine() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (void *a : allocs) (void)hipFree(a);
+    if (h_step) (void)hipHostFree(h_step);
+    if (st) (void)hipStreamDestroy(st);
+  }
+
+  int fail(int code, const std::string &msg) {
+    err = msg;
+    return code;
+  }
+  int hip_fail(hipError_t e, const char *what) {
+    return fail(MD_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+  }
+  template <typename T> T *dalloc(size_t n) {
+    if (n == 0) n = 1;
+    void *ptr = nullptr;
+    if (hipMalloc(&ptr, n * sizeof(T)) != hipSuccess) return nullptr;
+    (void)hipMemset(ptr, 0, n * sizeof(T));
+    allocs.push_back(ptr);
+    return (T *)ptr;
+  }
+  void dfree(void *ptr) {
+    auto it = std::find(allocs.begin(), allocs.end(), ptr);
+    if (it != allocs.end()) {
+      (void)hipFree(ptr);
+      allocs.erase(it);
+    }
+  }
+  int sync_check();
+  int setup(const md_params *pin);
+  int upload_desc() {
+    hipError_t e = hipMemcpy(plan.reps, &h, sizeof(Rep), hipMemcpyHostToDevice);
+    return e == hipSuccess ? MD_OK : hip_fail(e, "descriptor upload");
+  }
+  int size_lists(const double *pos);
+  int launch_compute_force(int step, int noise, const double *eta, bool in_step);
+  int build_graph();
+};
+
+// Reads the control block; reports the first device-side error.
+int md_engine::sync_check() {
+  hipError_t e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(e, "stream");
+  e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "launch");
+  Ctrl c;
+  e = hipMemcpy(&c, h.ctrl, sizeof c, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "control block");
+  char buf[256];
+  if (c.err == MD_EBOND) {
+    std::snprintf(buf, sizeof buf, "The bond is break off! particle %d step %d", c.err_info + 1,
+                  c.err_step);
+    return fail(MD_EBOND, buf);
+  }
+  if (c.err == MD_ECELL) {
+    std::snprintf(buf, sizeof buf, "particle %d outside the cell grid (x = Lx/2, z < 0 or z >= Lz)",
+                  c.err_info + 1);
+    return fail(MD_ECELL, buf);
+  }
+  if (c.err == MD_ENONFINITE) {
+    std::snprintf(buf, sizeof buf, "non-finite force on particle %d at step %d", c.err_info + 1,
+                  c.err_step);
+    return fail(MD_ENONFINITE, buf);
+  }
+  if (c.ovf) {
+    std::snprintf(buf, sizeof buf,
+                  "capacity exceeded (mask %d): LJ cell %d/%d list %d/%d, real cell %d/%d list %d/%d",
+                  c.ovf, c.max_cell1, h.ccap1, c.max_nb1, h.nbcap1, c.max_cell2, h.ccap2,
+                  c.max_nb2, h.nbcap2);
+    return fail(MD_EOVERFLOW, buf);
+  }
+  return MD_OK;
+}
+
+int md_engine::setup(const md_params *pin) {
+  p = *pin;
+  fp32 = p.precision == MD_FP32;
+  coulomb = p.qq != 0 && p.Nq > 0;
+  const int NN = p.NN, Nq = coulomb ? p.Nq : 0;
+  if (NN <= 0 || p.Lx <= 0 || p.dt <= 0 || p.multistep <= 0) return fail(MD_EINVAL, "bad sizes");
+  if (coulomb) {
+    for (int d = 0; d < 3; d++) {
+      if (!is_pow2(p.ng[d]))
+        return fail(MD_EINVAL, "SPME mesh sizes must be powers of two >= 4 (set ovr_ng)");
+      if (p.ordr[d] != p.ordr[0] || p.ordr[d] < 3 || p.ordr[d] > mdk::kMaxOrder)
+        return fail(MD_EINVAL, "SPME order must be equal on all axes and in 3..8");
+    }
+  }
+  hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e != hipSuccess) return hip_fail(e, "stream");
+  if (hipHostMalloc((void **)&h_step, sizeof(int)) != hipSuccess) return fail(MD_ENOMEM, "pinned");
+
+  // topology (compt_acc.f90:317-372, 433-488)
+  std::vector<int32_t> charge(std::max(Nq, 1)), fene(2 * (size_t)std::max(p.N_bond, 1)),
+      anchor(std::max(p.N_anchor, 1));
+  int rc = md_build_topology(&p, coulomb ? charge.data() : nullptr, fene.data(), anchor.data());
+  if (rc) return fail(rc, "topology");
+  topo.flags.assign(NN, 0);
+  for (int a = 0; a < p.N_anchor; a++) topo.flags[anchor[a] - 1] |= 1;
+  topo.charge.resize(Nq);
+  for (int m = 0; m < Nq; m++) {
+    topo.charge[m] = charge[m] - 1;
+    topo.flags[charge[m] - 1] |= 2;
+  }
+  std::vector<int> deg(NN + 1, 0);
+  for (int b = 0; b < p.N_bond; b++) {
+    deg[fene[b] - 1]++;
+    deg[fene[p.N_bond + b] - 1]++;
+  }
+  topo.bstart.assign(NN + 1, 0);
+  for (int i = 0; i < NN; i++) topo.bstart[i + 1] = topo.bstart[i] + deg[i];
+  topo.bpart.assign(std::max(topo.bstart[NN], 1), 0);
+  std::vector<int> fill(topo.bstart.begin(), topo.bstart.end() - 1);
+  for (int b = 0; b < p.N_bond; b++) {
+    const int j = fene[b] - 1, k = fene[p.N_bond + b] - 1;
+    topo.bpart[fill[j]++] = k;
+    topo.bpart[fill[k]++] = j;
+  }
+
+  // descriptor: scalars, computed as the reference computes them
+  Rep &S = h;
+  S.NN = NN;
+  S.Nq = Nq;
+  S.N_anchor = p.N_anchor;
+  S.real_verlet = coulomb ? p.real_verlet : 0;
+  S.Lx = p.Lx;
+  S.Ly = p.Ly;
+  S.Lz = p.Lz;
+  S.hLx = p.Lx / 2;
+  S.hLy = p.Ly / 2;
+  S.dt = p.dt;
+  S.c1m = 1 - p.xi * p.dt / 2;
+  S.c1p = 1 + p.xi * p.dt / 2.;
+  S.sig = std::sqrt(2. / p.Beta / p.dt * p.xi);
+  S.Beta = p.Beta;
+  S.EF = p.EF;
+  S.seed = p.seed;
+  S.rcl2 = p.rcl * p.rcl;
+  S.rcl3 = p.rcl_s * p.rcl_s;
+  S.rcl4 = p.rcl_sp * p.rcl_sp;
+  S.sigma2 = p.sigma_s * p.sigma_s;
+  S.sigma3 = p.sigma_sp * p.sigma_sp;
+  S.sigma_s = p.sigma_s;
+  const double c5 = (double)(float)std::pow(5.0, (double)(1.0f / 6.0f));
+  S.wall_r = p.rcl / c5;
+  S.wall_rs = p.rcl_s / c5;
+  S.wall_c = (double)0.4f;
+  S.kR02 = p.kFENE * p.R0_2;
+  S.R0_2 = p.R0_2;
+  S.c1 = p.lb / p.Beta;
+  S.alpha = p.alpha;
+  S.alpha2 = p.alpha * p.alpha;
+  S.c2 = 2 * p.alpha / std::sqrt(kPi);
+  S.rcc2 = p.rcc * p.rcc;
+  S.slab = 4 * kPi / p.Lx / p.Ly / p.Lz / p.Zfac;
+  S.rv1sq = p.rvl_eff * p.rvl_eff;
+  S.rv2sq = p.rvc * p.rvc;
+  S.skin1 = (p.rvl_eff - p.rcl_s) / 2;
+  S.skin2 = p.rsk / 2;
+  const double L3[3] = {p.Lx, p.Ly, p.Lz};
+  for (int d = 0; d < 3; d++) {
+    S.nc1[d] = (int)(L3[d] / p.rvl_eff);
+    S.rcel1[d] = S.nc1[d] > 0 ? L3[d] / S.nc1[d] : 0;
+    S.nc2[d] = S.real_verlet ? (int)(L3[d] / p.rvc) : 1;
+    S.rcel2[d] = S.real_verlet ? L3[d] / S.nc2[d] : L3[d];
+  }
+  if (S.nc1[0] < 1 || S.nc1[1] < 1 || S.nc1[2] < 1) return fail(MD_EINVAL, "box smaller than rvl");
+  ncell1 = S.nc1[0] * S.nc1[1] * S.nc1[2];
+  ncell2 = S.real_verlet ? S.nc2[0] * S.nc2[1] * S.nc2[2] : 0;
+
+  // device arrays
+  const size_t nn = NN;
+  S.pos = dalloc<double4>(nn);
+  S.pos32 = fp32 ? dalloc<float4>(nn) : nullptr;
+  double *v = dalloc<double>(3 * nn), *a = dalloc<double>(3 * nn);
+  if (!S.pos || !v || !a) return fail(MD_ENOMEM, "state");
+  S.vx = v; S.vy = v + nn; S.vz = v + 2 * nn;
+  S.ax = a; S.ay = a + nn; S.az = a + 2 * nn;
+  const size_t rs = fp32 ? 4 : 8;
+  char *cc = (char *)dalloc<char>(3 * nn * rs);
+  S.ccx = cc; S.ccy = cc + nn * rs; S.ccz = cc + 2 * nn * rs;
+  S.flags = dalloc<int>(nn);
+  S.charge = dalloc<int>(std::max(Nq, 1));
+  S.bstart = dalloc<int>(nn + 1);
+  S.bpart = dalloc<int>(topo.bpart.size());
+  S.ctrl = dalloc<Ctrl>(1);
+  d_eta = dalloc<double>(3 * nn);
+  d_out1 = dalloc<double>(3 * nn);
+  d_out2 = dalloc<double>(3 * nn);
+  if (!S.flags || !S.charge || !S.bstart || !S.bpart || !S.ctrl || !d_eta)
+    return fail(MD_ENOMEM, "topology arrays");
+  (void)hipMemcpy(S.flags, topo.flags.data(), nn * sizeof(int), hipMemcpyHostToDevice);
+  if (Nq) (void)hipMemcpy(S.charge, topo.charge.data(), Nq * sizeof(int), hipMemcpyHostToDevice);
+  (void)hipMemcpy(S.bstart, topo.bstart.data(), (nn + 1) * sizeof(int), hipMemcpyHostToDevice);
+  (void)hipMemcpy(S.bpart, topo.bpart.data(), topo.bpart.size() * sizeof(int),
+                  hipMemcpyHostToDevice);
+  // cell counts (double-buffered); members and lists are sized at upload
+  S.cnt1[0] = dalloc<int>(ncell1);
+  S.cnt1[1] = dalloc<int>(ncell1);
+  S.cnt2[0] = dalloc<int>(std::max(ncell2, 1));
+  S.cnt2[1] = dalloc<int>(std::max(ncell2, 1));
+  S.nbn1 = dalloc<int>(nn);
+  S.nbn2 = dalloc<int>(std::max(Nq, 1));
+
+  // SPME: mesh, spectrum, influence function, twiddles
+  const int csz = fp32 ? 8 : 16;
+  if (coulomb) {
+    const int K1 = p.ng[0], K2 = p.ng[1], K3 = p.ng[2];
+    for (int d = 0; d < 3; d++) {
+      S.K[d] = p.ng[d];
+      S.lK[d] = ilog2(p.ng[d]);
+      S.kg[d] = p.ng[d] / p.gdim[d];
+      S.fnorm[d] = p.ng[d] / p.gdim[d] / ((double)K1 * K2 * K3);
+    }
+    S.order = p.ordr[0];
+    S.Kh = K1 / 2 + 1;
+    const size_t M = (size_t)K1 * K2 * K3, Mh = (size_t)K3 * K2 * S.Kh;
+    S.meshq = dalloc<unsigned long long>(M);
+    S.spec = dalloc<char>(Mh * csz);
+    S.meshu = dalloc<char>(M * rs);
+    S.bc = dalloc<char>(Mh * rs);
+    if (!S.meshq || !S.spec || !S.meshu || !S.bc) return fail(MD_ENOMEM, "mesh");
+    // influence function of PME_BC (compt_acc.f90:1557-1598), half spectrum
+    const int o = S.order;
+    std::vector<double> b2[3], mv[3], ev[3];
+    for (int d = 0; d < 3; d++) {
+      const int K = p.ng[d];
+      b2[d] = bmod2(K, o);
+      mv[d].resize(K);
+      ev[d].resize(K);
+      for (int k = 0; k < K; k++) {
+        mv[d][k] = (k <= K / 2 ? k : k - K) / p.gdim[d];
+        const double t = kPi / p.alpha;
+        ev[d][k] = std::exp(-t * t * mv[d][k] * mv[d][k]);
+      }
+    }
+    const double pivol = (double)K1 * K2 * K3 / (kPi * p.gdim[0] * p.gdim[1] * p.gdim[2]);
+    std::vector<double> bcd(Mh);
+    for (int z = 0; z < K3; z++)
+      for (int y = 0; y < K2; y++)
+        for (int x = 0; x < S.Kh; x++) {
+          const double m2 = mv[0][x] * mv[0][x] + mv[1][y] * mv[1][y] + mv[2][z] * mv[2][z];
+          bcd[((size_t)z * K2 + y) * S.Kh + x] =
+              (x + y + z == 0) ? 0.0
+                               : b2[0][x] * b2[1][y] * b2[2][z] * pivol * ev[0][x] * ev[1][y] *
+                                     ev[2][z] / m2;
+        }
+    if (fp32) {
+      std::vector<float> f(bcd.begin(), bcd.end());
+      (void)hipMemcpy(S.bc, f.data(), Mh * 4, hipMemcpyHostToDevice);
+    } else {
+      (void)hipMemcpy(S.bc, bcd.data(), Mh * 8, hipMemcpyHostToDevice);
+    }
+    for (int d = 0; d < 3; d++) {
+      const int K = p.ng[d];
+      std::vector<double> tw(K);  // K/2 complex
+      for (int k = 0; k < K / 2; k++) {
+        tw[2 * k] = std::cos(2 * kPi * k / K);
+        tw[2 * k + 1] = -std::sin(2 * kPi * k / K);
+      }
+      void *dtw = dalloc<char>((size_t)K / 2 * csz);
+      if (fp32) {
+        std::vector<float> f(tw.begin(), tw.end());
+        (void)hipMemcpy(dtw, f.data(), (size_t)K / 2 * 8, hipMemcpyHostToDevice);
+      } else {
+        (void)hipMemcpy(dtw, tw.data(), (size_t)K / 2 * 16, hipMemcpyHostToDevice);
+      }
+      S.tw[d] = dtw;
+    }
+    // FFT tiles: about 32 KB of LDS per workgroup
+    const int budget = 32768;
+    S.px_tile = std::max(1, std::min(K2 * K3 / 2, budget / (K1 * csz)));
+    S.py_tile = std::max(1, std::min(S.Kh, budget / (K2 * csz)));
+    S.pz_tile = std::max(1, std::min(S.Kh, budget / (K3 * csz)));
+    S.nqblk = cdiv(Nq, mdk::kBlock);
+    S.fsp = dalloc<double>(3 * (size_t)Nq);
+    S.part = dalloc<double>(4 * (size_t)S.nqblk);
+    plan.lds_x = (size_t)S.px_tile * K1 * csz;
+    plan.lds_y = (size_t)S.py_tile * K2 * csz;
+    plan.lds_z = (size_t)S.pz_tile * K3 * csz;
+  }
+
+  // block maps (one replica)
+  const int nb_part = cdiv(NN, mdk::kBlock);
+  const int nb_chg = coulomb ? cdiv(Nq, mdk::kBlock) : 0;
+  const int nb_fx = coulomb ? cdiv((long)S.K[1] * S.K[2] / 2, S.px_tile) : 0;
+  const int nb_fy = coulomb ? S.K[2] * cdiv(S.Kh, S.py_tile) : 0;
+  const int nb_fz = coulomb ? S.K[1] * cdiv(S.Kh, S.pz_tile) : 0;
+  const int nb_c1 = cdiv(ncell1, mdk::kBlock);
+  const int nb_c2 = ncell2;
+  const int counts[7] = {nb_part, nb_chg, nb_fx, nb_fy, nb_fz, nb_c1, nb_c2};
+  maps.clear();
+  for (int k = 0; k < 7; k++) {
+    maps.push_back(0);
+    maps.push_back(counts[k]);
+  }
+  d_maps = dalloc<int>(maps.size());
+  (void)hipMemcpy(d_maps, maps.data(), maps.size() * sizeof(int), hipMemcpyHostToDevice);
+  WMap *w[7] = {&plan.part, &plan.chg, &plan.fx, &plan.fy, &plan.fz, &plan.cell1, &plan.cell2};
+  for (int k = 0; k < 7; k++) *w[k] = WMap{d_maps + 2 * k, 1, counts[k]};
+  plan.nrep = 1;
+  plan.fp32 = fp32;
+  plan.reps = dalloc<Rep>(1);
+  plan.vpart = dalloc<double>(4 * (size_t)nb_part);
+  if (!plan.reps || !plan.vpart) return fail(MD_ENOMEM, "plan");
+  S.mem1 = S.mem2 = nullptr;
+  S.nb1 = S.nb2 = nullptr;
+  S.ccap1 = S.ccap2 = S.nbcap1 = S.nbcap2 = 0;
+  return upload_desc();
+}
+
+// Sizes cell and list capacities from a configuration: twice the largest
+// occupancy and list length found on the host, plus headroom.
+int md_engine::size_lists(const double *pos) {
+  const int NN = p.NN;
+  const double *X = pos, *Y = pos + NN, *Z = pos + 2 * (size_t)NN;
+  auto scan = [&](const std::vector<int> &items, const int nc[3], double rv, int &maxcell,
+                  int &maxnb) {
+    const double rc[3] = {p.Lx / nc[0], p.Ly / nc[1], p.Lz / nc[2]};
+    const int ncell = nc[0] * nc[1] * nc[2];
+    std::vector<std::vector<int>> cells(ncell);
+    std::vector<int> cid(items.size());
+    for (size_t t = 0; t < items.size(); t++) {
+      const int i = items[t];
+      int a = (int)((X[i] + p.Lx / 2) / rc[0]), b = (int)((Y[i] + p.Ly / 2) / rc[1]),
+          c = (int)(Z[i] / rc[2]);
+      a = std::min(std::max(a, 0), nc[0] - 1);
+      b = std::min(std::max(b, 0), nc[1] - 1);
+      c = std::min(std::max(c, 0), nc[2] - 1);
+      cid[t] = (c * nc[1] + b) * nc[0] + a;
+      cells[cid[t]].push_back(i);
+    }
+    maxcell = 0;
+    for (auto &c : cells) maxcell = std::max(maxcell, (int)c.size());
+    maxnb = 0;
+    for (size_t t = 0; t < items.size(); t++) {
+      const int i = items[t], cell = cid[t];
+      const int a = cell % nc[0], b = (cell / nc[0]) % nc[1], c = cell / (nc[0] * nc[1]);
+      int n = 0;
+      for (int l = -1; l <= 1; l++)
+        for (int m = -1; m <= 1; m++)
+          for (int k = -1; k <= 1; k++) {
+            const int z = c + k;
+            if (z < 0 || z >= nc[2]) continue;
+            const int xx = (a + l + nc[0]) % nc[0], yy = (b + m + nc[1]) % nc[1];
+            for (int j : cells[(z * nc[1] + yy) * nc[0] + xx]) {
+              if (j == i) continue;
+              double dx = X[i] - X[j], dy = Y[i] - Y[j], dz = Z[i] - Z[j];
+              if (dx > p.Lx / 2) dx -= p.Lx; else if (dx <= -p.Lx / 2) dx += p.Lx;
+              if (dy > p.Ly / 2) dy -= p.Ly; else if (dy <= -p.Ly / 2) dy += p.Ly;
+              if (dx * dx + dy * dy + dz * dz < rv * rv) n++;
+            }
+          }
+      maxnb = std::max(maxnb, n);
+    }
+  };
+  std::vector<int> all(NN);
+  for (int i = 0; i < NN; i++) all[i] = i;
+  int mc1, mn1, mc2 = 0, mn2 = 0;
+  scan(all, h.nc1, p.rvl_eff, mc1, mn1);
+  if (h.real_verlet) scan(topo.charge, h.nc2, p.rvc, mc2, mn2);
+  auto round8 = [](int v) { return (v + 7) & ~7; };
+  // LJ: dense packing bounds occupancy near 1.4 per unit volume
+  const double v1 = h.rcel1[0] * h.rcel1[1] * h.rcel1[2];
+  const int ccap1 = round8(std::max(2 * mc1 + 16, (int)(1.5 * v1) + 16));
+  const int nbcap1 = round8(std::min(NN, std::max(2 * mn1 + 32, (int)(4.19 * std::pow(p.rvl_eff, 3) * 1.2) + 32)));
+  const int ccap2 = h.real_verlet ? round8(std::min(h.Nq, 2 * mc2 + 64)) : 0;
+  const int nbcap2 = h.real_verlet ? round8(std::min(h.Nq, 2 * mn2 + 64)) : 0;
+  if (ccap2 > 16384) return fail(MD_EINVAL, "real-space cells too full for the LDS sort");
+  if (ccap1 != h.ccap1) {
+    if (h.mem1) dfree(h.mem1);
+    h.mem1 = dalloc<int>((size_t)ncell1 * ccap1);
+    h.ccap1 = ccap1;
+  }
+  if (nbcap1 != h.nbcap1) {
+    if (h.nb1) dfree(h.nb1);
+    h.nb1 = dalloc<int>((size_t)NN * nbcap1);
+    h.nbcap1 = nbcap1;
+  }
+  if (h.real_verlet && ccap2 != h.ccap2) {
+    if (h.mem2) dfree(h.mem2);
+    h.mem2 = dalloc<int>((size_t)ncell2 * ccap2);
+    h.ccap2 = ccap2;
+  }
+  if (h.real_verlet && nbcap2 != h.nbcap2) {
+    if (h.nb2) dfree(h.nb2);
+    h.nb2 = dalloc<int>((size_t)h.Nq * nbcap2);
+    h.nbcap2 = nbcap2;
+  }
+  if (!h.mem1 || !h.nb1 || (h.real_verlet && (!h.mem2 || !h.nb2)))
+    return fail(MD_ENOMEM, "neighbour lists");
+  int np = 1;
+  while (np < h.ccap2) np <<= 1;
+  plan.lds_sort = h.real_verlet ? (size_t)np * sizeof(int) : 0;
+  return upload_desc();
+}
+
+int md_engine::launch_compute_force(int step, int noise, const double *eta, bool in_step) {
+  int mode = mdk::SR_HELD | mdk::SR_FIELD | mdk::SR_ANCHOR;
+  if (noise == MD_NOISE_ETA) {
+    if (!eta) return fail(MD_EINVAL, "eta is null");
+    hipError_t e = hipMemcpyAsync(d_eta, eta, 3 * sizeof(double) * p.NN, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return hip_fail(e, "eta upload");
+    mode |= mdk::SR_NOISE_ETA;
+  } else if (noise == MD_NOISE_PHILOX) {
+    mode |= mdk::SR_NOISE_PHILOX;
+  }
+  if (in_step) mode |= mdk::SR_KICK | mdk::SR_DRMAX;
+  if (coulomb && step % p.multistep == 0) {
+    if (fp32) mdk::launch_coulomb<float>(plan, st, nullptr);
+    else mdk::launch_coulomb<double>(plan, st, nullptr);
+  }
+  if (fp32) mdk::launch_short<float>(plan, st, mode, step, d_eta, nullptr);
+  else mdk::launch_short<double>(plan, st, mode, step, d_eta, nullptr);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? MD_OK : hip_fail(e, "compute_force launch");
+}
+
+// One multistep cycle of production steps, captured once and replayed:
+// kick-drift, Coulomb at the first step of the cycle, short range with the
+// second kick, list maintenance.  The step number comes from the control
+// block (advanced by the list kernel).
+int md_engine::build_graph() {
+  if (gexec) return MD_OK;
+  hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return hip_fail(e, "begin capture");
+  const int mode = mdk::SR_HELD | mdk::SR_FIELD | mdk::SR_ANCHOR | mdk::SR_NOISE_PHILOX |
+                   mdk::SR_KICK | mdk::SR_DRMAX;
+  for (int s = 0; s < p.multistep; s++) {
+    mdk::launch_kick_drift(plan, st);
+    if (coulomb && s == 0) {
+      if (fp32) mdk::launch_coulomb<float>(plan, st, nullptr);
+      else mdk::launch_coulomb<double>(plan, st, nullptr);
+    }
+    if (fp32) mdk::launch_short<float>(plan, st, mode, -1, nullptr, nullptr);
+    else mdk::launch_short<double>(plan, st, mode, -1, nullptr, nullptr);
+    mdk::launch_lists(plan, st, 1);
+  }
+  e = hipStreamEndCapture(st, &graph);
+  if (e != hipSuccess) return hip_fail(e, "end capture");
+  e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) return hip_fail(e, "instantiate");
+  return MD_OK;
+}
+
+// ------------------------------------------------------------------ C-ABI
+
+#define ENGINE_CHECK(e) \
+  if (!(e)) return MD_EINVAL
+#define STATE_CHECK(e)                                              \
+  if (!(e)->uploaded) return (e)->fail(MD_EINVAL, "no state uploaded")
+
+extern "C" int md_create(md_engine **out, const md_params *p) {
+  if (!out || !p) return MD_EINVAL;
+  *out = nullptr;
+  md_engine *e = new md_engine();
+  int rc = e->setup(p);
+  *out = e;
+  return rc;
+}
+
+extern "C" void md_destroy(md_engine *e) { delete e; }
+
+extern "C" int md_last_error(md_engine *e, char *buf, int32_t len) {
+  if (!e || !buf || len <= 0) return MD_EINVAL;
+  std::snprintf(buf, (size_t)len, "%s", e->err.c_str());
+  return MD_OK;
+}
+
+extern "C" int md_upload_state(md_engine *e, const double *pos, const double *vel,
+                               const double *acc) {
+  ENGINE_CHECK(e && pos && vel);
+  const int NN = e->p.NN;
+  std::vector<double4> p4(NN);
+  std::vector<float4> p32(NN);
+  for (int i = 0; i < NN; i++) {
+    p4[i] = make_double4(pos[i], pos[NN + i], pos[2 * (size_t)NN + i], pos[3 * (size_t)NN + i]);
+    p32[i] = make_float4((float)p4[i].x, (float)p4[i].y, (float)p4[i].z, (float)p4[i].w);
+  }
+  // no kernel may still use the list buffers that size_lists may replace
+  hipError_t serr = hipStreamSynchronize(e->st);
+  if (serr != hipSuccess) return e->hip_fail(serr, "upload sync");
+  int rc = e->size_lists(pos);
+  if (rc) return rc;
+  e->uploaded = true;
+  hipError_t err = hipMemcpy(e->h.pos, p4.data(), NN * sizeof(double4), hipMemcpyHostToDevice);
+  if (err == hipSuccess && e->fp32)
+    err = hipMemcpy(e->h.pos32, p32.data(), NN * sizeof(float4), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(e->h.vx, vel, 3 * sizeof(double) * NN, hipMemcpyHostToDevice);
+  if (err == hipSuccess) {
+    if (acc) err = hipMemcpy(e->h.ax, acc, 3 * sizeof(double) * NN, hipMemcpyHostToDevice);
+    else err = hipMemset(e->h.ax, 0, 3 * sizeof(double) * NN);
+  }
+  return err == hipSuccess ? MD_OK : e->hip_fail(err, "upload");
+}
+
+extern "C" int md_download_state(md_engine *e, double *pos, double *vel, double *acc,
+                                 double *dr_max) {
+  ENGINE_CHECK(e);
+  int rc = e->sync_check();
+  const int NN = e->p.NN;
+  if (pos) {
+    std::vector<double4> p4(NN);
+    (void)hipMemcpy(p4.data(), e->h.pos, NN * sizeof(double4), hipMemcpyDeviceToHost);
+    for (int i = 0; i < NN; i++) {
+      pos[i] = p4[i].x;
+      pos[NN + i] = p4[i].y;
+      pos[2 * (size_t)NN + i] = p4[i].z;
+      pos[3 * (size_t)NN + i] = p4[i].w;
+    }
+  }
+  if (vel) (void)hipMemcpy(vel, e->h.vx, 3 * sizeof(double) * NN, hipMemcpyDeviceToHost);
+  if (acc) (void)hipMemcpy(acc, e->h.ax, 3 * sizeof(double) * NN, hipMemcpyDeviceToHost);
+  if (dr_max) {
+    Ctrl c;
+    (void)hipMemcpy(&c, e->h.ctrl, sizeof c, hipMemcpyDeviceToHost);
+    dr_max[0] = c.dr_max1;
+    dr_max[1] = c.dr_max2;
+  }
+  return rc;
+}
+
+extern "C" int md_download_coulomb(md_engine *e, double *acc_c) {
+  ENGINE_CHECK(e && acc_c);
+  int rc = e->sync_check();
+  const size_t n = 3 * (size_t)e->p.NN;
+  if (e->fp32) {
+    std::vector<float> f(n);
+    (void)hipMemcpy(f.data(), e->h.ccx, n * 4, hipMemcpyDeviceToHost);
+    for (size_t k = 0; k < n; k++) acc_c[k] = f[k];
+  } else {
+    (void)hipMemcpy(acc_c, e->h.ccx, n * 8, hipMemcpyDeviceToHost);
+  }
+  return rc;
+}
+
+extern "C" int md_build_lists(md_engine *e) {
+  ENGINE_CHECK(e);
+  STATE_CHECK(e);
+  mdk::launch_set_flags(e->plan, e->st, 0);
+  mdk::launch_lists(e->plan, e->st, 0);
+  mdk::launch_set_flags(e->plan, e->st, 1);
+  return e->sync_check();
+}
+
+extern "C" int md_download_list(md_engine *e, int32_t which, int32_t *pairs, int64_t cap,
+                                int64_t *npair) {
+  ENGINE_CHECK(e && npair && (which == 0 || which == 1));
+  int rc = e->sync_check();
+  if (rc) return rc;
+  const bool lj = which == 0;
+  const int n = lj ? e->p.NN : e->h.Nq;
+  const int lcap = lj ? e->h.nbcap1 : e->h.nbcap2;
+  if (!lj && !e->h.real_verlet) {
+    *npair = 0;
+    return MD_OK;
+  }
+  std::vector<int> cnt(n), nb((size_t)n * lcap);
+  (void)hipMemcpy(cnt.data(), lj ? e->h.nbn1 : e->h.nbn2, n * sizeof(int), hipMemcpyDeviceToHost);
+  (void)hipMemcpy(nb.data(), lj ? e->h.nb1 : e->h.nb2, nb.size() * sizeof(int),
+                  hipMemcpyDeviceToHost);
+  std::vector<std::pair<int, int>> pr;
+  for (int t = 0; t < n; t++) {
+    const int i = lj ? t : e->topo.charge[t];
+    for (int k = 0; k < std::min(cnt[t], lcap); k++) pr.emplace_back(i + 1, nb[(size_t)k * n + t] + 1);
+  }
+  std::sort(pr.begin(), pr.end());
+  *npair = (int64_t)pr.size();
+  if (pairs)
+    for (int64_t k = 0; k < (int64_t)pr.size() && k < cap; k++) {
+      pairs[k] = pr[k].first;
+      pairs[cap + k] = pr[k].second;
+    }
+  return MD_OK;
+}
+
+extern "C" int md_compute_force(md_engine *e, int32_t step, int32_t noise, const double *eta) {
+  ENGINE_CHECK(e && step >= 0);
+  STATE_CHECK(e);
+  return e->launch_compute_force(step, noise, eta, false);
+}
+
+extern "C" int md_new_position(md_engine *e, int32_t step, int32_t noise, const double *eta) {
+  ENGINE_CHECK(e && step >= 0);
+  STATE_CHECK(e);
+  mdk::launch_kick_drift(e->plan, e->st);
+  return e->launch_compute_force(step, noise, eta, true);
+}
+
+extern "C" int md_update_verlet_list(md_engine *e) {
+  ENGINE_CHECK(e);
+  STATE_CHECK(e);
+  mdk::launch_lists(e->plan, e->st, 0);
+  mdk::launch_set_flags(e->plan, e->st, 1);
+  hipError_t err = hipGetLastError();
+  return err == hipSuccess ? MD_OK : e->hip_fail(err, "update_verlet_list");
+}
+
+extern "C" int md_rescale_velocity(md_engine *e) {
+  ENGINE_CHECK(e);
+  STATE_CHECK(e);
+  mdk::launch_rescale(e->plan, e->st);
+  hipError_t err = hipGetLastError();
+  return err == hipSuccess ? MD_OK : e->hip_fail(err, "rescale_velocity");
+}
+
+extern "C" int md_run(md_engine *e, int32_t step0, int32_t nsteps, int32_t preheat) {
+  ENGINE_CHECK(e && step0 >= 0 && nsteps >= 0);
+  STATE_CHECK(e);
+  int step = step0;
+  const int end = step0 + nsteps;
+  auto one = [&](int s) -> int {
+    int rc = md_new_position(e, s, MD_NOISE_PHILOX, nullptr);
+    if (rc) return rc;
+    if (preheat && s % 20 == 0 && s < 300000) {
+      rc = md_rescale_velocity(e);
+      if (rc) return rc;
+    }
+    return md_update_verlet_list(e);
+  };
+  if (preheat) {
+    for (; step < end; step++) {
+      int rc = one(step);
+      if (rc) return rc;
+    }
+    return e->sync_check();
+  }
+  const int ms = e->p.multistep;
+  while (step < end && step % ms != 0) {
+    int rc = one(step++);
+    if (rc) return rc;
+  }
+  if (end - step >= ms) {
+    int rc = e->build_graph();
+    if (rc) return rc;
+    // step counter for the replay: the graph reads it from the control block
+    *e->h_step = step;
+    hipError_t err = hipMemcpyAsync(&e->h.ctrl->step, e->h_step, sizeof(int),
+                                    hipMemcpyHostToDevice, e->st);
+    if (err != hipSuccess) return e->hip_fail(err, "step upload");
+    while (end - step >= ms) {
+      err = hipGraphLaunch(e->gexec, e->st);
+      if (err != hipSuccess) return e->hip_fail(err, "graph launch");
+      step += ms;
+    }
+  }
+  while (step < end) {
+    int rc = one(step++);
+    if (rc) return rc;
+  }
+  return e->sync_check();
+}
+
+extern "C" int md_force_parts(md_engine *e, double *f_short, double *f_coul) {
+  ENGINE_CHECK(e);
+  STATE_CHECK(e);
+  const size_t n = 3 * (size_t)e->p.NN;
+  if (e->fp32) mdk::launch_short<float>(e->plan, e->st, 0, 0, nullptr, e->d_out1);
+  else mdk::launch_short<double>(e->plan, e->st, 0, 0, nullptr, e->d_out1);
+  (void)hipMemsetAsync(e->d_out2, 0, n * sizeof(double), e->st);
+  if (e->coulomb) {
+    if (e->fp32) mdk::launch_coulomb<float>(e->plan, e->st, e->d_out2);
+    else mdk::launch_coulomb<double>(e->plan, e->st, e->d_out2);
+  }
+  int rc = e->sync_check();
+  if (f_short) (void)hipMemcpy(f_short, e->d_out1, n * sizeof(double), hipMemcpyDeviceToHost);
+  if (f_coul) (void)hipMemcpy(f_coul, e->d_out2, n * sizeof(double), hipMemcpyDeviceToHost);
+  return rc;
+}
+
+// Mean duration of each kernel of a step, from HIP events on the engine's
+// stream.  ms[k]: 0 kick_drift, 1 short_range, 2 spread, 3 fft_x_fwd,
+// 4 fft_y_fwd, 5 fft_z_conv, 6 fft_y_inv, 7 fft_x_inv, 8 gather, 9 real,
+// 10 bin, 11 cellsort, 12 build.  Leaves the state advanced by the timed
+// kick-drifts.
+extern "C" int md_time_kernels(md_engine *e, int32_t iters, double *ms, int32_t n) {
+  ENGINE_CHECK(e && ms && iters > 0 && n >= 13);
+  STATE_CHECK(e);
+  int rc = e->sync_check();
+  if (rc) return rc;
+  hipEvent_t ev[10];
+  for (auto &x : ev) (void)hipEventCreate(&x);
+  double acc[13] = {0};
+  float t;
+  auto span = [&](int a, int b) {
+    (void)hipEventSynchronize(ev[b]);
+    (void)hipEventElapsedTime(&t, ev[a], ev[b]);
+    return (double)t;
+  };
+  const int mode = mdk::SR_HELD | mdk::SR_FIELD | mdk::SR_ANCHOR | mdk::SR_NOISE_PHILOX;
+  for (int it = 0; it < iters; it++) {
+    (void)hipEventRecord(ev[0], e->st);
+    mdk::launch_kick_drift(e->plan, e->st);
+    (void)hipEventRecord(ev[1], e->st);
+    if (e->fp32) mdk::launch_short<float>(e->plan, e->st, mode, 0, nullptr, nullptr);
+    else mdk::launch_short<double>(e->plan, e->st, mode, 0, nullptr, nullptr);
+    (void)hipEventRecord(ev[2], e->st);
+    acc[0] += span(0, 1);
+    acc[1] += span(1, 2);
+    if (e->coulomb) {
+      if (e->fp32) mdk::launch_coulomb<float>(e->plan, e->st, nullptr, ev);
+      else mdk::launch_coulomb<double>(e->plan, e->st, nullptr, ev);
+      for (int k = 0; k < 8; k++) acc[2 + k] += span(k, k + 1);
+    }
+    mdk::launch_set_flags(e->plan, e->st, 0);
+    mdk::launch_lists(e->plan, e->st, 0, ev);
+    for (int k = 0; k < 3; k++) acc[10 + k] += span(k, k + 1);
+  }
+  mdk::launch_set_flags(e->plan, e->st, 1);
+  rc = e->sync_check();
+  for (auto &x : ev) (void)hipEventDestroy(x);
+  for (int k = 0; k < 13; k++) ms[k] = acc[k] / iters;
+  return rc;
+}

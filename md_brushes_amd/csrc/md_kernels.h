@@ -1,0 +1,38 @@
+// md_kernels.h -- launch plan and kernel launchers of the brush engine.
+#pragma once
+#include "md_device.h"
+
+namespace mdk {
+
+// Mode bits of the short-range kernel.
+enum {
+  SR_HELD = 1,           // start from the held Coulomb force acc_c
+  SR_NOISE_ETA = 2,      // add injected normals eta(NN,3) * sigma
+  SR_NOISE_PHILOX = 4,   // add Philox normals * sigma
+  SR_FIELD = 8,          // add q * E_z
+  SR_ANCHOR = 16,        // zero the force on anchors
+  SR_KICK = 32,          // second half kick of new_position
+  SR_DRMAX = 64          // dr_max bookkeeping and rebuild flags
+};
+
+// Everything a launch needs: device descriptors and the block maps.
+struct Plan {
+  Rep *reps = nullptr;
+  int nrep = 0;
+  WMap part{}, chg{}, fx{}, fy{}, fz{}, cell1{}, cell2{};
+  size_t lds_x = 0, lds_y = 0, lds_z = 0, lds_sort = 0;
+  double *vpart = nullptr;  // partial sums of rescale_velocity
+  bool fp32 = false;
+};
+
+template <typename R>
+void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev = nullptr);
+template <typename R>
+void launch_short(const Plan &P, hipStream_t st, int mode, int step, const double *eta,
+                  double *out);
+void launch_kick_drift(const Plan &P, hipStream_t st);
+void launch_lists(const Plan &P, hipStream_t st, int inc_step, hipEvent_t *ev = nullptr);
+void launch_set_flags(const Plan &P, hipStream_t st, int clear);
+void launch_rescale(const Plan &P, hipStream_t st);
+
+}  // namespace mdk

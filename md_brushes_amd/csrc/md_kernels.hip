@@ -1,0 +1,1066 @@
+// md_kernels.hip -- gfx950 kernels of the brush engine.
+//
+// Every kernel covers all replicas of an engine in one launch (block ranges
+// per replica, see WMap).  R is the real type of the force path: double for
+// the plain FP64 path, float for the FP32 path (positions, velocities, the
+// integrator, M_z and all reductions stay double).
+#include "md_kernels.h"
+
+#include "../../include/md_engine.h"
+
+namespace mdk {
+
+template <typename R> struct CT;
+template <> struct CT<float> { using C = float2; };
+template <> struct CT<double> { using C = double2; };
+
+template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
+  C r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { C r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { C r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
+
+__device__ __forceinline__ void set_error(Ctrl *c, int code, int info, int step) {
+  if (atomicCAS(&c->err, 0, code) == 0) {
+    c->err_info = info;
+    c->err_step = step;
+  }
+}
+
+// Squared distance with the x,y minimum image of rij_and_rr
+// (glob_var.f90:72-105), rounded term by term as the reference writes it.
+__device__ __forceinline__ double rsq_image(double4 a, double4 b, double Lx, double Ly,
+                                            double hLx, double hLy, double &dx, double &dy,
+                                            double &dz) {
+#pragma clang fp contract(off)
+  dx = a.x - b.x;
+  dy = a.y - b.y;
+  dz = a.z - b.z;
+  if (dx > hLx) dx = dx - Lx;
+  else if (dx <= -hLx) dx = dx + Lx;
+  if (dy > hLy) dy = dy - Ly;
+  else if (dy <= -hLy) dy = dy + Ly;
+  return dx * dx + dy * dy + dz * dz;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// ------------------------------------------------------------- integrator
+
+// First half of new_position (init_update.f90:581-601) and
+// period_condition_pos (init_update.f90:631-642): half kick, drift, wrap,
+// maximum displacement of this step (all, charged).
+__global__ __launch_bounds__(kBlock) void k_kick_drift(const Rep *__restrict__ reps, WMap wm) {
+#pragma clang fp contract(off)
+  const int r = find_rep(wm, blockIdx.x);
+  const Rep &S = reps[r];
+  const int i = (blockIdx.x - wm.blk[r]) * kBlock + threadIdx.x;
+  double d1 = 0, d2 = 0;
+  if (i < S.NN) {
+    const double dt = S.dt;
+    double vx = S.ax[i] * dt / 2 + S.c1m * S.vx[i];
+    double vy = S.ay[i] * dt / 2 + S.c1m * S.vy[i];
+    double vz = S.az[i] * dt / 2 + S.c1m * S.vz[i];
+    double4 p = S.pos[i];
+    double dx = vx * dt, dy = vy * dt, dz = vz * dt;
+    p.x = p.x + dx;
+    p.y = p.y + dy;
+    p.z = p.z + dz;
+    double dr = sqrt(dx * dx + dy * dy + dz * dz);
+    if (p.x > S.hLx) p.x = p.x - S.Lx;
+    else if (p.x <= -S.hLx) p.x = p.x + S.Lx;
+    if (p.y > S.hLy) p.y = p.y - S.Ly;
+    else if (p.y <= -S.hLy) p.y = p.y + S.Ly;
+    S.vx[i] = vx;
+    S.vy[i] = vy;
+    S.vz[i] = vz;
+    S.pos[i] = p;
+    if (S.pos32) S.pos32[i] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
+    d1 = dr;
+    if (p.w != 0) d2 = dr;
+  }
+  __shared__ double red[2][kBlock / 64];
+  d1 = wave_max(d1);
+  d2 = wave_max(d2);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = d1;
+    red[1][threadIdx.x >> 6] = d2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kBlock / 64; w++) {
+      d1 = fmax(d1, red[0][w]);
+      d2 = fmax(d2, red[1][w]);
+    }
+    // non-negative doubles order like their bit patterns
+    atomicMax(&S.ctrl->dmax1_bits, (unsigned long long)__double_as_longlong(d1));
+    atomicMax(&S.ctrl->dmax2_bits, (unsigned long long)__double_as_longlong(d2));
+  }
+}
+
+// --------------------------------------------------------- short range
+
+// Compute_Force without the Coulomb evaluation (compt_acc.f90:847-861):
+// held Coulomb force + lj_force (compt_acc.f90:895-977) + fene_force
+// (compt_acc.f90:865-892) + noise + field, anchors zeroed; optionally the
+// second half kick of new_position (init_update.f90:606) and the dr_max
+// bookkeeping of new_position / update_verlet_list
+// (init_update.f90:600-601, compt_acc.f90:717-724).
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_short_range(const Rep *__restrict__ reps, WMap wm,
+                                                        int mode, int step_arg,
+                                                        const double *__restrict__ eta,
+                                                        double *__restrict__ out) {
+  const int r = find_rep(wm, blockIdx.x);
+  const Rep &S = reps[r];
+  const int blk = blockIdx.x - wm.blk[r];
+  const int i = blk * kBlock + threadIdx.x;
+  const int step = step_arg >= 0 ? step_arg : S.ctrl->step;
+  if ((mode & SR_DRMAX) && blk == 0 && threadIdx.x == 0) {
+    Ctrl *c = S.ctrl;
+    double d1 = __longlong_as_double((long long)c->dmax1_bits);
+    double d2 = __longlong_as_double((long long)c->dmax2_bits);
+    c->dmax1_bits = 0;
+    c->dmax2_bits = 0;
+    c->dr_max1 = c->dr_max1 + d1;
+    c->dr_max2 = c->dr_max2 + d2;
+    int f1 = c->dr_max1 > S.skin1;
+    int f2 = (c->dr_max2 > S.skin2) && S.real_verlet != 0;
+    if (f1) { c->dr_max1 = 0; c->gen1++; }
+    if (f2) { c->dr_max2 = 0; c->gen2++; }
+    c->flag1 = f1;
+    c->flag2 = f2;
+  }
+  if (i >= S.NN) return;
+  const double4 pi = S.pos[i];
+  R fx = 0, fy = 0, fz = 0;
+  // WCA-LJ over the full list: class by |q_i|+|q_j| (compt_acc.f90:922-952)
+  {
+    const int n = min(S.nbn1[i], S.nbcap1);
+    const R hLx = (R)S.hLx, hLy = (R)S.hLy, Lx = (R)S.Lx, Ly = (R)S.Ly;
+    const R rcl2 = (R)S.rcl2, rcl3 = (R)S.rcl3, rcl4 = (R)S.rcl4;
+    const R sig2 = (R)S.sigma2, sig3 = (R)S.sigma3;
+    R xi, yi, zi, qa;
+    if constexpr (sizeof(R) == 4) {
+      float4 p = S.pos32[i];
+      xi = p.x; yi = p.y; zi = p.z; qa = fabsf(p.w);
+    } else {
+      xi = pi.x; yi = pi.y; zi = pi.z; qa = fabs(pi.w);
+    }
+    const int *nb = S.nb1 + i;
+    for (int k = 0; k < n; k++) {
+      const int j = nb[(size_t)k * S.NN];
+      R xj, yj, zj, qj;
+      if constexpr (sizeof(R) == 4) {
+        float4 p = S.pos32[j];
+        xj = p.x; yj = p.y; zj = p.z; qj = p.w;
+      } else {
+        double4 p = S.pos[j];
+        xj = p.x; yj = p.y; zj = p.z; qj = p.w;
+      }
+      R dx = xi - xj, dy = yi - yj, dz = zi - zj;
+      if (dx > hLx) dx -= Lx; else if (dx <= -hLx) dx += Lx;
+      if (dy > hLy) dy -= Ly; else if (dy <= -hLy) dy += Ly;
+      const R rsq = dx * dx + dy * dy + dz * dz;
+      const int m = (int)(qa + (sizeof(R) == 4 ? fabsf((float)qj) : fabs((double)qj)));
+      R cut, s2;
+      if (m == 6) { cut = rcl3; s2 = sig2; }
+      else if (m < 3) { cut = rcl2; s2 = (R)1; }
+      else { cut = rcl4; s2 = sig3; }
+      if (rsq < cut) {
+        const R ir2 = s2 / rsq;
+        const R ir6 = ir2 * ir2 * ir2;
+        const R g = ir6 * (ir6 - (R)0.5) / rsq;
+        fx += g * dx;
+        fy += g * dy;
+        fz += g * dz;
+      }
+    }
+    fx *= (R)48;
+    fy *= (R)48;
+    fz *= (R)48;
+  }
+  double wz = 0;
+  {  // wall force (compt_acc.f90:956-975)
+    const double z = pi.z, Lz = S.Lz;
+    if (!(fabs(z) < 0.01 || fabs(z - Lz) < 0.01)) {
+      if (pi.w != 3) {
+        if (z < S.wall_r) {
+          double dl = z, d5 = dl * dl * dl * dl * dl, d11 = d5 * d5 * dl;
+          wz = S.wall_c * (3 / d11 - 1 / d5) * dl;
+        } else if (Lz - z < S.wall_r) {
+          double dl = Lz - z, d5 = dl * dl * dl * dl * dl, d11 = d5 * d5 * dl;
+          wz = -S.wall_c * (3 / d11 - 1 / d5) * dl;
+        }
+      } else {
+        const double ss = S.sigma_s;
+        if (z < S.wall_rs) {
+          double dl = z, t = ss / dl, t5 = t * t * t * t * t, t11 = t5 * t5 * t;
+          wz = S.wall_c * (3 * t11 - t5) * dl / ss / ss;
+        } else if (Lz - z < S.wall_rs) {
+          double dl = Lz - z, t = ss / dl, t5 = t * t * t * t * t, t11 = t5 * t5 * t;
+          wz = -S.wall_c * (3 * t11 - t5) * dl / ss / ss;
+        }
+      }
+    }
+  }
+  {  // FENE over the bonded partners of i
+    const int b0 = S.bstart[i], b1 = S.bstart[i + 1];
+    const R R02 = (R)S.R0_2, kR = (R)S.kR02;
+    for (int b = b0; b < b1; b++) {
+      const int n = S.bpart[b];
+      double dxd, dyd, dzd;
+      const double rs = rsq_image(pi, S.pos[n], S.Lx, S.Ly, S.hLx, S.hLy, dxd, dyd, dzd);
+      if (rs > S.R0_2) {
+        set_error(S.ctrl, MD_EBOND, i < n ? i : n, step);
+        continue;
+      }
+      const R g = -kR / (R02 - (R)rs);
+      fx += g * (R)dxd;
+      fy += g * (R)dyd;
+      fz += g * (R)dzd;
+    }
+  }
+  double ax = (double)fx, ay = (double)fy, az = (double)fz + wz;
+  if (mode & SR_HELD) {
+    if constexpr (sizeof(R) == 4) {
+      ax += ((const float *)S.ccx)[i];
+      ay += ((const float *)S.ccy)[i];
+      az += ((const float *)S.ccz)[i];
+    } else {
+      ax += ((const double *)S.ccx)[i];
+      ay += ((const double *)S.ccy)[i];
+      az += ((const double *)S.ccz)[i];
+    }
+  }
+  if (mode & (SR_NOISE_ETA | SR_NOISE_PHILOX)) {
+    double n0, n1, n2;
+    if (mode & SR_NOISE_ETA) {
+      n0 = eta[i];
+      n1 = eta[(size_t)S.NN + i];
+      n2 = eta[2 * (size_t)S.NN + i];
+    } else {
+      uint32_t w[4];
+      philox4x32_10((uint32_t)i, (uint32_t)step, kNoiseTag, 0u, (uint32_t)S.seed,
+                    (uint32_t)(S.seed >> 32), w);
+      const double s32 = 1.0 / 4294967296.0;
+      const double u0 = ((double)w[0] + 0.5) * s32, u1 = ((double)w[1] + 0.5) * s32;
+      const double u2 = ((double)w[2] + 0.5) * s32, u3 = ((double)w[3] + 0.5) * s32;
+      const double r1 = sqrt(-2 * log(u0)), r2 = sqrt(-2 * log(u2));
+      double sn, cs;
+      sincospi(2 * u1, &sn, &cs);
+      n0 = r1 * cs;
+      n1 = r1 * sn;
+      n2 = r2 * cospi(2 * u3);
+    }
+    ax += n0 * S.sig;
+    ay += n1 * S.sig;
+    az += n2 * S.sig;
+  }
+  if (mode & SR_FIELD) az += pi.w * S.EF;
+  if ((mode & SR_ANCHOR) && (S.flags[i] & 1)) ax = ay = az = 0;
+  if (!(isfinite(ax) && isfinite(ay) && isfinite(az))) set_error(S.ctrl, MD_ENONFINITE, i, step);
+  if (out) {
+    out[i] = ax;
+    out[(size_t)S.NN + i] = ay;
+    out[2 * (size_t)S.NN + i] = az;
+    return;
+  }
+  S.ax[i] = ax;
+  S.ay[i] = ay;
+  S.az[i] = az;
+  if (mode & SR_KICK) {
+#pragma clang fp contract(off)
+    const double dt = S.dt;
+    S.vx[i] = (S.vx[i] + ax * dt / 2.) / S.c1p;
+    S.vy[i] = (S.vy[i] + ay * dt / 2.) / S.c1p;
+    S.vz[i] = (S.vz[i] + az * dt / 2.) / S.c1p;
+  }
+}
+
+// ------------------------------------------------------- Verlet lists
+
+__global__ void k_set_flags(const Rep *__restrict__ reps, int nrep, int clear) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrep) return;
+  Ctrl *c = reps[r].ctrl;
+  if (clear) {
+    c->flag1 = c->flag2 = 0;
+    return;
+  }
+  c->flag1 = 1;
+  c->gen1++;
+  c->flag2 = reps[r].real_verlet != 0 && reps[r].Nq > 0;
+  if (c->flag2) c->gen2++;
+}
+
+struct CellGeo {
+  int nc[3];
+  double rc[3];
+};
+
+// Cell of a particle as lj_verlet_list computes it (compt_acc.f90:219-222).
+__device__ __forceinline__ bool cell_of(double4 p, const int nc[3], const double rc[3],
+                                        double hLx, double hLy, int &a, int &b, int &c) {
+  a = (int)((p.x + hLx) / rc[0]);
+  b = (int)((p.y + hLy) / rc[1]);
+  c = (int)(p.z / rc[2]);
+  return a >= 0 && a < nc[0] && b >= 0 && b < nc[1] && c >= 0 && c < nc[2];
+}
+
+// Linked-cell binning of lj_verlet_list / real_verlet_list: blocks
+// [0, wm1.total) bin all particles into LJ cells, the rest bin charges into
+// real-space cells.
+__global__ __launch_bounds__(kBlock) void k_bin(const Rep *__restrict__ reps, WMap wm1, WMap wm2) {
+  const bool lj = (int)blockIdx.x < wm1.total;
+  const int bx = lj ? blockIdx.x : blockIdx.x - wm1.total;
+  const WMap &wm = lj ? wm1 : wm2;
+  const int r = find_rep(wm, bx);
+  const Rep &S = reps[r];
+  Ctrl *c = S.ctrl;
+  if (lj ? !c->flag1 : !c->flag2) return;
+  const int t = (bx - wm.blk[r]) * kBlock + threadIdx.x;
+  if (t >= (lj ? S.NN : S.Nq)) return;
+  const int i = lj ? t : S.charge[t];
+  const int *nc = lj ? S.nc1 : S.nc2;
+  const double *rc = lj ? S.rcel1 : S.rcel2;
+  int a, b, cz;
+  if (!cell_of(S.pos[i], nc, rc, S.hLx, S.hLy, a, b, cz)) {
+    set_error(c, MD_ECELL, i, -1);
+    return;
+  }
+  const int cell = (cz * nc[1] + b) * nc[0] + a;
+  const int buf = (lj ? c->gen1 : c->gen2) & 1;
+  int *cnt = lj ? S.cnt1[buf] : S.cnt2[buf];
+  const int cap = lj ? S.ccap1 : S.ccap2;
+  const int slot = atomicAdd(&cnt[cell], 1);
+  if (slot < cap) (lj ? S.mem1 : S.mem2)[(size_t)cell * cap + slot] = t;
+  else atomicOr(&c->ovf, lj ? 1 : 2);
+}
+
+// Sorts every cell's members ascending so that list order, and hence every
+// force sum, is independent of the order of the binning atomics.  LJ cells:
+// one thread per cell (insertion sort, cells hold a few tens).  Real-space
+// cells: one workgroup per cell (bitonic sort in LDS).
+__global__ __launch_bounds__(kBlock) void k_cellsort(const Rep *__restrict__ reps, WMap wm1,
+                                                     WMap wm2) {
+  const bool lj = (int)blockIdx.x < wm1.total;
+  const int bx = lj ? blockIdx.x : blockIdx.x - wm1.total;
+  const WMap &wm = lj ? wm1 : wm2;
+  const int r = find_rep(wm, bx);
+  const Rep &S = reps[r];
+  Ctrl *c = S.ctrl;
+  if (lj) {
+    if (!c->flag1) return;
+    const int cell = (bx - wm.blk[r]) * kBlock + threadIdx.x;
+    if (cell >= S.nc1[0] * S.nc1[1] * S.nc1[2]) return;
+    const int n = min(S.cnt1[c->gen1 & 1][cell], S.ccap1);
+    int *m = S.mem1 + (size_t)cell * S.ccap1;
+    for (int a = 1; a < n; a++) {
+      int v = m[a], b = a - 1;
+      while (b >= 0 && m[b] > v) {
+        m[b + 1] = m[b];
+        b--;
+      }
+      m[b + 1] = v;
+    }
+    return;
+  }
+  if (!c->flag2) return;
+  const int cell = bx - wm.blk[r];
+  const int n = min(S.cnt2[c->gen2 & 1][cell], S.ccap2);
+  int *m = S.mem2 + (size_t)cell * S.ccap2;
+  extern __shared__ int sbuf[];
+  int np = 1;
+  while (np < n) np <<= 1;
+  for (int k = threadIdx.x; k < np; k += blockDim.x) sbuf[k] = k < n ? m[k] : 0x7fffffff;
+  __syncthreads();
+  for (int size = 2; size <= np; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int k = threadIdx.x; k < np; k += blockDim.x) {
+        const int l = k ^ stride;
+        if (l > k) {
+          const bool up = (k & size) == 0;
+          const int x = sbuf[k], y = sbuf[l];
+          if ((x > y) == up) {
+            sbuf[k] = y;
+            sbuf[l] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  for (int k = threadIdx.x; k < n; k += blockDim.x) m[k] = sbuf[k];
+}
+
+// Fill pass of lj_verlet_list (compt_acc.f90:271-313) and real_verlet_list
+// (compt_acc.f90:580-624): 27-cell stencil in the reference's order, x,y
+// periodic, z open; pairs with i != j and r^2 < rv^2 (r^2 rounded as
+// rij_and_rr rounds it).  Lists are ELL: slot k of item t at [k*n + t].
+__global__ __launch_bounds__(kBlock) void k_build(const Rep *__restrict__ reps, WMap wm1, WMap wm2,
+                                                  int inc_step) {
+  const bool lj = (int)blockIdx.x < wm1.total;
+  const int bx = lj ? blockIdx.x : blockIdx.x - wm1.total;
+  const WMap &wm = lj ? wm1 : wm2;
+  const int r = find_rep(wm, bx);
+  const Rep &S = reps[r];
+  Ctrl *c = S.ctrl;
+  const int blk = bx - wm.blk[r];
+  // last kernel of a step: advance the step counter of graph-replayed steps
+  if (inc_step && lj && blk == 0 && threadIdx.x == 0) c->step++;
+  if (lj ? !c->flag1 : !c->flag2) return;
+  const int t = blk * kBlock + threadIdx.x;
+  const int *nc = lj ? S.nc1 : S.nc2;
+  const int ncell = nc[0] * nc[1] * nc[2];
+  const int buf = (lj ? c->gen1 : c->gen2) & 1;
+  {  // clear the other count buffer for the next build of this list
+    int *other = lj ? S.cnt1[buf ^ 1] : S.cnt2[buf ^ 1];
+    const int nblk = (lj ? wm1.blk[r + 1] : wm2.blk[r + 1]) - wm.blk[r];
+    for (int k = blk * kBlock + threadIdx.x; k < ncell; k += nblk * kBlock) other[k] = 0;
+  }
+  const int nitem = lj ? S.NN : S.Nq;
+  if (t >= nitem) return;
+  const int i = lj ? t : S.charge[t];
+  const double *rc = lj ? S.rcel1 : S.rcel2;
+  const double rvsq = lj ? S.rv1sq : S.rv2sq;
+  const int *cnt = lj ? S.cnt1[buf] : S.cnt2[buf];
+  const int *mem = lj ? S.mem1 : S.mem2;
+  const int ccap = lj ? S.ccap1 : S.ccap2;
+  const int cap = lj ? S.nbcap1 : S.nbcap2;
+  int *out = lj ? S.nb1 + t : S.nb2 + t;
+  const double4 pi = S.pos[i];
+  int a, b, cz;
+  if (!cell_of(pi, nc, rc, S.hLx, S.hLy, a, b, cz)) return;  // reported by k_bin
+  int n = 0;
+  for (int l = -1; l <= 1; l++) {
+    int p = a + l;
+    p = p >= nc[0] ? p - nc[0] : (p < 0 ? p + nc[0] : p);
+    for (int mm = -1; mm <= 1; mm++) {
+      int q = b + mm;
+      q = q >= nc[1] ? q - nc[1] : (q < 0 ? q + nc[1] : q);
+      for (int nn = -1; nn <= 1; nn++) {
+        const int z = cz + nn;
+        if (z >= nc[2] || z < 0) continue;
+        const int cell = (z * nc[1] + q) * nc[0] + p;
+        const int cn = min(cnt[cell], ccap);
+        const int *cm = mem + (size_t)cell * ccap;
+        for (int u = 0; u < cn; u++) {
+          const int tj = cm[u];
+          const int j = lj ? tj : S.charge[tj];
+          if (j == i) continue;
+          double dx, dy, dz;
+          const double rs = rsq_image(pi, S.pos[j], S.Lx, S.Ly, S.hLx, S.hLy, dx, dy, dz);
+          if (rs < rvsq) {
+            if (n < cap) out[(size_t)n * nitem] = j;
+            n++;
+          }
+        }
+      }
+    }
+  }
+  (lj ? S.nbn1 : S.nbn2)[t] = n;
+  if (n > cap) atomicOr(&c->ovf, lj ? 4 : 8);
+  atomicMax(lj ? &c->max_nb1 : &c->max_nb2, n);
+  int occ = 0;  // own cell occupancy, for capacity tracking
+  occ = cnt[(cz * nc[1] + b) * nc[0] + a];
+  atomicMax(lj ? &c->max_cell1 : &c->max_cell2, occ);
+}
+
+// ------------------------------------------------------------- SPME
+
+// B-spline weights of splcof (compt_acc.f90:1250-1321).  Returns the mesh
+// index of weight 0; w[t], dw[t] for t = 0..p-1 are SP(:,t+1), dSP(:,t+1):
+// weight t sits at grid point uuc - p + t.
+template <typename R>
+__device__ __forceinline__ int spline(double coord, double kg, int p, R *w, R *dw) {
+  const double uu = kg * coord;
+  const double uuc = ceil(uu);
+  const R ww = (R)(uu - uuc + 1);
+  if (p == 4) {  // closed form, compt_acc.f90:1272-1287
+    R s3 = (R)0.5 * ww * ww;
+    R s1 = (R)0.5 * ((R)1 - ww) * ((R)1 - ww);
+    R s2 = (R)1 - s1 - s3;
+    dw[0] = -s1;
+    dw[1] = s1 - s2;
+    dw[2] = s2 - s3;
+    dw[3] = s3;
+    w[3] = ww * s3 / (R)3;
+    w[2] = ((ww + (R)1) * s2 + ((R)3 - ww) * s3) / (R)3;
+    w[0] = ((R)1 - ww) * s1 / (R)3;
+    w[1] = (R)1 - w[0] - w[2] - w[3];
+  } else {
+    // A[t] = M_n(ww + t), t < n, built up from order 2 by the recursion
+    // M_n(x) = (x M_{n-1}(x) + (n-x) M_{n-1}(x-1)) / (n-1); the derivative
+    // is dM_p(x) = M_{p-1}(x) - M_{p-1}(x-1).
+    R A[kMaxOrder];
+    A[0] = ww;
+    A[1] = (R)1 - ww;
+    for (int n = 3; n <= p; n++) {
+      if (n == p)
+        for (int t = 0; t < p; t++) dw[t] = (t < n - 1 ? A[t] : (R)0) - (t >= 1 ? A[t - 1] : (R)0);
+      R B[kMaxOrder];
+      for (int t = 0; t < n; t++) {
+        const R x = ww + (R)t;                  // argument of M_n for slot t
+        const R hi = t < n - 1 ? A[t] : (R)0;   // M_{n-1}(x)
+        const R lo = t >= 1 ? A[t - 1] : (R)0;  // M_{n-1}(x-1)
+        B[t] = (x * hi + ((R)n - x) * lo) / (R)(n - 1);
+      }
+      for (int t = 0; t < n; t++) A[t] = B[t];
+    }
+    // SP(:,j) = M_p(ww + p - j) (compt_acc.f90:1290-1301): weight j-1 is slot p-j
+    for (int j = 0; j < p; j++) w[j] = A[p - 1 - j];
+    R d[kMaxOrder];
+    for (int j = 0; j < p; j++) d[j] = dw[p - 1 - j];
+    for (int j = 0; j < p; j++) dw[j] = d[j];
+  }
+  return (int)uuc - p;
+}
+
+__device__ __forceinline__ int wrapk(int v, int K) {
+  v %= K;
+  return v < 0 ? v + K : v;
+}
+
+// MapCharges (compt_acc.f90:1324-1365): charge spreading into a
+// fixed-point mesh (integer atomics: the sum is independent of order).
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_spread(const Rep *__restrict__ reps, WMap wm) {
+  const int r = find_rep(wm, blockIdx.x);
+  const Rep &S = reps[r];
+  const int m = (blockIdx.x - wm.blk[r]) * kBlock + threadIdx.x;
+  if (m >= S.Nq) return;
+  const int i = S.charge[m];
+  const double4 p = S.pos[i];
+  const int o = S.order;
+  R wx[kMaxOrder], wy[kMaxOrder], wz[kMaxOrder], d[kMaxOrder];
+  const int bx = spline<R>(p.x, S.kg[0], o, wx, d);
+  const int by = spline<R>(p.y, S.kg[1], o, wy, d);
+  const int bz = spline<R>(p.z, S.kg[2], o, wz, d);
+  const R q = (R)p.w;
+  for (int a = 0; a < o; a++) wx[a] *= q;
+  const int K1 = S.K[0], K2 = S.K[1], K3 = S.K[2];
+  int ix[kMaxOrder];
+  for (int a = 0; a < o; a++) ix[a] = wrapk(bx + a, K1);
+  for (int c = 0; c < o; c++) {
+    const int iz = wrapk(bz + c, K3);
+    for (int b = 0; b < o; b++) {
+      const int iy = wrapk(by + b, K2);
+      unsigned long long *row = S.meshq + ((size_t)iz * K2 + iy) * K1;
+      const R wzy = wz[c];
+      for (int a = 0; a < o; a++) {
+        const double v = (double)(wzy * (wx[a] * wy[b]));
+        const long long f = (long long)rint(v * kQScale);
+        atomicAdd(row + ix[a], (unsigned long long)f);
+      }
+    }
+  }
+}
+
+// In-LDS radix-2 FFTs over `nl` lines of length n = 2^ln stored line-major.
+// DIT: bit-reversed in, natural out.  DIF: natural in, bit-reversed out.
+// tw[k] = exp(-2 pi i k / n), k < n/2; sgn = -1 forward, +1 inverse.
+template <typename C>
+__device__ void fft_dit(C *buf, int ln, int nl, const C *__restrict__ tw, int sgn) {
+  const int n = 1 << ln, h = n >> 1;
+  const int nb = nl * h;
+  for (int s = 1; s <= ln; s++) {
+    const int half = 1 << (s - 1);
+    const int ts = n >> s;
+    for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+      const int line = t >> (ln - 1), b = t & (h - 1);
+      const int k = b & (half - 1), g = b >> (s - 1);
+      const int i0 = line * n + (g << s) + k, i1 = i0 + half;
+      C w = tw[k * ts];
+      if (sgn > 0) w.y = -w.y;
+      const C u = buf[i0], v = cmul(buf[i1], w);
+      buf[i0] = cadd(u, v);
+      buf[i1] = csub(u, v);
+    }
+    __syncthreads();
+  }
+}
+
+template <typename C>
+__device__ void fft_dif(C *buf, int ln, int nl, const C *__restrict__ tw, int sgn) {
+  const int n = 1 << ln, h = n >> 1;
+  const int nb = nl * h;
+  for (int s = ln; s >= 1; s--) {
+    const int half = 1 << (s - 1);
+    const int ts = n >> s;
+    for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+      const int line = t >> (ln - 1), b = t & (h - 1);
+      const int k = b & (half - 1), g = b >> (s - 1);
+      const int i0 = line * n + (g << s) + k, i1 = i0 + half;
+      C w = tw[k * ts];
+      if (sgn > 0) w.y = -w.y;
+      const C u = buf[i0], v = buf[i1];
+      buf[i0] = cadd(u, v);
+      buf[i1] = cmul(csub(u, v), w);
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ int bitrev(int x, int ln) { return (int)(__brev((unsigned)x) >> (32 - ln)); }
+
+// Forward x pass: pairs of real mesh rows (fixed point, cleared after the
+// read) as one complex FFT of length K1, split into two half spectra.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_fftx_fwd(const Rep *__restrict__ reps, WMap wm) {
+  using C = typename CT<R>::C;
+  extern __shared__ char smem[];
+  C *buf = (C *)smem;
+  const int r = find_rep(wm, blockIdx.x);
+  const Rep &S = reps[r];
+  const int K1 = S.K[0], ln = S.lK[0], Kh = S.Kh;
+  const int npairs = S.K[1] * S.K[2] / 2;
+  const int P = S.px_tile;
+  const int pair0 = (blockIdx.x - wm.blk[r]) * P;
+  const int nl = min(P, npairs - pair0);
+  for (int t = threadIdx.x; t < nl * K1; t += blockDim.x) {
+    const int l = t / K1, x = t - l * K1;
+    const size_t row = 2 * (size_t)(pair0 + l);
+    unsigned long long *q0 = S.meshq + row * K1 + x, *q1 = q0 + K1;
+    C v;
+    v.x = (R)((double)(long long)*q0 * kQInv);
+    v.y = (R)((double)(long long)*q1 * kQInv);
+    *q0 = 0ull;
+    *q1 = 0ull;
+    buf[l * K1 + bitrev(x, ln)] = v;
+  }
+  __syncthreads();
+  fft_dit<C>(buf, ln, nl, (const C *)S.tw[0], -1);
+  C *spec = (C *)S.spec;
+  for (int t = threadIdx.x; t < nl * Kh; t += blockDim.x) {
+    const int l = t / Kh, k = t - l * Kh;
+    const C zk = buf[l * K1 + k], zm = buf[l * K1 + ((K1 - k) & (K1 - 1))];
+    C A, B;  // A = (Z_k + conj Z_-k)/2, B = (Z_k - conj Z_-k)/(2i)
+    A.x = (R)0.5 * (zk.x + zm.x);
+    A.y = (R)0.5 * (zk.y - zm.y);
+    B.x = (R)0.5 * (zk.y + zm.y);
+    B.y = (R)0.5 * (zm.x - zk.x);
+    const size_t row = 2 * (size_t)(pair0 + l);
+    spec[row * Kh + k] = A;
+    spec[(row + 1) * Kh + k] = B;
+  }
+}
+
+// Inverse x pass: two half spectra -> one complex inverse FFT -> two real
+// rows of the potential mesh.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_fftx_inv(const Rep *__restrict__ reps, WMap wm) {
+  using C = typename CT<R>::C;
+  extern __shared__ char smem[];
+  C *buf = (C *)smem;
+  const int r = find_rep(wm, blockIdx.x);
+  const Rep &S = reps[r];
+  const int K1 = S.K[0], ln = S.lK[0], Kh = S.Kh;
+  const int npairs = S.K[1] * S.K[2] / 2;
+  const int P = S.px_tile;
+  const int pair0 = (blockIdx.x - wm.blk[r]) * P;
+  const int nl = min(P, npairs - pair0);
+  const C *spec = (const C *)S.spec;
+  for (int t = threadIdx.x; t < nl * K1; t += blockDim.x) {
+    const int l = t / K1, k = t - l * K1;
+    const size_t row = 2 * (size_t)(pair0 + l);
+    C A, B, z;
+    if (k < Kh) {
+      A = spec[row * Kh + k];
+      B = spec[(row + 1) * Kh + k];
+    } else {
+      A = spec[row * Kh + (K1 - k)];
+      B = spec[(row + 1) * Kh + (K1 - k)];
+      A.y = -A.y;
+      B.y = -B.y;
+    }
+    z.x = A.x - B.y;  // Z = A + iB
+    z.y = A.y + B.x;
+    buf[l * K1 + k] = z;
+  }
+  __syncthreads();
+  fft_dif<C>(buf, ln, nl, (const C *)S.tw[0], +1);
+  R *u = (R *)S.meshu;
+  for (int t = threadIdx.x; t < nl * K1; t += blockDim.x) {
+    const int l = t / K1, x = t - l * K1;
+    const C v = buf[l * K1 + bitrev(x, ln)];
+    const size_t row = 2 * (size_t)(pair0 + l);
+    u[row * K1 + x] = v.x;
+    u[(row + 1) * K1 + x] = v.y;
+  }
+}
+
+// y pass (forward or inverse), in place on the half spectrum: tiles of Ty
+// adjacent kx lines for one kz.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_ffty(const Rep *__restrict__ reps, WMap wm, int sgn) {
+  using C = typename CT<R>::C;
+  extern __shared__ char smem[];
+  C *buf = (C *)smem;
+  const int r = find_rep(wm, blockIdx.x);
+  const Rep &S = reps[r];
+  const int K2 = S.K[1], ln = S.lK[1], Kh = S.Kh, T = S.py_tile;
+  const int tiles = (Kh + T - 1) / T;
+  const int b = blockIdx.x - wm.blk[r];
+  const int z = b / tiles, x0 = (b - z * tiles) * T;
+  const int nl = min(T, Kh - x0);
+  C *spec = (C *)S.spec + (size_t)z * K2 * Kh + x0;
+  for (int t = threadIdx.x; t < nl * K2; t += blockDim.x) {
+    const int y = t / nl, l = t - y * nl;
+    buf[l * K2 + bitrev(y, ln)] = spec[(size_t)y * Kh + l];
+  }
+  __syncthreads();
+  fft_dit<C>(buf, ln, nl, (const C *)S.tw[1], sgn);
+  for (int t = threadIdx.x; t < nl * K2; t += blockDim.x) {
+    const int y = t / nl, l = t - y * nl;
+    spec[(size_t)y * Kh + l] = buf[l * K2 + y];
+  }
+}
+
+// z pass with the convolution of pmeOrthoConvBC (compt_acc.f90:1382-1396):
+// forward FFT along z, multiply by BC, inverse FFT along z; tiles of Tz
+// adjacent kx lines for one ky.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_fftz_conv(const Rep *__restrict__ reps, WMap wm) {
+  using C = typename CT<R>::C;
+  extern __shared__ char smem[];
+  C *buf = (C *)smem;
+  const int r = find_rep(wm, blockIdx.x);
+  const Rep &S = reps[r];
+  const int K2 = S.K[1], K3 = S.K[2], ln = S.lK[2], Kh = S.Kh, T = S.pz_tile;
+  const int tiles = (Kh + T - 1) / T;
+  const int b = blockIdx.x - wm.blk[r];
+  const int y = b / tiles, x0 = (b - y * tiles) * T;
+  const int nl = min(T, Kh - x0);
+  const size_t zs = (size_t)K2 * Kh;
+  C *spec = (C *)S.spec + (size_t)y * Kh + x0;
+  const R *bc = (const R *)S.bc + (size_t)y * Kh + x0;
+  for (int t = threadIdx.x; t < nl * K3; t += blockDim.x) {
+    const int z = t / nl, l = t - z * nl;
+    buf[l * K3 + bitrev(z, ln)] = spec[z * zs + l];
+  }
+  __syncthreads();
+  fft_dit<C>(buf, ln, nl, (const C *)S.tw[2], -1);
+  for (int t = threadIdx.x; t < nl * K3; t += blockDim.x) {
+    const int z = t / nl, l = t - z * nl;
+    const R g = bc[z * zs + l];
+    C v = buf[l * K3 + z];
+    v.x *= g;
+    v.y *= g;
+    buf[l * K3 + z] = v;
+  }
+  __syncthreads();
+  fft_dif<C>(buf, ln, nl, (const C *)S.tw[2], +1);
+  for (int t = threadIdx.x; t < nl * K3; t += blockDim.x) {
+    const int z = t / nl, l = t - z * nl;
+    spec[z * zs + l] = buf[l * K3 + bitrev(z, ln)];
+  }
+}
+
+// IFrc (compt_acc.f90:1402-1467): reciprocal force of each charge from the
+// potential mesh, plus per-block sums of the force (for ZeroForce,
+// compt_acc.f90:1470-1492) and of q z lb/Beta (for the slab dipole M_z,
+// compt_acc.f90:1041-1044), all in double.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_gather(const Rep *__restrict__ reps, WMap wm) {
+  const int r = find_rep(wm, blockIdx.x);
+  const Rep &S = reps[r];
+  const int blk = blockIdx.x - wm.blk[r];
+  const int m = blk * kBlock + threadIdx.x;
+  double f[4] = {0, 0, 0, 0};
+  if (m < S.Nq) {
+    const int i = S.charge[m];
+    const double4 p = S.pos[i];
+    const int o = S.order;
+    R wx[kMaxOrder], wy[kMaxOrder], wz[kMaxOrder], dx[kMaxOrder], dy[kMaxOrder], dz[kMaxOrder];
+    const int bx = spline<R>(p.x, S.kg[0], o, wx, dx);
+    const int by = spline<R>(p.y, S.kg[1], o, wy, dy);
+    const int bz = spline<R>(p.z, S.kg[2], o, wz, dz);
+    const R q = (R)p.w;
+    for (int a = 0; a < o; a++) {
+      wx[a] *= q;
+      dx[a] *= q;
+    }
+    const int K1 = S.K[0], K2 = S.K[1], K3 = S.K[2];
+    const R *u = (const R *)S.meshu;
+    int ix[kMaxOrder];
+    for (int a = 0; a < o; a++) ix[a] = wrapk(bx + a, K1);
+    R f1 = 0, f2 = 0, f3 = 0;
+    for (int c = 0; c < o; c++) {
+      const int iz = wrapk(bz + c, K3);
+      for (int b = 0; b < o; b++) {
+        const int iy = wrapk(by + b, K2);
+        const R *row = u + ((size_t)iz * K2 + iy) * K1;
+        R s1 = 0, s2 = 0;
+        for (int a = 0; a < o; a++) {
+          const R v = row[ix[a]];
+          s1 += dx[a] * v;
+          s2 += wx[a] * v;
+        }
+        f1 += wz[c] * wy[b] * s1;
+        f2 += wz[c] * dy[b] * s2;
+        f3 += dz[c] * wy[b] * s2;
+      }
+    }
+    f[0] = -(double)f1 * S.fnorm[0];
+    f[1] = -(double)f2 * S.fnorm[1];
+    f[2] = -(double)f3 * S.fnorm[2];
+    f[3] = p.w * p.z * S.c1;
+    S.fsp[m] = f[0];
+    S.fsp[(size_t)S.Nq + m] = f[1];
+    S.fsp[2 * (size_t)S.Nq + m] = f[2];
+  }
+  __shared__ double red[4][kBlock / 64];
+  for (int d = 0; d < 4; d++) {
+    double v = wave_sum(f[d]);
+    if ((threadIdx.x & 63) == 0) red[d][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double s = 0;
+    for (int w = 0; w < kBlock / 64; w++) s += red[threadIdx.x][w];
+    S.part[(size_t)blk * 4 + threadIdx.x] = s;
+  }
+}
+
+// real_space (compt_acc.f90:980-1049) and the assembly of acc_c: the
+// erfc-screened pair force over the real-space list (all charge pairs when
+// real_verlet == 0), the reciprocal force with its mean removed and scaled
+// by lb/Beta (compt_acc.f90:1242-1244), and the slab correction
+// (compt_acc.f90:1045-1048).  With out != null writes acc_c(NN,3) there.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_real(const Rep *__restrict__ reps, WMap wm,
+                                                 double *__restrict__ out) {
+  const int r = find_rep(wm, blockIdx.x);
+  const Rep &S = reps[r];
+  const int m = (blockIdx.x - wm.blk[r]) * kBlock + threadIdx.x;
+  __shared__ double tot[4];
+  if (threadIdx.x < 4) {  // same fixed-order sum in every block
+    double s = 0;
+    for (int b = 0; b < S.nqblk; b++) s += S.part[(size_t)b * 4 + threadIdx.x];
+    tot[threadIdx.x] = s;
+  }
+  __syncthreads();
+  if (m >= S.Nq) return;
+  const int i = S.charge[m];
+  const double4 pi = S.pos[i];
+  R fx = 0, fy = 0, fz = 0;
+  const R alpha = (R)S.alpha, alpha2 = (R)S.alpha2, c2 = (R)S.c2, rcc2 = (R)S.rcc2;
+  const R hLx = (R)S.hLx, hLy = (R)S.hLy, Lx = (R)S.Lx, Ly = (R)S.Ly;
+  R xi, yi, zi, qi;
+  if constexpr (sizeof(R) == 4) {
+    const float4 p = S.pos32[i];
+    xi = p.x; yi = p.y; zi = p.z; qi = p.w;
+  } else {
+    xi = pi.x; yi = pi.y; zi = pi.z; qi = pi.w;
+  }
+  auto pair = [&](int j) {
+    R xj, yj, zj, qj;
+    if constexpr (sizeof(R) == 4) {
+      const float4 p = S.pos32[j];
+      xj = p.x; yj = p.y; zj = p.z; qj = p.w;
+    } else {
+      const double4 p = S.pos[j];
+      xj = p.x; yj = p.y; zj = p.z; qj = p.w;
+    }
+    R dx = xi - xj, dy = yi - yj, dz = zi - zj;
+    if (dx > hLx) dx -= Lx; else if (dx <= -hLx) dx += Lx;
+    if (dy > hLy) dy -= Ly; else if (dy <= -hLy) dy += Ly;
+    const R rsq = dx * dx + dy * dy + dz * dz;
+    if (rsq < rcc2) {
+      const R rr = sqrt(rsq);
+      const R g = qi * qj / rsq * (erfc(alpha * rr) / rr + c2 * exp(-alpha2 * rsq));
+      fx += g * dx;
+      fy += g * dy;
+      fz += g * dz;
+    }
+  };
+  if (S.real_verlet) {
+    const int n = min(S.nbn2[m], S.nbcap2);
+    const int *nb = S.nb2 + m;
+    for (int k = 0; k < n; k++) pair(nb[(size_t)k * S.Nq]);
+  } else {
+    for (int k = 0; k < S.Nq; k++)
+      if (k != m) pair(S.charge[k]);
+  }
+  const double c1 = S.c1;
+  const double inv = 1.0 / S.Nq;
+  const double Mz = tot[3];
+  const double ax = (S.fsp[m] - tot[0] * inv) * c1 + (double)fx * c1;
+  const double ay = (S.fsp[(size_t)S.Nq + m] - tot[1] * inv) * c1 + (double)fy * c1;
+  const double az = (S.fsp[2 * (size_t)S.Nq + m] - tot[2] * inv) * c1 + (double)fz * c1 -
+                    S.slab * pi.w * Mz;
+  if (out) {
+    out[i] = ax;
+    out[(size_t)S.NN + i] = ay;
+    out[2 * (size_t)S.NN + i] = az;
+    return;
+  }
+  if constexpr (sizeof(R) == 4) {
+    ((float *)S.ccx)[i] = (float)ax;
+    ((float *)S.ccy)[i] = (float)ay;
+    ((float *)S.ccz)[i] = (float)az;
+  } else {
+    ((double *)S.ccx)[i] = ax;
+    ((double *)S.ccy)[i] = ay;
+    ((double *)S.ccz)[i] = az;
+  }
+}
+
+// ---------------------------------------------------- rescale_velocity
+
+// rescale_velocity (init_update.f90:498-552) in three passes with fixed-order
+// block sums: mean velocity over NN-N_anchor, mean square over the moving
+// particles, rescale to 3/Beta.
+__global__ __launch_bounds__(kBlock) void k_vsum(const Rep *__restrict__ reps, WMap wm, double *part,
+                                                 int pass) {
+  const int r = find_rep(wm, blockIdx.x);
+  const Rep &S = reps[r];
+  const int blk = blockIdx.x - wm.blk[r];
+  const int i = blk * kBlock + threadIdx.x;
+  double *pr = part + (size_t)wm.blk[r] * 4;
+  const int nb = wm.blk[r + 1] - wm.blk[r];
+  __shared__ double mean[3];
+  if (pass == 1) {
+    if (threadIdx.x < 3) {
+      double s = 0;
+      for (int b = 0; b < nb; b++) s += pr[(size_t)b * 4 + threadIdx.x];
+      mean[threadIdx.x] = s / (S.NN - S.N_anchor);
+    }
+    __syncthreads();
+  }
+  double v[4] = {0, 0, 0, 0};
+  if (i < S.NN) {
+    if (pass == 0) {
+      v[0] = S.vx[i];
+      v[1] = S.vy[i];
+      v[2] = S.vz[i];
+    } else if (!(S.flags[i] & 1)) {
+      const double x = S.vx[i] - mean[0], y = S.vy[i] - mean[1], z = S.vz[i] - mean[2];
+      v[3] = x * x + y * y + z * z;
+    }
+  }
+  __shared__ double red[4][kBlock / 64];
+  __syncthreads();
+  for (int d = 0; d < 4; d++) {
+    double s = wave_sum(v[d]);
+    if ((threadIdx.x & 63) == 0) red[d][threadIdx.x >> 6] = s;
+  }
+  __syncthreads();
+  // pass 0 writes slots 0..2 of its block; pass 1 writes slot 3 (keeping the
+  // sums of pass 0 intact for the apply kernel).
+  if (pass == 0 && threadIdx.x < 3) {
+    double s = 0;
+    for (int w = 0; w < kBlock / 64; w++) s += red[threadIdx.x][w];
+    pr[(size_t)blk * 4 + threadIdx.x] = s;
+  }
+  if (pass == 1 && threadIdx.x == 3) {
+    double s = 0;
+    for (int w = 0; w < kBlock / 64; w++) s += red[3][w];
+    pr[(size_t)blk * 4 + 3] = s;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_vscale(const Rep *__restrict__ reps, WMap wm,
+                                                   const double *part) {
+  const int r = find_rep(wm, blockIdx.x);
+  const Rep &S = reps[r];
+  const int i = (blockIdx.x - wm.blk[r]) * kBlock + threadIdx.x;
+  const double *pr = part + (size_t)wm.blk[r] * 4;
+  const int nb = wm.blk[r + 1] - wm.blk[r];
+  __shared__ double sh[4];
+  if (threadIdx.x < 4) {
+    double s = 0;
+    for (int b = 0; b < nb; b++) s += pr[(size_t)b * 4 + threadIdx.x];
+    sh[threadIdx.x] = s;
+  }
+  __syncthreads();
+  if (i >= S.NN) return;
+  if (S.flags[i] & 1) {
+    S.vx[i] = S.vy[i] = S.vz[i] = 0;
+    return;
+  }
+  const double n = S.NN - S.N_anchor;
+  const double v2 = sh[3] / (S.NN - S.N_anchor - 1);
+  const double s = sqrt(3.0 / S.Beta / v2);
+  S.vx[i] = (S.vx[i] - sh[0] / n) * s;
+  S.vy[i] = (S.vy[i] - sh[1] / n) * s;
+  S.vz[i] = (S.vz[i] - sh[2] / n) * s;
+}
+
+// --------------------------------------------------------- launchers
+
+// With ev != null, records ev[k] before kernel k and ev[8] after the last
+// (kernel timing); never used under graph capture.
+template <typename R>
+void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) {
+  auto mark = [&](int k) {
+    if (ev) (void)hipEventRecord(ev[k], st);
+  };
+  mark(0);
+  hipLaunchKernelGGL(k_spread<R>, dim3(P.chg.total), dim3(kBlock), 0, st, P.reps, P.chg);
+  mark(1);
+  hipLaunchKernelGGL(k_fftx_fwd<R>, dim3(P.fx.total), dim3(kBlock), P.lds_x, st, P.reps, P.fx);
+  mark(2);
+  hipLaunchKernelGGL(k_ffty<R>, dim3(P.fy.total), dim3(kBlock), P.lds_y, st, P.reps, P.fy, -1);
+  mark(3);
+  hipLaunchKernelGGL(k_fftz_conv<R>, dim3(P.fz.total), dim3(kBlock), P.lds_z, st, P.reps, P.fz);
+  mark(4);
+  hipLaunchKernelGGL(k_ffty<R>, dim3(P.fy.total), dim3(kBlock), P.lds_y, st, P.reps, P.fy, +1);
+  mark(5);
+  hipLaunchKernelGGL(k_fftx_inv<R>, dim3(P.fx.total), dim3(kBlock), P.lds_x, st, P.reps, P.fx);
+  mark(6);
+  hipLaunchKernelGGL(k_gather<R>, dim3(P.chg.total), dim3(kBlock), 0, st, P.reps, P.chg);
+  mark(7);
+  hipLaunchKernelGGL(k_real<R>, dim3(P.chg.total), dim3(kBlock), 0, st, P.reps, P.chg, out);
+  mark(8);
+}
+
+template <typename R>
+void launch_short(const Plan &P, hipStream_t st, int mode, int step, const double *eta,
+                  double *out) {
+  hipLaunchKernelGGL(k_short_range<R>, dim3(P.part.total), dim3(kBlock), 0, st, P.reps, P.part,
+                     mode, step, eta, out);
+}
+
+void launch_kick_drift(const Plan &P, hipStream_t st) {
+  hipLaunchKernelGGL(k_kick_drift, dim3(P.part.total), dim3(kBlock), 0, st, P.reps, P.part);
+}
+
+void launch_lists(const Plan &P, hipStream_t st, int inc_step, hipEvent_t *ev) {
+  const int nb = P.part.total + P.chg.total;
+  if (ev) (void)hipEventRecord(ev[0], st);
+  hipLaunchKernelGGL(k_bin, dim3(nb), dim3(kBlock), 0, st, P.reps, P.part, P.chg);
+  if (ev) (void)hipEventRecord(ev[1], st);
+  hipLaunchKernelGGL(k_cellsort, dim3(P.cell1.total + P.cell2.total), dim3(kBlock), P.lds_sort, st,
+                     P.reps, P.cell1, P.cell2);
+  if (ev) (void)hipEventRecord(ev[2], st);
+  hipLaunchKernelGGL(k_build, dim3(nb), dim3(kBlock), 0, st, P.reps, P.part, P.chg, inc_step);
+  if (ev) (void)hipEventRecord(ev[3], st);
+}
+
+void launch_set_flags(const Plan &P, hipStream_t st, int clear) {
+  hipLaunchKernelGGL(k_set_flags, dim3((P.nrep + 63) / 64), dim3(64), 0, st, P.reps, P.nrep, clear);
+}
+
+void launch_rescale(const Plan &P, hipStream_t st) {
+  hipLaunchKernelGGL(k_vsum, dim3(P.part.total), dim3(kBlock), 0, st, P.reps, P.part, P.vpart, 0);
+  hipLaunchKernelGGL(k_vsum, dim3(P.part.total), dim3(kBlock), 0, st, P.reps, P.part, P.vpart, 1);
+  hipLaunchKernelGGL(k_vscale, dim3(P.part.total), dim3(kBlock), 0, st, P.reps, P.part, P.vpart);
+}
+
+template void launch_coulomb<float>(const Plan &, hipStream_t, double *, hipEvent_t *);
+template void launch_coulomb<double>(const Plan &, hipStream_t, double *, hipEvent_t *);
+template void launch_short<float>(const Plan &, hipStream_t, int, int, const double *, double *);
+template void launch_short<double>(const Plan &, hipStream_t, int, int, const double *, double *);
+
+}  // namespace mdk

@@ -1,0 +1,178 @@
+"""ctypes access to the FP64 CPU oracle (oracle/liborc.so) for the tests.
+
+Test infrastructure: the oracle is the checker of the HIP engine.
+"""
+import ctypes
+import os
+import subprocess
+from ctypes import POINTER, byref, c_double, c_int32, c_int64, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+from md_brushes_amd.abi import MdParams
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORC_PATH = os.path.join(ROOT, "oracle", "liborc.so")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(ORC_PATH):
+        subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+    L = ctypes.CDLL(ORC_PATH)
+    P = POINTER(MdParams)
+    dp, ip = POINTER(c_double), POINTER(c_int32)
+    sig = {
+        "orc_derive": ([P], c_int32),
+        "orc_build_topology": ([P, ip, ip, ip], c_int32),
+        "orc_create": ([P], c_void_p),
+        "orc_destroy": ([c_void_p], None),
+        "orc_set_state": ([c_void_p, dp, dp, dp], None),
+        "orc_get_state": ([c_void_p, dp, dp, dp, dp, dp], None),
+        "orc_set_drmax": ([c_void_p, c_double, c_double], None),
+        "orc_lj_verlet_list": ([c_void_p], c_int64),
+        "orc_real_verlet_list": ([c_void_p], c_int64),
+        "orc_get_list": ([c_void_p, c_int32, ip, c_int64], c_int64),
+        "orc_lj_force": ([c_void_p, dp], c_int32),
+        "orc_fene_force": ([c_void_p, dp], c_int32),
+        "orc_real_space": ([c_void_p, dp], None),
+        "orc_spme": ([c_void_p, dp], None),
+        "orc_standard_ewald": ([c_void_p, c_double, c_int32, c_int32, c_int32, dp], None),
+        "orc_compute_force": ([c_void_p, c_int32, c_int32, dp], c_int32),
+        "orc_new_position": ([c_void_p, c_int32, c_int32, dp], c_int32),
+        "orc_update_verlet_list": ([c_void_p], c_int32),
+        "orc_rescale_velocity": ([c_void_p], None),
+        "orc_run": ([c_void_p, c_int32, c_int32, c_int32], c_int32),
+        "orc_fft3d": ([c_int32, c_int32, c_int32, dp, c_int32], None),
+        "orc_bspln_coeffs": ([c_int32, dp, dp], None),
+        "orc_bspln": ([c_double, c_int32], c_double),
+        "orc_splcof": ([c_int32, dp, c_int32, dp, dp, dp, dp], None),
+        "orc_pme_bc_axis": ([c_int32, c_int32, c_double, c_double, dp, dp, dp], None),
+        "orc_philox4x32_10": ([POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint32)], None),
+        "orc_noise_normals": ([c_uint64, c_int32, c_int32, dp], None),
+    }
+    for name, (a, r) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = a
+        f.restype = r
+    _lib = L
+    return L
+
+
+def dp(a):
+    return None if a is None else a.ctypes.data_as(POINTER(c_double))
+
+
+def ip(a):
+    return None if a is None else a.ctypes.data_as(POINTER(c_int32))
+
+
+def colmajor(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).T).reshape(-1)
+
+
+def derive(p):
+    q = p.copy()
+    rc = lib().orc_derive(byref(q))
+    assert rc == 0
+    return q
+
+
+class Oracle:
+    """One system in the oracle; arrays are (NN,k)."""
+
+    def __init__(self, p):
+        self.p = p.copy()
+        self.NN = p.NN
+        self.h = lib().orc_create(byref(self.p))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def set_state(self, pos, vel=None, acc=None):
+        lib().orc_set_state(self.h, dp(colmajor(pos)), dp(None if vel is None else colmajor(vel)),
+                            dp(None if acc is None else colmajor(acc)))
+
+    def state(self):
+        N = self.NN
+        pos, vel, acc, accc, dr = (np.zeros(4 * N), np.zeros(3 * N), np.zeros(3 * N), np.zeros(3 * N),
+                                   np.zeros(2))
+        lib().orc_get_state(self.h, dp(pos), dp(vel), dp(acc), dp(accc), dp(dr))
+        return pos.reshape(4, N).T, vel.reshape(3, N).T, acc.reshape(3, N).T, accc.reshape(3, N).T, dr
+
+    def set_drmax(self, d1, d2):
+        lib().orc_set_drmax(self.h, d1, d2)
+
+    def build_lists(self):
+        n1 = lib().orc_lj_verlet_list(self.h)
+        assert n1 >= 0
+        if self.p.real_verlet and self.p.qq != 0:
+            assert lib().orc_real_verlet_list(self.h) >= 0
+
+    def lists(self, which):
+        n = lib().orc_get_list(self.h, which, None, 0)
+        pairs = np.zeros(2 * max(n, 1), np.int32)
+        lib().orc_get_list(self.h, which, ip(pairs), n)
+        return pairs[: 2 * n].reshape(2, n).T.copy()
+
+    def _f(self, fn):
+        f = np.zeros(3 * self.NN)
+        rc = fn(self.h, dp(f))
+        return f.reshape(3, self.NN).T.copy(), rc
+
+    def lj_force(self):
+        return self._f(lib().orc_lj_force)[0]
+
+    def fene_force(self):
+        f, rc = self._f(lib().orc_fene_force)
+        return f, rc
+
+    def real_space(self):
+        return self._f(lib().orc_real_space)[0]
+
+    def spme(self):
+        return self._f(lib().orc_spme)[0]
+
+    def coulomb(self):
+        return self.spme() + self.real_space()
+
+    def standard_ewald(self, alpha, K):
+        f = np.zeros(3 * self.NN)
+        lib().orc_standard_ewald(self.h, alpha, K[0], K[1], K[2], dp(f))
+        return f.reshape(3, self.NN).T.copy()
+
+    def compute_force(self, step, noise=0, eta=None):
+        return lib().orc_compute_force(self.h, step, noise, dp(None if eta is None else colmajor(eta)))
+
+    def new_position(self, step, noise=0, eta=None):
+        return lib().orc_new_position(self.h, step, noise, dp(None if eta is None else colmajor(eta)))
+
+    def update_verlet_list(self):
+        return lib().orc_update_verlet_list(self.h)
+
+    def rescale_velocity(self):
+        lib().orc_rescale_velocity(self.h)
+
+    def run(self, step0, n, preheat=False):
+        return lib().orc_run(self.h, step0, n, int(preheat))
+
+
+def philox(ctr, key):
+    c = (c_uint32 * 4)(*ctr)
+    k = (c_uint32 * 2)(*key)
+    o = (c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def rel_rms(a, b):
+    """sqrt(sum|a-b|^2 / sum|b|^2): the relative force RMS of error_analysis
+    (compt_acc.f90:795-803)."""
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2)))

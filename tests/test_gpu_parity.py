@@ -1,0 +1,200 @@
+"""Parity of the HIP engine with the FP64 oracle, through the C-ABI.
+
+Bars: neighbour lists bit-exact (as sorted pair sets); FP64 forces within
+1e-11 relative RMS of the oracle; FP32 forces within 1e-4 relative RMS
+(the FP32 accuracy gate); one step's drift bit-exact; injected-noise
+trajectories step for step within 1e-9 (FP64).
+"""
+import numpy as np
+import pytest
+
+import orc
+from md_brushes_amd import MD_FP32, MD_FP64, MD_NOISE_ETA, MD_NOISE_PHILOX, Engine, MDError
+from systems import etas, system
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ["small", "default"]
+
+
+def _engine(name, prec=MD_FP64):
+    p, pos, vel = system(name, prec)
+    e = Engine(p)
+    e.upload(pos, vel)
+    e.build_lists()
+    return p, pos, vel, e
+
+
+def _oracle(name):
+    p, pos, vel = system(name, MD_FP64)
+    o = orc.Oracle(p)
+    o.set_state(pos, vel)
+    o.build_lists()
+    return o
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_lists_bit_exact(name):
+    print(f"[lists] {name}", flush=True)
+    p, pos, vel, e = _engine(name)
+    o = _oracle(name)
+    for which in (0, 1):
+        if which == 1 and not p.real_verlet:
+            continue
+        g = e.lists(which)
+        r = o.lists(which)
+        r = r[np.lexsort((r[:, 1], r[:, 0]))]
+        assert g.shape == r.shape, (which, g.shape, r.shape)
+        assert np.array_equal(g, r)
+    e.close()
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_forces_fp64(name):
+    print(f"[forces fp64] {name}", flush=True)
+    p, pos, vel, e = _engine(name)
+    o = _oracle(name)
+    fs, fc = e.force_parts()
+    ref_s = o.lj_force() + o.fene_force()[0]
+    ref_c = o.coulomb()
+    assert orc.rel_rms(fs, ref_s) < 1e-11
+    assert orc.rel_rms(fc, ref_c) < 1e-10
+    e.close()
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_forces_fp32_gate(name):
+    print(f"[forces fp32] {name}", flush=True)
+    p, pos, vel, e = _engine(name, MD_FP32)
+    o = _oracle(name)
+    fs, fc = e.force_parts()
+    ref = o.lj_force() + o.fene_force()[0] + o.coulomb()
+    err = orc.rel_rms(fs + fc, ref)
+    print(f"fp32 relative force rms {err:.3e}")
+    assert err < 1e-4
+    e.close()
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_step_with_injected_noise(name):
+    print(f"[trajectory] {name}", flush=True)
+    p, pos, vel, e = _engine(name)
+    o = _oracle(name)
+    noise = etas(p.NN, 12)
+    assert o.compute_force(0, MD_NOISE_ETA, noise[0]) == 0
+    e.compute_force(0, MD_NOISE_ETA, noise[0])
+    pg, vg, ag, _ = e.download()
+    po, vo, ao, _, _ = o.state()
+    assert orc.rel_rms(ag, ao) < 1e-10
+    # one step: the drift uses only state both sides hold bit-identically
+    acc0 = ao.copy()
+    e.upload(pos, vel, acc0)
+    e.build_lists()
+    o.set_state(pos, vel, acc0)
+    o.set_drmax(0, 0)
+    o.build_lists()
+    for s in range(1, 12):
+        assert o.new_position(s, MD_NOISE_ETA, noise[s]) == 0
+        assert o.update_verlet_list() == 0
+        e.new_position(s, MD_NOISE_ETA, noise[s])
+        e.update_verlet_list()
+        pg, vg, ag, dg = e.download()
+        po, vo, ao, _, do = o.state()
+        if s == 1:
+            assert np.array_equal(pg, po), np.abs(pg - po).max()
+        assert np.abs(pg - po).max() < 1e-9
+        assert orc.rel_rms(vg, vo) < 1e-9
+        assert np.allclose(dg, do, rtol=0, atol=1e-12)
+    e.close()
+
+
+def test_graph_run_matches_stepwise():
+    print("[graph vs stepwise] default", flush=True)
+    p, pos, vel, e1 = _engine("default")
+    p, pos, vel, e2 = _engine("default")
+    e1.compute_force(0, MD_NOISE_PHILOX)
+    e2.compute_force(0, MD_NOISE_PHILOX)
+    e1.run(1, 25)
+    for s in range(1, 26):
+        e2.new_position(s, MD_NOISE_PHILOX)
+        e2.update_verlet_list()
+    a = e1.download()
+    b = e2.download()
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    e1.close()
+    e2.close()
+
+
+def test_run_is_deterministic():
+    print("[determinism] default fp32", flush=True)
+    out = []
+    for _ in range(2):
+        p, pos, vel, e = _engine("default", MD_FP32)
+        e.compute_force(0, MD_NOISE_PHILOX)
+        e.run(1, 40)
+        out.append(e.download())
+        e.close()
+    for x, y in zip(out[0], out[1]):
+        assert np.array_equal(x, y)
+
+
+def test_philox_run_matches_oracle():
+    print("[philox run] small", flush=True)
+    p, pos, vel, e = _engine("small")
+    o = _oracle("small")
+    assert o.compute_force(0, MD_NOISE_PHILOX) == 0
+    e.compute_force(0, MD_NOISE_PHILOX)
+    assert o.run(1, 20) == 0
+    e.run(1, 20)
+    pg, vg, _, _ = e.download()
+    po, vo, _, _, _ = o.state()
+    assert np.abs(pg - po).max() < 1e-8
+    e.close()
+
+
+def test_rescale_velocity():
+    print("[rescale] small", flush=True)
+    p, pos, vel, e = _engine("small")
+    o = _oracle("small")
+    v = np.array(vel) * 1.7 + 0.05
+    e.upload(pos, v)
+    o.set_state(pos, v)
+    e.rescale_velocity()
+    o.rescale_velocity()
+    _, vg, _, _ = e.download()
+    _, vo, _, _, _ = o.state()
+    assert np.abs(vg - vo).max() < 1e-12
+    e.close()
+
+
+def test_bond_break_reported():
+    print("[bond break] small", flush=True)
+    p, pos, vel = system("small")
+    q = np.array(pos)
+    q[5, 2] += 3.0  # stretch a bond past R0 = 1.5
+    e = Engine(p)
+    e.upload(q, vel)
+    e.build_lists()
+    e.compute_force(0)
+    with pytest.raises(MDError) as ei:
+        e.download()
+    assert ei.value.code == 2 and "bond" in str(ei.value)
+    e.close()
+
+
+@pytest.mark.parametrize("order", [3, 5, 6])
+def test_spme_general_order_fp64(order):
+    print(f"[spme order {order}] small", flush=True)
+    from md_brushes_amd import config
+    p = config("small", order=order)
+    _, pos, vel = system("small")
+    e = Engine(p)
+    e.upload(pos, vel)
+    e.build_lists()
+    o = orc.Oracle(p)
+    o.set_state(pos, vel)
+    o.build_lists()
+    _, fc = e.force_parts()
+    assert orc.rel_rms(fc, o.coulomb()) < 1e-10
+    e.close()

@@ -1,0 +1,412 @@
+"""Pins of the FP64 oracle against facts it does not compute itself:
+published known answers, closed forms, brute force, numpy, finite
+differences of the textbook potentials, and standard Ewald.  (The reference
+ships no tests or golden vectors; see tests/golden/README.md.)
+"""
+import math
+
+import numpy as np
+import pytest
+
+import orc
+from md_brushes_amd import MdParams, config, initialize_position, initialize_velocity
+from md_brushes_amd.configs import _apply, _base
+
+
+# ----------------------------------------------------------------- noise
+
+def test_philox_known_answers():
+    # Random123 kat_vectors, philox4x32 with 10 rounds
+    assert orc.philox([0, 0, 0, 0], [0, 0]) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert orc.philox([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2) == [0x408F276D, 0x41C83B0E, 0xA20BC7C6,
+                                                              0x6D5451FD]
+    assert orc.philox([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0]) == [
+        0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+def test_noise_normals_moments():
+    import ctypes
+    L = orc.lib()
+    out = np.zeros(3)
+    draws = []
+    for i in range(20000):
+        L.orc_noise_normals(12345, 7, i, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        draws.append(out.copy())
+    d = np.array(draws)
+    assert np.all(np.abs(d.mean(0)) < 0.03)
+    assert np.all(np.abs(d.var(0) - 1) < 0.04)
+    c = np.corrcoef(d.T)
+    assert np.all(np.abs(c - np.eye(3)) < 0.03)
+    # fourth moment of a normal is 3
+    assert np.all(np.abs((d ** 4).mean(0) - 3) < 0.2)
+
+
+# ------------------------------------------------------------ parameters
+
+@pytest.mark.parametrize("name,NN,Nq,NB", [("small", 1072, 512, 800), ("default", 21396, 21332, 6400),
+                                          ("star3", 21184, 21120, 6336), ("mixed", 85588, 85332, 25600)])
+def test_particle_counts(name, NN, Nq, NB):
+    # hand counts from the reference's formulas (in_output.f90:250-283)
+    p = orc.derive(config(name))
+    assert (p.NN, p.Nq, p.N_bond) == (NN, Nq, NB)
+    assert p.NN == p.Npe + p.Nq - p.Nq_PE  # every ion charged, PE charges counted once
+
+
+def test_reference_ewald_rule():
+    # reference rule at the default brush geometry: alpha = 0.2079,
+    # r_c = 9.139, mesh 45x45x162 (computed independently from the formulas
+    # of compt_acc.f90:387-420 while surveying the reference)
+    p = _base()
+    _apply(p, [("Nga", 0), ("Ngl", 64), ("Nml", 101), ("man_l", 1), ("man_s", 1), ("qqi", 3.0),
+               ("ion_ratio", 1.0), ("tol", 1.9), ("tau_rf", 800.0), ("ordr", (6, 6, 6))])
+    q = orc.derive(p)
+    assert abs(q.alpha - 0.2079) < 5e-5
+    assert abs(q.rcc - 9.139) < 1e-3
+    assert list(q.ng) == [45, 45, 162]
+    assert q.real_verlet == 1
+    assert abs(q.Lx - 113.137) < 1e-3
+
+
+def test_single_precision_constants():
+    p = orc.derive(config("default"))
+    two6 = float(np.float32(2.0) ** np.float32(1.0 / 6.0))  # default-real 2**(1./6)
+    assert p.rcl_s == two6 * 2.0
+    assert p.rvl_eff == two6 * 2.0 + 1
+
+
+# ------------------------------------------------------------- B-splines
+
+def _splines(order, w):
+    import ctypes
+    L = orc.lib()
+    cof = np.zeros(144)
+    dcof = np.zeros(144)
+    L.orc_bspln_coeffs(order, orc.dp(cof), orc.dp(dcof))
+    sp = np.zeros(order * len(w))
+    dsp = np.zeros(order * len(w))
+    L.orc_splcof(order, orc.dp(np.asarray(w, float)), len(w), orc.dp(cof), orc.dp(dcof), orc.dp(sp),
+                 orc.dp(dsp))
+    return sp.reshape(len(w), order), dsp.reshape(len(w), order)
+
+
+@pytest.mark.parametrize("order", [3, 4, 5, 6, 7, 8])
+def test_bspline_pieces_match_recursion(order):
+    L = orc.lib()
+    w = np.linspace(0.01, 1.0, 37)
+    sp, dsp = _splines(order, w)
+    # the piecewise coefficients grow with the order; Horner at x ~ p carries
+    # their rounding (~1e-12 at order 6, ~1e-11 at order 8)
+    tol = 5e-12 if order <= 6 else 5e-11
+    for t, ww in enumerate(w):
+        for j in range(1, order + 1):
+            x = ww + order - j  # SP(:,j) = M_p(w + p - j)
+            assert abs(sp[t, j - 1] - L.orc_bspln(x, order)) < tol
+            d = L.orc_bspln(x, order - 1) - L.orc_bspln(x - 1, order - 1)
+            assert abs(dsp[t, j - 1] - d) < 2 * tol
+        assert abs(sp[t].sum() - 1) < tol  # partition of unity
+        assert abs(dsp[t].sum()) < 2 * tol
+
+
+def test_cubic_bspline_closed_form():
+    L = orc.lib()
+    for x in np.linspace(0, 4, 41):
+        # M_4 of de Boor: piecewise cubic with knots 0..4
+        if x < 1:
+            m = x ** 3 / 6
+        elif x < 2:
+            m = (-3 * x ** 3 + 12 * x ** 2 - 12 * x + 4) / 6
+        elif x < 3:
+            m = (3 * x ** 3 - 24 * x ** 2 + 60 * x - 44) / 6
+        else:
+            m = (4 - x) ** 3 / 6
+        assert abs(L.orc_bspln(x, 4) - m) < 1e-13
+
+
+# ------------------------------------------------------------------- FFT
+
+@pytest.mark.parametrize("shape", [(8, 4, 16), (6, 5, 9), (32, 2, 4)])
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_fft_matches_numpy(shape, sign):
+    n1, n2, n3 = shape
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal((n3, n2, n1)) + 1j * rng.standard_normal((n3, n2, n1))
+    buf = np.ascontiguousarray(a).view(np.float64).copy()
+    orc.lib().orc_fft3d(n1, n2, n3, orc.dp(buf), sign)
+    got = buf.view(np.complex128).reshape(n3, n2, n1)
+    ref = np.fft.fftn(a) if sign < 0 else np.fft.ifftn(a) * a.size
+    assert np.abs(got - ref).max() < 1e-11 * np.abs(ref).max()
+
+
+# ---------------------------------------------------------- Verlet lists
+
+def _charged_box():
+    """~1.9k particles in a 40x40x40 box with salt; real-space list in use."""
+    p = _base()
+    _apply(p, [("Nga", 2), ("Ngl", 14), ("arm", 3), ("Nma", 10), ("Nml", 31), ("man_l", 1),
+               ("man_s", 2), ("qqi", 3.0), ("ion_ratio", 0.5), ("Lz", 40.0), ("Z_empty", 1.0),
+               ("sigmag", 0.01), ("ovr_alpha", 0.5), ("ovr_ng", (32, 32, 64)), ("charged_arm", [1, 0, 1])])
+    return orc.derive(p)
+
+
+def _brute_pairs(pos, items, rv, Lx, Ly):
+    x, y, z = pos[:, 0], pos[:, 1], pos[:, 2]
+    out = []
+    items = np.asarray(items)
+    for i in items:
+        dx = x[i] - x[items]
+        dy = y[i] - y[items]
+        dz = z[i] - z[items]
+        dx = np.where(dx > Lx / 2, dx - Lx, np.where(dx <= -Lx / 2, dx + Lx, dx))
+        dy = np.where(dy > Ly / 2, dy - Ly, np.where(dy <= -Ly / 2, dy + Ly, dy))
+        r2 = dx * dx + dy * dy + dz * dz
+        js = items[(r2 < rv * rv) & (items != i)]
+        out.extend((i + 1, j + 1) for j in js)
+    return np.array(sorted(out), np.int32).reshape(-1, 2)
+
+
+def test_lists_match_brute_force():
+    p = _charged_box()
+    assert p.real_verlet == 1
+    pos = initialize_position(p, 5)
+    o = orc.Oracle(p)
+    o.set_state(pos)
+    o.build_lists()
+    lj = o.lists(0)
+    lj = lj[np.lexsort((lj[:, 1], lj[:, 0]))]
+    assert np.array_equal(lj, _brute_pairs(pos, np.arange(p.NN), p.rvl_eff, p.Lx, p.Ly))
+    from md_brushes_amd import build_topology
+    charge = build_topology(p)[0] - 1
+    re = o.lists(1)
+    re = re[np.lexsort((re[:, 1], re[:, 0]))]
+    assert np.array_equal(re, _brute_pairs(pos, charge, p.rvc, p.Lx, p.Ly))
+
+
+# ------------------------------------------------------ short-range forces
+
+def _short_energy(p, pos, bonds):
+    """Textbook potentials: WCA with the three size classes, FENE, and the
+    wall potential whose negative derivative is the reference's wall force."""
+    x = pos[:, :3]
+    q = np.abs(pos[:, 3])
+    N = len(x)
+    E = 0.0
+    d = x[:, None, :] - x[None, :, :]
+    d[..., 0] -= p.Lx * np.round(d[..., 0] / p.Lx)
+    d[..., 1] -= p.Ly * np.round(d[..., 1] / p.Ly)
+    r2 = (d ** 2).sum(-1)
+    m = (q[:, None] + q[None, :]).astype(int)
+    sig2 = np.where(m == 6, p.sigma_s ** 2, np.where(m < 3, 1.0, p.sigma_sp ** 2))
+    cut2 = np.where(m == 6, p.rcl_s ** 2, np.where(m < 3, p.rcl ** 2, p.rcl_sp ** 2))
+    iu = np.triu_indices(N, 1)
+    r2u, s2u, c2u = r2[iu], sig2[iu], cut2[iu]
+    sel = r2u < c2u
+    s6 = (s2u[sel] / r2u[sel]) ** 3
+    E += np.sum(4 * (s6 * s6 - s6))
+    for j, k in bonds:
+        dd = x[j] - x[k]
+        dd[0] -= p.Lx * round(dd[0] / p.Lx)
+        dd[1] -= p.Ly * round(dd[1] / p.Ly)
+        E += -0.5 * p.kFENE * p.R0_2 * math.log(1 - dd @ dd / p.R0_2)
+    c04 = float(np.float32(0.4))
+    c5 = float(np.float32(5.0) ** np.float32(1.0 / 6.0))
+    for i in range(N):
+        z = x[i, 2]
+        if abs(z) < 0.01 or abs(z - p.Lz) < 0.01:
+            continue
+        s = p.sigma_s if pos[i, 3] == 3 else 1.0
+        rw = (p.rcl_s if pos[i, 3] == 3 else p.rcl) / c5
+        for dl in (z, p.Lz - z):
+            if dl < rw:  # U = 0.4/3 ((s/dl)^9 - (s/dl)^3)
+                E += c04 / 3 * ((s / dl) ** 9 - (s / dl) ** 3)
+                break
+    return E
+
+
+def test_short_range_is_gradient():
+    p = _charged_box()
+    pos = initialize_position(p, 9)
+    # put ions into the walls' range: a salt cation (q = 3) near the bottom,
+    # counterions near the bottom and the top wall
+    pos[p.NN - 1, 2] = 1.5
+    pos[p.Npe, 2] = 0.8
+    pos[p.Npe + 1, 2] = p.Lz - 0.8
+    o = orc.Oracle(p)
+    o.set_state(pos)
+    o.build_lists()
+    f = o.lj_force() + o.fene_force()[0]
+    from md_brushes_amd import build_topology
+    fene = build_topology(p)[1] - 1
+    h = 1e-6
+    rng = np.random.default_rng(0)
+    picks = list(rng.choice(p.NN, 12, replace=False)) + [p.NN - 1, p.Npe, p.Npe + 1, 5, 40]
+    for i in picks:
+        for dim in range(3):
+            if abs(pos[i, 2]) < 0.01:
+                continue
+            pp, pm = pos.copy(), pos.copy()
+            pp[i, dim] += h
+            pm[i, dim] -= h
+            g = -(_short_energy(p, pp, fene) - _short_energy(p, pm, fene)) / (2 * h)
+            assert abs(f[i, dim] - g) < 1e-5 * max(1.0, abs(g)), (i, dim, f[i, dim], g)
+
+
+def test_pair_forces_balance():
+    p = _charged_box()
+    pos = initialize_position(p, 9)
+    o = orc.Oracle(p)
+    o.set_state(pos)
+    o.build_lists()
+    f = o.lj_force()
+    fb, rc = o.fene_force()
+    assert rc == 0
+    assert np.abs(f[:, :2].sum(0)).max() < 1e-9  # walls act along z only
+    assert np.abs(fb.sum(0)).max() < 1e-9
+    cr = o.real_space()
+    assert np.abs(cr[:, :2].sum(0)).max() < 1e-9
+
+
+def test_bond_break_status():
+    p = _charged_box()
+    pos = initialize_position(p, 9)
+    pos[3, 2] += 2.0
+    o = orc.Oracle(p)
+    o.set_state(pos)
+    assert o.fene_force()[1] == 2
+
+
+# --------------------------------------------------------------- Coulomb
+
+def _two_charges(sep):
+    p = _charged_box()
+    pos = initialize_position(p, 4)
+    pos[:, 3] = 0.0
+    a, b = p.NN - 1, p.NN - 2
+    pos[a, :] = [0.0, 0.0, 20.0, 1.0]
+    pos[b, :] = [sep, 0.0, 20.0, -1.0]
+    return p, pos, a, b
+
+
+def test_two_charges_follow_coulomb():
+    p, pos, a, b = _two_charges(2.0)
+    q = p.copy()
+    q.ovr_ng[0], q.ovr_ng[1], q.ovr_ng[2] = 64, 64, 128
+    q.ordr[0] = q.ordr[1] = q.ordr[2] = 6
+    q = orc.derive(q)
+    o = orc.Oracle(q)
+    o.set_state(pos)
+    o.build_lists()
+    f = o.coulomb()
+    # lB kT q_a q_b / r^2 toward b, with r = 2 (images at 40 sigma add < 0.3%)
+    expect = p.lb / p.Beta / 4.0
+    assert abs(f[a, 0] - expect) < 0.004 * expect
+    assert abs(f[b, 0] + expect) < 0.004 * expect
+    assert abs(f[a, 1]) < 1e-3 and abs(f[a, 2]) < 2e-3
+
+
+def test_spme_converges_to_standard_ewald():
+    p = _charged_box()
+    pos = initialize_position(p, 6)
+    def spme(ng):
+        q = p.copy()
+        q.ovr_ng[0], q.ovr_ng[1], q.ovr_ng[2] = ng
+        q.ordr[0] = q.ordr[1] = q.ordr[2] = 6
+        q = orc.derive(q)
+        o = orc.Oracle(q)
+        o.set_state(pos)
+        o.build_lists()
+        return o.coulomb()
+    # standard Ewald as error_analysis sets it up (compt_acc.f90:756-772):
+    # tol = 5, and a k-range of ceil(tol L alpha / pi)
+    e = p.copy()
+    e.tol = 5.0
+    e.ovr_alpha = 0.5
+    e.ovr_rcc = 0.0
+    e = orc.derive(e)
+    oe = orc.Oracle(e)
+    oe.set_state(pos)
+    oe.build_lists()
+    K = [math.ceil(5.0 * L * e.alpha / math.pi) for L in (e.Lx, e.Ly, e.Lz * e.Zfac)]
+    f_ew = oe.standard_ewald(e.alpha, K) + oe.real_space()
+    coarse = orc.rel_rms(spme((64, 64, 64)), f_ew)
+    fine = orc.rel_rms(spme((64, 64, 128)), f_ew)
+    # the mesh error falls with the spacing down to the real-space cut of the
+    # SPME side, erfc(3) = 2.2e-5
+    assert fine < 5e-5, fine
+    assert fine < coarse / 10, (coarse, fine)
+
+
+# ------------------------------------------------------------ integrator
+
+def _neutral_chains():
+    p = _base()
+    _apply(p, [("Nga", 0), ("Ngl", 9), ("Nml", 20), ("qq", 0.0), ("qqi", 0.0), ("Lz", 30.0),
+               ("sigmag", 0.05), ("xi", 0.0)])
+    return orc.derive(p)
+
+
+def test_energy_conserved_without_friction():
+    p = _neutral_chains()
+    pos = initialize_position(p, 3)
+    vel = initialize_velocity(p, 4) * 0.3
+    from md_brushes_amd import build_topology
+    fene = build_topology(p)[1] - 1
+    anchors = build_topology(p)[2] - 1
+    vel[anchors] = 0
+    o = orc.Oracle(p)
+    o.set_state(pos, vel)
+    o.build_lists()
+    assert o.compute_force(0) == 0
+
+    def energy():
+        ps, vs = o.state()[:2]
+        return _short_energy(p, ps, fene) + 0.5 * np.sum(vs ** 2)
+
+    e0 = energy()
+    es = []
+    for s in range(1, 401):
+        assert o.new_position(s) == 0
+        assert o.update_verlet_list() == 0
+        es.append(energy())
+    drift = np.abs(np.array(es) - e0).max()
+    assert drift < 2e-3 * abs(e0), (drift, e0)
+
+
+def test_friction_decay_closed_form():
+    # a free particle's velocity obeys v_{n+1} = v_n (1 - xi dt/2)/(1 + xi dt/2)
+    p = _neutral_chains()
+    p.xi = 1.0
+    pos = initialize_position(p, 3)
+    pos[:, 2] += 0.0
+    vel = np.zeros((p.NN, 3))
+    o = orc.Oracle(p)
+    # one particle far from every other one and from the walls
+    far = p.NN - 1
+    pos[far, :3] = [0.0, 0.0, 15.0]
+    vel[far] = [0.3, -0.2, 0.1]
+    o.set_state(pos, vel)
+    o.build_lists()
+    o.compute_force(0)
+    acc0 = o.state()[2][far]
+    if np.abs(acc0).max() > 0:
+        pytest.skip("particle not isolated")
+    for s in range(1, 6):
+        o.new_position(s)
+    v = o.state()[1][far]
+    c = (1 - p.xi * p.dt / 2) / (1 + p.xi * p.dt / 2)
+    assert np.allclose(v, np.array([0.3, -0.2, 0.1]) * c ** 5, rtol=1e-13)
+
+
+def test_rescale_sets_temperature():
+    p = config("small")
+    pos = initialize_position(p, 3)
+    vel = initialize_velocity(p, 4) * 2.3 + 0.1
+    o = orc.Oracle(p)
+    o.set_state(pos, vel)
+    o.rescale_velocity()
+    v = o.state()[1]
+    from md_brushes_amd import build_topology
+    anchors = build_topology(p)[2] - 1
+    assert np.all(v[anchors] == 0)
+    moving = np.setdiff1d(np.arange(p.NN), anchors)
+    v2 = np.sum(v[moving] ** 2) / (p.NN - p.N_anchor - 1)
+    assert abs(v2 - 3 / p.Beta) < 1e-10

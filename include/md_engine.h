@@ -181,6 +181,45 @@ int md_force_parts(md_engine *e, double *f_short, double *f_coul);
  * cellsort, build.  Leaves the state advanced by the timed kick-drifts. */
 int md_time_kernels(md_engine *e, int32_t iters, double *ms, int32_t n);
 
+/* ---- Replica batches (no reference counterpart: the reference runs one
+ * system per process).  One engine holds many independent replicas of
+ * uneven size and one precision; every kernel launch covers all of them, and
+ * a replica's result does not depend on which others share the launch. */
+
+/* Empty engine for replicas of the given precision and Coulomb interval. */
+int md_batch_create(md_engine **e, int32_t precision, int32_t multistep);
+
+/* Adds a replica with state pos(NN,4), vel(NN,3): builds its lists and its
+ * step-0 force (Compute_Force at main.f90:37); its first step is 1.
+ * *slot is a stable handle. */
+int md_batch_add(md_engine *e, const md_params *p, const double *pos,
+                 const double *vel, int32_t *slot);
+
+/* Frees a replica's device memory. */
+int md_batch_remove(md_engine *e, int32_t slot);
+
+/* Runs ncycles * multistep steps (new_position + update_verlet_list, Philox
+ * noise) of every replica in one graph-replayed launch sequence. */
+int md_batch_run(md_engine *e, int32_t ncycles);
+
+/* Status of one replica (MD_OK or its device-side error) and its next step. */
+int md_batch_status(md_engine *e, int32_t slot, int32_t *step, char *buf,
+                    int32_t len);
+
+/* State of one replica; any pointer may be NULL. */
+int md_batch_download(md_engine *e, int32_t slot, double *pos, double *vel,
+                      double *acc);
+
+/* Device bytes held by one replica (slot >= 0) or by all (slot < 0), and the
+ * replica count. */
+int md_batch_info(md_engine *e, int32_t slot, int64_t *bytes, int32_t *nrep);
+
+/* Observables computed on the device: brush height (mean z of the polymer
+ * beads, in_output.f90:1017-1020) and the z-density histogram of the polymer
+ * beads, nbins = 500 bins of Lz/500 (in_output.f90:1878-1886). */
+int md_batch_observe(md_engine *e, int32_t slot, double *height,
+                     double *profile, int32_t nbins);
+
 /* Copies the last error message (NUL-terminated) into buf. */
 int md_last_error(md_engine *e, char *buf, int32_t len);
 

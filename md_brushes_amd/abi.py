@@ -102,6 +102,14 @@ def load_library(path=None):
         "md_last_error": ([c_void_p, c_char_p, _I], _I),
         "md_time_kernels": ([c_void_p, _I, dp, _I], _I),
         "md_destroy": ([c_void_p], None),
+        "md_batch_create": ([POINTER(c_void_p), _I, _I], _I),
+        "md_batch_add": ([c_void_p, P, dp, dp, ip], _I),
+        "md_batch_remove": ([c_void_p, _I], _I),
+        "md_batch_run": ([c_void_p, _I], _I),
+        "md_batch_status": ([c_void_p, _I, ip, c_char_p, _I], _I),
+        "md_batch_download": ([c_void_p, _I, dp, dp, dp], _I),
+        "md_batch_info": ([c_void_p, _I, POINTER(c_int64), ip], _I),
+        "md_batch_observe": ([c_void_p, _I, dp, dp, _I], _I),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -256,3 +264,70 @@ class Engine:
         fs, fc = np.zeros(3 * N), np.zeros(3 * N)
         self._check(self.lib.md_force_parts(self.h, _dp(fs), _dp(fc)), "md_force_parts")
         return fs.reshape(3, N).T, fc.reshape(3, N).T
+
+
+class Batch:
+    """Many replicas of uneven size in one engine (md_batch_* entry points)."""
+
+    HIST_BINS = 500
+
+    def __init__(self, precision, multistep=10):
+        self.lib = load_library()
+        self.h = c_void_p()
+        self.sizes = {}
+        rc = self.lib.md_batch_create(byref(self.h), precision, multistep)
+        self._check(rc, "md_batch_create")
+
+    _check = Engine._check
+
+    def close(self):
+        if self.h:
+            self.lib.md_destroy(self.h)
+            self.h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, params, pos, vel):
+        slot = c_int32()
+        self._check(self.lib.md_batch_add(self.h, byref(params), _dp(_colmajor(pos)), _dp(_colmajor(vel)),
+                                          byref(slot)), "md_batch_add")
+        self.sizes[slot.value] = params.NN
+        return slot.value
+
+    def remove(self, slot):
+        self._check(self.lib.md_batch_remove(self.h, slot), "md_batch_remove")
+        self.sizes.pop(slot, None)
+
+    def run(self, ncycles):
+        self._check(self.lib.md_batch_run(self.h, ncycles), "md_batch_run")
+
+    def status(self, slot):
+        """(status code, next step, message)."""
+        step = c_int32()
+        buf = ctypes.create_string_buffer(256)
+        rc = self.lib.md_batch_status(self.h, slot, byref(step), buf, 256)
+        return rc, step.value, buf.value.decode(errors="replace")
+
+    def download(self, slot):
+        N = self.sizes[slot]
+        pos, vel, acc = np.zeros(4 * N), np.zeros(3 * N), np.zeros(3 * N)
+        self._check(self.lib.md_batch_download(self.h, slot, _dp(pos), _dp(vel), _dp(acc)), "md_batch_download")
+        return pos.reshape(4, N).T, vel.reshape(3, N).T, acc.reshape(3, N).T
+
+    def info(self, slot=-1):
+        b = c_int64()
+        n = c_int32()
+        self._check(self.lib.md_batch_info(self.h, slot, byref(b), byref(n)), "md_batch_info")
+        return b.value, n.value
+
+    def observe(self, slot):
+        """(brush height, z-density histogram of the polymer beads)."""
+        h = c_double()
+        prof = np.zeros(self.HIST_BINS)
+        self._check(self.lib.md_batch_observe(self.h, slot, byref(h), _dp(prof), self.HIST_BINS),
+                    "md_batch_observe")
+        return h.value, prof

@@ -13,6 +13,7 @@ namespace mdk {
 
 constexpr int kBlock = 256;
 constexpr int kMaxOrder = 8;
+constexpr int kHistBins = 500;  // SizeHist of the z-density profile (glob_var.f90:59)
 
 // Per-replica control block, written by kernels.
 struct Ctrl {
@@ -35,6 +36,7 @@ struct Ctrl {
 struct Rep {
   // sizes
   int NN, Nq, N_anchor, real_verlet;
+  int Npe, pad_npe;                // polymer beads are particles 0..Npe-1
   // geometry
   double Lx, Ly, Lz, hLx, hLy;
   // integrator (init_update.f90:555-607)

@@ -1,11 +1,19 @@
 // md_engine.cpp -- host side of the brush engine: device allocation, launch
-// plans, graph replay of the step loop, and the C-ABI of include/md_engine.h.
+// plans over many replicas, graph replay of the step loop, and the C-ABI of
+// include/md_engine.h.
+//
+// An engine holds replicas (independent systems, possibly of different
+// sizes, one precision).  Every kernel launch covers all replicas; a replica
+// owns its device buffers, so adding or removing one never moves another's
+// state.  The single-system entry points act on replica 0.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -20,6 +28,7 @@ using mdk::WMap;
 namespace {
 
 constexpr double kPi = 3.141592653589793;
+constexpr int kMaps = 7;  // part, chg, fx, fy, fz, cell1, cell2
 
 bool is_pow2(int n) { return n >= 4 && (n & (n - 1)) == 0; }
 int ilog2(int n) {
@@ -54,47 +63,32 @@ std::vector<double> bmod2(int K, int p) {
   return out;
 }
 
-struct HostTopo {
-  std::vector<int> charge;   // 0-based particle of charge m
-  std::vector<int> flags;    // bit0 anchor, bit1 charged
-  std::vector<int> bstart, bpart;
-};
+std::string hip_msg(hipError_t e, const char *what) {
+  return std::string(what) + ": " + hipGetErrorString(e);
+}
 
-}  // namespace
-
-struct md_engine {
+// One replica: parameters, descriptor, owned device memory.
+struct Sys {
   md_params p{};
-  bool fp32 = false;
-  bool coulomb = false;
-  hipStream_t st = nullptr;
-  Plan plan;
-  Rep h{};                      // host copy of the single replica's descriptor
+  bool fp32 = false, coulomb = false, uploaded = false;
+  Rep h{};
+  std::vector<int> charge, flags, bstart, bpart;  // host topology
   std::vector<void *> allocs;
-  std::vector<int> maps;
-  int *d_maps = nullptr;
-  double *d_eta = nullptr, *d_out1 = nullptr, *d_out2 = nullptr;
-  int *h_step = nullptr;        // pinned
-  HostTopo topo;
-  std::string err;
-  hipGraphExec_t gexec = nullptr;
-  hipGraph_t graph = nullptr;
+  size_t bytes = 0;
   int ncell1 = 0, ncell2 = 0;
-  bool uploaded = false;
+  int nb[kMaps] = {0};
+  size_t lds_x = 0, lds_y = 0, lds_z = 0, lds_sort = 0;
+  double *d_eta = nullptr, *d_out1 = nullptr, *d_out2 = nullptr;
+  int *d_hist = nullptr;     // observables: z histogram of the chains
+  double *d_hsum = nullptr;  // observables: per-block sums of z
+  std::string err;
 
-  ~md_engine() {
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    if (graph) (void)hipGraphDestroy(graph);
+  ~Sys() {
     for (void *a : allocs) (void)hipFree(a);
-    if (h_step) (void)hipHostFree(h_step);
-    if (st) (void)hipStreamDestroy(st);
   }
-
-  int fail(int code, const std::string &msg) {
-    err = msg;
+  int fail(int code, const std::string &m) {
+    err = m;
     return code;
-  }
-  int hip_fail(hipError_t e, const char *what) {
-    return fail(MD_EHIP, std::string(what) + ": " + hipGetErrorString(e));
   }
   template <typename T> T *dalloc(size_t n) {
     if (n == 0) n = 1;
@@ -102,64 +96,24 @@ struct md_engine {
     if (hipMalloc(&ptr, n * sizeof(T)) != hipSuccess) return nullptr;
     (void)hipMemset(ptr, 0, n * sizeof(T));
     allocs.push_back(ptr);
+    bytes += n * sizeof(T);
     return (T *)ptr;
   }
-  void dfree(void *ptr) {
+  void dfree(void *ptr, size_t nbytes) {
     auto it = std::find(allocs.begin(), allocs.end(), ptr);
     if (it != allocs.end()) {
       (void)hipFree(ptr);
       allocs.erase(it);
+      bytes -= nbytes;
     }
   }
-  int sync_check();
-  int setup(const md_params *pin);
-  int upload_desc() {
-    hipError_t e = hipMemcpy(plan.reps, &h, sizeof(Rep), hipMemcpyHostToDevice);
-    return e == hipSuccess ? MD_OK : hip_fail(e, "descriptor upload");
-  }
+  int setup(const md_params *pin, bool want_fp32);
   int size_lists(const double *pos);
-  int launch_compute_force(int step, int noise, const double *eta, bool in_step);
-  int build_graph();
 };
 
-// Reads the control block; reports the first device-side error.
-int md_engine::sync_check() {
-  hipError_t e = hipStreamSynchronize(st);
-  if (e != hipSuccess) return hip_fail(e, "stream");
-  e = hipGetLastError();
-  if (e != hipSuccess) return hip_fail(e, "launch");
-  Ctrl c;
-  e = hipMemcpy(&c, h.ctrl, sizeof c, hipMemcpyDeviceToHost);
-  if (e != hipSuccess) return hip_fail(e, "control block");
-  char buf[256];
-  if (c.err == MD_EBOND) {
-    std::snprintf(buf, sizeof buf, "The bond is break off! particle %d step %d", c.err_info + 1,
-                  c.err_step);
-    return fail(MD_EBOND, buf);
-  }
-  if (c.err == MD_ECELL) {
-    std::snprintf(buf, sizeof buf, "particle %d outside the cell grid (x = Lx/2, z < 0 or z >= Lz)",
-                  c.err_info + 1);
-    return fail(MD_ECELL, buf);
-  }
-  if (c.err == MD_ENONFINITE) {
-    std::snprintf(buf, sizeof buf, "non-finite force on particle %d at step %d", c.err_info + 1,
-                  c.err_step);
-    return fail(MD_ENONFINITE, buf);
-  }
-  if (c.ovf) {
-    std::snprintf(buf, sizeof buf,
-                  "capacity exceeded (mask %d): LJ cell %d/%d list %d/%d, real cell %d/%d list %d/%d",
-                  c.ovf, c.max_cell1, h.ccap1, c.max_nb1, h.nbcap1, c.max_cell2, h.ccap2,
-                  c.max_nb2, h.nbcap2);
-    return fail(MD_EOVERFLOW, buf);
-  }
-  return MD_OK;
-}
-
-int md_engine::setup(const md_params *pin) {
+int Sys::setup(const md_params *pin, bool want_fp32) {
   p = *pin;
-  fp32 = p.precision == MD_FP32;
+  fp32 = want_fp32;
   coulomb = p.qq != 0 && p.Nq > 0;
   const int NN = p.NN, Nq = coulomb ? p.Nq : 0;
   if (NN <= 0 || p.Lx <= 0 || p.dt <= 0 || p.multistep <= 0) return fail(MD_EINVAL, "bad sizes");
@@ -171,41 +125,38 @@ int md_engine::setup(const md_params *pin) {
         return fail(MD_EINVAL, "SPME order must be equal on all axes and in 3..8");
     }
   }
-  hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
-  if (e != hipSuccess) return hip_fail(e, "stream");
-  if (hipHostMalloc((void **)&h_step, sizeof(int)) != hipSuccess) return fail(MD_ENOMEM, "pinned");
-
   // topology (compt_acc.f90:317-372, 433-488)
-  std::vector<int32_t> charge(std::max(Nq, 1)), fene(2 * (size_t)std::max(p.N_bond, 1)),
+  std::vector<int32_t> chg(std::max(Nq, 1)), fene(2 * (size_t)std::max(p.N_bond, 1)),
       anchor(std::max(p.N_anchor, 1));
-  int rc = md_build_topology(&p, coulomb ? charge.data() : nullptr, fene.data(), anchor.data());
+  int rc = md_build_topology(&p, coulomb ? chg.data() : nullptr, fene.data(), anchor.data());
   if (rc) return fail(rc, "topology");
-  topo.flags.assign(NN, 0);
-  for (int a = 0; a < p.N_anchor; a++) topo.flags[anchor[a] - 1] |= 1;
-  topo.charge.resize(Nq);
+  flags.assign(NN, 0);
+  for (int a = 0; a < p.N_anchor; a++) flags[anchor[a] - 1] |= 1;
+  charge.resize(Nq);
   for (int m = 0; m < Nq; m++) {
-    topo.charge[m] = charge[m] - 1;
-    topo.flags[charge[m] - 1] |= 2;
+    charge[m] = chg[m] - 1;
+    flags[chg[m] - 1] |= 2;
   }
   std::vector<int> deg(NN + 1, 0);
   for (int b = 0; b < p.N_bond; b++) {
     deg[fene[b] - 1]++;
     deg[fene[p.N_bond + b] - 1]++;
   }
-  topo.bstart.assign(NN + 1, 0);
-  for (int i = 0; i < NN; i++) topo.bstart[i + 1] = topo.bstart[i] + deg[i];
-  topo.bpart.assign(std::max(topo.bstart[NN], 1), 0);
-  std::vector<int> fill(topo.bstart.begin(), topo.bstart.end() - 1);
+  bstart.assign(NN + 1, 0);
+  for (int i = 0; i < NN; i++) bstart[i + 1] = bstart[i] + deg[i];
+  bpart.assign(std::max(bstart[NN], 1), 0);
+  std::vector<int> fill(bstart.begin(), bstart.end() - 1);
   for (int b = 0; b < p.N_bond; b++) {
     const int j = fene[b] - 1, k = fene[p.N_bond + b] - 1;
-    topo.bpart[fill[j]++] = k;
-    topo.bpart[fill[k]++] = j;
+    bpart[fill[j]++] = k;
+    bpart[fill[k]++] = j;
   }
 
-  // descriptor: scalars, computed as the reference computes them
+  // descriptor scalars, computed as the reference computes them
   Rep &S = h;
   S.NN = NN;
   S.Nq = Nq;
+  S.Npe = p.Npe;
   S.N_anchor = p.N_anchor;
   S.real_verlet = coulomb ? p.real_verlet : 0;
   S.Lx = p.Lx;
@@ -253,39 +204,42 @@ int md_engine::setup(const md_params *pin) {
   ncell1 = S.nc1[0] * S.nc1[1] * S.nc1[2];
   ncell2 = S.real_verlet ? S.nc2[0] * S.nc2[1] * S.nc2[2] : 0;
 
-  // device arrays
+  // state and topology on the device
   const size_t nn = NN;
   S.pos = dalloc<double4>(nn);
   S.pos32 = fp32 ? dalloc<float4>(nn) : nullptr;
   double *v = dalloc<double>(3 * nn), *a = dalloc<double>(3 * nn);
-  if (!S.pos || !v || !a) return fail(MD_ENOMEM, "state");
+  if (!S.pos || !v || !a || (fp32 && !S.pos32)) return fail(MD_ENOMEM, "state");
   S.vx = v; S.vy = v + nn; S.vz = v + 2 * nn;
   S.ax = a; S.ay = a + nn; S.az = a + 2 * nn;
   const size_t rs = fp32 ? 4 : 8;
-  char *cc = (char *)dalloc<char>(3 * nn * rs);
+  char *cc = dalloc<char>(3 * nn * rs);
   S.ccx = cc; S.ccy = cc + nn * rs; S.ccz = cc + 2 * nn * rs;
   S.flags = dalloc<int>(nn);
   S.charge = dalloc<int>(std::max(Nq, 1));
   S.bstart = dalloc<int>(nn + 1);
-  S.bpart = dalloc<int>(topo.bpart.size());
+  S.bpart = dalloc<int>(bpart.size());
   S.ctrl = dalloc<Ctrl>(1);
   d_eta = dalloc<double>(3 * nn);
   d_out1 = dalloc<double>(3 * nn);
   d_out2 = dalloc<double>(3 * nn);
-  if (!S.flags || !S.charge || !S.bstart || !S.bpart || !S.ctrl || !d_eta)
+  d_hist = dalloc<int>(mdk::kHistBins);
+  d_hsum = dalloc<double>(cdiv(NN, mdk::kBlock));  // one per particle block of k_observe
+  if (!cc || !S.flags || !S.charge || !S.bstart || !S.bpart || !S.ctrl || !d_eta || !d_out1 ||
+      !d_out2 || !d_hist || !d_hsum)
     return fail(MD_ENOMEM, "topology arrays");
-  (void)hipMemcpy(S.flags, topo.flags.data(), nn * sizeof(int), hipMemcpyHostToDevice);
-  if (Nq) (void)hipMemcpy(S.charge, topo.charge.data(), Nq * sizeof(int), hipMemcpyHostToDevice);
-  (void)hipMemcpy(S.bstart, topo.bstart.data(), (nn + 1) * sizeof(int), hipMemcpyHostToDevice);
-  (void)hipMemcpy(S.bpart, topo.bpart.data(), topo.bpart.size() * sizeof(int),
-                  hipMemcpyHostToDevice);
-  // cell counts (double-buffered); members and lists are sized at upload
+  (void)hipMemcpy(S.flags, flags.data(), nn * sizeof(int), hipMemcpyHostToDevice);
+  if (Nq) (void)hipMemcpy(S.charge, charge.data(), Nq * sizeof(int), hipMemcpyHostToDevice);
+  (void)hipMemcpy(S.bstart, bstart.data(), (nn + 1) * sizeof(int), hipMemcpyHostToDevice);
+  (void)hipMemcpy(S.bpart, bpart.data(), bpart.size() * sizeof(int), hipMemcpyHostToDevice);
   S.cnt1[0] = dalloc<int>(ncell1);
   S.cnt1[1] = dalloc<int>(ncell1);
   S.cnt2[0] = dalloc<int>(std::max(ncell2, 1));
   S.cnt2[1] = dalloc<int>(std::max(ncell2, 1));
   S.nbn1 = dalloc<int>(nn);
   S.nbn2 = dalloc<int>(std::max(Nq, 1));
+  if (!S.cnt1[0] || !S.cnt1[1] || !S.cnt2[0] || !S.cnt2[1] || !S.nbn1 || !S.nbn2)
+    return fail(MD_ENOMEM, "cells");
 
   // SPME: mesh, spectrum, influence function, twiddles
   const int csz = fp32 ? 8 : 16;
@@ -344,6 +298,7 @@ int md_engine::setup(const md_params *pin) {
         tw[2 * k + 1] = -std::sin(2 * kPi * k / K);
       }
       void *dtw = dalloc<char>((size_t)K / 2 * csz);
+      if (!dtw) return fail(MD_ENOMEM, "twiddles");
       if (fp32) {
         std::vector<float> f(tw.begin(), tw.end());
         (void)hipMemcpy(dtw, f.data(), (size_t)K / 2 * 8, hipMemcpyHostToDevice);
@@ -360,43 +315,27 @@ int md_engine::setup(const md_params *pin) {
     S.nqblk = cdiv(Nq, mdk::kBlock);
     S.fsp = dalloc<double>(3 * (size_t)Nq);
     S.part = dalloc<double>(4 * (size_t)S.nqblk);
-    plan.lds_x = (size_t)S.px_tile * K1 * csz;
-    plan.lds_y = (size_t)S.py_tile * K2 * csz;
-    plan.lds_z = (size_t)S.pz_tile * K3 * csz;
+    if (!S.fsp || !S.part) return fail(MD_ENOMEM, "spme scratch");
+    lds_x = (size_t)S.px_tile * K1 * csz;
+    lds_y = (size_t)S.py_tile * K2 * csz;
+    lds_z = (size_t)S.pz_tile * K3 * csz;
   }
-
-  // block maps (one replica)
-  const int nb_part = cdiv(NN, mdk::kBlock);
-  const int nb_chg = coulomb ? cdiv(Nq, mdk::kBlock) : 0;
-  const int nb_fx = coulomb ? cdiv((long)S.K[1] * S.K[2] / 2, S.px_tile) : 0;
-  const int nb_fy = coulomb ? S.K[2] * cdiv(S.Kh, S.py_tile) : 0;
-  const int nb_fz = coulomb ? S.K[1] * cdiv(S.Kh, S.pz_tile) : 0;
-  const int nb_c1 = cdiv(ncell1, mdk::kBlock);
-  const int nb_c2 = ncell2;
-  const int counts[7] = {nb_part, nb_chg, nb_fx, nb_fy, nb_fz, nb_c1, nb_c2};
-  maps.clear();
-  for (int k = 0; k < 7; k++) {
-    maps.push_back(0);
-    maps.push_back(counts[k]);
-  }
-  d_maps = dalloc<int>(maps.size());
-  (void)hipMemcpy(d_maps, maps.data(), maps.size() * sizeof(int), hipMemcpyHostToDevice);
-  WMap *w[7] = {&plan.part, &plan.chg, &plan.fx, &plan.fy, &plan.fz, &plan.cell1, &plan.cell2};
-  for (int k = 0; k < 7; k++) *w[k] = WMap{d_maps + 2 * k, 1, counts[k]};
-  plan.nrep = 1;
-  plan.fp32 = fp32;
-  plan.reps = dalloc<Rep>(1);
-  plan.vpart = dalloc<double>(4 * (size_t)nb_part);
-  if (!plan.reps || !plan.vpart) return fail(MD_ENOMEM, "plan");
+  nb[0] = cdiv(NN, mdk::kBlock);
+  nb[1] = coulomb ? cdiv(Nq, mdk::kBlock) : 0;
+  nb[2] = coulomb ? cdiv((long)S.K[1] * S.K[2] / 2, S.px_tile) : 0;
+  nb[3] = coulomb ? S.K[2] * cdiv(S.Kh, S.py_tile) : 0;
+  nb[4] = coulomb ? S.K[1] * cdiv(S.Kh, S.pz_tile) : 0;
+  nb[5] = cdiv(ncell1, mdk::kBlock);
+  nb[6] = ncell2;
   S.mem1 = S.mem2 = nullptr;
   S.nb1 = S.nb2 = nullptr;
   S.ccap1 = S.ccap2 = S.nbcap1 = S.nbcap2 = 0;
-  return upload_desc();
+  return MD_OK;
 }
 
 // Sizes cell and list capacities from a configuration: twice the largest
 // occupancy and list length found on the host, plus headroom.
-int md_engine::size_lists(const double *pos) {
+int Sys::size_lists(const double *pos) {
   const int NN = p.NN;
   const double *X = pos, *Y = pos + NN, *Z = pos + 2 * (size_t)NN;
   auto scan = [&](const std::vector<int> &items, const int nc[3], double rv, int &maxcell,
@@ -443,32 +382,33 @@ int md_engine::size_lists(const double *pos) {
   for (int i = 0; i < NN; i++) all[i] = i;
   int mc1, mn1, mc2 = 0, mn2 = 0;
   scan(all, h.nc1, p.rvl_eff, mc1, mn1);
-  if (h.real_verlet) scan(topo.charge, h.nc2, p.rvc, mc2, mn2);
+  if (h.real_verlet) scan(charge, h.nc2, p.rvc, mc2, mn2);
   auto round8 = [](int v) { return (v + 7) & ~7; };
   // LJ: dense packing bounds occupancy near 1.4 per unit volume
   const double v1 = h.rcel1[0] * h.rcel1[1] * h.rcel1[2];
   const int ccap1 = round8(std::max(2 * mc1 + 16, (int)(1.5 * v1) + 16));
-  const int nbcap1 = round8(std::min(NN, std::max(2 * mn1 + 32, (int)(4.19 * std::pow(p.rvl_eff, 3) * 1.2) + 32)));
+  const int nbcap1 =
+      round8(std::min(NN, std::max(2 * mn1 + 32, (int)(4.19 * std::pow(p.rvl_eff, 3) * 1.2) + 32)));
   const int ccap2 = h.real_verlet ? round8(std::min(h.Nq, 2 * mc2 + 64)) : 0;
   const int nbcap2 = h.real_verlet ? round8(std::min(h.Nq, 2 * mn2 + 64)) : 0;
   if (ccap2 > 16384) return fail(MD_EINVAL, "real-space cells too full for the LDS sort");
   if (ccap1 != h.ccap1) {
-    if (h.mem1) dfree(h.mem1);
+    if (h.mem1) dfree(h.mem1, (size_t)ncell1 * h.ccap1 * 4);
     h.mem1 = dalloc<int>((size_t)ncell1 * ccap1);
     h.ccap1 = ccap1;
   }
   if (nbcap1 != h.nbcap1) {
-    if (h.nb1) dfree(h.nb1);
+    if (h.nb1) dfree(h.nb1, (size_t)NN * h.nbcap1 * 4);
     h.nb1 = dalloc<int>((size_t)NN * nbcap1);
     h.nbcap1 = nbcap1;
   }
   if (h.real_verlet && ccap2 != h.ccap2) {
-    if (h.mem2) dfree(h.mem2);
+    if (h.mem2) dfree(h.mem2, (size_t)ncell2 * h.ccap2 * 4);
     h.mem2 = dalloc<int>((size_t)ncell2 * ccap2);
     h.ccap2 = ccap2;
   }
   if (h.real_verlet && nbcap2 != h.nbcap2) {
-    if (h.nb2) dfree(h.nb2);
+    if (h.nb2) dfree(h.nb2, (size_t)h.Nq * h.nbcap2 * 4);
     h.nb2 = dalloc<int>((size_t)h.Nq * nbcap2);
     h.nbcap2 = nbcap2;
   }
@@ -476,44 +416,279 @@ int md_engine::size_lists(const double *pos) {
     return fail(MD_ENOMEM, "neighbour lists");
   int np = 1;
   while (np < h.ccap2) np <<= 1;
-  plan.lds_sort = h.real_verlet ? (size_t)np * sizeof(int) : 0;
-  return upload_desc();
+  lds_sort = h.real_verlet ? (size_t)np * sizeof(int) : 0;
+  return MD_OK;
 }
 
-int md_engine::launch_compute_force(int step, int noise, const double *eta, bool in_step) {
+}  // namespace
+
+struct md_engine {
+  bool fp32 = false;
+  int multistep = 0;
+  hipStream_t st = nullptr;
+  std::vector<Sys *> sys;          // launch order
+  std::map<int, Sys *> slots;      // stable handles
+  int next_slot = 0;
+  Plan plan;                       // all replicas
+  std::vector<Plan> single;        // one replica each (same order as sys)
+  Rep *d_reps = nullptr;
+  int reps_cap = 0;
+  int *d_maps = nullptr;
+  int maps_cap = 0;
+  double *vpart = nullptr;
+  size_t vpart_cap = 0;
+  int *h_steps = nullptr;          // pinned, one per replica
+  int steps_cap = 0;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  bool dirty = true;
+  std::string err;
+
+  ~md_engine() {
+    if (st) (void)hipStreamSynchronize(st);
+    drop_graph();
+    for (Sys *s : sys) delete s;
+    if (d_reps) (void)hipFree(d_reps);
+    if (d_maps) (void)hipFree(d_maps);
+    if (vpart) (void)hipFree(vpart);
+    if (h_steps) (void)hipHostFree(h_steps);
+    if (st) (void)hipStreamDestroy(st);
+  }
+  int fail(int code, const std::string &msg) {
+    err = msg;
+    return code;
+  }
+  int hip_fail(hipError_t e, const char *what) { return fail(MD_EHIP, hip_msg(e, what)); }
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+  }
+  int init(int precision, int ms) {
+    fp32 = precision == MD_FP32;
+    multistep = ms;
+    if (ms <= 0) return fail(MD_EINVAL, "multistep must be positive");
+    hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    return e == hipSuccess ? MD_OK : hip_fail(e, "stream");
+  }
+  int rebuild();
+  int sync();
+  int check_one(const Sys *s, const Ctrl &c, std::string &msg) const;
+  int check_all();
+  int add(const md_params *p, int *slot);
+  int launch_compute_force(const Plan &P, Sys *s, int step, int noise, const double *eta, bool in_step);
+  int build_graph();
+  int run_cycles(int ncycles);
+  int index_of(const Sys *s) const {
+    for (size_t k = 0; k < sys.size(); k++)
+      if (sys[k] == s) return (int)k;
+    return -1;
+  }
+};
+
+// Uploads every descriptor and the block maps; per-replica plans point into
+// the same arrays.  Invalidates the captured graph.
+int md_engine::rebuild() {
+  const int R = (int)sys.size();
+  hipError_t e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(e, "sync");
+  drop_graph();
+  dirty = false;
+  if (R == 0) {
+    plan = Plan{};
+    single.clear();
+    return MD_OK;
+  }
+  if (R > reps_cap) {
+    if (d_reps) (void)hipFree(d_reps);
+    reps_cap = std::max(2 * R, 8);
+    if (hipMalloc(&d_reps, reps_cap * sizeof(Rep)) != hipSuccess) return fail(MD_ENOMEM, "descriptors");
+  }
+  std::vector<Rep> hr(R);
+  for (int r = 0; r < R; r++) hr[r] = sys[r]->h;
+  e = hipMemcpy(d_reps, hr.data(), R * sizeof(Rep), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, "descriptors");
+  // maps: batch maps kMaps*(R+1), then per replica kMaps*2
+  std::vector<int> m;
+  int tot[kMaps];
+  for (int k = 0; k < kMaps; k++) {
+    int acc = 0;
+    m.push_back(0);
+    for (int r = 0; r < R; r++) {
+      acc += sys[r]->nb[k];
+      m.push_back(acc);
+    }
+    tot[k] = acc;
+  }
+  const int single_off = (int)m.size();
+  for (int r = 0; r < R; r++)
+    for (int k = 0; k < kMaps; k++) {
+      m.push_back(0);
+      m.push_back(sys[r]->nb[k]);
+    }
+  if ((int)m.size() > maps_cap) {
+    if (d_maps) (void)hipFree(d_maps);
+    maps_cap = (int)m.size() * 2;
+    if (hipMalloc(&d_maps, maps_cap * sizeof(int)) != hipSuccess) return fail(MD_ENOMEM, "maps");
+  }
+  e = hipMemcpy(d_maps, m.data(), m.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, "maps");
+  const size_t vneed = 4 * (size_t)std::max(tot[0], 1);
+  if (vneed > vpart_cap) {
+    if (vpart) (void)hipFree(vpart);
+    vpart_cap = 2 * vneed;
+    if (hipMalloc(&vpart, vpart_cap * sizeof(double)) != hipSuccess) return fail(MD_ENOMEM, "vpart");
+  }
+  if (R > steps_cap) {
+    if (h_steps) (void)hipHostFree(h_steps);
+    steps_cap = std::max(2 * R, 8);
+    if (hipHostMalloc((void **)&h_steps, steps_cap * sizeof(int)) != hipSuccess)
+      return fail(MD_ENOMEM, "pinned");
+  }
+  auto fill = [&](Plan &P, Rep *reps, int nrep, const int *maps, const int *totals) {
+    P.reps = reps;
+    P.nrep = nrep;
+    WMap *w[kMaps] = {&P.part, &P.chg, &P.fx, &P.fy, &P.fz, &P.cell1, &P.cell2};
+    for (int k = 0; k < kMaps; k++) *w[k] = WMap{maps + k * (nrep + 1), nrep, totals[k]};
+    P.vpart = vpart;
+    P.fp32 = fp32;
+  };
+  fill(plan, d_reps, R, d_maps, tot);
+  plan.lds_x = plan.lds_y = plan.lds_z = plan.lds_sort = 0;
+  for (Sys *s : sys) {
+    plan.lds_x = std::max(plan.lds_x, s->lds_x);
+    plan.lds_y = std::max(plan.lds_y, s->lds_y);
+    plan.lds_z = std::max(plan.lds_z, s->lds_z);
+    plan.lds_sort = std::max(plan.lds_sort, s->lds_sort);
+  }
+  single.assign(R, Plan{});
+  for (int r = 0; r < R; r++) {
+    fill(single[r], d_reps + r, 1, d_maps + single_off + r * kMaps * 2, sys[r]->nb);
+    single[r].lds_x = sys[r]->lds_x;
+    single[r].lds_y = sys[r]->lds_y;
+    single[r].lds_z = sys[r]->lds_z;
+    single[r].lds_sort = sys[r]->lds_sort;
+  }
+  return MD_OK;
+}
+
+int md_engine::sync() {
+  hipError_t e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(e, "stream");
+  e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "launch");
+  return MD_OK;
+}
+
+// Device-side status of one replica from its control block.
+int md_engine::check_one(const Sys *s, const Ctrl &c, std::string &msg) const {
+  char buf[256];
+  if (c.err == MD_EBOND) {
+    std::snprintf(buf, sizeof buf, "The bond is break off! particle %d step %d", c.err_info + 1,
+                  c.err_step);
+    msg = buf;
+    return MD_EBOND;
+  }
+  if (c.err == MD_ECELL) {
+    std::snprintf(buf, sizeof buf, "particle %d outside the cell grid (x = Lx/2, z < 0 or z >= Lz)",
+                  c.err_info + 1);
+    msg = buf;
+    return MD_ECELL;
+  }
+  if (c.err == MD_ENONFINITE) {
+    std::snprintf(buf, sizeof buf, "non-finite force on particle %d at step %d", c.err_info + 1,
+                  c.err_step);
+    msg = buf;
+    return MD_ENONFINITE;
+  }
+  if (c.ovf) {
+    std::snprintf(buf, sizeof buf,
+                  "capacity exceeded (mask %d): LJ cell %d/%d list %d/%d, real cell %d/%d list %d/%d",
+                  c.ovf, c.max_cell1, s->h.ccap1, c.max_nb1, s->h.nbcap1, c.max_cell2, s->h.ccap2,
+                  c.max_nb2, s->h.nbcap2);
+    msg = buf;
+    return MD_EOVERFLOW;
+  }
+  return MD_OK;
+}
+
+int md_engine::check_all() {
+  int rc = sync();
+  if (rc) return rc;
+  for (Sys *s : sys) {
+    Ctrl c;
+    hipError_t e = hipMemcpy(&c, s->h.ctrl, sizeof c, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "control block");
+    std::string msg;
+    rc = check_one(s, c, msg);
+    if (rc) return fail(rc, msg);
+  }
+  return MD_OK;
+}
+
+int md_engine::add(const md_params *p, int *slot) {
+  if (p->multistep != multistep) return fail(MD_EINVAL, "replicas must share multistep");
+  if ((p->precision == MD_FP32) != fp32) return fail(MD_EINVAL, "replicas must share precision");
+  Sys *s = new Sys();
+  int rc = s->setup(p, fp32);
+  if (rc) {
+    err = s->err;
+    delete s;
+    return rc;
+  }
+  sys.push_back(s);
+  *slot = next_slot++;
+  slots[*slot] = s;
+  dirty = true;
+  return MD_OK;
+}
+
+int md_engine::launch_compute_force(const Plan &P, Sys *s, int step, int noise, const double *eta,
+                                    bool in_step) {
   int mode = mdk::SR_HELD | mdk::SR_FIELD | mdk::SR_ANCHOR;
   if (noise == MD_NOISE_ETA) {
     if (!eta) return fail(MD_EINVAL, "eta is null");
-    hipError_t e = hipMemcpyAsync(d_eta, eta, 3 * sizeof(double) * p.NN, hipMemcpyHostToDevice, st);
+    hipError_t e =
+        hipMemcpyAsync(s->d_eta, eta, 3 * sizeof(double) * s->p.NN, hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return hip_fail(e, "eta upload");
     mode |= mdk::SR_NOISE_ETA;
   } else if (noise == MD_NOISE_PHILOX) {
     mode |= mdk::SR_NOISE_PHILOX;
   }
   if (in_step) mode |= mdk::SR_KICK | mdk::SR_DRMAX;
-  if (coulomb && step % p.multistep == 0) {
-    if (fp32) mdk::launch_coulomb<float>(plan, st, nullptr);
-    else mdk::launch_coulomb<double>(plan, st, nullptr);
+  if (s->coulomb && step % multistep == 0) {
+    if (fp32) mdk::launch_coulomb<float>(P, st, nullptr);
+    else mdk::launch_coulomb<double>(P, st, nullptr);
   }
-  if (fp32) mdk::launch_short<float>(plan, st, mode, step, d_eta, nullptr);
-  else mdk::launch_short<double>(plan, st, mode, step, d_eta, nullptr);
+  if (fp32) mdk::launch_short<float>(P, st, mode, step, s->d_eta, nullptr);
+  else mdk::launch_short<double>(P, st, mode, step, s->d_eta, nullptr);
   hipError_t e = hipGetLastError();
+  if (noise == MD_NOISE_ETA) {  // eta is host memory of the caller
+    hipError_t e2 = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = e2;
+  }
   return e == hipSuccess ? MD_OK : hip_fail(e, "compute_force launch");
 }
 
-// One multistep cycle of production steps, captured once and replayed:
-// kick-drift, Coulomb at the first step of the cycle, short range with the
-// second kick, list maintenance.  The step number comes from the control
-// block (advanced by the list kernel).
+// One multistep cycle of steps, captured once and replayed for every
+// replica: kick-drift, short range with the second kick, list maintenance,
+// and the Coulomb evaluation at the cycle's last step.  A replica enters the
+// cycle at a step n with n % multistep == 1, so its Coulomb steps are the
+// multiples of multistep, as in Compute_Force (compt_acc.f90:841).  Step
+// numbers come from each replica's control block (advanced by the list
+// kernel).
 int md_engine::build_graph() {
   if (gexec) return MD_OK;
   hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return hip_fail(e, "begin capture");
   const int mode = mdk::SR_HELD | mdk::SR_FIELD | mdk::SR_ANCHOR | mdk::SR_NOISE_PHILOX |
                    mdk::SR_KICK | mdk::SR_DRMAX;
-  for (int s = 0; s < p.multistep; s++) {
+  bool any_coulomb = false;
+  for (Sys *s : sys) any_coulomb |= s->coulomb;
+  for (int s = 1; s <= multistep; s++) {
     mdk::launch_kick_drift(plan, st);
-    if (coulomb && s == 0) {
+    if (any_coulomb && s == multistep) {
       if (fp32) mdk::launch_coulomb<float>(plan, st, nullptr);
       else mdk::launch_coulomb<double>(plan, st, nullptr);
     }
@@ -528,20 +703,125 @@ int md_engine::build_graph() {
   return MD_OK;
 }
 
+// Same launch sequence as the captured cycle, issued on the stream
+// (MD_NO_GRAPH=1: for tracing tools that do not follow graph launches).
+static void launch_cycle(md_engine *e) {
+  const int mode = mdk::SR_HELD | mdk::SR_FIELD | mdk::SR_ANCHOR | mdk::SR_NOISE_PHILOX |
+                   mdk::SR_KICK | mdk::SR_DRMAX;
+  bool any_coulomb = false;
+  for (Sys *s : e->sys) any_coulomb |= s->coulomb;
+  for (int s = 1; s <= e->multistep; s++) {
+    mdk::launch_kick_drift(e->plan, e->st);
+    if (any_coulomb && s == e->multistep) {
+      if (e->fp32) mdk::launch_coulomb<float>(e->plan, e->st, nullptr);
+      else mdk::launch_coulomb<double>(e->plan, e->st, nullptr);
+    }
+    if (e->fp32) mdk::launch_short<float>(e->plan, e->st, mode, -1, nullptr, nullptr);
+    else mdk::launch_short<double>(e->plan, e->st, mode, -1, nullptr, nullptr);
+    mdk::launch_lists(e->plan, e->st, 1);
+  }
+}
+
+int md_engine::run_cycles(int ncycles) {
+  if (ncycles <= 0 || sys.empty()) return MD_OK;
+  if (dirty) {
+    int rc = rebuild();
+    if (rc) return rc;
+  }
+  const char *ng = std::getenv("MD_NO_GRAPH");
+  if (ng && ng[0] == '1') {
+    for (int c = 0; c < ncycles; c++) launch_cycle(this);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? MD_OK : hip_fail(e, "cycle launch");
+  }
+  int rc = build_graph();
+  if (rc) return rc;
+  for (int c = 0; c < ncycles; c++) {
+    hipError_t e = hipGraphLaunch(gexec, st);
+    if (e != hipSuccess) return hip_fail(e, "graph launch");
+  }
+  return MD_OK;
+}
+
 // ------------------------------------------------------------------ C-ABI
 
 #define ENGINE_CHECK(e) \
   if (!(e)) return MD_EINVAL
-#define STATE_CHECK(e)                                              \
-  if (!(e)->uploaded) return (e)->fail(MD_EINVAL, "no state uploaded")
+#define FIRST(e)                                                       \
+  if ((e)->sys.empty()) return (e)->fail(MD_EINVAL, "engine is empty"); \
+  if ((e)->dirty) {                                                    \
+    int rc_ = (e)->rebuild();                                          \
+    if (rc_) return rc_;                                               \
+  }                                                                    \
+  Sys *s = (e)->sys[0];                                                \
+  const Plan &P = (e)->single[0];                                      \
+  (void)P
+#define STATE_CHECK(e) \
+  if (!s->uploaded) return (e)->fail(MD_EINVAL, "no state uploaded")
+
+static int upload_sys(md_engine *e, Sys *s, const double *pos, const double *vel,
+                      const double *acc) {
+  const int NN = s->p.NN;
+  std::vector<double4> p4(NN);
+  std::vector<float4> p32(NN);
+  for (int i = 0; i < NN; i++) {
+    p4[i] = make_double4(pos[i], pos[NN + i], pos[2 * (size_t)NN + i], pos[3 * (size_t)NN + i]);
+    p32[i] = make_float4((float)p4[i].x, (float)p4[i].y, (float)p4[i].z, (float)p4[i].w);
+  }
+  // no kernel may still use the list buffers that size_lists may replace
+  int rc = e->sync();
+  if (rc) return rc;
+  rc = s->size_lists(pos);
+  if (rc) return e->fail(rc, s->err);
+  e->dirty = true;
+  hipError_t err = hipMemcpy(s->h.pos, p4.data(), NN * sizeof(double4), hipMemcpyHostToDevice);
+  if (err == hipSuccess && s->fp32)
+    err = hipMemcpy(s->h.pos32, p32.data(), NN * sizeof(float4), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(s->h.vx, vel, 3 * sizeof(double) * NN, hipMemcpyHostToDevice);
+  if (err == hipSuccess) {
+    if (acc) err = hipMemcpy(s->h.ax, acc, 3 * sizeof(double) * NN, hipMemcpyHostToDevice);
+    else err = hipMemset(s->h.ax, 0, 3 * sizeof(double) * NN);
+  }
+  if (err != hipSuccess) return e->hip_fail(err, "upload");
+  s->uploaded = true;
+  return MD_OK;
+}
+
+static int download_sys(md_engine *e, Sys *s, double *pos, double *vel, double *acc, Ctrl *c) {
+  const int NN = s->p.NN;
+  int rc = e->sync();
+  if (rc) return rc;
+  if (pos) {
+    std::vector<double4> p4(NN);
+    (void)hipMemcpy(p4.data(), s->h.pos, NN * sizeof(double4), hipMemcpyDeviceToHost);
+    for (int i = 0; i < NN; i++) {
+      pos[i] = p4[i].x;
+      pos[NN + i] = p4[i].y;
+      pos[2 * (size_t)NN + i] = p4[i].z;
+      pos[3 * (size_t)NN + i] = p4[i].w;
+    }
+  }
+  if (vel) (void)hipMemcpy(vel, s->h.vx, 3 * sizeof(double) * NN, hipMemcpyDeviceToHost);
+  if (acc) (void)hipMemcpy(acc, s->h.ax, 3 * sizeof(double) * NN, hipMemcpyDeviceToHost);
+  Ctrl cc;
+  hipError_t err = hipMemcpy(&cc, s->h.ctrl, sizeof cc, hipMemcpyDeviceToHost);
+  if (err != hipSuccess) return e->hip_fail(err, "control block");
+  if (c) *c = cc;
+  std::string msg;
+  rc = e->check_one(s, cc, msg);
+  return rc ? e->fail(rc, msg) : MD_OK;
+}
 
 extern "C" int md_create(md_engine **out, const md_params *p) {
   if (!out || !p) return MD_EINVAL;
-  *out = nullptr;
   md_engine *e = new md_engine();
-  int rc = e->setup(p);
   *out = e;
-  return rc;
+  int rc = e->init(p->precision, p->multistep);
+  if (rc) return rc;
+  int slot;
+  rc = e->add(p, &slot);
+  if (rc) return rc;
+  return e->rebuild();
 }
 
 extern "C" void md_destroy(md_engine *e) { delete e; }
@@ -555,50 +835,19 @@ extern "C" int md_last_error(md_engine *e, char *buf, int32_t len) {
 extern "C" int md_upload_state(md_engine *e, const double *pos, const double *vel,
                                const double *acc) {
   ENGINE_CHECK(e && pos && vel);
-  const int NN = e->p.NN;
-  std::vector<double4> p4(NN);
-  std::vector<float4> p32(NN);
-  for (int i = 0; i < NN; i++) {
-    p4[i] = make_double4(pos[i], pos[NN + i], pos[2 * (size_t)NN + i], pos[3 * (size_t)NN + i]);
-    p32[i] = make_float4((float)p4[i].x, (float)p4[i].y, (float)p4[i].z, (float)p4[i].w);
-  }
-  // no kernel may still use the list buffers that size_lists may replace
-  hipError_t serr = hipStreamSynchronize(e->st);
-  if (serr != hipSuccess) return e->hip_fail(serr, "upload sync");
-  int rc = e->size_lists(pos);
+  FIRST(e);
+  int rc = upload_sys(e, s, pos, vel, acc);
   if (rc) return rc;
-  e->uploaded = true;
-  hipError_t err = hipMemcpy(e->h.pos, p4.data(), NN * sizeof(double4), hipMemcpyHostToDevice);
-  if (err == hipSuccess && e->fp32)
-    err = hipMemcpy(e->h.pos32, p32.data(), NN * sizeof(float4), hipMemcpyHostToDevice);
-  if (err == hipSuccess) err = hipMemcpy(e->h.vx, vel, 3 * sizeof(double) * NN, hipMemcpyHostToDevice);
-  if (err == hipSuccess) {
-    if (acc) err = hipMemcpy(e->h.ax, acc, 3 * sizeof(double) * NN, hipMemcpyHostToDevice);
-    else err = hipMemset(e->h.ax, 0, 3 * sizeof(double) * NN);
-  }
-  return err == hipSuccess ? MD_OK : e->hip_fail(err, "upload");
+  return e->rebuild();
 }
 
 extern "C" int md_download_state(md_engine *e, double *pos, double *vel, double *acc,
                                  double *dr_max) {
   ENGINE_CHECK(e);
-  int rc = e->sync_check();
-  const int NN = e->p.NN;
-  if (pos) {
-    std::vector<double4> p4(NN);
-    (void)hipMemcpy(p4.data(), e->h.pos, NN * sizeof(double4), hipMemcpyDeviceToHost);
-    for (int i = 0; i < NN; i++) {
-      pos[i] = p4[i].x;
-      pos[NN + i] = p4[i].y;
-      pos[2 * (size_t)NN + i] = p4[i].z;
-      pos[3 * (size_t)NN + i] = p4[i].w;
-    }
-  }
-  if (vel) (void)hipMemcpy(vel, e->h.vx, 3 * sizeof(double) * NN, hipMemcpyDeviceToHost);
-  if (acc) (void)hipMemcpy(acc, e->h.ax, 3 * sizeof(double) * NN, hipMemcpyDeviceToHost);
+  FIRST(e);
+  Ctrl c{};
+  int rc = download_sys(e, s, pos, vel, acc, &c);
   if (dr_max) {
-    Ctrl c;
-    (void)hipMemcpy(&c, e->h.ctrl, sizeof c, hipMemcpyDeviceToHost);
     dr_max[0] = c.dr_max1;
     dr_max[1] = c.dr_max2;
   }
@@ -607,46 +856,49 @@ extern "C" int md_download_state(md_engine *e, double *pos, double *vel, double 
 
 extern "C" int md_download_coulomb(md_engine *e, double *acc_c) {
   ENGINE_CHECK(e && acc_c);
-  int rc = e->sync_check();
-  const size_t n = 3 * (size_t)e->p.NN;
-  if (e->fp32) {
+  FIRST(e);
+  int rc = e->sync();
+  if (rc) return rc;
+  const size_t n = 3 * (size_t)s->p.NN;
+  if (s->fp32) {
     std::vector<float> f(n);
-    (void)hipMemcpy(f.data(), e->h.ccx, n * 4, hipMemcpyDeviceToHost);
+    (void)hipMemcpy(f.data(), s->h.ccx, n * 4, hipMemcpyDeviceToHost);
     for (size_t k = 0; k < n; k++) acc_c[k] = f[k];
   } else {
-    (void)hipMemcpy(acc_c, e->h.ccx, n * 8, hipMemcpyDeviceToHost);
+    (void)hipMemcpy(acc_c, s->h.ccx, n * 8, hipMemcpyDeviceToHost);
   }
-  return rc;
+  return e->check_all();
 }
 
 extern "C" int md_build_lists(md_engine *e) {
   ENGINE_CHECK(e);
+  FIRST(e);
   STATE_CHECK(e);
-  mdk::launch_set_flags(e->plan, e->st, 0);
-  mdk::launch_lists(e->plan, e->st, 0);
-  mdk::launch_set_flags(e->plan, e->st, 1);
-  return e->sync_check();
+  mdk::launch_set_flags(P, e->st, 0);
+  mdk::launch_lists(P, e->st, 0);
+  mdk::launch_set_flags(P, e->st, 1);
+  return e->check_all();
 }
 
 extern "C" int md_download_list(md_engine *e, int32_t which, int32_t *pairs, int64_t cap,
                                 int64_t *npair) {
   ENGINE_CHECK(e && npair && (which == 0 || which == 1));
-  int rc = e->sync_check();
+  FIRST(e);
+  int rc = e->check_all();
   if (rc) return rc;
   const bool lj = which == 0;
-  const int n = lj ? e->p.NN : e->h.Nq;
-  const int lcap = lj ? e->h.nbcap1 : e->h.nbcap2;
-  if (!lj && !e->h.real_verlet) {
+  const int n = lj ? s->p.NN : s->h.Nq;
+  const int lcap = lj ? s->h.nbcap1 : s->h.nbcap2;
+  if (!lj && !s->h.real_verlet) {
     *npair = 0;
     return MD_OK;
   }
   std::vector<int> cnt(n), nb((size_t)n * lcap);
-  (void)hipMemcpy(cnt.data(), lj ? e->h.nbn1 : e->h.nbn2, n * sizeof(int), hipMemcpyDeviceToHost);
-  (void)hipMemcpy(nb.data(), lj ? e->h.nb1 : e->h.nb2, nb.size() * sizeof(int),
-                  hipMemcpyDeviceToHost);
+  (void)hipMemcpy(cnt.data(), lj ? s->h.nbn1 : s->h.nbn2, n * sizeof(int), hipMemcpyDeviceToHost);
+  (void)hipMemcpy(nb.data(), lj ? s->h.nb1 : s->h.nb2, nb.size() * sizeof(int), hipMemcpyDeviceToHost);
   std::vector<std::pair<int, int>> pr;
   for (int t = 0; t < n; t++) {
-    const int i = lj ? t : e->topo.charge[t];
+    const int i = lj ? t : s->charge[t];
     for (int k = 0; k < std::min(cnt[t], lcap); k++) pr.emplace_back(i + 1, nb[(size_t)k * n + t] + 1);
   }
   std::sort(pr.begin(), pr.end());
@@ -661,43 +913,49 @@ extern "C" int md_download_list(md_engine *e, int32_t which, int32_t *pairs, int
 
 extern "C" int md_compute_force(md_engine *e, int32_t step, int32_t noise, const double *eta) {
   ENGINE_CHECK(e && step >= 0);
+  FIRST(e);
   STATE_CHECK(e);
-  return e->launch_compute_force(step, noise, eta, false);
+  return e->launch_compute_force(P, s, step, noise, eta, false);
 }
 
 extern "C" int md_new_position(md_engine *e, int32_t step, int32_t noise, const double *eta) {
   ENGINE_CHECK(e && step >= 0);
+  FIRST(e);
   STATE_CHECK(e);
-  mdk::launch_kick_drift(e->plan, e->st);
-  return e->launch_compute_force(step, noise, eta, true);
+  mdk::launch_kick_drift(P, e->st);
+  return e->launch_compute_force(P, s, step, noise, eta, true);
 }
 
 extern "C" int md_update_verlet_list(md_engine *e) {
   ENGINE_CHECK(e);
+  FIRST(e);
   STATE_CHECK(e);
-  mdk::launch_lists(e->plan, e->st, 0);
-  mdk::launch_set_flags(e->plan, e->st, 1);
+  mdk::launch_lists(P, e->st, 0);
+  mdk::launch_set_flags(P, e->st, 1);
   hipError_t err = hipGetLastError();
   return err == hipSuccess ? MD_OK : e->hip_fail(err, "update_verlet_list");
 }
 
 extern "C" int md_rescale_velocity(md_engine *e) {
   ENGINE_CHECK(e);
+  FIRST(e);
   STATE_CHECK(e);
-  mdk::launch_rescale(e->plan, e->st);
+  mdk::launch_rescale(P, e->st);
   hipError_t err = hipGetLastError();
   return err == hipSuccess ? MD_OK : e->hip_fail(err, "rescale_velocity");
 }
 
 extern "C" int md_run(md_engine *e, int32_t step0, int32_t nsteps, int32_t preheat) {
   ENGINE_CHECK(e && step0 >= 0 && nsteps >= 0);
+  FIRST(e);
   STATE_CHECK(e);
+  if (e->sys.size() != 1) return e->fail(MD_EINVAL, "md_run drives single-system engines");
   int step = step0;
   const int end = step0 + nsteps;
-  auto one = [&](int s) -> int {
-    int rc = md_new_position(e, s, MD_NOISE_PHILOX, nullptr);
+  auto one = [&](int st) -> int {
+    int rc = md_new_position(e, st, MD_NOISE_PHILOX, nullptr);
     if (rc) return rc;
-    if (preheat && s % 20 == 0 && s < 300000) {
+    if (preheat && st % 20 == 0 && st < 300000) {
       rc = md_rescale_velocity(e);
       if (rc) return rc;
     }
@@ -708,61 +966,62 @@ extern "C" int md_run(md_engine *e, int32_t step0, int32_t nsteps, int32_t prehe
       int rc = one(step);
       if (rc) return rc;
     }
-    return e->sync_check();
+    return e->check_all();
   }
-  const int ms = e->p.multistep;
-  while (step < end && step % ms != 0) {
+  const int ms = e->multistep;
+  while (step < end && (step - 1) % ms != 0) {
     int rc = one(step++);
     if (rc) return rc;
   }
   if (end - step >= ms) {
-    int rc = e->build_graph();
-    if (rc) return rc;
-    // step counter for the replay: the graph reads it from the control block
-    *e->h_step = step;
-    hipError_t err = hipMemcpyAsync(&e->h.ctrl->step, e->h_step, sizeof(int),
+    e->h_steps[0] = step;  // the replayed steps read their numbers here
+    hipError_t err = hipMemcpyAsync(&s->h.ctrl->step, e->h_steps, sizeof(int),
                                     hipMemcpyHostToDevice, e->st);
     if (err != hipSuccess) return e->hip_fail(err, "step upload");
-    while (end - step >= ms) {
-      err = hipGraphLaunch(e->gexec, e->st);
-      if (err != hipSuccess) return e->hip_fail(err, "graph launch");
-      step += ms;
-    }
+    const int nc = (end - step) / ms;
+    int rc = e->run_cycles(nc);
+    if (rc) return rc;
+    step += nc * ms;
   }
   while (step < end) {
     int rc = one(step++);
     if (rc) return rc;
   }
-  return e->sync_check();
+  return e->check_all();
 }
 
 extern "C" int md_force_parts(md_engine *e, double *f_short, double *f_coul) {
   ENGINE_CHECK(e);
+  FIRST(e);
   STATE_CHECK(e);
-  const size_t n = 3 * (size_t)e->p.NN;
-  if (e->fp32) mdk::launch_short<float>(e->plan, e->st, 0, 0, nullptr, e->d_out1);
-  else mdk::launch_short<double>(e->plan, e->st, 0, 0, nullptr, e->d_out1);
-  (void)hipMemsetAsync(e->d_out2, 0, n * sizeof(double), e->st);
-  if (e->coulomb) {
-    if (e->fp32) mdk::launch_coulomb<float>(e->plan, e->st, e->d_out2);
-    else mdk::launch_coulomb<double>(e->plan, e->st, e->d_out2);
+  const size_t n = 3 * (size_t)s->p.NN;
+  if (e->fp32) mdk::launch_short<float>(P, e->st, 0, 0, nullptr, s->d_out1);
+  else mdk::launch_short<double>(P, e->st, 0, 0, nullptr, s->d_out1);
+  (void)hipMemsetAsync(s->d_out2, 0, n * sizeof(double), e->st);
+  if (s->coulomb) {
+    if (e->fp32) mdk::launch_coulomb<float>(P, e->st, s->d_out2);
+    else mdk::launch_coulomb<double>(P, e->st, s->d_out2);
   }
-  int rc = e->sync_check();
-  if (f_short) (void)hipMemcpy(f_short, e->d_out1, n * sizeof(double), hipMemcpyDeviceToHost);
-  if (f_coul) (void)hipMemcpy(f_coul, e->d_out2, n * sizeof(double), hipMemcpyDeviceToHost);
+  int rc = e->check_all();
+  if (f_short) (void)hipMemcpy(f_short, s->d_out1, n * sizeof(double), hipMemcpyDeviceToHost);
+  if (f_coul) (void)hipMemcpy(f_coul, s->d_out2, n * sizeof(double), hipMemcpyDeviceToHost);
   return rc;
 }
 
 // Mean duration of each kernel of a step, from HIP events on the engine's
-// stream.  ms[k]: 0 kick_drift, 1 short_range, 2 spread, 3 fft_x_fwd,
-// 4 fft_y_fwd, 5 fft_z_conv, 6 fft_y_inv, 7 fft_x_inv, 8 gather, 9 real,
-// 10 bin, 11 cellsort, 12 build.  Leaves the state advanced by the timed
-// kick-drifts.
+// stream, over all replicas.  ms[k]: 0 kick_drift, 1 short_range, 2 spread,
+// 3 fft_x_fwd, 4 fft_y_fwd, 5 fft_z_conv, 6 fft_y_inv, 7 fft_x_inv,
+// 8 gather, 9 real, 10 bin, 11 cellsort, 12 build.  Leaves the state
+// advanced by the timed kick-drifts.
 extern "C" int md_time_kernels(md_engine *e, int32_t iters, double *ms, int32_t n) {
   ENGINE_CHECK(e && ms && iters > 0 && n >= 13);
+  FIRST(e);
   STATE_CHECK(e);
-  int rc = e->sync_check();
+  int rc = e->check_all();
   if (rc) return rc;
+  const Plan &B = e->plan;
+  bool any_coulomb = false;
+  for (Sys *x : e->sys) any_coulomb |= x->coulomb;
   hipEvent_t ev[10];
   for (auto &x : ev) (void)hipEventCreate(&x);
   double acc[13] = {0};
@@ -775,25 +1034,156 @@ extern "C" int md_time_kernels(md_engine *e, int32_t iters, double *ms, int32_t 
   const int mode = mdk::SR_HELD | mdk::SR_FIELD | mdk::SR_ANCHOR | mdk::SR_NOISE_PHILOX;
   for (int it = 0; it < iters; it++) {
     (void)hipEventRecord(ev[0], e->st);
-    mdk::launch_kick_drift(e->plan, e->st);
+    mdk::launch_kick_drift(B, e->st);
     (void)hipEventRecord(ev[1], e->st);
-    if (e->fp32) mdk::launch_short<float>(e->plan, e->st, mode, 0, nullptr, nullptr);
-    else mdk::launch_short<double>(e->plan, e->st, mode, 0, nullptr, nullptr);
+    if (e->fp32) mdk::launch_short<float>(B, e->st, mode, 0, nullptr, nullptr);
+    else mdk::launch_short<double>(B, e->st, mode, 0, nullptr, nullptr);
     (void)hipEventRecord(ev[2], e->st);
     acc[0] += span(0, 1);
     acc[1] += span(1, 2);
-    if (e->coulomb) {
-      if (e->fp32) mdk::launch_coulomb<float>(e->plan, e->st, nullptr, ev);
-      else mdk::launch_coulomb<double>(e->plan, e->st, nullptr, ev);
+    if (any_coulomb) {
+      if (e->fp32) mdk::launch_coulomb<float>(B, e->st, nullptr, ev);
+      else mdk::launch_coulomb<double>(B, e->st, nullptr, ev);
       for (int k = 0; k < 8; k++) acc[2 + k] += span(k, k + 1);
     }
-    mdk::launch_set_flags(e->plan, e->st, 0);
-    mdk::launch_lists(e->plan, e->st, 0, ev);
+    mdk::launch_set_flags(B, e->st, 0);
+    mdk::launch_lists(B, e->st, 0, ev);
     for (int k = 0; k < 3; k++) acc[10 + k] += span(k, k + 1);
   }
-  mdk::launch_set_flags(e->plan, e->st, 1);
-  rc = e->sync_check();
+  mdk::launch_set_flags(B, e->st, 1);
+  rc = e->check_all();
   for (auto &x : ev) (void)hipEventDestroy(x);
   for (int k = 0; k < 13; k++) ms[k] = acc[k] / iters;
   return rc;
+}
+
+// ------------------------------------------------------------- batches
+
+extern "C" int md_batch_create(md_engine **out, int32_t precision, int32_t multistep) {
+  if (!out) return MD_EINVAL;
+  md_engine *e = new md_engine();
+  *out = e;
+  return e->init(precision, multistep);
+}
+
+extern "C" int md_batch_add(md_engine *e, const md_params *p, const double *pos, const double *vel,
+                            int32_t *slot) {
+  ENGINE_CHECK(e && p && pos && vel && slot);
+  int sl;
+  int rc = e->add(p, &sl);
+  if (rc) return rc;
+  Sys *s = e->slots[sl];
+  rc = upload_sys(e, s, pos, vel, nullptr);
+  if (rc == MD_OK) rc = e->rebuild();
+  if (rc == MD_OK) {
+    const Plan &P = e->single[e->index_of(s)];
+    mdk::launch_set_flags(P, e->st, 0);
+    mdk::launch_lists(P, e->st, 0);
+    mdk::launch_set_flags(P, e->st, 1);
+    rc = e->launch_compute_force(P, s, 0, MD_NOISE_PHILOX, nullptr, false);
+  }
+  if (rc == MD_OK) {
+    int one = 1;  // the first replayed step of this replica
+    hipError_t err = hipMemcpyAsync(&s->h.ctrl->step, &one, sizeof one, hipMemcpyHostToDevice, e->st);
+    if (err == hipSuccess) err = hipStreamSynchronize(e->st);
+    if (err != hipSuccess) rc = e->hip_fail(err, "control block");
+  }
+  if (rc) {
+    (void)hipStreamSynchronize(e->st);
+    e->slots.erase(sl);
+    e->sys.erase(std::find(e->sys.begin(), e->sys.end(), s));
+    delete s;
+    e->dirty = true;
+    return rc;
+  }
+  *slot = sl;
+  return MD_OK;
+}
+
+extern "C" int md_batch_remove(md_engine *e, int32_t slot) {
+  ENGINE_CHECK(e);
+  auto it = e->slots.find(slot);
+  if (it == e->slots.end()) return e->fail(MD_EINVAL, "no such slot");
+  int rc = e->sync();
+  if (rc) return rc;
+  Sys *s = it->second;
+  e->sys.erase(std::find(e->sys.begin(), e->sys.end(), s));
+  e->slots.erase(it);
+  delete s;
+  e->dirty = true;
+  return MD_OK;
+}
+
+extern "C" int md_batch_run(md_engine *e, int32_t ncycles) {
+  ENGINE_CHECK(e && ncycles >= 0);
+  int rc = e->run_cycles(ncycles);
+  if (rc) return rc;
+  return e->sync();
+}
+
+extern "C" int md_batch_status(md_engine *e, int32_t slot, int32_t *step, char *buf, int32_t len) {
+  ENGINE_CHECK(e);
+  auto it = e->slots.find(slot);
+  if (it == e->slots.end()) return e->fail(MD_EINVAL, "no such slot");
+  int rc = e->sync();
+  if (rc) return rc;
+  Ctrl c;
+  hipError_t err = hipMemcpy(&c, it->second->h.ctrl, sizeof c, hipMemcpyDeviceToHost);
+  if (err != hipSuccess) return e->hip_fail(err, "control block");
+  if (step) *step = c.step;
+  std::string msg;
+  rc = e->check_one(it->second, c, msg);
+  if (buf && len > 0) std::snprintf(buf, (size_t)len, "%s", msg.c_str());
+  return rc;
+}
+
+extern "C" int md_batch_download(md_engine *e, int32_t slot, double *pos, double *vel, double *acc) {
+  ENGINE_CHECK(e);
+  auto it = e->slots.find(slot);
+  if (it == e->slots.end()) return e->fail(MD_EINVAL, "no such slot");
+  return download_sys(e, it->second, pos, vel, acc, nullptr);
+}
+
+extern "C" int md_batch_info(md_engine *e, int32_t slot, int64_t *bytes, int32_t *nrep) {
+  ENGINE_CHECK(e);
+  if (nrep) *nrep = (int32_t)e->sys.size();
+  if (bytes) {
+    if (slot < 0) {
+      int64_t b = 0;
+      for (Sys *s : e->sys) b += (int64_t)s->bytes;
+      *bytes = b;
+    } else {
+      auto it = e->slots.find(slot);
+      if (it == e->slots.end()) return e->fail(MD_EINVAL, "no such slot");
+      *bytes = (int64_t)it->second->bytes;
+    }
+  }
+  return MD_OK;
+}
+
+extern "C" int md_batch_observe(md_engine *e, int32_t slot, double *height, double *profile,
+                                int32_t nbins) {
+  ENGINE_CHECK(e && nbins == mdk::kHistBins);
+  auto it = e->slots.find(slot);
+  if (it == e->slots.end()) return e->fail(MD_EINVAL, "no such slot");
+  Sys *s = it->second;
+  if (e->dirty) {
+    int rc = e->rebuild();
+    if (rc) return rc;
+  }
+  const int idx = e->index_of(s);
+  mdk::launch_observe(e->single[idx], e->st, s->d_hist, s->d_hsum);
+  int rc = e->sync();
+  if (rc) return rc;
+  const int nb = cdiv(s->p.NN, mdk::kBlock);
+  std::vector<double> hs(nb);
+  std::vector<int> hist(mdk::kHistBins);
+  (void)hipMemcpy(hs.data(), s->d_hsum, nb * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipMemcpy(hist.data(), s->d_hist, hist.size() * sizeof(int), hipMemcpyDeviceToHost);
+  double sum = 0;
+  for (double v : hs) sum += v;
+  if (height) *height = sum / s->p.Npe;
+  if (profile)
+    for (int k = 0; k < mdk::kHistBins; k++) profile[k] = hist[k];
+  return MD_OK;
 }

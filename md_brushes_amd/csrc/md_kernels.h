@@ -34,5 +34,6 @@ void launch_kick_drift(const Plan &P, hipStream_t st);
 void launch_lists(const Plan &P, hipStream_t st, int inc_step, hipEvent_t *ev = nullptr);
 void launch_set_flags(const Plan &P, hipStream_t st, int clear);
 void launch_rescale(const Plan &P, hipStream_t st);
+void launch_observe(const Plan &P, hipStream_t st, int *hist, double *hsum);
 
 }  // namespace mdk

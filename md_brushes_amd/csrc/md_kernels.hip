@@ -997,7 +997,46 @@ __global__ __launch_bounds__(kBlock) void k_vscale(const Rep *__restrict__ reps,
   S.vz[i] = (S.vz[i] - sh[2] / n) * s;
 }
 
+// ------------------------------------------------------- observables
+
+// Brush height and z-density of the polymer beads: per-block sums of z
+// (height = mean z over the Npe beads, in_output.f90:1017-1020) and a
+// histogram of SizeHist bins of Lz/SizeHist with ceiling binning
+// (in_output.f90:1878-1886).  Integer bins and fixed-order block sums keep
+// the result independent of scheduling.
+__global__ void k_hist_clear(int *hist) {
+  for (int k = threadIdx.x; k < kHistBins; k += blockDim.x) hist[k] = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_observe(const Rep *__restrict__ reps, int *hist,
+                                                    double *hsum) {
+  const Rep &S = reps[0];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double z = 0;
+  if (i < S.Npe) {
+    z = S.pos[i].z;
+    const int k = (int)ceil(z / (S.Lz / kHistBins));
+    if (k >= 1 && k <= kHistBins) atomicAdd(&hist[k - 1], 1);
+  }
+  __shared__ double red[kBlock / 64];
+  z = wave_sum(z);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = z;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0;
+    for (int w = 0; w < kBlock / 64; w++) s += red[w];
+    hsum[blockIdx.x] = s;
+  }
+}
+
 // --------------------------------------------------------- launchers
+
+// One replica (P.nrep == 1); Npe is read from the descriptor.
+void launch_observe(const Plan &P, hipStream_t st, int *hist, double *hsum) {
+  hipLaunchKernelGGL(k_hist_clear, dim3(1), dim3(kBlock), 0, st, hist);
+  const int nb = P.part.total;  // enough blocks for Npe <= NN
+  hipLaunchKernelGGL(k_observe, dim3(nb), dim3(kBlock), 0, st, P.reps, hist, hsum);
+}
 
 // With ev != null, records ev[k] before kernel k and ev[8] after the last
 // (kernel timing); never used under graph capture.

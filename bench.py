@@ -173,9 +173,8 @@ def main():
 
     # per-kernel durations (HIP events on the engine's stream)
     ms = e32.time_kernels(20)
-    per_step = {k: v * (1.0 / p32.multistep if k in ("spread", "fft_x_fwd", "fft_y_fwd", "fft_z_conv",
-                                                    "fft_y_inv", "fft_x_inv", "gather", "real") else 1.0)
-                for k, v in ms.items() if k not in ("bin", "cellsort", "build")}
+    per_step = {k: v * (1.0 / p32.multistep if k in Engine.COULOMB else 1.0)
+                for k, v in ms.items() if k not in Engine.LISTS}
     dom = max(per_step, key=per_step.get)
     b = kernel_bytes(p32, dom)
     achieved = b / (ms[dom] * 1e-3)

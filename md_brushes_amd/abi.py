@@ -250,13 +250,15 @@ class Engine:
     def run(self, step0, nsteps, preheat=False):
         self._check(self.lib.md_run(self.h, step0, nsteps, int(preheat)), "md_run")
 
-    KERNELS = ["kick_drift", "short_range", "spread", "fft_x_fwd", "fft_y_fwd", "fft_z_conv",
-               "fft_y_inv", "fft_x_inv", "gather", "real", "bin", "cellsort", "build"]
+    KERNELS = ["kick_drift", "short_range", "bin_q", "sort_q", "spread", "fft_x_fwd", "fft_y_fwd",
+               "fft_z_conv", "fft_y_inv", "fft_x_inv", "gather", "real", "bin_lj", "sort_lj", "build_lj"]
+    COULOMB = KERNELS[2:12]
+    LISTS = KERNELS[12:]
 
     def time_kernels(self, iters=20):
         """Mean ms per launch of each kernel (HIP events); advances the state."""
-        ms = np.zeros(13)
-        self._check(self.lib.md_time_kernels(self.h, iters, _dp(ms), 13), "md_time_kernels")
+        ms = np.zeros(15)
+        self._check(self.lib.md_time_kernels(self.h, iters, _dp(ms), 15), "md_time_kernels")
         return dict(zip(self.KERNELS, ms))
 
     def force_parts(self):

@@ -13,6 +13,7 @@ namespace mdk {
 
 constexpr int kBlock = 256;
 constexpr int kMaxOrder = 8;
+constexpr int kRealChunk = 16;  // target charges per workgroup of the real-space sweep
 constexpr int kHistBins = 500;  // SizeHist of the z-density profile (glob_var.f90:59)
 
 // Per-replica control block, written by kernels.
@@ -73,8 +74,15 @@ struct Rep {
   int *charge;                     // Nq: particle index of charge m
   int *bstart;                     // NN+1 CSR of bonded partners
   int *bpart;
-  int *cnt1[2], *mem1;             // LJ cells: count (double-buffered), members
-  int *cnt2[2], *mem2;             // real cells
+  int *cnt1, *mem1, *scnt1;        // LJ cells: binning counts, members, sorted counts
+  int *cnt2[2], *mem2;             // cells of the reference-form real-space list
+  // real-space cell sweep at each Coulomb evaluation (edge >= r_c)
+  int real_cells;                  // 1: 27-cell sweep; 0: all charge pairs
+  int nc3[3];
+  double rcel3[3];
+  int ccap3;
+  int *cnt3, *mem3, *scnt3;
+  int G;                           // lanes per charge in spread and gather
   int *nbn1, *nb1;                 // LJ list: length per particle, ELL slots
   int *nbn2, *nb2;                 // real list: per charge slot m
   unsigned long long *meshq;       // fixed-point spread mesh K3*K2*K1
@@ -83,6 +91,7 @@ struct Rep {
   void *bc;                        // influence function K3*K2*Kh real
   const void *tw[3];               // twiddles per axis (K_d/2 complex)
   double *fsp;                     // Nq*3 reciprocal force before mean removal
+  double *rsum;                    // Nq*3 real-space sums across candidate tiles
   double *part;                    // block partial sums: 4 per charge block
   int nqblk;                       // number of charge blocks
   Ctrl *ctrl;

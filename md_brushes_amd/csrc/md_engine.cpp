@@ -28,7 +28,7 @@ using mdk::WMap;
 namespace {
 
 constexpr double kPi = 3.141592653589793;
-constexpr int kMaps = 7;  // part, chg, fx, fy, fz, cell1, cell2
+constexpr int kMaps = 11;  // part, chg, chgG, fx, fy, fz, cell1, cell1w, cell2, cell3, cell3c
 
 bool is_pow2(int n) { return n >= 4 && (n & (n - 1)) == 0; }
 int ilog2(int n) {
@@ -75,9 +75,9 @@ struct Sys {
   std::vector<int> charge, flags, bstart, bpart;  // host topology
   std::vector<void *> allocs;
   size_t bytes = 0;
-  int ncell1 = 0, ncell2 = 0;
+  int ncell1 = 0, ncell2 = 0, ncell3 = 0;
   int nb[kMaps] = {0};
-  size_t lds_x = 0, lds_y = 0, lds_z = 0, lds_sort = 0;
+  size_t lds_x = 0, lds_y = 0, lds_z = 0, lds_sort = 0, lds_sort3 = 0;
   double *d_eta = nullptr, *d_out1 = nullptr, *d_out2 = nullptr;
   int *d_hist = nullptr;     // observables: z histogram of the chains
   double *d_hsum = nullptr;  // observables: per-block sums of z
@@ -203,6 +203,20 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   if (S.nc1[0] < 1 || S.nc1[1] < 1 || S.nc1[2] < 1) return fail(MD_EINVAL, "box smaller than rvl");
   ncell1 = S.nc1[0] * S.nc1[1] * S.nc1[2];
   ncell2 = S.real_verlet ? S.nc2[0] * S.nc2[1] * S.nc2[2] : 0;
+  // real-space sweep cells: edge >= r_c; three or more per axis, or all pairs
+  S.real_cells = 0;
+  ncell3 = 0;
+  if (coulomb) {
+    bool ok = true;
+    for (int d = 0; d < 3; d++) {
+      S.nc3[d] = (int)(L3[d] / p.rcc);
+      ok = ok && S.nc3[d] >= 3;
+      S.rcel3[d] = S.nc3[d] > 0 ? L3[d] / S.nc3[d] : L3[d];
+    }
+    S.real_cells = ok ? 1 : 0;
+    if (ok) ncell3 = S.nc3[0] * S.nc3[1] * S.nc3[2];
+  }
+  S.G = (coulomb && p.ordr[0] > 4) ? 64 : 16;
 
   // state and topology on the device
   const size_t nn = NN;
@@ -232,13 +246,17 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   if (Nq) (void)hipMemcpy(S.charge, charge.data(), Nq * sizeof(int), hipMemcpyHostToDevice);
   (void)hipMemcpy(S.bstart, bstart.data(), (nn + 1) * sizeof(int), hipMemcpyHostToDevice);
   (void)hipMemcpy(S.bpart, bpart.data(), bpart.size() * sizeof(int), hipMemcpyHostToDevice);
-  S.cnt1[0] = dalloc<int>(ncell1);
-  S.cnt1[1] = dalloc<int>(ncell1);
+  S.cnt1 = dalloc<int>(ncell1);
+  S.scnt1 = dalloc<int>(ncell1);
   S.cnt2[0] = dalloc<int>(std::max(ncell2, 1));
   S.cnt2[1] = dalloc<int>(std::max(ncell2, 1));
+  S.cnt3 = dalloc<int>(std::max(ncell3, 1));
+  S.scnt3 = dalloc<int>(std::max(ncell3, 1));
   S.nbn1 = dalloc<int>(nn);
   S.nbn2 = dalloc<int>(std::max(Nq, 1));
-  if (!S.cnt1[0] || !S.cnt1[1] || !S.cnt2[0] || !S.cnt2[1] || !S.nbn1 || !S.nbn2)
+  S.rsum = dalloc<double>(3 * (size_t)std::max(Nq, 1));
+  if (!S.cnt1 || !S.scnt1 || !S.cnt2[0] || !S.cnt2[1] || !S.cnt3 || !S.scnt3 || !S.nbn1 ||
+      !S.nbn2 || !S.rsum)
     return fail(MD_ENOMEM, "cells");
 
   // SPME: mesh, spectrum, influence function, twiddles
@@ -312,7 +330,7 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
     S.px_tile = std::max(1, std::min(K2 * K3 / 2, budget / (K1 * csz)));
     S.py_tile = std::max(1, std::min(S.Kh, budget / (K2 * csz)));
     S.pz_tile = std::max(1, std::min(S.Kh, budget / (K3 * csz)));
-    S.nqblk = cdiv(Nq, mdk::kBlock);
+    S.nqblk = cdiv((long)Nq * S.G, mdk::kBlock);
     S.fsp = dalloc<double>(3 * (size_t)Nq);
     S.part = dalloc<double>(4 * (size_t)S.nqblk);
     if (!S.fsp || !S.part) return fail(MD_ENOMEM, "spme scratch");
@@ -322,14 +340,19 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   }
   nb[0] = cdiv(NN, mdk::kBlock);
   nb[1] = coulomb ? cdiv(Nq, mdk::kBlock) : 0;
-  nb[2] = coulomb ? cdiv((long)S.K[1] * S.K[2] / 2, S.px_tile) : 0;
-  nb[3] = coulomb ? S.K[2] * cdiv(S.Kh, S.py_tile) : 0;
-  nb[4] = coulomb ? S.K[1] * cdiv(S.Kh, S.pz_tile) : 0;
-  nb[5] = cdiv(ncell1, mdk::kBlock);
-  nb[6] = ncell2;
-  S.mem1 = S.mem2 = nullptr;
+  nb[2] = coulomb ? S.nqblk : 0;
+  nb[3] = coulomb ? cdiv((long)S.K[1] * S.K[2] / 2, S.px_tile) : 0;
+  nb[4] = coulomb ? S.K[2] * cdiv(S.Kh, S.py_tile) : 0;
+  nb[5] = coulomb ? S.K[1] * cdiv(S.Kh, S.pz_tile) : 0;
+  // cell kernels stride over the cells with a grid of modest size
+  nb[6] = std::min(ncell1, 1024);
+  nb[7] = std::min(cdiv(ncell1, mdk::kBlock / 64), 512);
+  nb[8] = ncell2;
+  nb[9] = ncell3;
+  nb[10] = 0;  // set with the cell capacity (size_lists)
+  S.mem1 = S.mem2 = S.mem3 = nullptr;
   S.nb1 = S.nb2 = nullptr;
-  S.ccap1 = S.ccap2 = S.nbcap1 = S.nbcap2 = 0;
+  S.ccap1 = S.ccap2 = S.ccap3 = S.nbcap1 = S.nbcap2 = 0;
   return MD_OK;
 }
 
@@ -380,18 +403,32 @@ int Sys::size_lists(const double *pos) {
   };
   std::vector<int> all(NN);
   for (int i = 0; i < NN; i++) all[i] = i;
-  int mc1, mn1, mc2 = 0, mn2 = 0;
+  int mc1, mn1, mc2 = 0, mn2 = 0, mc3 = 0, mn3 = 0;
   scan(all, h.nc1, p.rvl_eff, mc1, mn1);
   if (h.real_verlet) scan(charge, h.nc2, p.rvc, mc2, mn2);
+  if (h.real_cells) scan(charge, h.nc3, 0.0, mc3, mn3);
   auto round8 = [](int v) { return (v + 7) & ~7; };
   // LJ: dense packing bounds occupancy near 1.4 per unit volume
   const double v1 = h.rcel1[0] * h.rcel1[1] * h.rcel1[2];
-  const int ccap1 = round8(std::max(2 * mc1 + 16, (int)(1.5 * v1) + 16));
+  // at most 128: the LJ cell sort holds two members per lane of a wavefront
+  const int ccap1 = std::min(128, round8(std::max(2 * mc1 + 16, (int)(1.5 * v1) + 16)));
   const int nbcap1 =
       round8(std::min(NN, std::max(2 * mn1 + 32, (int)(4.19 * std::pow(p.rvl_eff, 3) * 1.2) + 32)));
   const int ccap2 = h.real_verlet ? round8(std::min(h.Nq, 2 * mc2 + 64)) : 0;
   const int nbcap2 = h.real_verlet ? round8(std::min(h.Nq, 2 * mn2 + 64)) : 0;
-  if (ccap2 > 16384) return fail(MD_EINVAL, "real-space cells too full for the LDS sort");
+  const int ccap3 = h.real_cells ? round8(std::min(h.Nq, 2 * mc3 + 64)) : 0;
+  if (ccap2 > 16384 || ccap3 > 16384)
+    return fail(MD_EINVAL, "real-space cells too full for the LDS sort");
+  if (h.real_cells && ccap3 != h.ccap3) {
+    if (h.mem3) dfree(h.mem3, (size_t)ncell3 * h.ccap3 * 4);
+    h.mem3 = dalloc<int>((size_t)ncell3 * ccap3);
+    h.ccap3 = ccap3;
+    if (!h.mem3) return fail(MD_ENOMEM, "real-space cells");
+  }
+  int np3 = 1;
+  while (np3 < h.ccap3) np3 <<= 1;
+  lds_sort3 = h.real_cells ? (size_t)np3 * sizeof(int) : 0;
+  nb[10] = h.real_cells ? std::min(ncell3 * cdiv(h.ccap3, mdk::kRealChunk), 1024) : 0;
   if (ccap1 != h.ccap1) {
     if (h.mem1) dfree(h.mem1, (size_t)ncell1 * h.ccap1 * 4);
     h.mem1 = dalloc<int>((size_t)ncell1 * ccap1);
@@ -549,18 +586,21 @@ int md_engine::rebuild() {
   auto fill = [&](Plan &P, Rep *reps, int nrep, const int *maps, const int *totals) {
     P.reps = reps;
     P.nrep = nrep;
-    WMap *w[kMaps] = {&P.part, &P.chg, &P.fx, &P.fy, &P.fz, &P.cell1, &P.cell2};
+    WMap *w[kMaps] = {&P.part, &P.chg, &P.chgG, &P.fx, &P.fy, &P.fz, &P.cell1, &P.cell1w, &P.cell2, &P.cell3, &P.cell3c};
     for (int k = 0; k < kMaps; k++) *w[k] = WMap{maps + k * (nrep + 1), nrep, totals[k]};
     P.vpart = vpart;
     P.fp32 = fp32;
   };
   fill(plan, d_reps, R, d_maps, tot);
-  plan.lds_x = plan.lds_y = plan.lds_z = plan.lds_sort = 0;
+  plan.lds_x = plan.lds_y = plan.lds_z = plan.lds_sort = plan.lds_sort3 = 0;
+  plan.pairs_any = false;
   for (Sys *s : sys) {
     plan.lds_x = std::max(plan.lds_x, s->lds_x);
     plan.lds_y = std::max(plan.lds_y, s->lds_y);
     plan.lds_z = std::max(plan.lds_z, s->lds_z);
     plan.lds_sort = std::max(plan.lds_sort, s->lds_sort);
+    plan.lds_sort3 = std::max(plan.lds_sort3, s->lds_sort3);
+    plan.pairs_any = plan.pairs_any || (s->coulomb && !s->h.real_cells);
   }
   single.assign(R, Plan{});
   for (int r = 0; r < R; r++) {
@@ -569,6 +609,8 @@ int md_engine::rebuild() {
     single[r].lds_y = sys[r]->lds_y;
     single[r].lds_z = sys[r]->lds_z;
     single[r].lds_sort = sys[r]->lds_sort;
+    single[r].lds_sort3 = sys[r]->lds_sort3;
+    single[r].pairs_any = sys[r]->coulomb && !sys[r]->h.real_cells;
   }
   return MD_OK;
 }
@@ -604,8 +646,9 @@ int md_engine::check_one(const Sys *s, const Ctrl &c, std::string &msg) const {
   }
   if (c.ovf) {
     std::snprintf(buf, sizeof buf,
-                  "capacity exceeded (mask %d): LJ cell %d/%d list %d/%d, real cell %d/%d list %d/%d",
-                  c.ovf, c.max_cell1, s->h.ccap1, c.max_nb1, s->h.nbcap1, c.max_cell2, s->h.ccap2,
+                  "capacity exceeded (mask %d): LJ cell %d/%d list %d/%d, real-space cell %d/%d, "
+                  "real list %d/%d",
+                  c.ovf, c.max_cell1, s->h.ccap1, c.max_nb1, s->h.nbcap1, c.max_cell2, s->h.ccap3,
                   c.max_nb2, s->h.nbcap2);
     msg = buf;
     return MD_EOVERFLOW;
@@ -876,6 +919,7 @@ extern "C" int md_build_lists(md_engine *e) {
   STATE_CHECK(e);
   mdk::launch_set_flags(P, e->st, 0);
   mdk::launch_lists(P, e->st, 0);
+  mdk::launch_real_list(P, e->st);
   mdk::launch_set_flags(P, e->st, 1);
   return e->check_all();
 }
@@ -1009,12 +1053,12 @@ extern "C" int md_force_parts(md_engine *e, double *f_short, double *f_coul) {
 }
 
 // Mean duration of each kernel of a step, from HIP events on the engine's
-// stream, over all replicas.  ms[k]: 0 kick_drift, 1 short_range, 2 spread,
-// 3 fft_x_fwd, 4 fft_y_fwd, 5 fft_z_conv, 6 fft_y_inv, 7 fft_x_inv,
-// 8 gather, 9 real, 10 bin, 11 cellsort, 12 build.  Leaves the state
-// advanced by the timed kick-drifts.
+// stream, over all replicas.  ms[k]: 0 kick_drift, 1 short_range, 2 bin_q,
+// 3 sort_q, 4 spread, 5 fft_x_fwd, 6 fft_y_fwd, 7 fft_z_conv, 8 fft_y_inv,
+// 9 fft_x_inv, 10 gather, 11 real, 12 bin_lj, 13 sort_lj, 14 build_lj.
+// Leaves the state advanced by the timed kick-drifts.
 extern "C" int md_time_kernels(md_engine *e, int32_t iters, double *ms, int32_t n) {
-  ENGINE_CHECK(e && ms && iters > 0 && n >= 13);
+  ENGINE_CHECK(e && ms && iters > 0 && n >= 15);
   FIRST(e);
   STATE_CHECK(e);
   int rc = e->check_all();
@@ -1022,9 +1066,9 @@ extern "C" int md_time_kernels(md_engine *e, int32_t iters, double *ms, int32_t 
   const Plan &B = e->plan;
   bool any_coulomb = false;
   for (Sys *x : e->sys) any_coulomb |= x->coulomb;
-  hipEvent_t ev[10];
+  hipEvent_t ev[12];
   for (auto &x : ev) (void)hipEventCreate(&x);
-  double acc[13] = {0};
+  double acc[15] = {0};
   float t;
   auto span = [&](int a, int b) {
     (void)hipEventSynchronize(ev[b]);
@@ -1044,16 +1088,16 @@ extern "C" int md_time_kernels(md_engine *e, int32_t iters, double *ms, int32_t 
     if (any_coulomb) {
       if (e->fp32) mdk::launch_coulomb<float>(B, e->st, nullptr, ev);
       else mdk::launch_coulomb<double>(B, e->st, nullptr, ev);
-      for (int k = 0; k < 8; k++) acc[2 + k] += span(k, k + 1);
+      for (int k = 0; k < 10; k++) acc[2 + k] += span(k, k + 1);
     }
     mdk::launch_set_flags(B, e->st, 0);
     mdk::launch_lists(B, e->st, 0, ev);
-    for (int k = 0; k < 3; k++) acc[10 + k] += span(k, k + 1);
+    for (int k = 0; k < 3; k++) acc[12 + k] += span(k, k + 1);
   }
   mdk::launch_set_flags(B, e->st, 1);
   rc = e->check_all();
   for (auto &x : ev) (void)hipEventDestroy(x);
-  for (int k = 0; k < 13; k++) ms[k] = acc[k] / iters;
+  for (int k = 0; k < 15; k++) ms[k] = acc[k] / iters;
   return rc;
 }
 
@@ -1086,7 +1130,15 @@ extern "C" int md_batch_add(md_engine *e, const md_params *p, const double *pos,
     int one = 1;  // the first replayed step of this replica
     hipError_t err = hipMemcpyAsync(&s->h.ctrl->step, &one, sizeof one, hipMemcpyHostToDevice, e->st);
     if (err == hipSuccess) err = hipStreamSynchronize(e->st);
-    if (err != hipSuccess) rc = e->hip_fail(err, "control block");
+    Ctrl c;
+    if (err == hipSuccess) err = hipMemcpy(&c, s->h.ctrl, sizeof c, hipMemcpyDeviceToHost);
+    if (err != hipSuccess) {
+      rc = e->hip_fail(err, "control block");
+    } else {  // a request whose initial state is invalid is refused here
+      std::string msg;
+      rc = e->check_one(s, c, msg);
+      if (rc) e->fail(rc, msg);
+    }
   }
   if (rc) {
     (void)hipStreamSynchronize(e->st);

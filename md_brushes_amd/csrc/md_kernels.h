@@ -19,8 +19,12 @@ enum {
 struct Plan {
   Rep *reps = nullptr;
   int nrep = 0;
-  WMap part{}, chg{}, fx{}, fy{}, fz{}, cell1{}, cell2{};
-  size_t lds_x = 0, lds_y = 0, lds_z = 0, lds_sort = 0;
+  // block maps: particles (thread each), charges, charges x G lanes, FFT
+  // tiles, LJ cells (workgroup each, wavefront each), reference real-list
+  // cells, real-space sweep cells (workgroup each, per target chunk)
+  WMap part{}, chg{}, chgG{}, fx{}, fy{}, fz{}, cell1{}, cell1w{}, cell2{}, cell3{}, cell3c{};
+  size_t lds_x = 0, lds_y = 0, lds_z = 0, lds_sort = 0, lds_sort3 = 0;
+  bool pairs_any = false;   // some replica uses the all-pairs real space
   double *vpart = nullptr;  // partial sums of rescale_velocity
   bool fp32 = false;
 };
@@ -32,6 +36,7 @@ void launch_short(const Plan &P, hipStream_t st, int mode, int step, const doubl
                   double *out);
 void launch_kick_drift(const Plan &P, hipStream_t st);
 void launch_lists(const Plan &P, hipStream_t st, int inc_step, hipEvent_t *ev = nullptr);
+void launch_real_list(const Plan &P, hipStream_t st);
 void launch_set_flags(const Plan &P, hipStream_t st, int clear);
 void launch_rescale(const Plan &P, hipStream_t st);
 void launch_observe(const Plan &P, hipStream_t st, int *hist, double *hsum);

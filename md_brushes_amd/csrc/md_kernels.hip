@@ -6,6 +6,8 @@
 // integrator, M_z and all reductions stay double).
 #include "md_kernels.h"
 
+#include <type_traits>
+
 #include "../../include/md_engine.h"
 
 namespace mdk {
@@ -299,15 +301,9 @@ __global__ void k_set_flags(const Rep *__restrict__ reps, int nrep, int clear) {
     return;
   }
   c->flag1 = 1;
-  c->gen1++;
   c->flag2 = reps[r].real_verlet != 0 && reps[r].Nq > 0;
   if (c->flag2) c->gen2++;
 }
-
-struct CellGeo {
-  int nc[3];
-  double rc[3];
-};
 
 // Cell of a particle as lj_verlet_list computes it (compt_acc.f90:219-222).
 __device__ __forceinline__ bool cell_of(double4 p, const int nc[3], const double rc[3],
@@ -318,69 +314,244 @@ __device__ __forceinline__ bool cell_of(double4 p, const int nc[3], const double
   return a >= 0 && a < nc[0] && b >= 0 && b < nc[1] && c >= 0 && c < nc[2];
 }
 
-// Linked-cell binning of lj_verlet_list / real_verlet_list: blocks
-// [0, wm1.total) bin all particles into LJ cells, the rest bin charges into
-// real-space cells.
-__global__ __launch_bounds__(kBlock) void k_bin(const Rep *__restrict__ reps, WMap wm1, WMap wm2) {
-  const bool lj = (int)blockIdx.x < wm1.total;
-  const int bx = lj ? blockIdx.x : blockIdx.x - wm1.total;
-  const WMap &wm = lj ? wm1 : wm2;
-  const int r = find_rep(wm, bx);
+// Neighbour cell l (0..26) of cell (a,b,c) in the loop order of
+// lj_verlet_list (compt_acc.f90:232-255): x outer, y, z inner; x,y periodic,
+// z open (-1 when outside).
+__device__ __forceinline__ int stencil_cell(int l, int a, int b, int c, const int nc[3]) {
+  int p = a + l / 9 - 1, q = b + (l / 3) % 3 - 1, z = c + l % 3 - 1;
+  if (z < 0 || z >= nc[2]) return -1;
+  p = p >= nc[0] ? p - nc[0] : (p < 0 ? p + nc[0] : p);
+  q = q >= nc[1] ? q - nc[1] : (q < 0 ? q + nc[1] : q);
+  return (z * nc[1] + q) * nc[0] + p;
+}
+
+// LJ cells, binning pass of lj_verlet_list (compt_acc.f90:219-225).
+__global__ __launch_bounds__(kBlock) void k_bin_lj(const Rep *__restrict__ reps, WMap wm) {
+  const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
   Ctrl *c = S.ctrl;
-  if (lj ? !c->flag1 : !c->flag2) return;
-  const int t = (bx - wm.blk[r]) * kBlock + threadIdx.x;
-  if (t >= (lj ? S.NN : S.Nq)) return;
-  const int i = lj ? t : S.charge[t];
-  const int *nc = lj ? S.nc1 : S.nc2;
-  const double *rc = lj ? S.rcel1 : S.rcel2;
+  if (!c->flag1) return;
+  const int i = (blockIdx.x - wm.blk[r]) * kBlock + threadIdx.x;
+  if (i >= S.NN) return;
   int a, b, cz;
-  if (!cell_of(S.pos[i], nc, rc, S.hLx, S.hLy, a, b, cz)) {
+  if (!cell_of(S.pos[i], S.nc1, S.rcel1, S.hLx, S.hLy, a, b, cz)) {
     set_error(c, MD_ECELL, i, -1);
     return;
   }
-  const int cell = (cz * nc[1] + b) * nc[0] + a;
-  const int buf = (lj ? c->gen1 : c->gen2) & 1;
-  int *cnt = lj ? S.cnt1[buf] : S.cnt2[buf];
-  const int cap = lj ? S.ccap1 : S.ccap2;
-  const int slot = atomicAdd(&cnt[cell], 1);
-  if (slot < cap) (lj ? S.mem1 : S.mem2)[(size_t)cell * cap + slot] = t;
-  else atomicOr(&c->ovf, lj ? 1 : 2);
+  const int cell = (cz * S.nc1[1] + b) * S.nc1[0] + a;
+  const int slot = atomicAdd(&S.cnt1[cell], 1);
+  if (slot < S.ccap1) S.mem1[(size_t)cell * S.ccap1 + slot] = i;
+  else atomicOr(&c->ovf, 1);
 }
 
-// Sorts every cell's members ascending so that list order, and hence every
-// force sum, is independent of the order of the binning atomics.  LJ cells:
-// one thread per cell (insertion sort, cells hold a few tens).  Real-space
-// cells: one workgroup per cell (bitonic sort in LDS).
-__global__ __launch_bounds__(kBlock) void k_cellsort(const Rep *__restrict__ reps, WMap wm1,
-                                                     WMap wm2) {
-  const bool lj = (int)blockIdx.x < wm1.total;
-  const int bx = lj ? blockIdx.x : blockIdx.x - wm1.total;
-  const WMap &wm = lj ? wm1 : wm2;
-  const int r = find_rep(wm, bx);
-  const Rep &S = reps[r];
-  Ctrl *c = S.ctrl;
-  if (lj) {
-    if (!c->flag1) return;
-    const int cell = (bx - wm.blk[r]) * kBlock + threadIdx.x;
-    if (cell >= S.nc1[0] * S.nc1[1] * S.nc1[2]) return;
-    const int n = min(S.cnt1[c->gen1 & 1][cell], S.ccap1);
-    int *m = S.mem1 + (size_t)cell * S.ccap1;
-    for (int a = 1; a < n; a++) {
-      int v = m[a], b = a - 1;
-      while (b >= 0 && m[b] > v) {
-        m[b + 1] = m[b];
-        b--;
-      }
-      m[b + 1] = v;
-    }
+// One compare-exchange stage of a bitonic network whose element e of lane
+// `lane` is in register v (partner e ^ stride in lane ^ stride).
+__device__ __forceinline__ int bitonic_step(int v, int e, int stride, int size) {
+  const int pv = __shfl_xor(v, stride, 64);
+  const bool lower = (e & stride) == 0, up = (e & size) == 0;
+  return (lower == up) ? min(v, pv) : max(v, pv);
+}
+
+// Bitonic sort of up to 128 ints held two per lane (elements lane and
+// lane + 64), by shuffles: no LDS, no barriers.  n <= 64 sorts v0 only.
+__device__ __forceinline__ void wave_sort128(int &v0, int &v1, int n) {
+  const int lane = threadIdx.x & 63;
+  if (n <= 64) {
+    for (int size = 2; size <= 64; size <<= 1)
+      for (int stride = size >> 1; stride > 0; stride >>= 1) v0 = bitonic_step(v0, lane, stride, size);
     return;
   }
+  for (int size = 2; size <= 128; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      if (stride == 64) {  // the two values of one lane; size 128 sorts ascending
+        const int lo = min(v0, v1), hi = max(v0, v1);
+        v0 = lo;
+        v1 = hi;
+      } else {
+        v0 = bitonic_step(v0, lane, stride, size);
+        v1 = bitonic_step(v1, lane + 64, stride, size);
+      }
+    }
+}
+
+// Sorts each LJ cell's members ascending (so that lists, and every force
+// sum over them, do not depend on the order of the binning atomics), keeps
+// the sorted count and clears the binning count.  One wavefront per cell;
+// the replica's wavefronts stride over its cells.
+__global__ __launch_bounds__(kBlock) void k_sort_lj(const Rep *__restrict__ reps, WMap wm) {
+  const int r = find_rep(wm, blockIdx.x);
+  const Rep &S = reps[r];
+  Ctrl *c = S.ctrl;
+  if (!c->flag1) return;
+  const int lane = threadIdx.x & 63;
+  const int nw = (wm.blk[r + 1] - wm.blk[r]) * (kBlock / 64);
+  const int ncell = S.nc1[0] * S.nc1[1] * S.nc1[2];
+  int maxraw = 0;
+  for (int cell = (blockIdx.x - wm.blk[r]) * (kBlock / 64) + (threadIdx.x >> 6); cell < ncell;
+       cell += nw) {
+    const int raw = S.cnt1[cell];
+    if (raw == 0) {
+      if (lane == 0) S.scnt1[cell] = 0;
+      continue;
+    }
+    const int n = min(raw, S.ccap1);
+    int *m = S.mem1 + (size_t)cell * S.ccap1;
+    int v0 = lane < n ? m[lane] : 0x7fffffff;
+    int v1 = lane + 64 < n ? m[lane + 64] : 0x7fffffff;
+    if (n > 1) wave_sort128(v0, v1, n);
+    if (lane < n) m[lane] = v0;
+    if (lane + 64 < n) m[lane + 64] = v1;
+    if (lane == 0) {
+      S.scnt1[cell] = n;
+      S.cnt1[cell] = 0;
+    }
+    maxraw = max(maxraw, raw);
+  }
+  if (lane == 0 && maxraw) atomicMax(&c->max_cell1, maxraw);
+}
+
+// Shared 27-cell candidate set of one cell: offs[l] = first candidate of
+// stencil cell l (reference order), cids[l] = its index (-1 outside in z).
+// Wavefront 0 loads the 27 counts in parallel and scans them.
+__device__ __forceinline__ int stencil_offsets(int cell, const int nc[3], const int *scnt, int *offs,
+                                               int *cids) {
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    const int a = cell % nc[0], b = (cell / nc[0]) % nc[1], c = cell / (nc[0] * nc[1]);
+    int cnt = 0;
+    if (lane < 27) {
+      const int cl = stencil_cell(lane, a, b, c, nc);
+      cids[lane] = cl;
+      cnt = cl >= 0 ? scnt[cl] : 0;
+    }
+    int incl = cnt;
+    for (int o = 1; o < 32; o <<= 1) {
+      const int v = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += v;
+    }
+    if (lane < 27) offs[lane] = incl - cnt;
+    if (lane == 26) offs[27] = incl;
+  }
+  __syncthreads();
+  return offs[27];
+}
+
+__device__ __forceinline__ int stencil_slot(const int *offs, int t) {
+  int l = 0;  // largest l with offs[l] <= t
+  for (int k = 16; k > 0; k >>= 1)
+    if (l + k < 27 && offs[l + k] <= t) l += k;
+  return l;
+}
+
+constexpr int kLjTile = 1024;  // candidates per LDS tile of k_build_lj
+
+// Fill pass of lj_verlet_list (compt_acc.f90:271-313) cell by cell: the
+// members of a cell's 27 stencil cells (reference cell order, ascending
+// within a cell) are staged in LDS tiles shared by all targets of the cell;
+// each wavefront takes one target at a time, tests 64 candidates per
+// iteration and appends hits in order with a ballot.  Pairs: i != j and
+// r^2 < rvl^2, r^2 rounded as rij_and_rr rounds it.  The list is ELL: slot
+// k of particle i at k*NN + i.  The replica's workgroups stride over its
+// cells.  The last kernel of a step also advances the step counter.
+__global__ __launch_bounds__(kBlock) void k_build_lj(const Rep *__restrict__ reps, WMap wm,
+                                                     int inc_step) {
+  const int r = find_rep(wm, blockIdx.x);
+  const Rep &S = reps[r];
+  Ctrl *c = S.ctrl;
+  const int b0 = blockIdx.x - wm.blk[r], nb = wm.blk[r + 1] - wm.blk[r];
+  if (inc_step && b0 == 0 && threadIdx.x == 0) c->step++;
+  if (!c->flag1) return;
+  __shared__ int offs[28], cids[27];
+  __shared__ double cx[kLjTile], cy[kLjTile], cz[kLjTile];
+  __shared__ int cj[kLjTile];
+  __shared__ int tcount[128];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int cap = S.nbcap1;
+  const int ncell = S.nc1[0] * S.nc1[1] * S.nc1[2];
+  int maxn = 0;
+  for (int cell = b0; cell < ncell; cell += nb) {
+    const int nt = S.scnt1[cell];
+    if (nt == 0) continue;
+    const int total = stencil_offsets(cell, S.nc1, S.scnt1, offs, cids);
+    for (int k = threadIdx.x; k < nt; k += blockDim.x) tcount[k] = 0;
+    const int *tm = S.mem1 + (size_t)cell * S.ccap1;
+    for (int base = 0; base < total; base += kLjTile) {
+      const int nc = min(kLjTile, total - base);
+      __syncthreads();  // previous tile fully used, tcount zeroed
+      for (int k = threadIdx.x; k < nc; k += blockDim.x) {
+        const int t = base + k, l = stencil_slot(offs, t);
+        const int j = S.mem1[(size_t)cids[l] * S.ccap1 + (t - offs[l])];
+        const double4 pj = S.pos[j];
+        cx[k] = pj.x;
+        cy[k] = pj.y;
+        cz[k] = pj.z;
+        cj[k] = j;
+      }
+      __syncthreads();
+      for (int tg = w; tg < nt; tg += kBlock / 64) {
+        const int i = tm[tg];
+        const double4 pi = S.pos[i];
+        int n = tcount[tg];
+        for (int k0 = 0; k0 < nc; k0 += 64) {
+          const int k = k0 + lane;
+          bool hit = false;
+          int j = 0;
+          if (k < nc) {
+            j = cj[k];
+            double dx, dy, dz;
+            hit = j != i &&
+                  rsq_image(pi, make_double4(cx[k], cy[k], cz[k], 0.0), S.Lx, S.Ly, S.hLx, S.hLy, dx,
+                            dy, dz) < S.rv1sq;
+          }
+          const unsigned long long mask = __ballot(hit);
+          const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+          if (hit && n + rank < cap) S.nb1[(size_t)(n + rank) * S.NN + i] = j;
+          n += __popcll(mask);
+        }
+        if (lane == 0) tcount[tg] = n;
+      }
+    }
+    __syncthreads();
+    for (int tg = threadIdx.x; tg < nt; tg += blockDim.x) {
+      const int n = tcount[tg];
+      S.nbn1[tm[tg]] = n;
+      maxn = max(maxn, n);
+    }
+    __syncthreads();  // offs, cids and tcount are reused by the next cell
+  }
+  if (maxn) {
+    if (maxn > cap) atomicOr(&c->ovf, 4);
+    atomicMax(&c->max_nb1, maxn);
+  }
+}
+
+// The reference-form real-space list, real_verlet_list
+// (compt_acc.f90:491-626): cells of edge >= r_c + skin over the charges,
+// binned, sorted, then one thread per charge fills its full list.  Built on
+// request (md_build_lists); the step loop finds real-space pairs with the
+// cell sweep of k_real_cells instead.
+__global__ __launch_bounds__(kBlock) void k_bin_rl(const Rep *__restrict__ reps, WMap wm) {
+  const int r = find_rep(wm, blockIdx.x);
+  const Rep &S = reps[r];
+  Ctrl *c = S.ctrl;
   if (!c->flag2) return;
-  const int cell = bx - wm.blk[r];
-  const int n = min(S.cnt2[c->gen2 & 1][cell], S.ccap2);
-  int *m = S.mem2 + (size_t)cell * S.ccap2;
-  extern __shared__ int sbuf[];
+  const int t = (blockIdx.x - wm.blk[r]) * kBlock + threadIdx.x;
+  if (t >= S.Nq) return;
+  const int i = S.charge[t];
+  int a, b, cz;
+  if (!cell_of(S.pos[i], S.nc2, S.rcel2, S.hLx, S.hLy, a, b, cz)) {
+    set_error(c, MD_ECELL, i, -1);
+    return;
+  }
+  const int cell = (cz * S.nc2[1] + b) * S.nc2[0] + a;
+  int *cnt = S.cnt2[c->gen2 & 1];
+  const int slot = atomicAdd(&cnt[cell], 1);
+  if (slot < S.ccap2) S.mem2[(size_t)cell * S.ccap2 + slot] = t;
+  else atomicOr(&c->ovf, 2);
+}
+
+// Bitonic sort of one cell's members in LDS (one workgroup per cell).
+__device__ void sort_cell_lds(int *m, int n, int *sbuf) {
   int np = 1;
   while (np < n) np <<= 1;
   for (int k = threadIdx.x; k < np; k += blockDim.x) sbuf[k] = k < n ? m[k] : 0x7fffffff;
@@ -403,77 +574,55 @@ __global__ __launch_bounds__(kBlock) void k_cellsort(const Rep *__restrict__ rep
   for (int k = threadIdx.x; k < n; k += blockDim.x) m[k] = sbuf[k];
 }
 
-// Fill pass of lj_verlet_list (compt_acc.f90:271-313) and real_verlet_list
-// (compt_acc.f90:580-624): 27-cell stencil in the reference's order, x,y
-// periodic, z open; pairs with i != j and r^2 < rv^2 (r^2 rounded as
-// rij_and_rr rounds it).  Lists are ELL: slot k of item t at [k*n + t].
-__global__ __launch_bounds__(kBlock) void k_build(const Rep *__restrict__ reps, WMap wm1, WMap wm2,
-                                                  int inc_step) {
-  const bool lj = (int)blockIdx.x < wm1.total;
-  const int bx = lj ? blockIdx.x : blockIdx.x - wm1.total;
-  const WMap &wm = lj ? wm1 : wm2;
-  const int r = find_rep(wm, bx);
+__global__ __launch_bounds__(kBlock) void k_sort_rl(const Rep *__restrict__ reps, WMap wm) {
+  const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
   Ctrl *c = S.ctrl;
-  const int blk = bx - wm.blk[r];
-  // last kernel of a step: advance the step counter of graph-replayed steps
-  if (inc_step && lj && blk == 0 && threadIdx.x == 0) c->step++;
-  if (lj ? !c->flag1 : !c->flag2) return;
+  if (!c->flag2) return;
+  const int cell = blockIdx.x - wm.blk[r];
+  const int n = min(S.cnt2[c->gen2 & 1][cell], S.ccap2);
+  extern __shared__ int sbuf[];
+  sort_cell_lds(S.mem2 + (size_t)cell * S.ccap2, n, sbuf);
+}
+
+__global__ __launch_bounds__(kBlock) void k_build_rl(const Rep *__restrict__ reps, WMap wm) {
+  const int r = find_rep(wm, blockIdx.x);
+  const Rep &S = reps[r];
+  Ctrl *c = S.ctrl;
+  if (!c->flag2) return;
+  const int blk = blockIdx.x - wm.blk[r];
   const int t = blk * kBlock + threadIdx.x;
-  const int *nc = lj ? S.nc1 : S.nc2;
-  const int ncell = nc[0] * nc[1] * nc[2];
-  const int buf = (lj ? c->gen1 : c->gen2) & 1;
-  {  // clear the other count buffer for the next build of this list
-    int *other = lj ? S.cnt1[buf ^ 1] : S.cnt2[buf ^ 1];
-    const int nblk = (lj ? wm1.blk[r + 1] : wm2.blk[r + 1]) - wm.blk[r];
-    for (int k = blk * kBlock + threadIdx.x; k < ncell; k += nblk * kBlock) other[k] = 0;
+  const int ncell = S.nc2[0] * S.nc2[1] * S.nc2[2];
+  const int buf = c->gen2 & 1;
+  {  // clear the other count buffer for the next build
+    const int nblk = wm.blk[r + 1] - wm.blk[r];
+    for (int k = blk * kBlock + threadIdx.x; k < ncell; k += nblk * kBlock) S.cnt2[buf ^ 1][k] = 0;
   }
-  const int nitem = lj ? S.NN : S.Nq;
-  if (t >= nitem) return;
-  const int i = lj ? t : S.charge[t];
-  const double *rc = lj ? S.rcel1 : S.rcel2;
-  const double rvsq = lj ? S.rv1sq : S.rv2sq;
-  const int *cnt = lj ? S.cnt1[buf] : S.cnt2[buf];
-  const int *mem = lj ? S.mem1 : S.mem2;
-  const int ccap = lj ? S.ccap1 : S.ccap2;
-  const int cap = lj ? S.nbcap1 : S.nbcap2;
-  int *out = lj ? S.nb1 + t : S.nb2 + t;
+  if (t >= S.Nq) return;
+  const int i = S.charge[t];
+  const int *cnt = S.cnt2[buf];
   const double4 pi = S.pos[i];
   int a, b, cz;
-  if (!cell_of(pi, nc, rc, S.hLx, S.hLy, a, b, cz)) return;  // reported by k_bin
+  if (!cell_of(pi, S.nc2, S.rcel2, S.hLx, S.hLy, a, b, cz)) return;
   int n = 0;
-  for (int l = -1; l <= 1; l++) {
-    int p = a + l;
-    p = p >= nc[0] ? p - nc[0] : (p < 0 ? p + nc[0] : p);
-    for (int mm = -1; mm <= 1; mm++) {
-      int q = b + mm;
-      q = q >= nc[1] ? q - nc[1] : (q < 0 ? q + nc[1] : q);
-      for (int nn = -1; nn <= 1; nn++) {
-        const int z = cz + nn;
-        if (z >= nc[2] || z < 0) continue;
-        const int cell = (z * nc[1] + q) * nc[0] + p;
-        const int cn = min(cnt[cell], ccap);
-        const int *cm = mem + (size_t)cell * ccap;
-        for (int u = 0; u < cn; u++) {
-          const int tj = cm[u];
-          const int j = lj ? tj : S.charge[tj];
-          if (j == i) continue;
-          double dx, dy, dz;
-          const double rs = rsq_image(pi, S.pos[j], S.Lx, S.Ly, S.hLx, S.hLy, dx, dy, dz);
-          if (rs < rvsq) {
-            if (n < cap) out[(size_t)n * nitem] = j;
-            n++;
-          }
-        }
+  for (int l = 0; l < 27; l++) {
+    const int cell = stencil_cell(l, a, b, cz, S.nc2);
+    if (cell < 0) continue;
+    const int cn = min(cnt[cell], S.ccap2);
+    const int *cm = S.mem2 + (size_t)cell * S.ccap2;
+    for (int u = 0; u < cn; u++) {
+      const int j = S.charge[cm[u]];
+      if (j == i) continue;
+      double dx, dy, dz;
+      if (rsq_image(pi, S.pos[j], S.Lx, S.Ly, S.hLx, S.hLy, dx, dy, dz) < S.rv2sq) {
+        if (n < S.nbcap2) S.nb2[(size_t)n * S.Nq + t] = j;
+        n++;
       }
     }
   }
-  (lj ? S.nbn1 : S.nbn2)[t] = n;
-  if (n > cap) atomicOr(&c->ovf, lj ? 4 : 8);
-  atomicMax(lj ? &c->max_nb1 : &c->max_nb2, n);
-  int occ = 0;  // own cell occupancy, for capacity tracking
-  occ = cnt[(cz * nc[1] + b) * nc[0] + a];
-  atomicMax(lj ? &c->max_cell1 : &c->max_cell2, occ);
+  S.nbn2[t] = n;
+  if (n > S.nbcap2) atomicOr(&c->ovf, 8);
+  atomicMax(&c->max_nb2, n);
 }
 
 // ------------------------------------------------------------- SPME
@@ -533,36 +682,29 @@ __device__ __forceinline__ int wrapk(int v, int K) {
 
 // MapCharges (compt_acc.f90:1324-1365): charge spreading into a
 // fixed-point mesh (integer atomics: the sum is independent of order).
+// G lanes per charge: lane (b,c) adds the p points of mesh row (y_b, z_c).
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_spread(const Rep *__restrict__ reps, WMap wm) {
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
-  const int m = (blockIdx.x - wm.blk[r]) * kBlock + threadIdx.x;
-  if (m >= S.Nq) return;
-  const int i = S.charge[m];
-  const double4 p = S.pos[i];
-  const int o = S.order;
+  const int t = (blockIdx.x - wm.blk[r]) * kBlock + threadIdx.x;
+  const int G = S.G, o = S.order;
+  const int m = t / G, g = t - m * G;
+  if (m >= S.Nq || g >= o * o) return;
+  const int bb = g % o, cc = g / o;
+  const double4 p = S.pos[S.charge[m]];
   R wx[kMaxOrder], wy[kMaxOrder], wz[kMaxOrder], d[kMaxOrder];
   const int bx = spline<R>(p.x, S.kg[0], o, wx, d);
   const int by = spline<R>(p.y, S.kg[1], o, wy, d);
   const int bz = spline<R>(p.z, S.kg[2], o, wz, d);
   const R q = (R)p.w;
-  for (int a = 0; a < o; a++) wx[a] *= q;
   const int K1 = S.K[0], K2 = S.K[1], K3 = S.K[2];
-  int ix[kMaxOrder];
-  for (int a = 0; a < o; a++) ix[a] = wrapk(bx + a, K1);
-  for (int c = 0; c < o; c++) {
-    const int iz = wrapk(bz + c, K3);
-    for (int b = 0; b < o; b++) {
-      const int iy = wrapk(by + b, K2);
-      unsigned long long *row = S.meshq + ((size_t)iz * K2 + iy) * K1;
-      const R wzy = wz[c];
-      for (int a = 0; a < o; a++) {
-        const double v = (double)(wzy * (wx[a] * wy[b]));
-        const long long f = (long long)rint(v * kQScale);
-        atomicAdd(row + ix[a], (unsigned long long)f);
-      }
-    }
+  unsigned long long *row =
+      S.meshq + ((size_t)wrapk(bz + cc, K3) * K2 + wrapk(by + bb, K2)) * K1;
+  const R wzy = wz[cc];
+  for (int a = 0; a < o; a++) {
+    const double v = (double)(wzy * ((wx[a] * q) * wy[bb]));
+    atomicAdd(row + wrapk(bx + a, K1), (unsigned long long)(long long)rint(v * kQScale));
   }
 }
 
@@ -767,50 +909,48 @@ __global__ __launch_bounds__(kBlock) void k_fftz_conv(const Rep *__restrict__ re
 }
 
 // IFrc (compt_acc.f90:1402-1467): reciprocal force of each charge from the
-// potential mesh, plus per-block sums of the force (for ZeroForce,
-// compt_acc.f90:1470-1492) and of q z lb/Beta (for the slab dipole M_z,
-// compt_acc.f90:1041-1044), all in double.
+// potential mesh, G lanes per charge (lane (b,c) reads mesh row (y_b, z_c)),
+// reduced across the group; plus per-block sums of the force (for
+// ZeroForce, compt_acc.f90:1470-1492) and of q z lb/Beta (the slab dipole
+// M_z, compt_acc.f90:1041-1044), all in double.
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_gather(const Rep *__restrict__ reps, WMap wm) {
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
   const int blk = blockIdx.x - wm.blk[r];
-  const int m = blk * kBlock + threadIdx.x;
-  double f[4] = {0, 0, 0, 0};
-  if (m < S.Nq) {
-    const int i = S.charge[m];
-    const double4 p = S.pos[i];
-    const int o = S.order;
+  const int t = blk * kBlock + threadIdx.x;
+  const int G = S.G, o = S.order;
+  const int m = t / G, g = t - m * G;
+  const bool live = m < S.Nq;
+  R f1 = 0, f2 = 0, f3 = 0;
+  double4 p = make_double4(0, 0, 0, 0);
+  if (live) p = S.pos[S.charge[m]];
+  if (live && g < o * o) {
+    const int bb = g % o, cc = g / o;
     R wx[kMaxOrder], wy[kMaxOrder], wz[kMaxOrder], dx[kMaxOrder], dy[kMaxOrder], dz[kMaxOrder];
     const int bx = spline<R>(p.x, S.kg[0], o, wx, dx);
     const int by = spline<R>(p.y, S.kg[1], o, wy, dy);
     const int bz = spline<R>(p.z, S.kg[2], o, wz, dz);
     const R q = (R)p.w;
-    for (int a = 0; a < o; a++) {
-      wx[a] *= q;
-      dx[a] *= q;
-    }
     const int K1 = S.K[0], K2 = S.K[1], K3 = S.K[2];
-    const R *u = (const R *)S.meshu;
-    int ix[kMaxOrder];
-    for (int a = 0; a < o; a++) ix[a] = wrapk(bx + a, K1);
-    R f1 = 0, f2 = 0, f3 = 0;
-    for (int c = 0; c < o; c++) {
-      const int iz = wrapk(bz + c, K3);
-      for (int b = 0; b < o; b++) {
-        const int iy = wrapk(by + b, K2);
-        const R *row = u + ((size_t)iz * K2 + iy) * K1;
-        R s1 = 0, s2 = 0;
-        for (int a = 0; a < o; a++) {
-          const R v = row[ix[a]];
-          s1 += dx[a] * v;
-          s2 += wx[a] * v;
-        }
-        f1 += wz[c] * wy[b] * s1;
-        f2 += wz[c] * dy[b] * s2;
-        f3 += dz[c] * wy[b] * s2;
-      }
+    const R *row = (const R *)S.meshu + ((size_t)wrapk(bz + cc, K3) * K2 + wrapk(by + bb, K2)) * K1;
+    R s1 = 0, s2 = 0;
+    for (int a = 0; a < o; a++) {
+      const R v = row[wrapk(bx + a, K1)];
+      s1 += (dx[a] * q) * v;
+      s2 += (wx[a] * q) * v;
     }
+    f1 = wz[cc] * wy[bb] * s1;
+    f2 = wz[cc] * dy[bb] * s2;
+    f3 = dz[cc] * wy[bb] * s2;
+  }
+  for (int off = G >> 1; off > 0; off >>= 1) {  // groups are aligned, G divides 64
+    f1 += __shfl_xor(f1, off, 64);
+    f2 += __shfl_xor(f2, off, 64);
+    f3 += __shfl_xor(f3, off, 64);
+  }
+  double f[4] = {0, 0, 0, 0};
+  if (live && g == 0) {
     f[0] = -(double)f1 * S.fnorm[0];
     f[1] = -(double)f2 * S.fnorm[1];
     f[2] = -(double)f3 * S.fnorm[2];
@@ -832,80 +972,22 @@ __global__ __launch_bounds__(kBlock) void k_gather(const Rep *__restrict__ reps,
   }
 }
 
-// real_space (compt_acc.f90:980-1049) and the assembly of acc_c: the
-// erfc-screened pair force over the real-space list (all charge pairs when
-// real_verlet == 0), the reciprocal force with its mean removed and scaled
-// by lb/Beta (compt_acc.f90:1242-1244), and the slab correction
-// (compt_acc.f90:1045-1048).  With out != null writes acc_c(NN,3) there.
+// acc_c of one charge: the reciprocal force with its mean removed, scaled by
+// lb/Beta (compt_acc.f90:1242-1244), plus the real-space pair force (scaled
+// here), minus the slab correction (compt_acc.f90:1045-1048).
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_real(const Rep *__restrict__ reps, WMap wm,
-                                                 double *__restrict__ out) {
-  const int r = find_rep(wm, blockIdx.x);
-  const Rep &S = reps[r];
-  const int m = (blockIdx.x - wm.blk[r]) * kBlock + threadIdx.x;
-  __shared__ double tot[4];
-  if (threadIdx.x < 4) {  // same fixed-order sum in every block
-    double s = 0;
-    for (int b = 0; b < S.nqblk; b++) s += S.part[(size_t)b * 4 + threadIdx.x];
-    tot[threadIdx.x] = s;
-  }
-  __syncthreads();
-  if (m >= S.Nq) return;
-  const int i = S.charge[m];
-  const double4 pi = S.pos[i];
-  R fx = 0, fy = 0, fz = 0;
-  const R alpha = (R)S.alpha, alpha2 = (R)S.alpha2, c2 = (R)S.c2, rcc2 = (R)S.rcc2;
-  const R hLx = (R)S.hLx, hLy = (R)S.hLy, Lx = (R)S.Lx, Ly = (R)S.Ly;
-  R xi, yi, zi, qi;
-  if constexpr (sizeof(R) == 4) {
-    const float4 p = S.pos32[i];
-    xi = p.x; yi = p.y; zi = p.z; qi = p.w;
-  } else {
-    xi = pi.x; yi = pi.y; zi = pi.z; qi = pi.w;
-  }
-  auto pair = [&](int j) {
-    R xj, yj, zj, qj;
-    if constexpr (sizeof(R) == 4) {
-      const float4 p = S.pos32[j];
-      xj = p.x; yj = p.y; zj = p.z; qj = p.w;
-    } else {
-      const double4 p = S.pos[j];
-      xj = p.x; yj = p.y; zj = p.z; qj = p.w;
-    }
-    R dx = xi - xj, dy = yi - yj, dz = zi - zj;
-    if (dx > hLx) dx -= Lx; else if (dx <= -hLx) dx += Lx;
-    if (dy > hLy) dy -= Ly; else if (dy <= -hLy) dy += Ly;
-    const R rsq = dx * dx + dy * dy + dz * dz;
-    if (rsq < rcc2) {
-      const R rr = sqrt(rsq);
-      const R g = qi * qj / rsq * (erfc(alpha * rr) / rr + c2 * exp(-alpha2 * rsq));
-      fx += g * dx;
-      fy += g * dy;
-      fz += g * dz;
-    }
-  };
-  if (S.real_verlet) {
-    const int n = min(S.nbn2[m], S.nbcap2);
-    const int *nb = S.nb2 + m;
-    for (int k = 0; k < n; k++) pair(nb[(size_t)k * S.Nq]);
-  } else {
-    for (int k = 0; k < S.Nq; k++)
-      if (k != m) pair(S.charge[k]);
-  }
-  const double c1 = S.c1;
-  const double inv = 1.0 / S.Nq;
-  const double Mz = tot[3];
+__device__ __forceinline__ void store_coulomb(const Rep &S, int m, int i, double q, const double *tot,
+                                              R fx, R fy, R fz, double *out) {
+  const double c1 = S.c1, inv = 1.0 / S.Nq;
   const double ax = (S.fsp[m] - tot[0] * inv) * c1 + (double)fx * c1;
   const double ay = (S.fsp[(size_t)S.Nq + m] - tot[1] * inv) * c1 + (double)fy * c1;
   const double az = (S.fsp[2 * (size_t)S.Nq + m] - tot[2] * inv) * c1 + (double)fz * c1 -
-                    S.slab * pi.w * Mz;
+                    S.slab * q * tot[3];
   if (out) {
     out[i] = ax;
     out[(size_t)S.NN + i] = ay;
     out[2 * (size_t)S.NN + i] = az;
-    return;
-  }
-  if constexpr (sizeof(R) == 4) {
+  } else if constexpr (sizeof(R) == 4) {
     ((float *)S.ccx)[i] = (float)ax;
     ((float *)S.ccy)[i] = (float)ay;
     ((float *)S.ccz)[i] = (float)az;
@@ -914,6 +996,212 @@ __global__ __launch_bounds__(kBlock) void k_real(const Rep *__restrict__ reps, W
     ((double *)S.ccy)[i] = ay;
     ((double *)S.ccz)[i] = az;
   }
+}
+
+// Sum of the gather kernel's block partials (mean reciprocal force, 3, and
+// M_z): each thread sums a fixed strided subset, then a fixed tree over the
+// block, so every block gets the same bits.
+__device__ __forceinline__ void coulomb_totals(const Rep &S, double *tot) {
+  __shared__ double red[4][kBlock / 64];
+  double v[4] = {0, 0, 0, 0};
+  for (int b = threadIdx.x; b < S.nqblk; b += kBlock)
+    for (int d = 0; d < 4; d++) v[d] += S.part[(size_t)b * 4 + d];
+  for (int d = 0; d < 4; d++) {
+    const double x = wave_sum(v[d]);
+    if ((threadIdx.x & 63) == 0) red[d][threadIdx.x >> 6] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double s = 0;
+    for (int w = 0; w < kBlock / 64; w++) s += red[threadIdx.x][w];
+    tot[threadIdx.x] = s;
+  }
+  __syncthreads();
+}
+
+// erfc-screened Coulomb pair force of real_space (compt_acc.f90:1034-1036)
+// without the lb/Beta factor: q_i q_j / r^2 (erfc(a r)/r + 2a/sqrt(pi)
+// exp(-a^2 r^2)) r_ij, for r < r_c.
+template <typename R>
+__device__ __forceinline__ void real_pair(R xi, R yi, R zi, R qi, R xj, R yj, R zj, R qj, R Lx, R Ly,
+                                          R hLx, R hLy, R rcc2, R alpha, R alpha2, R c2, R &fx, R &fy,
+                                          R &fz) {
+  R dx = xi - xj, dy = yi - yj, dz = zi - zj;
+  if (dx > hLx) dx -= Lx; else if (dx <= -hLx) dx += Lx;
+  if (dy > hLy) dy -= Ly; else if (dy <= -hLy) dy += Ly;
+  const R rsq = dx * dx + dy * dy + dz * dz;
+  if (rsq < rcc2) {
+    const R rr = sqrt(rsq);
+    const R g = qi * qj / rsq * (erfc(alpha * rr) / rr + c2 * exp(-alpha2 * rsq));
+    fx += g * dx;
+    fy += g * dy;
+    fz += g * dz;
+  }
+}
+
+// Charges into the cells of the real-space sweep (edge >= r_c).
+__global__ __launch_bounds__(kBlock) void k_bin_q(const Rep *__restrict__ reps, WMap wm) {
+  const int r = find_rep(wm, blockIdx.x);
+  const Rep &S = reps[r];
+  if (!S.real_cells) return;
+  const int m = (blockIdx.x - wm.blk[r]) * kBlock + threadIdx.x;
+  if (m >= S.Nq) return;
+  const int i = S.charge[m];
+  int a, b, c;
+  if (!cell_of(S.pos[i], S.nc3, S.rcel3, S.hLx, S.hLy, a, b, c)) {
+    set_error(S.ctrl, MD_ECELL, i, -1);
+    return;
+  }
+  const int cell = (c * S.nc3[1] + b) * S.nc3[0] + a;
+  const int slot = atomicAdd(&S.cnt3[cell], 1);
+  if (slot < S.ccap3) S.mem3[(size_t)cell * S.ccap3 + slot] = m;
+  else atomicOr(&S.ctrl->ovf, 16);
+}
+
+// One workgroup per cell: sort the members, keep the sorted count, clear
+// the binning count.
+__global__ __launch_bounds__(kBlock) void k_sort_q(const Rep *__restrict__ reps, WMap wm) {
+  const int r = find_rep(wm, blockIdx.x);
+  const Rep &S = reps[r];
+  const int cell = blockIdx.x - wm.blk[r];
+  const int raw = S.cnt3[cell];
+  const int n = min(raw, S.ccap3);
+  extern __shared__ int sbuf[];
+  sort_cell_lds(S.mem3 + (size_t)cell * S.ccap3, n, sbuf);
+  if (threadIdx.x == 0) {
+    S.scnt3[cell] = n;
+    S.cnt3[cell] = 0;
+    if (raw) atomicMax(&S.ctrl->max_cell2, raw);
+  }
+}
+
+// Candidates staged in LDS per pass of k_real_cells.
+template <typename R> struct RealTile { static constexpr int N = 1024; };
+template <> struct RealTile<float> { static constexpr int N = 2048; };
+
+// real_space (compt_acc.f90:980-1049) by a cell sweep: one workgroup per
+// cell of edge >= r_c; the members of its 27 neighbour cells (reference
+// stencil order, ascending within a cell) are staged in LDS, each wavefront
+// takes one target charge of the cell at a time, its 64 lanes stride over
+// the candidates, and a shuffle reduction (fixed order) sums the lanes.
+// Same pair set as the full real-space list: every j != i with r < r_c.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_real_cells(const Rep *__restrict__ reps, WMap wm,
+                                                       double *__restrict__ out) {
+  using V = typename std::conditional<sizeof(R) == 4, float4, double4>::type;
+  constexpr int TN = RealTile<R>::N;
+  const int r = find_rep(wm, blockIdx.x);
+  const Rep &S = reps[r];
+  const int b0 = blockIdx.x - wm.blk[r], nb = wm.blk[r + 1] - wm.blk[r];
+  // work items (cell, chunk of kRealChunk targets); workgroups stride over them
+  const int nch = (S.ccap3 + kRealChunk - 1) / kRealChunk;
+  const int nitem = S.nc3[0] * S.nc3[1] * S.nc3[2] * nch;
+  __shared__ double tot[4];
+  __shared__ V cand[TN];
+  __shared__ int cidx[TN];
+  __shared__ int offs[28], cids[27];
+  coulomb_totals(S, tot);
+  const R Lx = (R)S.Lx, Ly = (R)S.Ly, hLx = (R)S.hLx, hLy = (R)S.hLy, rcc2 = (R)S.rcc2;
+  const R alpha = (R)S.alpha, alpha2 = (R)S.alpha2, c2 = (R)S.c2;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int item = b0; item < nitem; item += nb) {
+    const int cell = item / nch, t0 = (item - cell * nch) * kRealChunk;
+    const int nt = min(S.scnt3[cell] - t0, kRealChunk);
+    if (nt <= 0) continue;
+    const int total = stencil_offsets(cell, S.nc3, S.scnt3, offs, cids);
+    const int *tm = S.mem3 + (size_t)cell * S.ccap3 + t0;
+    for (int base = 0; base < total; base += TN) {
+      const int nc = min(TN, total - base);
+      for (int k = threadIdx.x; k < nc; k += blockDim.x) {
+        const int t = base + k, l = stencil_slot(offs, t);
+        const int mm = S.mem3[(size_t)cids[l] * S.ccap3 + (t - offs[l])];
+        const int j = S.charge[mm];
+        V v;
+        if constexpr (sizeof(R) == 4) v = S.pos32[j];
+        else v = S.pos[j];
+        cand[k] = v;
+        cidx[k] = j;
+      }
+      __syncthreads();
+      for (int tg = w; tg < nt; tg += kBlock / 64) {
+        const int m = tm[tg];
+        const int i = S.charge[m];
+        V pi;
+        if constexpr (sizeof(R) == 4) pi = S.pos32[i];
+        else pi = S.pos[i];
+        R fx = 0, fy = 0, fz = 0;
+        for (int k = lane; k < nc; k += 64) {
+          if (cidx[k] == i) continue;
+          const V pj = cand[k];
+          real_pair<R>(pi.x, pi.y, pi.z, pi.w, pj.x, pj.y, pj.z, pj.w, Lx, Ly, hLx, hLy, rcc2, alpha,
+                       alpha2, c2, fx, fy, fz);
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+          fx += __shfl_xor(fx, off, 64);
+          fy += __shfl_xor(fy, off, 64);
+          fz += __shfl_xor(fz, off, 64);
+        }
+        if (lane == 0) {
+          if (total <= TN) {
+            store_coulomb<R>(S, m, i, S.pos[i].w, tot, fx, fy, fz, out);
+          } else {
+            // several tiles: the same wave owns this target in every tile; the
+            // first tile writes its sum, later tiles add, the last one stores
+            double *acc = S.rsum + 3 * (size_t)m;
+            if (base == 0) {
+              acc[0] = fx; acc[1] = fy; acc[2] = fz;
+            } else {
+              acc[0] += fx; acc[1] += fy; acc[2] += fz;
+            }
+            if (base + TN >= total)
+              store_coulomb<R>(S, m, i, S.pos[i].w, tot, (R)acc[0], (R)acc[1], (R)acc[2], out);
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// real_space over all charge pairs (the reference's real_verlet == 0 loop,
+// compt_acc.f90:997-1010, and boxes too small for three cells of edge r_c),
+// one thread per charge.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_real(const Rep *__restrict__ reps, WMap wm,
+                                                 double *__restrict__ out) {
+  const int r = find_rep(wm, blockIdx.x);
+  const Rep &S = reps[r];
+  if (S.real_cells) return;
+  const int m = (blockIdx.x - wm.blk[r]) * kBlock + threadIdx.x;
+  __shared__ double tot[4];
+  coulomb_totals(S, tot);
+  if (m >= S.Nq) return;
+  const int i = S.charge[m];
+  const double4 pd = S.pos[i];
+  const R alpha = (R)S.alpha, alpha2 = (R)S.alpha2, c2 = (R)S.c2, rcc2 = (R)S.rcc2;
+  const R hLx = (R)S.hLx, hLy = (R)S.hLy, Lx = (R)S.Lx, Ly = (R)S.Ly;
+  R xi, yi, zi, qi;
+  if constexpr (sizeof(R) == 4) {
+    const float4 p = S.pos32[i];
+    xi = p.x; yi = p.y; zi = p.z; qi = p.w;
+  } else {
+    xi = pd.x; yi = pd.y; zi = pd.z; qi = pd.w;
+  }
+  R fx = 0, fy = 0, fz = 0;
+  for (int k = 0; k < S.Nq; k++) {
+    if (k == m) continue;
+    const int j = S.charge[k];
+    R xj, yj, zj, qj;
+    if constexpr (sizeof(R) == 4) {
+      const float4 p = S.pos32[j];
+      xj = p.x; yj = p.y; zj = p.z; qj = p.w;
+    } else {
+      const double4 p = S.pos[j];
+      xj = p.x; yj = p.y; zj = p.z; qj = p.w;
+    }
+    real_pair<R>(xi, yi, zi, qi, xj, yj, zj, qj, Lx, Ly, hLx, hLy, rcc2, alpha, alpha2, c2, fx, fy, fz);
+  }
+  store_coulomb<R>(S, m, i, pd.w, tot, fx, fy, fz, out);
 }
 
 // ---------------------------------------------------- rescale_velocity
@@ -1042,26 +1330,39 @@ void launch_observe(const Plan &P, hipStream_t st, int *hist, double *hsum) {
 // (kernel timing); never used under graph capture.
 template <typename R>
 void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) {
-  auto mark = [&](int k) {
-    if (ev) (void)hipEventRecord(ev[k], st);
+  int k = 0;
+  auto mark = [&]() {
+    if (ev) (void)hipEventRecord(ev[k++], st);
   };
-  mark(0);
-  hipLaunchKernelGGL(k_spread<R>, dim3(P.chg.total), dim3(kBlock), 0, st, P.reps, P.chg);
-  mark(1);
+  mark();
+  if (P.cell3.total) {
+    hipLaunchKernelGGL(k_bin_q, dim3(P.chg.total), dim3(kBlock), 0, st, P.reps, P.chg);
+    mark();
+    hipLaunchKernelGGL(k_sort_q, dim3(P.cell3.total), dim3(kBlock), P.lds_sort3, st, P.reps, P.cell3);
+    mark();
+  } else if (ev) {
+    mark();
+    mark();
+  }
+  hipLaunchKernelGGL(k_spread<R>, dim3(P.chgG.total), dim3(kBlock), 0, st, P.reps, P.chgG);
+  mark();
   hipLaunchKernelGGL(k_fftx_fwd<R>, dim3(P.fx.total), dim3(kBlock), P.lds_x, st, P.reps, P.fx);
-  mark(2);
+  mark();
   hipLaunchKernelGGL(k_ffty<R>, dim3(P.fy.total), dim3(kBlock), P.lds_y, st, P.reps, P.fy, -1);
-  mark(3);
+  mark();
   hipLaunchKernelGGL(k_fftz_conv<R>, dim3(P.fz.total), dim3(kBlock), P.lds_z, st, P.reps, P.fz);
-  mark(4);
+  mark();
   hipLaunchKernelGGL(k_ffty<R>, dim3(P.fy.total), dim3(kBlock), P.lds_y, st, P.reps, P.fy, +1);
-  mark(5);
+  mark();
   hipLaunchKernelGGL(k_fftx_inv<R>, dim3(P.fx.total), dim3(kBlock), P.lds_x, st, P.reps, P.fx);
-  mark(6);
-  hipLaunchKernelGGL(k_gather<R>, dim3(P.chg.total), dim3(kBlock), 0, st, P.reps, P.chg);
-  mark(7);
-  hipLaunchKernelGGL(k_real<R>, dim3(P.chg.total), dim3(kBlock), 0, st, P.reps, P.chg, out);
-  mark(8);
+  mark();
+  hipLaunchKernelGGL(k_gather<R>, dim3(P.chgG.total), dim3(kBlock), 0, st, P.reps, P.chgG);
+  mark();
+  if (P.cell3.total)
+    hipLaunchKernelGGL(k_real_cells<R>, dim3(P.cell3c.total), dim3(kBlock), 0, st, P.reps, P.cell3c, out);
+  if (P.pairs_any)
+    hipLaunchKernelGGL(k_real<R>, dim3(P.chg.total), dim3(kBlock), 0, st, P.reps, P.chg, out);
+  mark();
 }
 
 template <typename R>
@@ -1075,16 +1376,24 @@ void launch_kick_drift(const Plan &P, hipStream_t st) {
   hipLaunchKernelGGL(k_kick_drift, dim3(P.part.total), dim3(kBlock), 0, st, P.reps, P.part);
 }
 
+// update_verlet_list's LJ rebuild (compt_acc.f90:717-720); every kernel
+// returns at once for replicas whose rebuild flag is clear.
 void launch_lists(const Plan &P, hipStream_t st, int inc_step, hipEvent_t *ev) {
-  const int nb = P.part.total + P.chg.total;
   if (ev) (void)hipEventRecord(ev[0], st);
-  hipLaunchKernelGGL(k_bin, dim3(nb), dim3(kBlock), 0, st, P.reps, P.part, P.chg);
+  hipLaunchKernelGGL(k_bin_lj, dim3(P.part.total), dim3(kBlock), 0, st, P.reps, P.part);
   if (ev) (void)hipEventRecord(ev[1], st);
-  hipLaunchKernelGGL(k_cellsort, dim3(P.cell1.total + P.cell2.total), dim3(kBlock), P.lds_sort, st,
-                     P.reps, P.cell1, P.cell2);
+  hipLaunchKernelGGL(k_sort_lj, dim3(P.cell1w.total), dim3(kBlock), 0, st, P.reps, P.cell1w);
   if (ev) (void)hipEventRecord(ev[2], st);
-  hipLaunchKernelGGL(k_build, dim3(nb), dim3(kBlock), 0, st, P.reps, P.part, P.chg, inc_step);
+  hipLaunchKernelGGL(k_build_lj, dim3(P.cell1.total), dim3(kBlock), 0, st, P.reps, P.cell1, inc_step);
   if (ev) (void)hipEventRecord(ev[3], st);
+}
+
+// The reference-form real-space list (real_verlet_list), on request.
+void launch_real_list(const Plan &P, hipStream_t st) {
+  if (!P.cell2.total) return;
+  hipLaunchKernelGGL(k_bin_rl, dim3(P.chg.total), dim3(kBlock), 0, st, P.reps, P.chg);
+  hipLaunchKernelGGL(k_sort_rl, dim3(P.cell2.total), dim3(kBlock), P.lds_sort, st, P.reps, P.cell2);
+  hipLaunchKernelGGL(k_build_rl, dim3(P.chg.total), dim3(kBlock), 0, st, P.reps, P.chg);
 }
 
 void launch_set_flags(const Plan &P, hipStream_t st, int clear) {

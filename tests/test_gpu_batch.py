@@ -1,0 +1,115 @@
+"""Replica batches and the serving loop on the GPU: a replica's result is the
+same alone and in a mix (bitwise: no cross-replica reductions and
+order-independent kernels), a batch replica equals the single-system
+engine, observables match the host-side definitions, and a failing replica
+is reported without disturbing the others."""
+import numpy as np
+import pytest
+
+from md_brushes_amd import MD_FP32, MD_FP64, MD_NOISE_PHILOX, Batch, Engine, initialize_position, initialize_velocity
+from md_brushes_amd.configs import replica
+from md_brushes_amd.serve import Server, make_stream, run_alone
+
+pytestmark = pytest.mark.gpu
+
+
+def _req(side, nb, arch, val, ez, seed, prec=MD_FP32):
+    p = replica(side, nb, arch, val, ez, seed, prec)
+    return p, initialize_position(p, seed), initialize_velocity(p, seed + 1)
+
+
+def test_alone_equals_mixed():
+    print("[batch] alone vs mixed", flush=True)
+    target = _req(3, 40, 3, 3, 0.4, 11)
+    others = [_req(2, 25, 1, 1, 0.9, 12), _req(4, 60, 4, 3, 0.1, 13), _req(2, 80, 1, 3, 0.5, 14)]
+    b = Batch(MD_FP32)
+    s_alone = b.add(*target)
+    b.run(12)
+    pa = b.download(s_alone)[0]
+    b.close()
+    b = Batch(MD_FP32)
+    b.add(*others[0])
+    b.run(3)
+    s_mix = b.add(*target)  # joins while another replica is mid-run
+    b.add(*others[1])
+    b.run(5)
+    b.add(*others[2])
+    b.run(7)
+    pm = b.download(s_mix)[0]
+    b.close()
+    assert np.array_equal(pa, pm)
+
+
+def test_batch_replica_equals_single_engine():
+    print("[batch] vs single engine fp64", flush=True)
+    p, pos, vel = _req(3, 30, 1, 3, 0.3, 21, MD_FP64)
+    e = Engine(p)
+    e.upload(pos, vel)
+    e.build_lists()
+    e.compute_force(0, MD_NOISE_PHILOX)
+    e.run(1, 80)
+    pe, ve, _, _ = e.download()
+    e.close()
+    b = Batch(MD_FP64)
+    sl = b.add(p, pos, vel)
+    b.add(*_req(2, 20, 4, 1, 0.7, 22, MD_FP64))
+    b.run(8)
+    pb, vb, _ = b.download(sl)
+    rc, step, _ = b.status(sl)
+    b.close()
+    assert rc == 0 and step == 81
+    assert np.array_equal(pe, pb) and np.array_equal(ve, vb)
+
+
+def test_observables_match_definitions():
+    print("[batch] observables", flush=True)
+    p, pos, vel = _req(4, 50, 1, 1, 0.2, 31)
+    b = Batch(MD_FP32)
+    sl = b.add(p, pos, vel)
+    b.run(5)
+    h, prof = b.observe(sl)
+    ps = b.download(sl)[0]
+    b.close()
+    z = ps[: p.Npe, 2]
+    assert abs(h - z.mean()) < 1e-12 * max(1.0, abs(h))
+    k = np.ceil(z / (p.Lz / 500)).astype(int)
+    ref = np.bincount(k[(k >= 1) & (k <= 500)] - 1, minlength=500)
+    assert np.array_equal(prof, ref)
+
+
+def test_failing_replica_is_isolated():
+    print("[batch] failing replica", flush=True)
+    good = _req(3, 30, 1, 3, 0.3, 41)
+    p, pos, vel = _req(2, 30, 1, 1, 0.3, 42)
+    bad = np.array(pos)
+    bad[3, 2] += 3.0  # a bond past R0
+    b = Batch(MD_FP32)
+    sg = b.add(*good)
+    with pytest.raises(Exception):
+        b.add(p, bad, vel)  # its step-0 force reports the break
+    sg2 = b.add(*_req(2, 20, 1, 1, 0.6, 43))
+    b.run(4)
+    assert b.status(sg)[0] == 0 and b.status(sg2)[0] == 0
+    ref = run_alone(_Fake(good, 40))[2]
+    assert np.array_equal(b.download(sg)[0], ref)
+    b.close()
+
+
+class _Fake:
+    def __init__(self, r, n):
+        self.params, self.pos, self.vel = r
+        self.nsteps = n
+
+
+def test_serving_stream_results_match_alone():
+    print("[serve] small stream", flush=True)
+    reqs = make_stream(8, 200, rate=400.0, seed=3, max_side=3)
+    srv = Server(chunk_cycles=4)
+    done, wall = srv.serve(reqs)
+    assert len(done) == 8 and all(r.result["status"] == 0 for r in done)
+    for r in done[:3]:
+        h, prof, _, _ = run_alone(r)
+        assert h == r.result["height"]
+        assert np.array_equal(prof, r.result["profile"])
+    lat = [r.t_done - r.arrival for r in done]
+    assert min(lat) > 0

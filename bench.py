@@ -44,11 +44,12 @@ def kernel_bytes(p, name):
     if name == "real":
         return Nq * (4 + 32 + (16 if f32 else 0) + 24 + 4 + 3 * r)
     if name == "spread":
-        return Nq * (4 + 32) + 8 * M
+        # bin members, charge index, position of each charge; mesh written once
+        return Nq * (4 + 4 + 32) + r * M
     if name == "gather":
         return Nq * (4 + 32 + 24) + r * M
     if name in ("fft_x_fwd",):
-        return 8 * M + 2 * r * Mh
+        return r * M + 2 * r * Mh
     if name in ("fft_x_inv",):
         return 2 * r * Mh + r * M
     if name in ("fft_y_fwd", "fft_y_inv"):
@@ -125,6 +126,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--serve", type=int, default=64, help="requests in the serving leg (0: skip)")
+    ap.add_argument("--serve-steps", type=int, default=2000)
+    ap.add_argument("--serve-rate", type=float, default=256.0)
+    ap.add_argument("--serve-seq", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (replicas shard "
@@ -183,6 +188,23 @@ def main():
                        "traffic": None, "bytes_per_launch": b, "avg_launch_us": round(ms[dom] * 1e3, 3)}
     out["kernel_us"] = {k: round(v * 1e3, 3) for k, v in ms.items()}
     e32.close()
+
+    if args.serve > 0:
+        from md_brushes_amd.serve import Server, make_stream, sequential_fp64, summarize
+
+        reqs = make_stream(args.serve, args.serve_steps, rate=args.serve_rate, seed=5)
+        done, wall = Server().serve(reqs)
+        sv = summarize(done, wall)
+        nseq = min(len(reqs), args.serve_seq)
+        sps_seq, _ = sequential_fp64(reqs[:nseq])
+        sv["sequential_fp64_steps_per_s"] = round(sps_seq, 1)
+        sv["speedup_vs_sequential"] = round(sv["aggregate_steps_per_s"] / sps_seq, 2)
+        sv["stream"] = (f"{args.serve} requests, Poisson rate {args.serve_rate}/s, {args.serve_steps} steps "
+                        f"each, chains 2x2..8x8, N 20..100, linear/3-/4-arm stars, salt valence 1/3, "
+                        f"Ez in [0,1], FP32 forces; sequential comparator on the first {nseq}")
+        out["serving"] = sv
+        out["replica_latency_p50_s"] = sv["latency_p50_s"]
+        out["replica_latency_p99_s"] = sv["latency_p99_s"]
 
     if not args.no_cpu:
         sps_cpu, n, el = cpu_baseline(state, args.seed, args.cpu_seconds)

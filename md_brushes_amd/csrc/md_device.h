@@ -85,7 +85,13 @@ struct Rep {
   int G;                           // lanes per charge in spread and gather
   int *nbn1, *nb1;                 // LJ list: length per particle, ELL slots
   int *nbn2, *nb2;                 // real list: per charge slot m
-  unsigned long long *meshq;       // fixed-point spread mesh K3*K2*K1
+  void *meshr;                     // charge mesh Q (real type of the path) K3*K2*K1
+  // charges binned by the mesh brick of their first spline point
+  int bsz[3], nbr[3];              // brick edge (mesh points) and bricks per axis
+  int capb;                        // charges per bin
+  int *bcnt, *bmem;
+  int plane_fused;                 // x and y FFT passes fused per z-plane; the
+                                   // spectrum is then kept in bit-reversed y order
   void *spec;                      // half spectrum K3*K2*Kh complex
   void *meshu;                     // real potential mesh K3*K2*K1
   void *bc;                        // influence function K3*K2*Kh real

@@ -28,7 +28,10 @@ using mdk::WMap;
 namespace {
 
 constexpr double kPi = 3.141592653589793;
-constexpr int kMaps = 11;  // part, chg, chgG, fx, fy, fz, cell1, cell1w, cell2, cell3, cell3c
+// part, chg, chgG, fx, fy, fz, cell1, cell1w, cell2, cell3, cell3c, brick, plane
+constexpr int kMaps = 13;
+// LDS a fused FFT plane may take (both planes of the kernel)
+constexpr size_t kPlaneLds = 65536;
 
 bool is_pow2(int n) { return n >= 4 && (n & (n - 1)) == 0; }
 int ilog2(int n) {
@@ -77,7 +80,8 @@ struct Sys {
   size_t bytes = 0;
   int ncell1 = 0, ncell2 = 0, ncell3 = 0;
   int nb[kMaps] = {0};
-  size_t lds_x = 0, lds_y = 0, lds_z = 0, lds_sort = 0, lds_sort3 = 0;
+  size_t lds_x = 0, lds_y = 0, lds_z = 0, lds_sort = 0, lds_sort3 = 0, lds_plane = 0;
+  int nbin = 0;  // mesh bins of the spreading kernel
   double *d_eta = nullptr, *d_out1 = nullptr, *d_out2 = nullptr;
   int *d_hist = nullptr;     // observables: z histogram of the chains
   double *d_hsum = nullptr;  // observables: per-block sums of z
@@ -123,6 +127,7 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
         return fail(MD_EINVAL, "SPME mesh sizes must be powers of two >= 4 (set ovr_ng)");
       if (p.ordr[d] != p.ordr[0] || p.ordr[d] < 3 || p.ordr[d] > mdk::kMaxOrder)
         return fail(MD_EINVAL, "SPME order must be equal on all axes and in 3..8");
+      if (p.ng[d] < p.ordr[d]) return fail(MD_EINVAL, "SPME mesh smaller than the spline order");
     }
   }
   // topology (compt_acc.f90:317-372, 433-488)
@@ -271,12 +276,29 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
     }
     S.order = p.ordr[0];
     S.Kh = K1 / 2 + 1;
+
     const size_t M = (size_t)K1 * K2 * K3, Mh = (size_t)K3 * K2 * S.Kh;
-    S.meshq = dalloc<unsigned long long>(M);
+    S.meshr = dalloc<char>(M * rs);
     S.spec = dalloc<char>(Mh * csz);
     S.meshu = dalloc<char>(M * rs);
     S.bc = dalloc<char>(Mh * rs);
-    if (!S.meshq || !S.spec || !S.meshu || !S.bc) return fail(MD_ENOMEM, "mesh");
+    if (!S.meshr || !S.spec || !S.meshu || !S.bc) return fail(MD_ENOMEM, "mesh");
+    // bricks of the spreading kernel: edge >= order, so a charge's points
+    // fall in its own bin's brick and the next one along each axis
+    nbin = 1;
+    for (int d = 0; d < 3; d++) {
+      S.bsz[d] = std::min((p.ordr[0] <= 4 ? 4 : 8), p.ng[d]);
+      S.nbr[d] = p.ng[d] / S.bsz[d];
+      nbin *= S.nbr[d];
+    }
+    S.bcnt = dalloc<int>(nbin);
+    S.bmem = nullptr;
+    S.capb = 0;
+    if (!S.bcnt) return fail(MD_ENOMEM, "mesh bins");
+    // x and y passes fused per z-plane when both planes fit in LDS
+    lds_plane = ((size_t)(K2 / 2) * K1 + (size_t)S.Kh * K2) * csz;
+    S.plane_fused = lds_plane <= kPlaneLds;
+    if (!S.plane_fused) lds_plane = 0;
     // influence function of PME_BC (compt_acc.f90:1557-1598), half spectrum
     const int o = S.order;
     std::vector<double> b2[3], mv[3], ev[3];
@@ -302,6 +324,16 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
                                : b2[0][x] * b2[1][y] * b2[2][z] * pivol * ev[0][x] * ev[1][y] *
                                      ev[2][z] / m2;
         }
+    if (S.plane_fused) {  // the fused planes keep ky in bit-reversed order
+      std::vector<double> t(bcd);
+      const int l2 = ilog2(K2);
+      for (int z = 0; z < K3; z++)
+        for (int y = 0; y < K2; y++) {
+          int yr = 0;
+          for (int b = 0; b < l2; b++) yr |= ((y >> b) & 1) << (l2 - 1 - b);
+          std::copy_n(&t[((size_t)z * K2 + yr) * S.Kh], S.Kh, &bcd[((size_t)z * K2 + y) * S.Kh]);
+        }
+    }
     if (fp32) {
       std::vector<float> f(bcd.begin(), bcd.end());
       (void)hipMemcpy(S.bc, f.data(), Mh * 4, hipMemcpyHostToDevice);
@@ -341,8 +373,8 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   nb[0] = cdiv(NN, mdk::kBlock);
   nb[1] = coulomb ? cdiv(Nq, mdk::kBlock) : 0;
   nb[2] = coulomb ? S.nqblk : 0;
-  nb[3] = coulomb ? cdiv((long)S.K[1] * S.K[2] / 2, S.px_tile) : 0;
-  nb[4] = coulomb ? S.K[2] * cdiv(S.Kh, S.py_tile) : 0;
+  nb[3] = coulomb && !S.plane_fused ? cdiv((long)S.K[1] * S.K[2] / 2, S.px_tile) : 0;
+  nb[4] = coulomb && !S.plane_fused ? S.K[2] * cdiv(S.Kh, S.py_tile) : 0;
   nb[5] = coulomb ? S.K[1] * cdiv(S.Kh, S.pz_tile) : 0;
   // cell kernels stride over the cells with a grid of modest size
   nb[6] = std::min(ncell1, 1024);
@@ -350,6 +382,8 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   nb[8] = ncell2;
   nb[9] = ncell3;
   nb[10] = 0;  // set with the cell capacity (size_lists)
+  nb[11] = coulomb ? nbin : 0;
+  nb[12] = coulomb && S.plane_fused ? S.K[2] : 0;
   S.mem1 = S.mem2 = S.mem3 = nullptr;
   S.nb1 = S.nb2 = nullptr;
   S.ccap1 = S.ccap2 = S.ccap3 = S.nbcap1 = S.nbcap2 = 0;
@@ -424,6 +458,26 @@ int Sys::size_lists(const double *pos) {
     h.mem3 = dalloc<int>((size_t)ncell3 * ccap3);
     h.ccap3 = ccap3;
     if (!h.mem3) return fail(MD_ENOMEM, "real-space cells");
+  }
+  if (coulomb) {  // mesh bins of the spreading kernel
+    std::vector<int> cnt(nbin, 0);
+    int mb = 0;
+    auto base = [&](double c, int d) {
+      const int b = ((int)std::ceil(h.kg[d] * c) - h.order) % h.K[d];
+      return b < 0 ? b + h.K[d] : b;
+    };
+    for (int i : charge) {
+      const int bx = base(X[i], 0) / h.bsz[0], by = base(Y[i], 1) / h.bsz[1],
+                bz = base(Z[i], 2) / h.bsz[2];
+      mb = std::max(mb, ++cnt[((size_t)bz * h.nbr[1] + by) * h.nbr[0] + bx]);
+    }
+    const int capb = round8(std::min(h.Nq, 2 * mb + 32));
+    if (capb != h.capb) {
+      if (h.bmem) dfree(h.bmem, (size_t)nbin * h.capb * 4);
+      h.bmem = dalloc<int>((size_t)nbin * capb);
+      h.capb = capb;
+      if (!h.bmem) return fail(MD_ENOMEM, "mesh bins");
+    }
   }
   int np3 = 1;
   while (np3 < h.ccap3) np3 <<= 1;
@@ -586,18 +640,20 @@ int md_engine::rebuild() {
   auto fill = [&](Plan &P, Rep *reps, int nrep, const int *maps, const int *totals) {
     P.reps = reps;
     P.nrep = nrep;
-    WMap *w[kMaps] = {&P.part, &P.chg, &P.chgG, &P.fx, &P.fy, &P.fz, &P.cell1, &P.cell1w, &P.cell2, &P.cell3, &P.cell3c};
+    WMap *w[kMaps] = {&P.part,  &P.chg,   &P.chgG,  &P.fx,    &P.fy,    &P.fz,   &P.cell1,
+                      &P.cell1w, &P.cell2, &P.cell3, &P.cell3c, &P.brick, &P.plane};
     for (int k = 0; k < kMaps; k++) *w[k] = WMap{maps + k * (nrep + 1), nrep, totals[k]};
     P.vpart = vpart;
     P.fp32 = fp32;
   };
   fill(plan, d_reps, R, d_maps, tot);
-  plan.lds_x = plan.lds_y = plan.lds_z = plan.lds_sort = plan.lds_sort3 = 0;
+  plan.lds_x = plan.lds_y = plan.lds_z = plan.lds_sort = plan.lds_sort3 = plan.lds_plane = 0;
   plan.pairs_any = false;
   for (Sys *s : sys) {
     plan.lds_x = std::max(plan.lds_x, s->lds_x);
     plan.lds_y = std::max(plan.lds_y, s->lds_y);
     plan.lds_z = std::max(plan.lds_z, s->lds_z);
+    plan.lds_plane = std::max(plan.lds_plane, s->lds_plane);
     plan.lds_sort = std::max(plan.lds_sort, s->lds_sort);
     plan.lds_sort3 = std::max(plan.lds_sort3, s->lds_sort3);
     plan.pairs_any = plan.pairs_any || (s->coulomb && !s->h.real_cells);
@@ -608,6 +664,7 @@ int md_engine::rebuild() {
     single[r].lds_x = sys[r]->lds_x;
     single[r].lds_y = sys[r]->lds_y;
     single[r].lds_z = sys[r]->lds_z;
+    single[r].lds_plane = sys[r]->lds_plane;
     single[r].lds_sort = sys[r]->lds_sort;
     single[r].lds_sort3 = sys[r]->lds_sort3;
     single[r].pairs_any = sys[r]->coulomb && !sys[r]->h.real_cells;
@@ -647,9 +704,9 @@ int md_engine::check_one(const Sys *s, const Ctrl &c, std::string &msg) const {
   if (c.ovf) {
     std::snprintf(buf, sizeof buf,
                   "capacity exceeded (mask %d): LJ cell %d/%d list %d/%d, real-space cell %d/%d, "
-                  "real list %d/%d",
+                  "real list %d/%d, mesh bin capacity %d",
                   c.ovf, c.max_cell1, s->h.ccap1, c.max_nb1, s->h.nbcap1, c.max_cell2, s->h.ccap3,
-                  c.max_nb2, s->h.nbcap2);
+                  c.max_nb2, s->h.nbcap2, s->h.capb);
     msg = buf;
     return MD_EOVERFLOW;
   }

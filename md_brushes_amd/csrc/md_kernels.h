@@ -22,8 +22,10 @@ struct Plan {
   // block maps: particles (thread each), charges, charges x G lanes, FFT
   // tiles, LJ cells (workgroup each, wavefront each), reference real-list
   // cells, real-space sweep cells (workgroup each, per target chunk)
-  WMap part{}, chg{}, chgG{}, fx{}, fy{}, fz{}, cell1{}, cell1w{}, cell2{}, cell3{}, cell3c{};
-  size_t lds_x = 0, lds_y = 0, lds_z = 0, lds_sort = 0, lds_sort3 = 0;
+  // ... mesh bricks (one wavefront each), z-planes
+  WMap part{}, chg{}, chgG{}, fx{}, fy{}, fz{}, cell1{}, cell1w{}, cell2{}, cell3{}, cell3c{},
+      brick{}, plane{};
+  size_t lds_x = 0, lds_y = 0, lds_z = 0, lds_sort = 0, lds_sort3 = 0, lds_plane = 0;
   bool pairs_any = false;   // some replica uses the all-pairs real space
   double *vpart = nullptr;  // partial sums of rescale_velocity
   bool fp32 = false;

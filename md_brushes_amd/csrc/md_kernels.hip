@@ -680,32 +680,90 @@ __device__ __forceinline__ int wrapk(int v, int K) {
   return v < 0 ? v + K : v;
 }
 
-// MapCharges (compt_acc.f90:1324-1365): charge spreading into a
-// fixed-point mesh (integer atomics: the sum is independent of order).
-// G lanes per charge: lane (b,c) adds the p points of mesh row (y_b, z_c).
+// First mesh index of a charge's spline weights along one axis, wrapped
+// into [0,K): uuc - p of splcof (compt_acc.f90:1269-1270,1304-1308).
+__device__ __forceinline__ int spline_base(double coord, double kg, int p, int K) {
+  return wrapk((int)ceil(kg * coord) - p, K);
+}
+
+// MapCharges (compt_acc.f90:1324-1365) without atomics: one workgroup (one
+// wavefront) per mesh brick.  The charges whose first spline point lies in
+// the brick's bin or in the bins below it along each axis are staged in LDS
+// with their weights; each lane owns mesh points of the brick and adds the
+// products that land on them.  Sums are kept in 2^-40 fixed point, so the
+// result does not depend on the order of the charges in a bin; every mesh
+// point is written once.
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_spread(const Rep *__restrict__ reps, WMap wm) {
+__global__ __launch_bounds__(64) void k_spread(const Rep *__restrict__ reps, WMap wm) {
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
-  const int t = (blockIdx.x - wm.blk[r]) * kBlock + threadIdx.x;
-  const int G = S.G, o = S.order;
-  const int m = t / G, g = t - m * G;
-  if (m >= S.Nq || g >= o * o) return;
-  const int bb = g % o, cc = g / o;
-  const double4 p = S.pos[S.charge[m]];
-  R wx[kMaxOrder], wy[kMaxOrder], wz[kMaxOrder], d[kMaxOrder];
-  const int bx = spline<R>(p.x, S.kg[0], o, wx, d);
-  const int by = spline<R>(p.y, S.kg[1], o, wy, d);
-  const int bz = spline<R>(p.z, S.kg[2], o, wz, d);
-  const R q = (R)p.w;
+  const int brick = blockIdx.x - wm.blk[r];
+  const int lane = threadIdx.x;
+  const int o = S.order;
+  const int bx = brick % S.nbr[0], by = (brick / S.nbr[0]) % S.nbr[1], bz = brick / (S.nbr[0] * S.nbr[1]);
+  const int Bx = S.bsz[0], By = S.bsz[1], Bz = S.bsz[2];
+  const int npt = Bx * By * Bz;  // <= 512: at most 8 points per lane
+  const int kpt = (npt - lane + 63) >> 6;  // points of this lane
   const int K1 = S.K[0], K2 = S.K[1], K3 = S.K[2];
-  unsigned long long *row =
-      S.meshq + ((size_t)wrapk(bz + cc, K3) * K2 + wrapk(by + bb, K2)) * K1;
-  const R wzy = wz[cc];
-  for (int a = 0; a < o; a++) {
-    const double v = (double)(wzy * ((wx[a] * q) * wy[bb]));
-    atomicAdd(row + wrapk(bx + a, K1), (unsigned long long)(long long)rint(v * kQScale));
+  __shared__ R wx[64][kMaxOrder], wy[64][kMaxOrder], wz[64][kMaxOrder];
+  __shared__ int b3[64][3];
+  long long acc[8];
+  int gx[8], gy[8], gz[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    acc[k] = 0;
+    const int pt = lane + 64 * k;
+    gx[k] = bx * Bx + pt % Bx;
+    gy[k] = by * By + (pt / Bx) % By;
+    gz[k] = bz * Bz + pt / (Bx * By);
   }
+  for (int nb = 0; nb < 8; nb++) {
+    const int ox = nb & 1, oy = (nb >> 1) & 1, oz = nb >> 2;  // bin offsets 0 / -1
+    if ((ox && S.nbr[0] == 1) || (oy && S.nbr[1] == 1) || (oz && S.nbr[2] == 1)) continue;
+    const int cx = (bx - ox + S.nbr[0]) % S.nbr[0], cy = (by - oy + S.nbr[1]) % S.nbr[1],
+              cz = (bz - oz + S.nbr[2]) % S.nbr[2];
+    const int bin = (cz * S.nbr[1] + cy) * S.nbr[0] + cx;
+    const int n = min(S.bcnt[bin], S.capb);
+    for (int c0 = 0; c0 < n; c0 += 64) {
+      const int nc = min(64, n - c0);
+      if (lane < nc) {
+        const int m = S.bmem[(size_t)bin * S.capb + c0 + lane];
+        const double4 p = S.pos[S.charge[m]];
+        R d[kMaxOrder], tx[kMaxOrder], ty[kMaxOrder], tz[kMaxOrder];
+        b3[lane][0] = wrapk(spline<R>(p.x, S.kg[0], o, tx, d), K1);
+        b3[lane][1] = wrapk(spline<R>(p.y, S.kg[1], o, ty, d), K2);
+        b3[lane][2] = wrapk(spline<R>(p.z, S.kg[2], o, tz, d), K3);
+        const R q = (R)p.w;
+        for (int a = 0; a < o; a++) {
+          wx[lane][a] = tx[a] * q;
+          wy[lane][a] = ty[a];
+          wz[lane][a] = tz[a];
+        }
+      }
+      __syncthreads();
+      for (int c = 0; c < nc; c++) {
+        const int cx0 = b3[c][0], cy0 = b3[c][1], cz0 = b3[c][2];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          if (k < kpt) {
+            int dx = gx[k] - cx0, dy = gy[k] - cy0, dz = gz[k] - cz0;
+            dx += dx < 0 ? K1 : 0;
+            dy += dy < 0 ? K2 : 0;
+            dz += dz < 0 ? K3 : 0;
+            if (dx < o && dy < o && dz < o) {
+              const double v = (double)(wz[c][dz] * (wx[c][dx] * wy[c][dy]));
+              acc[k] += (long long)rint(v * kQScale);
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  R *q = (R *)S.meshr;
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+    if (k < kpt) q[((size_t)gz[k] * K2 + gy[k]) * K1 + gx[k]] = (R)((double)acc[k] * kQInv);
 }
 
 // In-LDS radix-2 FFTs over `nl` lines of length n = 2^ln stored line-major.
@@ -755,8 +813,8 @@ __device__ void fft_dif(C *buf, int ln, int nl, const C *__restrict__ tw, int sg
 
 __device__ __forceinline__ int bitrev(int x, int ln) { return (int)(__brev((unsigned)x) >> (32 - ln)); }
 
-// Forward x pass: pairs of real mesh rows (fixed point, cleared after the
-// read) as one complex FFT of length K1, split into two half spectra.
+// Forward x pass: pairs of real mesh rows as one complex FFT of length K1,
+// split into two half spectra.
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_fftx_fwd(const Rep *__restrict__ reps, WMap wm) {
   using C = typename CT<R>::C;
@@ -769,15 +827,13 @@ __global__ __launch_bounds__(kBlock) void k_fftx_fwd(const Rep *__restrict__ rep
   const int P = S.px_tile;
   const int pair0 = (blockIdx.x - wm.blk[r]) * P;
   const int nl = min(P, npairs - pair0);
+  const R *q = (const R *)S.meshr;
   for (int t = threadIdx.x; t < nl * K1; t += blockDim.x) {
     const int l = t / K1, x = t - l * K1;
     const size_t row = 2 * (size_t)(pair0 + l);
-    unsigned long long *q0 = S.meshq + row * K1 + x, *q1 = q0 + K1;
     C v;
-    v.x = (R)((double)(long long)*q0 * kQInv);
-    v.y = (R)((double)(long long)*q1 * kQInv);
-    *q0 = 0ull;
-    *q1 = 0ull;
+    v.x = q[row * K1 + x];
+    v.y = q[(row + 1) * K1 + x];
     buf[l * K1 + bitrev(x, ln)] = v;
   }
   __syncthreads();
@@ -868,6 +924,98 @@ __global__ __launch_bounds__(kBlock) void k_ffty(const Rep *__restrict__ reps, W
   }
 }
 
+// Forward x and y passes of one z-plane in LDS: pairs of real rows -> one
+// complex FFT of length K1 each (bit-reversed load, DIT), split into two
+// half spectra stored line-major by kx, then DIF along y.  The plane leaves
+// in bit-reversed y order; the z pass and the influence function use that
+// order, and the inverse y pass (DIT) takes it back.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_plane_fwd(const Rep *__restrict__ reps, WMap wm) {
+  using C = typename CT<R>::C;
+  extern __shared__ char smem[];
+  const int r = find_rep(wm, blockIdx.x);
+  const Rep &S = reps[r];
+  const int z = blockIdx.x - wm.blk[r];
+  const int K1 = S.K[0], K2 = S.K[1], Kh = S.Kh, l1 = S.lK[0], l2 = S.lK[1];
+  C *A = (C *)smem;                  // K2/2 lines of K1
+  C *P = A + (size_t)(K2 / 2) * K1;  // Kh lines of K2
+  const R *q = (const R *)S.meshr + (size_t)z * K2 * K1;
+  for (int t = threadIdx.x; t < (K2 / 2) * K1; t += blockDim.x) {
+    const int l = t / K1, x = t - l * K1;
+    C v;
+    v.x = q[(size_t)(2 * l) * K1 + x];
+    v.y = q[(size_t)(2 * l + 1) * K1 + x];
+    A[l * K1 + bitrev(x, l1)] = v;
+  }
+  __syncthreads();
+  fft_dit<C>(A, l1, K2 / 2, (const C *)S.tw[0], -1);
+  for (int t = threadIdx.x; t < (K2 / 2) * Kh; t += blockDim.x) {
+    const int l = t / Kh, k = t - l * Kh;
+    const C zk = A[l * K1 + k], zm = A[l * K1 + ((K1 - k) & (K1 - 1))];
+    C a, b;
+    a.x = (R)0.5 * (zk.x + zm.x);
+    a.y = (R)0.5 * (zk.y - zm.y);
+    b.x = (R)0.5 * (zk.y + zm.y);
+    b.y = (R)0.5 * (zm.x - zk.x);
+    P[k * K2 + 2 * l] = a;
+    P[k * K2 + 2 * l + 1] = b;
+  }
+  __syncthreads();
+  fft_dif<C>(P, l2, Kh, (const C *)S.tw[1], -1);
+  C *spec = (C *)S.spec + (size_t)z * K2 * Kh;
+  for (int t = threadIdx.x; t < K2 * Kh; t += blockDim.x) {
+    const int yp = t / Kh, k = t - yp * Kh;
+    spec[t] = P[k * K2 + yp];
+  }
+}
+
+// Inverse y and x passes of one z-plane: DIT along y from the bit-reversed
+// order, rebuild pairs of rows Z = A + iB over the full x range (A, B
+// Hermitian), DIF along x, real and imaginary parts are the two rows.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_plane_inv(const Rep *__restrict__ reps, WMap wm) {
+  using C = typename CT<R>::C;
+  extern __shared__ char smem[];
+  const int r = find_rep(wm, blockIdx.x);
+  const Rep &S = reps[r];
+  const int z = blockIdx.x - wm.blk[r];
+  const int K1 = S.K[0], K2 = S.K[1], Kh = S.Kh, l1 = S.lK[0], l2 = S.lK[1];
+  C *A = (C *)smem;
+  C *P = A + (size_t)(K2 / 2) * K1;
+  const C *spec = (const C *)S.spec + (size_t)z * K2 * Kh;
+  for (int t = threadIdx.x; t < K2 * Kh; t += blockDim.x) {
+    const int yp = t / Kh, k = t - yp * Kh;
+    P[k * K2 + yp] = spec[t];
+  }
+  __syncthreads();
+  fft_dit<C>(P, l2, Kh, (const C *)S.tw[1], +1);
+  for (int t = threadIdx.x; t < (K2 / 2) * K1; t += blockDim.x) {
+    const int l = t / K1, k = t - l * K1;
+    C a, b, v;
+    if (k < Kh) {
+      a = P[k * K2 + 2 * l];
+      b = P[k * K2 + 2 * l + 1];
+    } else {
+      a = P[(K1 - k) * K2 + 2 * l];
+      b = P[(K1 - k) * K2 + 2 * l + 1];
+      a.y = -a.y;
+      b.y = -b.y;
+    }
+    v.x = a.x - b.y;
+    v.y = a.y + b.x;
+    A[l * K1 + k] = v;
+  }
+  __syncthreads();
+  fft_dif<C>(A, l1, K2 / 2, (const C *)S.tw[0], +1);
+  R *u = (R *)S.meshu + (size_t)z * K2 * K1;
+  for (int t = threadIdx.x; t < (K2 / 2) * K1; t += blockDim.x) {
+    const int l = t / K1, x = t - l * K1;
+    const C v = A[l * K1 + bitrev(x, l1)];
+    u[(size_t)(2 * l) * K1 + x] = v.x;
+    u[(size_t)(2 * l + 1) * K1 + x] = v.y;
+  }
+}
+
 // z pass with the convolution of pmeOrthoConvBC (compt_acc.f90:1382-1396):
 // forward FFT along z, multiply by BC, inverse FFT along z; tiles of Tz
 // adjacent kx lines for one ky.
@@ -922,6 +1070,11 @@ __global__ __launch_bounds__(kBlock) void k_gather(const Rep *__restrict__ reps,
   const int G = S.G, o = S.order;
   const int m = t / G, g = t - m * G;
   const bool live = m < S.Nq;
+  {  // the mesh bins are consumed: clear them for the next evaluation
+    const int nbin = S.nbr[0] * S.nbr[1] * S.nbr[2];
+    const int nth = (wm.blk[r + 1] - wm.blk[r]) * kBlock;
+    for (int k = t; k < nbin; k += nth) S.bcnt[k] = 0;
+  }
   R f1 = 0, f2 = 0, f3 = 0;
   double4 p = make_double4(0, 0, 0, 0);
   if (live) p = S.pos[S.charge[m]];
@@ -1039,16 +1192,28 @@ __device__ __forceinline__ void real_pair(R xi, R yi, R zi, R qi, R xj, R yj, R 
   }
 }
 
-// Charges into the cells of the real-space sweep (edge >= r_c).
+// Charges into the mesh bins of the spreading kernel and, when the sweep
+// is used, into the cells of the real-space sweep (edge >= r_c).
 __global__ __launch_bounds__(kBlock) void k_bin_q(const Rep *__restrict__ reps, WMap wm) {
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
-  if (!S.real_cells) return;
   const int m = (blockIdx.x - wm.blk[r]) * kBlock + threadIdx.x;
   if (m >= S.Nq) return;
   const int i = S.charge[m];
+  const double4 p = S.pos[i];
+  {
+    const int o = S.order;
+    const int gx = spline_base(p.x, S.kg[0], o, S.K[0]) / S.bsz[0];
+    const int gy = spline_base(p.y, S.kg[1], o, S.K[1]) / S.bsz[1];
+    const int gz = spline_base(p.z, S.kg[2], o, S.K[2]) / S.bsz[2];
+    const int bin = (gz * S.nbr[1] + gy) * S.nbr[0] + gx;
+    const int slot = atomicAdd(&S.bcnt[bin], 1);
+    if (slot < S.capb) S.bmem[(size_t)bin * S.capb + slot] = m;
+    else atomicOr(&S.ctrl->ovf, 32);
+  }
+  if (!S.real_cells) return;
   int a, b, c;
-  if (!cell_of(S.pos[i], S.nc3, S.rcel3, S.hLx, S.hLy, a, b, c)) {
+  if (!cell_of(p, S.nc3, S.rcel3, S.hLx, S.hLy, a, b, c)) {
     set_error(S.ctrl, MD_ECELL, i, -1);
     return;
   }
@@ -1335,26 +1500,33 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) 
     if (ev) (void)hipEventRecord(ev[k++], st);
   };
   mark();
-  if (P.cell3.total) {
-    hipLaunchKernelGGL(k_bin_q, dim3(P.chg.total), dim3(kBlock), 0, st, P.reps, P.chg);
-    mark();
+  hipLaunchKernelGGL(k_bin_q, dim3(P.chg.total), dim3(kBlock), 0, st, P.reps, P.chg);
+  mark();
+  if (P.cell3.total)
     hipLaunchKernelGGL(k_sort_q, dim3(P.cell3.total), dim3(kBlock), P.lds_sort3, st, P.reps, P.cell3);
-    mark();
-  } else if (ev) {
-    mark();
-    mark();
-  }
-  hipLaunchKernelGGL(k_spread<R>, dim3(P.chgG.total), dim3(kBlock), 0, st, P.reps, P.chgG);
   mark();
-  hipLaunchKernelGGL(k_fftx_fwd<R>, dim3(P.fx.total), dim3(kBlock), P.lds_x, st, P.reps, P.fx);
+  hipLaunchKernelGGL(k_spread<R>, dim3(P.brick.total), dim3(64), 0, st, P.reps, P.brick);
   mark();
-  hipLaunchKernelGGL(k_ffty<R>, dim3(P.fy.total), dim3(kBlock), P.lds_y, st, P.reps, P.fy, -1);
+  // replicas with fused planes take k_plane_*, the others the x and y passes
+  if (P.plane.total)
+    hipLaunchKernelGGL(k_plane_fwd<R>, dim3(P.plane.total), dim3(kBlock), P.lds_plane, st, P.reps,
+                       P.plane);
+  if (P.fx.total)
+    hipLaunchKernelGGL(k_fftx_fwd<R>, dim3(P.fx.total), dim3(kBlock), P.lds_x, st, P.reps, P.fx);
+  mark();
+  if (P.fy.total)
+    hipLaunchKernelGGL(k_ffty<R>, dim3(P.fy.total), dim3(kBlock), P.lds_y, st, P.reps, P.fy, -1);
   mark();
   hipLaunchKernelGGL(k_fftz_conv<R>, dim3(P.fz.total), dim3(kBlock), P.lds_z, st, P.reps, P.fz);
   mark();
-  hipLaunchKernelGGL(k_ffty<R>, dim3(P.fy.total), dim3(kBlock), P.lds_y, st, P.reps, P.fy, +1);
+  if (P.fy.total)
+    hipLaunchKernelGGL(k_ffty<R>, dim3(P.fy.total), dim3(kBlock), P.lds_y, st, P.reps, P.fy, +1);
   mark();
-  hipLaunchKernelGGL(k_fftx_inv<R>, dim3(P.fx.total), dim3(kBlock), P.lds_x, st, P.reps, P.fx);
+  if (P.plane.total)
+    hipLaunchKernelGGL(k_plane_inv<R>, dim3(P.plane.total), dim3(kBlock), P.lds_plane, st, P.reps,
+                       P.plane);
+  if (P.fx.total)
+    hipLaunchKernelGGL(k_fftx_inv<R>, dim3(P.fx.total), dim3(kBlock), P.lds_x, st, P.reps, P.fx);
   mark();
   hipLaunchKernelGGL(k_gather<R>, dim3(P.chgG.total), dim3(kBlock), 0, st, P.reps, P.chgG);
   mark();

@@ -120,10 +120,12 @@ def config(name, precision=MD_FP64, ez=None, seed=20261203, order=None):
     return derive(p)
 
 
-def replica(chains_side, n_beads, arch, valence, ez, seed, precision=MD_FP32):
+def replica(chains_side, n_beads, arch, valence, ez, seed, precision=MD_FP32, preheat=0):
     """One request of the serving stream: chains_side^2 grafted molecules of
     n_beads beads (linear, or stars with arch arms sharing the beads), all
-    charged, counterions plus salt of the given valence at charge ratio 1."""
+    charged, counterions plus salt of the given valence at charge ratio 1.
+    In a batch, steps 1..preheat follow the pre-heat form (StepNum0 of
+    system_data.txt: velocities rescaled every 20 steps, main.f90:67-75)."""
     p = _base()
     n = chains_side * chains_side
     if arch == 1:
@@ -144,6 +146,7 @@ def replica(chains_side, n_beads, arch, valence, ez, seed, precision=MD_FP32):
     p.ovr_alpha = ALPHA_H17_P4
     p.precision = precision
     p.seed = seed
+    p.StepNum0 = preheat
     return derive(p)
 
 

@@ -13,6 +13,8 @@ namespace mdk {
 
 constexpr int kBlock = 256;
 constexpr int kMaxOrder = 8;
+constexpr int kTile = 16;         // spreading tile edge (mesh points)
+constexpr int kPlaneBlock = 512;  // threads of the fused FFT plane kernels
 constexpr int kRealChunk = 16;  // target charges per workgroup of the real-space sweep
 constexpr int kHistBins = 500;  // SizeHist of the z-density profile (glob_var.f90:59)
 
@@ -37,7 +39,9 @@ struct Ctrl {
 struct Rep {
   // sizes
   int NN, Nq, N_anchor, real_verlet;
-  int Npe, pad_npe;                // polymer beads are particles 0..Npe-1
+  int Npe;                         // polymer beads are particles 0..Npe-1
+  int preheat_end;                 // batch replicas: steps 1..preheat_end rescale
+                                   // velocities every 20 steps (0: none)
   // geometry
   double Lx, Ly, Lz, hLx, hLy;
   // integrator (init_update.f90:555-607)
@@ -74,7 +78,10 @@ struct Rep {
   int *charge;                     // Nq: particle index of charge m
   int *bstart;                     // NN+1 CSR of bonded partners
   int *bpart;
-  int *cnt1, *mem1, *scnt1;        // LJ cells: binning counts, members, sorted counts
+  // LJ cells: two count buffers used in turn (gen1 & 1 selects the one a
+  // build fills; it clears the other at its particles' previous cells),
+  // members as binned, members ascending, each particle's cell (-1: none)
+  int *cnt1, *mem1, *mem1s, *cell1_of;
   int *cnt2[2], *mem2;             // cells of the reference-form real-space list
   // real-space cell sweep at each Coulomb evaluation (edge >= r_c)
   int real_cells;                  // 1: 27-cell sweep; 0: all charge pairs
@@ -82,14 +89,16 @@ struct Rep {
   double rcel3[3];
   int ccap3;
   int *cnt3, *mem3, *scnt3;
+  void *pos3s;                     // positions in cell-sorted order (float4 / double4)
+  double *qtot;                    // mean reciprocal force (3) and M_z of the evaluation
   int G;                           // lanes per charge in spread and gather
   int *nbn1, *nb1;                 // LJ list: length per particle, ELL slots
   int *nbn2, *nb2;                 // real list: per charge slot m
   void *meshr;                     // charge mesh Q (real type of the path) K3*K2*K1
-  // charges binned by the mesh brick of their first spline point
-  int bsz[3], nbr[3];              // brick edge (mesh points) and bricks per axis
-  int capb;                        // charges per bin
-  int *bcnt, *bmem;
+  // spreading tiles: each tile lists the charges with spline points in it
+  int tsz[3], ntl[3];              // tile edge (mesh points) and tiles per axis
+  int capt;                        // list capacity per tile
+  int *tcnt, *tmem;                // counts; particle indices, capt per tile
   int plane_fused;                 // x and y FFT passes fused per z-plane; the
                                    // spectrum is then kept in bit-reversed y order
   void *spec;                      // half spectrum K3*K2*Kh complex

@@ -81,7 +81,7 @@ struct Sys {
   int ncell1 = 0, ncell2 = 0, ncell3 = 0;
   int nb[kMaps] = {0};
   size_t lds_x = 0, lds_y = 0, lds_z = 0, lds_sort = 0, lds_sort3 = 0, lds_plane = 0;
-  int nbin = 0;  // mesh bins of the spreading kernel
+  int nbin = 0;  // spreading tiles
   double *d_eta = nullptr, *d_out1 = nullptr, *d_out2 = nullptr;
   int *d_hist = nullptr;     // observables: z histogram of the chains
   double *d_hsum = nullptr;  // observables: per-block sums of z
@@ -208,18 +208,17 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   if (S.nc1[0] < 1 || S.nc1[1] < 1 || S.nc1[2] < 1) return fail(MD_EINVAL, "box smaller than rvl");
   ncell1 = S.nc1[0] * S.nc1[1] * S.nc1[2];
   ncell2 = S.real_verlet ? S.nc2[0] * S.nc2[1] * S.nc2[2] : 0;
-  // real-space sweep cells: edge >= r_c; three or more per axis, or all pairs
+  // real-space sweep cells: edge >= r_c (one cell when the box is shorter);
+  // with fewer than three cells along x or y the stencil visits each once
   S.real_cells = 0;
   ncell3 = 0;
   if (coulomb) {
-    bool ok = true;
     for (int d = 0; d < 3; d++) {
-      S.nc3[d] = (int)(L3[d] / p.rcc);
-      ok = ok && S.nc3[d] >= 3;
-      S.rcel3[d] = S.nc3[d] > 0 ? L3[d] / S.nc3[d] : L3[d];
+      S.nc3[d] = std::max(1, (int)(L3[d] / p.rcc));
+      S.rcel3[d] = L3[d] / S.nc3[d];
     }
-    S.real_cells = ok ? 1 : 0;
-    if (ok) ncell3 = S.nc3[0] * S.nc3[1] * S.nc3[2];
+    S.real_cells = 1;
+    ncell3 = S.nc3[0] * S.nc3[1] * S.nc3[2];
   }
   S.G = (coulomb && p.ordr[0] > 4) ? 64 : 16;
 
@@ -251,8 +250,9 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   if (Nq) (void)hipMemcpy(S.charge, charge.data(), Nq * sizeof(int), hipMemcpyHostToDevice);
   (void)hipMemcpy(S.bstart, bstart.data(), (nn + 1) * sizeof(int), hipMemcpyHostToDevice);
   (void)hipMemcpy(S.bpart, bpart.data(), bpart.size() * sizeof(int), hipMemcpyHostToDevice);
-  S.cnt1 = dalloc<int>(ncell1);
-  S.scnt1 = dalloc<int>(ncell1);
+  S.cnt1 = dalloc<int>(2 * (size_t)ncell1);  // two count buffers, used in turn
+  S.cell1_of = dalloc<int>(NN);
+  if (S.cell1_of) (void)hipMemset(S.cell1_of, 0xff, NN * sizeof(int));  // no cell yet
   S.cnt2[0] = dalloc<int>(std::max(ncell2, 1));
   S.cnt2[1] = dalloc<int>(std::max(ncell2, 1));
   S.cnt3 = dalloc<int>(std::max(ncell3, 1));
@@ -260,7 +260,7 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   S.nbn1 = dalloc<int>(nn);
   S.nbn2 = dalloc<int>(std::max(Nq, 1));
   S.rsum = dalloc<double>(3 * (size_t)std::max(Nq, 1));
-  if (!S.cnt1 || !S.scnt1 || !S.cnt2[0] || !S.cnt2[1] || !S.cnt3 || !S.scnt3 || !S.nbn1 ||
+  if (!S.cnt1 || !S.cell1_of || !S.cnt2[0] || !S.cnt2[1] || !S.cnt3 || !S.scnt3 || !S.nbn1 ||
       !S.nbn2 || !S.rsum)
     return fail(MD_ENOMEM, "cells");
 
@@ -283,18 +283,18 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
     S.meshu = dalloc<char>(M * rs);
     S.bc = dalloc<char>(Mh * rs);
     if (!S.meshr || !S.spec || !S.meshu || !S.bc) return fail(MD_ENOMEM, "mesh");
-    // bricks of the spreading kernel: edge >= order, so a charge's points
-    // fall in its own bin's brick and the next one along each axis
+    // spreading tiles: edge >= order, so a charge's points fall in at most
+    // two tiles along each axis
     nbin = 1;
     for (int d = 0; d < 3; d++) {
-      S.bsz[d] = std::min((p.ordr[0] <= 4 ? 4 : 8), p.ng[d]);
-      S.nbr[d] = p.ng[d] / S.bsz[d];
-      nbin *= S.nbr[d];
+      S.tsz[d] = std::min(mdk::kTile, p.ng[d]);
+      S.ntl[d] = p.ng[d] / S.tsz[d];
+      nbin *= S.ntl[d];
     }
-    S.bcnt = dalloc<int>(nbin);
-    S.bmem = nullptr;
-    S.capb = 0;
-    if (!S.bcnt) return fail(MD_ENOMEM, "mesh bins");
+    S.tcnt = dalloc<int>(nbin);
+    S.tmem = nullptr;
+    S.capt = 0;
+    if (!S.tcnt) return fail(MD_ENOMEM, "spreading tiles");
     // x and y passes fused per z-plane when both planes fit in LDS
     lds_plane = ((size_t)(K2 / 2) * K1 + (size_t)S.Kh * K2) * csz;
     S.plane_fused = lds_plane <= kPlaneLds;
@@ -361,9 +361,12 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
     const int budget = 32768;
     S.px_tile = std::max(1, std::min(K2 * K3 / 2, budget / (K1 * csz)));
     S.py_tile = std::max(1, std::min(S.Kh, budget / (K2 * csz)));
-    S.pz_tile = std::max(1, std::min(S.Kh, budget / (K3 * csz)));
+    // z pass: few lines per workgroup, so that the launch fills the chip
+    S.pz_tile = std::max(1, std::min(4, budget / (K3 * csz)));
     S.nqblk = cdiv((long)Nq * S.G, mdk::kBlock);
     S.fsp = dalloc<double>(3 * (size_t)Nq);
+    S.qtot = dalloc<double>(4);
+    if (!S.qtot) return fail(MD_ENOMEM, "spme totals");
     S.part = dalloc<double>(4 * (size_t)S.nqblk);
     if (!S.fsp || !S.part) return fail(MD_ENOMEM, "spme scratch");
     lds_x = (size_t)S.px_tile * K1 * csz;
@@ -375,16 +378,16 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   nb[2] = coulomb ? S.nqblk : 0;
   nb[3] = coulomb && !S.plane_fused ? cdiv((long)S.K[1] * S.K[2] / 2, S.px_tile) : 0;
   nb[4] = coulomb && !S.plane_fused ? S.K[2] * cdiv(S.Kh, S.py_tile) : 0;
-  nb[5] = coulomb ? S.K[1] * cdiv(S.Kh, S.pz_tile) : 0;
+  nb[5] = coulomb ? cdiv((long)S.K[1] * S.Kh, S.pz_tile) : 0;
   // cell kernels stride over the cells with a grid of modest size
-  nb[6] = std::min(ncell1, 1024);
-  nb[7] = std::min(cdiv(ncell1, mdk::kBlock / 64), 512);
+  nb[6] = nb[7] = 0;  // the LJ list kernels use the rebuild queue
   nb[8] = ncell2;
   nb[9] = ncell3;
   nb[10] = 0;  // set with the cell capacity (size_lists)
   nb[11] = coulomb ? nbin : 0;
   nb[12] = coulomb && S.plane_fused ? S.K[2] : 0;
-  S.mem1 = S.mem2 = S.mem3 = nullptr;
+  S.mem1 = S.mem1s = S.mem2 = S.mem3 = nullptr;
+  S.pos3s = nullptr;
   S.nb1 = S.nb2 = nullptr;
   S.ccap1 = S.ccap2 = S.ccap3 = S.nbcap1 = S.nbcap2 = 0;
   return MD_OK;
@@ -454,29 +457,41 @@ int Sys::size_lists(const double *pos) {
   if (ccap2 > 16384 || ccap3 > 16384)
     return fail(MD_EINVAL, "real-space cells too full for the LDS sort");
   if (h.real_cells && ccap3 != h.ccap3) {
+    const size_t vsz = fp32 ? 16 : 32;
     if (h.mem3) dfree(h.mem3, (size_t)ncell3 * h.ccap3 * 4);
+    if (h.pos3s) dfree(h.pos3s, (size_t)ncell3 * h.ccap3 * vsz);
     h.mem3 = dalloc<int>((size_t)ncell3 * ccap3);
+    h.pos3s = dalloc<char>((size_t)ncell3 * ccap3 * vsz);
+    if (!h.pos3s) return fail(MD_ENOMEM, "real-space cells");
     h.ccap3 = ccap3;
     if (!h.mem3) return fail(MD_ENOMEM, "real-space cells");
   }
-  if (coulomb) {  // mesh bins of the spreading kernel
+  if (coulomb) {  // spreading tile lists
     std::vector<int> cnt(nbin, 0);
     int mb = 0;
-    auto base = [&](double c, int d) {
-      const int b = ((int)std::ceil(h.kg[d] * c) - h.order) % h.K[d];
-      return b < 0 ? b + h.K[d] : b;
-    };
     for (int i : charge) {
-      const int bx = base(X[i], 0) / h.bsz[0], by = base(Y[i], 1) / h.bsz[1],
-                bz = base(Z[i], 2) / h.bsz[2];
-      mb = std::max(mb, ++cnt[((size_t)bz * h.nbr[1] + by) * h.nbr[0] + bx]);
+      const double c3[3] = {X[i], Y[i], Z[i]};
+      int t0[3], nt[3];
+      for (int d = 0; d < 3; d++) {
+        int b = ((int)std::ceil(h.kg[d] * c3[d]) - h.order) % h.K[d];
+        b += b < 0 ? h.K[d] : 0;
+        t0[d] = b / h.tsz[d];
+        nt[d] = ((b + h.order - 1) % h.K[d]) / h.tsz[d] != t0[d] ? 2 : 1;
+      }
+      for (int c = 0; c < nt[2]; c++)
+        for (int b = 0; b < nt[1]; b++)
+          for (int a = 0; a < nt[0]; a++) {
+            const int t = (((t0[2] + c) % h.ntl[2]) * h.ntl[1] + (t0[1] + b) % h.ntl[1]) * h.ntl[0] +
+                          (t0[0] + a) % h.ntl[0];
+            mb = std::max(mb, ++cnt[t]);
+          }
     }
-    const int capb = round8(std::min(h.Nq, 2 * mb + 32));
-    if (capb != h.capb) {
-      if (h.bmem) dfree(h.bmem, (size_t)nbin * h.capb * 4);
-      h.bmem = dalloc<int>((size_t)nbin * capb);
-      h.capb = capb;
-      if (!h.bmem) return fail(MD_ENOMEM, "mesh bins");
+    const int capt = round8(std::min(h.Nq, 2 * mb + 64));
+    if (capt != h.capt) {
+      if (h.tmem) dfree(h.tmem, (size_t)nbin * h.capt * 4);
+      h.tmem = dalloc<int>((size_t)nbin * capt);
+      h.capt = capt;
+      if (!h.tmem) return fail(MD_ENOMEM, "spreading tiles");
     }
   }
   int np3 = 1;
@@ -485,7 +500,9 @@ int Sys::size_lists(const double *pos) {
   nb[10] = h.real_cells ? std::min(ncell3 * cdiv(h.ccap3, mdk::kRealChunk), 1024) : 0;
   if (ccap1 != h.ccap1) {
     if (h.mem1) dfree(h.mem1, (size_t)ncell1 * h.ccap1 * 4);
+    if (h.mem1s) dfree(h.mem1s, (size_t)ncell1 * h.ccap1 * 4);
     h.mem1 = dalloc<int>((size_t)ncell1 * ccap1);
+    h.mem1s = dalloc<int>((size_t)ncell1 * ccap1);
     h.ccap1 = ccap1;
   }
   if (nbcap1 != h.nbcap1) {
@@ -503,7 +520,7 @@ int Sys::size_lists(const double *pos) {
     h.nb2 = dalloc<int>((size_t)h.Nq * nbcap2);
     h.nbcap2 = nbcap2;
   }
-  if (!h.mem1 || !h.nb1 || (h.real_verlet && (!h.mem2 || !h.nb2)))
+  if (!h.mem1 || !h.mem1s || !h.nb1 || (h.real_verlet && (!h.mem2 || !h.nb2)))
     return fail(MD_ENOMEM, "neighbour lists");
   int np = 1;
   while (np < h.ccap2) np <<= 1;
@@ -526,6 +543,8 @@ struct md_engine {
   int reps_cap = 0;
   int *d_maps = nullptr;
   int maps_cap = 0;
+  unsigned long long *d_lq = nullptr;  // LJ rebuild queues: the batch's, then one per replica
+  size_t lq_words = 0;
   double *vpart = nullptr;
   size_t vpart_cap = 0;
   int *h_steps = nullptr;          // pinned, one per replica
@@ -533,6 +552,7 @@ struct md_engine {
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
   bool dirty = true;
+  bool batch = false;              // made by md_batch_create (replicas pre-heat)
   std::string err;
 
   ~md_engine() {
@@ -541,6 +561,7 @@ struct md_engine {
     for (Sys *s : sys) delete s;
     if (d_reps) (void)hipFree(d_reps);
     if (d_maps) (void)hipFree(d_maps);
+    if (d_lq) (void)hipFree(d_lq);
     if (vpart) (void)hipFree(vpart);
     if (h_steps) (void)hipHostFree(h_steps);
     if (st) (void)hipStreamDestroy(st);
@@ -637,6 +658,19 @@ int md_engine::rebuild() {
     if (hipHostMalloc((void **)&h_steps, steps_cap * sizeof(int)) != hipSuccess)
       return fail(MD_ENOMEM, "pinned");
   }
+  // rebuild queues: 2 header words + rep/start ints per entry
+  const size_t need_lq = (2 + (size_t)R) + (size_t)R * 3;
+  if (need_lq > lq_words) {
+    if (d_lq) (void)hipFree(d_lq);
+    lq_words = 2 * need_lq;
+    if (hipMalloc(&d_lq, lq_words * 8) != hipSuccess) return fail(MD_ENOMEM, "rebuild queue");
+  }
+  e = hipMemset(d_lq, 0, lq_words * 8);
+  if (e != hipSuccess) return hip_fail(e, "rebuild queue");
+  auto grids = [](Plan &P, long nn) {
+    P.grid_bin = (int)std::max(1L, std::min<long>(cdiv(nn, mdk::kBlock), 2048));
+    P.grid_build = (int)std::max(1L, std::min<long>(cdiv(nn, 8), 2048));
+  };
   auto fill = [&](Plan &P, Rep *reps, int nrep, const int *maps, const int *totals) {
     P.reps = reps;
     P.nrep = nrep;
@@ -647,6 +681,11 @@ int md_engine::rebuild() {
     P.fp32 = fp32;
   };
   fill(plan, d_reps, R, d_maps, tot);
+  long nn_all = 0;
+  for (Sys *s : sys) nn_all += s->p.NN;
+  plan.lq = d_lq;
+  plan.lq_cap = R;
+  grids(plan, nn_all);
   plan.lds_x = plan.lds_y = plan.lds_z = plan.lds_sort = plan.lds_sort3 = plan.lds_plane = 0;
   plan.pairs_any = false;
   for (Sys *s : sys) {
@@ -661,6 +700,9 @@ int md_engine::rebuild() {
   single.assign(R, Plan{});
   for (int r = 0; r < R; r++) {
     fill(single[r], d_reps + r, 1, d_maps + single_off + r * kMaps * 2, sys[r]->nb);
+    single[r].lq = d_lq + (2 + R) + 3 * (size_t)r;
+    single[r].lq_cap = 1;
+    grids(single[r], sys[r]->p.NN);
     single[r].lds_x = sys[r]->lds_x;
     single[r].lds_y = sys[r]->lds_y;
     single[r].lds_z = sys[r]->lds_z;
@@ -704,9 +746,9 @@ int md_engine::check_one(const Sys *s, const Ctrl &c, std::string &msg) const {
   if (c.ovf) {
     std::snprintf(buf, sizeof buf,
                   "capacity exceeded (mask %d): LJ cell %d/%d list %d/%d, real-space cell %d/%d, "
-                  "real list %d/%d, mesh bin capacity %d",
+                  "real list %d/%d, spreading tile %d",
                   c.ovf, c.max_cell1, s->h.ccap1, c.max_nb1, s->h.nbcap1, c.max_cell2, s->h.ccap3,
-                  c.max_nb2, s->h.nbcap2, s->h.capb);
+                  c.max_nb2, s->h.nbcap2, s->h.capt);
     msg = buf;
     return MD_EOVERFLOW;
   }
@@ -737,6 +779,7 @@ int md_engine::add(const md_params *p, int *slot) {
     delete s;
     return rc;
   }
+  s->h.preheat_end = batch ? std::max(0, p->StepNum0) : 0;
   sys.push_back(s);
   *slot = next_slot++;
   slots[*slot] = s;
@@ -784,8 +827,11 @@ int md_engine::build_graph() {
   if (e != hipSuccess) return hip_fail(e, "begin capture");
   const int mode = mdk::SR_HELD | mdk::SR_FIELD | mdk::SR_ANCHOR | mdk::SR_NOISE_PHILOX |
                    mdk::SR_KICK | mdk::SR_DRMAX;
-  bool any_coulomb = false;
-  for (Sys *s : sys) any_coulomb |= s->coulomb;
+  bool any_coulomb = false, any_preheat = false;
+  for (Sys *s : sys) {
+    any_coulomb |= s->coulomb;
+    any_preheat |= s->h.preheat_end > 0;
+  }
   for (int s = 1; s <= multistep; s++) {
     mdk::launch_kick_drift(plan, st);
     if (any_coulomb && s == multistep) {
@@ -795,6 +841,9 @@ int md_engine::build_graph() {
     if (fp32) mdk::launch_short<float>(plan, st, mode, -1, nullptr, nullptr);
     else mdk::launch_short<double>(plan, st, mode, -1, nullptr, nullptr);
     mdk::launch_lists(plan, st, 1);
+    // pre-heat rescale between steps (its step is a multiple of 20, so of
+    // multistep when that divides 20; otherwise checked after every step)
+    if (any_preheat && (20 % multistep != 0 || s == multistep)) mdk::launch_rescale(plan, st, 1);
   }
   e = hipStreamEndCapture(st, &graph);
   if (e != hipSuccess) return hip_fail(e, "end capture");
@@ -808,8 +857,11 @@ int md_engine::build_graph() {
 static void launch_cycle(md_engine *e) {
   const int mode = mdk::SR_HELD | mdk::SR_FIELD | mdk::SR_ANCHOR | mdk::SR_NOISE_PHILOX |
                    mdk::SR_KICK | mdk::SR_DRMAX;
-  bool any_coulomb = false;
-  for (Sys *s : e->sys) any_coulomb |= s->coulomb;
+  bool any_coulomb = false, any_preheat = false;
+  for (Sys *s : e->sys) {
+    any_coulomb |= s->coulomb;
+    any_preheat |= s->h.preheat_end > 0;
+  }
   for (int s = 1; s <= e->multistep; s++) {
     mdk::launch_kick_drift(e->plan, e->st);
     if (any_coulomb && s == e->multistep) {
@@ -819,6 +871,8 @@ static void launch_cycle(md_engine *e) {
     if (e->fp32) mdk::launch_short<float>(e->plan, e->st, mode, -1, nullptr, nullptr);
     else mdk::launch_short<double>(e->plan, e->st, mode, -1, nullptr, nullptr);
     mdk::launch_lists(e->plan, e->st, 1);
+    if (any_preheat && (20 % e->multistep != 0 || s == e->multistep))
+      mdk::launch_rescale(e->plan, e->st, 1);
   }
 }
 
@@ -1062,12 +1116,23 @@ extern "C" int md_run(md_engine *e, int32_t step0, int32_t nsteps, int32_t prehe
     }
     return md_update_verlet_list(e);
   };
-  if (preheat) {
-    for (; step < end; step++) {
-      int rc = one(step);
-      if (rc) return rc;
+  // pre-heat: the replayed cycles rescale through the gated kernels, up to
+  // this call's last step; the descriptor goes back to no pre-heat after
+  struct Restore {
+    md_engine *e;
+    Sys *s;
+    bool on;
+    ~Restore() {
+      if (on) {
+        s->h.preheat_end = 0;
+        e->dirty = true;
+      }
     }
-    return e->check_all();
+  } restore{e, s, preheat != 0};
+  if (preheat) {
+    s->h.preheat_end = end - 1;
+    int rc = e->rebuild();
+    if (rc) return rc;
   }
   const int ms = e->multistep;
   while (step < end && (step - 1) % ms != 0) {
@@ -1164,6 +1229,7 @@ extern "C" int md_batch_create(md_engine **out, int32_t precision, int32_t multi
   if (!out) return MD_EINVAL;
   md_engine *e = new md_engine();
   *out = e;
+  e->batch = true;
   return e->init(precision, multistep);
 }
 

@@ -26,6 +26,11 @@ struct Plan {
   WMap part{}, chg{}, chgG{}, fx{}, fy{}, fz{}, cell1{}, cell1w{}, cell2{}, cell3{}, cell3c{},
       brick{}, plane{};
   size_t lds_x = 0, lds_y = 0, lds_z = 0, lds_sort = 0, lds_sort3 = 0, lds_plane = 0;
+  // LJ rebuild queue (replicas flagged this step) and the fixed grids of the
+  // list kernels, which stride over the queued replicas' particles
+  unsigned long long *lq = nullptr;
+  int lq_cap = 0;
+  int grid_bin = 1, grid_build = 1;
   bool pairs_any = false;   // some replica uses the all-pairs real space
   double *vpart = nullptr;  // partial sums of rescale_velocity
   bool fp32 = false;
@@ -37,10 +42,14 @@ template <typename R>
 void launch_short(const Plan &P, hipStream_t st, int mode, int step, const double *eta,
                   double *out);
 void launch_kick_drift(const Plan &P, hipStream_t st);
-void launch_lists(const Plan &P, hipStream_t st, int inc_step, hipEvent_t *ev = nullptr);
+// clear_queue: empty the rebuild queue afterwards (list updates outside the
+// replayed step cycle, where the next kick does it)
+void launch_lists(const Plan &P, hipStream_t st, int inc_step, hipEvent_t *ev = nullptr,
+                  bool clear_queue = false);
 void launch_real_list(const Plan &P, hipStream_t st);
 void launch_set_flags(const Plan &P, hipStream_t st, int clear);
-void launch_rescale(const Plan &P, hipStream_t st);
+// gated: only batch replicas whose pre-heat rule is due (main.f90:67-75)
+void launch_rescale(const Plan &P, hipStream_t st, int gated = 0);
 void launch_observe(const Plan &P, hipStream_t st, int *hist, double *hsum);
 
 }  // namespace mdk

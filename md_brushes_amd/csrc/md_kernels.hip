@@ -32,6 +32,55 @@ __device__ __forceinline__ void set_error(Ctrl *c, int code, int info, int step)
   }
 }
 
+// Slot of this lane's entry in the list counted by *ctr (null: no entry).
+// Runs of adjacent lanes with the same counter (neighbouring particles along
+// a chain mostly share their cell) take one atomic per run; the runs' atomics
+// are issued together.  Every lane of the wavefront calls it.
+__device__ __forceinline__ int wave_append(int *ctr) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long me = (unsigned long long)ctr;
+  const unsigned long long prev = __shfl_up(me, 1, 64);
+  const bool head = lane == 0 || prev != me;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long le = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+  const int h = 63 - __clzll((long long)(heads & le));  // head of this lane's run
+  const unsigned long long after = heads & ~le;
+  const int next = after ? __ffsll((long long)after) - 1 : 64;  // first head past this lane
+  int base = 0;
+  if (head && ctr) base = atomicAdd(ctr, next - lane);
+  base = __shfl(base, h, 64);
+  return ctr ? base + (lane - h) : -1;
+}
+
+// LJ rebuild queue: the replicas whose list is rebuilt after this step and
+// the range [start, start + NN) each takes in the global particle index the
+// list kernels stride over.  One 64-bit head packs (entries << 40) | total,
+// so that one atomic hands out the entry and its range in the same order;
+// a spare 64-bit word, then rep[cap] and start[cap].
+constexpr int kQShift = 40;
+__device__ __forceinline__ int *lq_rep(unsigned long long *lq) { return (int *)(lq + 2); }
+__device__ __forceinline__ int *lq_start(unsigned long long *lq, int cap) {
+  return (int *)(lq + 2) + cap;
+}
+__device__ __forceinline__ void lq_push(unsigned long long *lq, int cap, int r, int nn) {
+  const unsigned long long old = atomicAdd(lq, (1ull << kQShift) + (unsigned long long)nn);
+  const int e = (int)(old >> kQShift);
+  if (e < cap) {
+    lq_rep(lq)[e] = r;
+    lq_start(lq, cap)[e] = (int)(old & ((1ull << kQShift) - 1));
+  }
+}
+// Entry of global particle index g: the largest e with start[e] <= g.
+__device__ __forceinline__ int lq_find(const int *start, int n, int g) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 // Squared distance with the x,y minimum image of rij_and_rr
 // (glob_var.f90:72-105), rounded term by term as the reference writes it.
 __device__ __forceinline__ double rsq_image(double4 a, double4 b, double Lx, double Ly,
@@ -62,8 +111,10 @@ __device__ __forceinline__ double wave_sum(double v) {
 // First half of new_position (init_update.f90:581-601) and
 // period_condition_pos (init_update.f90:631-642): half kick, drift, wrap,
 // maximum displacement of this step (all, charged).
-__global__ __launch_bounds__(kBlock) void k_kick_drift(const Rep *__restrict__ reps, WMap wm) {
+__global__ __launch_bounds__(kBlock) void k_kick_drift(const Rep *__restrict__ reps, WMap wm,
+                                                       unsigned long long *lq) {
 #pragma clang fp contract(off)
+  if (blockIdx.x == 0 && threadIdx.x == 0) *lq = 0ull;  // this step's LJ rebuild queue
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
   const int i = (blockIdx.x - wm.blk[r]) * kBlock + threadIdx.x;
@@ -122,7 +173,8 @@ template <typename R>
 __global__ __launch_bounds__(kBlock) void k_short_range(const Rep *__restrict__ reps, WMap wm,
                                                         int mode, int step_arg,
                                                         const double *__restrict__ eta,
-                                                        double *__restrict__ out) {
+                                                        double *__restrict__ out,
+                                                        unsigned long long *lq, int lq_cap) {
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
   const int blk = blockIdx.x - wm.blk[r];
@@ -138,7 +190,11 @@ __global__ __launch_bounds__(kBlock) void k_short_range(const Rep *__restrict__ 
     c->dr_max2 = c->dr_max2 + d2;
     int f1 = c->dr_max1 > S.skin1;
     int f2 = (c->dr_max2 > S.skin2) && S.real_verlet != 0;
-    if (f1) { c->dr_max1 = 0; c->gen1++; }
+    if (f1) {
+      c->dr_max1 = 0;
+      c->gen1++;
+      lq_push(lq, lq_cap, r, S.NN);
+    }
     if (f2) { c->dr_max2 = 0; c->gen2++; }
     c->flag1 = f1;
     c->flag2 = f2;
@@ -292,17 +348,24 @@ __global__ __launch_bounds__(kBlock) void k_short_range(const Rep *__restrict__ 
 
 // ------------------------------------------------------- Verlet lists
 
-__global__ void k_set_flags(const Rep *__restrict__ reps, int nrep, int clear) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrep) return;
-  Ctrl *c = reps[r].ctrl;
-  if (clear) {
-    c->flag1 = c->flag2 = 0;
-    return;
+// Forces (clear = 0) or clears (clear = 1) the rebuild flags of every
+// replica; forcing also queues every replica's LJ rebuild.  One workgroup.
+__global__ void k_set_flags(const Rep *__restrict__ reps, int nrep, int clear,
+                            unsigned long long *lq, int cap) {
+  if (threadIdx.x == 0) *lq = 0ull;
+  __syncthreads();
+  for (int r = threadIdx.x; r < nrep; r += blockDim.x) {
+    Ctrl *c = reps[r].ctrl;
+    if (clear) {
+      c->flag1 = c->flag2 = 0;
+      continue;
+    }
+    c->flag1 = 1;
+    c->gen1++;
+    lq_push(lq, cap, r, reps[r].NN);
+    c->flag2 = reps[r].real_verlet != 0 && reps[r].Nq > 0;
+    if (c->flag2) c->gen2++;
   }
-  c->flag1 = 1;
-  c->flag2 = reps[r].real_verlet != 0 && reps[r].Nq > 0;
-  if (c->flag2) c->gen2++;
 }
 
 // Cell of a particle as lj_verlet_list computes it (compt_acc.f90:219-222).
@@ -318,96 +381,85 @@ __device__ __forceinline__ bool cell_of(double4 p, const int nc[3], const double
 // lj_verlet_list (compt_acc.f90:232-255): x outer, y, z inner; x,y periodic,
 // z open (-1 when outside).
 __device__ __forceinline__ int stencil_cell(int l, int a, int b, int c, const int nc[3]) {
-  int p = a + l / 9 - 1, q = b + (l / 3) % 3 - 1, z = c + l % 3 - 1;
+  const int dp = l / 9 - 1, dq = (l / 3) % 3 - 1;
+  // fewer than three cells along a periodic axis: each cell once
+  if ((nc[0] == 1 && dp != 0) || (nc[0] == 2 && dp < 0)) return -1;
+  if ((nc[1] == 1 && dq != 0) || (nc[1] == 2 && dq < 0)) return -1;
+  int p = a + dp, q = b + dq, z = c + l % 3 - 1;
   if (z < 0 || z >= nc[2]) return -1;
   p = p >= nc[0] ? p - nc[0] : (p < 0 ? p + nc[0] : p);
   q = q >= nc[1] ? q - nc[1] : (q < 0 ? q + nc[1] : q);
   return (z * nc[1] + q) * nc[0] + p;
 }
 
-// LJ cells, binning pass of lj_verlet_list (compt_acc.f90:219-225).
-__global__ __launch_bounds__(kBlock) void k_bin_lj(const Rep *__restrict__ reps, WMap wm) {
-  const int r = find_rep(wm, blockIdx.x);
-  const Rep &S = reps[r];
-  Ctrl *c = S.ctrl;
-  if (!c->flag1) return;
-  const int i = (blockIdx.x - wm.blk[r]) * kBlock + threadIdx.x;
-  if (i >= S.NN) return;
-  int a, b, cz;
-  if (!cell_of(S.pos[i], S.nc1, S.rcel1, S.hLx, S.hLy, a, b, cz)) {
-    set_error(c, MD_ECELL, i, -1);
-    return;
-  }
-  const int cell = (cz * S.nc1[1] + b) * S.nc1[0] + a;
-  const int slot = atomicAdd(&S.cnt1[cell], 1);
-  if (slot < S.ccap1) S.mem1[(size_t)cell * S.ccap1 + slot] = i;
-  else atomicOr(&c->ovf, 1);
-}
+// LJ list rebuild (lj_verlet_list, compt_acc.f90:185-314) of the queued
+// replicas in three particle-parallel passes: bin, rank within the cell,
+// fill.  The kernels stride over the queue's global particle index, so the
+// work is proportional to the particles rebuilt, whatever the cell count.
 
-// One compare-exchange stage of a bitonic network whose element e of lane
-// `lane` is in register v (partner e ^ stride in lane ^ stride).
-__device__ __forceinline__ int bitonic_step(int v, int e, int stride, int size) {
-  const int pv = __shfl_xor(v, stride, 64);
-  const bool lower = (e & stride) == 0, up = (e & size) == 0;
-  return (lower == up) ? min(v, pv) : max(v, pv);
-}
-
-// Bitonic sort of up to 128 ints held two per lane (elements lane and
-// lane + 64), by shuffles: no LDS, no barriers.  n <= 64 sorts v0 only.
-__device__ __forceinline__ void wave_sort128(int &v0, int &v1, int n) {
-  const int lane = threadIdx.x & 63;
-  if (n <= 64) {
-    for (int size = 2; size <= 64; size <<= 1)
-      for (int stride = size >> 1; stride > 0; stride >>= 1) v0 = bitonic_step(v0, lane, stride, size);
-    return;
-  }
-  for (int size = 2; size <= 128; size <<= 1)
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      if (stride == 64) {  // the two values of one lane; size 128 sorts ascending
-        const int lo = min(v0, v1), hi = max(v0, v1);
-        v0 = lo;
-        v1 = hi;
+// Binning pass (compt_acc.f90:219-225): cell of each particle, appended to
+// the cell's members (order fixed by the rank pass).  Also clears the
+// other count buffer at the particle's previous cell, for the next build.
+__global__ __launch_bounds__(kBlock) void k_bin_lj(const Rep *__restrict__ reps,
+                                                   unsigned long long *lq, int cap) {
+  const unsigned long long head = *lq;
+  const int ne = min((int)(head >> kQShift), cap);
+  const int total = (int)(head & ((1ull << kQShift) - 1));
+  const int *qrep = lq_rep(lq), *qst = lq_start(lq, cap);
+  for (int base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
+    const int g = base + threadIdx.x;
+    int *ctr = nullptr;
+    int e = 0, i = 0, cell = -1;
+    if (g < total) {
+      e = lq_find(qst, ne, g);
+      const Rep &S = reps[qrep[e]];
+      i = g - qst[e];
+      const int ncell = S.nc1[0] * S.nc1[1] * S.nc1[2];
+      const int par = S.ctrl->gen1 & 1;
+      const int old = S.cell1_of[i];
+      if (old >= 0) S.cnt1[(size_t)(par ^ 1) * ncell + old] = 0;
+      int a, b, cz;
+      if (cell_of(S.pos[i], S.nc1, S.rcel1, S.hLx, S.hLy, a, b, cz)) {
+        cell = (cz * S.nc1[1] + b) * S.nc1[0] + a;
+        ctr = S.cnt1 + (size_t)par * ncell + cell;
       } else {
-        v0 = bitonic_step(v0, lane, stride, size);
-        v1 = bitonic_step(v1, lane + 64, stride, size);
+        set_error(S.ctrl, MD_ECELL, i, -1);
       }
+      S.cell1_of[i] = cell;
     }
+    const int slot = wave_append(ctr);
+    if (ctr) {
+      const Rep &S = reps[qrep[e]];
+      if (slot < S.ccap1) S.mem1[(size_t)cell * S.ccap1 + slot] = i;
+      else atomicOr(&S.ctrl->ovf, 1);
+    }
+  }
 }
 
-// Sorts each LJ cell's members ascending (so that lists, and every force
-// sum over them, do not depend on the order of the binning atomics), keeps
-// the sorted count and clears the binning count.  One wavefront per cell;
-// the replica's wavefronts stride over its cells.
-__global__ __launch_bounds__(kBlock) void k_sort_lj(const Rep *__restrict__ reps, WMap wm) {
-  const int r = find_rep(wm, blockIdx.x);
-  const Rep &S = reps[r];
-  Ctrl *c = S.ctrl;
-  if (!c->flag1) return;
-  const int lane = threadIdx.x & 63;
-  const int nw = (wm.blk[r + 1] - wm.blk[r]) * (kBlock / 64);
-  const int ncell = S.nc1[0] * S.nc1[1] * S.nc1[2];
-  int maxraw = 0;
-  for (int cell = (blockIdx.x - wm.blk[r]) * (kBlock / 64) + (threadIdx.x >> 6); cell < ncell;
-       cell += nw) {
-    const int raw = S.cnt1[cell];
-    if (raw == 0) {
-      if (lane == 0) S.scnt1[cell] = 0;
-      continue;
-    }
+// Rank pass: each particle's place among its cell's members in ascending
+// index order (members are distinct), so that lists and every force sum
+// over them do not depend on the order of the binning atomics.
+__global__ __launch_bounds__(kBlock) void k_sort_lj(const Rep *__restrict__ reps,
+                                                    unsigned long long *lq, int cap) {
+  const unsigned long long head = *lq;
+  const int ne = min((int)(head >> kQShift), cap);
+  const int total = (int)(head & ((1ull << kQShift) - 1));
+  const int *qrep = lq_rep(lq), *qst = lq_start(lq, cap);
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
+    const int e = lq_find(qst, ne, g);
+    const Rep &S = reps[qrep[e]];
+    const int i = g - qst[e];
+    const int cell = S.cell1_of[i];
+    if (cell < 0) continue;
+    const int ncell = S.nc1[0] * S.nc1[1] * S.nc1[2];
+    const int raw = S.cnt1[(size_t)(S.ctrl->gen1 & 1) * ncell + cell];
     const int n = min(raw, S.ccap1);
-    int *m = S.mem1 + (size_t)cell * S.ccap1;
-    int v0 = lane < n ? m[lane] : 0x7fffffff;
-    int v1 = lane + 64 < n ? m[lane + 64] : 0x7fffffff;
-    if (n > 1) wave_sort128(v0, v1, n);
-    if (lane < n) m[lane] = v0;
-    if (lane + 64 < n) m[lane + 64] = v1;
-    if (lane == 0) {
-      S.scnt1[cell] = n;
-      S.cnt1[cell] = 0;
-    }
-    maxraw = max(maxraw, raw);
+    const int *m = S.mem1 + (size_t)cell * S.ccap1;
+    int rank = 0;
+    for (int k = 0; k < n; k++) rank += m[k] < i;
+    if (rank < n) S.mem1s[(size_t)cell * S.ccap1 + rank] = i;
+    if (rank == 0 && raw > S.ctrl->max_cell1) atomicMax(&S.ctrl->max_cell1, raw);
   }
-  if (lane == 0 && maxraw) atomicMax(&c->max_cell1, maxraw);
 }
 
 // Shared 27-cell candidate set of one cell: offs[l] = first candidate of
@@ -443,87 +495,98 @@ __device__ __forceinline__ int stencil_slot(const int *offs, int t) {
   return l;
 }
 
-constexpr int kLjTile = 1024;  // candidates per LDS tile of k_build_lj
-
-// Fill pass of lj_verlet_list (compt_acc.f90:271-313) cell by cell: the
-// members of a cell's 27 stencil cells (reference cell order, ascending
-// within a cell) are staged in LDS tiles shared by all targets of the cell;
-// each wavefront takes one target at a time, tests 64 candidates per
-// iteration and appends hits in order with a ballot.  Pairs: i != j and
-// r^2 < rvl^2, r^2 rounded as rij_and_rr rounds it.  The list is ELL: slot
-// k of particle i at k*NN + i.  The replica's workgroups stride over its
-// cells.  The last kernel of a step also advances the step counter.
-__global__ __launch_bounds__(kBlock) void k_build_lj(const Rep *__restrict__ reps, WMap wm,
+// Fill pass of lj_verlet_list (compt_acc.f90:271-313): 32 lanes per target
+// particle.  The candidates are the members of the 27 stencil cells in the
+// reference order (stencil order, ascending within a cell); the lanes test
+// 32 of them per round and append the hits in that order with a ballot.
+// Pairs: i != j and r^2 < rvl^2, r^2 rounded as rij_and_rr rounds it.  The
+// list is ELL: slot k of particle i at k*NN + i.  The last kernel of a step
+// also advances every replica's step counter.
+__global__ __launch_bounds__(kBlock) void k_build_lj(const Rep *__restrict__ reps, int nrep,
+                                                     unsigned long long *lq, int cap,
                                                      int inc_step) {
-  const int r = find_rep(wm, blockIdx.x);
-  const Rep &S = reps[r];
-  Ctrl *c = S.ctrl;
-  const int b0 = blockIdx.x - wm.blk[r], nb = wm.blk[r + 1] - wm.blk[r];
-  if (inc_step && b0 == 0 && threadIdx.x == 0) c->step++;
-  if (!c->flag1) return;
-  __shared__ int offs[28], cids[27];
-  __shared__ double cx[kLjTile], cy[kLjTile], cz[kLjTile];
-  __shared__ int cj[kLjTile];
-  __shared__ int tcount[128];
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int cap = S.nbcap1;
-  const int ncell = S.nc1[0] * S.nc1[1] * S.nc1[2];
-  int maxn = 0;
-  for (int cell = b0; cell < ncell; cell += nb) {
-    const int nt = S.scnt1[cell];
-    if (nt == 0) continue;
-    const int total = stencil_offsets(cell, S.nc1, S.scnt1, offs, cids);
-    for (int k = threadIdx.x; k < nt; k += blockDim.x) tcount[k] = 0;
-    const int *tm = S.mem1 + (size_t)cell * S.ccap1;
-    for (int base = 0; base < total; base += kLjTile) {
-      const int nc = min(kLjTile, total - base);
-      __syncthreads();  // previous tile fully used, tcount zeroed
-      for (int k = threadIdx.x; k < nc; k += blockDim.x) {
-        const int t = base + k, l = stencil_slot(offs, t);
-        const int j = S.mem1[(size_t)cids[l] * S.ccap1 + (t - offs[l])];
-        const double4 pj = S.pos[j];
-        cx[k] = pj.x;
-        cy[k] = pj.y;
-        cz[k] = pj.z;
-        cj[k] = j;
-      }
-      __syncthreads();
-      for (int tg = w; tg < nt; tg += kBlock / 64) {
-        const int i = tm[tg];
-        const double4 pi = S.pos[i];
-        int n = tcount[tg];
-        for (int k0 = 0; k0 < nc; k0 += 64) {
-          const int k = k0 + lane;
-          bool hit = false;
-          int j = 0;
-          if (k < nc) {
-            j = cj[k];
-            double dx, dy, dz;
-            hit = j != i &&
-                  rsq_image(pi, make_double4(cx[k], cy[k], cz[k], 0.0), S.Lx, S.Ly, S.hLx, S.hLy, dx,
-                            dy, dz) < S.rv1sq;
-          }
-          const unsigned long long mask = __ballot(hit);
-          const int rank = __popcll(mask & ((1ull << lane) - 1ull));
-          if (hit && n + rank < cap) S.nb1[(size_t)(n + rank) * S.NN + i] = j;
-          n += __popcll(mask);
-        }
-        if (lane == 0) tcount[tg] = n;
+  if (inc_step && blockIdx.x == 0)
+    for (int r = threadIdx.x; r < nrep; r += blockDim.x) reps[r].ctrl->step++;
+  const unsigned long long head = *lq;
+  const int ne = min((int)(head >> kQShift), cap);
+  const int total = (int)(head & ((1ull << kQShift) - 1));
+  const int *qrep = lq_rep(lq), *qst = lq_start(lq, cap);
+  const int lane = threadIdx.x & 63, g = lane & 31, half = lane >> 5;
+  const unsigned long long gmask = half ? 0xffffffff00000000ull : 0xffffffffull;
+  const unsigned long long below = gmask & ((1ull << lane) - 1ull);
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int stride = gridDim.x * blockDim.x / 32;
+  for (int t0 = 2 * wave; t0 < total; t0 += stride) {
+    const int gi = t0 + half;
+    bool live = gi < total;
+    const Rep *Sp = reps;
+    int i = 0, a = 0, b = 0, cz = 0;
+    double4 pi = make_double4(0, 0, 0, 0);
+    if (live) {
+      const int e = lq_find(qst, ne, gi);
+      Sp = reps + qrep[e];
+      i = gi - qst[e];
+      pi = Sp->pos[i];
+      const int cell = Sp->cell1_of[i];
+      live = cell >= 0;
+      if (live) {
+        a = cell % Sp->nc1[0];
+        b = (cell / Sp->nc1[0]) % Sp->nc1[1];
+        cz = cell / (Sp->nc1[0] * Sp->nc1[1]);
       }
     }
-    __syncthreads();
-    for (int tg = threadIdx.x; tg < nt; tg += blockDim.x) {
-      const int n = tcount[tg];
-      S.nbn1[tm[tg]] = n;
-      maxn = max(maxn, n);
+    const Rep &S = *Sp;
+    // stencil cell of lane g (< 27), its count, exclusive offsets in the group
+    int cl = -1, cn = 0;
+    if (live && g < 27) {
+      cl = stencil_cell(g, a, b, cz, S.nc1);
+      const int ncell = S.nc1[0] * S.nc1[1] * S.nc1[2];
+      if (cl >= 0) cn = min(S.cnt1[(size_t)(S.ctrl->gen1 & 1) * ncell + cl], S.ccap1);
     }
-    __syncthreads();  // offs, cids and tcount are reused by the next cell
-  }
-  if (maxn) {
-    if (maxn > cap) atomicOr(&c->ovf, 4);
-    atomicMax(&c->max_nb1, maxn);
+    int incl = cn;
+    for (int o = 1; o < 32; o <<= 1) {
+      const int v = __shfl_up(incl, o, 32);
+      if (g >= o) incl += v;
+    }
+    const int excl = incl - cn;
+    const int tot = __shfl(incl, 26, 32);
+    const int tmax = max(tot, __shfl_xor(tot, 32, 64));
+    int n = 0;
+    for (int k0 = 0; k0 < tmax; k0 += 32) {
+      const int t = k0 + g;
+      // stencil cell holding candidate t: the largest l with excl_l <= t
+      int l = 0;
+      for (int s = 16; s > 0; s >>= 1) {
+        const int c = l + s;
+        const int ex = __shfl(excl, c < 27 ? c : 26, 32);
+        if (c < 27 && ex <= t) l = c;
+      }
+      const int lcell = __shfl(cl, l, 32), lex = __shfl(excl, l, 32);
+      bool hit = false;
+      int j = 0;
+      if (t < tot) {
+        j = S.mem1s[(size_t)lcell * S.ccap1 + (t - lex)];
+        double dx, dy, dz;
+        hit = j != i && rsq_image(pi, S.pos[j], S.Lx, S.Ly, S.hLx, S.hLy, dx, dy, dz) < S.rv1sq;
+      }
+      const unsigned long long mask = __ballot(hit);
+      if (hit) {
+        const int slot = n + __popcll(mask & below);
+        if (slot < S.nbcap1) S.nb1[(size_t)slot * S.NN + i] = j;
+      }
+      n += __popcll(mask & gmask);
+    }
+    if (live && g == 0) {
+      S.nbn1[i] = n;
+      if (n > S.nbcap1) atomicOr(&S.ctrl->ovf, 4);
+      if (n > S.ctrl->max_nb1) atomicMax(&S.ctrl->max_nb1, n);
+    }
   }
 }
+
+// Empties a rebuild queue (after a list update outside the step cycle, whose
+// next kick would empty it otherwise).
+__global__ void k_lq_clear(unsigned long long *lq) { *lq = 0ull; }
 
 // The reference-form real-space list, real_verlet_list
 // (compt_acc.f90:491-626): cells of edge >= r_c + skin over the charges,
@@ -630,8 +693,10 @@ __global__ __launch_bounds__(kBlock) void k_build_rl(const Rep *__restrict__ rep
 // B-spline weights of splcof (compt_acc.f90:1250-1321).  Returns the mesh
 // index of weight 0; w[t], dw[t] for t = 0..p-1 are SP(:,t+1), dSP(:,t+1):
 // weight t sits at grid point uuc - p + t.
-template <typename R>
-__device__ __forceinline__ int spline(double coord, double kg, int p, R *w, R *dw) {
+// O > 0 fixes the order at compile time (all loops unroll); O = 0 takes p_.
+template <typename R, int O = 0>
+__device__ __forceinline__ int spline(double coord, double kg, int p_, R *w, R *dw) {
+  const int p = O > 0 ? O : p_;
   const double uu = kg * coord;
   const double uuc = ceil(uu);
   const R ww = (R)(uu - uuc + 1);
@@ -686,84 +751,83 @@ __device__ __forceinline__ int spline_base(double coord, double kg, int p, int K
   return wrapk((int)ceil(kg * coord) - p, K);
 }
 
-// MapCharges (compt_acc.f90:1324-1365) without atomics: one workgroup (one
-// wavefront) per mesh brick.  The charges whose first spline point lies in
-// the brick's bin or in the bins below it along each axis are staged in LDS
-// with their weights; each lane owns mesh points of the brick and adds the
-// products that land on them.  Sums are kept in 2^-40 fixed point, so the
-// result does not depend on the order of the charges in a bin; every mesh
-// point is written once.
-template <typename R>
-__global__ __launch_bounds__(64) void k_spread(const Rep *__restrict__ reps, WMap wm) {
-  const int r = find_rep(wm, blockIdx.x);
-  const Rep &S = reps[r];
-  const int brick = blockIdx.x - wm.blk[r];
-  const int lane = threadIdx.x;
-  const int o = S.order;
-  const int bx = brick % S.nbr[0], by = (brick / S.nbr[0]) % S.nbr[1], bz = brick / (S.nbr[0] * S.nbr[1]);
-  const int Bx = S.bsz[0], By = S.bsz[1], Bz = S.bsz[2];
-  const int npt = Bx * By * Bz;  // <= 512: at most 8 points per lane
-  const int kpt = (npt - lane + 63) >> 6;  // points of this lane
-  const int K1 = S.K[0], K2 = S.K[1], K3 = S.K[2];
-  __shared__ R wx[64][kMaxOrder], wy[64][kMaxOrder], wz[64][kMaxOrder];
-  __shared__ int b3[64][3];
-  long long acc[8];
-  int gx[8], gy[8], gz[8];
+// One charge's p^3 products into the LDS tile (int64 fixed point) at
+// (x0,y0,z0) with edges E: only the points inside the tile.  The order is a
+// template parameter so that every loop unrolls and the weights stay in
+// registers.
+template <typename R, int O>
+__device__ __forceinline__ void tile_add(unsigned long long *acc, double4 p, const double *kg,
+                                         const int *K, int x0, int y0, int z0, const int *E) {
+  R wx[kMaxOrder], wy[kMaxOrder], wz[kMaxOrder], d[kMaxOrder];
+  const int bx = wrapk(spline<R, O>(p.x, kg[0], O, wx, d), K[0]);
+  const int by = wrapk(spline<R, O>(p.y, kg[1], O, wy, d), K[1]);
+  const int bz = wrapk(spline<R, O>(p.z, kg[2], O, wz, d), K[2]);
+  const R q = (R)p.w;
 #pragma unroll
-  for (int k = 0; k < 8; k++) {
-    acc[k] = 0;
-    const int pt = lane + 64 * k;
-    gx[k] = bx * Bx + pt % Bx;
-    gy[k] = by * By + (pt / Bx) % By;
-    gz[k] = bz * Bz + pt / (Bx * By);
-  }
-  for (int nb = 0; nb < 8; nb++) {
-    const int ox = nb & 1, oy = (nb >> 1) & 1, oz = nb >> 2;  // bin offsets 0 / -1
-    if ((ox && S.nbr[0] == 1) || (oy && S.nbr[1] == 1) || (oz && S.nbr[2] == 1)) continue;
-    const int cx = (bx - ox + S.nbr[0]) % S.nbr[0], cy = (by - oy + S.nbr[1]) % S.nbr[1],
-              cz = (bz - oz + S.nbr[2]) % S.nbr[2];
-    const int bin = (cz * S.nbr[1] + cy) * S.nbr[0] + cx;
-    const int n = min(S.bcnt[bin], S.capb);
-    for (int c0 = 0; c0 < n; c0 += 64) {
-      const int nc = min(64, n - c0);
-      if (lane < nc) {
-        const int m = S.bmem[(size_t)bin * S.capb + c0 + lane];
-        const double4 p = S.pos[S.charge[m]];
-        R d[kMaxOrder], tx[kMaxOrder], ty[kMaxOrder], tz[kMaxOrder];
-        b3[lane][0] = wrapk(spline<R>(p.x, S.kg[0], o, tx, d), K1);
-        b3[lane][1] = wrapk(spline<R>(p.y, S.kg[1], o, ty, d), K2);
-        b3[lane][2] = wrapk(spline<R>(p.z, S.kg[2], o, tz, d), K3);
-        const R q = (R)p.w;
-        for (int a = 0; a < o; a++) {
-          wx[lane][a] = tx[a] * q;
-          wy[lane][a] = ty[a];
-          wz[lane][a] = tz[a];
+  for (int c = 0; c < O; c++) {
+    int gz = bz + c;
+    gz -= gz >= K[2] ? K[2] : 0;
+    const int lz = gz - z0;
+    if ((unsigned)lz >= (unsigned)E[2]) continue;
+#pragma unroll
+    for (int b = 0; b < O; b++) {
+      int gy = by + b;
+      gy -= gy >= K[1] ? K[1] : 0;
+      const int ly = gy - y0;
+      if ((unsigned)ly >= (unsigned)E[1]) continue;
+      unsigned long long *row = acc + (lz * E[1] + ly) * E[0];
+#pragma unroll
+      for (int a = 0; a < O; a++) {
+        int gx = bx + a;
+        gx -= gx >= K[0] ? K[0] : 0;
+        const int lx = gx - x0;
+        if ((unsigned)lx < (unsigned)E[0]) {
+          const double v = (double)(wz[c] * ((wx[a] * q) * wy[b]));
+          atomicAdd(row + lx, (unsigned long long)(long long)rint(v * kQScale));
         }
       }
-      __syncthreads();
-      for (int c = 0; c < nc; c++) {
-        const int cx0 = b3[c][0], cy0 = b3[c][1], cz0 = b3[c][2];
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-          if (k < kpt) {
-            int dx = gx[k] - cx0, dy = gy[k] - cy0, dz = gz[k] - cz0;
-            dx += dx < 0 ? K1 : 0;
-            dy += dy < 0 ? K2 : 0;
-            dz += dz < 0 ? K3 : 0;
-            if (dx < o && dy < o && dz < o) {
-              const double v = (double)(wz[c][dz] * (wx[c][dx] * wy[c][dy]));
-              acc[k] += (long long)rint(v * kQScale);
-            }
-          }
-        }
-      }
-      __syncthreads();
     }
   }
+}
+
+// MapCharges (compt_acc.f90:1324-1365) by mesh tiles: one workgroup per
+// tile accumulates the charges listed for it (those with spline points in
+// the tile) in LDS, in 2^-40 fixed point, so the sums do not depend on the
+// order of the additions; every mesh point is then written once.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_spread(const Rep *__restrict__ reps, WMap wm) {
+  const int r = find_rep(wm, blockIdx.x);
+  const Rep &S = reps[r];
+  const int tile = blockIdx.x - wm.blk[r];
+  const int o = S.order;
+  const int E[3] = {S.tsz[0], S.tsz[1], S.tsz[2]};
+  const int K[3] = {S.K[0], S.K[1], S.K[2]};
+  const int x0 = (tile % S.ntl[0]) * E[0], y0 = ((tile / S.ntl[0]) % S.ntl[1]) * E[1],
+            z0 = tile / (S.ntl[0] * S.ntl[1]) * E[2];
+  const int npt = E[0] * E[1] * E[2];
+  __shared__ unsigned long long acc[kTile * kTile * kTile];
+  for (int k = threadIdx.x; k < npt; k += blockDim.x) acc[k] = 0ull;
+  __syncthreads();
+  const int n = min(S.tcnt[tile], S.capt);
+  const int *mem = S.tmem + (size_t)tile * S.capt;
+  for (int c = threadIdx.x; c < n; c += blockDim.x) {
+    const double4 p = S.pos[mem[c]];
+    switch (o) {
+      case 3: tile_add<R, 3>(acc, p, S.kg, K, x0, y0, z0, E); break;
+      case 4: tile_add<R, 4>(acc, p, S.kg, K, x0, y0, z0, E); break;
+      case 5: tile_add<R, 5>(acc, p, S.kg, K, x0, y0, z0, E); break;
+      case 6: tile_add<R, 6>(acc, p, S.kg, K, x0, y0, z0, E); break;
+      case 7: tile_add<R, 7>(acc, p, S.kg, K, x0, y0, z0, E); break;
+      default: tile_add<R, 8>(acc, p, S.kg, K, x0, y0, z0, E); break;
+    }
+  }
+  __syncthreads();
   R *q = (R *)S.meshr;
-#pragma unroll
-  for (int k = 0; k < 8; k++)
-    if (k < kpt) q[((size_t)gz[k] * K2 + gy[k]) * K1 + gx[k]] = (R)((double)acc[k] * kQInv);
+  for (int k = threadIdx.x; k < npt; k += blockDim.x) {
+    const int lx = k % E[0], ly = (k / E[0]) % E[1], lz = k / (E[0] * E[1]);
+    q[((size_t)(z0 + lz) * K[1] + y0 + ly) * K[0] + x0 + lx] =
+        (R)((double)(long long)acc[k] * kQInv);
+  }
 }
 
 // In-LDS radix-2 FFTs over `nl` lines of length n = 2^ln stored line-major.
@@ -930,7 +994,7 @@ __global__ __launch_bounds__(kBlock) void k_ffty(const Rep *__restrict__ reps, W
 // in bit-reversed y order; the z pass and the influence function use that
 // order, and the inverse y pass (DIT) takes it back.
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_plane_fwd(const Rep *__restrict__ reps, WMap wm) {
+__global__ __launch_bounds__(kPlaneBlock) void k_plane_fwd(const Rep *__restrict__ reps, WMap wm) {
   using C = typename CT<R>::C;
   extern __shared__ char smem[];
   const int r = find_rep(wm, blockIdx.x);
@@ -973,7 +1037,7 @@ __global__ __launch_bounds__(kBlock) void k_plane_fwd(const Rep *__restrict__ re
 // order, rebuild pairs of rows Z = A + iB over the full x range (A, B
 // Hermitian), DIF along x, real and imaginary parts are the two rows.
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_plane_inv(const Rep *__restrict__ reps, WMap wm) {
+__global__ __launch_bounds__(kPlaneBlock) void k_plane_inv(const Rep *__restrict__ reps, WMap wm) {
   using C = typename CT<R>::C;
   extern __shared__ char smem[];
   const int r = find_rep(wm, blockIdx.x);
@@ -1018,7 +1082,7 @@ __global__ __launch_bounds__(kBlock) void k_plane_inv(const Rep *__restrict__ re
 
 // z pass with the convolution of pmeOrthoConvBC (compt_acc.f90:1382-1396):
 // forward FFT along z, multiply by BC, inverse FFT along z; tiles of Tz
-// adjacent kx lines for one ky.
+// adjacent (ky, kx) lines, contiguous in the half-spectrum layout.
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_fftz_conv(const Rep *__restrict__ reps, WMap wm) {
   using C = typename CT<R>::C;
@@ -1026,14 +1090,12 @@ __global__ __launch_bounds__(kBlock) void k_fftz_conv(const Rep *__restrict__ re
   C *buf = (C *)smem;
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
-  const int K2 = S.K[1], K3 = S.K[2], ln = S.lK[2], Kh = S.Kh, T = S.pz_tile;
-  const int tiles = (Kh + T - 1) / T;
-  const int b = blockIdx.x - wm.blk[r];
-  const int y = b / tiles, x0 = (b - y * tiles) * T;
-  const int nl = min(T, Kh - x0);
-  const size_t zs = (size_t)K2 * Kh;
-  C *spec = (C *)S.spec + (size_t)y * Kh + x0;
-  const R *bc = (const R *)S.bc + (size_t)y * Kh + x0;
+  const int K3 = S.K[2], ln = S.lK[2], T = S.pz_tile;
+  const int f0 = (blockIdx.x - wm.blk[r]) * T;
+  const size_t zs = (size_t)S.K[1] * S.Kh;
+  const int nl = min(T, (int)zs - f0);
+  C *spec = (C *)S.spec + f0;
+  const R *bc = (const R *)S.bc + f0;
   for (int t = threadIdx.x; t < nl * K3; t += blockDim.x) {
     const int z = t / nl, l = t - z * nl;
     buf[l * K3 + bitrev(z, ln)] = spec[z * zs + l];
@@ -1070,10 +1132,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(const Rep *__restrict__ reps,
   const int G = S.G, o = S.order;
   const int m = t / G, g = t - m * G;
   const bool live = m < S.Nq;
-  {  // the mesh bins are consumed: clear them for the next evaluation
-    const int nbin = S.nbr[0] * S.nbr[1] * S.nbr[2];
+  {  // the tile lists are consumed: clear them for the next evaluation
+    const int ntile = S.ntl[0] * S.ntl[1] * S.ntl[2];
     const int nth = (wm.blk[r + 1] - wm.blk[r]) * kBlock;
-    for (int k = t; k < nbin; k += nth) S.bcnt[k] = 0;
+    for (int k = t; k < ntile; k += nth) S.tcnt[k] = 0;
   }
   R f1 = 0, f2 = 0, f3 = 0;
   double4 p = make_double4(0, 0, 0, 0);
@@ -1152,9 +1214,12 @@ __device__ __forceinline__ void store_coulomb(const Rep &S, int m, int i, double
 }
 
 // Sum of the gather kernel's block partials (mean reciprocal force, 3, and
-// M_z): each thread sums a fixed strided subset, then a fixed tree over the
-// block, so every block gets the same bits.
-__device__ __forceinline__ void coulomb_totals(const Rep &S, double *tot) {
+// M_z) into qtot: each thread sums a fixed strided subset, then a fixed tree
+// over the block, so the bits do not depend on scheduling.  One workgroup
+// per replica.
+__global__ __launch_bounds__(kBlock) void k_qtot(const Rep *__restrict__ reps) {
+  const Rep &S = reps[blockIdx.x];
+  if (S.Nq == 0 || !S.qtot) return;
   __shared__ double red[4][kBlock / 64];
   double v[4] = {0, 0, 0, 0};
   for (int b = threadIdx.x; b < S.nqblk; b += kBlock)
@@ -1165,11 +1230,10 @@ __device__ __forceinline__ void coulomb_totals(const Rep &S, double *tot) {
   }
   __syncthreads();
   if (threadIdx.x < 4) {
-    double s = 0;
-    for (int w = 0; w < kBlock / 64; w++) s += red[threadIdx.x][w];
-    tot[threadIdx.x] = s;
+    double t = 0;
+    for (int w = 0; w < kBlock / 64; w++) t += red[threadIdx.x][w];
+    S.qtot[threadIdx.x] = t;
   }
-  __syncthreads();
 }
 
 // erfc-screened Coulomb pair force of real_space (compt_acc.f90:1034-1036)
@@ -1192,47 +1256,80 @@ __device__ __forceinline__ void real_pair(R xi, R yi, R zi, R qi, R xj, R yj, R 
   }
 }
 
-// Charges into the mesh bins of the spreading kernel and, when the sweep
-// is used, into the cells of the real-space sweep (edge >= r_c).
+// Charges into the lists of the spreading tiles that hold their spline
+// points (one to eight) and into the cells of the real-space sweep.
 __global__ __launch_bounds__(kBlock) void k_bin_q(const Rep *__restrict__ reps, WMap wm) {
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
   const int m = (blockIdx.x - wm.blk[r]) * kBlock + threadIdx.x;
-  if (m >= S.Nq) return;
-  const int i = S.charge[m];
-  const double4 p = S.pos[i];
-  {
+  const bool live = m < S.Nq;
+  int i = 0;
+  double4 p = make_double4(0, 0, 0, 0);
+  if (live) {
+    i = S.charge[m];
+    p = S.pos[i];
+  }
+  int t0[3] = {0, 0, 0}, nt[3] = {0, 0, 0};
+  if (live) {
     const int o = S.order;
-    const int gx = spline_base(p.x, S.kg[0], o, S.K[0]) / S.bsz[0];
-    const int gy = spline_base(p.y, S.kg[1], o, S.K[1]) / S.bsz[1];
-    const int gz = spline_base(p.z, S.kg[2], o, S.K[2]) / S.bsz[2];
-    const int bin = (gz * S.nbr[1] + gy) * S.nbr[0] + gx;
-    const int slot = atomicAdd(&S.bcnt[bin], 1);
-    if (slot < S.capb) S.bmem[(size_t)bin * S.capb + slot] = m;
-    else atomicOr(&S.ctrl->ovf, 32);
+    const double c3[3] = {p.x, p.y, p.z};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const int b = spline_base(c3[d], S.kg[d], o, S.K[d]);
+      t0[d] = b / S.tsz[d];
+      int e = b + o - 1;
+      e -= e >= S.K[d] ? S.K[d] : 0;
+      nt[d] = e / S.tsz[d] != t0[d] ? 2 : 1;
+    }
   }
-  if (!S.real_cells) return;
-  int a, b, c;
-  if (!cell_of(p, S.nc3, S.rcel3, S.hLx, S.hLy, a, b, c)) {
-    set_error(S.ctrl, MD_ECELL, i, -1);
-    return;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const int a = k & 1, b = (k >> 1) & 1, c = k >> 2;
+    int key = -1;
+    if (a < nt[0] && b < nt[1] && c < nt[2]) {
+      const int tx = (t0[0] + a) % S.ntl[0], ty = (t0[1] + b) % S.ntl[1],
+                tz = (t0[2] + c) % S.ntl[2];
+      key = (tz * S.ntl[1] + ty) * S.ntl[0] + tx;
+    }
+    const int slot = wave_append(key >= 0 ? S.tcnt + key : nullptr);
+    if (key >= 0) {
+      if (slot < S.capt) S.tmem[(size_t)key * S.capt + slot] = i;
+      else atomicOr(&S.ctrl->ovf, 32);
+    }
   }
-  const int cell = (c * S.nc3[1] + b) * S.nc3[0] + a;
-  const int slot = atomicAdd(&S.cnt3[cell], 1);
-  if (slot < S.ccap3) S.mem3[(size_t)cell * S.ccap3 + slot] = m;
-  else atomicOr(&S.ctrl->ovf, 16);
+  int key = -1;
+  if (live && S.real_cells) {
+    int a, b, c;
+    if (cell_of(p, S.nc3, S.rcel3, S.hLx, S.hLy, a, b, c)) key = (c * S.nc3[1] + b) * S.nc3[0] + a;
+    else set_error(S.ctrl, MD_ECELL, i, -1);
+  }
+  const int slot = wave_append(key >= 0 ? S.cnt3 + key : nullptr);
+  if (key >= 0) {
+    if (slot < S.ccap3) S.mem3[(size_t)key * S.ccap3 + slot] = m;
+    else atomicOr(&S.ctrl->ovf, 16);
+  }
 }
 
 // One workgroup per cell: sort the members, keep the sorted count, clear
 // the binning count.
+template <typename R>
 __global__ __launch_bounds__(kBlock) void k_sort_q(const Rep *__restrict__ reps, WMap wm) {
+  using V = typename std::conditional<sizeof(R) == 4, float4, double4>::type;
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
   const int cell = blockIdx.x - wm.blk[r];
   const int raw = S.cnt3[cell];
   const int n = min(raw, S.ccap3);
   extern __shared__ int sbuf[];
-  sort_cell_lds(S.mem3 + (size_t)cell * S.ccap3, n, sbuf);
+  int *mem = S.mem3 + (size_t)cell * S.ccap3;
+  sort_cell_lds(mem, n, sbuf);
+  // cell-sorted copy of the positions (the same thread wrote mem[k])
+  V *ps = (V *)S.pos3s + (size_t)cell * S.ccap3;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const int i = S.charge[mem[k]];
+    if constexpr (sizeof(R) == 4) ps[k] = S.pos32[i];
+    else ps[k] = S.pos[i];
+  }
   if (threadIdx.x == 0) {
     S.scnt3[cell] = n;
     S.cnt3[cell] = 0;
@@ -1244,11 +1341,13 @@ __global__ __launch_bounds__(kBlock) void k_sort_q(const Rep *__restrict__ reps,
 template <typename R> struct RealTile { static constexpr int N = 1024; };
 template <> struct RealTile<float> { static constexpr int N = 2048; };
 
-// real_space (compt_acc.f90:980-1049) by a cell sweep: one workgroup per
-// cell of edge >= r_c; the members of its 27 neighbour cells (reference
-// stencil order, ascending within a cell) are staged in LDS, each wavefront
-// takes one target charge of the cell at a time, its 64 lanes stride over
+// real_space (compt_acc.f90:980-1049) by a cell sweep over cells of edge
+// >= r_c: a work item is a chunk of kRealChunk target charges of one cell.
+// The members of the cell's 27 neighbour cells (reference stencil order,
+// ascending within a cell) are staged in LDS from the cell-sorted position
+// copy, each wavefront takes one target at a time, its 64 lanes stride over
 // the candidates, and a shuffle reduction (fixed order) sums the lanes.
+// Workgroups stride over the items and skip the empty ones in bulk.
 // Same pair set as the full real-space list: every j != i with r < r_c.
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_real_cells(const Rep *__restrict__ reps, WMap wm,
@@ -1261,70 +1360,86 @@ __global__ __launch_bounds__(kBlock) void k_real_cells(const Rep *__restrict__ r
   // work items (cell, chunk of kRealChunk targets); workgroups stride over them
   const int nch = (S.ccap3 + kRealChunk - 1) / kRealChunk;
   const int nitem = S.nc3[0] * S.nc3[1] * S.nc3[2] * nch;
-  __shared__ double tot[4];
   __shared__ V cand[TN];
-  __shared__ int cidx[TN];
   __shared__ int offs[28], cids[27];
-  coulomb_totals(S, tot);
+  __shared__ int work[kBlock], wcnt[kBlock / 64];
+  const double *tot = S.qtot;
+  const V *ps = (const V *)S.pos3s;
   const R Lx = (R)S.Lx, Ly = (R)S.Ly, hLx = (R)S.hLx, hLy = (R)S.hLy, rcc2 = (R)S.rcc2;
   const R alpha = (R)S.alpha, alpha2 = (R)S.alpha2, c2 = (R)S.c2;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int item = b0; item < nitem; item += nb) {
-    const int cell = item / nch, t0 = (item - cell * nch) * kRealChunk;
-    const int nt = min(S.scnt3[cell] - t0, kRealChunk);
-    if (nt <= 0) continue;
-    const int total = stencil_offsets(cell, S.nc3, S.scnt3, offs, cids);
-    const int *tm = S.mem3 + (size_t)cell * S.ccap3 + t0;
-    for (int base = 0; base < total; base += TN) {
-      const int nc = min(TN, total - base);
-      for (int k = threadIdx.x; k < nc; k += blockDim.x) {
-        const int t = base + k, l = stencil_slot(offs, t);
-        const int mm = S.mem3[(size_t)cids[l] * S.ccap3 + (t - offs[l])];
-        const int j = S.charge[mm];
-        V v;
-        if constexpr (sizeof(R) == 4) v = S.pos32[j];
-        else v = S.pos[j];
-        cand[k] = v;
-        cidx[k] = j;
-      }
-      __syncthreads();
-      for (int tg = w; tg < nt; tg += kBlock / 64) {
-        const int m = tm[tg];
-        const int i = S.charge[m];
-        V pi;
-        if constexpr (sizeof(R) == 4) pi = S.pos32[i];
-        else pi = S.pos[i];
-        R fx = 0, fy = 0, fz = 0;
-        for (int k = lane; k < nc; k += 64) {
-          if (cidx[k] == i) continue;
-          const V pj = cand[k];
-          real_pair<R>(pi.x, pi.y, pi.z, pi.w, pj.x, pj.y, pj.z, pj.w, Lx, Ly, hLx, hLy, rcc2, alpha,
-                       alpha2, c2, fx, fy, fz);
+  for (int it0 = b0; it0 < nitem; it0 += nb * kBlock) {
+    // the workgroup's next kBlock items (its items are b0, b0 + nb, ...):
+    // the non-empty ones, compacted in order
+    const int item = it0 + threadIdx.x * nb;
+    bool ne = false;
+    if (item < nitem) {
+      const int cell = item / nch;
+      ne = S.scnt3[cell] > (item - cell * nch) * kRealChunk;
+    }
+    const unsigned long long bal = __ballot(ne);
+    if (lane == 0) wcnt[w] = __popcll(bal);
+    __syncthreads();
+    int off = 0, nwork = 0;
+    for (int k = 0; k < kBlock / 64; k++) {
+      off += k < w ? wcnt[k] : 0;
+      nwork += wcnt[k];
+    }
+    if (ne) work[off + __popcll(bal & ((1ull << lane) - 1ull))] = item;
+    __syncthreads();
+    for (int q = 0; q < nwork; q++) {
+      const int it = work[q];
+      const int cell = it / nch, t0 = (it - cell * nch) * kRealChunk;
+      const int nt = min(S.scnt3[cell] - t0, kRealChunk);
+      const int total = stencil_offsets(cell, S.nc3, S.scnt3, offs, cids);
+      // stencil slot 13 is the cell itself: candidate offs[13] + t0 + tg is target tg
+      const int self0 = offs[13] + t0;
+      const V *tp = ps + (size_t)cell * S.ccap3 + t0;
+      for (int base = 0; base < total; base += TN) {
+        const int nc = min(TN, total - base);
+        for (int k = threadIdx.x; k < nc; k += blockDim.x) {
+          const int t = base + k, l = stencil_slot(offs, t);
+          cand[k] = ps[(size_t)cids[l] * S.ccap3 + (t - offs[l])];
         }
-        for (int off = 32; off > 0; off >>= 1) {
-          fx += __shfl_xor(fx, off, 64);
-          fy += __shfl_xor(fy, off, 64);
-          fz += __shfl_xor(fz, off, 64);
-        }
-        if (lane == 0) {
-          if (total <= TN) {
-            store_coulomb<R>(S, m, i, S.pos[i].w, tot, fx, fy, fz, out);
-          } else {
-            // several tiles: the same wave owns this target in every tile; the
-            // first tile writes its sum, later tiles add, the last one stores
-            double *acc = S.rsum + 3 * (size_t)m;
-            if (base == 0) {
-              acc[0] = fx; acc[1] = fy; acc[2] = fz;
+        __syncthreads();
+        for (int tg = w; tg < nt; tg += kBlock / 64) {
+          const V pi = tp[tg];
+          const int self = self0 + tg - base;
+          R fx = 0, fy = 0, fz = 0;
+          for (int k = lane; k < nc; k += 64) {
+            if (k == self) continue;
+            const V pj = cand[k];
+            real_pair<R>(pi.x, pi.y, pi.z, pi.w, pj.x, pj.y, pj.z, pj.w, Lx, Ly, hLx, hLy, rcc2,
+                         alpha, alpha2, c2, fx, fy, fz);
+          }
+          for (int o = 32; o > 0; o >>= 1) {
+            fx += __shfl_xor(fx, o, 64);
+            fy += __shfl_xor(fy, o, 64);
+            fz += __shfl_xor(fz, o, 64);
+          }
+          if (lane == 0) {
+            const int m = S.mem3[(size_t)cell * S.ccap3 + t0 + tg];
+            const int i = S.charge[m];
+            if (total <= TN) {
+              store_coulomb<R>(S, m, i, S.pos[i].w, tot, fx, fy, fz, out);
             } else {
-              acc[0] += fx; acc[1] += fy; acc[2] += fz;
+              // several tiles: the same wave owns this target in every tile; the
+              // first tile writes its sum, later tiles add, the last one stores
+              double *acc = S.rsum + 3 * (size_t)m;
+              if (base == 0) {
+                acc[0] = fx; acc[1] = fy; acc[2] = fz;
+              } else {
+                acc[0] += fx; acc[1] += fy; acc[2] += fz;
+              }
+              if (base + TN >= total)
+                store_coulomb<R>(S, m, i, S.pos[i].w, tot, (R)acc[0], (R)acc[1], (R)acc[2], out);
             }
-            if (base + TN >= total)
-              store_coulomb<R>(S, m, i, S.pos[i].w, tot, (R)acc[0], (R)acc[1], (R)acc[2], out);
           }
         }
+        __syncthreads();
       }
-      __syncthreads();
     }
+    __syncthreads();  // work[] is refilled
   }
 }
 
@@ -1338,9 +1453,8 @@ __global__ __launch_bounds__(kBlock) void k_real(const Rep *__restrict__ reps, W
   const Rep &S = reps[r];
   if (S.real_cells) return;
   const int m = (blockIdx.x - wm.blk[r]) * kBlock + threadIdx.x;
-  __shared__ double tot[4];
-  coulomb_totals(S, tot);
   if (m >= S.Nq) return;
+  const double *tot = S.qtot;
   const int i = S.charge[m];
   const double4 pd = S.pos[i];
   const R alpha = (R)S.alpha, alpha2 = (R)S.alpha2, c2 = (R)S.c2, rcc2 = (R)S.rcc2;
@@ -1371,13 +1485,23 @@ __global__ __launch_bounds__(kBlock) void k_real(const Rep *__restrict__ reps, W
 
 // ---------------------------------------------------- rescale_velocity
 
+// Whether a replica of a batch rescales after the step just completed:
+// the pre-heat rule of main.f90:67-75 (every 20 steps below 300000, up to
+// its last pre-heat step).  The list kernel has already advanced the step.
+__device__ __forceinline__ bool preheat_due(const Rep &S) {
+  const int last = S.ctrl->step - 1;
+  return last % 20 == 0 && last <= S.preheat_end && last < 300000;
+}
+
 // rescale_velocity (init_update.f90:498-552) in three passes with fixed-order
 // block sums: mean velocity over NN-N_anchor, mean square over the moving
-// particles, rescale to 3/Beta.
+// particles, rescale to 3/Beta.  gated: batch pre-heat, per replica.
+
 __global__ __launch_bounds__(kBlock) void k_vsum(const Rep *__restrict__ reps, WMap wm, double *part,
-                                                 int pass) {
+                                                 int pass, int gated) {
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
+  if (gated && !preheat_due(S)) return;
   const int blk = blockIdx.x - wm.blk[r];
   const int i = blk * kBlock + threadIdx.x;
   double *pr = part + (size_t)wm.blk[r] * 4;
@@ -1424,9 +1548,10 @@ __global__ __launch_bounds__(kBlock) void k_vsum(const Rep *__restrict__ reps, W
 }
 
 __global__ __launch_bounds__(kBlock) void k_vscale(const Rep *__restrict__ reps, WMap wm,
-                                                   const double *part) {
+                                                   const double *part, int gated) {
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
+  if (gated && !preheat_due(S)) return;
   const int i = (blockIdx.x - wm.blk[r]) * kBlock + threadIdx.x;
   const double *pr = part + (size_t)wm.blk[r] * 4;
   const int nb = wm.blk[r + 1] - wm.blk[r];
@@ -1503,13 +1628,13 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) 
   hipLaunchKernelGGL(k_bin_q, dim3(P.chg.total), dim3(kBlock), 0, st, P.reps, P.chg);
   mark();
   if (P.cell3.total)
-    hipLaunchKernelGGL(k_sort_q, dim3(P.cell3.total), dim3(kBlock), P.lds_sort3, st, P.reps, P.cell3);
+    hipLaunchKernelGGL(k_sort_q<R>, dim3(P.cell3.total), dim3(kBlock), P.lds_sort3, st, P.reps, P.cell3);
   mark();
-  hipLaunchKernelGGL(k_spread<R>, dim3(P.brick.total), dim3(64), 0, st, P.reps, P.brick);
+  hipLaunchKernelGGL(k_spread<R>, dim3(P.brick.total), dim3(kBlock), 0, st, P.reps, P.brick);
   mark();
   // replicas with fused planes take k_plane_*, the others the x and y passes
   if (P.plane.total)
-    hipLaunchKernelGGL(k_plane_fwd<R>, dim3(P.plane.total), dim3(kBlock), P.lds_plane, st, P.reps,
+    hipLaunchKernelGGL(k_plane_fwd<R>, dim3(P.plane.total), dim3(kPlaneBlock), P.lds_plane, st, P.reps,
                        P.plane);
   if (P.fx.total)
     hipLaunchKernelGGL(k_fftx_fwd<R>, dim3(P.fx.total), dim3(kBlock), P.lds_x, st, P.reps, P.fx);
@@ -1523,12 +1648,13 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) 
     hipLaunchKernelGGL(k_ffty<R>, dim3(P.fy.total), dim3(kBlock), P.lds_y, st, P.reps, P.fy, +1);
   mark();
   if (P.plane.total)
-    hipLaunchKernelGGL(k_plane_inv<R>, dim3(P.plane.total), dim3(kBlock), P.lds_plane, st, P.reps,
+    hipLaunchKernelGGL(k_plane_inv<R>, dim3(P.plane.total), dim3(kPlaneBlock), P.lds_plane, st, P.reps,
                        P.plane);
   if (P.fx.total)
     hipLaunchKernelGGL(k_fftx_inv<R>, dim3(P.fx.total), dim3(kBlock), P.lds_x, st, P.reps, P.fx);
   mark();
   hipLaunchKernelGGL(k_gather<R>, dim3(P.chgG.total), dim3(kBlock), 0, st, P.reps, P.chgG);
+  hipLaunchKernelGGL(k_qtot, dim3(P.nrep), dim3(kBlock), 0, st, P.reps);
   mark();
   if (P.cell3.total)
     hipLaunchKernelGGL(k_real_cells<R>, dim3(P.cell3c.total), dim3(kBlock), 0, st, P.reps, P.cell3c, out);
@@ -1541,23 +1667,25 @@ template <typename R>
 void launch_short(const Plan &P, hipStream_t st, int mode, int step, const double *eta,
                   double *out) {
   hipLaunchKernelGGL(k_short_range<R>, dim3(P.part.total), dim3(kBlock), 0, st, P.reps, P.part,
-                     mode, step, eta, out);
+                     mode, step, eta, out, P.lq, P.lq_cap);
 }
 
 void launch_kick_drift(const Plan &P, hipStream_t st) {
-  hipLaunchKernelGGL(k_kick_drift, dim3(P.part.total), dim3(kBlock), 0, st, P.reps, P.part);
+  hipLaunchKernelGGL(k_kick_drift, dim3(P.part.total), dim3(kBlock), 0, st, P.reps, P.part, P.lq);
 }
 
 // update_verlet_list's LJ rebuild (compt_acc.f90:717-720); every kernel
 // returns at once for replicas whose rebuild flag is clear.
-void launch_lists(const Plan &P, hipStream_t st, int inc_step, hipEvent_t *ev) {
+void launch_lists(const Plan &P, hipStream_t st, int inc_step, hipEvent_t *ev, bool clear_queue) {
   if (ev) (void)hipEventRecord(ev[0], st);
-  hipLaunchKernelGGL(k_bin_lj, dim3(P.part.total), dim3(kBlock), 0, st, P.reps, P.part);
+  hipLaunchKernelGGL(k_bin_lj, dim3(P.grid_bin), dim3(kBlock), 0, st, P.reps, P.lq, P.lq_cap);
   if (ev) (void)hipEventRecord(ev[1], st);
-  hipLaunchKernelGGL(k_sort_lj, dim3(P.cell1w.total), dim3(kBlock), 0, st, P.reps, P.cell1w);
+  hipLaunchKernelGGL(k_sort_lj, dim3(P.grid_bin), dim3(kBlock), 0, st, P.reps, P.lq, P.lq_cap);
   if (ev) (void)hipEventRecord(ev[2], st);
-  hipLaunchKernelGGL(k_build_lj, dim3(P.cell1.total), dim3(kBlock), 0, st, P.reps, P.cell1, inc_step);
+  hipLaunchKernelGGL(k_build_lj, dim3(P.grid_build), dim3(kBlock), 0, st, P.reps, P.nrep, P.lq,
+                     P.lq_cap, inc_step);
   if (ev) (void)hipEventRecord(ev[3], st);
+  if (clear_queue) hipLaunchKernelGGL(k_lq_clear, dim3(1), dim3(1), 0, st, P.lq);
 }
 
 // The reference-form real-space list (real_verlet_list), on request.
@@ -1569,13 +1697,17 @@ void launch_real_list(const Plan &P, hipStream_t st) {
 }
 
 void launch_set_flags(const Plan &P, hipStream_t st, int clear) {
-  hipLaunchKernelGGL(k_set_flags, dim3((P.nrep + 63) / 64), dim3(64), 0, st, P.reps, P.nrep, clear);
+  hipLaunchKernelGGL(k_set_flags, dim3(1), dim3(kBlock), 0, st, P.reps, P.nrep, clear, P.lq,
+                     P.lq_cap);
 }
 
-void launch_rescale(const Plan &P, hipStream_t st) {
-  hipLaunchKernelGGL(k_vsum, dim3(P.part.total), dim3(kBlock), 0, st, P.reps, P.part, P.vpart, 0);
-  hipLaunchKernelGGL(k_vsum, dim3(P.part.total), dim3(kBlock), 0, st, P.reps, P.part, P.vpart, 1);
-  hipLaunchKernelGGL(k_vscale, dim3(P.part.total), dim3(kBlock), 0, st, P.reps, P.part, P.vpart);
+void launch_rescale(const Plan &P, hipStream_t st, int gated) {
+  hipLaunchKernelGGL(k_vsum, dim3(P.part.total), dim3(kBlock), 0, st, P.reps, P.part, P.vpart, 0,
+                     gated);
+  hipLaunchKernelGGL(k_vsum, dim3(P.part.total), dim3(kBlock), 0, st, P.reps, P.part, P.vpart, 1,
+                     gated);
+  hipLaunchKernelGGL(k_vscale, dim3(P.part.total), dim3(kBlock), 0, st, P.reps, P.part, P.vpart,
+                     gated);
 }
 
 template void launch_coulomb<float>(const Plan &, hipStream_t, double *, hipEvent_t *);

@@ -32,10 +32,14 @@ class Request:
         self.result = None
 
 
-def make_stream(n, nsteps, rate, seed, precision=MD_FP32, max_side=8):
+def make_stream(n, nsteps, rate, seed, precision=MD_FP32, max_side=8, preheat=None):
     """Seeded stream of n requests with Poisson arrivals of the given rate
     (1/s): chains 2x2..max_side^2, N uniform in 20..100, linear or 3/4-arm
-    star, salt valence 1 or 3, E_z uniform in [0,1]."""
+    star, salt valence 1 or 3, E_z uniform in [0,1].  Each runs nsteps steps,
+    the first `preheat` of them (default a fifth, a multiple of 20) in the
+    reference's pre-heat form, from its random initial configuration."""
+    if preheat is None:
+        preheat = (nsteps // 5) // 20 * 20
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -47,7 +51,7 @@ def make_stream(n, nsteps, rate, seed, precision=MD_FP32, max_side=8):
         val = int(rng.choice([1, 3]))
         ez = float(rng.uniform(0, 1))
         rs = int(seed * 1000 + k)
-        p = replica(side, nb, arch, val, ez, rs, precision)
+        p = replica(side, nb, arch, val, ez, rs, precision, preheat)
         pos = initialize_position(p, rs)
         vel = initialize_velocity(p, rs + 1)
         out.append(Request(k, t, p, nsteps, pos, vel,
@@ -156,7 +160,10 @@ def sequential_fp64(requests, multistep=10):
         e.upload(r.pos, r.vel)
         e.build_lists()
         e.compute_force(0, MD_NOISE_PHILOX)
-        e.run(1, r.nsteps)
+        pre = min(r.params.StepNum0, r.nsteps)
+        if pre:
+            e.run(1, pre, preheat=True)
+        e.run(1 + pre, r.nsteps - pre)
         e.close()
         total_t += time.perf_counter() - t0
         total_steps += r.nsteps
@@ -167,10 +174,16 @@ def summarize(done, wall):
     lat = np.array([r.t_done - r.arrival for r in done])
     steps = sum(r.nsteps for r in done)
     ok = sum(1 for r in done if r.result and r.result["status"] == 0)
-    return {"requests": len(done), "ok": ok, "replica_steps": steps, "wall_s": round(wall, 3),
-            "aggregate_steps_per_s": round(steps / wall, 1),
-            "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
-            "latency_p99_s": round(float(np.percentile(lat, 99)), 4)}
+    out = {"requests": len(done), "ok": ok, "replica_steps": steps, "wall_s": round(wall, 3),
+           "aggregate_steps_per_s": round(steps / wall, 1),
+           "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
+           "latency_p99_s": round(float(np.percentile(lat, 99)), 4)}
+    failed = [r for r in done if not r.result or r.result["status"] != 0]
+    if failed:
+        out["failures"] = [{"rid": r.rid, "status": r.result["status"] if r.result else None,
+                            "message": r.result.get("message") if r.result else None, **r.desc}
+                           for r in failed[:8]]
+    return out
 
 
 def main():

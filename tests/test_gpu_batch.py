@@ -61,6 +61,30 @@ def test_batch_replica_equals_single_engine():
     assert np.array_equal(pe, pb) and np.array_equal(ve, vb)
 
 
+def test_batch_preheat_equals_single_engine():
+    """A batch replica with StepNum0 = 60 pre-heats on its own step count,
+    beside a replica without pre-heat, exactly as md_run's pre-heat form."""
+    print("[batch] preheat vs single engine fp64", flush=True)
+    p = replica(3, 30, 1, 3, 0.3, 51, MD_FP64, preheat=60)
+    pos, vel = initialize_position(p, 51), initialize_velocity(p, 52)
+    e = Engine(p)
+    e.upload(pos, vel)
+    e.build_lists()
+    e.compute_force(0, MD_NOISE_PHILOX)
+    e.run(1, 60, preheat=True)
+    e.run(61, 40)
+    pe, ve, _, _ = e.download()
+    e.close()
+    b = Batch(MD_FP64)
+    b.add(*_req(2, 20, 4, 1, 0.7, 53, MD_FP64))
+    b.run(2)
+    sl = b.add(p, pos, vel)  # joins two cycles later: its own step count
+    b.run(10)
+    pb, vb, _ = b.download(sl)
+    b.close()
+    assert np.array_equal(pe, pb) and np.array_equal(ve, vb)
+
+
 def test_observables_match_definitions():
     print("[batch] observables", flush=True)
     p, pos, vel = _req(4, 50, 1, 1, 0.2, 31)

@@ -126,6 +126,31 @@ def test_graph_run_matches_stepwise():
     e2.close()
 
 
+def test_preheat_graph_matches_stepwise():
+    """Pre-heat (main.f90:67-75) replayed in graph cycles with the gated
+    rescale kernels equals the step-by-step sequence with rescale_velocity
+    after every 20th step; the run starts off the cycle alignment."""
+    print("[preheat graph vs stepwise] small", flush=True)
+    p, pos, vel, e1 = _engine("small")
+    p, pos, vel, e2 = _engine("small")
+    e1.compute_force(0, MD_NOISE_PHILOX)
+    e2.compute_force(0, MD_NOISE_PHILOX)
+    e1.run(1, 3)
+    e1.run(4, 63, preheat=True)
+    e1.run(67, 14)
+    for s in range(1, 81):
+        e2.new_position(s, MD_NOISE_PHILOX)
+        if 4 <= s <= 66 and s % 20 == 0:
+            e2.rescale_velocity()
+        e2.update_verlet_list()
+    a = e1.download()
+    b = e2.download()
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    e1.close()
+    e2.close()
+
+
 def test_run_is_deterministic():
     print("[determinism] default fp32", flush=True)
     out = []

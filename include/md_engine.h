@@ -199,8 +199,10 @@ int md_batch_add(md_engine *e, const md_params *p, const double *pos,
 /* Frees a replica's device memory. */
 int md_batch_remove(md_engine *e, int32_t slot);
 
-/* Runs ncycles * multistep steps (new_position + update_verlet_list, Philox
- * noise) of every replica in one graph-replayed launch sequence. */
+/* Queues ncycles * multistep steps (new_position + update_verlet_list,
+ * Philox noise) of every replica as graph-replayed launch sequences and
+ * returns without waiting: md_batch_add, _status, _download and _observe
+ * wait for them, so host work between calls overlaps the device. */
 int md_batch_run(md_engine *e, int32_t ncycles);
 
 /* Status of one replica (MD_OK or its device-side error) and its next step. */

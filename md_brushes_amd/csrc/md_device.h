@@ -14,7 +14,10 @@ namespace mdk {
 constexpr int kBlock = 256;
 constexpr int kMaxOrder = 8;
 constexpr int kTile = 16;         // spreading tile edge (mesh points)
+constexpr int kTileSub = 8;       // sub-lists per spreading tile (spreads the appends)
 constexpr int kPlaneBlock = 512;  // threads of the fused FFT plane kernels
+constexpr int kSRLanes = 8;       // lanes per particle in the LJ pass of the short-range kernel
+constexpr int kSRBlock = 512;     // its workgroup: 64 particles
 constexpr int kRealChunk = 16;  // target charges per workgroup of the real-space sweep
 constexpr int kHistBins = 500;  // SizeHist of the z-density profile (glob_var.f90:59)
 
@@ -97,8 +100,8 @@ struct Rep {
   void *meshr;                     // charge mesh Q (real type of the path) K3*K2*K1
   // spreading tiles: each tile lists the charges with spline points in it
   int tsz[3], ntl[3];              // tile edge (mesh points) and tiles per axis
-  int capt;                        // list capacity per tile
-  int *tcnt, *tmem;                // counts; particle indices, capt per tile
+  int capt;                        // capacity of each of a tile's kTileSub sub-lists
+  int *tcnt, *tmem;                // counts; particle indices (tile, sub, slot)
   int plane_fused;                 // x and y FFT passes fused per z-plane; the
                                    // spectrum is then kept in bit-reversed y order
   void *spec;                      // half spectrum K3*K2*Kh complex

@@ -13,7 +13,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -28,10 +31,23 @@ using mdk::WMap;
 namespace {
 
 constexpr double kPi = 3.141592653589793;
-// part, chg, chgG, fx, fy, fz, cell1, cell1w, cell2, cell3, cell3c, brick, plane
-constexpr int kMaps = 13;
-// LDS a fused FFT plane may take (both planes of the kernel)
-constexpr size_t kPlaneLds = 65536;
+// part, chg, chgG, fx, fy, fz, cell1, cell1w, cell2, cell3, cell3c, brick, plane, partG
+constexpr int kMaps = 14;
+// LDS a workgroup may take on this device (both planes of a fused FFT
+// plane kernel must fit)
+size_t lds_limit() {
+  static size_t lim = 0;
+  if (!lim) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+        v > 0)
+      lim = (size_t)v;
+    else
+      lim = 65536;
+  }
+  return lim;
+}
 
 bool is_pow2(int n) { return n >= 4 && (n & (n - 1)) == 0; }
 int ilog2(int n) {
@@ -66,6 +82,38 @@ std::vector<double> bmod2(int K, int p) {
   return out;
 }
 
+// Read-only device tables (influence function, twiddles) shared by every
+// replica with the same mesh, order, splitting and box: built once, freed
+// with their last user.
+struct DevTable {
+  void *ptr = nullptr;
+  ~DevTable() {
+    if (ptr) (void)hipFree(ptr);
+  }
+};
+std::mutex g_tables_mu;
+std::map<std::string, std::weak_ptr<DevTable>> g_tables;
+
+std::shared_ptr<DevTable> shared_table(const std::string &key,
+                                       const std::function<std::vector<char>()> &build) {
+  std::lock_guard<std::mutex> lock(g_tables_mu);
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  const std::string k = std::to_string(dev) + "|" + key;
+  auto it = g_tables.find(k);
+  if (it != g_tables.end())
+    if (auto t = it->second.lock()) return t;
+  std::vector<char> host = build();
+  auto t = std::make_shared<DevTable>();
+  if (hipMalloc(&t->ptr, host.size()) != hipSuccess) {
+    t->ptr = nullptr;
+    return nullptr;
+  }
+  if (hipMemcpy(t->ptr, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+  g_tables[k] = t;
+  return t;
+}
+
 std::string hip_msg(hipError_t e, const char *what) {
   return std::string(what) + ": " + hipGetErrorString(e);
 }
@@ -77,6 +125,7 @@ struct Sys {
   Rep h{};
   std::vector<int> charge, flags, bstart, bpart;  // host topology
   std::vector<void *> allocs;
+  std::vector<std::shared_ptr<DevTable>> tables;  // shared read-only tables
   size_t bytes = 0;
   int ncell1 = 0, ncell2 = 0, ncell3 = 0;
   int nb[kMaps] = {0};
@@ -281,8 +330,7 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
     S.meshr = dalloc<char>(M * rs);
     S.spec = dalloc<char>(Mh * csz);
     S.meshu = dalloc<char>(M * rs);
-    S.bc = dalloc<char>(Mh * rs);
-    if (!S.meshr || !S.spec || !S.meshu || !S.bc) return fail(MD_ENOMEM, "mesh");
+    if (!S.meshr || !S.spec || !S.meshu) return fail(MD_ENOMEM, "mesh");
     // spreading tiles: edge >= order, so a charge's points fall in at most
     // two tiles along each axis
     nbin = 1;
@@ -291,71 +339,81 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
       S.ntl[d] = p.ng[d] / S.tsz[d];
       nbin *= S.ntl[d];
     }
-    S.tcnt = dalloc<int>(nbin);
+    S.tcnt = dalloc<int>((size_t)nbin * mdk::kTileSub);
     S.tmem = nullptr;
     S.capt = 0;
     if (!S.tcnt) return fail(MD_ENOMEM, "spreading tiles");
     // x and y passes fused per z-plane when both planes fit in LDS
     lds_plane = ((size_t)(K2 / 2) * K1 + (size_t)S.Kh * K2) * csz;
-    S.plane_fused = lds_plane <= kPlaneLds;
+    // MD_FFT_PLANES=0 keeps the separate x and y passes (for comparison)
+    const char *fp = std::getenv("MD_FFT_PLANES");
+    // (at most 64 KB: larger planes leave one workgroup per CU and lose)
+    S.plane_fused = lds_plane <= std::min<size_t>(65536, lds_limit()) && !(fp && fp[0] == '0');
     if (!S.plane_fused) lds_plane = 0;
     // influence function of PME_BC (compt_acc.f90:1557-1598), half spectrum
     const int o = S.order;
-    std::vector<double> b2[3], mv[3], ev[3];
-    for (int d = 0; d < 3; d++) {
-      const int K = p.ng[d];
-      b2[d] = bmod2(K, o);
-      mv[d].resize(K);
-      ev[d].resize(K);
-      for (int k = 0; k < K; k++) {
-        mv[d][k] = (k <= K / 2 ? k : k - K) / p.gdim[d];
-        const double t = kPi / p.alpha;
-        ev[d][k] = std::exp(-t * t * mv[d][k] * mv[d][k]);
-      }
-    }
-    const double pivol = (double)K1 * K2 * K3 / (kPi * p.gdim[0] * p.gdim[1] * p.gdim[2]);
-    std::vector<double> bcd(Mh);
-    for (int z = 0; z < K3; z++)
-      for (int y = 0; y < K2; y++)
-        for (int x = 0; x < S.Kh; x++) {
-          const double m2 = mv[0][x] * mv[0][x] + mv[1][y] * mv[1][y] + mv[2][z] * mv[2][z];
-          bcd[((size_t)z * K2 + y) * S.Kh + x] =
-              (x + y + z == 0) ? 0.0
-                               : b2[0][x] * b2[1][y] * b2[2][z] * pivol * ev[0][x] * ev[1][y] *
-                                     ev[2][z] / m2;
+    char key[256];
+    std::snprintf(key, sizeof key, "bc %d %d %d %d %a %a %a %a %d %d", K1, K2, K3, o, p.alpha,
+                  p.gdim[0], p.gdim[1], p.gdim[2], (int)fp32, S.plane_fused);
+    auto bc = shared_table(key, [&]() {
+      std::vector<double> b2[3], mv[3], ev[3];
+      for (int d = 0; d < 3; d++) {
+        const int K = p.ng[d];
+        b2[d] = bmod2(K, o);
+        mv[d].resize(K);
+        ev[d].resize(K);
+        for (int k = 0; k < K; k++) {
+          mv[d][k] = (k <= K / 2 ? k : k - K) / p.gdim[d];
+          const double t = kPi / p.alpha;
+          ev[d][k] = std::exp(-t * t * mv[d][k] * mv[d][k]);
         }
-    if (S.plane_fused) {  // the fused planes keep ky in bit-reversed order
-      std::vector<double> t(bcd);
+      }
+      const double pivol = (double)K1 * K2 * K3 / (kPi * p.gdim[0] * p.gdim[1] * p.gdim[2]);
       const int l2 = ilog2(K2);
+      std::vector<char> out(Mh * rs);
       for (int z = 0; z < K3; z++)
         for (int y = 0; y < K2; y++) {
-          int yr = 0;
-          for (int b = 0; b < l2; b++) yr |= ((y >> b) & 1) << (l2 - 1 - b);
-          std::copy_n(&t[((size_t)z * K2 + yr) * S.Kh], S.Kh, &bcd[((size_t)z * K2 + y) * S.Kh]);
+          // the fused planes keep ky in bit-reversed order
+          int ky = y;
+          if (S.plane_fused) {
+            ky = 0;
+            for (int b = 0; b < l2; b++) ky |= ((y >> b) & 1) << (l2 - 1 - b);
+          }
+          for (int x = 0; x < S.Kh; x++) {
+            const double m2 = mv[0][x] * mv[0][x] + mv[1][ky] * mv[1][ky] + mv[2][z] * mv[2][z];
+            const double v = (x + ky + z == 0) ? 0.0
+                                                : b2[0][x] * b2[1][ky] * b2[2][z] * pivol * ev[0][x] *
+                                                      ev[1][ky] * ev[2][z] / m2;
+            const size_t at = ((size_t)z * K2 + y) * S.Kh + x;
+            if (fp32) ((float *)out.data())[at] = (float)v;
+            else ((double *)out.data())[at] = v;
+          }
         }
-    }
-    if (fp32) {
-      std::vector<float> f(bcd.begin(), bcd.end());
-      (void)hipMemcpy(S.bc, f.data(), Mh * 4, hipMemcpyHostToDevice);
-    } else {
-      (void)hipMemcpy(S.bc, bcd.data(), Mh * 8, hipMemcpyHostToDevice);
-    }
+      return out;
+    });
+    if (!bc) return fail(MD_ENOMEM, "influence function");
+    tables.push_back(bc);
+    S.bc = bc->ptr;
     for (int d = 0; d < 3; d++) {
       const int K = p.ng[d];
-      std::vector<double> tw(K);  // K/2 complex
-      for (int k = 0; k < K / 2; k++) {
-        tw[2 * k] = std::cos(2 * kPi * k / K);
-        tw[2 * k + 1] = -std::sin(2 * kPi * k / K);
-      }
-      void *dtw = dalloc<char>((size_t)K / 2 * csz);
-      if (!dtw) return fail(MD_ENOMEM, "twiddles");
-      if (fp32) {
-        std::vector<float> f(tw.begin(), tw.end());
-        (void)hipMemcpy(dtw, f.data(), (size_t)K / 2 * 8, hipMemcpyHostToDevice);
-      } else {
-        (void)hipMemcpy(dtw, tw.data(), (size_t)K / 2 * 16, hipMemcpyHostToDevice);
-      }
-      S.tw[d] = dtw;
+      std::snprintf(key, sizeof key, "tw %d %d", K, (int)fp32);
+      auto tw = shared_table(key, [&]() {
+        std::vector<char> out((size_t)K / 2 * csz);  // K/2 complex
+        for (int k = 0; k < K / 2; k++) {
+          const double c = std::cos(2 * kPi * k / K), sn = -std::sin(2 * kPi * k / K);
+          if (fp32) {
+            ((float *)out.data())[2 * k] = (float)c;
+            ((float *)out.data())[2 * k + 1] = (float)sn;
+          } else {
+            ((double *)out.data())[2 * k] = c;
+            ((double *)out.data())[2 * k + 1] = sn;
+          }
+        }
+        return out;
+      });
+      if (!tw) return fail(MD_ENOMEM, "twiddles");
+      tables.push_back(tw);
+      S.tw[d] = tw->ptr;
     }
     // FFT tiles: about 32 KB of LDS per workgroup
     const int budget = 32768;
@@ -386,6 +444,7 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   nb[10] = 0;  // set with the cell capacity (size_lists)
   nb[11] = coulomb ? nbin : 0;
   nb[12] = coulomb && S.plane_fused ? S.K[2] : 0;
+  nb[13] = cdiv(NN, mdk::kSRBlock / mdk::kSRLanes);
   S.mem1 = S.mem1s = S.mem2 = S.mem3 = nullptr;
   S.pos3s = nullptr;
   S.nb1 = S.nb2 = nullptr;
@@ -417,6 +476,7 @@ int Sys::size_lists(const double *pos) {
     maxcell = 0;
     for (auto &c : cells) maxcell = std::max(maxcell, (int)c.size());
     maxnb = 0;
+    if (rv <= 0) return;  // occupancy only
     for (size_t t = 0; t < items.size(); t++) {
       const int i = items[t], cell = cid[t];
       const int a = cell % nc[0], b = (cell / nc[0]) % nc[1], c = cell / (nc[0] * nc[1]);
@@ -467,9 +527,11 @@ int Sys::size_lists(const double *pos) {
     if (!h.mem3) return fail(MD_ENOMEM, "real-space cells");
   }
   if (coulomb) {  // spreading tile lists
-    std::vector<int> cnt(nbin, 0);
+    std::vector<int> cnt((size_t)nbin * mdk::kTileSub, 0);
     int mb = 0;
-    for (int i : charge) {
+    for (int m = 0; m < (int)charge.size(); m++) {
+      const int i = charge[m];
+      const int sub = (m >> 3) & (mdk::kTileSub - 1);
       const double c3[3] = {X[i], Y[i], Z[i]};
       int t0[3], nt[3];
       for (int d = 0; d < 3; d++) {
@@ -483,13 +545,13 @@ int Sys::size_lists(const double *pos) {
           for (int a = 0; a < nt[0]; a++) {
             const int t = (((t0[2] + c) % h.ntl[2]) * h.ntl[1] + (t0[1] + b) % h.ntl[1]) * h.ntl[0] +
                           (t0[0] + a) % h.ntl[0];
-            mb = std::max(mb, ++cnt[t]);
+            mb = std::max(mb, ++cnt[(size_t)t * mdk::kTileSub + sub]);
           }
     }
-    const int capt = round8(std::min(h.Nq, 2 * mb + 64));
+    const int capt = round8(std::min(h.Nq, 2 * mb + 32));
     if (capt != h.capt) {
-      if (h.tmem) dfree(h.tmem, (size_t)nbin * h.capt * 4);
-      h.tmem = dalloc<int>((size_t)nbin * capt);
+      if (h.tmem) dfree(h.tmem, (size_t)nbin * mdk::kTileSub * h.capt * 4);
+      h.tmem = dalloc<int>((size_t)nbin * mdk::kTileSub * capt);
       h.capt = capt;
       if (!h.tmem) return fail(MD_ENOMEM, "spreading tiles");
     }
@@ -675,7 +737,7 @@ int md_engine::rebuild() {
     P.reps = reps;
     P.nrep = nrep;
     WMap *w[kMaps] = {&P.part,  &P.chg,   &P.chgG,  &P.fx,    &P.fy,    &P.fz,   &P.cell1,
-                      &P.cell1w, &P.cell2, &P.cell3, &P.cell3c, &P.brick, &P.plane};
+                      &P.cell1w, &P.cell2, &P.cell3, &P.cell3c, &P.brick, &P.plane, &P.partG};
     for (int k = 0; k < kMaps; k++) *w[k] = WMap{maps + k * (nrep + 1), nrep, totals[k]};
     P.vpart = vpart;
     P.fp32 = fp32;
@@ -923,7 +985,8 @@ static int upload_sys(md_engine *e, Sys *s, const double *pos, const double *vel
     p32[i] = make_float4((float)p4[i].x, (float)p4[i].y, (float)p4[i].z, (float)p4[i].w);
   }
   // no kernel may still use the list buffers that size_lists may replace
-  int rc = e->sync();
+  // (a replica not yet uploaded is in no launch: the device keeps running)
+  int rc = s->uploaded ? e->sync() : MD_OK;
   if (rc) return rc;
   rc = s->size_lists(pos);
   if (rc) return e->fail(rc, s->err);
@@ -1293,7 +1356,8 @@ extern "C" int md_batch_run(md_engine *e, int32_t ncycles) {
   ENGINE_CHECK(e && ncycles >= 0);
   int rc = e->run_cycles(ncycles);
   if (rc) return rc;
-  return e->sync();
+  const hipError_t err = hipGetLastError();
+  return err == hipSuccess ? MD_OK : e->hip_fail(err, "batch run");
 }
 
 extern "C" int md_batch_status(md_engine *e, int32_t slot, int32_t *step, char *buf, int32_t len) {

@@ -168,17 +168,20 @@ __global__ __launch_bounds__(kBlock) void k_kick_drift(const Rep *__restrict__ r
 // (compt_acc.f90:865-892) + noise + field, anchors zeroed; optionally the
 // second half kick of new_position (init_update.f90:606) and the dr_max
 // bookkeeping of new_position / update_verlet_list
-// (init_update.f90:600-601, compt_acc.f90:717-724).
+// (init_update.f90:600-601, compt_acc.f90:717-724).  A workgroup takes
+// kSRBlock / kSRLanes particles: first kSRLanes lanes per particle share its
+// LJ list (lane g takes slots g, g + kSRLanes, ..., summed in a fixed tree),
+// then one lane per particle adds the rest and stores.
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_short_range(const Rep *__restrict__ reps, WMap wm,
-                                                        int mode, int step_arg,
-                                                        const double *__restrict__ eta,
-                                                        double *__restrict__ out,
-                                                        unsigned long long *lq, int lq_cap) {
+__global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict__ reps, WMap wm,
+                                                          int mode, int step_arg,
+                                                          const double *__restrict__ eta,
+                                                          double *__restrict__ out,
+                                                          unsigned long long *lq, int lq_cap) {
+  constexpr int G = kSRLanes, NP = kSRBlock / kSRLanes;
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
   const int blk = blockIdx.x - wm.blk[r];
-  const int i = blk * kBlock + threadIdx.x;
   const int step = step_arg >= 0 ? step_arg : S.ctrl->step;
   if ((mode & SR_DRMAX) && blk == 0 && threadIdx.x == 0) {
     Ctrl *c = S.ctrl;
@@ -199,24 +202,28 @@ __global__ __launch_bounds__(kBlock) void k_short_range(const Rep *__restrict__ 
     c->flag1 = f1;
     c->flag2 = f2;
   }
-  if (i >= S.NN) return;
-  const double4 pi = S.pos[i];
-  R fx = 0, fy = 0, fz = 0;
-  // WCA-LJ over the full list: class by |q_i|+|q_j| (compt_acc.f90:922-952)
-  {
-    const int n = min(S.nbn1[i], S.nbcap1);
+  __shared__ R lj[3][NP];
+  {  // WCA-LJ over the full list: class by |q_i|+|q_j| (compt_acc.f90:922-952)
+    const int g = threadIdx.x & (G - 1), pl = threadIdx.x / G;
+    const int i = blk * NP + pl;
+    const bool live = i < S.NN;
+    const int n = live ? min(S.nbn1[i], S.nbcap1) : 0;
     const R hLx = (R)S.hLx, hLy = (R)S.hLy, Lx = (R)S.Lx, Ly = (R)S.Ly;
     const R rcl2 = (R)S.rcl2, rcl3 = (R)S.rcl3, rcl4 = (R)S.rcl4;
     const R sig2 = (R)S.sigma2, sig3 = (R)S.sigma3;
-    R xi, yi, zi, qa;
-    if constexpr (sizeof(R) == 4) {
-      float4 p = S.pos32[i];
-      xi = p.x; yi = p.y; zi = p.z; qa = fabsf(p.w);
-    } else {
-      xi = pi.x; yi = pi.y; zi = pi.z; qa = fabs(pi.w);
+    R xi = 0, yi = 0, zi = 0, qa = 0;
+    if (live) {
+      if constexpr (sizeof(R) == 4) {
+        float4 p = S.pos32[i];
+        xi = p.x; yi = p.y; zi = p.z; qa = fabsf(p.w);
+      } else {
+        double4 p = S.pos[i];
+        xi = p.x; yi = p.y; zi = p.z; qa = fabs(p.w);
+      }
     }
+    R fx = 0, fy = 0, fz = 0;
     const int *nb = S.nb1 + i;
-    for (int k = 0; k < n; k++) {
+    for (int k = g; k < n; k += G) {
       const int j = nb[(size_t)k * S.NN];
       R xj, yj, zj, qj;
       if constexpr (sizeof(R) == 4) {
@@ -238,16 +245,29 @@ __global__ __launch_bounds__(kBlock) void k_short_range(const Rep *__restrict__ 
       if (rsq < cut) {
         const R ir2 = s2 / rsq;
         const R ir6 = ir2 * ir2 * ir2;
-        const R g = ir6 * (ir6 - (R)0.5) / rsq;
-        fx += g * dx;
-        fy += g * dy;
-        fz += g * dz;
+        const R gg = ir6 * (ir6 - (R)0.5) / rsq;
+        fx += gg * dx;
+        fy += gg * dy;
+        fz += gg * dz;
       }
     }
-    fx *= (R)48;
-    fy *= (R)48;
-    fz *= (R)48;
+    for (int o = 1; o < G; o <<= 1) {  // fixed tree over the lanes of the particle
+      fx += __shfl_xor(fx, o, G);
+      fy += __shfl_xor(fy, o, G);
+      fz += __shfl_xor(fz, o, G);
+    }
+    if (g == 0) {
+      lj[0][pl] = fx * (R)48;
+      lj[1][pl] = fy * (R)48;
+      lj[2][pl] = fz * (R)48;
+    }
   }
+  __syncthreads();
+  if (threadIdx.x >= NP) return;
+  const int i = blk * NP + threadIdx.x;
+  if (i >= S.NN) return;
+  const double4 pi = S.pos[i];
+  R fx = lj[0][threadIdx.x], fy = lj[1][threadIdx.x], fz = lj[2][threadIdx.x];
   double wz = 0;
   {  // wall force (compt_acc.f90:956-975)
     const double z = pi.z, Lz = S.Lz;
@@ -276,17 +296,17 @@ __global__ __launch_bounds__(kBlock) void k_short_range(const Rep *__restrict__ 
     const int b0 = S.bstart[i], b1 = S.bstart[i + 1];
     const R R02 = (R)S.R0_2, kR = (R)S.kR02;
     for (int b = b0; b < b1; b++) {
-      const int n = S.bpart[b];
+      const int nn = S.bpart[b];
       double dxd, dyd, dzd;
-      const double rs = rsq_image(pi, S.pos[n], S.Lx, S.Ly, S.hLx, S.hLy, dxd, dyd, dzd);
+      const double rs = rsq_image(pi, S.pos[nn], S.Lx, S.Ly, S.hLx, S.hLy, dxd, dyd, dzd);
       if (rs > S.R0_2) {
-        set_error(S.ctrl, MD_EBOND, i < n ? i : n, step);
+        set_error(S.ctrl, MD_EBOND, i < nn ? i : nn, step);
         continue;
       }
-      const R g = -kR / (R02 - (R)rs);
-      fx += g * (R)dxd;
-      fy += g * (R)dyd;
-      fz += g * (R)dzd;
+      const R gg = -kR / (R02 - (R)rs);
+      fx += gg * (R)dxd;
+      fy += gg * (R)dyd;
+      fz += gg * (R)dzd;
     }
   }
   double ax = (double)fx, ay = (double)fy, az = (double)fz + wz;
@@ -808,10 +828,22 @@ __global__ __launch_bounds__(kBlock) void k_spread(const Rep *__restrict__ reps,
   __shared__ unsigned long long acc[kTile * kTile * kTile];
   for (int k = threadIdx.x; k < npt; k += blockDim.x) acc[k] = 0ull;
   __syncthreads();
-  const int n = min(S.tcnt[tile], S.capt);
-  const int *mem = S.tmem + (size_t)tile * S.capt;
+  // the tile's sub-lists, concatenated
+  int off[kTileSub + 1];
+  off[0] = 0;
+#pragma unroll
+  for (int u = 0; u < kTileSub; u++) off[u + 1] = off[u] + min(S.tcnt[tile * kTileSub + u], S.capt);
+  const int n = off[kTileSub];
+  const int *mem = S.tmem + (size_t)tile * kTileSub * S.capt;
   for (int c = threadIdx.x; c < n; c += blockDim.x) {
-    const double4 p = S.pos[mem[c]];
+    int u = 0, ub = 0;
+#pragma unroll
+    for (int v = 1; v < kTileSub; v++)
+      if (c >= off[v]) {
+        u = v;
+        ub = off[v];
+      }
+    const double4 p = S.pos[mem[(size_t)u * S.capt + (c - ub)]];
     switch (o) {
       case 3: tile_add<R, 3>(acc, p, S.kg, K, x0, y0, z0, E); break;
       case 4: tile_add<R, 4>(acc, p, S.kg, K, x0, y0, z0, E); break;
@@ -1133,7 +1165,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(const Rep *__restrict__ reps,
   const int m = t / G, g = t - m * G;
   const bool live = m < S.Nq;
   {  // the tile lists are consumed: clear them for the next evaluation
-    const int ntile = S.ntl[0] * S.ntl[1] * S.ntl[2];
+    const int ntile = S.ntl[0] * S.ntl[1] * S.ntl[2] * kTileSub;
     const int nth = (wm.blk[r + 1] - wm.blk[r]) * kBlock;
     for (int k = t; k < ntile; k += nth) S.tcnt[k] = 0;
   }
@@ -1256,6 +1288,33 @@ __device__ __forceinline__ void real_pair(R xi, R yi, R zi, R qi, R xj, R yj, R 
   }
 }
 
+// FP32 form: erfc by Abramowitz & Stegun 7.1.26, t = 1/(1 + p x),
+// erfc(x) = t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-x^2), absolute
+// error <= 1.5e-7, which shares exp(-a^2 r^2) with the second term.
+template <>
+__device__ __forceinline__ void real_pair<float>(float xi, float yi, float zi, float qi, float xj,
+                                                 float yj, float zj, float qj, float Lx, float Ly,
+                                                 float hLx, float hLy, float rcc2, float alpha,
+                                                 float alpha2, float c2, float &fx, float &fy,
+                                                 float &fz) {
+  float dx = xi - xj, dy = yi - yj, dz = zi - zj;
+  if (dx > hLx) dx -= Lx; else if (dx <= -hLx) dx += Lx;
+  if (dy > hLy) dy -= Ly; else if (dy <= -hLy) dy += Ly;
+  const float rsq = dx * dx + dy * dy + dz * dz;
+  if (rsq < rcc2) {
+    const float rr = sqrtf(rsq);
+    const float e = expf(-alpha2 * rsq);
+    const float t = 1.0f / (1.0f + 0.3275911f * alpha * rr);
+    const float poly =
+        t * (0.254829592f +
+             t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
+    const float g = qi * qj / rsq * e * (poly / rr + c2);
+    fx += g * dx;
+    fy += g * dy;
+    fz += g * dz;
+  }
+}
+
 // Charges into the lists of the spreading tiles that hold their spline
 // points (one to eight) and into the cells of the real-space sweep.
 __global__ __launch_bounds__(kBlock) void k_bin_q(const Rep *__restrict__ reps, WMap wm) {
@@ -1289,7 +1348,8 @@ __global__ __launch_bounds__(kBlock) void k_bin_q(const Rep *__restrict__ reps, 
     if (a < nt[0] && b < nt[1] && c < nt[2]) {
       const int tx = (t0[0] + a) % S.ntl[0], ty = (t0[1] + b) % S.ntl[1],
                 tz = (t0[2] + c) % S.ntl[2];
-      key = (tz * S.ntl[1] + ty) * S.ntl[0] + tx;
+      // sub-list by groups of eight consecutive charges
+      key = ((tz * S.ntl[1] + ty) * S.ntl[0] + tx) * kTileSub + ((m >> 3) & (kTileSub - 1));
     }
     const int slot = wave_append(key >= 0 ? S.tcnt + key : nullptr);
     if (key >= 0) {
@@ -1666,7 +1726,7 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) 
 template <typename R>
 void launch_short(const Plan &P, hipStream_t st, int mode, int step, const double *eta,
                   double *out) {
-  hipLaunchKernelGGL(k_short_range<R>, dim3(P.part.total), dim3(kBlock), 0, st, P.reps, P.part,
+  hipLaunchKernelGGL(k_short_range<R>, dim3(P.partG.total), dim3(kSRBlock), 0, st, P.reps, P.partG,
                      mode, step, eta, out, P.lq, P.lq_cap);
 }
 

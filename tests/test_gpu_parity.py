@@ -223,3 +223,21 @@ def test_spme_general_order_fp64(order):
     _, fc = e.force_parts()
     assert orc.rel_rms(fc, o.coulomb()) < 1e-10
     e.close()
+
+
+@pytest.mark.parametrize("prec", [MD_FP64, MD_FP32])
+def test_fft_plane_paths_agree(prec, monkeypatch):
+    """The fused x/y FFT planes (spectrum in bit-reversed y order) and the
+    separate x and y passes give the same Coulomb forces."""
+    print(f"[fft planes vs passes] default prec={prec}", flush=True)
+    p, pos, vel = system("default", prec)
+    out = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("MD_FFT_PLANES", flag)
+        e = Engine(p)
+        e.upload(pos, vel)
+        e.build_lists()
+        out.append(e.force_parts()[1])
+        e.close()
+    tol = 1e-12 if prec == MD_FP64 else 2e-6
+    assert orc.rel_rms(out[0], out[1]) < tol

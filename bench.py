@@ -59,6 +59,20 @@ def kernel_bytes(p, name):
     return 0
 
 
+def roofline(ms, plist, multistep):
+    """HBM roofline of the kernel with the largest time per step (Coulomb
+    kernels run every multistep steps; list kernels only on rebuilds):
+    algorithmic bytes of one launch over all replicas / its mean duration."""
+    per_step = {k: v * (1.0 / multistep if k in Engine.COULOMB else 1.0)
+                for k, v in ms.items() if k not in Engine.LISTS}
+    dom = max(per_step, key=per_step.get)
+    b = sum(kernel_bytes(p, dom) for p in plist)
+    achieved = b / (ms[dom] * 1e-3)
+    return {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 5), "traffic": None, "bytes_per_launch": b,
+            "avg_launch_us": round(ms[dom] * 1e3, 3)}
+
+
 def prepare(seed, equil_steps):
     """Seeded default brush after FP64 equilibration in the reference's
     pre-heat form (velocities rescaled every 20 steps, main.f90:67-75): the
@@ -90,6 +104,86 @@ def time_engine(prec, state, steps, warmup, seed):
     e.run(1 + warmup, steps)  # ends with a stream synchronize
     dt = time.perf_counter() - t0
     return p, e, steps / dt, dt
+
+
+COPY_BYTES = 256 << 20  # calibration copy for the HBM byte counters
+
+
+def pmc_workload(seed):
+    """Kernels of the default brush (time_kernels launches each kernel of a
+    step and of a Coulomb evaluation), then one device copy of COPY_BYTES,
+    whose counter readings give the scale from counter units to bytes."""
+    import torch
+
+    state = prepare(seed, 300)
+    p, e, _, _ = time_engine(MD_FP32, state, 100, 10, seed)
+    e.time_kernels(10)
+    e.close()
+    a = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"pmc workload done: copy of {COPY_BYTES} bytes", flush=True)
+
+
+KERNEL_SYMBOLS = {  # kernel names of time_kernels -> device symbols (FP32 path)
+    "kick_drift": "k_kick_drift", "short_range": "k_short_range<float>", "bin_q": "k_bin_q",
+    "sort_q": "k_sort_q<float>", "spread": "k_spread<float>", "fft_x_fwd": "k_plane_fwd<float>",
+    "fft_z_conv": "k_fftz_conv<float>", "fft_x_inv": "k_plane_inv<float>", "gather": "k_gather<float>",
+    "real": "k_real_cells<float>", "bin_lj": "k_bin_lj", "sort_lj": "k_sort_lj", "build_lj": "k_build_lj"}
+
+
+def _counter_rows(path, counter):
+    import csv
+
+    rows = []
+    with open(path, newline="") as f:
+        for row in csv.DictReader(f):
+            if row.get("Counter_Name") == counter:
+                rows.append((row["Kernel_Name"], float(row["Counter_Value"])))
+    return rows
+
+
+def traffic_summary(fetch_csv, write_csv):
+    """Per-kernel HBM bytes per launch from a FETCH_SIZE pass and a WRITE_SIZE
+    pass (KiB per dispatch), each scaled so that the plain copy of
+    COPY_BYTES reads COPY_BYTES fetched and COPY_BYTES written."""
+    fetch, write = _counter_rows(fetch_csv, "FETCH_SIZE"), _counter_rows(write_csv, "WRITE_SIZE")
+
+    def copy_reading(rows):  # the copy: largest reading among kernels outside this library
+        return max(v for n, v in rows if "mdk::" not in n)
+
+    fcopy, wcopy = copy_reading(fetch), copy_reading(write)
+    sf, sw = COPY_BYTES / (fcopy * 1024.0), COPY_BYTES / (wcopy * 1024.0)
+    out = {"copy_bytes": COPY_BYTES, "copy_fetch_kib": fcopy, "copy_write_kib": wcopy,
+           "fetch_scale": sf, "write_scale": sw, "kernels": {}}
+    for name, sym in KERNEL_SYMBOLS.items():
+        fv = [v for n, v in fetch if sym in n]
+        wv = [v for n, v in write if sym in n]
+        if not fv or not wv:
+            continue
+        fk, wk = sum(fv) / len(fv), sum(wv) / len(wv)
+        out["kernels"][name] = {"dispatches": len(fv), "fetch_kib": round(fk, 1), "write_kib": round(wk, 1),
+                                "bytes_per_launch": int(fk * 1024 * sf + wk * 1024 * sw)}
+    path = os.path.join(ROOT, "profiles", "traffic_default_brush.json")
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out, indent=1))
+
+
+def traffic_from_profile(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (two rocprofv3 --pmc passes, FETCH_SIZE and WRITE_SIZE, scaled by the
+    readings of a plain copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_default_brush.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else k.get("bytes_per_launch")
 
 
 def cpu_baseline(state, seed, budget_s):
@@ -126,11 +220,23 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--serve", type=int, default=64, help="requests in the serving leg (0: skip)")
-    ap.add_argument("--serve-steps", type=int, default=2000)
-    ap.add_argument("--serve-rate", type=float, default=256.0)
+    ap.add_argument("--serve", type=int, default=256, help="requests in the serving leg (0: skip)")
+    ap.add_argument("--serve-steps", type=int, default=10000)
+    ap.add_argument("--serve-rate", type=float, default=128.0)
     ap.add_argument("--serve-seq", type=int, default=16)
+    ap.add_argument("--pmc-workload", action="store_true",
+                    help="short run for hardware-counter passes: a few hundred steps of the default brush "
+                         "and a plain device copy of known size (counter calibration); prints no JSON")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="counter_collection.csv files of the FETCH_SIZE and WRITE_SIZE passes over "
+                         "--pmc-workload: writes profiles/traffic_default_brush.json")
     args = ap.parse_args()
+    if args.traffic_from:
+        traffic_summary(*args.traffic_from)
+        return
+    if args.pmc_workload:
+        pmc_workload(args.seed)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (replicas shard "
               "across GPUs with no communication; see DESIGN.md)", file=sys.stderr)
@@ -178,14 +284,8 @@ def main():
 
     # per-kernel durations (HIP events on the engine's stream)
     ms = e32.time_kernels(20)
-    per_step = {k: v * (1.0 / p32.multistep if k in Engine.COULOMB else 1.0)
-                for k, v in ms.items() if k not in Engine.LISTS}
-    dom = max(per_step, key=per_step.get)
-    b = kernel_bytes(p32, dom)
-    achieved = b / (ms[dom] * 1e-3)
-    out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 2),
-                       "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 5),
-                       "traffic": None, "bytes_per_launch": b, "avg_launch_us": round(ms[dom] * 1e3, 3)}
+    out["roofline"] = roofline(ms, [p32], p32.multistep)
+    out["roofline"]["traffic"] = traffic_from_profile(out["roofline"]["kernel"])
     out["kernel_us"] = {k: round(v * 1e3, 3) for k, v in ms.items()}
     e32.close()
 
@@ -205,6 +305,19 @@ def main():
         out["serving"] = sv
         out["replica_latency_p50_s"] = sv["latency_p50_s"]
         out["replica_latency_p99_s"] = sv["latency_p99_s"]
+        # the same kernels at batch scale: every request of the stream at once
+        from md_brushes_amd import Batch
+        bt = Batch(MD_FP32)
+        for r in reqs:
+            bt.add(r.params, r.pos, r.vel)
+        bt.run(2)
+        msb = bt.time_kernels(10)
+        bt.close()
+        rb = roofline(msb, [r.params for r in reqs], reqs[0].params.multistep)
+        rb["replicas"] = len(reqs)
+        rb["particles"] = int(sum(r.params.NN for r in reqs))
+        out["roofline_batch"] = rb
+        out["kernel_us_batch"] = {k: round(v * 1e3, 3) for k, v in msb.items()}
 
     if not args.no_cpu:
         sps_cpu, n, el = cpu_baseline(state, args.seed, args.cpu_seconds)

@@ -293,6 +293,13 @@ class Batch:
         except Exception:
             pass
 
+    def time_kernels(self, iters=20):
+        """Mean ms per launch of each kernel over all replicas (HIP events);
+        advances the state of every replica."""
+        ms = np.zeros(15)
+        self._check(self.lib.md_time_kernels(self.h, iters, _dp(ms), 15), "md_time_kernels")
+        return dict(zip(Engine.KERNELS, ms))
+
     def add(self, params, pos, vel):
         slot = c_int32()
         self._check(self.lib.md_batch_add(self.h, byref(params), _dp(_colmajor(pos)), _dp(_colmajor(vel)),

@@ -18,7 +18,8 @@ constexpr int kTileSub = 8;       // sub-lists per spreading tile (spreads the a
 constexpr int kPlaneBlock = 512;  // threads of the fused FFT plane kernels
 constexpr int kSRLanes = 8;       // lanes per particle in the LJ pass of the short-range kernel
 constexpr int kSRBlock = 512;     // its workgroup: 64 particles
-constexpr int kRealChunk = 16;  // target charges per workgroup of the real-space sweep
+constexpr int kRealChunk = 4;     // target charges per wavefront work item of the real-space sweep
+constexpr int kRealChunkWG = 16;  // ... per workgroup work item (replicas with dense cells)
 constexpr int kHistBins = 500;  // SizeHist of the z-density profile (glob_var.f90:59)
 
 // Per-replica control block, written by kernels.
@@ -88,6 +89,7 @@ struct Rep {
   int *cnt2[2], *mem2;             // cells of the reference-form real-space list
   // real-space cell sweep at each Coulomb evaluation (edge >= r_c)
   int real_cells;                  // 1: 27-cell sweep; 0: all charge pairs
+  int real_wg;                     // sweep items taken by workgroups (dense cells)
   int nc3[3];
   double rcel3[3];
   int ccap3;
@@ -109,7 +111,6 @@ struct Rep {
   void *bc;                        // influence function K3*K2*Kh real
   const void *tw[3];               // twiddles per axis (K_d/2 complex)
   double *fsp;                     // Nq*3 reciprocal force before mean removal
-  double *rsum;                    // Nq*3 real-space sums across candidate tiles
   double *part;                    // block partial sums: 4 per charge block
   int nqblk;                       // number of charge blocks
   Ctrl *ctrl;

@@ -131,6 +131,7 @@ struct Sys {
   int nb[kMaps] = {0};
   size_t lds_x = 0, lds_y = 0, lds_z = 0, lds_sort = 0, lds_sort3 = 0, lds_plane = 0;
   int nbin = 0;  // spreading tiles
+  int nb10_wave = 0, nb10_wg = 0;  // real-space sweep grids of the two forms
   double *d_eta = nullptr, *d_out1 = nullptr, *d_out2 = nullptr;
   int *d_hist = nullptr;     // observables: z histogram of the chains
   double *d_hsum = nullptr;  // observables: per-block sums of z
@@ -308,9 +309,8 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   S.scnt3 = dalloc<int>(std::max(ncell3, 1));
   S.nbn1 = dalloc<int>(nn);
   S.nbn2 = dalloc<int>(std::max(Nq, 1));
-  S.rsum = dalloc<double>(3 * (size_t)std::max(Nq, 1));
   if (!S.cnt1 || !S.cell1_of || !S.cnt2[0] || !S.cnt2[1] || !S.cnt3 || !S.scnt3 || !S.nbn1 ||
-      !S.nbn2 || !S.rsum)
+      !S.nbn2)
     return fail(MD_ENOMEM, "cells");
 
   // SPME: mesh, spectrum, influence function, twiddles
@@ -457,6 +457,7 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
 int Sys::size_lists(const double *pos) {
   const int NN = p.NN;
   const double *X = pos, *Y = pos + NN, *Z = pos + 2 * (size_t)NN;
+  double occ3 = 0;  // mean occupancy of a charge's real-space cell
   auto scan = [&](const std::vector<int> &items, const int nc[3], double rv, int &maxcell,
                   int &maxnb) {
     const double rc[3] = {p.Lx / nc[0], p.Ly / nc[1], p.Lz / nc[2]};
@@ -474,7 +475,12 @@ int Sys::size_lists(const double *pos) {
       cells[cid[t]].push_back(i);
     }
     maxcell = 0;
-    for (auto &c : cells) maxcell = std::max(maxcell, (int)c.size());
+    double s2 = 0;
+    for (auto &c : cells) {
+      maxcell = std::max(maxcell, (int)c.size());
+      s2 += (double)c.size() * c.size();
+    }
+    occ3 = items.empty() ? 0 : s2 / items.size();
     maxnb = 0;
     if (rv <= 0) return;  // occupancy only
     for (size_t t = 0; t < items.size(); t++) {
@@ -503,7 +509,10 @@ int Sys::size_lists(const double *pos) {
   int mc1, mn1, mc2 = 0, mn2 = 0, mc3 = 0, mn3 = 0;
   scan(all, h.nc1, p.rvl_eff, mc1, mn1);
   if (h.real_verlet) scan(charge, h.nc2, p.rvc, mc2, mn2);
-  if (h.real_cells) scan(charge, h.nc3, 0.0, mc3, mn3);
+  if (h.real_cells) {
+    scan(charge, h.nc3, 0.0, mc3, mn3);
+    (void)occ3;
+  }
   auto round8 = [](int v) { return (v + 7) & ~7; };
   // LJ: dense packing bounds occupancy near 1.4 per unit volume
   const double v1 = h.rcel1[0] * h.rcel1[1] * h.rcel1[2];
@@ -559,7 +568,12 @@ int Sys::size_lists(const double *pos) {
   int np3 = 1;
   while (np3 < h.ccap3) np3 <<= 1;
   lds_sort3 = h.real_cells ? (size_t)np3 * sizeof(int) : 0;
-  nb[10] = h.real_cells ? std::min(ncell3 * cdiv(h.ccap3, mdk::kRealChunk), 1024) : 0;
+  // about one non-empty item per wavefront, or per workgroup (the form is
+  // chosen for the whole engine at rebuild)
+  nb10_wg = h.real_cells ? std::max(1, std::min(cdiv(h.Nq, mdk::kRealChunkWG), 1024)) : 0;
+  nb10_wave =
+      h.real_cells ? std::max(1, std::min(cdiv(h.Nq, mdk::kRealChunk * (mdk::kBlock / 64)), 4096)) : 0;
+  nb[10] = h.real_wg ? nb10_wg : nb10_wave;
   if (ccap1 != h.ccap1) {
     if (h.mem1) dfree(h.mem1, (size_t)ncell1 * h.ccap1 * 4);
     if (h.mem1s) dfree(h.mem1s, (size_t)ncell1 * h.ccap1 * 4);
@@ -678,6 +692,19 @@ int md_engine::rebuild() {
     if (d_reps) (void)hipFree(d_reps);
     reps_cap = std::max(2 * R, 8);
     if (hipMalloc(&d_reps, reps_cap * sizeof(Rep)) != hipSuccess) return fail(MD_ENOMEM, "descriptors");
+  }
+  // real-space sweep form: a workgroup per item while the engine holds few
+  // charges (its items alone do not fill the chip), a wavefront per item
+  // otherwise; both give the same sums (MD_REAL_SWEEP=wave|wg forces one)
+  long nq_all = 0;
+  for (Sys *x : sys) nq_all += x->h.Nq;
+  int wg = nq_all < 200000;
+  const char *rsw = std::getenv("MD_REAL_SWEEP");
+  if (rsw && !std::strcmp(rsw, "wave")) wg = 0;
+  if (rsw && !std::strcmp(rsw, "wg")) wg = 1;
+  for (Sys *x : sys) {
+    x->h.real_wg = wg;
+    x->nb[10] = wg ? x->nb10_wg : x->nb10_wave;
   }
   std::vector<Rep> hr(R);
   for (int r = 0; r < R; r++) hr[r] = sys[r]->h;

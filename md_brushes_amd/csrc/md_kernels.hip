@@ -1397,37 +1397,82 @@ __global__ __launch_bounds__(kBlock) void k_sort_q(const Rep *__restrict__ reps,
   }
 }
 
-// Candidates staged in LDS per pass of k_real_cells.
-template <typename R> struct RealTile { static constexpr int N = 1024; };
-template <> struct RealTile<float> { static constexpr int N = 2048; };
+// Orders LDS accesses of the lanes of one wavefront (LDS operations of a
+// wavefront complete in order; the fences keep the compiler from moving
+// memory accesses across).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
-// real_space (compt_acc.f90:980-1049) by a cell sweep over cells of edge
-// >= r_c: a work item is a chunk of kRealChunk target charges of one cell.
-// The members of the cell's 27 neighbour cells (reference stencil order,
-// ascending within a cell) are staged in LDS from the cell-sorted position
-// copy, each wavefront takes one target at a time, its 64 lanes stride over
-// the candidates, and a shuffle reduction (fixed order) sums the lanes.
-// Workgroups stride over the items and skip the empty ones in bulk.
-// Same pair set as the full real-space list: every j != i with r < r_c.
-template <typename R>
-__global__ __launch_bounds__(kBlock) void k_real_cells(const Rep *__restrict__ reps, WMap wm,
-                                                       double *__restrict__ out) {
-  using V = typename std::conditional<sizeof(R) == 4, float4, double4>::type;
-  constexpr int TN = RealTile<R>::N;
-  const int r = find_rep(wm, blockIdx.x);
-  const Rep &S = reps[r];
-  const int b0 = blockIdx.x - wm.blk[r], nb = wm.blk[r + 1] - wm.blk[r];
-  // work items (cell, chunk of kRealChunk targets); workgroups stride over them
-  const int nch = (S.ccap3 + kRealChunk - 1) / kRealChunk;
+// Candidates staged in LDS per pass of a k_real_cells wavefront.
+template <typename R> struct RealTile { static constexpr int N = 256; };
+template <> struct RealTile<float> { static constexpr int N = 512; };
+
+// Force on one target from a staged tile of nc candidates (candidate `self`
+// is the target itself), by one wavefront: the candidates inside r_c are
+// compacted in candidate order into hit[], the lanes stride over them, and
+// a fixed shuffle tree sums the lanes.  Both forms of the sweep use it with
+// the same tiles, so a target's sum does not depend on the form.
+template <typename R, typename V>
+__device__ __forceinline__ void target_tile(const V pi, const V *cand, int nc, int self, short *hit,
+                                            R Lx, R Ly, R hLx, R hLy, R rcc2, R alpha, R alpha2,
+                                            R c2, R &fx, R &fy, R &fz) {
+  const int lane = threadIdx.x & 63;
+  int nh = 0;
+  for (int k0 = 0; k0 < nc; k0 += 64) {
+    const int k = k0 + lane;
+    bool in = false;
+    if (k < nc && k != self) {
+      const V pj = cand[k];
+      R dx = pi.x - pj.x, dy = pi.y - pj.y;
+      const R dz = pi.z - pj.z;
+      if (dx > hLx) dx -= Lx; else if (dx <= -hLx) dx += Lx;
+      if (dy > hLy) dy -= Ly; else if (dy <= -hLy) dy += Ly;
+      in = dx * dx + dy * dy + dz * dz < rcc2;
+    }
+    const unsigned long long mk = __ballot(in);
+    if (in) hit[nh + __popcll(mk & ((1ull << lane) - 1ull))] = (short)k;
+    nh += __popcll(mk);
+  }
+  wave_sync();
+  fx = fy = fz = 0;
+  for (int h = lane; h < nh; h += 64) {
+    const V pj = cand[hit[h]];
+    real_pair<R>(pi.x, pi.y, pi.z, pi.w, pj.x, pj.y, pj.z, pj.w, Lx, Ly, hLx, hLy, rcc2, alpha,
+                 alpha2, c2, fx, fy, fz);
+  }
+  wave_sync();  // hit[] is refilled by the next target
+  for (int o = 32; o > 0; o >>= 1) {
+    fx += __shfl_xor(fx, o, 64);
+    fy += __shfl_xor(fy, o, 64);
+    fz += __shfl_xor(fz, o, 64);
+  }
+}
+
+// The sweep form for engines with few charges: a work item is a chunk of
+// kRealChunkWG targets of one cell, taken by a whole workgroup.  The 27
+// stencil cells' members are staged once per tile of RealTile<R>::N
+// candidates by all kBlock lanes; the wavefronts share the item's targets
+// (target_tile each) and sum them over the tiles in LDS (double), in the
+// same order as the wavefront form.  Workgroups stride over the items and
+// skip the empty ones in bulk.
+template <typename R, typename V>
+__device__ void real_cells_wg(const Rep &S, int b0, int nb, V *cand, short *hit_all,
+                              double *__restrict__ out) {
+  constexpr int TN = RealTile<R>::N, NW = kBlock / 64;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nch = (S.ccap3 + kRealChunkWG - 1) / kRealChunkWG;
   const int nitem = S.nc3[0] * S.nc3[1] * S.nc3[2] * nch;
-  __shared__ V cand[TN];
   __shared__ int offs[28], cids[27];
-  __shared__ int work[kBlock], wcnt[kBlock / 64];
+  __shared__ int work[kBlock], wcnt[NW];
+  __shared__ double accs[kRealChunkWG][3];
+  short *hit = hit_all + w * TN;
   const double *tot = S.qtot;
   const V *ps = (const V *)S.pos3s;
   const R Lx = (R)S.Lx, Ly = (R)S.Ly, hLx = (R)S.hLx, hLy = (R)S.hLy, rcc2 = (R)S.rcc2;
   const R alpha = (R)S.alpha, alpha2 = (R)S.alpha2, c2 = (R)S.c2;
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int it0 = b0; it0 < nitem; it0 += nb * kBlock) {
     // the workgroup's next kBlock items (its items are b0, b0 + nb, ...):
     // the non-empty ones, compacted in order
@@ -1435,13 +1480,13 @@ __global__ __launch_bounds__(kBlock) void k_real_cells(const Rep *__restrict__ r
     bool ne = false;
     if (item < nitem) {
       const int cell = item / nch;
-      ne = S.scnt3[cell] > (item - cell * nch) * kRealChunk;
+      ne = S.scnt3[cell] > (item - cell * nch) * kRealChunkWG;
     }
     const unsigned long long bal = __ballot(ne);
     if (lane == 0) wcnt[w] = __popcll(bal);
     __syncthreads();
     int off = 0, nwork = 0;
-    for (int k = 0; k < kBlock / 64; k++) {
+    for (int k = 0; k < NW; k++) {
       off += k < w ? wcnt[k] : 0;
       nwork += wcnt[k];
     }
@@ -1449,57 +1494,162 @@ __global__ __launch_bounds__(kBlock) void k_real_cells(const Rep *__restrict__ r
     __syncthreads();
     for (int q = 0; q < nwork; q++) {
       const int it = work[q];
-      const int cell = it / nch, t0 = (it - cell * nch) * kRealChunk;
-      const int nt = min(S.scnt3[cell] - t0, kRealChunk);
+      const int cell = it / nch, t0 = (it - cell * nch) * kRealChunkWG;
+      const int nt = min(S.scnt3[cell] - t0, kRealChunkWG);
       const int total = stencil_offsets(cell, S.nc3, S.scnt3, offs, cids);
       // stencil slot 13 is the cell itself: candidate offs[13] + t0 + tg is target tg
       const int self0 = offs[13] + t0;
       const V *tp = ps + (size_t)cell * S.ccap3 + t0;
+      if (threadIdx.x < 3 * kRealChunkWG) accs[threadIdx.x / 3][threadIdx.x % 3] = 0.0;
       for (int base = 0; base < total; base += TN) {
         const int nc = min(TN, total - base);
-        for (int k = threadIdx.x; k < nc; k += blockDim.x) {
+        for (int k = threadIdx.x; k < nc; k += kBlock) {
           const int t = base + k, l = stencil_slot(offs, t);
           cand[k] = ps[(size_t)cids[l] * S.ccap3 + (t - offs[l])];
         }
         __syncthreads();
-        for (int tg = w; tg < nt; tg += kBlock / 64) {
-          const V pi = tp[tg];
-          const int self = self0 + tg - base;
-          R fx = 0, fy = 0, fz = 0;
-          for (int k = lane; k < nc; k += 64) {
-            if (k == self) continue;
-            const V pj = cand[k];
-            real_pair<R>(pi.x, pi.y, pi.z, pi.w, pj.x, pj.y, pj.z, pj.w, Lx, Ly, hLx, hLy, rcc2,
-                         alpha, alpha2, c2, fx, fy, fz);
-          }
-          for (int o = 32; o > 0; o >>= 1) {
-            fx += __shfl_xor(fx, o, 64);
-            fy += __shfl_xor(fy, o, 64);
-            fz += __shfl_xor(fz, o, 64);
-          }
+        for (int tg = w; tg < nt; tg += NW) {
+          R fx, fy, fz;
+          target_tile<R>(tp[tg], cand, nc, self0 + tg - base, hit, Lx, Ly, hLx, hLy, rcc2, alpha,
+                         alpha2, c2, fx, fy, fz);
           if (lane == 0) {
-            const int m = S.mem3[(size_t)cell * S.ccap3 + t0 + tg];
-            const int i = S.charge[m];
-            if (total <= TN) {
-              store_coulomb<R>(S, m, i, S.pos[i].w, tot, fx, fy, fz, out);
-            } else {
-              // several tiles: the same wave owns this target in every tile; the
-              // first tile writes its sum, later tiles add, the last one stores
-              double *acc = S.rsum + 3 * (size_t)m;
-              if (base == 0) {
-                acc[0] = fx; acc[1] = fy; acc[2] = fz;
-              } else {
-                acc[0] += fx; acc[1] += fy; acc[2] += fz;
-              }
-              if (base + TN >= total)
-                store_coulomb<R>(S, m, i, S.pos[i].w, tot, (R)acc[0], (R)acc[1], (R)acc[2], out);
-            }
+            accs[tg][0] += (double)fx;
+            accs[tg][1] += (double)fy;
+            accs[tg][2] += (double)fz;
           }
         }
         __syncthreads();
       }
+      if (threadIdx.x < nt) {
+        const int m = S.mem3[(size_t)cell * S.ccap3 + t0 + threadIdx.x];
+        const int i = S.charge[m];
+        store_coulomb<R>(S, m, i, S.pos[i].w, tot, (R)accs[threadIdx.x][0],
+                         (R)accs[threadIdx.x][1], (R)accs[threadIdx.x][2], out);
+      }
+      __syncthreads();  // accs, offs and the tile are reused
     }
     __syncthreads();  // work[] is refilled
+  }
+}
+
+// real_space (compt_acc.f90:980-1049) by a cell sweep over cells of edge
+// >= r_c: a work item is a chunk of kRealChunk target charges of one cell,
+// and each wavefront works through its own items (engines with few charges,
+// whose items are too few to fill the chip, take real_cells_wg instead).  The members of the
+// cell's 27 neighbour cells (reference stencil order, ascending within a
+// cell) are staged in the wavefront's LDS tile from the cell-sorted position
+// copy; for each target the candidates inside r_c are compacted first, the
+// 64 lanes stride over those and a shuffle reduction (fixed order) sums the
+// lanes; lane tg keeps target tg's
+// sum over the tiles and the lanes store their targets together.
+// Wavefronts stride over the items and skip the empty ones 64 at a time.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_real_cells(const Rep *__restrict__ reps, WMap wm,
+                                                       double *__restrict__ out) {
+  using V = typename std::conditional<sizeof(R) == 4, float4, double4>::type;
+  constexpr int TN = RealTile<R>::N, NW = kBlock / 64;
+  const int r = find_rep(wm, blockIdx.x);
+  const Rep &S = reps[r];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int gw = (blockIdx.x - wm.blk[r]) * NW + w, nwv = (wm.blk[r + 1] - wm.blk[r]) * NW;
+  // item = cell * nch + chunk; wavefront gw takes items gw, gw + nwv, ...,
+  // so that the chunks of a full cell go to different wavefronts
+  const int nch = (S.ccap3 + kRealChunk - 1) / kRealChunk;
+  const int nitem = S.nc3[0] * S.nc3[1] * S.nc3[2] * nch;
+  __shared__ V cand_all[NW][TN];
+  __shared__ short hit_all[NW][TN];
+  V *cand = cand_all[w];
+  short *hit = hit_all[w];
+  const double *tot = S.qtot;
+  const V *ps = (const V *)S.pos3s;
+  const R Lx = (R)S.Lx, Ly = (R)S.Ly, hLx = (R)S.hLx, hLy = (R)S.hLy, rcc2 = (R)S.rcc2;
+  const R alpha = (R)S.alpha, alpha2 = (R)S.alpha2, c2 = (R)S.c2;
+  if (S.real_wg) {
+    real_cells_wg<R>(S, blockIdx.x - wm.blk[r], wm.blk[r + 1] - wm.blk[r], &cand_all[0][0],
+                     &hit_all[0][0], out);
+    return;
+  }
+  for (int it0 = gw; it0 < nitem; it0 += nwv * 64) {
+    // this wavefront's next 64 items
+    const int item = it0 + lane * nwv;
+    bool ne = false;
+    if (item < nitem) {
+      const int cell = item / nch;
+      ne = S.scnt3[cell] > (item - cell * nch) * kRealChunk;
+    }
+    unsigned long long pend = __ballot(ne);
+    while (pend) {
+      const int src = __ffsll((long long)pend) - 1;
+      pend &= pend - 1;
+      const int it = __shfl(item, src, 64);
+      const int cell = it / nch, t0 = (it - cell * nch) * kRealChunk;
+      const int nt = min(S.scnt3[cell] - t0, kRealChunk);
+      const int ca = cell % S.nc3[0], cb = (cell / S.nc3[0]) % S.nc3[1],
+                cc = cell / (S.nc3[0] * S.nc3[1]);
+      // stencil cell of lane l < 27, its count and exclusive offset
+      int cl = -1, cn = 0;
+      if (lane < 27) {
+        cl = stencil_cell(lane, ca, cb, cc, S.nc3);
+        cn = cl >= 0 ? S.scnt3[cl] : 0;
+      }
+      int incl = cn;
+      for (int o = 1; o < 32; o <<= 1) {
+        const int v = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += v;
+      }
+      const int excl = incl - cn;
+      const int total = __shfl(incl, 26, 64);
+      // stencil slot 13 is the cell itself: candidate self0 + tg is target tg
+      const int self0 = __shfl(excl, 13, 64) + t0;
+      // lane tg holds target tg's position and accumulates its force
+      V pt{};
+      if (lane < nt) pt = ps[(size_t)cell * S.ccap3 + t0 + lane];
+      double ax = 0, ay = 0, az = 0;
+      for (int base = 0; base < total; base += TN) {
+        const int nc = min(TN, total - base);
+        // stage the tile: TN/64 candidates per lane, loads issued together
+        constexpr int U = TN / 64;
+        size_t src[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int t = base + u * 64 + lane;
+          int l = 0;  // the largest stencil slot with excl_l <= t
+#pragma unroll
+          for (int st = 16; st > 0; st >>= 1) {
+            const int c = l + st;
+            const int ex = __shfl(excl, c < 27 ? c : 26, 64);
+            if (c < 27 && ex <= t) l = c;
+          }
+          const int lc = __shfl(cl, l, 64), lx = __shfl(excl, l, 64);
+          src[u] = (size_t)lc * S.ccap3 + (t - lx);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+          if (u * 64 + lane < nc) cand[u * 64 + lane] = ps[src[u]];
+        wave_sync();
+        for (int tg = 0; tg < nt; tg++) {
+          V pi;
+          pi.x = __shfl(pt.x, tg, 64);
+          pi.y = __shfl(pt.y, tg, 64);
+          pi.z = __shfl(pt.z, tg, 64);
+          pi.w = __shfl(pt.w, tg, 64);
+          R fx, fy, fz;
+          target_tile<R>(pi, cand, nc, self0 + tg - base, hit, Lx, Ly, hLx, hLy, rcc2, alpha,
+                         alpha2, c2, fx, fy, fz);
+          if (lane == tg) {
+            ax += (double)fx;
+            ay += (double)fy;
+            az += (double)fz;
+          }
+        }
+        wave_sync();  // the tile is refilled
+      }
+      if (lane < nt) {
+        const int m = S.mem3[(size_t)cell * S.ccap3 + t0 + lane];
+        const int i = S.charge[m];
+        store_coulomb<R>(S, m, i, S.pos[i].w, tot, (R)ax, (R)ay, (R)az, out);
+      }
+    }
   }
 }
 

@@ -241,3 +241,34 @@ def test_fft_plane_paths_agree(prec, monkeypatch):
         e.close()
     tol = 1e-12 if prec == MD_FP64 else 2e-6
     assert orc.rel_rms(out[0], out[1]) < tol
+
+
+@pytest.mark.parametrize("sweep", ["wave", "wg"])
+@pytest.mark.parametrize("name", NAMES)
+def test_real_space_sweep_forms(name, sweep, monkeypatch):
+    """Both forms of the real-space cell sweep (a wavefront or a workgroup
+    per work item) give the oracle's Coulomb forces."""
+    print(f"[real-space sweep {sweep}] {name}", flush=True)
+    monkeypatch.setenv("MD_REAL_SWEEP", sweep)
+    p, pos, vel, e = _engine(name)
+    fc = e.force_parts()[1]
+    e.close()
+    o = _oracle(name)
+    assert orc.rel_rms(fc, o.coulomb()) < 1e-10
+
+
+@pytest.mark.parametrize("prec", [MD_FP64, MD_FP32])
+def test_real_space_sweep_forms_identical(prec, monkeypatch):
+    """The two sweep forms sum every target in the same order: bitwise equal
+    Coulomb forces, so an engine may switch form with its size."""
+    print(f"[real-space sweep forms bitwise] default prec={prec}", flush=True)
+    p, pos, vel = system("default", prec)
+    out = []
+    for sweep in ("wave", "wg"):
+        monkeypatch.setenv("MD_REAL_SWEEP", sweep)
+        e = Engine(p)
+        e.upload(pos, vel)
+        e.build_lists()
+        out.append(e.force_parts()[1])
+        e.close()
+    assert np.array_equal(out[0], out[1])

@@ -115,7 +115,7 @@ def pmc_workload(seed):
     whose counter readings give the scale from counter units to bytes."""
     import torch
 
-    state = prepare(seed, 300)
+    state = prepare(seed, 2000)
     p, e, _, _ = time_engine(MD_FP32, state, 100, 10, seed)
     e.time_kernels(10)
     e.close()

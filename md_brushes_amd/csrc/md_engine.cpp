@@ -82,6 +82,13 @@ std::vector<double> bmod2(int K, int p) {
   return out;
 }
 
+// Waits for the copies and fills issued on the null stream.  A hipMemcpy
+// from pageable host memory, and a hipMemset, may return before the device
+// has finished them, and the engine's stream (non-blocking) does not wait
+// for the null stream: every such operation a kernel will read is followed
+// by this before the kernel is launched.
+hipError_t null_stream_done() { return hipStreamSynchronize(nullptr); }
+
 // Read-only device tables (influence function, twiddles) shared by every
 // replica with the same mesh, order, splitting and box: built once, freed
 // with their last user.
@@ -109,7 +116,9 @@ std::shared_ptr<DevTable> shared_table(const std::string &key,
     t->ptr = nullptr;
     return nullptr;
   }
-  if (hipMemcpy(t->ptr, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+  if (hipMemcpy(t->ptr, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      null_stream_done() != hipSuccess)
+    return nullptr;
   g_tables[k] = t;
   return t;
 }
@@ -132,6 +141,7 @@ struct Sys {
   size_t lds_x = 0, lds_y = 0, lds_z = 0, lds_sort = 0, lds_sort3 = 0, lds_plane = 0;
   int nbin = 0;  // spreading tiles
   int nb10_wave = 0, nb10_wg = 0;  // real-space sweep grids of the two forms
+  int pz_big = 1;                  // z-pass lines per workgroup in large engines
   double *d_eta = nullptr, *d_out1 = nullptr, *d_out2 = nullptr;
   int *d_hist = nullptr;     // observables: z histogram of the chains
   double *d_hsum = nullptr;  // observables: per-block sums of z
@@ -419,8 +429,9 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
     const int budget = 32768;
     S.px_tile = std::max(1, std::min(K2 * K3 / 2, budget / (K1 * csz)));
     S.py_tile = std::max(1, std::min(S.Kh, budget / (K2 * csz)));
-    // z pass: few lines per workgroup, so that the launch fills the chip
+    // z pass lines per workgroup: set at rebuild with the engine's size
     S.pz_tile = std::max(1, std::min(4, budget / (K3 * csz)));
+    pz_big = std::max(1, std::min(16, budget / (K3 * csz)));
     S.nqblk = cdiv((long)Nq * S.G, mdk::kBlock);
     S.fsp = dalloc<double>(3 * (size_t)Nq);
     S.qtot = dalloc<double>(4);
@@ -449,6 +460,7 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   S.pos3s = nullptr;
   S.nb1 = S.nb2 = nullptr;
   S.ccap1 = S.ccap2 = S.ccap3 = S.nbcap1 = S.nbcap2 = 0;
+  if (null_stream_done() != hipSuccess) return fail(MD_EHIP, "set-up copies");
   return MD_OK;
 }
 
@@ -601,6 +613,7 @@ int Sys::size_lists(const double *pos) {
   int np = 1;
   while (np < h.ccap2) np <<= 1;
   lds_sort = h.real_verlet ? (size_t)np * sizeof(int) : 0;
+  if (null_stream_done() != hipSuccess) return fail(MD_EHIP, "list buffers");
   return MD_OK;
 }
 
@@ -706,6 +719,21 @@ int md_engine::rebuild() {
     x->h.real_wg = wg;
     x->nb[10] = wg ? x->nb10_wg : x->nb10_wave;
   }
+  // z pass: few lines per workgroup (32-byte row pieces) while the engine's
+  // mesh is small, so that the launch fills the chip; whole 128-byte pieces
+  // when the engine holds many mesh points.  The lines' arithmetic is the
+  // same either way.
+  long mesh_all = 0;
+  for (Sys *x : sys)
+    if (x->coulomb) mesh_all += (long)x->h.K[0] * x->h.K[1] * x->h.K[2];
+  for (Sys *x : sys) {
+    if (!x->coulomb) continue;
+    const int csz = x->fp32 ? 8 : 16;
+    const int small = std::max(1, std::min(4, 32768 / (x->h.K[2] * csz)));
+    x->h.pz_tile = mesh_all > (16L << 20) ? x->pz_big : small;
+    x->nb[5] = cdiv((long)x->h.K[1] * x->h.Kh, x->h.pz_tile);
+    x->lds_z = (size_t)x->h.pz_tile * x->h.K[2] * csz;
+  }
   std::vector<Rep> hr(R);
   for (int r = 0; r < R; r++) hr[r] = sys[r]->h;
   e = hipMemcpy(d_reps, hr.data(), R * sizeof(Rep), hipMemcpyHostToDevice);
@@ -735,6 +763,8 @@ int md_engine::rebuild() {
   }
   e = hipMemcpy(d_maps, m.data(), m.size() * sizeof(int), hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_fail(e, "maps");
+  e = null_stream_done();
+  if (e != hipSuccess) return hip_fail(e, "maps");
   const size_t vneed = 4 * (size_t)std::max(tot[0], 1);
   if (vneed > vpart_cap) {
     if (vpart) (void)hipFree(vpart);
@@ -755,6 +785,7 @@ int md_engine::rebuild() {
     if (hipMalloc(&d_lq, lq_words * 8) != hipSuccess) return fail(MD_ENOMEM, "rebuild queue");
   }
   e = hipMemset(d_lq, 0, lq_words * 8);
+  if (e == hipSuccess) e = null_stream_done();
   if (e != hipSuccess) return hip_fail(e, "rebuild queue");
   auto grids = [](Plan &P, long nn) {
     P.grid_bin = (int)std::max(1L, std::min<long>(cdiv(nn, mdk::kBlock), 2048));
@@ -1026,6 +1057,7 @@ static int upload_sys(md_engine *e, Sys *s, const double *pos, const double *vel
     if (acc) err = hipMemcpy(s->h.ax, acc, 3 * sizeof(double) * NN, hipMemcpyHostToDevice);
     else err = hipMemset(s->h.ax, 0, 3 * sizeof(double) * NN);
   }
+  if (err == hipSuccess) err = null_stream_done();
   if (err != hipSuccess) return e->hip_fail(err, "upload");
   s->uploaded = true;
   return MD_OK;

@@ -176,10 +176,13 @@ int md_force_parts(md_engine *e, double *f_short, double *f_coul);
 
 /* Mean duration in milliseconds of each kernel of a step, timed with HIP
  * events on the engine's stream over `iters` launches (no reference
- * counterpart: measurement only).  ms[15]: kick_drift, short_range, bin_q,
- * sort_q, spread, fft_x_fwd, fft_y_fwd, fft_z_conv, fft_y_inv, fft_x_inv,
- * gather, real, bin_lj, sort_lj, build_lj.  Leaves the state advanced by the
- * timed kick-drifts. */
+ * counterpart: measurement only; short_range over 8 back-to-back launches
+ * per iteration, kick_drift as the difference between it followed by those
+ * launches and the launches alone).  ms[15]: kick_drift, short_range, bin_q, sort_q, spread,
+ * fft_x_fwd, fft_y_fwd, fft_z_conv, fft_y_inv, fft_x_inv, gather, real,
+ * bin_lj, sort_lj, build_lj (with fused FFT planes, fft_x_* time the plane
+ * kernels and fft_y_* are empty).  Leaves the state advanced by the timed
+ * kernels. */
 int md_time_kernels(md_engine *e, int32_t iters, double *ms, int32_t n);
 
 /* ---- Replica batches (no reference counterpart: the reference runs one

@@ -862,52 +862,121 @@ __global__ __launch_bounds__(kBlock) void k_spread(const Rep *__restrict__ reps,
   }
 }
 
-// In-LDS radix-2 FFTs over `nl` lines of length n = 2^ln stored line-major.
-// DIT: bit-reversed in, natural out.  DIF: natural in, bit-reversed out.
-// tw[k] = exp(-2 pi i k / n), k < n/2; sgn = -1 forward, +1 inverse.
-template <typename C>
-__device__ void fft_dit(C *buf, int ln, int nl, const C *__restrict__ tw, int sgn) {
-  const int n = 1 << ln, h = n >> 1;
-  const int nb = nl * h;
-  for (int s = 1; s <= ln; s++) {
-    const int half = 1 << (s - 1);
-    const int ts = n >> s;
-    for (int t = threadIdx.x; t < nb; t += blockDim.x) {
-      const int line = t >> (ln - 1), b = t & (h - 1);
-      const int k = b & (half - 1), g = b >> (s - 1);
-      const int i0 = line * n + (g << s) + k, i1 = i0 + half;
-      C w = tw[k * ts];
-      if (sgn > 0) w.y = -w.y;
-      const C u = buf[i0], v = cmul(buf[i1], w);
-      buf[i0] = cadd(u, v);
-      buf[i1] = csub(u, v);
-    }
-    __syncthreads();
+// LDS FFTs over `nl` lines of length n = 2^ln stored line-major: Stockham
+// autosort passes (natural order in and out), radix 8 while three or more
+// factors of two remain, then radix 4 or 2, each butterfly held in
+// registers.  A pass reads one buffer and writes the other; fft_lines
+// returns the buffer that holds the result.  tw[k] = exp(-2 pi i k / n),
+// k < n; SGN = -1 forward, +1 inverse (unnormalised, FFTW's convention).
+template <typename C, int SGN>
+__device__ __forceinline__ C rot90(C a) {  // a * (SGN i)
+  C r;
+  if (SGN < 0) { r.x = a.y; r.y = -a.x; }
+  else { r.x = -a.y; r.y = a.x; }
+  return r;
+}
+
+template <typename C, int SGN>
+__device__ __forceinline__ C mul_w8(C a) {  // a * exp(SGN i pi / 4)
+  using R = decltype(a.x);
+  const R h = (R)0.70710678118654752440;
+  C r;
+  if (SGN < 0) { r.x = (a.x + a.y) * h; r.y = (a.y - a.x) * h; }
+  else { r.x = (a.x - a.y) * h; r.y = (a.x + a.y) * h; }
+  return r;
+}
+
+template <typename C, int SGN>
+__device__ __forceinline__ void dft4(C &a0, C &a1, C &a2, C &a3) {
+  const C t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = rot90<C, SGN>(csub(a1, a3));
+  a0 = cadd(t0, t2);
+  a2 = csub(t0, t2);
+  a1 = cadd(t1, t3);
+  a3 = csub(t1, t3);
+}
+
+template <typename C, int SGN, int R>
+__device__ __forceinline__ void dft_reg(C *v) {
+  if constexpr (R == 2) {
+    const C a = v[0];
+    v[0] = cadd(a, v[1]);
+    v[1] = csub(a, v[1]);
+  } else if constexpr (R == 4) {
+    dft4<C, SGN>(v[0], v[1], v[2], v[3]);
+  } else {  // 8: two DFT4 over the even and odd points, then W8 butterflies
+    C e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    C o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<C, SGN>(e0, e1, e2, e3);
+    dft4<C, SGN>(o0, o1, o2, o3);
+    o1 = mul_w8<C, SGN>(o1);
+    o2 = rot90<C, SGN>(o2);
+    o3 = rot90<C, SGN>(mul_w8<C, SGN>(o3));
+    v[0] = cadd(e0, o0);
+    v[4] = csub(e0, o0);
+    v[1] = cadd(e1, o1);
+    v[5] = csub(e1, o1);
+    v[2] = cadd(e2, o2);
+    v[6] = csub(e2, o2);
+    v[3] = cadd(e3, o3);
+    v[7] = csub(e3, o3);
   }
 }
 
-template <typename C>
-__device__ void fft_dif(C *buf, int ln, int nl, const C *__restrict__ tw, int sgn) {
-  const int n = 1 << ln, h = n >> 1;
-  const int nb = nl * h;
-  for (int s = ln; s >= 1; s--) {
-    const int half = 1 << (s - 1);
-    const int ts = n >> s;
-    for (int t = threadIdx.x; t < nb; t += blockDim.x) {
-      const int line = t >> (ln - 1), b = t & (h - 1);
-      const int k = b & (half - 1), g = b >> (s - 1);
-      const int i0 = line * n + (g << s) + k, i1 = i0 + half;
-      C w = tw[k * ts];
-      if (sgn > 0) w.y = -w.y;
-      const C u = buf[i0], v = buf[i1];
-      buf[i0] = cadd(u, v);
-      buf[i1] = cmul(csub(u, v), w);
+// One pass: blocks of Ns outputs already transformed; radix R joins R of
+// them (input stride n/R, output stride Ns) after the twiddles of the block.
+template <typename C, int SGN, int R>
+__device__ __forceinline__ void stockham_pass(const C *src, C *dst, int ln, int nl, int Ns,
+                                              const C *__restrict__ tw) {
+  const int n = 1 << ln, nr = n / R;
+  const int lnr = ln - (R == 8 ? 3 : R == 4 ? 2 : 1);
+  for (int t = threadIdx.x; t < nl * nr; t += blockDim.x) {
+    const int line = t >> lnr, j = t & (nr - 1);
+    const int k = j & (Ns - 1);
+    const C *s = src + (line << ln) + j;
+    C v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) v[r] = s[r * nr];
+    if (Ns > 1) {
+      const int step = k * (n / (Ns * R));
+#pragma unroll
+      for (int r = 1; r < R; r++) {
+        C w = tw[r * step];
+        if (SGN > 0) w.y = -w.y;
+        v[r] = cmul(v[r], w);
+      }
     }
-    __syncthreads();
+    dft_reg<C, SGN, R>(v);
+    C *d = dst + (line << ln) + (j - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; r++) d[r * Ns] = v[r];
   }
 }
 
-__device__ __forceinline__ int bitrev(int x, int ln) { return (int)(__brev((unsigned)x) >> (32 - ln)); }
+template <typename C, int SGN>
+__device__ C *fft_lines(C *a, C *b, int ln, int nl, const C *__restrict__ tw) {
+  int rem = ln, Ns = 1;
+  C *src = a, *dst = b;
+  while (rem > 0) {
+    if (rem >= 3) {
+      stockham_pass<C, SGN, 8>(src, dst, ln, nl, Ns, tw);
+      Ns <<= 3;
+      rem -= 3;
+    } else if (rem == 2) {
+      stockham_pass<C, SGN, 4>(src, dst, ln, nl, Ns, tw);
+      Ns <<= 2;
+      rem -= 2;
+    } else {
+      stockham_pass<C, SGN, 2>(src, dst, ln, nl, Ns, tw);
+      Ns <<= 1;
+      rem -= 1;
+    }
+    __syncthreads();
+    C *x = src;
+    src = dst;
+    dst = x;
+  }
+  return src;
+}
 
 // Forward x pass: pairs of real mesh rows as one complex FFT of length K1,
 // split into two half spectra.
@@ -915,12 +984,12 @@ template <typename R>
 __global__ __launch_bounds__(kBlock) void k_fftx_fwd(const Rep *__restrict__ reps, WMap wm) {
   using C = typename CT<R>::C;
   extern __shared__ char smem[];
-  C *buf = (C *)smem;
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
   const int K1 = S.K[0], ln = S.lK[0], Kh = S.Kh;
   const int npairs = S.K[1] * S.K[2] / 2;
   const int P = S.px_tile;
+  C *ba = (C *)smem, *bb = ba + (size_t)P * K1;
   const int pair0 = (blockIdx.x - wm.blk[r]) * P;
   const int nl = min(P, npairs - pair0);
   const R *q = (const R *)S.meshr;
@@ -930,10 +999,10 @@ __global__ __launch_bounds__(kBlock) void k_fftx_fwd(const Rep *__restrict__ rep
     C v;
     v.x = q[row * K1 + x];
     v.y = q[(row + 1) * K1 + x];
-    buf[l * K1 + bitrev(x, ln)] = v;
+    ba[l * K1 + x] = v;
   }
   __syncthreads();
-  fft_dit<C>(buf, ln, nl, (const C *)S.tw[0], -1);
+  const C *buf = fft_lines<C, -1>(ba, bb, ln, nl, (const C *)S.tw[0]);
   C *spec = (C *)S.spec;
   for (int t = threadIdx.x; t < nl * Kh; t += blockDim.x) {
     const int l = t / Kh, k = t - l * Kh;
@@ -955,12 +1024,12 @@ template <typename R>
 __global__ __launch_bounds__(kBlock) void k_fftx_inv(const Rep *__restrict__ reps, WMap wm) {
   using C = typename CT<R>::C;
   extern __shared__ char smem[];
-  C *buf = (C *)smem;
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
   const int K1 = S.K[0], ln = S.lK[0], Kh = S.Kh;
   const int npairs = S.K[1] * S.K[2] / 2;
   const int P = S.px_tile;
+  C *ba = (C *)smem, *bb = ba + (size_t)P * K1;
   const int pair0 = (blockIdx.x - wm.blk[r]) * P;
   const int nl = min(P, npairs - pair0);
   const C *spec = (const C *)S.spec;
@@ -979,30 +1048,30 @@ __global__ __launch_bounds__(kBlock) void k_fftx_inv(const Rep *__restrict__ rep
     }
     z.x = A.x - B.y;  // Z = A + iB
     z.y = A.y + B.x;
-    buf[l * K1 + k] = z;
+    ba[l * K1 + k] = z;
   }
   __syncthreads();
-  fft_dif<C>(buf, ln, nl, (const C *)S.tw[0], +1);
+  const C *buf = fft_lines<C, +1>(ba, bb, ln, nl, (const C *)S.tw[0]);
   R *u = (R *)S.meshu;
   for (int t = threadIdx.x; t < nl * K1; t += blockDim.x) {
     const int l = t / K1, x = t - l * K1;
-    const C v = buf[l * K1 + bitrev(x, ln)];
+    const C v = buf[l * K1 + x];
     const size_t row = 2 * (size_t)(pair0 + l);
     u[row * K1 + x] = v.x;
     u[(row + 1) * K1 + x] = v.y;
   }
 }
 
-// y pass (forward or inverse), in place on the half spectrum: tiles of Ty
-// adjacent kx lines for one kz.
-template <typename R>
-__global__ __launch_bounds__(kBlock) void k_ffty(const Rep *__restrict__ reps, WMap wm, int sgn) {
+// y pass (forward or inverse), on the half spectrum: tiles of Ty adjacent
+// kx lines for one kz.
+template <typename R, int SGN>
+__global__ __launch_bounds__(kBlock) void k_ffty(const Rep *__restrict__ reps, WMap wm) {
   using C = typename CT<R>::C;
   extern __shared__ char smem[];
-  C *buf = (C *)smem;
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
   const int K2 = S.K[1], ln = S.lK[1], Kh = S.Kh, T = S.py_tile;
+  C *ba = (C *)smem, *bb = ba + (size_t)T * K2;
   const int tiles = (Kh + T - 1) / T;
   const int b = blockIdx.x - wm.blk[r];
   const int z = b / tiles, x0 = (b - z * tiles) * T;
@@ -1010,10 +1079,10 @@ __global__ __launch_bounds__(kBlock) void k_ffty(const Rep *__restrict__ reps, W
   C *spec = (C *)S.spec + (size_t)z * K2 * Kh + x0;
   for (int t = threadIdx.x; t < nl * K2; t += blockDim.x) {
     const int y = t / nl, l = t - y * nl;
-    buf[l * K2 + bitrev(y, ln)] = spec[(size_t)y * Kh + l];
+    ba[l * K2 + y] = spec[(size_t)y * Kh + l];
   }
   __syncthreads();
-  fft_dit<C>(buf, ln, nl, (const C *)S.tw[1], sgn);
+  const C *buf = fft_lines<C, SGN>(ba, bb, ln, nl, (const C *)S.tw[1]);
   for (int t = threadIdx.x; t < nl * K2; t += blockDim.x) {
     const int y = t / nl, l = t - y * nl;
     spec[(size_t)y * Kh + l] = buf[l * K2 + y];
@@ -1021,10 +1090,9 @@ __global__ __launch_bounds__(kBlock) void k_ffty(const Rep *__restrict__ reps, W
 }
 
 // Forward x and y passes of one z-plane in LDS: pairs of real rows -> one
-// complex FFT of length K1 each (bit-reversed load, DIT), split into two
-// half spectra stored line-major by kx, then DIF along y.  The plane leaves
-// in bit-reversed y order; the z pass and the influence function use that
-// order, and the inverse y pass (DIT) takes it back.
+// complex FFT of length K1 each, split into two half spectra stored
+// line-major by kx, then the y FFT of each kx line.  Two LDS regions of
+// max(K2/2 * K1, Kh * K2) points: the passes alternate between them.
 template <typename R>
 __global__ __launch_bounds__(kPlaneBlock) void k_plane_fwd(const Rep *__restrict__ reps, WMap wm) {
   using C = typename CT<R>::C;
@@ -1033,18 +1101,19 @@ __global__ __launch_bounds__(kPlaneBlock) void k_plane_fwd(const Rep *__restrict
   const Rep &S = reps[r];
   const int z = blockIdx.x - wm.blk[r];
   const int K1 = S.K[0], K2 = S.K[1], Kh = S.Kh, l1 = S.lK[0], l2 = S.lK[1];
-  C *A = (C *)smem;                  // K2/2 lines of K1
-  C *P = A + (size_t)(K2 / 2) * K1;  // Kh lines of K2
+  const int reg = max((K2 / 2) * K1, Kh * K2);
+  C *ra = (C *)smem, *rb = ra + reg;
   const R *q = (const R *)S.meshr + (size_t)z * K2 * K1;
   for (int t = threadIdx.x; t < (K2 / 2) * K1; t += blockDim.x) {
     const int l = t / K1, x = t - l * K1;
     C v;
     v.x = q[(size_t)(2 * l) * K1 + x];
     v.y = q[(size_t)(2 * l + 1) * K1 + x];
-    A[l * K1 + bitrev(x, l1)] = v;
+    ra[l * K1 + x] = v;
   }
   __syncthreads();
-  fft_dit<C>(A, l1, K2 / 2, (const C *)S.tw[0], -1);
+  const C *A = fft_lines<C, -1>(ra, rb, l1, K2 / 2, (const C *)S.tw[0]);
+  C *P = A == ra ? rb : ra;
   for (int t = threadIdx.x; t < (K2 / 2) * Kh; t += blockDim.x) {
     const int l = t / Kh, k = t - l * Kh;
     const C zk = A[l * K1 + k], zm = A[l * K1 + ((K1 - k) & (K1 - 1))];
@@ -1057,17 +1126,17 @@ __global__ __launch_bounds__(kPlaneBlock) void k_plane_fwd(const Rep *__restrict
     P[k * K2 + 2 * l + 1] = b;
   }
   __syncthreads();
-  fft_dif<C>(P, l2, Kh, (const C *)S.tw[1], -1);
+  const C *Y = fft_lines<C, -1>(P, (C *)A, l2, Kh, (const C *)S.tw[1]);
   C *spec = (C *)S.spec + (size_t)z * K2 * Kh;
   for (int t = threadIdx.x; t < K2 * Kh; t += blockDim.x) {
-    const int yp = t / Kh, k = t - yp * Kh;
-    spec[t] = P[k * K2 + yp];
+    const int y = t / Kh, k = t - y * Kh;
+    spec[t] = Y[k * K2 + y];
   }
 }
 
-// Inverse y and x passes of one z-plane: DIT along y from the bit-reversed
-// order, rebuild pairs of rows Z = A + iB over the full x range (A, B
-// Hermitian), DIF along x, real and imaginary parts are the two rows.
+// Inverse y and x passes of one z-plane: inverse y FFT of each kx line,
+// pairs of rows Z = A + iB rebuilt over the full x range (A, B Hermitian),
+// inverse x FFT, real and imaginary parts are the two rows.
 template <typename R>
 __global__ __launch_bounds__(kPlaneBlock) void k_plane_inv(const Rep *__restrict__ reps, WMap wm) {
   using C = typename CT<R>::C;
@@ -1076,15 +1145,16 @@ __global__ __launch_bounds__(kPlaneBlock) void k_plane_inv(const Rep *__restrict
   const Rep &S = reps[r];
   const int z = blockIdx.x - wm.blk[r];
   const int K1 = S.K[0], K2 = S.K[1], Kh = S.Kh, l1 = S.lK[0], l2 = S.lK[1];
-  C *A = (C *)smem;
-  C *P = A + (size_t)(K2 / 2) * K1;
+  const int reg = max((K2 / 2) * K1, Kh * K2);
+  C *ra = (C *)smem, *rb = ra + reg;
   const C *spec = (const C *)S.spec + (size_t)z * K2 * Kh;
   for (int t = threadIdx.x; t < K2 * Kh; t += blockDim.x) {
-    const int yp = t / Kh, k = t - yp * Kh;
-    P[k * K2 + yp] = spec[t];
+    const int y = t / Kh, k = t - y * Kh;
+    ra[k * K2 + y] = spec[t];
   }
   __syncthreads();
-  fft_dit<C>(P, l2, Kh, (const C *)S.tw[1], +1);
+  const C *P = fft_lines<C, +1>(ra, rb, l2, Kh, (const C *)S.tw[1]);
+  C *A = P == ra ? rb : ra;
   for (int t = threadIdx.x; t < (K2 / 2) * K1; t += blockDim.x) {
     const int l = t / K1, k = t - l * K1;
     C a, b, v;
@@ -1102,11 +1172,11 @@ __global__ __launch_bounds__(kPlaneBlock) void k_plane_inv(const Rep *__restrict
     A[l * K1 + k] = v;
   }
   __syncthreads();
-  fft_dif<C>(A, l1, K2 / 2, (const C *)S.tw[0], +1);
+  const C *X = fft_lines<C, +1>(A, (C *)P, l1, K2 / 2, (const C *)S.tw[0]);
   R *u = (R *)S.meshu + (size_t)z * K2 * K1;
   for (int t = threadIdx.x; t < (K2 / 2) * K1; t += blockDim.x) {
     const int l = t / K1, x = t - l * K1;
-    const C v = A[l * K1 + bitrev(x, l1)];
+    const C v = X[l * K1 + x];
     u[(size_t)(2 * l) * K1 + x] = v.x;
     u[(size_t)(2 * l + 1) * K1 + x] = v.y;
   }
@@ -1119,10 +1189,10 @@ template <typename R>
 __global__ __launch_bounds__(kBlock) void k_fftz_conv(const Rep *__restrict__ reps, WMap wm) {
   using C = typename CT<R>::C;
   extern __shared__ char smem[];
-  C *buf = (C *)smem;
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
   const int K3 = S.K[2], ln = S.lK[2], T = S.pz_tile;
+  C *ba = (C *)smem, *bb = ba + (size_t)T * K3;
   const int f0 = (blockIdx.x - wm.blk[r]) * T;
   const size_t zs = (size_t)S.K[1] * S.Kh;
   const int nl = min(T, (int)zs - f0);
@@ -1130,10 +1200,10 @@ __global__ __launch_bounds__(kBlock) void k_fftz_conv(const Rep *__restrict__ re
   const R *bc = (const R *)S.bc + f0;
   for (int t = threadIdx.x; t < nl * K3; t += blockDim.x) {
     const int z = t / nl, l = t - z * nl;
-    buf[l * K3 + bitrev(z, ln)] = spec[z * zs + l];
+    ba[l * K3 + z] = spec[z * zs + l];
   }
   __syncthreads();
-  fft_dit<C>(buf, ln, nl, (const C *)S.tw[2], -1);
+  C *buf = fft_lines<C, -1>(ba, bb, ln, nl, (const C *)S.tw[2]);
   for (int t = threadIdx.x; t < nl * K3; t += blockDim.x) {
     const int z = t / nl, l = t - z * nl;
     const R g = bc[z * zs + l];
@@ -1143,10 +1213,10 @@ __global__ __launch_bounds__(kBlock) void k_fftz_conv(const Rep *__restrict__ re
     buf[l * K3 + z] = v;
   }
   __syncthreads();
-  fft_dif<C>(buf, ln, nl, (const C *)S.tw[2], +1);
+  const C *out = fft_lines<C, +1>(buf, buf == ba ? bb : ba, ln, nl, (const C *)S.tw[2]);
   for (int t = threadIdx.x; t < nl * K3; t += blockDim.x) {
     const int z = t / nl, l = t - z * nl;
-    spec[z * zs + l] = buf[l * K3 + bitrev(z, ln)];
+    spec[z * zs + l] = out[l * K3 + z];
   }
 }
 
@@ -1850,12 +1920,12 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) 
     hipLaunchKernelGGL(k_fftx_fwd<R>, dim3(P.fx.total), dim3(kBlock), P.lds_x, st, P.reps, P.fx);
   mark();
   if (P.fy.total)
-    hipLaunchKernelGGL(k_ffty<R>, dim3(P.fy.total), dim3(kBlock), P.lds_y, st, P.reps, P.fy, -1);
+    hipLaunchKernelGGL((k_ffty<R, -1>), dim3(P.fy.total), dim3(kBlock), P.lds_y, st, P.reps, P.fy);
   mark();
   hipLaunchKernelGGL(k_fftz_conv<R>, dim3(P.fz.total), dim3(kBlock), P.lds_z, st, P.reps, P.fz);
   mark();
   if (P.fy.total)
-    hipLaunchKernelGGL(k_ffty<R>, dim3(P.fy.total), dim3(kBlock), P.lds_y, st, P.reps, P.fy, +1);
+    hipLaunchKernelGGL((k_ffty<R, +1>), dim3(P.fy.total), dim3(kBlock), P.lds_y, st, P.reps, P.fy);
   mark();
   if (P.plane.total)
     hipLaunchKernelGGL(k_plane_inv<R>, dim3(P.plane.total), dim3(kPlaneBlock), P.lds_plane, st, P.reps,

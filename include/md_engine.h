@@ -185,6 +185,15 @@ int md_force_parts(md_engine *e, double *f_short, double *f_coul);
  * kernels. */
 int md_time_kernels(md_engine *e, int32_t iters, double *ms, int32_t n);
 
+/* Chooses the engine's launch forms by timing its Coulomb evaluation under
+ * each (no reference counterpart: tuning only): fused FFT planes or
+ * separate x/y passes, z-pass lines per workgroup, a wavefront or a
+ * workgroup per real-space work item.  The forms give the same results.
+ * choice[3] (may be null) receives fused (0/1), z lines, workgroup (0/1).
+ * Leaves the state as it was; the held Coulomb force is recomputed for the
+ * current positions. */
+int md_autotune(md_engine *e, int32_t iters, int32_t *choice);
+
 /* ---- Replica batches (no reference counterpart: the reference runs one
  * system per process).  One engine holds many independent replicas of
  * uneven size and one precision; every kernel launch covers all of them, and

@@ -101,6 +101,7 @@ def load_library(path=None):
         "md_force_parts": ([c_void_p, dp, dp], _I),
         "md_last_error": ([c_void_p, c_char_p, _I], _I),
         "md_time_kernels": ([c_void_p, _I, dp, _I], _I),
+        "md_autotune": ([c_void_p, _I, ip], _I),
         "md_destroy": ([c_void_p], None),
         "md_batch_create": ([POINTER(c_void_p), _I, _I], _I),
         "md_batch_add": ([c_void_p, P, dp, dp, ip], _I),
@@ -250,6 +251,14 @@ class Engine:
     def run(self, step0, nsteps, preheat=False):
         self._check(self.lib.md_run(self.h, step0, nsteps, int(preheat)), "md_run")
 
+    def autotune(self, iters=5):
+        """Times the launch forms (fused FFT planes, z-pass lines per
+        workgroup, real-space sweep form) and keeps the fastest; results do
+        not depend on them.  Returns the choice."""
+        ch = (c_int32 * 3)()
+        self._check(self.lib.md_autotune(self.h, iters, ch), "md_autotune")
+        return {"fused_planes": ch[0], "z_lines": ch[1], "real_workgroup": ch[2]}
+
     KERNELS = ["kick_drift", "short_range", "bin_q", "sort_q", "spread", "fft_x_fwd", "fft_y_fwd",
                "fft_z_conv", "fft_y_inv", "fft_x_inv", "gather", "real", "bin_lj", "sort_lj", "build_lj"]
     COULOMB = KERNELS[2:12]
@@ -292,6 +301,12 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+    def autotune(self, iters=5):
+        """As Engine.autotune, over all replicas of the batch."""
+        ch = (c_int32 * 3)()
+        self._check(self.lib.md_autotune(self.h, iters, ch), "md_autotune")
+        return {"fused_planes": ch[0], "z_lines": ch[1], "real_workgroup": ch[2]}
 
     def time_kernels(self, iters=20):
         """Mean ms per launch of each kernel over all replicas (HIP events);

@@ -142,6 +142,8 @@ struct Sys {
   int nbin = 0;  // spreading tiles
   int nb10_wave = 0, nb10_wg = 0;  // real-space sweep grids of the two forms
   int pz_big = 1;                  // z-pass lines per workgroup in large engines
+  bool can_fuse = false;           // both planes of a fused FFT kernel fit in LDS
+  size_t plane_bytes = 0;
   double *d_eta = nullptr, *d_out1 = nullptr, *d_out2 = nullptr;
   int *d_hist = nullptr;     // observables: z histogram of the chains
   double *d_hsum = nullptr;  // observables: per-block sums of z
@@ -355,11 +357,13 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
     if (!S.tcnt) return fail(MD_ENOMEM, "spreading tiles");
     // x and y passes fused per z-plane when both planes fit in LDS
     // two regions of max(K2/2 * K1, Kh * K2) points (the FFT passes alternate)
-    lds_plane = 2 * std::max((size_t)(K2 / 2) * K1, (size_t)S.Kh * K2) * csz;
+    plane_bytes = 2 * std::max((size_t)(K2 / 2) * K1, (size_t)S.Kh * K2) * csz;
+    lds_plane = plane_bytes;
     // MD_FFT_PLANES=0 keeps the separate x and y passes (for comparison)
     const char *fp = std::getenv("MD_FFT_PLANES");
     // (at most 64 KB: larger planes leave one workgroup per CU and lose)
-    S.plane_fused = lds_plane <= std::min<size_t>(65536, lds_limit()) && !(fp && fp[0] == '0');
+    can_fuse = lds_plane <= std::min<size_t>(65536, lds_limit()) && !(fp && fp[0] == '0');
+    S.plane_fused = can_fuse;
     if (!S.plane_fused) lds_plane = 0;
     // influence function of PME_BC (compt_acc.f90:1557-1598), half spectrum
     const int o = S.order;
@@ -637,6 +641,11 @@ struct md_engine {
   hipGraph_t graph = nullptr;
   bool dirty = true;
   bool batch = false;              // made by md_batch_create (replicas pre-heat)
+  // launch forms set by md_autotune (-1 / 0: the size heuristics); none
+  // of them changes a result
+  int tune_real = -1;              // real-space sweep: 0 wavefront, 1 workgroup per item
+  int tune_pz = 0;                 // z-pass lines per workgroup
+  int tune_fuse = -1;              // fused FFT planes where they fit: 0 no, 1 yes
   std::string err;
 
   ~md_engine() {
@@ -706,7 +715,7 @@ int md_engine::rebuild() {
   // otherwise; both give the same sums (MD_REAL_SWEEP=wave|wg forces one)
   long nq_all = 0;
   for (Sys *x : sys) nq_all += x->h.Nq;
-  int wg = nq_all < 200000;
+  int wg = tune_real >= 0 ? tune_real : nq_all < 200000;
   const char *rsw = std::getenv("MD_REAL_SWEEP");
   if (rsw && !std::strcmp(rsw, "wave")) wg = 0;
   if (rsw && !std::strcmp(rsw, "wg")) wg = 1;
@@ -726,8 +735,19 @@ int md_engine::rebuild() {
     const int csz = x->fp32 ? 8 : 16;
     const int small = std::max(1, std::min(4, 16384 / (x->h.K[2] * csz)));
     x->h.pz_tile = mesh_all > (16L << 20) ? x->pz_big : small;
+    const char *zl = std::getenv("MD_Z_LINES");  // forces the z-pass lines, for comparison
+    const int want_pz = zl ? std::atoi(zl) : tune_pz;
+    if (want_pz > 0) x->h.pz_tile = std::max(1, std::min(want_pz, 16384 / (x->h.K[2] * csz)));
     x->nb[5] = cdiv((long)x->h.K[1] * x->h.Kh, x->h.pz_tile);
     x->lds_z = 2 * (size_t)x->h.pz_tile * x->h.K[2] * csz;
+    // fused planes where they fit, unless tuned off
+    Rep &S = x->h;
+    const char *fp = std::getenv("MD_FFT_PLANES");  // =0 keeps separate x and y passes
+    S.plane_fused = x->can_fuse && tune_fuse != 0 && !(fp && fp[0] == '0');
+    x->lds_plane = S.plane_fused ? x->plane_bytes : 0;
+    x->nb[3] = !S.plane_fused ? cdiv((long)S.K[1] * S.K[2] / 2, S.px_tile) : 0;
+    x->nb[4] = !S.plane_fused ? S.K[2] * cdiv(S.Kh, S.py_tile) : 0;
+    x->nb[12] = S.plane_fused ? S.K[2] : 0;
   }
   std::vector<Rep> hr(R);
   for (int r = 0; r < R; r++) hr[r] = sys[r]->h;
@@ -1349,6 +1369,74 @@ extern "C" int md_time_kernels(md_engine *e, int32_t iters, double *ms, int32_t 
   rc = e->check_all();
   for (auto &x : ev) (void)hipEventDestroy(x);
   for (int k = 0; k < 15; k++) ms[k] = acc[k] / iters;
+  return rc;
+}
+
+// Chooses the launch forms (real-space sweep form, z-pass lines per
+// workgroup, fused FFT planes) by timing one Coulomb evaluation of the
+// engine's replicas under each: the forms give the same results, so only
+// the time differs.  Knobs are tuned one after the other, `iters`
+// evaluations each.
+extern "C" int md_autotune(md_engine *e, int32_t iters, int32_t *choice) {
+  ENGINE_CHECK(e && iters > 0);
+  FIRST(e);
+  STATE_CHECK(e);
+  bool any_coulomb = false;
+  for (Sys *x : e->sys) any_coulomb |= x->coulomb;
+  if (!any_coulomb) return MD_OK;
+  int rc = e->check_all();
+  if (rc) return rc;
+  hipEvent_t a, b;
+  (void)hipEventCreate(&a);
+  (void)hipEventCreate(&b);
+  auto time_coulomb = [&](double &ms) -> int {
+    int r2 = e->rebuild();
+    if (r2) return r2;
+    const Plan &B = e->plan;
+    auto one = [&]() {
+      if (e->fp32) mdk::launch_coulomb<float>(B, e->st, nullptr);
+      else mdk::launch_coulomb<double>(B, e->st, nullptr);
+    };
+    one();  // warm
+    (void)hipEventRecord(a, e->st);
+    for (int k = 0; k < iters; k++) one();
+    (void)hipEventRecord(b, e->st);
+    (void)hipEventSynchronize(b);
+    float t = 0;
+    (void)hipEventElapsedTime(&t, a, b);
+    ms = t / iters;
+    return MD_OK;
+  };
+  struct Knob {
+    int *field;
+    std::vector<int> options;
+  };
+  Knob knobs[] = {{&e->tune_fuse, {0, 1}}, {&e->tune_pz, {2, 4, 8, 16}}, {&e->tune_real, {0, 1}}};
+  for (Knob &k : knobs) {
+    int best = *k.field;
+    double best_ms = 1e30;
+    for (int opt : k.options) {
+      *k.field = opt;
+      double ms = 0;
+      rc = time_coulomb(ms);
+      if (rc) break;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = opt;
+      }
+    }
+    *k.field = best;
+    if (rc) break;
+  }
+  e->dirty = true;
+  if (rc == MD_OK) rc = e->check_all();
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (choice) {
+    choice[0] = e->tune_fuse;
+    choice[1] = e->tune_pz;
+    choice[2] = e->tune_real;
+  }
   return rc;
 }
 

@@ -239,8 +239,40 @@ def test_fft_plane_paths_agree(prec, monkeypatch):
         e.build_lists()
         out.append(e.force_parts()[1])
         e.close()
-    tol = 1e-12 if prec == MD_FP64 else 2e-6
-    assert orc.rel_rms(out[0], out[1]) < tol
+    assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("prec", [MD_FP64, MD_FP32])
+def test_launch_forms_do_not_change_results(prec, monkeypatch):
+    """Every launch form the autotuner may choose (fused planes or x/y
+    passes, z-pass lines per workgroup, sweep form) gives bitwise the same
+    Coulomb forces, and autotuning keeps them."""
+    print(f"[launch forms] default prec={prec}", flush=True)
+    p, pos, vel = system("default", prec)
+
+    def coulomb(env):
+        for k in ("MD_FFT_PLANES", "MD_Z_LINES", "MD_REAL_SWEEP"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        e = Engine(p)
+        e.upload(pos, vel)
+        e.build_lists()
+        f = e.force_parts()[1]
+        return e, f
+
+    e, ref = coulomb({})
+    e.close()
+    for env in ({"MD_FFT_PLANES": "0", "MD_Z_LINES": "2"}, {"MD_Z_LINES": "16", "MD_REAL_SWEEP": "wave"},
+                {"MD_FFT_PLANES": "0", "MD_Z_LINES": "8", "MD_REAL_SWEEP": "wg"}):
+        e, f = coulomb(env)
+        e.close()
+        assert np.array_equal(f, ref), env
+    e, _ = coulomb({})
+    choice = e.autotune(2)
+    assert choice["z_lines"] in (2, 4, 8, 16) and choice["fused_planes"] in (0, 1)
+    assert np.array_equal(e.force_parts()[1], ref)
+    e.close()
 
 
 @pytest.mark.parametrize("sweep", ["wave", "wg"])

@@ -286,6 +286,7 @@ class Batch:
         self.lib = load_library()
         self.h = c_void_p()
         self.sizes = {}
+        self.multistep = multistep
         rc = self.lib.md_batch_create(byref(self.h), precision, multistep)
         self._check(rc, "md_batch_create")
 

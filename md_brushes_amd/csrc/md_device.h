@@ -88,7 +88,7 @@ struct Rep {
   int *cnt1, *mem1, *mem1s, *cell1_of;
   int *cnt2[2], *mem2;             // cells of the reference-form real-space list
   // real-space cell sweep at each Coulomb evaluation (edge >= r_c)
-  int real_cells;                  // 1: 27-cell sweep; 0: all charge pairs
+  int real_cells;                  // 1: the real-space cell sweep runs (charged systems)
   int real_wg;                     // sweep items taken by workgroups (dense cells)
   int nc3[3];
   double rcel3[3];
@@ -96,7 +96,6 @@ struct Rep {
   int *cnt3, *mem3, *scnt3;
   void *pos3s;                     // positions in cell-sorted order (float4 / double4)
   double *qtot;                    // mean reciprocal force (3) and M_z of the evaluation
-  int G;                           // lanes per charge in spread and gather
   int *nbn1, *nb1;                 // LJ list: length per particle, ELL slots
   int *nbn2, *nb2;                 // real list: per charge slot m
   void *meshr;                     // charge mesh Q (real type of the path) K3*K2*K1

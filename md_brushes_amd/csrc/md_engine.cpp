@@ -282,7 +282,6 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
     S.real_cells = 1;
     ncell3 = S.nc3[0] * S.nc3[1] * S.nc3[2];
   }
-  S.G = (coulomb && p.ordr[0] > 4) ? 64 : 16;
 
   // state and topology on the device
   const size_t nn = NN;
@@ -431,7 +430,7 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
     // z pass lines per workgroup: set at rebuild with the engine's size
     S.pz_tile = std::max(1, std::min(4, budget / (K3 * csz)));
     pz_big = std::max(1, std::min(16, budget / (K3 * csz)));
-    S.nqblk = cdiv((long)Nq * S.G, mdk::kBlock);
+    S.nqblk = cdiv((long)Nq, mdk::kBlock);  // gather blocks (one lane per charge)
     S.fsp = dalloc<double>(3 * (size_t)Nq);
     S.qtot = dalloc<double>(4);
     if (!S.qtot) return fail(MD_ENOMEM, "spme totals");
@@ -822,7 +821,6 @@ int md_engine::rebuild() {
   plan.lq_cap = R;
   grids(plan, nn_all);
   plan.lds_x = plan.lds_y = plan.lds_z = plan.lds_sort = plan.lds_sort3 = plan.lds_plane = 0;
-  plan.pairs_any = false;
   for (Sys *s : sys) {
     plan.lds_x = std::max(plan.lds_x, s->lds_x);
     plan.lds_y = std::max(plan.lds_y, s->lds_y);
@@ -830,7 +828,6 @@ int md_engine::rebuild() {
     plan.lds_plane = std::max(plan.lds_plane, s->lds_plane);
     plan.lds_sort = std::max(plan.lds_sort, s->lds_sort);
     plan.lds_sort3 = std::max(plan.lds_sort3, s->lds_sort3);
-    plan.pairs_any = plan.pairs_any || (s->coulomb && !s->h.real_cells);
   }
   single.assign(R, Plan{});
   for (int r = 0; r < R; r++) {
@@ -844,7 +841,6 @@ int md_engine::rebuild() {
     single[r].lds_plane = sys[r]->lds_plane;
     single[r].lds_sort = sys[r]->lds_sort;
     single[r].lds_sort3 = sys[r]->lds_sort3;
-    single[r].pairs_any = sys[r]->coulomb && !sys[r]->h.real_cells;
   }
   return MD_OK;
 }

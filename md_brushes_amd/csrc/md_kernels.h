@@ -31,7 +31,6 @@ struct Plan {
   unsigned long long *lq = nullptr;
   int lq_cap = 0;
   int grid_bin = 1, grid_build = 1;
-  bool pairs_any = false;   // some replica uses the all-pairs real space
   double *vpart = nullptr;  // partial sums of rescale_velocity
   bool fp32 = false;
 };

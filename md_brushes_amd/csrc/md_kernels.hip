@@ -1220,54 +1220,77 @@ __global__ __launch_bounds__(kBlock) void k_fftz_conv(const Rep *__restrict__ re
   }
 }
 
-// IFrc (compt_acc.f90:1402-1467): reciprocal force of each charge from the
-// potential mesh, G lanes per charge (lane (b,c) reads mesh row (y_b, z_c)),
-// reduced across the group; plus per-block sums of the force (for
-// ZeroForce, compt_acc.f90:1470-1492) and of q z lb/Beta (the slab dipole
-// M_z, compt_acc.f90:1041-1044), all in double.
+// One charge's reciprocal force from the potential mesh (IFrc,
+// compt_acc.f90:1402-1467): F = -q sum over the p^3 points of the spline
+// weight products (one of them differentiated) times the potential; rows of
+// p points along x, rows summed z outer, y inner.  The order is a template
+// parameter so that the weights stay in registers.
+template <typename R, int O>
+__device__ __forceinline__ void gather_one(const Rep &S, double4 p, R &f1, R &f2, R &f3) {
+  R wx[kMaxOrder], wy[kMaxOrder], wz[kMaxOrder], dx[kMaxOrder], dy[kMaxOrder], dz[kMaxOrder];
+  const int K1 = S.K[0], K2 = S.K[1], K3 = S.K[2];
+  const int bx = wrapk(spline<R, O>(p.x, S.kg[0], O, wx, dx), K1);
+  const int by = wrapk(spline<R, O>(p.y, S.kg[1], O, wy, dy), K2);
+  const int bz = wrapk(spline<R, O>(p.z, S.kg[2], O, wz, dz), K3);
+  const R q = (R)p.w;
+  int xs[O];
+#pragma unroll
+  for (int a = 0; a < O; a++) {
+    int x = bx + a;
+    xs[a] = x >= K1 ? x - K1 : x;
+  }
+  const R *mesh = (const R *)S.meshu;
+  f1 = f2 = f3 = 0;
+#pragma unroll
+  for (int c = 0; c < O; c++) {
+    int z = bz + c;
+    z -= z >= K3 ? K3 : 0;
+#pragma unroll
+    for (int b = 0; b < O; b++) {
+      int y = by + b;
+      y -= y >= K2 ? K2 : 0;
+      const R *row = mesh + ((size_t)z * K2 + y) * K1;
+      R s1 = 0, s2 = 0;
+#pragma unroll
+      for (int a = 0; a < O; a++) {
+        const R v = row[xs[a]];
+        s1 += (dx[a] * q) * v;
+        s2 += (wx[a] * q) * v;
+      }
+      f1 += wz[c] * wy[b] * s1;
+      f2 += wz[c] * dy[b] * s2;
+      f3 += dz[c] * wy[b] * s2;
+    }
+  }
+}
+
+// Reciprocal force of each charge (one lane per charge), plus per-block
+// sums of the force (for ZeroForce, compt_acc.f90:1470-1492) and of
+// q z lb/Beta (the slab dipole M_z, compt_acc.f90:1041-1044), in double.
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_gather(const Rep *__restrict__ reps, WMap wm) {
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
   const int blk = blockIdx.x - wm.blk[r];
-  const int t = blk * kBlock + threadIdx.x;
-  const int G = S.G, o = S.order;
-  const int m = t / G, g = t - m * G;
+  const int m = blk * kBlock + threadIdx.x;
   const bool live = m < S.Nq;
   {  // the tile lists are consumed: clear them for the next evaluation
     const int ntile = S.ntl[0] * S.ntl[1] * S.ntl[2] * kTileSub;
     const int nth = (wm.blk[r + 1] - wm.blk[r]) * kBlock;
-    for (int k = t; k < ntile; k += nth) S.tcnt[k] = 0;
-  }
-  R f1 = 0, f2 = 0, f3 = 0;
-  double4 p = make_double4(0, 0, 0, 0);
-  if (live) p = S.pos[S.charge[m]];
-  if (live && g < o * o) {
-    const int bb = g % o, cc = g / o;
-    R wx[kMaxOrder], wy[kMaxOrder], wz[kMaxOrder], dx[kMaxOrder], dy[kMaxOrder], dz[kMaxOrder];
-    const int bx = spline<R>(p.x, S.kg[0], o, wx, dx);
-    const int by = spline<R>(p.y, S.kg[1], o, wy, dy);
-    const int bz = spline<R>(p.z, S.kg[2], o, wz, dz);
-    const R q = (R)p.w;
-    const int K1 = S.K[0], K2 = S.K[1], K3 = S.K[2];
-    const R *row = (const R *)S.meshu + ((size_t)wrapk(bz + cc, K3) * K2 + wrapk(by + bb, K2)) * K1;
-    R s1 = 0, s2 = 0;
-    for (int a = 0; a < o; a++) {
-      const R v = row[wrapk(bx + a, K1)];
-      s1 += (dx[a] * q) * v;
-      s2 += (wx[a] * q) * v;
-    }
-    f1 = wz[cc] * wy[bb] * s1;
-    f2 = wz[cc] * dy[bb] * s2;
-    f3 = dz[cc] * wy[bb] * s2;
-  }
-  for (int off = G >> 1; off > 0; off >>= 1) {  // groups are aligned, G divides 64
-    f1 += __shfl_xor(f1, off, 64);
-    f2 += __shfl_xor(f2, off, 64);
-    f3 += __shfl_xor(f3, off, 64);
+    for (int k = m; k < ntile; k += nth) S.tcnt[k] = 0;
   }
   double f[4] = {0, 0, 0, 0};
-  if (live && g == 0) {
+  if (live) {
+    const double4 p = S.pos[S.charge[m]];
+    R f1, f2, f3;
+    switch (S.order) {
+      case 3: gather_one<R, 3>(S, p, f1, f2, f3); break;
+      case 4: gather_one<R, 4>(S, p, f1, f2, f3); break;
+      case 5: gather_one<R, 5>(S, p, f1, f2, f3); break;
+      case 6: gather_one<R, 6>(S, p, f1, f2, f3); break;
+      case 7: gather_one<R, 7>(S, p, f1, f2, f3); break;
+      default: gather_one<R, 8>(S, p, f1, f2, f3); break;
+    }
     f[0] = -(double)f1 * S.fnorm[0];
     f[1] = -(double)f2 * S.fnorm[1];
     f[2] = -(double)f3 * S.fnorm[2];
@@ -1723,46 +1746,6 @@ __global__ __launch_bounds__(kBlock) void k_real_cells(const Rep *__restrict__ r
   }
 }
 
-// real_space over all charge pairs (the reference's real_verlet == 0 loop,
-// compt_acc.f90:997-1010, and boxes too small for three cells of edge r_c),
-// one thread per charge.
-template <typename R>
-__global__ __launch_bounds__(kBlock) void k_real(const Rep *__restrict__ reps, WMap wm,
-                                                 double *__restrict__ out) {
-  const int r = find_rep(wm, blockIdx.x);
-  const Rep &S = reps[r];
-  if (S.real_cells) return;
-  const int m = (blockIdx.x - wm.blk[r]) * kBlock + threadIdx.x;
-  if (m >= S.Nq) return;
-  const double *tot = S.qtot;
-  const int i = S.charge[m];
-  const double4 pd = S.pos[i];
-  const R alpha = (R)S.alpha, alpha2 = (R)S.alpha2, c2 = (R)S.c2, rcc2 = (R)S.rcc2;
-  const R hLx = (R)S.hLx, hLy = (R)S.hLy, Lx = (R)S.Lx, Ly = (R)S.Ly;
-  R xi, yi, zi, qi;
-  if constexpr (sizeof(R) == 4) {
-    const float4 p = S.pos32[i];
-    xi = p.x; yi = p.y; zi = p.z; qi = p.w;
-  } else {
-    xi = pd.x; yi = pd.y; zi = pd.z; qi = pd.w;
-  }
-  R fx = 0, fy = 0, fz = 0;
-  for (int k = 0; k < S.Nq; k++) {
-    if (k == m) continue;
-    const int j = S.charge[k];
-    R xj, yj, zj, qj;
-    if constexpr (sizeof(R) == 4) {
-      const float4 p = S.pos32[j];
-      xj = p.x; yj = p.y; zj = p.z; qj = p.w;
-    } else {
-      const double4 p = S.pos[j];
-      xj = p.x; yj = p.y; zj = p.z; qj = p.w;
-    }
-    real_pair<R>(xi, yi, zi, qi, xj, yj, zj, qj, Lx, Ly, hLx, hLy, rcc2, alpha, alpha2, c2, fx, fy, fz);
-  }
-  store_coulomb<R>(S, m, i, pd.w, tot, fx, fy, fz, out);
-}
-
 // ---------------------------------------------------- rescale_velocity
 
 // Whether a replica of a batch rescales after the step just completed:
@@ -1938,8 +1921,6 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) 
   mark();
   if (P.cell3.total)
     hipLaunchKernelGGL(k_real_cells<R>, dim3(P.cell3c.total), dim3(kBlock), 0, st, P.reps, P.cell3c, out);
-  if (P.pairs_any)
-    hipLaunchKernelGGL(k_real<R>, dim3(P.chg.total), dim3(kBlock), 0, st, P.reps, P.chg, out);
   mark();
 }
 

@@ -194,8 +194,26 @@ def main():
     ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--sequential", type=int, default=-1,
                     help="requests in the sequential FP64 comparator (-1: all)")
+    ap.add_argument("--batch-only", type=int, default=0, metavar="CYCLES",
+                    help="no serving loop: add every request at once and time CYCLES cycles of the batch "
+                         "(a fixed workload for kernel traces)")
     args = ap.parse_args()
     reqs = make_stream(args.requests, args.steps, args.rate, args.seed)
+    if args.batch_only:
+        b = Batch(MD_FP32)
+        for r in reqs:
+            b.add(r.params, r.pos, r.vel)
+        b.run(2)
+        b.status(0)
+        t0 = time.perf_counter()
+        b.run(args.batch_only)
+        b.status(0)  # waits for the queued cycles
+        dt = time.perf_counter() - t0
+        steps = len(reqs) * args.batch_only * b.multistep
+        print(json.dumps({"replicas": len(reqs), "cycles": args.batch_only,
+                          "replica_steps_per_s": round(steps / dt, 1)}))
+        b.close()
+        return
     srv = Server()
     done, wall = srv.serve(reqs)
     out = summarize(done, wall)

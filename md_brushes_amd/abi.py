@@ -263,9 +263,11 @@ class Engine:
                "fft_z_conv", "fft_y_inv", "fft_x_inv", "gather", "real", "bin_lj", "sort_lj", "build_lj"]
     COULOMB = KERNELS[2:12]
     LISTS = KERNELS[12:]
+    RUN_START = KERNELS[:1]  # once per run: replayed steps drift in short_range
 
     def time_kernels(self, iters=20):
-        """Mean ms per launch of each kernel (HIP events); advances the state."""
+        """Mean ms per launch of each kernel (HIP events); positions, velocities
+        and forces are restored, lists and the held Coulomb force recomputed."""
         ms = np.zeros(15)
         self._check(self.lib.md_time_kernels(self.h, iters, _dp(ms), 15), "md_time_kernels")
         return dict(zip(self.KERNELS, ms))
@@ -311,7 +313,7 @@ class Batch:
 
     def time_kernels(self, iters=20):
         """Mean ms per launch of each kernel over all replicas (HIP events);
-        advances the state of every replica."""
+        every replica's positions, velocities and forces are restored."""
         ms = np.zeros(15)
         self._check(self.lib.md_time_kernels(self.h, iters, _dp(ms), 15), "md_time_kernels")
         return dict(zip(Engine.KERNELS, ms))

@@ -24,8 +24,6 @@ constexpr int kHistBins = 500;  // SizeHist of the z-density profile (glob_var.f
 
 // Per-replica control block, written by kernels.
 struct Ctrl {
-  unsigned long long dmax1_bits;  // max |dr| of this step, all particles
-  unsigned long long dmax2_bits;  // max |dr| of this step, charged particles
   double dr_max1, dr_max2;        // accumulated (compt_acc.f90:16,41)
   int flag1, flag2;               // rebuild LJ / real list after this step
   int gen1, gen2;                 // rebuild generations (count-buffer parity)
@@ -75,6 +73,16 @@ struct Rep {
   // --- device pointers ---
   double4 *pos;                    // (x,y,z,q) FP64, canonical
   float4 *pos32;                   // FP32 copy for the FP32 force path
+  // the other position buffer: replayed steps read pos and write the next
+  // step's drifted positions here (two descriptor tables swap the pair)
+  double4 *pos_nx;
+  float4 *pos32_nx;
+  int par;                         // which buffer pos is (0 / 1)
+  // max |dr| of a step (all, charged) per group of 64 consecutive particles,
+  // one slot of ngrp groups per position buffer: the drift that produces
+  // buffer q's positions writes slot q (plain stores, every group)
+  double2 *dpart;
+  int ngrp;
   double *vx, *vy, *vz;
   double *ax, *ay, *az;
   void *ccx, *ccy, *ccz;           // held Coulomb force, real type of the path

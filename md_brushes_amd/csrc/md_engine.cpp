@@ -145,6 +145,8 @@ struct Sys {
   bool can_fuse = false;           // both planes of a fused FFT kernel fit in LDS
   size_t plane_bytes = 0;
   double *d_eta = nullptr, *d_out1 = nullptr, *d_out2 = nullptr;
+  double4 *posb[2] = {nullptr, nullptr};  // the two position buffers (h.pos is posb[par])
+  float4 *pos32b[2] = {nullptr, nullptr};
   int *d_hist = nullptr;     // observables: z histogram of the chains
   double *d_hsum = nullptr;  // observables: per-block sums of z
   std::string err;
@@ -175,6 +177,17 @@ struct Sys {
   }
   int setup(const md_params *pin, bool want_fp32);
   int size_lists(const double *pos);
+  // descriptor whose canonical positions are buffer q
+  Rep table(int q) const {
+    Rep t = h;
+    t.pos = posb[q];
+    t.pos32 = pos32b[q];
+    t.pos_nx = posb[q ^ 1];
+    t.pos32_nx = pos32b[q ^ 1];
+    t.par = q;
+    return t;
+  }
+  void set_par(int q) { h = table(q); }
 };
 
 int Sys::setup(const md_params *pin, bool want_fp32) {
@@ -285,10 +298,16 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
 
   // state and topology on the device
   const size_t nn = NN;
-  S.pos = dalloc<double4>(nn);
-  S.pos32 = fp32 ? dalloc<float4>(nn) : nullptr;
+  for (int q = 0; q < 2; q++) {
+    posb[q] = dalloc<double4>(nn);
+    pos32b[q] = fp32 ? dalloc<float4>(nn) : nullptr;
+    if (!posb[q] || (fp32 && !pos32b[q])) return fail(MD_ENOMEM, "state");
+  }
+  set_par(0);
   double *v = dalloc<double>(3 * nn), *a = dalloc<double>(3 * nn);
-  if (!S.pos || !v || !a || (fp32 && !S.pos32)) return fail(MD_ENOMEM, "state");
+  S.ngrp = cdiv(NN, 64);  // the short-range kernel's particle groups
+  S.dpart = dalloc<double2>(2 * (size_t)S.ngrp);
+  if (!v || !a || !S.dpart) return fail(MD_ENOMEM, "state");
   S.vx = v; S.vy = v + nn; S.vz = v + 2 * nn;
   S.ax = a; S.ay = a + nn; S.az = a + 2 * nn;
   const size_t rs = fp32 ? 4 : 8;
@@ -624,9 +643,16 @@ struct md_engine {
   std::vector<Sys *> sys;          // launch order
   std::map<int, Sys *> slots;      // stable handles
   int next_slot = 0;
-  Plan plan;                       // all replicas
-  std::vector<Plan> single;        // one replica each (same order as sys)
-  Rep *d_reps = nullptr;
+  // Replayed steps alternate between two position buffers: a step reads
+  // buffer q and its short-range kernel drifts the next step's positions
+  // into buffer q ^ 1.  Descriptor table q (and plans planp[q], singlep[q])
+  // hold buffer q as canonical; cur is the buffer of the state now.
+  int cur = 0;
+  Plan planp[2];
+  std::vector<Plan> singlep[2];
+  Plan plan;                       // all replicas, buffer cur
+  std::vector<Plan> single;        // one replica each (same order as sys), buffer cur
+  Rep *d_reps = nullptr;           // table 0, then table 1, reps_cap each
   int reps_cap = 0;
   int *d_maps = nullptr;
   int maps_cap = 0;
@@ -636,8 +662,8 @@ struct md_engine {
   size_t vpart_cap = 0;
   int *h_steps = nullptr;          // pinned, one per replica
   int steps_cap = 0;
-  hipGraphExec_t gexec = nullptr;
-  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};  // cycle graphs by the buffer of their first step
+  hipGraph_t graph[2] = {nullptr, nullptr};
   bool dirty = true;
   bool batch = false;              // made by md_batch_create (replicas pre-heat)
   // launch forms set by md_autotune (-1 / 0: the size heuristics); none
@@ -664,10 +690,19 @@ struct md_engine {
   }
   int hip_fail(hipError_t e, const char *what) { return fail(MD_EHIP, hip_msg(e, what)); }
   void drop_graph() {
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    if (graph) (void)hipGraphDestroy(graph);
-    gexec = nullptr;
-    graph = nullptr;
+    for (int q = 0; q < 2; q++) {
+      if (gexec[q]) (void)hipGraphExecDestroy(gexec[q]);
+      if (graph[q]) (void)hipGraphDestroy(graph[q]);
+      gexec[q] = nullptr;
+      graph[q] = nullptr;
+    }
+  }
+  // makes buffer q the canonical one (host descriptors and current plans)
+  void select(int q) {
+    cur = q;
+    for (Sys *x : sys) x->set_par(q);
+    plan = planp[q];
+    single = singlep[q];
   }
   int init(int precision, int ms) {
     fp32 = precision == MD_FP32;
@@ -682,7 +717,7 @@ struct md_engine {
   int check_all();
   int add(const md_params *p, int *slot);
   int launch_compute_force(const Plan &P, Sys *s, int step, int noise, const double *eta, bool in_step);
-  int build_graph();
+  int build_graph(int q);
   int run_cycles(int ncycles);
   int index_of(const Sys *s) const {
     for (size_t k = 0; k < sys.size(); k++)
@@ -700,14 +735,17 @@ int md_engine::rebuild() {
   drop_graph();
   dirty = false;
   if (R == 0) {
-    plan = Plan{};
-    single.clear();
+    planp[0] = planp[1] = Plan{};
+    singlep[0].clear();
+    singlep[1].clear();
+    select(cur);
     return MD_OK;
   }
   if (R > reps_cap) {
     if (d_reps) (void)hipFree(d_reps);
     reps_cap = std::max(2 * R, 8);
-    if (hipMalloc(&d_reps, reps_cap * sizeof(Rep)) != hipSuccess) return fail(MD_ENOMEM, "descriptors");
+    if (hipMalloc(&d_reps, 2 * (size_t)reps_cap * sizeof(Rep)) != hipSuccess)
+      return fail(MD_ENOMEM, "descriptors");
   }
   // real-space sweep form: a workgroup per item while the engine holds few
   // charges (its items alone do not fill the chip), a wavefront per item
@@ -749,9 +787,11 @@ int md_engine::rebuild() {
     x->nb[12] = S.plane_fused ? S.K[2] : 0;
   }
   std::vector<Rep> hr(R);
-  for (int r = 0; r < R; r++) hr[r] = sys[r]->h;
-  e = hipMemcpy(d_reps, hr.data(), R * sizeof(Rep), hipMemcpyHostToDevice);
-  if (e != hipSuccess) return hip_fail(e, "descriptors");
+  for (int q = 0; q < 2; q++) {
+    for (int r = 0; r < R; r++) hr[r] = sys[r]->table(q);
+    e = hipMemcpy(d_reps + (size_t)q * reps_cap, hr.data(), R * sizeof(Rep), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "descriptors");
+  }
   // maps: batch maps kMaps*(R+1), then per replica kMaps*2
   std::vector<int> m;
   int tot[kMaps];
@@ -791,11 +831,12 @@ int md_engine::rebuild() {
     if (hipHostMalloc((void **)&h_steps, steps_cap * sizeof(int)) != hipSuccess)
       return fail(MD_ENOMEM, "pinned");
   }
-  // rebuild queues: 2 header words + rep/start ints per entry
+  // rebuild queues, one set per position buffer: 2 header words + rep/start
+  // ints per entry
   const size_t need_lq = (2 + (size_t)R) + (size_t)R * 3;
-  if (need_lq > lq_words) {
+  if (2 * need_lq > lq_words) {
     if (d_lq) (void)hipFree(d_lq);
-    lq_words = 2 * need_lq;
+    lq_words = 4 * need_lq;
     if (hipMalloc(&d_lq, lq_words * 8) != hipSuccess) return fail(MD_ENOMEM, "rebuild queue");
   }
   e = hipMemset(d_lq, 0, lq_words * 8);
@@ -814,34 +855,43 @@ int md_engine::rebuild() {
     P.vpart = vpart;
     P.fp32 = fp32;
   };
-  fill(plan, d_reps, R, d_maps, tot);
   long nn_all = 0;
   for (Sys *s : sys) nn_all += s->p.NN;
-  plan.lq = d_lq;
-  plan.lq_cap = R;
-  grids(plan, nn_all);
-  plan.lds_x = plan.lds_y = plan.lds_z = plan.lds_sort = plan.lds_sort3 = plan.lds_plane = 0;
-  for (Sys *s : sys) {
-    plan.lds_x = std::max(plan.lds_x, s->lds_x);
-    plan.lds_y = std::max(plan.lds_y, s->lds_y);
-    plan.lds_z = std::max(plan.lds_z, s->lds_z);
-    plan.lds_plane = std::max(plan.lds_plane, s->lds_plane);
-    plan.lds_sort = std::max(plan.lds_sort, s->lds_sort);
-    plan.lds_sort3 = std::max(plan.lds_sort3, s->lds_sort3);
+  for (int q = 0; q < 2; q++) {
+    Plan &B = planp[q];
+    unsigned long long *lq = d_lq + (size_t)q * need_lq, *lq_nx = d_lq + (size_t)(q ^ 1) * need_lq;
+    Rep *tab = d_reps + (size_t)q * reps_cap;
+    fill(B, tab, R, d_maps, tot);
+    B.lq = lq;
+    B.lq_nx = lq_nx;
+    B.lq_cap = R;
+    grids(B, nn_all);
+    B.lds_x = B.lds_y = B.lds_z = B.lds_sort = B.lds_sort3 = B.lds_plane = 0;
+    for (Sys *s : sys) {
+      B.lds_x = std::max(B.lds_x, s->lds_x);
+      B.lds_y = std::max(B.lds_y, s->lds_y);
+      B.lds_z = std::max(B.lds_z, s->lds_z);
+      B.lds_plane = std::max(B.lds_plane, s->lds_plane);
+      B.lds_sort = std::max(B.lds_sort, s->lds_sort);
+      B.lds_sort3 = std::max(B.lds_sort3, s->lds_sort3);
+    }
+    std::vector<Plan> &one = singlep[q];
+    one.assign(R, Plan{});
+    for (int r = 0; r < R; r++) {
+      fill(one[r], tab + r, 1, d_maps + single_off + r * kMaps * 2, sys[r]->nb);
+      one[r].lq = lq + (2 + R) + 3 * (size_t)r;
+      one[r].lq_nx = lq_nx + (2 + R) + 3 * (size_t)r;
+      one[r].lq_cap = 1;
+      grids(one[r], sys[r]->p.NN);
+      one[r].lds_x = sys[r]->lds_x;
+      one[r].lds_y = sys[r]->lds_y;
+      one[r].lds_z = sys[r]->lds_z;
+      one[r].lds_plane = sys[r]->lds_plane;
+      one[r].lds_sort = sys[r]->lds_sort;
+      one[r].lds_sort3 = sys[r]->lds_sort3;
+    }
   }
-  single.assign(R, Plan{});
-  for (int r = 0; r < R; r++) {
-    fill(single[r], d_reps + r, 1, d_maps + single_off + r * kMaps * 2, sys[r]->nb);
-    single[r].lq = d_lq + (2 + R) + 3 * (size_t)r;
-    single[r].lq_cap = 1;
-    grids(single[r], sys[r]->p.NN);
-    single[r].lds_x = sys[r]->lds_x;
-    single[r].lds_y = sys[r]->lds_y;
-    single[r].lds_z = sys[r]->lds_z;
-    single[r].lds_plane = sys[r]->lds_plane;
-    single[r].lds_sort = sys[r]->lds_sort;
-    single[r].lds_sort3 = sys[r]->lds_sort3;
-  }
+  select(cur);
   return MD_OK;
 }
 
@@ -911,6 +961,7 @@ int md_engine::add(const md_params *p, int *slot) {
     return rc;
   }
   s->h.preheat_end = batch ? std::max(0, p->StepNum0) : 0;
+  s->set_par(cur);  // its state is uploaded into the engine's current buffer
   sys.push_back(s);
   *slot = next_slot++;
   slots[*slot] = s;
@@ -946,67 +997,56 @@ int md_engine::launch_compute_force(const Plan &P, Sys *s, int step, int noise, 
 }
 
 // One multistep cycle of steps, captured once and replayed for every
-// replica: kick-drift, short range with the second kick, list maintenance,
-// and the Coulomb evaluation at the cycle's last step.  A replica enters the
-// cycle at a step n with n % multistep == 1, so its Coulomb steps are the
-// multiples of multistep, as in Compute_Force (compt_acc.f90:841).  Step
-// numbers come from each replica's control block (advanced by the list
-// kernel).
-int md_engine::build_graph() {
-  if (gexec) return MD_OK;
-  hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
-  if (e != hipSuccess) return hip_fail(e, "begin capture");
+// replica: Coulomb evaluation at the cycle's last step, short range with
+// the second kick and the next step's kick and drift (SR_DRIFT), list
+// maintenance, gated pre-heat rescale.  A replica enters the cycle at a step
+// n with n % multistep == 1, so its Coulomb steps are the multiples of
+// multistep, as in Compute_Force (compt_acc.f90:841).  Step numbers come
+// from each replica's control block (advanced by the list kernel).  Step k
+// of the cycle reads position buffer q ^ ((k - 1) & 1); a run starts with
+// one kick-drift (run_cycles) that puts its first step's positions in place.
+static void launch_cycle(md_engine *e, int q) {
   const int mode = mdk::SR_HELD | mdk::SR_FIELD | mdk::SR_ANCHOR | mdk::SR_NOISE_PHILOX |
-                   mdk::SR_KICK | mdk::SR_DRMAX;
-  bool any_coulomb = false, any_preheat = false;
-  for (Sys *s : sys) {
-    any_coulomb |= s->coulomb;
-    any_preheat |= s->h.preheat_end > 0;
-  }
-  for (int s = 1; s <= multistep; s++) {
-    mdk::launch_kick_drift(plan, st);
-    if (any_coulomb && s == multistep) {
-      if (fp32) mdk::launch_coulomb<float>(plan, st, nullptr);
-      else mdk::launch_coulomb<double>(plan, st, nullptr);
-    }
-    if (fp32) mdk::launch_short<float>(plan, st, mode, -1, nullptr, nullptr);
-    else mdk::launch_short<double>(plan, st, mode, -1, nullptr, nullptr);
-    mdk::launch_lists(plan, st, 1);
-    // pre-heat rescale between steps (its step is a multiple of 20, so of
-    // multistep when that divides 20; otherwise checked after every step)
-    if (any_preheat && (20 % multistep != 0 || s == multistep)) mdk::launch_rescale(plan, st, 1);
-  }
-  e = hipStreamEndCapture(st, &graph);
-  if (e != hipSuccess) return hip_fail(e, "end capture");
-  e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
-  if (e != hipSuccess) return hip_fail(e, "instantiate");
-  return MD_OK;
-}
-
-// Same launch sequence as the captured cycle, issued on the stream
-// (MD_NO_GRAPH=1: for tracing tools that do not follow graph launches).
-static void launch_cycle(md_engine *e) {
-  const int mode = mdk::SR_HELD | mdk::SR_FIELD | mdk::SR_ANCHOR | mdk::SR_NOISE_PHILOX |
-                   mdk::SR_KICK | mdk::SR_DRMAX;
+                   mdk::SR_KICK | mdk::SR_DRMAX | mdk::SR_DRIFT;
   bool any_coulomb = false, any_preheat = false;
   for (Sys *s : e->sys) {
     any_coulomb |= s->coulomb;
     any_preheat |= s->h.preheat_end > 0;
   }
-  for (int s = 1; s <= e->multistep; s++) {
-    mdk::launch_kick_drift(e->plan, e->st);
-    if (any_coulomb && s == e->multistep) {
-      if (e->fp32) mdk::launch_coulomb<float>(e->plan, e->st, nullptr);
-      else mdk::launch_coulomb<double>(e->plan, e->st, nullptr);
+  for (int k = 1; k <= e->multistep; k++) {
+    const Plan &P = e->planp[q ^ ((k - 1) & 1)];
+    if (any_coulomb && k == e->multistep) {
+      if (e->fp32) mdk::launch_coulomb<float>(P, e->st, nullptr);
+      else mdk::launch_coulomb<double>(P, e->st, nullptr);
     }
-    if (e->fp32) mdk::launch_short<float>(e->plan, e->st, mode, -1, nullptr, nullptr);
-    else mdk::launch_short<double>(e->plan, e->st, mode, -1, nullptr, nullptr);
-    mdk::launch_lists(e->plan, e->st, 1);
-    if (any_preheat && (20 % e->multistep != 0 || s == e->multistep))
-      mdk::launch_rescale(e->plan, e->st, 1);
+    if (e->fp32) mdk::launch_short<float>(P, e->st, mode, -1, nullptr, nullptr);
+    else mdk::launch_short<double>(P, e->st, mode, -1, nullptr, nullptr);
+    mdk::launch_lists(P, e->st, 1);
+    // pre-heat rescale between steps (its step is a multiple of 20, so of
+    // multistep when that divides 20; otherwise checked after every step);
+    // it redoes the next step's drift with the rescaled velocities
+    if (any_preheat && (20 % e->multistep != 0 || k == e->multistep))
+      mdk::launch_rescale(P, e->st, 1, 1);
   }
 }
 
+int md_engine::build_graph(int q) {
+  if (gexec[q]) return MD_OK;
+  hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return hip_fail(e, "begin capture");
+  launch_cycle(this, q);
+  e = hipStreamEndCapture(st, &graph[q]);
+  if (e != hipSuccess) return hip_fail(e, "end capture");
+  e = hipGraphInstantiate(&gexec[q], graph[q], nullptr, nullptr, 0);
+  if (e != hipSuccess) return hip_fail(e, "instantiate");
+  return MD_OK;
+}
+
+// ncycles cycles from the current state: the first step's kick-drift (its
+// velocities kept: the step's short-range kernel redoes the kick), then the
+// cycles, graph-replayed (MD_NO_GRAPH=1 issues the same launches on the
+// stream, for tracing tools that do not follow graph launches).  The last
+// step's positions become the current buffer.
 int md_engine::run_cycles(int ncycles) {
   if (ncycles <= 0 || sys.empty()) return MD_OK;
   if (dirty) {
@@ -1014,18 +1054,28 @@ int md_engine::run_cycles(int ncycles) {
     if (rc) return rc;
   }
   const char *ng = std::getenv("MD_NO_GRAPH");
-  if (ng && ng[0] == '1') {
-    for (int c = 0; c < ncycles; c++) launch_cycle(this);
-    hipError_t e = hipGetLastError();
-    return e == hipSuccess ? MD_OK : hip_fail(e, "cycle launch");
+  const bool eager = ng && ng[0] == '1';
+  const int q0 = cur;
+  if (!eager) {
+    for (int q = 0; q < 2; q++) {
+      if (q != q0 && multistep % 2 == 0) continue;  // even cycles all start on q0
+      int rc = build_graph(q);
+      if (rc) return rc;
+    }
   }
-  int rc = build_graph();
-  if (rc) return rc;
+  mdk::launch_kick_drift(planp[q0], st, 0);
   for (int c = 0; c < ncycles; c++) {
-    hipError_t e = hipGraphLaunch(gexec, st);
+    const int q = q0 ^ ((c * multistep) & 1);
+    if (eager) {
+      launch_cycle(this, q);
+      continue;
+    }
+    hipError_t e = hipGraphLaunch(gexec[q], st);
     if (e != hipSuccess) return hip_fail(e, "graph launch");
   }
-  return MD_OK;
+  select(q0 ^ (((long)ncycles * multistep - 1) & 1));
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? MD_OK : hip_fail(e, "cycle launch");
 }
 
 // ------------------------------------------------------------------ C-ABI
@@ -1308,10 +1358,13 @@ extern "C" int md_force_parts(md_engine *e, double *f_short, double *f_coul) {
 }
 
 // Mean duration of each kernel of a step, from HIP events on the engine's
-// stream, over all replicas.  ms[k]: 0 kick_drift, 1 short_range, 2 bin_q,
-// 3 sort_q, 4 spread, 5 fft_x_fwd, 6 fft_y_fwd, 7 fft_z_conv, 8 fft_y_inv,
-// 9 fft_x_inv, 10 gather, 11 real, 12 bin_lj, 13 sort_lj, 14 build_lj.
-// Leaves the state advanced by the timed kick-drifts.
+// stream, over all replicas.  ms[k]: 0 kick_drift (the stepwise form; in
+// replayed steps the short-range kernel drifts), 1 short_range (the replayed
+// form: second kick and the next step's kick and drift), 2 bin_q, 3 sort_q,
+// 4 spread, 5 fft_x_fwd, 6 fft_y_fwd, 7 fft_z_conv, 8 fft_y_inv, 9 fft_x_inv,
+// 10 gather, 11 real, 12 bin_lj, 13 sort_lj, 14 build_lj.  Positions,
+// velocities and forces are restored; the lists are rebuilt and the held
+// Coulomb force recomputed at the current positions.
 extern "C" int md_time_kernels(md_engine *e, int32_t iters, double *ms, int32_t n) {
   ENGINE_CHECK(e && ms && iters > 0 && n >= 15);
   FIRST(e);
@@ -1330,12 +1383,43 @@ extern "C" int md_time_kernels(md_engine *e, int32_t iters, double *ms, int32_t 
     (void)hipEventElapsedTime(&t, ev[a], ev[b]);
     return (double)t;
   };
-  const int mode = mdk::SR_HELD | mdk::SR_FIELD | mdk::SR_ANCHOR | mdk::SR_NOISE_PHILOX;
+  // positions, velocities and forces, which the timed kick-drift and
+  // short-range launches move: restored after every iteration, before the
+  // Coulomb and list kernels run at the restored positions
+  std::vector<char *> snap(e->sys.size(), nullptr);
+  for (size_t k = 0; k < snap.size(); k++) {
+    const size_t NN = e->sys[k]->p.NN;
+    if (hipMalloc((void **)&snap[k], NN * (32 + 16 + 24 + 24)) != hipSuccess) snap[k] = nullptr;
+    if (!snap[k]) {
+      for (char *x : snap)
+        if (x) (void)hipFree(x);
+      for (auto &x : ev) (void)hipEventDestroy(x);
+      return e->fail(MD_ENOMEM, "time_kernels snapshot");
+    }
+  }
+  auto keep = [&](bool restore) {
+    for (size_t k = 0; k < snap.size(); k++) {
+      Sys *x = e->sys[k];
+      const size_t NN = x->p.NN;
+      void *dev[4] = {x->h.pos, x->h.pos32, x->h.vx, x->h.ax};
+      const size_t nb[4] = {32 * NN, 16 * NN, 24 * NN, 24 * NN};
+      char *h = snap[k];
+      for (int j = 0; j < 4; j++) {
+        if (dev[j]) {
+          if (restore) (void)hipMemcpyAsync(dev[j], h, nb[j], hipMemcpyDeviceToDevice, e->st);
+          else (void)hipMemcpyAsync(h, dev[j], nb[j], hipMemcpyDeviceToDevice, e->st);
+        }
+        h += nb[j];
+      }
+    }
+  };
+  const int mode = mdk::SR_HELD | mdk::SR_FIELD | mdk::SR_ANCHOR | mdk::SR_NOISE_PHILOX |
+                   mdk::SR_KICK | mdk::SR_DRIFT;
   for (int it = 0; it < iters; it++) {
-    // the short-range kernel leaves positions alone: timed over back-to-back
-    // launches, so that the per-launch mean excludes the event overhead; the
-    // kick-drift (which moves the particles) as the difference between
-    // [kick, shorts] and [shorts]
+    // the short-range kernel writes the other position buffer: timed over
+    // back-to-back launches, so that the per-launch mean excludes the event
+    // overhead; the kick-drift as the difference between [kick, shorts] and
+    // [shorts]
     constexpr int kRepeat = 8;
     auto shorts = [&]() {
       for (int k = 0; k < kRepeat; k++) {
@@ -1343,16 +1427,18 @@ extern "C" int md_time_kernels(md_engine *e, int32_t iters, double *ms, int32_t 
         else mdk::launch_short<double>(B, e->st, mode, 0, nullptr, nullptr);
       }
     };
+    keep(false);
     (void)hipEventRecord(ev[0], e->st);
     mdk::launch_kick_drift(B, e->st);
     shorts();
     (void)hipEventRecord(ev[1], e->st);
     shorts();
     (void)hipEventRecord(ev[2], e->st);
+    keep(true);
     const double both = span(0, 1), sh = span(1, 2);
     acc[0] += std::max(0.0, both - sh);
     acc[1] += sh / kRepeat;
-    if (any_coulomb) {
+  if (any_coulomb) {
       if (e->fp32) mdk::launch_coulomb<float>(B, e->st, nullptr, ev);
       else mdk::launch_coulomb<double>(B, e->st, nullptr, ev);
       for (int k = 0; k < 10; k++) acc[2 + k] += span(k, k + 1);
@@ -1363,6 +1449,7 @@ extern "C" int md_time_kernels(md_engine *e, int32_t iters, double *ms, int32_t 
   }
   mdk::launch_set_flags(B, e->st, 1);
   rc = e->check_all();
+  for (char *x : snap) (void)hipFree(x);
   for (auto &x : ev) (void)hipEventDestroy(x);
   for (int k = 0; k < 15; k++) ms[k] = acc[k] / iters;
   return rc;

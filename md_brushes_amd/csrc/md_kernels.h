@@ -12,7 +12,10 @@ enum {
   SR_FIELD = 8,          // add q * E_z
   SR_ANCHOR = 16,        // zero the force on anchors
   SR_KICK = 32,          // second half kick of new_position
-  SR_DRMAX = 64          // dr_max bookkeeping and rebuild flags
+  SR_DRMAX = 64,         // dr_max bookkeeping and rebuild flags
+  SR_DRIFT = 128         // with SR_KICK, for replayed steps: v holds v(n-1) (the
+                         // first half kick is redone), and the next step's
+                         // kick and drift go to the other position buffer
 };
 
 // Everything a launch needs: device descriptors and the block maps.
@@ -29,6 +32,7 @@ struct Plan {
   // LJ rebuild queue (replicas flagged this step) and the fixed grids of the
   // list kernels, which stride over the queued replicas' particles
   unsigned long long *lq = nullptr;
+  unsigned long long *lq_nx = nullptr;  // the queue of the other position buffer's steps
   int lq_cap = 0;
   int grid_bin = 1, grid_build = 1;
   double *vpart = nullptr;  // partial sums of rescale_velocity
@@ -40,15 +44,18 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev =
 template <typename R>
 void launch_short(const Plan &P, hipStream_t st, int mode, int step, const double *eta,
                   double *out);
-void launch_kick_drift(const Plan &P, hipStream_t st);
+// store_v = 0: the first step of a run of replayed steps (SR_DRIFT)
+void launch_kick_drift(const Plan &P, hipStream_t st, int store_v = 1);
 // clear_queue: empty the rebuild queue afterwards (list updates outside the
 // replayed step cycle, where the next kick does it)
 void launch_lists(const Plan &P, hipStream_t st, int inc_step, hipEvent_t *ev = nullptr,
                   bool clear_queue = false);
 void launch_real_list(const Plan &P, hipStream_t st);
 void launch_set_flags(const Plan &P, hipStream_t st, int clear);
-// gated: only batch replicas whose pre-heat rule is due (main.f90:67-75)
-void launch_rescale(const Plan &P, hipStream_t st, int gated = 0);
+// gated: only batch replicas whose pre-heat rule is due (main.f90:67-75);
+// redrift: between replayed steps, whose next drift already ran with the
+// velocities before the rescale: redo it with the rescaled ones
+void launch_rescale(const Plan &P, hipStream_t st, int gated = 0, int redrift = 0);
 void launch_observe(const Plan &P, hipStream_t st, int *hist, double *hsum);
 
 }  // namespace mdk

@@ -108,166 +108,83 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // ------------------------------------------------------------- integrator
 
-// First half of new_position (init_update.f90:581-601) and
-// period_condition_pos (init_update.f90:631-642): half kick, drift, wrap,
-// maximum displacement of this step (all, charged).
-__global__ __launch_bounds__(kBlock) void k_kick_drift(const Rep *__restrict__ reps, WMap wm,
-                                                       unsigned long long *lq) {
+// First half kick of new_position (init_update.f90:581-589), mass 1.
+__device__ __forceinline__ double half_kick(const Rep &S, double a, double v) {
 #pragma clang fp contract(off)
-  if (blockIdx.x == 0 && threadIdx.x == 0) *lq = 0ull;  // this step's LJ rebuild queue
+  return a * S.dt / 2 + S.c1m * v;
+}
+
+// Drift and period_condition_pos (init_update.f90:594-601, 631-642): x += v dt,
+// x and y wrapped to (-L/2, L/2]; returns |v dt|.
+__device__ __forceinline__ double drift(const Rep &S, double4 &p, double vx, double vy, double vz) {
+#pragma clang fp contract(off)
+  const double dt = S.dt;
+  const double dx = vx * dt, dy = vy * dt, dz = vz * dt;
+  p.x = p.x + dx;
+  p.y = p.y + dy;
+  p.z = p.z + dz;
+  const double dr = sqrt(dx * dx + dy * dy + dz * dz);
+  if (p.x > S.hLx) p.x = p.x - S.Lx;
+  else if (p.x <= -S.hLx) p.x = p.x + S.Lx;
+  if (p.y > S.hLy) p.y = p.y - S.Ly;
+  else if (p.y <= -S.hLy) p.y = p.y + S.Ly;
+  return dr;
+}
+
+// Maximum displacement of one wavefront's 64 consecutive particles (group g)
+// into slot q of the group maxima; every lane calls it.
+__device__ __forceinline__ void dmax_group(const Rep &S, int q, int g, double d1, double d2) {
+  d1 = wave_max(d1);
+  d2 = wave_max(d2);
+  if ((threadIdx.x & 63) == 0 && g < S.ngrp) S.dpart[(size_t)q * S.ngrp + g] = make_double2(d1, d2);
+}
+
+// First half of new_position (init_update.f90:581-601) with
+// period_condition_pos: half kick, drift, wrap, maximum displacement of this
+// step per particle group.  store_v = 0 keeps v: the step's short-range
+// kernel then recomputes the kicked velocity (the form that starts a run of
+// replayed steps, which drift inside the short-range kernel).  Also empties
+// this step's LJ rebuild queue.
+__global__ __launch_bounds__(kBlock) void k_kick_drift(const Rep *__restrict__ reps, WMap wm,
+                                                       unsigned long long *lq, int store_v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *lq = 0ull;
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
-  const int i = (blockIdx.x - wm.blk[r]) * kBlock + threadIdx.x;
+  const int blk = blockIdx.x - wm.blk[r];
+  const int i = blk * kBlock + threadIdx.x;
   double d1 = 0, d2 = 0;
   if (i < S.NN) {
-    const double dt = S.dt;
-    double vx = S.ax[i] * dt / 2 + S.c1m * S.vx[i];
-    double vy = S.ay[i] * dt / 2 + S.c1m * S.vy[i];
-    double vz = S.az[i] * dt / 2 + S.c1m * S.vz[i];
+    const double vx = half_kick(S, S.ax[i], S.vx[i]);
+    const double vy = half_kick(S, S.ay[i], S.vy[i]);
+    const double vz = half_kick(S, S.az[i], S.vz[i]);
     double4 p = S.pos[i];
-    double dx = vx * dt, dy = vy * dt, dz = vz * dt;
-    p.x = p.x + dx;
-    p.y = p.y + dy;
-    p.z = p.z + dz;
-    double dr = sqrt(dx * dx + dy * dy + dz * dz);
-    if (p.x > S.hLx) p.x = p.x - S.Lx;
-    else if (p.x <= -S.hLx) p.x = p.x + S.Lx;
-    if (p.y > S.hLy) p.y = p.y - S.Ly;
-    else if (p.y <= -S.hLy) p.y = p.y + S.Ly;
-    S.vx[i] = vx;
-    S.vy[i] = vy;
-    S.vz[i] = vz;
+    const double dr = drift(S, p, vx, vy, vz);
+    if (store_v) {
+      S.vx[i] = vx;
+      S.vy[i] = vy;
+      S.vz[i] = vz;
+    }
     S.pos[i] = p;
     if (S.pos32) S.pos32[i] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
     d1 = dr;
     if (p.w != 0) d2 = dr;
   }
-  __shared__ double red[2][kBlock / 64];
-  d1 = wave_max(d1);
-  d2 = wave_max(d2);
-  if ((threadIdx.x & 63) == 0) {
-    red[0][threadIdx.x >> 6] = d1;
-    red[1][threadIdx.x >> 6] = d2;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 1; w < kBlock / 64; w++) {
-      d1 = fmax(d1, red[0][w]);
-      d2 = fmax(d2, red[1][w]);
-    }
-    // non-negative doubles order like their bit patterns
-    atomicMax(&S.ctrl->dmax1_bits, (unsigned long long)__double_as_longlong(d1));
-    atomicMax(&S.ctrl->dmax2_bits, (unsigned long long)__double_as_longlong(d2));
-  }
+  dmax_group(S, S.par, blk * (kBlock / 64) + (threadIdx.x >> 6), d1, d2);
 }
 
 // --------------------------------------------------------- short range
 
-// Compute_Force without the Coulomb evaluation (compt_acc.f90:847-861):
-// held Coulomb force + lj_force (compt_acc.f90:895-977) + fene_force
-// (compt_acc.f90:865-892) + noise + field, anchors zeroed; optionally the
-// second half kick of new_position (init_update.f90:606) and the dr_max
-// bookkeeping of new_position / update_verlet_list
-// (init_update.f90:600-601, compt_acc.f90:717-724).  A workgroup takes
-// kSRBlock / kSRLanes particles: first kSRLanes lanes per particle share its
-// LJ list (lane g takes slots g, g + kSRLanes, ..., summed in a fixed tree),
-// then one lane per particle adds the rest and stores.
+// Per-particle part of the short-range kernel: wall, FENE, held Coulomb
+// force, noise, field, anchors, the second half kick (init_update.f90:606)
+// and, for replayed steps (SR_DRIFT), the next step's half kick and drift
+// from the new velocity into the other position buffer, whose displacement
+// it returns in d1 (all) and d2 (charged).
 template <typename R>
-__global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict__ reps, WMap wm,
-                                                          int mode, int step_arg,
-                                                          const double *__restrict__ eta,
-                                                          double *__restrict__ out,
-                                                          unsigned long long *lq, int lq_cap) {
-  constexpr int G = kSRLanes, NP = kSRBlock / kSRLanes;
-  const int r = find_rep(wm, blockIdx.x);
-  const Rep &S = reps[r];
-  const int blk = blockIdx.x - wm.blk[r];
-  const int step = step_arg >= 0 ? step_arg : S.ctrl->step;
-  if ((mode & SR_DRMAX) && blk == 0 && threadIdx.x == 0) {
-    Ctrl *c = S.ctrl;
-    double d1 = __longlong_as_double((long long)c->dmax1_bits);
-    double d2 = __longlong_as_double((long long)c->dmax2_bits);
-    c->dmax1_bits = 0;
-    c->dmax2_bits = 0;
-    c->dr_max1 = c->dr_max1 + d1;
-    c->dr_max2 = c->dr_max2 + d2;
-    int f1 = c->dr_max1 > S.skin1;
-    int f2 = (c->dr_max2 > S.skin2) && S.real_verlet != 0;
-    if (f1) {
-      c->dr_max1 = 0;
-      c->gen1++;
-      lq_push(lq, lq_cap, r, S.NN);
-    }
-    if (f2) { c->dr_max2 = 0; c->gen2++; }
-    c->flag1 = f1;
-    c->flag2 = f2;
-  }
-  __shared__ R lj[3][NP];
-  {  // WCA-LJ over the full list: class by |q_i|+|q_j| (compt_acc.f90:922-952)
-    const int g = threadIdx.x & (G - 1), pl = threadIdx.x / G;
-    const int i = blk * NP + pl;
-    const bool live = i < S.NN;
-    const int n = live ? min(S.nbn1[i], S.nbcap1) : 0;
-    const R hLx = (R)S.hLx, hLy = (R)S.hLy, Lx = (R)S.Lx, Ly = (R)S.Ly;
-    const R rcl2 = (R)S.rcl2, rcl3 = (R)S.rcl3, rcl4 = (R)S.rcl4;
-    const R sig2 = (R)S.sigma2, sig3 = (R)S.sigma3;
-    R xi = 0, yi = 0, zi = 0, qa = 0;
-    if (live) {
-      if constexpr (sizeof(R) == 4) {
-        float4 p = S.pos32[i];
-        xi = p.x; yi = p.y; zi = p.z; qa = fabsf(p.w);
-      } else {
-        double4 p = S.pos[i];
-        xi = p.x; yi = p.y; zi = p.z; qa = fabs(p.w);
-      }
-    }
-    R fx = 0, fy = 0, fz = 0;
-    const int *nb = S.nb1 + i;
-    for (int k = g; k < n; k += G) {
-      const int j = nb[(size_t)k * S.NN];
-      R xj, yj, zj, qj;
-      if constexpr (sizeof(R) == 4) {
-        float4 p = S.pos32[j];
-        xj = p.x; yj = p.y; zj = p.z; qj = p.w;
-      } else {
-        double4 p = S.pos[j];
-        xj = p.x; yj = p.y; zj = p.z; qj = p.w;
-      }
-      R dx = xi - xj, dy = yi - yj, dz = zi - zj;
-      if (dx > hLx) dx -= Lx; else if (dx <= -hLx) dx += Lx;
-      if (dy > hLy) dy -= Ly; else if (dy <= -hLy) dy += Ly;
-      const R rsq = dx * dx + dy * dy + dz * dz;
-      const int m = (int)(qa + (sizeof(R) == 4 ? fabsf((float)qj) : fabs((double)qj)));
-      R cut, s2;
-      if (m == 6) { cut = rcl3; s2 = sig2; }
-      else if (m < 3) { cut = rcl2; s2 = (R)1; }
-      else { cut = rcl4; s2 = sig3; }
-      if (rsq < cut) {
-        const R ir2 = s2 / rsq;
-        const R ir6 = ir2 * ir2 * ir2;
-        const R gg = ir6 * (ir6 - (R)0.5) / rsq;
-        fx += gg * dx;
-        fy += gg * dy;
-        fz += gg * dz;
-      }
-    }
-    for (int o = 1; o < G; o <<= 1) {  // fixed tree over the lanes of the particle
-      fx += __shfl_xor(fx, o, G);
-      fy += __shfl_xor(fy, o, G);
-      fz += __shfl_xor(fz, o, G);
-    }
-    if (g == 0) {
-      lj[0][pl] = fx * (R)48;
-      lj[1][pl] = fy * (R)48;
-      lj[2][pl] = fz * (R)48;
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x >= NP) return;
-  const int i = blk * NP + threadIdx.x;
-  if (i >= S.NN) return;
+__device__ __forceinline__ void sr_particle(const Rep &S, int i, int mode, int step,
+                                            const double *__restrict__ eta,
+                                            double *__restrict__ out, R fx, R fy, R fz,
+                                            double &d1, double &d2) {
   const double4 pi = S.pos[i];
-  R fx = lj[0][threadIdx.x], fy = lj[1][threadIdx.x], fz = lj[2][threadIdx.x];
   double wz = 0;
   {  // wall force (compt_acc.f90:956-975)
     const double z = pi.z, Lz = S.Lz;
@@ -354,16 +271,159 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
     out[2 * (size_t)S.NN + i] = az;
     return;
   }
+  double vx, vy, vz;
+  if (mode & SR_KICK) {
+    vx = S.vx[i];
+    vy = S.vy[i];
+    vz = S.vz[i];
+    if (mode & SR_DRIFT) {  // v holds v(n-1): redo this step's first half kick
+      vx = half_kick(S, S.ax[i], vx);
+      vy = half_kick(S, S.ay[i], vy);
+      vz = half_kick(S, S.az[i], vz);
+    }
+  }
   S.ax[i] = ax;
   S.ay[i] = ay;
   S.az[i] = az;
   if (mode & SR_KICK) {
 #pragma clang fp contract(off)
     const double dt = S.dt;
-    S.vx[i] = (S.vx[i] + ax * dt / 2.) / S.c1p;
-    S.vy[i] = (S.vy[i] + ay * dt / 2.) / S.c1p;
-    S.vz[i] = (S.vz[i] + az * dt / 2.) / S.c1p;
+    vx = (vx + ax * dt / 2.) / S.c1p;
+    vy = (vy + ay * dt / 2.) / S.c1p;
+    vz = (vz + az * dt / 2.) / S.c1p;
+    S.vx[i] = vx;
+    S.vy[i] = vy;
+    S.vz[i] = vz;
+    if (mode & SR_DRIFT) {  // the next step's first half kick and drift, into the other buffer
+      double4 p = pi;
+      const double dr = drift(S, p, half_kick(S, ax, vx), half_kick(S, ay, vy), half_kick(S, az, vz));
+      S.pos_nx[i] = p;
+      if (S.pos32_nx) S.pos32_nx[i] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
+      d1 = dr;
+      if (p.w != 0) d2 = dr;
+    }
   }
+}
+
+
+// Compute_Force without the Coulomb evaluation (compt_acc.f90:847-861):
+// held Coulomb force + lj_force (compt_acc.f90:895-977) + fene_force
+// (compt_acc.f90:865-892) + noise + field, anchors zeroed; optionally the
+// second half kick of new_position (init_update.f90:606) and the dr_max
+// bookkeeping of new_position / update_verlet_list
+// (init_update.f90:600-601, compt_acc.f90:717-724).  A workgroup takes
+// kSRBlock / kSRLanes particles: first kSRLanes lanes per particle share its
+// LJ list (lane g takes slots g, g + kSRLanes, ..., summed in a fixed tree),
+// then one lane per particle adds the rest and stores.
+template <typename R>
+__global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict__ reps, WMap wm,
+                                                          int mode, int step_arg,
+                                                          const double *__restrict__ eta,
+                                                          double *__restrict__ out,
+                                                          unsigned long long *lq, int lq_cap,
+                                                          unsigned long long *lq_nx) {
+  constexpr int G = kSRLanes, NP = kSRBlock / kSRLanes;
+  const int r = find_rep(wm, blockIdx.x);
+  const Rep &S = reps[r];
+  const int blk = blockIdx.x - wm.blk[r];
+  const int step = step_arg >= 0 ? step_arg : S.ctrl->step;
+  // the next step's rebuild queue (its list kernels of two steps back are done)
+  if ((mode & SR_DRIFT) && blockIdx.x == 0 && threadIdx.x == 0) *lq_nx = 0ull;
+  if ((mode & SR_DRMAX) && blk == 0 && threadIdx.x < 64) {  // one wavefront
+    double d1 = 0, d2 = 0;
+    const double2 *dp = S.dpart + (size_t)S.par * S.ngrp;
+    for (int g = threadIdx.x; g < S.ngrp; g += 64) {
+      const double2 v = dp[g];
+      d1 = fmax(d1, v.x);
+      d2 = fmax(d2, v.y);
+    }
+    d1 = wave_max(d1);
+    d2 = wave_max(d2);
+    if (threadIdx.x == 0) {
+      Ctrl *c = S.ctrl;
+      c->dr_max1 = c->dr_max1 + d1;
+      c->dr_max2 = c->dr_max2 + d2;
+      int f1 = c->dr_max1 > S.skin1;
+      int f2 = (c->dr_max2 > S.skin2) && S.real_verlet != 0;
+      if (f1) {
+        c->dr_max1 = 0;
+        c->gen1++;
+        lq_push(lq, lq_cap, r, S.NN);
+      }
+      if (f2) { c->dr_max2 = 0; c->gen2++; }
+      c->flag1 = f1;
+      c->flag2 = f2;
+    }
+  }
+  __shared__ R lj[3][NP];
+  {  // WCA-LJ over the full list: class by |q_i|+|q_j| (compt_acc.f90:922-952)
+    const int g = threadIdx.x & (G - 1), pl = threadIdx.x / G;
+    const int i = blk * NP + pl;
+    const bool live = i < S.NN;
+    const int n = live ? min(S.nbn1[i], S.nbcap1) : 0;
+    const R hLx = (R)S.hLx, hLy = (R)S.hLy, Lx = (R)S.Lx, Ly = (R)S.Ly;
+    const R rcl2 = (R)S.rcl2, rcl3 = (R)S.rcl3, rcl4 = (R)S.rcl4;
+    const R sig2 = (R)S.sigma2, sig3 = (R)S.sigma3;
+    R xi = 0, yi = 0, zi = 0, qa = 0;
+    if (live) {
+      if constexpr (sizeof(R) == 4) {
+        float4 p = S.pos32[i];
+        xi = p.x; yi = p.y; zi = p.z; qa = fabsf(p.w);
+      } else {
+        double4 p = S.pos[i];
+        xi = p.x; yi = p.y; zi = p.z; qa = fabs(p.w);
+      }
+    }
+    R fx = 0, fy = 0, fz = 0;
+    const int *nb = S.nb1 + i;
+    for (int k = g; k < n; k += G) {
+      const int j = nb[(size_t)k * S.NN];
+      R xj, yj, zj, qj;
+      if constexpr (sizeof(R) == 4) {
+        float4 p = S.pos32[j];
+        xj = p.x; yj = p.y; zj = p.z; qj = p.w;
+      } else {
+        double4 p = S.pos[j];
+        xj = p.x; yj = p.y; zj = p.z; qj = p.w;
+      }
+      R dx = xi - xj, dy = yi - yj, dz = zi - zj;
+      if (dx > hLx) dx -= Lx; else if (dx <= -hLx) dx += Lx;
+      if (dy > hLy) dy -= Ly; else if (dy <= -hLy) dy += Ly;
+      const R rsq = dx * dx + dy * dy + dz * dz;
+      const int m = (int)(qa + (sizeof(R) == 4 ? fabsf((float)qj) : fabs((double)qj)));
+      R cut, s2;
+      if (m == 6) { cut = rcl3; s2 = sig2; }
+      else if (m < 3) { cut = rcl2; s2 = (R)1; }
+      else { cut = rcl4; s2 = sig3; }
+      if (rsq < cut) {
+        const R ir2 = s2 / rsq;
+        const R ir6 = ir2 * ir2 * ir2;
+        const R gg = ir6 * (ir6 - (R)0.5) / rsq;
+        fx += gg * dx;
+        fy += gg * dy;
+        fz += gg * dz;
+      }
+    }
+    for (int o = 1; o < G; o <<= 1) {  // fixed tree over the lanes of the particle
+      fx += __shfl_xor(fx, o, G);
+      fy += __shfl_xor(fy, o, G);
+      fz += __shfl_xor(fz, o, G);
+    }
+    if (g == 0) {
+      lj[0][pl] = fx * (R)48;
+      lj[1][pl] = fy * (R)48;
+      lj[2][pl] = fz * (R)48;
+    }
+  }
+  __syncthreads();
+  static_assert(NP == 64, "the per-particle part runs on one wavefront");
+  if (threadIdx.x >= NP) return;
+  const int i = blk * NP + threadIdx.x;
+  double d1 = 0, d2 = 0;
+  if (i < S.NN)
+    sr_particle<R>(S, i, mode, step, eta, out, lj[0][threadIdx.x], lj[1][threadIdx.x],
+                   lj[2][threadIdx.x], d1, d2);
+  if (mode & SR_DRIFT) dmax_group(S, S.par ^ 1, blk, d1, d2);
 }
 
 // ------------------------------------------------------- Verlet lists
@@ -1811,7 +1871,7 @@ __global__ __launch_bounds__(kBlock) void k_vsum(const Rep *__restrict__ reps, W
 }
 
 __global__ __launch_bounds__(kBlock) void k_vscale(const Rep *__restrict__ reps, WMap wm,
-                                                   const double *part, int gated) {
+                                                   const double *part, int gated, int redrift) {
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
   if (gated && !preheat_due(S)) return;
@@ -1825,17 +1885,32 @@ __global__ __launch_bounds__(kBlock) void k_vscale(const Rep *__restrict__ reps,
     sh[threadIdx.x] = s;
   }
   __syncthreads();
-  if (i >= S.NN) return;
-  if (S.flags[i] & 1) {
-    S.vx[i] = S.vy[i] = S.vz[i] = 0;
-    return;
+  double d1 = 0, d2 = 0;
+  if (i < S.NN) {
+    double vx = 0, vy = 0, vz = 0;  // anchors stay at rest
+    if (!(S.flags[i] & 1)) {
+      const double n = S.NN - S.N_anchor;
+      const double v2 = sh[3] / (S.NN - S.N_anchor - 1);
+      const double s = sqrt(3.0 / S.Beta / v2);
+      vx = (S.vx[i] - sh[0] / n) * s;
+      vy = (S.vy[i] - sh[1] / n) * s;
+      vz = (S.vz[i] - sh[2] / n) * s;
+    }
+    S.vx[i] = vx;
+    S.vy[i] = vy;
+    S.vz[i] = vz;
+    if (redrift) {
+      double4 p = S.pos[i];
+      const double dr = drift(S, p, half_kick(S, S.ax[i], vx), half_kick(S, S.ay[i], vy),
+                              half_kick(S, S.az[i], vz));
+      S.pos_nx[i] = p;
+      if (S.pos32_nx) S.pos32_nx[i] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
+      d1 = dr;
+      if (p.w != 0) d2 = dr;
+    }
   }
-  const double n = S.NN - S.N_anchor;
-  const double v2 = sh[3] / (S.NN - S.N_anchor - 1);
-  const double s = sqrt(3.0 / S.Beta / v2);
-  S.vx[i] = (S.vx[i] - sh[0] / n) * s;
-  S.vy[i] = (S.vy[i] - sh[1] / n) * s;
-  S.vz[i] = (S.vz[i] - sh[2] / n) * s;
+  if (redrift)
+    dmax_group(S, S.par ^ 1, (blockIdx.x - wm.blk[r]) * (kBlock / 64) + (threadIdx.x >> 6), d1, d2);
 }
 
 // ------------------------------------------------------- observables
@@ -1928,11 +2003,12 @@ template <typename R>
 void launch_short(const Plan &P, hipStream_t st, int mode, int step, const double *eta,
                   double *out) {
   hipLaunchKernelGGL(k_short_range<R>, dim3(P.partG.total), dim3(kSRBlock), 0, st, P.reps, P.partG,
-                     mode, step, eta, out, P.lq, P.lq_cap);
+                     mode, step, eta, out, P.lq, P.lq_cap, P.lq_nx);
 }
 
-void launch_kick_drift(const Plan &P, hipStream_t st) {
-  hipLaunchKernelGGL(k_kick_drift, dim3(P.part.total), dim3(kBlock), 0, st, P.reps, P.part, P.lq);
+void launch_kick_drift(const Plan &P, hipStream_t st, int store_v) {
+  hipLaunchKernelGGL(k_kick_drift, dim3(P.part.total), dim3(kBlock), 0, st, P.reps, P.part, P.lq,
+                     store_v);
 }
 
 // update_verlet_list's LJ rebuild (compt_acc.f90:717-720); every kernel
@@ -1962,13 +2038,13 @@ void launch_set_flags(const Plan &P, hipStream_t st, int clear) {
                      P.lq_cap);
 }
 
-void launch_rescale(const Plan &P, hipStream_t st, int gated) {
+void launch_rescale(const Plan &P, hipStream_t st, int gated, int redrift) {
   hipLaunchKernelGGL(k_vsum, dim3(P.part.total), dim3(kBlock), 0, st, P.reps, P.part, P.vpart, 0,
                      gated);
   hipLaunchKernelGGL(k_vsum, dim3(P.part.total), dim3(kBlock), 0, st, P.reps, P.part, P.vpart, 1,
                      gated);
   hipLaunchKernelGGL(k_vscale, dim3(P.part.total), dim3(kBlock), 0, st, P.reps, P.part, P.vpart,
-                     gated);
+                     gated, redrift);
 }
 
 template void launch_coulomb<float>(const Plan &, hipStream_t, double *, hipEvent_t *);

@@ -909,6 +909,63 @@ void orc_standard_ewald(orc_sys *s, double alpha, int K1, int K2, int K3,
   free(ex); free(ey); free(ez); free(eikr); free(zq); free(af);
 }
 
+/* error_analysis (compt_acc.f90:729-812).  Estimates of the production
+ * parameters (Kolafa & Perram 1992, compt_acc.f90:746-751; the exponents
+ * (1./3) are default real, and (Kmax1*Kmax2*Kmax3)**(1./3) is evaluated in
+ * default real); then the reference Coulomb force: parameters of the rule
+ * at tol = 5, tau_rf = 10 without overrides (compt_acc.f90:756-759), k-range
+ * ceiling(Kmax/3.) (:760-762), Standard_Ewald + real_space (:768-770);
+ * against SPME + real_space at the production parameters (:783-793);
+ * rmsf = sqrt(sum |f_se - f_c|^2 / sum |f_se|^2) over every particle
+ * (the reference sums rows 1..Nq, :797-801; corrected).
+ * out: rmsf, f_r, f_k, alpha', rcc', K'1, K'2, K'3.  acc_se, acc_c (NN,3)
+ * may be null. */
+void orc_error_analysis(orc_sys *s, double out[8], double *acc_se, double *acc_c) {
+  md_params *p = &s->p;
+  const int NN = s->NN;
+  double qtot = 0;
+  for (int m = 0; m < s->Nq; m++) qtot += QC(s->charge[m]) * QC(s->charge[m]);
+  const double third = (double)(1.0f / 3.0f), V = p->Lx * p->Ly * p->Lz;
+  const double ex = exp(-p->tol * p->tol);
+  const double f_r = 2 * fabs(p->qq) * sqrt(qtot / p->rcc / p->Lx / p->Ly / p->Lz) * ex;
+  const float kprod = powf((float)(p->ng[0] * p->ng[1] * p->ng[2]), 1.0f / 3.0f);
+  const double f_k = fabs(p->qq) * p->alpha / pow(V, third) / PI * sqrt(8 * qtot / (double)kprod) * ex;
+  md_params prod = *p, ref = *p;
+  ref.ovr_alpha = ref.ovr_rcc = 0;
+  ref.ovr_ng[0] = ref.ovr_ng[1] = ref.ovr_ng[2] = 0;
+  orc_ewald_rule(&ref, 5.0, 10.0);
+  int K[3];
+  for (int d = 0; d < 3; d++) K[d] = (int)ceilf((float)ref.ng[d] / 3.0f);
+  double *fse = calloc((size_t)NN * 3, sizeof(double));
+  double *fc = calloc((size_t)NN * 3, sizeof(double));
+  /* every pair within rcc', the set the reference's list or pair loop
+   * visits, in the pair-loop form */
+  ref.real_verlet = 0;
+  s->p = ref;
+  orc_standard_ewald(s, ref.alpha, K[0], K[1], K[2], fse);
+  orc_real_space(s, fse);
+  s->p = prod;
+  orc_spme(s, fc);
+  orc_real_space(s, fc);
+  double s1 = 0, s2 = 0;
+  for (size_t k = 0; k < (size_t)NN * 3; k++) {
+    s1 += (fse[k] - fc[k]) * (fse[k] - fc[k]);
+    s2 += fse[k] * fse[k];
+  }
+  out[0] = s2 > 0 ? sqrt(s1 / s2) : 0;
+  out[1] = f_r;
+  out[2] = f_k;
+  out[3] = ref.alpha;
+  out[4] = ref.rcc;
+  out[5] = K[0];
+  out[6] = K[1];
+  out[7] = K[2];
+  if (acc_se) memcpy(acc_se, fse, sizeof(double) * NN * 3);
+  if (acc_c) memcpy(acc_c, fc, sizeof(double) * NN * 3);
+  free(fse);
+  free(fc);
+}
+
 /* ------------------------------------------------------------- noise */
 
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2],

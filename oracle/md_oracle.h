@@ -45,6 +45,7 @@ void orc_real_space(orc_sys *s, double *f);       /* real space + slab     */
 void orc_spme(orc_sys *s, double *f);             /* SPME reciprocal       */
 void orc_standard_ewald(orc_sys *s, double alpha, int k1, int k2, int k3,
                         double *f);
+void orc_error_analysis(orc_sys *s, double out[8], double *acc_se, double *acc_c);
 int orc_compute_force(orc_sys *s, int32_t step, int32_t noise,
                       const double *eta);
 int orc_new_position(orc_sys *s, int32_t step, int32_t noise,

@@ -42,6 +42,7 @@ def lib():
         "orc_real_space": ([c_void_p, dp], None),
         "orc_spme": ([c_void_p, dp], None),
         "orc_standard_ewald": ([c_void_p, c_double, c_int32, c_int32, c_int32, dp], None),
+        "orc_error_analysis": ([c_void_p, dp, dp, dp], None),
         "orc_compute_force": ([c_void_p, c_int32, c_int32, dp], c_int32),
         "orc_new_position": ([c_void_p, c_int32, c_int32, dp], c_int32),
         "orc_update_verlet_list": ([c_void_p], c_int32),
@@ -146,6 +147,14 @@ class Oracle:
         f = np.zeros(3 * self.NN)
         lib().orc_standard_ewald(self.h, alpha, K[0], K[1], K[2], dp(f))
         return f.reshape(3, self.NN).T.copy()
+
+    def error_analysis(self):
+        """(summary dict, f_se (NN,3), f_c (NN,3)) of error_analysis."""
+        out = np.zeros(8)
+        fse, fc = np.zeros(3 * self.NN), np.zeros(3 * self.NN)
+        lib().orc_error_analysis(self.h, dp(out), dp(fse), dp(fc))
+        keys = ["rmsf", "f_r", "f_k", "alpha", "rcc", "K1", "K2", "K3"]
+        return dict(zip(keys, out)), fse.reshape(3, self.NN).T.copy(), fc.reshape(3, self.NN).T.copy()
 
     def compute_force(self, step, noise=0, eta=None):
         return lib().orc_compute_force(self.h, step, noise, dp(None if eta is None else colmajor(eta)))

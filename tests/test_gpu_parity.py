@@ -126,6 +126,54 @@ def test_graph_run_matches_stepwise():
     e2.close()
 
 
+def test_eager_cycles_match_graph(monkeypatch):
+    """The replayed cycle issued on the stream (MD_NO_GRAPH=1, the form
+    tracing tools see) equals graph replay, across runs of odd length."""
+    print("[eager vs graph cycles] default fp32", flush=True)
+    out = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("MD_NO_GRAPH", flag)
+        p, pos, vel, e = _engine("default", MD_FP32)
+        e.compute_force(0, MD_NOISE_PHILOX)
+        e.run(1, 30)
+        e.run(31, 17)
+        e.run(48, 25)
+        out.append(e.download())
+        e.close()
+    for x, y in zip(out[0], out[1]):
+        assert np.array_equal(x, y)
+
+
+def test_odd_multistep_run_matches_stepwise():
+    """With an odd multistep, consecutive replayed cycles start on
+    alternating position buffers (one graph per buffer): the run still
+    equals the stepwise sequence, Coulomb at the multiples of multistep."""
+    print("[odd multistep graph vs stepwise] small", flush=True)
+    from md_brushes_amd import config
+    p = config("small")
+    p.multistep = 5
+    _, pos, vel = system("small")
+    es = []
+    for _ in range(2):
+        e = Engine(p)
+        e.upload(pos, vel)
+        e.build_lists()
+        e.compute_force(0, MD_NOISE_PHILOX)
+        es.append(e)
+    e1, e2 = es
+    e1.run(1, 23)
+    e1.run(24, 18)
+    for s in range(1, 42):
+        e2.new_position(s, MD_NOISE_PHILOX)
+        e2.update_verlet_list()
+    a = e1.download()
+    b = e2.download()
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    e1.close()
+    e2.close()
+
+
 def test_preheat_graph_matches_stepwise():
     """Pre-heat (main.f90:67-75) replayed in graph cycles with the gated
     rescale kernels equals the step-by-step sequence with rescale_velocity
@@ -175,6 +223,24 @@ def test_philox_run_matches_oracle():
     pg, vg, _, _ = e.download()
     po, vo, _, _, _ = o.state()
     assert np.abs(pg - po).max() < 1e-8
+    e.close()
+
+
+def test_time_kernels_restores_state():
+    """Kernel timing leaves positions, velocities and forces as they were,
+    and a run after it stays on the oracle's lists (no bond breaks)."""
+    print("[time_kernels restores] default fp32", flush=True)
+    p, pos, vel, e = _engine("default", MD_FP32)
+    e.compute_force(0, MD_NOISE_PHILOX)
+    e.run(1, 20)
+    before = e.download()
+    ms = e.time_kernels(20)
+    after = e.download()
+    for x, y in zip(before[:3], after[:3]):
+        assert np.array_equal(x, y)
+    assert ms["short_range"] > 0 and ms["real"] > 0
+    e.run(21, 40)
+    e.download()
     e.close()
 
 

@@ -174,6 +174,19 @@ int md_run(md_engine *e, int32_t step0, int32_t nsteps, int32_t preheat);
  * evaluated at the current positions (no noise, no field, no anchor mask). */
 int md_force_parts(md_engine *e, double *f_short, double *f_coul);
 
+/* error_analysis (compt_acc.f90:729-812) on the device: the accuracy of the
+ * engine's Coulomb force against a standard-Ewald reference computed in FP64
+ * from the resident positions.  The reference force follows the routine:
+ * Initialize_ewald_parameters at tol = 5, tau_rf = 10 (no overrides),
+ * k-ranges ceiling(Kmax/3.), Standard_Ewald + real_space (every pair within
+ * r_c, x,y minimum image, slab correction).  out[8]: rmsf =
+ * sqrt(sum |F_se - F_c|^2 / sum |F_se|^2) over every particle (the reference
+ * sums rows 1..Nq), the Kolafa-Perram estimates f_r and f_k of the engine's
+ * own parameters, and the reference's alpha, r_c, K1, K2, K3.  acc_se (NN,3),
+ * may be null: the reference force.  Positions, velocities, forces and the
+ * held Coulomb force are left as they are. */
+int md_error_analysis(md_engine *e, double *out, double *acc_se);
+
 /* Mean duration in milliseconds of each kernel of a step, timed with HIP
  * events on the engine's stream over `iters` launches (no reference
  * counterpart: measurement only; short_range in the form md_run's replayed
@@ -238,6 +251,12 @@ int md_batch_info(md_engine *e, int32_t slot, int64_t *bytes, int32_t *nrep);
  * beads, nbins = 500 bins of Lz/500 (in_output.f90:1878-1886). */
 int md_batch_observe(md_engine *e, int32_t slot, double *height,
                      double *profile, int32_t nbins);
+
+/* md_error_analysis for one replica of a batch (an accuracy spot check of a
+ * served replica, compt_acc.f90:729-812); the replica's trajectory is not
+ * changed. */
+int md_batch_error_analysis(md_engine *e, int32_t slot, double *out,
+                            double *acc_se);
 
 /* Copies the last error message (NUL-terminated) into buf. */
 int md_last_error(md_engine *e, char *buf, int32_t len);

@@ -102,6 +102,8 @@ def load_library(path=None):
         "md_last_error": ([c_void_p, c_char_p, _I], _I),
         "md_time_kernels": ([c_void_p, _I, dp, _I], _I),
         "md_autotune": ([c_void_p, _I, ip], _I),
+        "md_error_analysis": ([c_void_p, dp, dp], _I),
+        "md_batch_error_analysis": ([c_void_p, _I, dp, dp], _I),
         "md_destroy": ([c_void_p], None),
         "md_batch_create": ([POINTER(c_void_p), _I, _I], _I),
         "md_batch_add": ([c_void_p, P, dp, dp, ip], _I),
@@ -119,6 +121,11 @@ def load_library(path=None):
     if path is None:
         _lib = lib
     return lib
+
+
+def _ewald_summary(out):
+    keys = ["rmsf", "f_r", "f_k", "alpha", "rcc", "K1", "K2", "K3"]
+    return {k: (int(v) if k[0] == "K" else float(v)) for k, v in zip(keys, out)}
 
 
 def _dp(a):
@@ -278,6 +285,14 @@ class Engine:
         self._check(self.lib.md_force_parts(self.h, _dp(fs), _dp(fc)), "md_force_parts")
         return fs.reshape(3, N).T, fc.reshape(3, N).T
 
+    def error_analysis(self):
+        """error_analysis (compt_acc.f90:729-812) on the device: (summary,
+        standard-Ewald reference Coulomb force (NN,3))."""
+        out = np.zeros(8)
+        f = np.zeros(3 * self.NN)
+        self._check(self.lib.md_error_analysis(self.h, _dp(out), _dp(f)), "md_error_analysis")
+        return _ewald_summary(out), f.reshape(3, self.NN).T.copy()
+
 
 class Batch:
     """Many replicas of uneven size in one engine (md_batch_* entry points)."""
@@ -350,6 +365,14 @@ class Batch:
         n = c_int32()
         self._check(self.lib.md_batch_info(self.h, slot, byref(b), byref(n)), "md_batch_info")
         return b.value, n.value
+
+    def error_analysis(self, slot):
+        """Accuracy spot check of one replica: (summary, reference force)."""
+        out = np.zeros(8)
+        f = np.zeros(3 * self.sizes[slot])
+        self._check(self.lib.md_batch_error_analysis(self.h, slot, _dp(out), _dp(f)),
+                    "md_batch_error_analysis")
+        return _ewald_summary(out), f.reshape(3, -1).T.copy()
 
     def observe(self, slot):
         """(brush height, z-density histogram of the polymer beads)."""

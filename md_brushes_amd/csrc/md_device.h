@@ -103,7 +103,7 @@ struct Rep {
   int ccap3;
   int *cnt3, *mem3, *scnt3;
   void *pos3s;                     // positions in cell-sorted order (float4 / double4)
-  double *qtot;                    // mean reciprocal force (3) and M_z of the evaluation
+  void *freal;                     // real-space pair sums per charge slot, 3 x Nq (real type)
   int *nbn1, *nb1;                 // LJ list: length per particle, ELL slots
   int *nbn2, *nb2;                 // real list: per charge slot m
   void *meshr;                     // charge mesh Q (real type of the path) K3*K2*K1

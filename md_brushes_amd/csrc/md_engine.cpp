@@ -451,10 +451,9 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
     pz_big = std::max(1, std::min(16, budget / (K3 * csz)));
     S.nqblk = cdiv((long)Nq, mdk::kBlock);  // gather blocks (one lane per charge)
     S.fsp = dalloc<double>(3 * (size_t)Nq);
-    S.qtot = dalloc<double>(4);
-    if (!S.qtot) return fail(MD_ENOMEM, "spme totals");
+    S.freal = dalloc<char>(3 * (size_t)Nq * (fp32 ? 4 : 8));
     S.part = dalloc<double>(4 * (size_t)S.nqblk);
-    if (!S.fsp || !S.part) return fail(MD_ENOMEM, "spme scratch");
+    if (!S.fsp || !S.part || !S.freal) return fail(MD_ENOMEM, "spme scratch");
     lds_x = 2 * (size_t)S.px_tile * K1 * csz;
     lds_y = 2 * (size_t)S.py_tile * K2 * csz;
     lds_z = 2 * (size_t)S.pz_tile * K3 * csz;
@@ -640,6 +639,8 @@ struct md_engine {
   bool fp32 = false;
   int multistep = 0;
   hipStream_t st = nullptr;
+  hipStream_t st2 = nullptr;       // the real-space sweep of a Coulomb evaluation
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::vector<Sys *> sys;          // launch order
   std::map<int, Sys *> slots;      // stable handles
   int next_slot = 0;
@@ -682,6 +683,10 @@ struct md_engine {
     if (d_lq) (void)hipFree(d_lq);
     if (vpart) (void)hipFree(vpart);
     if (h_steps) (void)hipHostFree(h_steps);
+    if (st2) (void)hipStreamSynchronize(st2);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (st2) (void)hipStreamDestroy(st2);
     if (st) (void)hipStreamDestroy(st);
   }
   int fail(int code, const std::string &msg) {
@@ -709,14 +714,19 @@ struct md_engine {
     multistep = ms;
     if (ms <= 0) return fail(MD_EINVAL, "multistep must be positive");
     hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st2, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_join, hipEventDisableTiming);
     return e == hipSuccess ? MD_OK : hip_fail(e, "stream");
   }
   int rebuild();
   int sync();
   int check_one(const Sys *s, const Ctrl &c, std::string &msg) const;
   int check_all();
+  int check_sys(const Sys *s);  // waits for the stream; the status of one replica
   int add(const md_params *p, int *slot);
   int launch_compute_force(const Plan &P, Sys *s, int step, int noise, const double *eta, bool in_step);
+  int error_analysis(Sys *s, const Plan &P, double *out, double *acc_se);
   int build_graph(int q);
   int run_cycles(int ncycles);
   int index_of(const Sys *s) const {
@@ -854,6 +864,13 @@ int md_engine::rebuild() {
     for (int k = 0; k < kMaps; k++) *w[k] = WMap{maps + k * (nrep + 1), nrep, totals[k]};
     P.vpart = vpart;
     P.fp32 = fp32;
+    // MD_TWO_STREAMS=1: the real-space sweep beside the SPME kernels on a
+    // second stream (the same results; slower in graph replay at the
+    // default brush, so off unless asked for)
+    const char *two = std::getenv("MD_TWO_STREAMS");
+    P.st2 = two && two[0] == '1' ? st2 : nullptr;
+    P.ev_fork = ev_fork;
+    P.ev_join = ev_join;
   };
   long nn_all = 0;
   for (Sys *s : sys) nn_all += s->p.NN;
@@ -934,6 +951,17 @@ int md_engine::check_one(const Sys *s, const Ctrl &c, std::string &msg) const {
     return MD_EOVERFLOW;
   }
   return MD_OK;
+}
+
+int md_engine::check_sys(const Sys *s) {
+  int rc = sync();
+  if (rc) return rc;
+  Ctrl c;
+  hipError_t e = hipMemcpy(&c, s->h.ctrl, sizeof c, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "control block");
+  std::string msg;
+  rc = check_one(s, c, msg);
+  return rc ? fail(rc, msg) : MD_OK;
 }
 
 int md_engine::check_all() {
@@ -1523,6 +1551,136 @@ extern "C" int md_autotune(md_engine *e, int32_t iters, int32_t *choice) {
   return rc;
 }
 
+// error_analysis (compt_acc.f90:729-812) for one replica.  The production
+// estimates f_r, f_k (compt_acc.f90:746-751: exponents (1./3) in default
+// real, (Kmax1*Kmax2*Kmax3)**(1./3) evaluated in default real, Kmax the
+// mesh); the reference force: the rule at tol = 5, tau_rf = 10 without
+// overrides (:756-759), k-ranges ceiling(Kmax/3.) (:760-762),
+// Standard_Ewald + real_space over every pair within r_c (:768-770), on the
+// device in FP64; against the replica's own Coulomb force (SPME + real
+// space + slab, in the engine's precision); rmsf over every particle (the
+// reference sums rows 1..Nq, :797-801).
+int md_engine::error_analysis(Sys *s, const Plan &P, double *out, double *acc_se) {
+  if (!out) return fail(MD_EINVAL, "out is null");
+  if (!s->coulomb) return fail(MD_EINVAL, "error_analysis needs a charged system");
+  int rc = check_sys(s);
+  if (rc) return rc;
+  const md_params &p = s->p;
+  const int NN = p.NN, Nq = s->h.Nq;
+  // production estimates
+  std::vector<double4> hp(NN);
+  hipError_t err = hipMemcpy(hp.data(), s->h.pos, NN * sizeof(double4), hipMemcpyDeviceToHost);
+  if (err != hipSuccess) return hip_fail(err, "positions");
+  double qtot = 0;
+  for (int m = 0; m < Nq; m++) qtot += hp[s->charge[m]].w * hp[s->charge[m]].w;
+  const double kPi = 3.141592653589793, third = (double)(1.0f / 3.0f);
+  const double ex = std::exp(-p.tol * p.tol);
+  const double f_r = 2 * std::fabs(p.qq) * std::sqrt(qtot / p.rcc / p.Lx / p.Ly / p.Lz) * ex;
+  const float kprod = std::pow((float)(p.ng[0] * p.ng[1] * p.ng[2]), 1.0f / 3.0f);
+  const double f_k = std::fabs(p.qq) * p.alpha / std::pow(p.Lx * p.Ly * p.Lz, third) / kPi *
+                     std::sqrt(8 * qtot / (double)kprod) * ex;
+  // parameters of the reference force
+  double alpha, rcc, rvc;
+  int k[3], rv;
+  mdk::ewald_rule(&p, 5.0, 10.0, false, &alpha, &rcc, &rvc, k, &rv);
+  int K[3];
+  for (int d = 0; d < 3; d++) K[d] = (int)std::ceil((float)(int)std::ceil((float)k[d] * 3.0f) / 3.0f);
+  // k-vectors in the reference's order (fourier_function, compt_acc.f90:650-668)
+  const double Ze = p.Zfac, c1 = 2 * kPi / p.Lx, c2 = 2 * kPi / p.Ly, c3 = 2 * kPi / p.Lz / Ze;
+  std::vector<double4> kv;
+  for (int r = 0; r <= K[2]; r++) {
+    const double factor = r == 0 ? 1 : 2;
+    for (int pp = -K[0]; pp <= K[0]; pp++)
+      for (int q = -K[1]; q <= K[1]; q++) {
+        const float a = 1.0f * pp / K[0], b = 1.0f * q / K[1], c = 1.0f * r / K[2];
+        const float kcut = a * a + b * b + c * c;
+        if (kcut > 1 || kcut == 0) continue;
+        const double kx = pp * c1, ky = q * c2, kz = r * c3;
+        const double ksqr = kx * kx + ky * ky + kz * kz;
+        const double ek = factor * 4 * kPi / (p.Lx * p.Ly * p.Lz * Ze) *
+                          std::exp(-ksqr / 4 / (alpha * alpha)) / ksqr * p.lb / p.Beta;
+        kv.push_back(make_double4(kx, ky, kz, ek));
+      }
+  }
+  mdk::EwaldArgs a;
+  a.Nq = Nq;
+  a.NN = NN;
+  a.nk = (int)kv.size();
+  const int bq = cdiv(Nq, 256), bk = cdiv(a.nk, 256);
+  a.ms = std::max(1, std::min(16, cdiv(1024, bk)));
+  a.ks = std::max(1, std::min(bk, cdiv(1024, bq)));
+  a.rs = std::max(1, std::min(bq, cdiv(1024, bq)));
+  a.charge = s->h.charge;
+  a.pos = s->h.pos;
+  a.Lx = p.Lx;
+  a.Ly = p.Ly;
+  a.hLx = p.Lx / 2;
+  a.hLy = p.Ly / 2;
+  a.alpha = alpha;
+  a.alpha2 = alpha * alpha;
+  a.c2 = 2 * alpha / std::sqrt(kPi);
+  a.rcc2 = rcc * rcc;
+  a.c1 = p.lb / p.Beta;
+  a.slab = 4 * kPi / p.Lx / p.Ly / p.Lz / Ze;
+  const size_t nq = std::max(Nq, 1), nk = std::max(a.nk, 1);
+  const size_t bytes = 32 * nq + 32 * nk + 16 * nk * a.ms + 16 * nk + 24 * nq * (a.ks + a.rs) + 256;
+  char *buf = nullptr;
+  if (hipMalloc((void **)&buf, bytes) != hipSuccess) return fail(MD_ENOMEM, "error_analysis buffers");
+  char *cur_b = buf;
+  auto carve = [&](size_t n) {
+    char *x = cur_b;
+    cur_b += (n + 255) / 256 * 256;
+    return x;
+  };
+  a.cq = (double4 *)carve(32 * nq);
+  double4 *dkv = (double4 *)carve(32 * nk);
+  a.kv = dkv;
+  a.sp = (double2 *)carve(16 * nk * a.ms);
+  a.sk = (double2 *)carve(16 * nk);
+  a.rp = (double *)carve(24 * nq * a.ks);
+  a.qp = (double *)carve(24 * nq * a.rs);
+  a.mz = (double *)carve(8);
+  a.out = s->d_out1;
+  const size_t n3 = 3 * (size_t)NN;
+  err = hipMemcpyAsync(dkv, kv.data(), kv.size() * sizeof(double4), hipMemcpyHostToDevice, st);
+  if (err == hipSuccess) err = hipMemsetAsync(s->d_out1, 0, n3 * sizeof(double), st);
+  if (err == hipSuccess) err = hipMemsetAsync(s->d_out2, 0, n3 * sizeof(double), st);
+  if (err == hipSuccess) {
+    if (a.nk > 0) mdk::launch_ewald(a, st);
+    if (fp32) mdk::launch_coulomb<float>(P, st, s->d_out2);
+    else mdk::launch_coulomb<double>(P, st, s->d_out2);
+    err = hipStreamSynchronize(st);
+  }
+  if (err == hipSuccess) err = hipGetLastError();
+  std::vector<double> fse(n3), fc(n3);
+  if (err == hipSuccess) err = hipMemcpy(fse.data(), s->d_out1, n3 * sizeof(double), hipMemcpyDeviceToHost);
+  if (err == hipSuccess) err = hipMemcpy(fc.data(), s->d_out2, n3 * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(buf);
+  if (err != hipSuccess) return hip_fail(err, "error_analysis");
+  double s1 = 0, s2 = 0;
+  for (size_t j = 0; j < n3; j++) {
+    s1 += (fse[j] - fc[j]) * (fse[j] - fc[j]);
+    s2 += fse[j] * fse[j];
+  }
+  out[0] = s2 > 0 ? std::sqrt(s1 / s2) : 0;
+  out[1] = f_r;
+  out[2] = f_k;
+  out[3] = alpha;
+  out[4] = rcc;
+  out[5] = K[0];
+  out[6] = K[1];
+  out[7] = K[2];
+  if (acc_se) std::memcpy(acc_se, fse.data(), n3 * sizeof(double));
+  return check_sys(s);
+}
+
+extern "C" int md_error_analysis(md_engine *e, double *out, double *acc_se) {
+  ENGINE_CHECK(e && out);
+  FIRST(e);
+  STATE_CHECK(e);
+  return e->error_analysis(s, P, out, acc_se);
+}
+
 // ------------------------------------------------------------- batches
 
 extern "C" int md_batch_create(md_engine **out, int32_t precision, int32_t multistep) {
@@ -1662,4 +1820,16 @@ extern "C" int md_batch_observe(md_engine *e, int32_t slot, double *height, doub
   if (profile)
     for (int k = 0; k < mdk::kHistBins; k++) profile[k] = hist[k];
   return MD_OK;
+}
+
+extern "C" int md_batch_error_analysis(md_engine *e, int32_t slot, double *out, double *acc_se) {
+  ENGINE_CHECK(e && out);
+  auto it = e->slots.find(slot);
+  if (it == e->slots.end()) return e->fail(MD_EINVAL, "no such slot");
+  if (e->dirty) {
+    int rc = e->rebuild();
+    if (rc) return rc;
+  }
+  Sys *s = it->second;
+  return e->error_analysis(s, e->single[e->index_of(s)], out, acc_se);
 }

@@ -11,7 +11,7 @@
 #include <vector>
 
 #include "../../include/md_engine.h"
-#include "md_device.h"
+#include "md_kernels.h"
 
 namespace {
 
@@ -127,6 +127,41 @@ extern "C" int md_read_params(const char *system_data, const char *force_data,
   return MD_OK;
 }
 
+namespace mdk {
+
+// Initialize_ewald_parameters (compt_acc.f90:387-414) at (tol, tau_rf):
+// alpha, r_c, r_v, the k-ranges before the mesh factor 3, and whether the
+// real space keeps a Verlet list; the overrides ovr_alpha / ovr_rcc apply
+// when use_ovr.
+void ewald_rule(const md_params *p, double tol, double tau_rf, bool use_ovr, double *alpha_out,
+                double *rcc_out, double *rvc_out, int k[3], int *real_verlet) {
+  const double V = p->Lx * p->Ly * p->Lz;
+  double alpha = std::pow(tau_rf * kPi * kPi * kPi * p->Nq / (V * V), 1.0 / 6);
+  double rcc = tol / alpha;
+  if (use_ovr && p->ovr_alpha > 0) {
+    alpha = p->ovr_alpha;
+    rcc = p->ovr_rcc > 0 ? p->ovr_rcc : tol / alpha;
+  }
+  double rvc = rcc + p->rsk;
+  const double len[3] = {p->Lx, p->Ly, p->Lz * p->Zfac};
+  if ((int)(p->Lx / rvc) * (int)(p->Ly / rvc) * (int)(p->Lz / rvc) > 27) {
+    for (int d = 0; d < 3; d++) k[d] = (int)std::ceil(tol * len[d] * alpha / kPi);
+    *real_verlet = 1;
+  } else {
+    rcc = std::min(p->Lx, p->Ly) / 2;
+    if (p->Lx <= p->Ly) rcc = p->Lx / 2;
+    for (int d = 0; d < 3; d++) k[d] = (int)std::ceil(tol * tol / kPi * len[d] / rcc);
+    alpha = tol / rcc;
+    rvc = rcc + p->rsk;
+    *real_verlet = 0;
+  }
+  *alpha_out = alpha;
+  *rcc_out = rcc;
+  *rvc_out = rvc;
+}
+
+}  // namespace mdk
+
 extern "C" int md_derive_params(md_params *p) {
   if (!p || p->arm < 0 || p->arm > MD_MAX_ARM || p->sigmag <= 0 || p->Lz <= 0)
     return MD_EINVAL;
@@ -169,27 +204,9 @@ extern "C" int md_derive_params(md_params *p) {
   p->alpha = p->rcc = p->rvc = 0;
   p->ng[0] = p->ng[1] = p->ng[2] = 0;
   if (p->qq == 0) return MD_OK;
-  // Ewald parameters: compt_acc.f90:387-420
-  const double V = p->Lx * p->Ly * p->Lz;
-  double alpha = std::pow(p->tau_rf * kPi * kPi * kPi * p->Nq / (V * V), 1.0 / 6);
-  double rcc = p->tol / alpha;
-  if (p->ovr_alpha > 0) {
-    alpha = p->ovr_alpha;
-    rcc = p->ovr_rcc > 0 ? p->ovr_rcc : p->tol / alpha;
-  }
-  double rvc = rcc + p->rsk;
+  double alpha, rcc, rvc;
   int k[3];
-  const double len[3] = {p->Lx, p->Ly, p->Lz * p->Zfac};
-  if ((int)(p->Lx / rvc) * (int)(p->Ly / rvc) * (int)(p->Lz / rvc) > 27) {
-    for (int d = 0; d < 3; d++) k[d] = (int)std::ceil(p->tol * len[d] * alpha / kPi);
-    p->real_verlet = 1;
-  } else {
-    rcc = std::min(p->Lx, p->Ly) / 2;
-    if (p->Lx <= p->Ly) rcc = p->Lx / 2;
-    for (int d = 0; d < 3; d++) k[d] = (int)std::ceil(p->tol * p->tol / kPi * len[d] / rcc);
-    alpha = p->tol / rcc;
-    rvc = rcc + p->rsk;
-  }
+  mdk::ewald_rule(p, p->tol, p->tau_rf, true, &alpha, &rcc, &rvc, k, &p->real_verlet);
   p->alpha = alpha;
   p->rcc = rcc;
   p->rvc = rvc;

@@ -1,5 +1,6 @@
 // md_kernels.h -- launch plan and kernel launchers of the brush engine.
 #pragma once
+#include "../../include/md_engine.h"
 #include "md_device.h"
 
 namespace mdk {
@@ -37,6 +38,10 @@ struct Plan {
   int grid_bin = 1, grid_build = 1;
   double *vpart = nullptr;  // partial sums of rescale_velocity
   bool fp32 = false;
+  // second stream of a Coulomb evaluation (the real-space sweep runs there,
+  // beside the SPME kernels) and its fork / join events; null: one stream
+  hipStream_t st2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 template <typename R>
@@ -57,5 +62,29 @@ void launch_set_flags(const Plan &P, hipStream_t st, int clear);
 // velocities before the rescale: redo it with the rescaled ones
 void launch_rescale(const Plan &P, hipStream_t st, int gated = 0, int redrift = 0);
 void launch_observe(const Plan &P, hipStream_t st, int *hist, double *hsum);
+
+// Standard-Ewald reference force of error_analysis (FP64, one replica).
+struct EwaldArgs {
+  int Nq = 0, NN = 0, nk = 0;
+  int ms = 1, ks = 1, rs = 1;       // slices: charges (S), k-vectors (F), sources (real)
+  const int *charge = nullptr;      // particle index of charge m
+  const double4 *pos = nullptr;     // canonical positions (x, y, z, q)
+  double Lx = 0, Ly = 0, hLx = 0, hLy = 0;
+  double alpha = 0, alpha2 = 0, c2 = 0, rcc2 = 0;  // c2 = 2 alpha / sqrt(pi)
+  double c1 = 0, slab = 0;          // lB/Beta, 4 pi / (Lx Ly Lz Zfac)
+  const double4 *kv = nullptr;      // k-vectors (kx, ky, kz, e_k)
+  double4 *cq = nullptr;            // charges (x, y, z, q)
+  double2 *sp = nullptr, *sk = nullptr;  // partial / summed structure factors
+  double *rp = nullptr, *qp = nullptr;   // reciprocal / real-space partial forces
+  double *mz = nullptr;
+  double *out = nullptr;            // (NN, 3): charged particles written
+};
+void launch_ewald(const EwaldArgs &a, hipStream_t st);
+
+// Initialize_ewald_parameters (compt_acc.f90:387-414) at (tol, tau_rf):
+// alpha, r_c, r_v, the k-ranges before the mesh factor 3, and whether the
+// real space keeps a Verlet list; ovr_alpha / ovr_rcc apply when use_ovr.
+void ewald_rule(const md_params *p, double tol, double tau_rf, bool use_ovr, double *alpha,
+                double *rcc, double *rvc, int k[3], int *real_verlet);
 
 }  // namespace mdk

@@ -765,7 +765,7 @@ __global__ __launch_bounds__(kBlock) void k_build_rl(const Rep *__restrict__ rep
   }
   S.nbn2[t] = n;
   if (n > S.nbcap2) atomicOr(&c->ovf, 8);
-  atomicMax(&c->max_nb2, n);
+  if (n > c->max_nb2) atomicMax(&c->max_nb2, n);
 }
 
 // ------------------------------------------------------------- SPME
@@ -1398,14 +1398,27 @@ __device__ __forceinline__ void store_coulomb(const Rep &S, int m, int i, double
   }
 }
 
-// Sum of the gather kernel's block partials (mean reciprocal force, 3, and
-// M_z) into qtot: each thread sums a fixed strided subset, then a fixed tree
-// over the block, so the bits do not depend on scheduling.  One workgroup
-// per replica.
-__global__ __launch_bounds__(kBlock) void k_qtot(const Rep *__restrict__ reps) {
-  const Rep &S = reps[blockIdx.x];
-  if (S.Nq == 0 || !S.qtot) return;
+// The real-space pair sums of charge slot m, before the lb/Beta factor.
+template <typename R>
+__device__ __forceinline__ void store_real(const Rep &S, int m, R fx, R fy, R fz) {
+  R *f = (R *)S.freal;
+  f[m] = fx;
+  f[(size_t)S.Nq + m] = fy;
+  f[2 * (size_t)S.Nq + m] = fz;
+}
+
+// Joins the two halves of a Coulomb evaluation, which run concurrently
+// (SPME on one stream, the real-space sweep on another): each workgroup sums
+// the gather kernel's block partials (mean reciprocal force, 3, and M_z) --
+// each thread a fixed strided subset, then a fixed tree, the same bits in
+// every workgroup -- and stores the Coulomb force of its charges.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_coul_store(const Rep *__restrict__ reps, WMap wm,
+                                                       double *__restrict__ out) {
+  const int r = find_rep(wm, blockIdx.x);
+  const Rep &S = reps[r];
   __shared__ double red[4][kBlock / 64];
+  __shared__ double tot[4];
   double v[4] = {0, 0, 0, 0};
   for (int b = threadIdx.x; b < S.nqblk; b += kBlock)
     for (int d = 0; d < 4; d++) v[d] += S.part[(size_t)b * 4 + d];
@@ -1417,8 +1430,15 @@ __global__ __launch_bounds__(kBlock) void k_qtot(const Rep *__restrict__ reps) {
   if (threadIdx.x < 4) {
     double t = 0;
     for (int w = 0; w < kBlock / 64; w++) t += red[threadIdx.x][w];
-    S.qtot[threadIdx.x] = t;
+    tot[threadIdx.x] = t;
   }
+  __syncthreads();
+  const int m = (blockIdx.x - wm.blk[r]) * kBlock + threadIdx.x;
+  if (m >= S.Nq) return;
+  const R *f = (const R *)S.freal;
+  const int i = S.charge[m];
+  store_coulomb<R>(S, m, i, S.pos[i].w, tot, f[m], f[(size_t)S.Nq + m], f[2 * (size_t)S.Nq + m],
+                   out);
 }
 
 // erfc-screened Coulomb pair force of real_space (compt_acc.f90:1034-1036)
@@ -1546,7 +1566,8 @@ __global__ __launch_bounds__(kBlock) void k_sort_q(const Rep *__restrict__ reps,
   if (threadIdx.x == 0) {
     S.scnt3[cell] = n;
     S.cnt3[cell] = 0;
-    if (raw) atomicMax(&S.ctrl->max_cell2, raw);
+    // the largest occupancy seen (read first: one address for every cell)
+    if (raw > S.ctrl->max_cell2) atomicMax(&S.ctrl->max_cell2, raw);
   }
 }
 
@@ -1612,8 +1633,7 @@ __device__ __forceinline__ void target_tile(const V pi, const V *cand, int nc, i
 // same order as the wavefront form.  Workgroups stride over the items and
 // skip the empty ones in bulk.
 template <typename R, typename V>
-__device__ void real_cells_wg(const Rep &S, int b0, int nb, V *cand, short *hit_all,
-                              double *__restrict__ out) {
+__device__ void real_cells_wg(const Rep &S, int b0, int nb, V *cand, short *hit_all) {
   constexpr int TN = RealTile<R>::N, NW = kBlock / 64;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nch = (S.ccap3 + kRealChunkWG - 1) / kRealChunkWG;
@@ -1622,7 +1642,6 @@ __device__ void real_cells_wg(const Rep &S, int b0, int nb, V *cand, short *hit_
   __shared__ int work[kBlock], wcnt[NW];
   __shared__ double accs[kRealChunkWG][3];
   short *hit = hit_all + w * TN;
-  const double *tot = S.qtot;
   const V *ps = (const V *)S.pos3s;
   const R Lx = (R)S.Lx, Ly = (R)S.Ly, hLx = (R)S.hLx, hLy = (R)S.hLy, rcc2 = (R)S.rcc2;
   const R alpha = (R)S.alpha, alpha2 = (R)S.alpha2, c2 = (R)S.c2;
@@ -1673,12 +1692,9 @@ __device__ void real_cells_wg(const Rep &S, int b0, int nb, V *cand, short *hit_
         }
         __syncthreads();
       }
-      if (threadIdx.x < nt) {
-        const int m = S.mem3[(size_t)cell * S.ccap3 + t0 + threadIdx.x];
-        const int i = S.charge[m];
-        store_coulomb<R>(S, m, i, S.pos[i].w, tot, (R)accs[threadIdx.x][0],
-                         (R)accs[threadIdx.x][1], (R)accs[threadIdx.x][2], out);
-      }
+      if (threadIdx.x < nt)
+        store_real<R>(S, S.mem3[(size_t)cell * S.ccap3 + t0 + threadIdx.x], (R)accs[threadIdx.x][0],
+                      (R)accs[threadIdx.x][1], (R)accs[threadIdx.x][2]);
       __syncthreads();  // accs, offs and the tile are reused
     }
     __syncthreads();  // work[] is refilled
@@ -1697,8 +1713,7 @@ __device__ void real_cells_wg(const Rep &S, int b0, int nb, V *cand, short *hit_
 // sum over the tiles and the lanes store their targets together.
 // Wavefronts stride over the items and skip the empty ones 64 at a time.
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_real_cells(const Rep *__restrict__ reps, WMap wm,
-                                                       double *__restrict__ out) {
+__global__ __launch_bounds__(kBlock) void k_real_cells(const Rep *__restrict__ reps, WMap wm) {
   using V = typename std::conditional<sizeof(R) == 4, float4, double4>::type;
   constexpr int TN = RealTile<R>::N, NW = kBlock / 64;
   const int r = find_rep(wm, blockIdx.x);
@@ -1713,13 +1728,12 @@ __global__ __launch_bounds__(kBlock) void k_real_cells(const Rep *__restrict__ r
   __shared__ short hit_all[NW][TN];
   V *cand = cand_all[w];
   short *hit = hit_all[w];
-  const double *tot = S.qtot;
   const V *ps = (const V *)S.pos3s;
   const R Lx = (R)S.Lx, Ly = (R)S.Ly, hLx = (R)S.hLx, hLy = (R)S.hLy, rcc2 = (R)S.rcc2;
   const R alpha = (R)S.alpha, alpha2 = (R)S.alpha2, c2 = (R)S.c2;
   if (S.real_wg) {
     real_cells_wg<R>(S, blockIdx.x - wm.blk[r], wm.blk[r + 1] - wm.blk[r], &cand_all[0][0],
-                     &hit_all[0][0], out);
+                     &hit_all[0][0]);
     return;
   }
   for (int it0 = gw; it0 < nitem; it0 += nwv * 64) {
@@ -1797,11 +1811,7 @@ __global__ __launch_bounds__(kBlock) void k_real_cells(const Rep *__restrict__ r
         }
         wave_sync();  // the tile is refilled
       }
-      if (lane < nt) {
-        const int m = S.mem3[(size_t)cell * S.ccap3 + t0 + lane];
-        const int i = S.charge[m];
-        store_coulomb<R>(S, m, i, S.pos[i].w, tot, (R)ax, (R)ay, (R)az, out);
-      }
+      if (lane < nt) store_real<R>(S, S.mem3[(size_t)cell * S.ccap3 + t0 + lane], (R)ax, (R)ay, (R)az);
     }
   }
 }
@@ -1962,12 +1972,21 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) 
   auto mark = [&]() {
     if (ev) (void)hipEventRecord(ev[k++], st);
   };
+  // the real-space sweep forks onto the plan's second stream once the cells
+  // are sorted and joins before the store (timed launches stay in order)
+  const bool fork = P.st2 && !ev && P.cell3.total;
   mark();
   hipLaunchKernelGGL(k_bin_q, dim3(P.chg.total), dim3(kBlock), 0, st, P.reps, P.chg);
   mark();
   if (P.cell3.total)
     hipLaunchKernelGGL(k_sort_q<R>, dim3(P.cell3.total), dim3(kBlock), P.lds_sort3, st, P.reps, P.cell3);
   mark();
+  if (fork) {
+    (void)hipEventRecord(P.ev_fork, st);
+    (void)hipStreamWaitEvent(P.st2, P.ev_fork, 0);
+    hipLaunchKernelGGL(k_real_cells<R>, dim3(P.cell3c.total), dim3(kBlock), 0, P.st2, P.reps, P.cell3c);
+    (void)hipEventRecord(P.ev_join, P.st2);
+  }
   hipLaunchKernelGGL(k_spread<R>, dim3(P.brick.total), dim3(kBlock), 0, st, P.reps, P.brick);
   mark();
   // replicas with fused planes take k_plane_*, the others the x and y passes
@@ -1992,10 +2011,11 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) 
     hipLaunchKernelGGL(k_fftx_inv<R>, dim3(P.fx.total), dim3(kBlock), P.lds_x, st, P.reps, P.fx);
   mark();
   hipLaunchKernelGGL(k_gather<R>, dim3(P.chgG.total), dim3(kBlock), 0, st, P.reps, P.chgG);
-  hipLaunchKernelGGL(k_qtot, dim3(P.nrep), dim3(kBlock), 0, st, P.reps);
   mark();
-  if (P.cell3.total)
-    hipLaunchKernelGGL(k_real_cells<R>, dim3(P.cell3c.total), dim3(kBlock), 0, st, P.reps, P.cell3c, out);
+  if (fork) (void)hipStreamWaitEvent(st, P.ev_join, 0);
+  else if (P.cell3.total)
+    hipLaunchKernelGGL(k_real_cells<R>, dim3(P.cell3c.total), dim3(kBlock), 0, st, P.reps, P.cell3c);
+  hipLaunchKernelGGL(k_coul_store<R>, dim3(P.chgG.total), dim3(kBlock), 0, st, P.reps, P.chgG, out);
   mark();
 }
 

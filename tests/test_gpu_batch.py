@@ -137,3 +137,32 @@ def test_serving_stream_results_match_alone():
         assert np.array_equal(prof, r.result["profile"])
     lat = [r.t_done - r.arrival for r in done]
     assert min(lat) > 0
+
+
+def test_batch_error_analysis_spot_check():
+    """The accuracy spot check of a served replica equals the single-system
+    engine's and leaves every replica's trajectory unchanged."""
+    print("[batch] error_analysis spot check", flush=True)
+    p, pos, vel = _req(3, 30, 1, 3, 0.3, 31, MD_FP64)
+    other = _req(2, 40, 3, 1, 0.6, 32, MD_FP64)
+    e = Engine(p)
+    e.upload(pos, vel)
+    e.build_lists()
+    ge, fe = e.error_analysis()
+    e.close()
+    out = []
+    for check in (True, False):
+        b = Batch(MD_FP64)
+        b.add(*other)
+        sl = b.add(p, pos, vel)
+        if check:
+            gb, fb = b.error_analysis(sl)
+            assert gb == ge
+            assert np.array_equal(fb, fe)
+        b.run(2)
+        if check:
+            b.error_analysis(sl)
+        b.run(2)
+        out.append(b.download(sl)[0])
+        b.close()
+    assert np.array_equal(out[0], out[1])

@@ -341,6 +341,24 @@ def test_launch_forms_do_not_change_results(prec, monkeypatch):
     e.close()
 
 
+@pytest.mark.parametrize("prec", [MD_FP64, MD_FP32])
+def test_concurrent_real_space_matches_one_stream(prec, monkeypatch):
+    """The real-space sweep on its own stream beside the SPME kernels gives
+    the states of the one-stream order, in graph replay and stepwise."""
+    print(f"[two streams vs one] default prec={prec}", flush=True)
+    out = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("MD_TWO_STREAMS", flag)
+        p, pos, vel, e = _engine("default", prec)
+        e.compute_force(0, MD_NOISE_PHILOX)
+        e.run(1, 33)
+        out.append(e.download())
+        out.append(e.force_parts())
+        e.close()
+    for x, y in zip(out[0] + out[1], out[2] + out[3]):
+        assert np.array_equal(x, y)
+
+
 @pytest.mark.parametrize("sweep", ["wave", "wg"])
 @pytest.mark.parametrize("name", NAMES)
 def test_real_space_sweep_forms(name, sweep, monkeypatch):
@@ -370,3 +388,47 @@ def test_real_space_sweep_forms_identical(prec, monkeypatch):
         out.append(e.force_parts()[1])
         e.close()
     assert np.array_equal(out[0], out[1])
+
+
+def test_error_analysis_matches_oracle():
+    """error_analysis on the device (standard Ewald at tol 5 + real space,
+    FP64) equals the oracle's restatement: the reference force, its
+    parameters and the SPME accuracy figure rmsf."""
+    print("[error_analysis] small fp64", flush=True)
+    p, pos, vel, e = _engine("small")
+    o = _oracle("small")
+    g, fse = e.error_analysis()
+    r, fse_o, _ = o.error_analysis()
+    e.close()
+    for k in ("K1", "K2", "K3"):
+        assert g[k] == int(r[k]), k
+    for k in ("alpha", "rcc", "f_r", "f_k"):
+        assert abs(g[k] - r[k]) <= 1e-12 * abs(r[k]), k
+    assert orc.rel_rms(fse, fse_o) < 1e-10
+    assert abs(g["rmsf"] - r["rmsf"]) < 1e-6 * r["rmsf"]
+
+
+def test_error_analysis_default_fp32():
+    """At the default brush the FP32 engine's Coulomb force is within 1e-3
+    of the standard-Ewald reference, whose device value equals the oracle's;
+    the check leaves the trajectory as it would have been."""
+    print("[error_analysis] default fp32", flush=True)
+    p, pos, vel, e = _engine("default", MD_FP32)
+    g, fse = e.error_analysis()
+    o = _oracle("default")
+    r, fse_o, _ = o.error_analysis()
+    assert orc.rel_rms(fse, fse_o) < 1e-10
+    assert g["rmsf"] < 1e-3
+    # the FP32 figure differs from the FP64 oracle's by the FP32 rounding only
+    assert abs(g["rmsf"] - r["rmsf"]) < 0.1 * r["rmsf"]
+    e2 = _engine("default", MD_FP32)[3]
+    for x in (e, e2):
+        x.compute_force(0, MD_NOISE_PHILOX)
+    e.run(1, 10)
+    e.error_analysis()
+    e.run(11, 10)
+    e2.run(1, 20)
+    for x, y in zip(e.download(), e2.download()):
+        assert np.array_equal(x, y)
+    e.close()
+    e2.close()

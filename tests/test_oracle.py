@@ -11,6 +11,7 @@ import pytest
 import orc
 from md_brushes_amd import MdParams, config, initialize_position, initialize_velocity
 from md_brushes_amd.configs import _apply, _base
+from systems import system
 
 
 # ----------------------------------------------------------------- noise
@@ -333,6 +334,59 @@ def test_spme_converges_to_standard_ewald():
     # SPME side, erfc(3) = 2.2e-5
     assert fine < 5e-5, fine
     assert fine < coarse / 10, (coarse, fine)
+
+
+def test_error_analysis_reference_parameters():
+    """error_analysis's reference set-up (compt_acc.f90:756-762) at the
+    small brush: tol = 5 gives r_v > L/3, the pair-loop branch of the rule:
+    r_c = L/2, alpha = tol/r_c, K = ceil(tol^2/pi len/r_c), and
+    ceiling(ceiling(3 K)/3.) = K."""
+    p, pos, vel = system("small")
+    o = orc.Oracle(p)
+    o.set_state(pos, vel)
+    o.build_lists()
+    r, fse, fc = o.error_analysis()
+    rc = p.Lx / 2
+    assert r["rcc"] == rc and abs(r["alpha"] - 5.0 / rc) < 1e-15
+    for k, L in zip(("K1", "K2", "K3"), (p.Lx, p.Ly, p.Lz * p.Zfac)):
+        assert r[k] == math.ceil(25.0 / math.pi * L / rc)
+    # f_c is the production Coulomb force; rmsf is its distance to f_se
+    assert orc.rel_rms(fc, o.coulomb()) < 1e-14
+    assert abs(r["rmsf"] - orc.rel_rms(fc, fse)) < 1e-12
+
+
+def test_real_space_pair_loop_equals_list():
+    """The pair-loop branch of real_space (compt_acc.f90:998-1010, every
+    pair within r_c), which error_analysis's reference force uses, equals
+    the Verlet-list branch (:1012-1033) on the same box."""
+    p = _charged_box()
+    assert p.real_verlet == 1
+    pos = initialize_position(p, 8)
+    o = orc.Oracle(p)
+    o.set_state(pos)
+    o.build_lists()
+    q = p.copy()
+    q.real_verlet = 0
+    o2 = orc.Oracle(q)
+    o2.set_state(pos)
+    assert orc.rel_rms(o2.real_space(), o.real_space()) < 1e-13
+
+
+def test_error_analysis_rmsf_falls_with_mesh():
+    """rmsf measures the SPME mesh error: a finer mesh brings the production
+    force closer to the standard-Ewald reference."""
+    p = _charged_box()
+    pos = initialize_position(p, 9)
+    out = []
+    for ng in ((16, 16, 32), (32, 32, 64)):
+        q = p.copy()
+        q.ovr_ng[0], q.ovr_ng[1], q.ovr_ng[2] = ng
+        q = orc.derive(q)
+        o = orc.Oracle(q)
+        o.set_state(pos)
+        o.build_lists()
+        out.append(o.error_analysis()[0]["rmsf"])
+    assert out[1] < out[0] / 4, out
 
 
 # ------------------------------------------------------------ integrator

@@ -90,6 +90,21 @@ module md_api
       type(c_ptr), value :: e
       integer(c_int32_t), value :: step0, nsteps, preheat
     end function
+    integer(c_int) function md_error_analysis(e, out, acc_se) bind(C, name="md_error_analysis")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: e
+      real(c_double), dimension(8), intent(out) :: out
+      type(c_ptr), value :: acc_se
+    end function
+    integer(c_int) function md_download_list(e, which, pairs, cap, npair) &
+        bind(C, name="md_download_list")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: e
+      integer(c_int32_t), value :: which
+      type(c_ptr), value :: pairs
+      integer(c_int64_t), value :: cap
+      integer(c_int64_t), intent(out) :: npair
+    end function
     integer(c_int) function md_last_error(e, buf, n) bind(C, name="md_last_error")
       import :: c_int, c_ptr, c_char, c_int32_t
       type(c_ptr), value :: e

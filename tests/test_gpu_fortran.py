@@ -37,6 +37,10 @@ def test_fortran_host_matches_c_abi(tmp_path):
     assert r.returncode == 0 and "Finished!" in r.stdout
     h = np.loadtxt(tmp_path / "data" / "height.txt")
     assert h.shape == (10, 2) and np.all(np.isfinite(h[:, 1]))
+    # error_analysis at the start (main.f90:30-33 of the reference), on the device
+    rms = np.loadtxt(tmp_path / "data" / "rms_force.txt", ndmin=2)
+    assert rms.shape == (1, 6) and rms[0, 0] == 0 and 0 < rms[0, 1] < 1e-2 and rms[0, 2] > 0
+    assert "Rms force of coulomb force is:" in r.stdout
     # the same run through the C-ABI: the host's mesh rounding, seed and flow
     p = derive(raw)
     for d in range(3):

@@ -16,8 +16,9 @@ constexpr int kMaxOrder = 8;
 constexpr int kTile = 16;         // spreading tile edge (mesh points)
 constexpr int kTileSub = 8;       // sub-lists per spreading tile (spreads the appends)
 constexpr int kPlaneBlock = 512;  // threads of the fused FFT plane kernels
-constexpr int kSRLanes = 8;       // lanes per particle in the LJ pass of the short-range kernel
-constexpr int kSRBlock = 512;     // its workgroup: 64 particles
+constexpr int kSRLanes = 4;       // lanes per particle in the LJ pass of the short-range kernel
+constexpr int kSRBlock = 512;     // its workgroup (LJ on 4 wavefronts, the per-particle
+constexpr int kSRParticles = 64;  // parts on the others) takes 64 particles
 constexpr int kRealChunk = 4;     // target charges per wavefront work item of the real-space sweep
 constexpr int kRealChunkWG = 16;  // ... per workgroup work item (replicas with dense cells)
 constexpr int kHistBins = 500;  // SizeHist of the z-density profile (glob_var.f90:59)

@@ -305,7 +305,7 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   }
   set_par(0);
   double *v = dalloc<double>(3 * nn), *a = dalloc<double>(3 * nn);
-  S.ngrp = cdiv(NN, 64);  // the short-range kernel's particle groups
+  S.ngrp = cdiv(NN, mdk::kSRParticles);  // the short-range kernel's particle groups
   S.dpart = dalloc<double2>(2 * (size_t)S.ngrp);
   if (!v || !a || !S.dpart) return fail(MD_ENOMEM, "state");
   S.vx = v; S.vy = v + nn; S.vz = v + 2 * nn;
@@ -471,7 +471,7 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   nb[10] = 0;  // set with the cell capacity (size_lists)
   nb[11] = coulomb ? nbin : 0;
   nb[12] = coulomb && S.plane_fused ? S.K[2] : 0;
-  nb[13] = cdiv(NN, mdk::kSRBlock / mdk::kSRLanes);
+  nb[13] = cdiv(NN, mdk::kSRParticles);
   S.mem1 = S.mem1s = S.mem2 = S.mem3 = nullptr;
   S.pos3s = nullptr;
   S.nb1 = S.nb2 = nullptr;

@@ -28,7 +28,7 @@ struct Plan {
   // cells, real-space sweep cells (workgroup each, per target chunk)
   // ... mesh bricks (one wavefront each), z-planes
   WMap part{}, chg{}, chgG{}, fx{}, fy{}, fz{}, cell1{}, cell1w{}, cell2{}, cell3{}, cell3c{},
-      brick{}, plane{}, partG{};  // partG: particles x kSRLanes lanes
+      brick{}, plane{}, partG{};  // partG: short-range workgroups of kSRParticles
   size_t lds_x = 0, lds_y = 0, lds_z = 0, lds_sort = 0, lds_sort3 = 0, lds_plane = 0;
   // LJ rebuild queue (replicas flagged this step) and the fixed grids of the
   // list kernels, which stride over the queued replicas' particles

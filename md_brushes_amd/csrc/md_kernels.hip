@@ -174,147 +174,68 @@ __global__ __launch_bounds__(kBlock) void k_kick_drift(const Rep *__restrict__ r
 
 // --------------------------------------------------------- short range
 
-// Per-particle part of the short-range kernel: wall, FENE, held Coulomb
-// force, noise, field, anchors, the second half kick (init_update.f90:606)
-// and, for replayed steps (SR_DRIFT), the next step's half kick and drift
-// from the new velocity into the other position buffer, whose displacement
-// it returns in d1 (all) and d2 (charged).
-template <typename R>
-__device__ __forceinline__ void sr_particle(const Rep &S, int i, int mode, int step,
-                                            const double *__restrict__ eta,
-                                            double *__restrict__ out, R fx, R fy, R fz,
-                                            double &d1, double &d2) {
-  const double4 pi = S.pos[i];
+// Wall force on a particle (compt_acc.f90:956-975).
+__device__ __forceinline__ double wall_force(const Rep &S, double4 pi) {
   double wz = 0;
-  {  // wall force (compt_acc.f90:956-975)
-    const double z = pi.z, Lz = S.Lz;
-    if (!(fabs(z) < 0.01 || fabs(z - Lz) < 0.01)) {
-      if (pi.w != 3) {
-        if (z < S.wall_r) {
-          double dl = z, d5 = dl * dl * dl * dl * dl, d11 = d5 * d5 * dl;
-          wz = S.wall_c * (3 / d11 - 1 / d5) * dl;
-        } else if (Lz - z < S.wall_r) {
-          double dl = Lz - z, d5 = dl * dl * dl * dl * dl, d11 = d5 * d5 * dl;
-          wz = -S.wall_c * (3 / d11 - 1 / d5) * dl;
-        }
-      } else {
-        const double ss = S.sigma_s;
-        if (z < S.wall_rs) {
-          double dl = z, t = ss / dl, t5 = t * t * t * t * t, t11 = t5 * t5 * t;
-          wz = S.wall_c * (3 * t11 - t5) * dl / ss / ss;
-        } else if (Lz - z < S.wall_rs) {
-          double dl = Lz - z, t = ss / dl, t5 = t * t * t * t * t, t11 = t5 * t5 * t;
-          wz = -S.wall_c * (3 * t11 - t5) * dl / ss / ss;
-        }
+  const double z = pi.z, Lz = S.Lz;
+  if (!(fabs(z) < 0.01 || fabs(z - Lz) < 0.01)) {
+    if (pi.w != 3) {
+      if (z < S.wall_r) {
+        double dl = z, d5 = dl * dl * dl * dl * dl, d11 = d5 * d5 * dl;
+        wz = S.wall_c * (3 / d11 - 1 / d5) * dl;
+      } else if (Lz - z < S.wall_r) {
+        double dl = Lz - z, d5 = dl * dl * dl * dl * dl, d11 = d5 * d5 * dl;
+        wz = -S.wall_c * (3 / d11 - 1 / d5) * dl;
+      }
+    } else {
+      const double ss = S.sigma_s;
+      if (z < S.wall_rs) {
+        double dl = z, t = ss / dl, t5 = t * t * t * t * t, t11 = t5 * t5 * t;
+        wz = S.wall_c * (3 * t11 - t5) * dl / ss / ss;
+      } else if (Lz - z < S.wall_rs) {
+        double dl = Lz - z, t = ss / dl, t5 = t * t * t * t * t, t11 = t5 * t5 * t;
+        wz = -S.wall_c * (3 * t11 - t5) * dl / ss / ss;
       }
     }
   }
-  {  // FENE over the bonded partners of i
-    const int b0 = S.bstart[i], b1 = S.bstart[i + 1];
-    const R R02 = (R)S.R0_2, kR = (R)S.kR02;
-    for (int b = b0; b < b1; b++) {
-      const int nn = S.bpart[b];
-      double dxd, dyd, dzd;
-      const double rs = rsq_image(pi, S.pos[nn], S.Lx, S.Ly, S.hLx, S.hLy, dxd, dyd, dzd);
-      if (rs > S.R0_2) {
-        set_error(S.ctrl, MD_EBOND, i < nn ? i : nn, step);
-        continue;
-      }
-      const R gg = -kR / (R02 - (R)rs);
-      fx += gg * (R)dxd;
-      fy += gg * (R)dyd;
-      fz += gg * (R)dzd;
-    }
-  }
-  double ax = (double)fx, ay = (double)fy, az = (double)fz + wz;
-  if (mode & SR_HELD) {
-    if constexpr (sizeof(R) == 4) {
-      ax += ((const float *)S.ccx)[i];
-      ay += ((const float *)S.ccy)[i];
-      az += ((const float *)S.ccz)[i];
-    } else {
-      ax += ((const double *)S.ccx)[i];
-      ay += ((const double *)S.ccy)[i];
-      az += ((const double *)S.ccz)[i];
-    }
-  }
-  if (mode & (SR_NOISE_ETA | SR_NOISE_PHILOX)) {
-    double n0, n1, n2;
-    if (mode & SR_NOISE_ETA) {
-      n0 = eta[i];
-      n1 = eta[(size_t)S.NN + i];
-      n2 = eta[2 * (size_t)S.NN + i];
-    } else {
-      uint32_t w[4];
-      philox4x32_10((uint32_t)i, (uint32_t)step, kNoiseTag, 0u, (uint32_t)S.seed,
-                    (uint32_t)(S.seed >> 32), w);
-      const double s32 = 1.0 / 4294967296.0;
-      const double u0 = ((double)w[0] + 0.5) * s32, u1 = ((double)w[1] + 0.5) * s32;
-      const double u2 = ((double)w[2] + 0.5) * s32, u3 = ((double)w[3] + 0.5) * s32;
-      const double r1 = sqrt(-2 * log(u0)), r2 = sqrt(-2 * log(u2));
-      double sn, cs;
-      sincospi(2 * u1, &sn, &cs);
-      n0 = r1 * cs;
-      n1 = r1 * sn;
-      n2 = r2 * cospi(2 * u3);
-    }
-    ax += n0 * S.sig;
-    ay += n1 * S.sig;
-    az += n2 * S.sig;
-  }
-  if (mode & SR_FIELD) az += pi.w * S.EF;
-  if ((mode & SR_ANCHOR) && (S.flags[i] & 1)) ax = ay = az = 0;
-  if (!(isfinite(ax) && isfinite(ay) && isfinite(az))) set_error(S.ctrl, MD_ENONFINITE, i, step);
-  if (out) {
-    out[i] = ax;
-    out[(size_t)S.NN + i] = ay;
-    out[2 * (size_t)S.NN + i] = az;
-    return;
-  }
-  double vx, vy, vz;
-  if (mode & SR_KICK) {
-    vx = S.vx[i];
-    vy = S.vy[i];
-    vz = S.vz[i];
-    if (mode & SR_DRIFT) {  // v holds v(n-1): redo this step's first half kick
-      vx = half_kick(S, S.ax[i], vx);
-      vy = half_kick(S, S.ay[i], vy);
-      vz = half_kick(S, S.az[i], vz);
-    }
-  }
-  S.ax[i] = ax;
-  S.ay[i] = ay;
-  S.az[i] = az;
-  if (mode & SR_KICK) {
-#pragma clang fp contract(off)
-    const double dt = S.dt;
-    vx = (vx + ax * dt / 2.) / S.c1p;
-    vy = (vy + ay * dt / 2.) / S.c1p;
-    vz = (vz + az * dt / 2.) / S.c1p;
-    S.vx[i] = vx;
-    S.vy[i] = vy;
-    S.vz[i] = vz;
-    if (mode & SR_DRIFT) {  // the next step's first half kick and drift, into the other buffer
-      double4 p = pi;
-      const double dr = drift(S, p, half_kick(S, ax, vx), half_kick(S, ay, vy), half_kick(S, az, vz));
-      S.pos_nx[i] = p;
-      if (S.pos32_nx) S.pos32_nx[i] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
-      d1 = dr;
-      if (p.w != 0) d2 = dr;
-    }
-  }
+  return wz;
 }
 
+// Three standard normals of particle i at `step`: one Philox4x32-10 block,
+// counter (i, step, tag, 0), key = seed; words mapped to (0,1) as
+// (w + 0.5) / 2^32; Box-Muller, both branches of the first pair and the
+// cosine branch of the second (the reference's gauss_dist,
+// glob_var.f90:108-121, draws from an OS-seeded stream instead).
+__device__ __forceinline__ void noise_normals(const Rep &S, int i, int step, double &n0, double &n1,
+                                              double &n2) {
+  uint32_t w[4];
+  philox4x32_10((uint32_t)i, (uint32_t)step, kNoiseTag, 0u, (uint32_t)S.seed,
+                (uint32_t)(S.seed >> 32), w);
+  const double s32 = 1.0 / 4294967296.0;
+  const double u0 = ((double)w[0] + 0.5) * s32, u1 = ((double)w[1] + 0.5) * s32;
+  const double u2 = ((double)w[2] + 0.5) * s32, u3 = ((double)w[3] + 0.5) * s32;
+  const double r1 = sqrt(-2 * log(u0)), r2 = sqrt(-2 * log(u2));
+  double sn, cs;
+  sincospi(2 * u1, &sn, &cs);
+  n0 = r1 * cs;
+  n1 = r1 * sn;
+  n2 = r2 * cospi(2 * u3);
+}
 
 // Compute_Force without the Coulomb evaluation (compt_acc.f90:847-861):
 // held Coulomb force + lj_force (compt_acc.f90:895-977) + fene_force
 // (compt_acc.f90:865-892) + noise + field, anchors zeroed; optionally the
-// second half kick of new_position (init_update.f90:606) and the dr_max
-// bookkeeping of new_position / update_verlet_list
-// (init_update.f90:600-601, compt_acc.f90:717-724).  A workgroup takes
-// kSRBlock / kSRLanes particles: first kSRLanes lanes per particle share its
-// LJ list (lane g takes slots g, g + kSRLanes, ..., summed in a fixed tree),
-// then one lane per particle adds the rest and stores.
+// second half kick of new_position (init_update.f90:606), for replayed
+// steps (SR_DRIFT) the next step's half kick and drift into the other
+// position buffer (init_update.f90:581-601, 631-642), and the dr_max
+// bookkeeping of new_position / update_verlet_list (init_update.f90:600-601,
+// compt_acc.f90:717-724).  A workgroup takes 64 particles and gives its
+// wavefronts the independent parts, so that their memory latencies overlap:
+// wavefronts 0-3 sum the LJ lists (kSRLanes lanes per particle, lane g
+// takes slots g, g + kSRLanes, ..., a fixed tree over the lanes), wavefront
+// 4 draws the noise, 5 the wall and FENE forces (bond order), 6 loads the
+// particle's state and then adds everything up, kicks, drifts and stores;
+// wavefront 7 takes the rebuild decision of the replica's first workgroup.
 template <typename R>
 __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict__ reps, WMap wm,
                                                           int mode, int step_arg,
@@ -322,60 +243,42 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
                                                           double *__restrict__ out,
                                                           unsigned long long *lq, int lq_cap,
                                                           unsigned long long *lq_nx) {
-  constexpr int G = kSRLanes, NP = kSRBlock / kSRLanes;
+  constexpr int G = kSRLanes, NP = kSRParticles;
+  static_assert(NP == 64 && G * NP == 256 && kSRBlock == 512,
+                "LJ on waves 0-3, one particle per lane on the others; groups of 64 for dmax_group");
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
   const int blk = blockIdx.x - wm.blk[r];
   const int step = step_arg >= 0 ? step_arg : S.ctrl->step;
-  // the next step's rebuild queue (its list kernels of two steps back are done)
-  if ((mode & SR_DRIFT) && blockIdx.x == 0 && threadIdx.x == 0) *lq_nx = 0ull;
-  if ((mode & SR_DRMAX) && blk == 0 && threadIdx.x < 64) {  // one wavefront
-    double d1 = 0, d2 = 0;
-    const double2 *dp = S.dpart + (size_t)S.par * S.ngrp;
-    for (int g = threadIdx.x; g < S.ngrp; g += 64) {
-      const double2 v = dp[g];
-      d1 = fmax(d1, v.x);
-      d2 = fmax(d2, v.y);
-    }
-    d1 = wave_max(d1);
-    d2 = wave_max(d2);
-    if (threadIdx.x == 0) {
-      Ctrl *c = S.ctrl;
-      c->dr_max1 = c->dr_max1 + d1;
-      c->dr_max2 = c->dr_max2 + d2;
-      int f1 = c->dr_max1 > S.skin1;
-      int f2 = (c->dr_max2 > S.skin2) && S.real_verlet != 0;
-      if (f1) {
-        c->dr_max1 = 0;
-        c->gen1++;
-        lq_push(lq, lq_cap, r, S.NN);
-      }
-      if (f2) { c->dr_max2 = 0; c->gen2++; }
-      c->flag1 = f1;
-      c->flag2 = f2;
-    }
-  }
-  __shared__ R lj[3][NP];
-  {  // WCA-LJ over the full list: class by |q_i|+|q_j| (compt_acc.f90:922-952)
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = blk * NP + lane;  // the particle of this lane on waves 4..7
+  const bool live = i < S.NN;
+  __shared__ R lj[3][NP], fb[3][NP];
+  __shared__ double wzs[NP], nzs[3][NP];
+  // wave 6's registers across the barrier
+  double4 pi = make_double4(0, 0, 0, 0);
+  double vx = 0, vy = 0, vz = 0, ax0 = 0, ay0 = 0, az0 = 0, cx = 0, cy = 0, cz = 0;
+  int anchor = 0;
+  if (w < 4) {  // WCA-LJ over the full list: class by |q_i|+|q_j| (compt_acc.f90:922-952)
     const int g = threadIdx.x & (G - 1), pl = threadIdx.x / G;
-    const int i = blk * NP + pl;
-    const bool live = i < S.NN;
-    const int n = live ? min(S.nbn1[i], S.nbcap1) : 0;
+    const int ip = blk * NP + pl;
+    const bool lv = ip < S.NN;
+    const int n = lv ? min(S.nbn1[ip], S.nbcap1) : 0;
     const R hLx = (R)S.hLx, hLy = (R)S.hLy, Lx = (R)S.Lx, Ly = (R)S.Ly;
     const R rcl2 = (R)S.rcl2, rcl3 = (R)S.rcl3, rcl4 = (R)S.rcl4;
     const R sig2 = (R)S.sigma2, sig3 = (R)S.sigma3;
     R xi = 0, yi = 0, zi = 0, qa = 0;
-    if (live) {
+    if (lv) {
       if constexpr (sizeof(R) == 4) {
-        float4 p = S.pos32[i];
+        float4 p = S.pos32[ip];
         xi = p.x; yi = p.y; zi = p.z; qa = fabsf(p.w);
       } else {
-        double4 p = S.pos[i];
+        double4 p = S.pos[ip];
         xi = p.x; yi = p.y; zi = p.z; qa = fabs(p.w);
       }
     }
     R fx = 0, fy = 0, fz = 0;
-    const int *nb = S.nb1 + i;
+    const int *nb = S.nb1 + ip;
     for (int k = g; k < n; k += G) {
       const int j = nb[(size_t)k * S.NN];
       R xj, yj, zj, qj;
@@ -414,15 +317,154 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
       lj[1][pl] = fy * (R)48;
       lj[2][pl] = fz * (R)48;
     }
+  } else if (w == 4) {  // noise
+    if (live && (mode & (SR_NOISE_ETA | SR_NOISE_PHILOX))) {
+      double n0, n1, n2;
+      if (mode & SR_NOISE_ETA) {
+        n0 = eta[i];
+        n1 = eta[(size_t)S.NN + i];
+        n2 = eta[2 * (size_t)S.NN + i];
+      } else {
+        noise_normals(S, i, step, n0, n1, n2);
+      }
+      nzs[0][lane] = n0;
+      nzs[1][lane] = n1;
+      nzs[2][lane] = n2;
+    }
+  } else if (w == 5) {  // wall and FENE over the bonded partners of i
+    if (live) {
+      const double4 p = S.pos[i];
+      const int b0 = S.bstart[i], b1 = S.bstart[i + 1];
+      const R R02 = (R)S.R0_2, kR = (R)S.kR02;
+      R fx = 0, fy = 0, fz = 0;
+      for (int b = b0; b < b1; b++) {
+        const int nn = S.bpart[b];
+        double dxd, dyd, dzd;
+        const double rs = rsq_image(p, S.pos[nn], S.Lx, S.Ly, S.hLx, S.hLy, dxd, dyd, dzd);
+        if (rs > S.R0_2) {
+          set_error(S.ctrl, MD_EBOND, i < nn ? i : nn, step);
+          continue;
+        }
+        const R gg = -kR / (R02 - (R)rs);
+        fx += gg * (R)dxd;
+        fy += gg * (R)dyd;
+        fz += gg * (R)dzd;
+      }
+      fb[0][lane] = fx;
+      fb[1][lane] = fy;
+      fb[2][lane] = fz;
+      wzs[lane] = wall_force(S, p);
+    }
+  } else if (w == 6) {  // the particle's state, for after the barrier
+    if (live) {
+      pi = S.pos[i];
+      anchor = S.flags[i] & 1;
+      if (mode & SR_HELD) {
+        if constexpr (sizeof(R) == 4) {
+          cx = ((const float *)S.ccx)[i];
+          cy = ((const float *)S.ccy)[i];
+          cz = ((const float *)S.ccz)[i];
+        } else {
+          cx = ((const double *)S.ccx)[i];
+          cy = ((const double *)S.ccy)[i];
+          cz = ((const double *)S.ccz)[i];
+        }
+      }
+      if (mode & SR_KICK) {
+        vx = S.vx[i];
+        vy = S.vy[i];
+        vz = S.vz[i];
+        if (mode & SR_DRIFT) {
+          ax0 = S.ax[i];
+          ay0 = S.ay[i];
+          az0 = S.az[i];
+        }
+      }
+    }
+  } else {  // wave 7
+    // the next step's rebuild queue (its list kernels of two steps back are done)
+    if ((mode & SR_DRIFT) && blockIdx.x == 0 && lane == 0) *lq_nx = 0ull;
+    if ((mode & SR_DRMAX) && blk == 0) {  // this step's displacements, all groups
+      double d1 = 0, d2 = 0;
+      const double2 *dp = S.dpart + (size_t)S.par * S.ngrp;
+      for (int g = lane; g < S.ngrp; g += 64) {
+        const double2 v = dp[g];
+        d1 = fmax(d1, v.x);
+        d2 = fmax(d2, v.y);
+      }
+      d1 = wave_max(d1);
+      d2 = wave_max(d2);
+      if (lane == 0) {
+        Ctrl *c = S.ctrl;
+        c->dr_max1 = c->dr_max1 + d1;
+        c->dr_max2 = c->dr_max2 + d2;
+        int f1 = c->dr_max1 > S.skin1;
+        int f2 = (c->dr_max2 > S.skin2) && S.real_verlet != 0;
+        if (f1) {
+          c->dr_max1 = 0;
+          c->gen1++;
+          lq_push(lq, lq_cap, r, S.NN);
+        }
+        if (f2) { c->dr_max2 = 0; c->gen2++; }
+        c->flag1 = f1;
+        c->flag2 = f2;
+      }
+    }
   }
   __syncthreads();
-  static_assert(NP == 64, "the per-particle part runs on one wavefront");
-  if (threadIdx.x >= NP) return;
-  const int i = blk * NP + threadIdx.x;
+  if (w != 6) return;
   double d1 = 0, d2 = 0;
-  if (i < S.NN)
-    sr_particle<R>(S, i, mode, step, eta, out, lj[0][threadIdx.x], lj[1][threadIdx.x],
-                   lj[2][threadIdx.x], d1, d2);
+  if (live) {
+    const R fx = lj[0][lane] + fb[0][lane], fy = lj[1][lane] + fb[1][lane],
+            fz = lj[2][lane] + fb[2][lane];
+    double ax = (double)fx, ay = (double)fy, az = (double)fz + wzs[lane];
+    if (mode & SR_HELD) {
+      ax += cx;
+      ay += cy;
+      az += cz;
+    }
+    if (mode & (SR_NOISE_ETA | SR_NOISE_PHILOX)) {
+      ax += nzs[0][lane] * S.sig;
+      ay += nzs[1][lane] * S.sig;
+      az += nzs[2][lane] * S.sig;
+    }
+    if (mode & SR_FIELD) az += pi.w * S.EF;
+    if ((mode & SR_ANCHOR) && anchor) ax = ay = az = 0;
+    if (!(isfinite(ax) && isfinite(ay) && isfinite(az))) set_error(S.ctrl, MD_ENONFINITE, i, step);
+    if (out) {
+      out[i] = ax;
+      out[(size_t)S.NN + i] = ay;
+      out[2 * (size_t)S.NN + i] = az;
+    } else {
+      S.ax[i] = ax;
+      S.ay[i] = ay;
+      S.az[i] = az;
+      if (mode & SR_KICK) {
+#pragma clang fp contract(off)
+        if (mode & SR_DRIFT) {  // v holds v(n-1): redo this step's first half kick
+          vx = half_kick(S, ax0, vx);
+          vy = half_kick(S, ay0, vy);
+          vz = half_kick(S, az0, vz);
+        }
+        const double dt = S.dt;
+        vx = (vx + ax * dt / 2.) / S.c1p;
+        vy = (vy + ay * dt / 2.) / S.c1p;
+        vz = (vz + az * dt / 2.) / S.c1p;
+        S.vx[i] = vx;
+        S.vy[i] = vy;
+        S.vz[i] = vz;
+        if (mode & SR_DRIFT) {  // the next step's first half kick and drift, into the other buffer
+          double4 p = pi;
+          const double dr =
+              drift(S, p, half_kick(S, ax, vx), half_kick(S, ay, vy), half_kick(S, az, vz));
+          S.pos_nx[i] = p;
+          if (S.pos32_nx) S.pos32_nx[i] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
+          d1 = dr;
+          if (p.w != 0) d2 = dr;
+        }
+      }
+    }
+  }
   if (mode & SR_DRIFT) dmax_group(S, S.par ^ 1, blk, d1, d2);
 }
 
@@ -1584,16 +1626,28 @@ __device__ __forceinline__ void wave_sync() {
 template <typename R> struct RealTile { static constexpr int N = 256; };
 template <> struct RealTile<float> { static constexpr int N = 512; };
 
+// Whether the 27 stencil cells of cell (a, b) are all direct x,y neighbours.
+__device__ __forceinline__ bool interior_cell(int a, int b, const int nc[3]) {
+  return nc[0] >= 3 && nc[1] >= 3 && a >= 1 && a <= nc[0] - 2 && b >= 1 && b <= nc[1] - 2;
+}
+
 // Force on one target from a staged tile of nc candidates (candidate `self`
 // is the target itself), by one wavefront: the candidates inside r_c are
 // compacted in candidate order into hit[], the lanes stride over them, and
-// a fixed shuffle tree sums the lanes.  Both forms of the sweep use it with
+// a fixed shuffle tree sums the lanes.  wrap = false for a cell whose 27
+// stencil cells are all direct neighbours in x and y (with three or more
+// cells along each, edge >= r_c): no in-range pair then needs the minimum
+// image (|dx| < r_c <= L/3), and no out-of-range pair comes within r_c
+// through it (|dx| <= 2 L/3), so the compaction test skips it.  Both forms of the sweep use it with
 // the same tiles, so a target's sum does not depend on the form.
 template <typename R, typename V>
-__device__ __forceinline__ void target_tile(const V pi, const V *cand, int nc, int self, short *hit,
-                                            R Lx, R Ly, R hLx, R hLy, R rcc2, R alpha, R alpha2,
-                                            R c2, R &fx, R &fy, R &fz) {
+__device__ __forceinline__ void target_tile(const V pi, const V *cand, bool wrap, int nc, int self,
+                                            short *hit, R Lx, R Ly, R hLx, R hLy, R rcc2, R alpha,
+                                            R alpha2, R c2, R &fx, R &fy, R &fz) {
   const int lane = threadIdx.x & 63;
+  // the compaction keeps a superset of the pairs within r_c (its radius is
+  // a few roundings wider); real_pair tests each exactly
+  const R rin2 = rcc2 * (R)(sizeof(R) == 4 ? 1.00001 : 1.000000000001);
   int nh = 0;
   for (int k0 = 0; k0 < nc; k0 += 64) {
     const int k = k0 + lane;
@@ -1602,9 +1656,11 @@ __device__ __forceinline__ void target_tile(const V pi, const V *cand, int nc, i
       const V pj = cand[k];
       R dx = pi.x - pj.x, dy = pi.y - pj.y;
       const R dz = pi.z - pj.z;
-      if (dx > hLx) dx -= Lx; else if (dx <= -hLx) dx += Lx;
-      if (dy > hLy) dy -= Ly; else if (dy <= -hLy) dy += Ly;
-      in = dx * dx + dy * dy + dz * dz < rcc2;
+      if (wrap) {
+        if (dx > hLx) dx -= Lx; else if (dx <= -hLx) dx += Lx;
+        if (dy > hLy) dy -= Ly; else if (dy <= -hLy) dy += Ly;
+      }
+      in = dx * dx + dy * dy + dz * dz < rin2;
     }
     const unsigned long long mk = __ballot(in);
     if (in) hit[nh + __popcll(mk & ((1ull << lane) - 1ull))] = (short)k;
@@ -1668,6 +1724,7 @@ __device__ void real_cells_wg(const Rep &S, int b0, int nb, V *cand, short *hit_
       const int it = work[q];
       const int cell = it / nch, t0 = (it - cell * nch) * kRealChunkWG;
       const int nt = min(S.scnt3[cell] - t0, kRealChunkWG);
+      const bool wrap = !interior_cell(cell % S.nc3[0], (cell / S.nc3[0]) % S.nc3[1], S.nc3);
       const int total = stencil_offsets(cell, S.nc3, S.scnt3, offs, cids);
       // stencil slot 13 is the cell itself: candidate offs[13] + t0 + tg is target tg
       const int self0 = offs[13] + t0;
@@ -1682,7 +1739,7 @@ __device__ void real_cells_wg(const Rep &S, int b0, int nb, V *cand, short *hit_
         __syncthreads();
         for (int tg = w; tg < nt; tg += NW) {
           R fx, fy, fz;
-          target_tile<R>(tp[tg], cand, nc, self0 + tg - base, hit, Lx, Ly, hLx, hLy, rcc2, alpha,
+          target_tile<R>(tp[tg], cand, wrap, nc, self0 + tg - base, hit, Lx, Ly, hLx, hLy, rcc2, alpha,
                          alpha2, c2, fx, fy, fz);
           if (lane == 0) {
             accs[tg][0] += (double)fx;
@@ -1753,6 +1810,7 @@ __global__ __launch_bounds__(kBlock) void k_real_cells(const Rep *__restrict__ r
       const int nt = min(S.scnt3[cell] - t0, kRealChunk);
       const int ca = cell % S.nc3[0], cb = (cell / S.nc3[0]) % S.nc3[1],
                 cc = cell / (S.nc3[0] * S.nc3[1]);
+      const bool wrap = !interior_cell(ca, cb, S.nc3);
       // stencil cell of lane l < 27, its count and exclusive offset
       int cl = -1, cn = 0;
       if (lane < 27) {
@@ -1801,7 +1859,7 @@ __global__ __launch_bounds__(kBlock) void k_real_cells(const Rep *__restrict__ r
           pi.z = __shfl(pt.z, tg, 64);
           pi.w = __shfl(pt.w, tg, 64);
           R fx, fy, fz;
-          target_tile<R>(pi, cand, nc, self0 + tg - base, hit, Lx, Ly, hLx, hLy, rcc2, alpha,
+          target_tile<R>(pi, cand, wrap, nc, self0 + tg - base, hit, Lx, Ly, hLx, hLy, rcc2, alpha,
                          alpha2, c2, fx, fy, fz);
           if (lane == tg) {
             ax += (double)fx;

@@ -43,7 +43,10 @@ def kernel_bytes(p, name):
     if name == "kick_drift":
         return NN * (24 + 48 + 64 + (16 if f32 else 0))
     if name == "real":
-        return Nq * (4 + 32 + (16 if f32 else 0) + 24 + 4 + 3 * r)
+        # sweep: cell-sorted position (4r), member index, pair sums written
+        # (3r); store: reciprocal force 24, pair sums 3r, charge index 4,
+        # position 32, held force written 3r
+        return Nq * (4 * r + 4 + 3 * r + 24 + 3 * r + 4 + 32 + 3 * r)
     if name == "spread":
         # bin members, charge index, position of each charge; mesh written once
         return Nq * (4 + 4 + 32) + r * M
@@ -134,7 +137,8 @@ KERNEL_SYMBOLS = {  # kernel names of time_kernels -> device symbols (FP32 path)
     "kick_drift": "k_kick_drift", "short_range": "k_short_range<float>", "bin_q": "k_bin_q",
     "sort_q": "k_sort_q<float>", "spread": "k_spread<float>", "fft_x_fwd": "k_plane_fwd<float>",
     "fft_z_conv": "k_fftz_conv<float>", "fft_x_inv": "k_plane_inv<float>", "gather": "k_gather<float>",
-    "real": "k_real_cells<float>", "bin_lj": "k_bin_lj", "sort_lj": "k_sort_lj", "build_lj": "k_build_lj"}
+    "real": ("k_real_cells<float>", "k_coul_store<float>"), "bin_lj": "k_bin_lj", "sort_lj": "k_sort_lj",
+    "build_lj": "k_build_lj"}
 
 
 def _counter_rows(path, counter):
@@ -161,13 +165,24 @@ def traffic_summary(fetch_csv, write_csv):
     sf, sw = COPY_BYTES / (fcopy * 1024.0), COPY_BYTES / (wcopy * 1024.0)
     out = {"copy_bytes": COPY_BYTES, "copy_fetch_kib": fcopy, "copy_write_kib": wcopy,
            "fetch_scale": sf, "write_scale": sw, "kernels": {}}
-    for name, sym in KERNEL_SYMBOLS.items():
-        fv = [v for n, v in fetch if sym in n]
-        wv = [v for n, v in write if sym in n]
-        if not fv or not wv:
+    for name, syms in KERNEL_SYMBOLS.items():
+        # a timed step may be several kernels (the sweep and the Coulomb store):
+        # the sum of their per-dispatch means
+        syms = (syms,) if isinstance(syms, str) else syms
+        fk = wk = 0.0
+        nd = None
+        for sym in syms:
+            fv = [v for n, v in fetch if sym in n]
+            wv = [v for n, v in write if sym in n]
+            if not fv or not wv:
+                nd = 0
+                break
+            fk += sum(fv) / len(fv)
+            wk += sum(wv) / len(wv)
+            nd = len(fv) if nd is None else min(nd, len(fv))
+        if not nd:
             continue
-        fk, wk = sum(fv) / len(fv), sum(wv) / len(wv)
-        out["kernels"][name] = {"dispatches": len(fv), "fetch_kib": round(fk, 1), "write_kib": round(wk, 1),
+        out["kernels"][name] = {"dispatches": nd, "fetch_kib": round(fk, 1), "write_kib": round(wk, 1),
                                 "bytes_per_launch": int(fk * 1024 * sf + wk * 1024 * sw)}
     path = os.path.join(ROOT, "profiles", "traffic_default_brush.json")
     with open(path, "w") as f:

@@ -134,12 +134,14 @@ def replica(chains_side, n_beads, arch, valence, ez, seed, precision=MD_FP32, pr
         nma = max(2, n_beads // arch)
         _apply(p, [("Nga", n), ("Ngl", 0), ("arm", arch), ("Nma", nma)])
     _apply(p, [("man_l", 1), ("man_s", 1), ("qqi", float(valence)), ("ion_ratio", 1.0), ("EF", ez)])
-    # mesh spacing ~1.7 sigma, rounded up to powers of two
+    # mesh: the smallest power of two with a spacing of at most 1.8 sigma,
+    # which keeps the default brush's spacing (113.1/64 = 1.77 sigma in x, y;
+    # 426.3/256 = 1.67 in z) for boxes of its size
     q = derive(p)
     ng = []
     for L in (q.Lx, q.Ly, q.Lz * q.Zfac):
         k = 8
-        while L / k > 1.75:
+        while L / k > 1.8:
             k *= 2
         ng.append(k)
     p.ovr_ng[0], p.ovr_ng[1], p.ovr_ng[2] = ng

@@ -672,6 +672,7 @@ struct md_engine {
   int tune_real = -1;              // real-space sweep: 0 wavefront, 1 workgroup per item
   int tune_pz = 0;                 // z-pass lines per workgroup
   int tune_fuse = -1;              // fused FFT planes where they fit: 0 no, 1 yes
+  int tune_ride = -1;              // real-space sweep inside the spreading / gather launches
   std::string err;
 
   ~md_engine() {
@@ -770,6 +771,7 @@ int md_engine::rebuild() {
     x->h.real_wg = wg;
     x->nb[10] = wg ? x->nb10_wg : x->nb10_wave;
   }
+  const bool reps_host_wg = wg != 0;  // one form for every replica of the engine
   // z pass: few lines per workgroup (32-byte row pieces) while the engine's
   // mesh is small, so that the launch fills the chip; whole 128-byte pieces
   // when the engine holds many mesh points.  The lines' arithmetic is the
@@ -864,6 +866,11 @@ int md_engine::rebuild() {
     for (int k = 0; k < kMaps; k++) *w[k] = WMap{maps + k * (nrep + 1), nrep, totals[k]};
     P.vpart = vpart;
     P.fp32 = fp32;
+    P.real_wg = nrep > 0 && reps_host_wg;
+    // the sweep inside the spreading and gather launches pays where it is
+    // latency-bound (one system), not in batches (MD_RIDE=0 / =1 forces)
+    const char *rd = std::getenv("MD_RIDE");
+    P.ride = rd ? rd[0] != '0' : (tune_ride >= 0 ? tune_ride != 0 : !batch);
     // MD_TWO_STREAMS=1: the real-space sweep beside the SPME kernels on a
     // second stream (the same results; slower in graph replay at the
     // default brush, so off unless asked for)
@@ -1483,8 +1490,9 @@ extern "C" int md_time_kernels(md_engine *e, int32_t iters, double *ms, int32_t 
   return rc;
 }
 
-// Chooses the launch forms (real-space sweep form, z-pass lines per
-// workgroup, fused FFT planes) by timing one Coulomb evaluation of the
+// Chooses the launch forms (real-space sweep form and whether it shares the
+// spreading and gather launches, z-pass lines per workgroup, fused FFT
+// planes) by timing one Coulomb evaluation of the
 // engine's replicas under each: the forms give the same results, so only
 // the time differs.  Knobs are tuned one after the other, `iters`
 // evaluations each.
@@ -1522,7 +1530,8 @@ extern "C" int md_autotune(md_engine *e, int32_t iters, int32_t *choice) {
     int *field;
     std::vector<int> options;
   };
-  Knob knobs[] = {{&e->tune_fuse, {0, 1}}, {&e->tune_pz, {2, 4, 8, 16}}, {&e->tune_real, {0, 1}}};
+  Knob knobs[] = {{&e->tune_fuse, {0, 1}}, {&e->tune_pz, {2, 4, 8, 16}}, {&e->tune_real, {0, 1}},
+                  {&e->tune_ride, {0, 1}}};
   for (Knob &k : knobs) {
     int best = *k.field;
     double best_ms = 1e30;
@@ -1547,6 +1556,7 @@ extern "C" int md_autotune(md_engine *e, int32_t iters, int32_t *choice) {
     choice[0] = e->tune_fuse;
     choice[1] = e->tune_pz;
     choice[2] = e->tune_real;
+    choice[3] = e->tune_ride;
   }
   return rc;
 }

@@ -38,6 +38,8 @@ struct Plan {
   int grid_bin = 1, grid_build = 1;
   double *vpart = nullptr;  // partial sums of rescale_velocity
   bool fp32 = false;
+  bool real_wg = false;     // real-space sweep form of the engine (workgroup per item)
+  bool ride = true;         // sweep workgroups inside the spreading and gather launches
   // second stream of a Coulomb evaluation (the real-space sweep runs there,
   // beside the SPME kernels) and its fork / join events; null: one stream
   hipStream_t st2 = nullptr;

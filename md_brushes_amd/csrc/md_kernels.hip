@@ -917,17 +917,17 @@ __device__ __forceinline__ void tile_add(unsigned long long *acc, double4 p, con
 // the tile) in LDS, in 2^-40 fixed point, so the sums do not depend on the
 // order of the additions; every mesh point is then written once.
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_spread(const Rep *__restrict__ reps, WMap wm) {
-  const int r = find_rep(wm, blockIdx.x);
+__device__ __forceinline__ void spread_block(const Rep *__restrict__ reps, WMap wm, int gb,
+                                             unsigned long long *acc) {
+  const int r = find_rep(wm, gb);
   const Rep &S = reps[r];
-  const int tile = blockIdx.x - wm.blk[r];
+  const int tile = gb - wm.blk[r];
   const int o = S.order;
   const int E[3] = {S.tsz[0], S.tsz[1], S.tsz[2]};
   const int K[3] = {S.K[0], S.K[1], S.K[2]};
   const int x0 = (tile % S.ntl[0]) * E[0], y0 = ((tile / S.ntl[0]) % S.ntl[1]) * E[1],
             z0 = tile / (S.ntl[0] * S.ntl[1]) * E[2];
   const int npt = E[0] * E[1] * E[2];
-  __shared__ unsigned long long acc[kTile * kTile * kTile];
   for (int k = threadIdx.x; k < npt; k += blockDim.x) acc[k] = 0ull;
   __syncthreads();
   // the tile's sub-lists, concatenated
@@ -962,6 +962,12 @@ __global__ __launch_bounds__(kBlock) void k_spread(const Rep *__restrict__ reps,
     q[((size_t)(z0 + lz) * K[1] + y0 + ly) * K[0] + x0 + lx] =
         (R)((double)(long long)acc[k] * kQInv);
   }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_spread(const Rep *__restrict__ reps, WMap wm) {
+  __shared__ unsigned long long acc[kTile * kTile * kTile];
+  spread_block<R>(reps, wm, blockIdx.x, acc);
 }
 
 // LDS FFTs over `nl` lines of length n = 2^ln stored line-major: Stockham
@@ -1370,10 +1376,10 @@ __device__ __forceinline__ void gather_one(const Rep &S, double4 p, R &f1, R &f2
 // sums of the force (for ZeroForce, compt_acc.f90:1470-1492) and of
 // q z lb/Beta (the slab dipole M_z, compt_acc.f90:1041-1044), in double.
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_gather(const Rep *__restrict__ reps, WMap wm) {
-  const int r = find_rep(wm, blockIdx.x);
+__device__ __forceinline__ void gather_block(const Rep *__restrict__ reps, WMap wm, int gb) {
+  const int r = find_rep(wm, gb);
   const Rep &S = reps[r];
-  const int blk = blockIdx.x - wm.blk[r];
+  const int blk = gb - wm.blk[r];
   const int m = blk * kBlock + threadIdx.x;
   const bool live = m < S.Nq;
   {  // the tile lists are consumed: clear them for the next evaluation
@@ -1412,6 +1418,11 @@ __global__ __launch_bounds__(kBlock) void k_gather(const Rep *__restrict__ reps,
     for (int w = 0; w < kBlock / 64; w++) s += red[threadIdx.x][w];
     S.part[(size_t)blk * 4 + threadIdx.x] = s;
   }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_gather(const Rep *__restrict__ reps, WMap wm) {
+  gather_block<R>(reps, wm, blockIdx.x);
 }
 
 // acc_c of one charge: the reciprocal force with its mean removed, scaled by
@@ -1649,22 +1660,32 @@ __device__ __forceinline__ void target_tile(const V pi, const V *cand, bool wrap
   // a few roundings wider); real_pair tests each exactly
   const R rin2 = rcc2 * (R)(sizeof(R) == 4 ? 1.00001 : 1.000000000001);
   int nh = 0;
-  for (int k0 = 0; k0 < nc; k0 += 64) {
-    const int k = k0 + lane;
-    bool in = false;
-    if (k < nc && k != self) {
-      const V pj = cand[k];
-      R dx = pi.x - pj.x, dy = pi.y - pj.y;
-      const R dz = pi.z - pj.z;
+  // four rounds of 64 candidates at a time: their LDS loads are issued
+  // together; the hits are appended round by round, in candidate order
+  constexpr int U = 4;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int k0 = 0; k0 < nc; k0 += 64 * U) {
+    V pj[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) pj[u] = cand[min(k0 + u * 64 + lane, nc - 1)];
+    bool in[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int k = k0 + u * 64 + lane;
+      R dx = pi.x - pj[u].x, dy = pi.y - pj[u].y;
+      const R dz = pi.z - pj[u].z;
       if (wrap) {
         if (dx > hLx) dx -= Lx; else if (dx <= -hLx) dx += Lx;
         if (dy > hLy) dy -= Ly; else if (dy <= -hLy) dy += Ly;
       }
-      in = dx * dx + dy * dy + dz * dz < rin2;
+      in[u] = k < nc && k != self && dx * dx + dy * dy + dz * dz < rin2;
     }
-    const unsigned long long mk = __ballot(in);
-    if (in) hit[nh + __popcll(mk & ((1ull << lane) - 1ull))] = (short)k;
-    nh += __popcll(mk);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const unsigned long long mk = __ballot(in[u]);
+      if (in[u]) hit[nh + __popcll(mk & below)] = (short)(k0 + u * 64 + lane);
+      nh += __popcll(mk);
+    }
   }
   wave_sync();
   fx = fy = fz = 0;
@@ -1686,14 +1707,15 @@ __device__ __forceinline__ void target_tile(const V pi, const V *cand, bool wrap
 // stencil cells' members are staged once per tile of RealTile<R>::N
 // candidates by all kBlock lanes; the wavefronts share the item's targets
 // (target_tile each) and sum them over the tiles in LDS (double), in the
-// same order as the wavefront form.  Workgroups stride over the items and
-// skip the empty ones in bulk.
+// same order as the wavefront form.  Workgroups stride over the items
+// (chunk-major, as in the wavefront form) and skip the empty ones in bulk.
 template <typename R, typename V>
-__device__ void real_cells_wg(const Rep &S, int b0, int nb, V *cand, short *hit_all) {
+__device__ void real_cells_wg(const Rep &S, int b0, int nb, V *cand, short *hit_all, int part,
+                              int nparts) {
   constexpr int TN = RealTile<R>::N, NW = kBlock / 64;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nch = (S.ccap3 + kRealChunkWG - 1) / kRealChunkWG;
-  const int nitem = S.nc3[0] * S.nc3[1] * S.nc3[2] * nch;
+  const int ncl = S.nc3[0] * S.nc3[1] * S.nc3[2], nitem = ncl * nch;
   __shared__ int offs[28], cids[27];
   __shared__ int work[kBlock], wcnt[NW];
   __shared__ double accs[kRealChunkWG][3];
@@ -1701,14 +1723,15 @@ __device__ void real_cells_wg(const Rep &S, int b0, int nb, V *cand, short *hit_
   const V *ps = (const V *)S.pos3s;
   const R Lx = (R)S.Lx, Ly = (R)S.Ly, hLx = (R)S.hLx, hLy = (R)S.hLy, rcc2 = (R)S.rcc2;
   const R alpha = (R)S.alpha, alpha2 = (R)S.alpha2, c2 = (R)S.c2;
-  for (int it0 = b0; it0 < nitem; it0 += nb * kBlock) {
-    // the workgroup's next kBlock items (its items are b0, b0 + nb, ...):
-    // the non-empty ones, compacted in order
-    const int item = it0 + threadIdx.x * nb;
+  // the workgroup's items are b0 + j nb; this launch takes those with
+  // j = part mod nparts, kBlock at a time, the non-empty ones compacted in order
+  const int step = nb * nparts;
+  for (int it0 = b0 + part * nb; it0 < nitem; it0 += step * kBlock) {
+    const int item = it0 + threadIdx.x * step;
     bool ne = false;
     if (item < nitem) {
-      const int cell = item / nch;
-      ne = S.scnt3[cell] > (item - cell * nch) * kRealChunkWG;
+      const int cell = item % ncl;
+      ne = S.scnt3[cell] > (item / ncl) * kRealChunkWG;
     }
     const unsigned long long bal = __ballot(ne);
     if (lane == 0) wcnt[w] = __popcll(bal);
@@ -1722,7 +1745,7 @@ __device__ void real_cells_wg(const Rep &S, int b0, int nb, V *cand, short *hit_
     __syncthreads();
     for (int q = 0; q < nwork; q++) {
       const int it = work[q];
-      const int cell = it / nch, t0 = (it - cell * nch) * kRealChunkWG;
+      const int cell = it % ncl, t0 = (it / ncl) * kRealChunkWG;
       const int nt = min(S.scnt3[cell] - t0, kRealChunkWG);
       const bool wrap = !interior_cell(cell % S.nc3[0], (cell / S.nc3[0]) % S.nc3[1], S.nc3);
       const int total = stencil_offsets(cell, S.nc3, S.scnt3, offs, cids);
@@ -1770,43 +1793,55 @@ __device__ void real_cells_wg(const Rep &S, int b0, int nb, V *cand, short *hit_
 // sum over the tiles and the lanes store their targets together.
 // Wavefronts stride over the items and skip the empty ones 64 at a time.
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_real_cells(const Rep *__restrict__ reps, WMap wm) {
+struct RealLDS {  // LDS of a real-space workgroup: candidate tiles, then hit lists
   using V = typename std::conditional<sizeof(R) == 4, float4, double4>::type;
+  static constexpr int NW = kBlock / 64, TN = RealTile<R>::N;
+  static constexpr size_t bytes = (size_t)NW * TN * (sizeof(V) + sizeof(short));
+};
+
+// Real-space workgroup gb of the sweep's block map (its LDS at lds).
+template <typename R, bool WG>
+__device__ __forceinline__ void real_cells_block(const Rep *__restrict__ reps, WMap wm, int gb,
+                                                 char *lds, int part = 0, int nparts = 1) {
+  using V = typename RealLDS<R>::V;
   constexpr int TN = RealTile<R>::N, NW = kBlock / 64;
-  const int r = find_rep(wm, blockIdx.x);
+  const int r = find_rep(wm, gb);
   const Rep &S = reps[r];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int gw = (blockIdx.x - wm.blk[r]) * NW + w, nwv = (wm.blk[r + 1] - wm.blk[r]) * NW;
-  // item = cell * nch + chunk; wavefront gw takes items gw, gw + nwv, ...,
-  // so that the chunks of a full cell go to different wavefronts
+  const int gw = (gb - wm.blk[r]) * NW + w, nwv = (wm.blk[r + 1] - wm.blk[r]) * NW;
+  // item = chunk * ncell + cell (every cell's first chunk, then every
+  // cell's second, ...: the non-empty items are dense at the front, so that
+  // the strided assignment deals them out evenly); wavefront gw takes items
+  // gw, gw + nwv, ...
   const int nch = (S.ccap3 + kRealChunk - 1) / kRealChunk;
-  const int nitem = S.nc3[0] * S.nc3[1] * S.nc3[2] * nch;
-  __shared__ V cand_all[NW][TN];
-  __shared__ short hit_all[NW][TN];
-  V *cand = cand_all[w];
-  short *hit = hit_all[w];
+  const int ncl = S.nc3[0] * S.nc3[1] * S.nc3[2], nitem = ncl * nch;
+  V *cand_all = (V *)lds;
+  short *hit_all = (short *)(lds + (size_t)NW * TN * sizeof(V));
+  V *cand = cand_all + w * TN;
+  short *hit = hit_all + w * TN;
   const V *ps = (const V *)S.pos3s;
   const R Lx = (R)S.Lx, Ly = (R)S.Ly, hLx = (R)S.hLx, hLy = (R)S.hLy, rcc2 = (R)S.rcc2;
   const R alpha = (R)S.alpha, alpha2 = (R)S.alpha2, c2 = (R)S.c2;
-  if (S.real_wg) {
-    real_cells_wg<R>(S, blockIdx.x - wm.blk[r], wm.blk[r + 1] - wm.blk[r], &cand_all[0][0],
-                     &hit_all[0][0]);
+  if constexpr (WG) {
+    real_cells_wg<R>(S, gb - wm.blk[r], wm.blk[r + 1] - wm.blk[r], cand_all, hit_all, part, nparts);
     return;
   }
-  for (int it0 = gw; it0 < nitem; it0 += nwv * 64) {
+  // the wavefront's items are gw + j nwv; this launch takes j = part mod nparts
+  const int step = nwv * nparts;
+  for (int it0 = gw + part * nwv; it0 < nitem; it0 += step * 64) {
     // this wavefront's next 64 items
-    const int item = it0 + lane * nwv;
+    const int item = it0 + lane * step;
     bool ne = false;
     if (item < nitem) {
-      const int cell = item / nch;
-      ne = S.scnt3[cell] > (item - cell * nch) * kRealChunk;
+      const int cell = item % ncl;
+      ne = S.scnt3[cell] > (item / ncl) * kRealChunk;
     }
     unsigned long long pend = __ballot(ne);
     while (pend) {
       const int src = __ffsll((long long)pend) - 1;
       pend &= pend - 1;
       const int it = __shfl(item, src, 64);
-      const int cell = it / nch, t0 = (it - cell * nch) * kRealChunk;
+      const int cell = it % ncl, t0 = (it / ncl) * kRealChunk;
       const int nt = min(S.scnt3[cell] - t0, kRealChunk);
       const int ca = cell % S.nc3[0], cb = (cell / S.nc3[0]) % S.nc3[1],
                 cc = cell / (S.nc3[0] * S.nc3[1]);
@@ -1872,6 +1907,37 @@ __global__ __launch_bounds__(kBlock) void k_real_cells(const Rep *__restrict__ r
       if (lane < nt) store_real<R>(S, S.mem3[(size_t)cell * S.ccap3 + t0 + lane], (R)ax, (R)ay, (R)az);
     }
   }
+}
+
+template <typename R, bool WG>
+__global__ __launch_bounds__(kBlock) void k_real_cells(const Rep *__restrict__ reps, WMap wm) {
+  __shared__ __align__(16) char lds[RealLDS<R>::bytes];
+  real_cells_block<R, WG>(reps, wm, blockIdx.x, lds);
+}
+
+// The real-space sweep shares launches with the SPME kernels that leave the
+// chip idle: workgroups past the spreading tiles (or past the gather's
+// charge blocks) are the sweep's workgroups, each doing every other of its
+// items (part 0 with the spreading, part 1 with the gather; a workgroup's
+// duration is set by its own items, so each launch takes about half), in one
+// LDS buffer sized for either kind.  Every item is computed as in the
+// sweep's own launch, so the sums are the same.
+template <typename R, bool WG>
+__global__ __launch_bounds__(kBlock) void k_spread_real(const Rep *__restrict__ reps, WMap ws,
+                                                        WMap wr) {
+  constexpr size_t B = RealLDS<R>::bytes > kTile * kTile * kTile * 8 ? RealLDS<R>::bytes
+                                                                     : kTile * kTile * kTile * 8;
+  __shared__ __align__(16) char lds[B];
+  if ((int)blockIdx.x < ws.total) spread_block<R>(reps, ws, blockIdx.x, (unsigned long long *)lds);
+  else real_cells_block<R, WG>(reps, wr, (int)blockIdx.x - ws.total, lds, 0, 2);
+}
+
+template <typename R, bool WG>
+__global__ __launch_bounds__(kBlock) void k_gather_real(const Rep *__restrict__ reps, WMap wg,
+                                                        WMap wr) {
+  __shared__ __align__(16) char lds[RealLDS<R>::bytes];
+  if ((int)blockIdx.x < wg.total) gather_block<R>(reps, wg, blockIdx.x);
+  else real_cells_block<R, WG>(reps, wr, (int)blockIdx.x - wg.total, lds, 1, 2);
 }
 
 // ---------------------------------------------------- rescale_velocity
@@ -2039,13 +2105,29 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) 
   if (P.cell3.total)
     hipLaunchKernelGGL(k_sort_q<R>, dim3(P.cell3.total), dim3(kBlock), P.lds_sort3, st, P.reps, P.cell3);
   mark();
+  // the sweep's workgroups ride along with the spreading and gather launches
+  // (half of their items in each), unless its launches are timed or forked
+  const bool ride = P.ride && !ev && !fork && P.cell3.total;
+  const int nr = ride ? P.cell3c.total : 0;
   if (fork) {
     (void)hipEventRecord(P.ev_fork, st);
     (void)hipStreamWaitEvent(P.st2, P.ev_fork, 0);
-    hipLaunchKernelGGL(k_real_cells<R>, dim3(P.cell3c.total), dim3(kBlock), 0, P.st2, P.reps, P.cell3c);
+    if (P.real_wg)
+      hipLaunchKernelGGL((k_real_cells<R, true>), dim3(P.cell3c.total), dim3(kBlock), 0, P.st2, P.reps,
+                         P.cell3c);
+    else
+      hipLaunchKernelGGL((k_real_cells<R, false>), dim3(P.cell3c.total), dim3(kBlock), 0, P.st2, P.reps,
+                         P.cell3c);
     (void)hipEventRecord(P.ev_join, P.st2);
   }
-  hipLaunchKernelGGL(k_spread<R>, dim3(P.brick.total), dim3(kBlock), 0, st, P.reps, P.brick);
+  if (ride && P.real_wg)
+    hipLaunchKernelGGL((k_spread_real<R, true>), dim3(P.brick.total + nr), dim3(kBlock), 0, st, P.reps,
+                       P.brick, P.cell3c);
+  else if (ride)
+    hipLaunchKernelGGL((k_spread_real<R, false>), dim3(P.brick.total + nr), dim3(kBlock), 0, st, P.reps,
+                       P.brick, P.cell3c);
+  else
+    hipLaunchKernelGGL(k_spread<R>, dim3(P.brick.total), dim3(kBlock), 0, st, P.reps, P.brick);
   mark();
   // replicas with fused planes take k_plane_*, the others the x and y passes
   if (P.plane.total)
@@ -2068,11 +2150,24 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) 
   if (P.fx.total)
     hipLaunchKernelGGL(k_fftx_inv<R>, dim3(P.fx.total), dim3(kBlock), P.lds_x, st, P.reps, P.fx);
   mark();
-  hipLaunchKernelGGL(k_gather<R>, dim3(P.chgG.total), dim3(kBlock), 0, st, P.reps, P.chgG);
+  if (ride && P.real_wg)
+    hipLaunchKernelGGL((k_gather_real<R, true>), dim3(P.chgG.total + nr), dim3(kBlock), 0, st, P.reps,
+                       P.chgG, P.cell3c);
+  else if (ride)
+    hipLaunchKernelGGL((k_gather_real<R, false>), dim3(P.chgG.total + nr), dim3(kBlock), 0, st, P.reps,
+                       P.chgG, P.cell3c);
+  else
+    hipLaunchKernelGGL(k_gather<R>, dim3(P.chgG.total), dim3(kBlock), 0, st, P.reps, P.chgG);
   mark();
   if (fork) (void)hipStreamWaitEvent(st, P.ev_join, 0);
-  else if (P.cell3.total)
-    hipLaunchKernelGGL(k_real_cells<R>, dim3(P.cell3c.total), dim3(kBlock), 0, st, P.reps, P.cell3c);
+  else if (P.cell3.total && !ride) {
+    if (P.real_wg)
+      hipLaunchKernelGGL((k_real_cells<R, true>), dim3(P.cell3c.total), dim3(kBlock), 0, st, P.reps,
+                         P.cell3c);
+    else
+      hipLaunchKernelGGL((k_real_cells<R, false>), dim3(P.cell3c.total), dim3(kBlock), 0, st, P.reps,
+                         P.cell3c);
+  }
   hipLaunchKernelGGL(k_coul_store<R>, dim3(P.chgG.total), dim3(kBlock), 0, st, P.reps, P.chgG, out);
   mark();
 }

@@ -317,7 +317,7 @@ def test_launch_forms_do_not_change_results(prec, monkeypatch):
     p, pos, vel = system("default", prec)
 
     def coulomb(env):
-        for k in ("MD_FFT_PLANES", "MD_Z_LINES", "MD_REAL_SWEEP"):
+        for k in ("MD_FFT_PLANES", "MD_Z_LINES", "MD_REAL_SWEEP", "MD_RIDE"):
             monkeypatch.delenv(k, raising=False)
         for k, v in env.items():
             monkeypatch.setenv(k, v)
@@ -330,13 +330,15 @@ def test_launch_forms_do_not_change_results(prec, monkeypatch):
     e, ref = coulomb({})
     e.close()
     for env in ({"MD_FFT_PLANES": "0", "MD_Z_LINES": "2"}, {"MD_Z_LINES": "16", "MD_REAL_SWEEP": "wave"},
-                {"MD_FFT_PLANES": "0", "MD_Z_LINES": "8", "MD_REAL_SWEEP": "wg"}):
+                {"MD_FFT_PLANES": "0", "MD_Z_LINES": "8", "MD_REAL_SWEEP": "wg"},
+                {"MD_RIDE": "0", "MD_REAL_SWEEP": "wg"}, {"MD_RIDE": "1", "MD_REAL_SWEEP": "wave"}):
         e, f = coulomb(env)
         e.close()
         assert np.array_equal(f, ref), env
     e, _ = coulomb({})
     choice = e.autotune(2)
     assert choice["z_lines"] in (2, 4, 8, 16) and choice["fused_planes"] in (0, 1)
+    assert choice["real_shared_launch"] in (0, 1)
     assert np.array_equal(e.force_parts()[1], ref)
     e.close()
 

@@ -137,8 +137,7 @@ KERNEL_SYMBOLS = {  # kernel names of time_kernels -> device symbols (FP32 path)
     "kick_drift": "k_kick_drift", "short_range": "k_short_range<float>", "bin_q": "k_bin_q",
     "sort_q": "k_sort_q<float>", "spread": "k_spread<float>", "fft_x_fwd": "k_plane_fwd<float>",
     "fft_z_conv": "k_fftz_conv<float>", "fft_x_inv": "k_plane_inv<float>", "gather": "k_gather<float>",
-    "real": ("k_real_cells<float>", "k_coul_store<float>"), "bin_lj": "k_bin_lj", "sort_lj": "k_sort_lj",
-    "build_lj": "k_build_lj"}
+    "real": ("k_real_cells<float", "k_coul_store<float>"), "bin_lj": "k_bin_lj", "build_lj": "k_build_lj"}
 
 
 def _counter_rows(path, counter):

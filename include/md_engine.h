@@ -197,7 +197,8 @@ int md_error_analysis(md_engine *e, double *out, double *acc_se);
  * ms[15]: kick_drift, short_range, bin_q, sort_q, spread, fft_x_fwd,
  * fft_y_fwd, fft_z_conv, fft_y_inv, fft_x_inv, gather, real, bin_lj,
  * sort_lj, build_lj (with fused FFT planes, fft_x_* time the plane kernels
- * and fft_y_* are empty).  Positions, velocities and forces are restored
+ * and fft_y_* are empty; sort_lj is empty, its rank pass is part of
+ * build_lj).  Positions, velocities and forces are restored
  * after each iteration; the lists are rebuilt and the held Coulomb force is
  * recomputed at the current positions. */
 int md_time_kernels(md_engine *e, int32_t iters, double *ms, int32_t n);

@@ -21,6 +21,7 @@ constexpr int kSRBlock = 512;     // its workgroup (LJ on 4 wavefronts, the per-
 constexpr int kSRParticles = 64;  // parts on the others) takes 64 particles
 constexpr int kRealChunk = 4;     // target charges per wavefront work item of the real-space sweep
 constexpr int kRealChunkWG = 16;  // ... per workgroup work item (replicas with dense cells)
+constexpr int kLJStage = 256;     // LJ fill pass: candidates staged in order per target
 constexpr int kHistBins = 500;  // SizeHist of the z-density profile (glob_var.f90:59)
 
 // Per-replica control block, written by kernels.
@@ -93,8 +94,8 @@ struct Rep {
   int *bpart;
   // LJ cells: two count buffers used in turn (gen1 & 1 selects the one a
   // build fills; it clears the other at its particles' previous cells),
-  // members as binned, members ascending, each particle's cell (-1: none)
-  int *cnt1, *mem1, *mem1s, *cell1_of;
+  // members as binned, each particle's cell (-1: none)
+  int *cnt1, *mem1, *cell1_of;
   int *cnt2[2], *mem2;             // cells of the reference-form real-space list
   // real-space cell sweep at each Coulomb evaluation (edge >= r_c)
   int real_cells;                  // 1: the real-space cell sweep runs (charged systems)

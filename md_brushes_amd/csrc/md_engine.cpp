@@ -472,7 +472,7 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   nb[11] = coulomb ? nbin : 0;
   nb[12] = coulomb && S.plane_fused ? S.K[2] : 0;
   nb[13] = cdiv(NN, mdk::kSRParticles);
-  S.mem1 = S.mem1s = S.mem2 = S.mem3 = nullptr;
+  S.mem1 = S.mem2 = S.mem3 = nullptr;
   S.pos3s = nullptr;
   S.nb1 = S.nb2 = nullptr;
   S.ccap1 = S.ccap2 = S.ccap3 = S.nbcap1 = S.nbcap2 = 0;
@@ -544,7 +544,7 @@ int Sys::size_lists(const double *pos) {
   auto round8 = [](int v) { return (v + 7) & ~7; };
   // LJ: dense packing bounds occupancy near 1.4 per unit volume
   const double v1 = h.rcel1[0] * h.rcel1[1] * h.rcel1[2];
-  // at most 128: the LJ cell sort holds two members per lane of a wavefront
+  // at most 128: the fill pass ranks a cell's members by counting
   const int ccap1 = std::min(128, round8(std::max(2 * mc1 + 16, (int)(1.5 * v1) + 16)));
   const int nbcap1 =
       round8(std::min(NN, std::max(2 * mn1 + 32, (int)(4.19 * std::pow(p.rvl_eff, 3) * 1.2) + 32)));
@@ -604,9 +604,7 @@ int Sys::size_lists(const double *pos) {
   nb[10] = h.real_wg ? nb10_wg : nb10_wave;
   if (ccap1 != h.ccap1) {
     if (h.mem1) dfree(h.mem1, (size_t)ncell1 * h.ccap1 * 4);
-    if (h.mem1s) dfree(h.mem1s, (size_t)ncell1 * h.ccap1 * 4);
     h.mem1 = dalloc<int>((size_t)ncell1 * ccap1);
-    h.mem1s = dalloc<int>((size_t)ncell1 * ccap1);
     h.ccap1 = ccap1;
   }
   if (nbcap1 != h.nbcap1) {
@@ -624,7 +622,7 @@ int Sys::size_lists(const double *pos) {
     h.nb2 = dalloc<int>((size_t)h.Nq * nbcap2);
     h.nbcap2 = nbcap2;
   }
-  if (!h.mem1 || !h.mem1s || !h.nb1 || (h.real_verlet && (!h.mem2 || !h.nb2)))
+  if (!h.mem1 || !h.nb1 || (h.real_verlet && (!h.mem2 || !h.nb2)))
     return fail(MD_ENOMEM, "neighbour lists");
   int np = 1;
   while (np < h.ccap2) np <<= 1;
@@ -1397,7 +1395,8 @@ extern "C" int md_force_parts(md_engine *e, double *f_short, double *f_coul) {
 // replayed steps the short-range kernel drifts), 1 short_range (the replayed
 // form: second kick and the next step's kick and drift), 2 bin_q, 3 sort_q,
 // 4 spread, 5 fft_x_fwd, 6 fft_y_fwd, 7 fft_z_conv, 8 fft_y_inv, 9 fft_x_inv,
-// 10 gather, 11 real, 12 bin_lj, 13 sort_lj, 14 build_lj.  Positions,
+// 10 gather, 11 real, 12 bin_lj, 13 (0: the rank pass is part of
+// build_lj), 14 build_lj.  Positions,
 // velocities and forces are restored; the lists are rebuilt and the held
 // Coulomb force recomputed at the current positions.
 extern "C" int md_time_kernels(md_engine *e, int32_t iters, double *ms, int32_t n) {

@@ -106,6 +106,15 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Orders LDS accesses of the lanes of one wavefront (LDS operations of a
+// wavefront complete in order; the fences keep the compiler from moving
+// memory accesses across).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ------------------------------------------------------------- integrator
 
 // First half kick of new_position (init_update.f90:581-589), mass 1.
@@ -554,33 +563,9 @@ __global__ __launch_bounds__(kBlock) void k_bin_lj(const Rep *__restrict__ reps,
       const Rep &S = reps[qrep[e]];
       if (slot < S.ccap1) S.mem1[(size_t)cell * S.ccap1 + slot] = i;
       else atomicOr(&S.ctrl->ovf, 1);
+      // the largest occupancy seen (read first: one address for every particle)
+      if (slot >= S.ctrl->max_cell1) atomicMax(&S.ctrl->max_cell1, slot + 1);
     }
-  }
-}
-
-// Rank pass: each particle's place among its cell's members in ascending
-// index order (members are distinct), so that lists and every force sum
-// over them do not depend on the order of the binning atomics.
-__global__ __launch_bounds__(kBlock) void k_sort_lj(const Rep *__restrict__ reps,
-                                                    unsigned long long *lq, int cap) {
-  const unsigned long long head = *lq;
-  const int ne = min((int)(head >> kQShift), cap);
-  const int total = (int)(head & ((1ull << kQShift) - 1));
-  const int *qrep = lq_rep(lq), *qst = lq_start(lq, cap);
-  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
-    const int e = lq_find(qst, ne, g);
-    const Rep &S = reps[qrep[e]];
-    const int i = g - qst[e];
-    const int cell = S.cell1_of[i];
-    if (cell < 0) continue;
-    const int ncell = S.nc1[0] * S.nc1[1] * S.nc1[2];
-    const int raw = S.cnt1[(size_t)(S.ctrl->gen1 & 1) * ncell + cell];
-    const int n = min(raw, S.ccap1);
-    const int *m = S.mem1 + (size_t)cell * S.ccap1;
-    int rank = 0;
-    for (int k = 0; k < n; k++) rank += m[k] < i;
-    if (rank < n) S.mem1s[(size_t)cell * S.ccap1 + rank] = i;
-    if (rank == 0 && raw > S.ctrl->max_cell1) atomicMax(&S.ctrl->max_cell1, raw);
   }
 }
 
@@ -619,8 +604,13 @@ __device__ __forceinline__ int stencil_slot(const int *offs, int t) {
 
 // Fill pass of lj_verlet_list (compt_acc.f90:271-313): 32 lanes per target
 // particle.  The candidates are the members of the 27 stencil cells in the
-// reference order (stencil order, ascending within a cell); the lanes test
-// 32 of them per round and append the hits in that order with a ballot.
+// reference order (stencil order, ascending within a cell): the lanes first
+// place each binned member at its rank among its cell's members (members
+// are distinct particles) in an LDS stage of kLJStage slots per target (a
+// crowded stencil finds the member of each rank by counting instead), then
+// test 32 candidates per round and append the hits in that order with a
+// ballot, so that lists and every force sum over them do not depend on the
+// order of the binning atomics.
 // Pairs: i != j and r^2 < rvl^2, r^2 rounded as rij_and_rr rounds it.  The
 // list is ELL: slot k of particle i at k*NN + i.  The last kernel of a step
 // also advances every replica's step counter.
@@ -637,6 +627,8 @@ __global__ __launch_bounds__(kBlock) void k_build_lj(const Rep *__restrict__ rep
   const unsigned long long gmask = half ? 0xffffffff00000000ull : 0xffffffffull;
   const unsigned long long below = gmask & ((1ull << lane) - 1ull);
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  __shared__ int stage_all[kBlock / 32][kLJStage];
+  int *stage = stage_all[threadIdx.x >> 5];
   const int stride = gridDim.x * blockDim.x / 32;
   for (int t0 = 2 * wave; t0 < total; t0 += stride) {
     const int gi = t0 + half;
@@ -673,21 +665,49 @@ __global__ __launch_bounds__(kBlock) void k_build_lj(const Rep *__restrict__ rep
     const int excl = incl - cn;
     const int tot = __shfl(incl, 26, 32);
     const int tmax = max(tot, __shfl_xor(tot, 32, 64));
-    int n = 0;
-    for (int k0 = 0; k0 < tmax; k0 += 32) {
-      const int t = k0 + g;
-      // stencil cell holding candidate t: the largest l with excl_l <= t
+    const bool staged = tot <= kLJStage;
+    // stencil slot of candidate t: the largest l with excl_l <= t
+    auto slot_of = [&](int t) {
       int l = 0;
       for (int s = 16; s > 0; s >>= 1) {
         const int c = l + s;
         const int ex = __shfl(excl, c < 27 ? c : 26, 32);
         if (c < 27 && ex <= t) l = c;
       }
-      const int lcell = __shfl(cl, l, 32), lex = __shfl(excl, l, 32);
+      return l;
+    };
+    for (int k0 = 0; k0 < tmax; k0 += 32) {  // binned member -> its place in order
+      const int t = k0 + g;
+      const int l = slot_of(t);
+      const int lcell = __shfl(cl, l, 32), lex = __shfl(excl, l, 32), lcn = __shfl(cn, l, 32);
+      if (staged && t < tot) {
+        const int *m = S.mem1 + (size_t)lcell * S.ccap1;
+        const int j = m[t - lex];
+        int rk = 0;
+        for (int k = 0; k < lcn; k++) rk += m[k] < j;
+        stage[lex + rk] = j;
+      }
+    }
+    wave_sync();
+    int n = 0;
+    for (int k0 = 0; k0 < tmax; k0 += 32) {
+      const int t = k0 + g;
+      const int l = slot_of(t);
+      const int lcell = __shfl(cl, l, 32), lex = __shfl(excl, l, 32), lcn = __shfl(cn, l, 32);
       bool hit = false;
       int j = 0;
       if (t < tot) {
-        j = S.mem1s[(size_t)lcell * S.ccap1 + (t - lex)];
+        if (staged) {
+          j = stage[t];
+        } else {  // the member of rank t - lex: the one with that many smaller members
+          const int *m = S.mem1 + (size_t)lcell * S.ccap1;
+          for (int k = 0; k < lcn; k++) {
+            const int v = m[k];
+            int rk = 0;
+            for (int k2 = 0; k2 < lcn; k2++) rk += m[k2] < v;
+            if (rk == t - lex) j = v;
+          }
+        }
         double dx, dy, dz;
         hit = j != i && rsq_image(pi, S.pos[j], S.Lx, S.Ly, S.hLx, S.hLy, dx, dy, dz) < S.rv1sq;
       }
@@ -698,6 +718,7 @@ __global__ __launch_bounds__(kBlock) void k_build_lj(const Rep *__restrict__ rep
       }
       n += __popcll(mask & gmask);
     }
+    wave_sync();  // the stage is refilled for the next target
     if (live && g == 0) {
       S.nbn1[i] = n;
       if (n > S.nbcap1) atomicOr(&S.ctrl->ovf, 4);
@@ -1624,14 +1645,6 @@ __global__ __launch_bounds__(kBlock) void k_sort_q(const Rep *__restrict__ reps,
   }
 }
 
-// Orders LDS accesses of the lanes of one wavefront (LDS operations of a
-// wavefront complete in order; the fences keep the compiler from moving
-// memory accesses across).
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Candidates staged in LDS per pass of a k_real_cells wavefront.
 template <typename R> struct RealTile { static constexpr int N = 256; };
@@ -2190,7 +2203,6 @@ void launch_lists(const Plan &P, hipStream_t st, int inc_step, hipEvent_t *ev, b
   if (ev) (void)hipEventRecord(ev[0], st);
   hipLaunchKernelGGL(k_bin_lj, dim3(P.grid_bin), dim3(kBlock), 0, st, P.reps, P.lq, P.lq_cap);
   if (ev) (void)hipEventRecord(ev[1], st);
-  hipLaunchKernelGGL(k_sort_lj, dim3(P.grid_bin), dim3(kBlock), 0, st, P.reps, P.lq, P.lq_cap);
   if (ev) (void)hipEventRecord(ev[2], st);
   hipLaunchKernelGGL(k_build_lj, dim3(P.grid_build), dim3(kBlock), 0, st, P.reps, P.nrep, P.lq,
                      P.lq_cap, inc_step);

@@ -756,12 +756,10 @@ int md_engine::rebuild() {
     if (hipMalloc(&d_reps, 2 * (size_t)reps_cap * sizeof(Rep)) != hipSuccess)
       return fail(MD_ENOMEM, "descriptors");
   }
-  // real-space sweep form: a workgroup per item while the engine holds few
-  // charges (its items alone do not fill the chip), a wavefront per item
-  // otherwise; both give the same sums (MD_REAL_SWEEP=wave|wg forces one)
-  long nq_all = 0;
-  for (Sys *x : sys) nq_all += x->h.Nq;
-  int wg = tune_real >= 0 ? tune_real : nq_all < 200000;
+  // real-space sweep form: a workgroup per item (measured faster from one
+  // default brush to the 256-replica serving batch) unless tuned; both forms
+  // give the same sums (MD_REAL_SWEEP=wave|wg forces one)
+  int wg = tune_real >= 0 ? tune_real : 1;
   const char *rsw = std::getenv("MD_REAL_SWEEP");
   if (rsw && !std::strcmp(rsw, "wave")) wg = 0;
   if (rsw && !std::strcmp(rsw, "wg")) wg = 1;

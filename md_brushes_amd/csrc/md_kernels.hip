@@ -36,20 +36,32 @@ __device__ __forceinline__ void set_error(Ctrl *c, int code, int info, int step)
 // Runs of adjacent lanes with the same counter (neighbouring particles along
 // a chain mostly share their cell) take one atomic per run; the runs' atomics
 // are issued together.  Every lane of the wavefront calls it.
-__device__ __forceinline__ int wave_append(int *ctr) {
+// A lane's run of equal counters in its wavefront: the run's head lane h,
+// and, at a head, the run's length n.
+struct WaveRun {
+  int h, n;
+  bool head;
+};
+__device__ __forceinline__ WaveRun wave_run(const int *ctr) {
   const int lane = threadIdx.x & 63;
   const unsigned long long me = (unsigned long long)ctr;
   const unsigned long long prev = __shfl_up(me, 1, 64);
   const bool head = lane == 0 || prev != me;
   const unsigned long long heads = __ballot(head);
   const unsigned long long le = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
-  const int h = 63 - __clzll((long long)(heads & le));  // head of this lane's run
+  WaveRun w;
+  w.h = 63 - __clzll((long long)(heads & le));  // head of this lane's run
   const unsigned long long after = heads & ~le;
-  const int next = after ? __ffsll((long long)after) - 1 : 64;  // first head past this lane
+  w.n = (after ? __ffsll((long long)after) - 1 : 64) - lane;  // to the first head past this lane
+  w.head = head;
+  return w;
+}
+__device__ __forceinline__ int wave_append(int *ctr) {
+  const WaveRun w = wave_run(ctr);
   int base = 0;
-  if (head && ctr) base = atomicAdd(ctr, next - lane);
-  base = __shfl(base, h, 64);
-  return ctr ? base + (lane - h) : -1;
+  if (w.head && ctr) base = atomicAdd(ctr, w.n);
+  base = __shfl(base, w.h, 64);
+  return ctr ? base + ((int)(threadIdx.x & 63) - w.h) : -1;
 }
 
 // LJ rebuild queue: the replicas whose list is rebuilt after this step and
@@ -1588,32 +1600,49 @@ __global__ __launch_bounds__(kBlock) void k_bin_q(const Rep *__restrict__ reps, 
       nt[d] = e / S.tsz[d] != t0[d] ? 2 : 1;
     }
   }
+  // nine appends (up to eight tiles, the sweep cell): the runs first, then
+  // every atomic issued before any result is used, so that their latencies
+  // overlap
+  int key[9];
 #pragma unroll
   for (int k = 0; k < 8; k++) {
     const int a = k & 1, b = (k >> 1) & 1, c = k >> 2;
-    int key = -1;
+    key[k] = -1;
     if (a < nt[0] && b < nt[1] && c < nt[2]) {
       const int tx = (t0[0] + a) % S.ntl[0], ty = (t0[1] + b) % S.ntl[1],
                 tz = (t0[2] + c) % S.ntl[2];
       // sub-list by groups of eight consecutive charges
-      key = ((tz * S.ntl[1] + ty) * S.ntl[0] + tx) * kTileSub + ((m >> 3) & (kTileSub - 1));
-    }
-    const int slot = wave_append(key >= 0 ? S.tcnt + key : nullptr);
-    if (key >= 0) {
-      if (slot < S.capt) S.tmem[(size_t)key * S.capt + slot] = i;
-      else atomicOr(&S.ctrl->ovf, 32);
+      key[k] = ((tz * S.ntl[1] + ty) * S.ntl[0] + tx) * kTileSub + ((m >> 3) & (kTileSub - 1));
     }
   }
-  int key = -1;
+  key[8] = -1;
   if (live && S.real_cells) {
     int a, b, c;
-    if (cell_of(p, S.nc3, S.rcel3, S.hLx, S.hLy, a, b, c)) key = (c * S.nc3[1] + b) * S.nc3[0] + a;
+    if (cell_of(p, S.nc3, S.rcel3, S.hLx, S.hLy, a, b, c)) key[8] = (c * S.nc3[1] + b) * S.nc3[0] + a;
     else set_error(S.ctrl, MD_ECELL, i, -1);
   }
-  const int slot = wave_append(key >= 0 ? S.cnt3 + key : nullptr);
-  if (key >= 0) {
-    if (slot < S.ccap3) S.mem3[(size_t)key * S.ccap3 + slot] = m;
-    else atomicOr(&S.ctrl->ovf, 16);
+  int *ctr[9];
+  WaveRun run[9];
+  int base[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    ctr[k] = key[k] < 0 ? nullptr : (k < 8 ? S.tcnt : S.cnt3) + key[k];
+    run[k] = wave_run(ctr[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < 9; k++) base[k] = run[k].head && ctr[k] ? atomicAdd(ctr[k], run[k].n) : 0;
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const int slot = __shfl(base[k], run[k].h, 64) + (lane - run[k].h);
+    if (!ctr[k]) continue;
+    if (k < 8) {
+      if (slot < S.capt) S.tmem[(size_t)key[k] * S.capt + slot] = i;
+      else atomicOr(&S.ctrl->ovf, 32);
+    } else {
+      if (slot < S.ccap3) S.mem3[(size_t)key[k] * S.ccap3 + slot] = m;
+      else atomicOr(&S.ctrl->ovf, 16);
+    }
   }
 }
 

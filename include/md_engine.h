@@ -261,6 +261,16 @@ int md_batch_observe(md_engine *e, int32_t slot, double *height,
 int md_batch_error_analysis(md_engine *e, int32_t slot, double *out,
                             double *acc_se);
 
+/* The height analysis of write_height (in_output.f90:1017-1079, columns 2-9
+ * of height.txt, :1342-1344) from the resident positions: out[8] = h_avg
+ * (polymer beads), hl_max, hl_end, hl_avg (linear chains: mean over chains
+ * of the highest bead and of the free end; mean over their beads), hs_max,
+ * hs_end, hs_branch, hs_avg (stars: mean of the highest bead, of the ends of
+ * arms 2..arm, of the branch point; mean over their beads).  A molecule kind
+ * the system lacks reads 0 (the reference divides by zero). */
+int md_heights(md_engine *e, double *out);
+int md_batch_heights(md_engine *e, int32_t slot, double *out);
+
 /* Copies the last error message (NUL-terminated) into buf. */
 int md_last_error(md_engine *e, char *buf, int32_t len);
 

@@ -103,6 +103,8 @@ def load_library(path=None):
         "md_time_kernels": ([c_void_p, _I, dp, _I], _I),
         "md_autotune": ([c_void_p, _I, ip], _I),
         "md_error_analysis": ([c_void_p, dp, dp], _I),
+        "md_heights": ([c_void_p, dp], _I),
+        "md_batch_heights": ([c_void_p, _I, dp], _I),
         "md_batch_error_analysis": ([c_void_p, _I, dp, dp], _I),
         "md_destroy": ([c_void_p], None),
         "md_batch_create": ([POINTER(c_void_p), _I, _I], _I),
@@ -285,6 +287,14 @@ class Engine:
         self._check(self.lib.md_force_parts(self.h, _dp(fs), _dp(fc)), "md_force_parts")
         return fs.reshape(3, N).T, fc.reshape(3, N).T
 
+    HEIGHTS = ["h_avg", "hl_max", "hl_end", "hl_avg", "hs_max", "hs_end", "hs_branch", "hs_avg"]
+
+    def heights(self):
+        """The height analysis of write_height (in_output.f90:1017-1079)."""
+        out = np.zeros(8)
+        self._check(self.lib.md_heights(self.h, _dp(out)), "md_heights")
+        return dict(zip(self.HEIGHTS, out))
+
     def error_analysis(self):
         """error_analysis (compt_acc.f90:729-812) on the device: (summary,
         standard-Ewald reference Coulomb force (NN,3))."""
@@ -365,6 +375,12 @@ class Batch:
         n = c_int32()
         self._check(self.lib.md_batch_info(self.h, slot, byref(b), byref(n)), "md_batch_info")
         return b.value, n.value
+
+    def heights(self, slot):
+        """Engine.heights of one replica."""
+        out = np.zeros(8)
+        self._check(self.lib.md_batch_heights(self.h, slot, _dp(out)), "md_batch_heights")
+        return dict(zip(Engine.HEIGHTS, out))
 
     def error_analysis(self, slot):
         """Accuracy spot check of one replica: (summary, reference force)."""

@@ -44,6 +44,7 @@ struct Rep {
   // sizes
   int NN, Nq, N_anchor, real_verlet;
   int Npe;                         // polymer beads are particles 0..Npe-1
+  int Nga, arm, Nma, Ngl, Nml;     // stars (first), then linear chains
   int preheat_end;                 // batch replicas: steps 1..preheat_end rescale
                                    // velocities every 20 steps (0: none)
   // geometry

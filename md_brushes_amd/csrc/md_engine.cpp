@@ -149,6 +149,7 @@ struct Sys {
   float4 *pos32b[2] = {nullptr, nullptr};
   int *d_hist = nullptr;     // observables: z histogram of the chains
   double *d_hsum = nullptr;  // observables: per-block sums of z
+  double *d_mol = nullptr;   // observables: per-molecule heights
   std::string err;
 
   ~Sys() {
@@ -323,8 +324,14 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   d_out2 = dalloc<double>(3 * nn);
   d_hist = dalloc<int>(mdk::kHistBins);
   d_hsum = dalloc<double>(cdiv(NN, mdk::kBlock));  // one per particle block of k_observe
+  d_mol = dalloc<double>(4 * (size_t)std::max(1, p.Nga + p.Ngl));  // per-molecule heights
+  S.Nga = p.Nga;
+  S.arm = p.arm;
+  S.Nma = p.Nma;
+  S.Ngl = p.Ngl;
+  S.Nml = p.Nml;
   if (!cc || !S.flags || !S.charge || !S.bstart || !S.bpart || !S.ctrl || !d_eta || !d_out1 ||
-      !d_out2 || !d_hist || !d_hsum)
+      !d_out2 || !d_hist || !d_hsum || !d_mol)
     return fail(MD_ENOMEM, "topology arrays");
   (void)hipMemcpy(S.flags, flags.data(), nn * sizeof(int), hipMemcpyHostToDevice);
   if (Nq) (void)hipMemcpy(S.charge, charge.data(), Nq * sizeof(int), hipMemcpyHostToDevice);
@@ -1839,4 +1846,66 @@ extern "C" int md_batch_error_analysis(md_engine *e, int32_t slot, double *out, 
   }
   Sys *s = it->second;
   return e->error_analysis(s, e->single[e->index_of(s)], out, acc_se);
+}
+
+// The height analysis of write_height (in_output.f90:1017-1079,1342-1344):
+// out[8] = h_avg, hl_max, hl_end, hl_avg, hs_max, hs_end, hs_branch,
+// hs_avg (the columns after the step of height.txt), from per-molecule
+// sums on the device added up here in molecule order.  A molecule kind
+// that the system lacks reads 0 (the reference divides by zero there).
+static int heights(md_engine *e, Sys *s, const Plan &P, double *out) {
+  if (!out) return e->fail(MD_EINVAL, "out is null");
+  const md_params &p = s->p;
+  const int nm = p.Nga + p.Ngl;
+  mdk::launch_observe(P, e->st, s->d_hist, s->d_hsum);
+  mdk::launch_heights(P, e->st, s->d_mol);
+  int rc = e->check_sys(s);
+  if (rc) return rc;
+  const int nb = cdiv(p.NN, mdk::kBlock);
+  std::vector<double> hs(nb), mol(4 * (size_t)std::max(nm, 1));
+  hipError_t err = hipMemcpy(hs.data(), s->d_hsum, nb * sizeof(double), hipMemcpyDeviceToHost);
+  if (err == hipSuccess) err = hipMemcpy(mol.data(), s->d_mol, mol.size() * sizeof(double), hipMemcpyDeviceToHost);
+  if (err != hipSuccess) return e->hip_fail(err, "heights");
+  double sum = 0;
+  for (double v : hs) sum += v;
+  double hl_avg = 0, hl_max = 0, hl_end = 0, hs_avg = 0, hs_max = 0, hs_branch = 0, hs_end = 0;
+  for (int k = 0; k < p.Ngl; k++) {
+    hl_avg += mol[4 * k];
+    hl_max += mol[4 * k + 1];
+    hl_end += mol[4 * k + 2];
+  }
+  for (int k = p.Ngl; k < nm; k++) {
+    hs_avg += mol[4 * k];
+    hs_max += mol[4 * k + 1];
+    hs_branch += mol[4 * k + 2];
+    hs_end += mol[4 * k + 3];
+  }
+  out[0] = p.Npe > 0 ? sum / p.Npe : 0;
+  out[1] = p.Ngl > 0 ? hl_max / p.Ngl : 0;
+  out[2] = p.Ngl > 0 ? hl_end / p.Ngl : 0;
+  out[3] = p.Ngl > 0 && p.Nml > 0 ? hl_avg / p.Ngl / p.Nml : 0;
+  out[4] = p.Nga > 0 ? hs_max / p.Nga : 0;
+  out[5] = p.Nga > 0 && p.arm > 1 ? hs_end / (p.Nga * (p.arm - 1)) : 0;
+  out[6] = p.Nga > 0 ? hs_branch / p.Nga : 0;
+  out[7] = p.Nga > 0 ? hs_avg / (p.Nga * (p.arm * p.Nma + 1)) : 0;
+  return MD_OK;
+}
+
+extern "C" int md_heights(md_engine *e, double *out) {
+  ENGINE_CHECK(e && out);
+  FIRST(e);
+  STATE_CHECK(e);
+  return heights(e, s, P, out);
+}
+
+extern "C" int md_batch_heights(md_engine *e, int32_t slot, double *out) {
+  ENGINE_CHECK(e && out);
+  auto it = e->slots.find(slot);
+  if (it == e->slots.end()) return e->fail(MD_EINVAL, "no such slot");
+  if (e->dirty) {
+    int rc = e->rebuild();
+    if (rc) return rc;
+  }
+  Sys *s = it->second;
+  return heights(e, s, e->single[e->index_of(s)], out);
 }

@@ -64,6 +64,8 @@ void launch_set_flags(const Plan &P, hipStream_t st, int clear);
 // velocities before the rescale: redo it with the rescaled ones
 void launch_rescale(const Plan &P, hipStream_t st, int gated = 0, int redrift = 0);
 void launch_observe(const Plan &P, hipStream_t st, int *hist, double *hsum);
+// one replica (P.nrep == 1): per-molecule heights, 4 doubles per molecule
+void launch_heights(const Plan &P, hipStream_t st, double *mol);
 
 // Standard-Ewald reference force of error_analysis (FP64, one replica).
 struct EwaldArgs {

@@ -2121,6 +2121,43 @@ __global__ __launch_bounds__(kBlock) void k_observe(const Rep *__restrict__ reps
   }
 }
 
+// Per-molecule heights of the height analysis (in_output.f90:1017-1079),
+// one thread per molecule, written to mol[4 k ..]: linear chain c (its Nml
+// beads from the grafted one): sum, maximum and end z; star s (the
+// arm*Nma+1 beads of one molecule: the grafted arm up to the branch point,
+// then arms 2..arm): sum and maximum z, branch-point z, and the sum of the
+// ends of arms 2..arm.  The host adds them up in molecule order.
+__global__ __launch_bounds__(kBlock) void k_heights(const Rep *__restrict__ reps, double *mol) {
+  const Rep &S = reps[0];
+  const int per = S.arm * S.Nma + 1, nta = S.Nga * per;
+  for (int k = threadIdx.x; k < S.Nga + S.Ngl; k += blockDim.x) {
+    double sum = 0, mx = 0, a = 0, b = 0;
+    if (k < S.Ngl) {
+      const int i0 = nta + k * S.Nml;
+      for (int j = 0; j < S.Nml; j++) {
+        const double z = S.pos[i0 + j].z;
+        sum += z;
+        if (mx < z) mx = z;
+      }
+      a = S.pos[i0 + S.Nml - 1].z;
+    } else {
+      const int i0 = (k - S.Ngl) * per;
+      for (int j = 0; j < per; j++) {
+        const double z = S.pos[i0 + j].z;
+        sum += z;
+        if (mx < z) mx = z;
+      }
+      a = S.pos[i0 + S.Nma].z;  // the branch point
+      for (int j = 2; j <= S.arm; j++) b += S.pos[i0 + j * S.Nma].z;
+    }
+    double *o = mol + 4 * (size_t)k;
+    o[0] = sum;
+    o[1] = mx;
+    o[2] = a;
+    o[3] = b;
+  }
+}
+
 // --------------------------------------------------------- launchers
 
 // One replica (P.nrep == 1); Npe is read from the descriptor.
@@ -2128,6 +2165,10 @@ void launch_observe(const Plan &P, hipStream_t st, int *hist, double *hsum) {
   hipLaunchKernelGGL(k_hist_clear, dim3(1), dim3(kBlock), 0, st, hist);
   const int nb = P.part.total;  // enough blocks for Npe <= NN
   hipLaunchKernelGGL(k_observe, dim3(nb), dim3(kBlock), 0, st, P.reps, hist, hsum);
+}
+
+void launch_heights(const Plan &P, hipStream_t st, double *mol) {
+  hipLaunchKernelGGL(k_heights, dim3(1), dim3(kBlock), 0, st, P.reps, mol);
 }
 
 // With ev != null, records ev[k] before kernel k and ev[8] after the last

@@ -10,7 +10,7 @@ program main
   use input_output
   implicit none
   integer :: i, nxt, last
-  real*8 :: h
+  real*8 :: h(8)
 
   call cpu_time(started)
   call initialize_parameters

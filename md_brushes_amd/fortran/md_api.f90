@@ -90,6 +90,11 @@ module md_api
       type(c_ptr), value :: e
       integer(c_int32_t), value :: step0, nsteps, preheat
     end function
+    integer(c_int) function md_heights(e, out) bind(C, name="md_heights")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: e
+      real(c_double), dimension(8), intent(out) :: out
+    end function
     integer(c_int) function md_error_analysis(e, out, acc_se) bind(C, name="md_error_analysis")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: e

@@ -237,17 +237,19 @@ contains
   end subroutine continue_read_data
 
   ! Brush height: mean z of the polymer beads (in_output.f90:1017-1020).
+  ! height (in_output.f90:1017-1079): h_avg, hl_max, hl_end, hl_avg,
+  ! hs_max, hs_end, hs_branch, hs_avg, computed on the device.
   subroutine height(h)
-    real*8, intent(out) :: h
-    call fetch_state
-    h = sum(pos(1:Npe,3)) / Npe
+    real*8, intent(out) :: h(8)
+    call check(md_heights(engine, h), 'height')
   end subroutine height
 
+  ! write_height's height.txt line (in_output.f90:1342-1344, format 9F17.6)
   subroutine write_height(j, h)
     integer, intent(in) :: j
-    real*8, intent(in) :: h
+    real*8, intent(in) :: h(8)
     open(36, position='append', file='./data/height.txt')
-      write(36,'(2F17.6)') j*1., h
+      write(36,'(9F17.6)') j*1., h
     close(36)
   end subroutine write_height
 

@@ -166,3 +166,40 @@ def test_batch_error_analysis_spot_check():
         out.append(b.download(sl)[0])
         b.close()
     assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("name", ["default", "star3", "mixed_small"])
+def test_heights_match_definitions(name):
+    """md_heights / md_batch_heights equal write_height's definitions
+    (in_output.f90:1017-1079) evaluated on the downloaded positions."""
+    import sys, os
+    sys.path.insert(0, os.path.dirname(__file__))
+    from systems import heights_ref, system
+    print(f"[heights] {name}", flush=True)
+    if name == "mixed_small":  # stars and chains together
+        p, pos, vel = _req(4, 40, 3, 3, 0.2, 41, MD_FP64)
+        p2 = p.copy()
+        from md_brushes_amd.configs import _apply
+        from md_brushes_amd import derive
+        _apply(p2, [("Ngl", 8), ("Nml", 21)])
+        p = derive(p2)
+        pos, vel = initialize_position(p, 41), initialize_velocity(p, 42)
+    else:
+        p, pos, vel = system(name)
+    e = Engine(p)
+    e.upload(pos, vel)
+    e.build_lists()
+    e.compute_force(0, MD_NOISE_PHILOX)
+    e.run(1, 20)
+    got = e.heights()
+    pg = e.download()[0]
+    e.close()
+    ref = heights_ref(pg, p)
+    for k, v in ref.items():
+        assert abs(got[k] - v) <= 1e-12 * max(1.0, abs(v)), (k, got[k], v)
+    b = Batch(p.precision)
+    sl = b.add(p, pg, np.zeros_like(vel))
+    hb = b.heights(sl)
+    b.close()
+    for k, v in ref.items():
+        assert abs(hb[k] - v) <= 1e-12 * max(1.0, abs(v)), (k, hb[k], v)

@@ -36,7 +36,10 @@ def test_fortran_host_matches_c_abi(tmp_path):
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0 and "Finished!" in r.stdout
     h = np.loadtxt(tmp_path / "data" / "height.txt")
-    assert h.shape == (10, 2) and np.all(np.isfinite(h[:, 1]))
+    # write_height's nine columns (in_output.f90:1342-1344): step, h_avg,
+    # linear hl_max, hl_end, hl_avg, star hs_max, hs_end, hs_branch, hs_avg
+    assert h.shape == (10, 9) and np.all(np.isfinite(h))
+    assert np.all(h[:, 2] >= h[:, 4]) and np.all(h[:, 5] >= h[:, 8])  # a chain's top above its mean
     # error_analysis at the start (main.f90:30-33 of the reference), on the device
     rms = np.loadtxt(tmp_path / "data" / "rms_force.txt", ndmin=2)
     assert rms.shape == (1, 6) and rms[0, 0] == 0 and 0 < rms[0, 1] < 1e-2 and rms[0, 2] > 0
@@ -61,4 +64,10 @@ def test_fortran_host_matches_c_abi(tmp_path):
     e.run(41, 60)
     pg, vg, _, _ = e.download()
     assert abs(pg[: p.Npe, 2].mean() - h[-1, 1]) < 1e-5
+    import sys
+    sys.path.insert(0, os.path.dirname(__file__))
+    from systems import heights_ref
+    ref = heights_ref(pg, p)
+    for c, k in enumerate(["h_avg", "hl_max", "hl_end", "hl_avg", "hs_max", "hs_end", "hs_branch", "hs_avg"]):
+        assert abs(h[-1, 1 + c] - ref[k]) < 1e-5, k
     e.close()

@@ -271,6 +271,17 @@ int md_batch_error_analysis(md_engine *e, int32_t slot, double *out,
 int md_heights(md_engine *e, double *out);
 int md_batch_heights(md_engine *e, int32_t slot, double *out);
 
+/* Exact resume (no reference counterpart: the reference restarts from its
+ * six-decimal text checkpoint and rescales the velocities, main.f90:40-64).
+ * md_state_save writes the engine's complete state between steps
+ * (positions, velocities, forces, held Coulomb force, LJ list and cells,
+ * accumulated displacements) into buf, md_state_size bytes; md_state_load
+ * restores it into an engine made from the same parameters and precision,
+ * after which the run continues bit for bit as if never interrupted. */
+int md_state_size(md_engine *e, int64_t *bytes);
+int md_state_save(md_engine *e, void *buf, int64_t cap);
+int md_state_load(md_engine *e, const void *buf, int64_t len);
+
 /* Copies the last error message (NUL-terminated) into buf. */
 int md_last_error(md_engine *e, char *buf, int32_t len);
 

@@ -104,6 +104,9 @@ def load_library(path=None):
         "md_autotune": ([c_void_p, _I, ip], _I),
         "md_error_analysis": ([c_void_p, dp, dp], _I),
         "md_heights": ([c_void_p, dp], _I),
+        "md_state_size": ([c_void_p, POINTER(c_int64)], _I),
+        "md_state_save": ([c_void_p, c_void_p, c_int64], _I),
+        "md_state_load": ([c_void_p, c_void_p, c_int64], _I),
         "md_batch_heights": ([c_void_p, _I, dp], _I),
         "md_batch_error_analysis": ([c_void_p, _I, dp, dp], _I),
         "md_destroy": ([c_void_p], None),
@@ -286,6 +289,19 @@ class Engine:
         fs, fc = np.zeros(3 * N), np.zeros(3 * N)
         self._check(self.lib.md_force_parts(self.h, _dp(fs), _dp(fc)), "md_force_parts")
         return fs.reshape(3, N).T, fc.reshape(3, N).T
+
+    def save_state(self):
+        """The engine's complete state (bytes) for an exact resume."""
+        n = c_int64()
+        self._check(self.lib.md_state_size(self.h, byref(n)), "md_state_size")
+        buf = np.zeros(n.value, dtype=np.uint8)
+        self._check(self.lib.md_state_save(self.h, buf.ctypes.data_as(c_void_p), n.value), "md_state_save")
+        return buf.tobytes()
+
+    def load_state(self, state):
+        """Restores save_state's bytes: the run continues bit for bit."""
+        buf = np.frombuffer(state, dtype=np.uint8)
+        self._check(self.lib.md_state_load(self.h, buf.ctypes.data_as(c_void_p), buf.size), "md_state_load")
 
     HEIGHTS = ["h_avg", "hl_max", "hl_end", "hl_avg", "hs_max", "hs_end", "hs_branch", "hs_avg"]
 

@@ -178,6 +178,7 @@ struct Sys {
   }
   int setup(const md_params *pin, bool want_fp32);
   int size_lists(const double *pos);
+  int set_list_caps(int ccap1, int nbcap1);  // LJ cell and list capacities as given
   // descriptor whose canonical positions are buffer q
   Rep table(int q) const {
     Rep t = h;
@@ -489,6 +490,23 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
 
 // Sizes cell and list capacities from a configuration: twice the largest
 // occupancy and list length found on the host, plus headroom.
+int Sys::set_list_caps(int ccap1, int nbcap1) {
+  if (ccap1 <= 0 || nbcap1 <= 0) return fail(MD_EINVAL, "list capacities");
+  if (ccap1 != h.ccap1) {
+    if (h.mem1) dfree(h.mem1, (size_t)ncell1 * h.ccap1 * 4);
+    h.mem1 = dalloc<int>((size_t)ncell1 * ccap1);
+    h.ccap1 = ccap1;
+  }
+  if (nbcap1 != h.nbcap1) {
+    if (h.nb1) dfree(h.nb1, (size_t)p.NN * h.nbcap1 * 4);
+    h.nb1 = dalloc<int>((size_t)p.NN * nbcap1);
+    h.nbcap1 = nbcap1;
+  }
+  if (!h.mem1 || !h.nb1) return fail(MD_ENOMEM, "neighbour lists");
+  if (null_stream_done() != hipSuccess) return fail(MD_EHIP, "list buffers");
+  return MD_OK;
+}
+
 int Sys::size_lists(const double *pos) {
   const int NN = p.NN;
   const double *X = pos, *Y = pos + NN, *Z = pos + 2 * (size_t)NN;
@@ -1908,4 +1926,104 @@ extern "C" int md_batch_heights(md_engine *e, int32_t slot, double *out) {
   }
   Sys *s = it->second;
   return heights(e, s, e->single[e->index_of(s)], out);
+}
+
+// ------------------------------------------------------- exact resume
+
+// The complete state of a single-system engine between steps: positions,
+// velocities, forces, the held Coulomb force, the LJ list with its cells
+// and the control block (accumulated displacements, rebuild generation).
+// Loading it into an engine made from the same parameters continues the
+// trajectory bit for bit (the reference restarts from its six-decimal text
+// checkpoint and rescales the velocities, main.f90:40-64).
+namespace {
+struct StateHead {
+  char magic[8];  // "MDSTATE1"
+  int64_t NN, fp32, ncell1, ccap1, nbcap1;
+};
+
+size_t state_bytes(const Sys *s) {
+  const size_t NN = s->p.NN, rs = s->fp32 ? 4 : 8;
+  return sizeof(StateHead) + NN * 10 * 8 + 3 * NN * rs + NN * 4 + (size_t)s->h.nbcap1 * NN * 4 +
+         2 * (size_t)s->ncell1 * 4 + (size_t)s->ncell1 * s->h.ccap1 * 4 + NN * 4 + sizeof(Ctrl);
+}
+}  // namespace
+
+extern "C" int md_state_size(md_engine *e, int64_t *bytes) {
+  ENGINE_CHECK(e && bytes);
+  FIRST(e);
+  STATE_CHECK(e);
+  *bytes = (int64_t)state_bytes(s);
+  return MD_OK;
+}
+
+extern "C" int md_state_save(md_engine *e, void *buf, int64_t cap) {
+  ENGINE_CHECK(e && buf);
+  FIRST(e);
+  STATE_CHECK(e);
+  const size_t need = state_bytes(s);
+  if ((size_t)cap < need) return e->fail(MD_EINVAL, "state buffer too small");
+  const size_t NN = s->p.NN, rs = s->fp32 ? 4 : 8;
+  char *o = (char *)buf;
+  StateHead hd{};
+  std::memcpy(hd.magic, "MDSTATE1", 8);
+  hd.NN = (int64_t)NN;
+  hd.fp32 = s->fp32;
+  hd.ncell1 = s->ncell1;
+  hd.ccap1 = s->h.ccap1;
+  hd.nbcap1 = s->h.nbcap1;
+  std::memcpy(o, &hd, sizeof hd);
+  o += sizeof hd;
+  Ctrl c{};
+  int rc = download_sys(e, s, (double *)o, (double *)o + 4 * NN, (double *)o + 7 * NN, &c);
+  if (rc) return rc;
+  o += NN * 10 * 8;
+  auto get = [&](const void *src, size_t n) {
+    hipError_t err = hipMemcpy(o, src, n, hipMemcpyDeviceToHost);
+    o += n;
+    return err;
+  };
+  hipError_t err = get(s->h.ccx, 3 * NN * rs);
+  if (err == hipSuccess) err = get(s->h.nbn1, NN * 4);
+  if (err == hipSuccess) err = get(s->h.nb1, (size_t)s->h.nbcap1 * NN * 4);
+  if (err == hipSuccess) err = get(s->h.cnt1, 2 * (size_t)s->ncell1 * 4);
+  if (err == hipSuccess) err = get(s->h.mem1, (size_t)s->ncell1 * s->h.ccap1 * 4);
+  if (err == hipSuccess) err = get(s->h.cell1_of, NN * 4);
+  if (err != hipSuccess) return e->hip_fail(err, "state save");
+  std::memcpy(o, &c, sizeof c);
+  return MD_OK;
+}
+
+extern "C" int md_state_load(md_engine *e, const void *buf, int64_t len) {
+  ENGINE_CHECK(e && buf);
+  FIRST(e);
+  StateHead hd;
+  if ((size_t)len < sizeof hd) return e->fail(MD_EINVAL, "state too short");
+  std::memcpy(&hd, buf, sizeof hd);
+  if (std::memcmp(hd.magic, "MDSTATE1", 8) || hd.NN != s->p.NN || hd.fp32 != (int64_t)s->fp32 ||
+      hd.ncell1 != s->ncell1)
+    return e->fail(MD_EINVAL, "state of another system or precision");
+  const size_t NN = s->p.NN, rs = s->fp32 ? 4 : 8;
+  const char *in = (const char *)buf + sizeof hd;
+  int rc = upload_sys(e, s, (const double *)in, (const double *)in + 4 * NN, (const double *)in + 7 * NN);
+  if (rc) return rc;
+  rc = s->set_list_caps((int)hd.ccap1, (int)hd.nbcap1);
+  if (rc) return e->fail(rc, s->err);
+  if ((size_t)len < state_bytes(s)) return e->fail(MD_EINVAL, "state too short");
+  in += NN * 10 * 8;
+  auto put = [&](void *dst, size_t n) {
+    hipError_t err = hipMemcpy(dst, in, n, hipMemcpyHostToDevice);
+    in += n;
+    return err;
+  };
+  hipError_t err = put(s->h.ccx, 3 * NN * rs);
+  if (err == hipSuccess) err = put(s->h.nbn1, NN * 4);
+  if (err == hipSuccess) err = put(s->h.nb1, (size_t)s->h.nbcap1 * NN * 4);
+  if (err == hipSuccess) err = put(s->h.cnt1, 2 * (size_t)s->ncell1 * 4);
+  if (err == hipSuccess) err = put(s->h.mem1, (size_t)s->ncell1 * s->h.ccap1 * 4);
+  if (err == hipSuccess) err = put(s->h.cell1_of, NN * 4);
+  if (err == hipSuccess) err = hipMemcpy(s->h.ctrl, in, sizeof(Ctrl), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = null_stream_done();
+  if (err != hipSuccess) return e->hip_fail(err, "state load");
+  return e->rebuild();
 }

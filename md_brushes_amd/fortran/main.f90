@@ -26,11 +26,16 @@ program main
     call compute_force
   else
     call continue_read_data(i)
-    call lj_verlet_list
-    if (prm%qq /= 0) call error_analysis
-    call rescale_velocity
-    step = i
-    call compute_force
+    if (read_state()) then  ! exact resume: the saved state, lists and forces
+      write(*,*) 'exact resume from data/state.bin at step', i
+      step = i
+    else
+      call lj_verlet_list
+      if (prm%qq /= 0) call error_analysis
+      call rescale_velocity
+      step = i
+      call compute_force
+    end if
   end if
 
   !##############preheating##############!

@@ -90,6 +90,23 @@ module md_api
       type(c_ptr), value :: e
       integer(c_int32_t), value :: step0, nsteps, preheat
     end function
+    integer(c_int) function md_state_size(e, n) bind(C, name="md_state_size")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: e
+      integer(c_int64_t), intent(out) :: n
+    end function
+    integer(c_int) function md_state_save(e, buf, cap) bind(C, name="md_state_save")
+      import :: c_int, c_ptr, c_int64_t, c_int8_t
+      type(c_ptr), value :: e
+      integer(c_int8_t), dimension(*), intent(out) :: buf
+      integer(c_int64_t), value :: cap
+    end function
+    integer(c_int) function md_state_load(e, buf, n) bind(C, name="md_state_load")
+      import :: c_int, c_ptr, c_int64_t, c_int8_t
+      type(c_ptr), value :: e
+      integer(c_int8_t), dimension(*), intent(in) :: buf
+      integer(c_int64_t), value :: n
+    end function
     integer(c_int) function md_heights(e, out) bind(C, name="md_heights")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: e

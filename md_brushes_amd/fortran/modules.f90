@@ -214,8 +214,8 @@ contains
     write(*,*) 'NN =', NN, ' Nq =', Nq, ' Lx =', Lx
   end subroutine initialize_parameters
 
-  ! continue_read_data (in_output.f90:605-618): positions, velocities and
-  ! the step of the last checkpoint.
+  ! continue_read_data (in_output.f90:605-618): positions, velocities, and
+  ! the step after the last checkpoint.
   subroutine continue_read_data(l)
     integer, intent(out) :: l
     integer :: i
@@ -231,6 +231,7 @@ contains
       read(11,*) l
       read(11,*) t
     close(11)
+    l = l + 1
     total_time = t
     acc = 0
     call check(md_upload_state(engine, pos, vel, acc), 'upload')
@@ -290,6 +291,7 @@ contains
         write(33,'(3F17.6)') vel(i,1), vel(i,2), vel(i,3)
       end do
     close(33)
+    call write_state
     call cpu_time(finished)
     open(34, file='./start_time.txt')
       write(34,*) 1
@@ -297,5 +299,42 @@ contains
       write(34,*) finished - started + total_time
     close(34)
   end subroutine write_checkpoint
+
+  ! The engine's complete state beside the text checkpoint (data/state.bin,
+  ! raw bytes): with MD_EXACT_RESUME=1 a restart loads it and continues bit
+  ! for bit instead of the reference's text restart.
+  subroutine write_state
+    integer(c_int64_t) :: n
+    integer(c_int8_t), allocatable :: buf(:)
+    call check(md_state_size(engine, n), 'state size')
+    allocate(buf(n))
+    call check(md_state_save(engine, buf, n), 'state save')
+    open(38, file='./data/state.bin', access='stream', form='unformatted', status='replace')
+      write(38) buf
+    close(38)
+    deallocate(buf)
+  end subroutine write_state
+
+  ! Exact resume from data/state.bin when MD_EXACT_RESUME=1 and the file
+  ! exists; .false. otherwise.
+  logical function read_state()
+    character(len=8) :: flag
+    integer :: st
+    integer(c_int64_t) :: n
+    integer(c_int8_t), allocatable :: buf(:)
+    logical :: alive
+    read_state = .false.
+    call get_environment_variable('MD_EXACT_RESUME', flag, status=st)
+    if (st /= 0 .or. flag(1:1) /= '1') return
+    inquire(file='./data/state.bin', exist=alive, size=n)
+    if (.not. alive .or. n <= 0) return
+    allocate(buf(n))
+    open(38, file='./data/state.bin', access='stream', form='unformatted', status='old')
+      read(38) buf
+    close(38)
+    call check(md_state_load(engine, buf, n), 'state load')
+    deallocate(buf)
+    read_state = .true.
+  end function read_state
 
 end module input_output

@@ -71,3 +71,56 @@ def test_fortran_host_matches_c_abi(tmp_path):
     for c, k in enumerate(["h_avg", "hl_max", "hl_end", "hl_avg", "hs_max", "hs_end", "hs_branch", "hs_avg"]):
         assert abs(h[-1, 1 + c] - ref[k]) < 1e-5, k
     e.close()
+
+
+def test_fortran_exact_resume(tmp_path):
+    """A run stopped at a checkpoint and restarted with MD_EXACT_RESUME=1
+    (data/state.bin) ends in the state of the uninterrupted run, bit for
+    bit; the restart continues at the step after the checkpoint
+    (in_output.f90:616)."""
+    assert os.path.exists(EXE), "build the Fortran host first"
+
+    def run(d, steps, env=None):
+        raw = _params()
+        _apply(raw, [("StepNum", steps), ("DeltaStep3", 50)])
+        write_param_files(raw, d / "system_data.txt", d / "force_data.txt", d / "q_arm.txt")
+        (d / "data").mkdir(exist_ok=True)
+        r = subprocess.run([EXE], cwd=d, capture_output=True, text=True, timeout=120,
+                           env=dict(os.environ, **(env or {})))
+        print(r.stdout[-1500:], r.stderr[-1500:])
+        assert r.returncode == 0 and "Finished!" in r.stdout
+        return r.stdout
+
+    full, part = tmp_path / "full", tmp_path / "part"
+    full.mkdir()
+    part.mkdir()
+    print("[fortran exact resume] uninterrupted", flush=True)
+    run(full, 60)  # steps 1..100, checkpoints at 50 and 100
+    print("[fortran exact resume] stop at 50, resume", flush=True)
+    run(part, 10)  # steps 1..50
+    out = run(part, 60, {"MD_EXACT_RESUME": "1"})
+    assert "exact resume from data/state.bin at step" in out
+    a = np.fromfile(full / "data" / "state.bin", dtype=np.uint8)
+    b = np.fromfile(part / "data" / "state.bin", dtype=np.uint8)
+    assert a.size == b.size
+    # md_state_save's layout: header, pos, vel, acc, held Coulomb force, list
+    # lengths, ELL list, the two cell-count buffers, cell members as binned,
+    # each particle's cell, control block
+    NN, fp32, ncell1, ccap1, nbcap1 = (int(x) for x in np.frombuffer(a[:48].tobytes(), np.int64)[1:6])
+    rs = 4 if fp32 else 8
+    sizes = [32 * NN, 24 * NN, 24 * NN, 3 * NN * rs, 4 * NN, 4 * nbcap1 * NN, 8 * ncell1,
+             4 * ncell1 * ccap1, 4 * NN]
+    o = 48
+    for k, n in enumerate(sizes):
+        x, y = a[o:o + n], b[o:o + n]
+        if k == 7:  # members in binning order: the order of the atomics is free
+            mx = np.frombuffer(x.tobytes(), np.int32).reshape(ncell1, ccap1)
+            my = np.frombuffer(y.tobytes(), np.int32).reshape(ncell1, ccap1)
+            cnt = np.frombuffer(a[o - 8 * ncell1:o].tobytes(), np.int32).reshape(2, ncell1)
+            for c in range(ncell1):
+                m = min(max(cnt[0, c], cnt[1, c]), ccap1)
+                assert np.array_equal(np.sort(mx[c, :m]), np.sort(my[c, :m]))
+        else:
+            assert np.array_equal(x, y), k
+        o += n
+    assert np.array_equal(a[o:], b[o:])  # control block

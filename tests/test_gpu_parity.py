@@ -434,3 +434,33 @@ def test_error_analysis_default_fp32():
         assert np.array_equal(x, y)
     e.close()
     e2.close()
+
+
+@pytest.mark.parametrize("prec", [MD_FP64, MD_FP32])
+def test_exact_resume(prec):
+    """A run saved mid-way (md_state_save) and loaded into a fresh engine
+    (md_state_load) continues bit for bit; a state of another system is
+    refused."""
+    print(f"[exact resume] default prec={prec}", flush=True)
+    p, pos, vel, e1 = _engine("default", prec)
+    e1.compute_force(0, MD_NOISE_PHILOX)
+    e1.run(1, 47)
+    ref = e1.download()
+    e1.close()
+    p, pos, vel, e2 = _engine("default", prec)
+    e2.compute_force(0, MD_NOISE_PHILOX)
+    e2.run(1, 23)
+    state = e2.save_state()
+    e2.close()
+    e3 = Engine(p)
+    e3.load_state(state)
+    e3.run(24, 24)
+    got = e3.download()
+    e3.close()
+    for x, y in zip(ref, got):
+        assert np.array_equal(x, y)
+    q, qpos, qvel = system("small", prec)
+    e4 = Engine(q)
+    with pytest.raises(MDError):
+        e4.load_state(state)
+    e4.close()

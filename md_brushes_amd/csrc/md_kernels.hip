@@ -93,6 +93,13 @@ __device__ __forceinline__ int lq_find(const int *start, int n, int g) {
   return lo;
 }
 
+// x or y minimum image of rij_and_rr (glob_var.f90:72-105): d - L above L/2,
+// d + L at or below -L/2, as selects (no divergent branch in a wavefront).
+template <typename R> __device__ __forceinline__ R image(R d, R L, R hL) {
+  const R lo = d - L, hi = d + L;
+  return d > hL ? lo : (d <= -hL ? hi : d);
+}
+
 // Squared distance with the x,y minimum image of rij_and_rr
 // (glob_var.f90:72-105), rounded term by term as the reference writes it.
 __device__ __forceinline__ double rsq_image(double4 a, double4 b, double Lx, double Ly,
@@ -102,10 +109,9 @@ __device__ __forceinline__ double rsq_image(double4 a, double4 b, double Lx, dou
   dx = a.x - b.x;
   dy = a.y - b.y;
   dz = a.z - b.z;
-  if (dx > hLx) dx = dx - Lx;
-  else if (dx <= -hLx) dx = dx + Lx;
-  if (dy > hLy) dy = dy - Ly;
-  else if (dy <= -hLy) dy = dy + Ly;
+  const double xl = dx - Lx, xh = dx + Lx, yl = dy - Ly, yh = dy + Ly;
+  dx = dx > hLx ? xl : (dx <= -hLx ? xh : dx);
+  dy = dy > hLy ? yl : (dy <= -hLy ? yh : dy);
   return dx * dx + dy * dy + dz * dz;
 }
 
@@ -311,8 +317,8 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
         xj = p.x; yj = p.y; zj = p.z; qj = p.w;
       }
       R dx = xi - xj, dy = yi - yj, dz = zi - zj;
-      if (dx > hLx) dx -= Lx; else if (dx <= -hLx) dx += Lx;
-      if (dy > hLy) dy -= Ly; else if (dy <= -hLy) dy += Ly;
+      dx = image(dx, Lx, hLx);
+      dy = image(dy, Ly, hLy);
       const R rsq = dx * dx + dy * dy + dz * dz;
       const int m = (int)(qa + (sizeof(R) == 4 ? fabsf((float)qj) : fabs((double)qj)));
       R cut, s2;
@@ -320,9 +326,14 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
       else if (m < 3) { cut = rcl2; s2 = (R)1; }
       else { cut = rcl4; s2 = sig3; }
       if (rsq < cut) {
-        const R ir2 = s2 / rsq;
-        const R ir6 = ir2 * ir2 * ir2;
-        const R gg = ir6 * (ir6 - (R)0.5) / rsq;
+        R gg;
+        if constexpr (sizeof(R) == 4) {  // one v_rcp_f32 (about 1 ulp) for both divisions by r^2
+          const float ir = __builtin_amdgcn_rcpf(rsq), ir2 = s2 * ir, ir6 = ir2 * ir2 * ir2;
+          gg = ir6 * (ir6 - 0.5f) * ir;
+        } else {
+          const R ir2 = s2 / rsq, ir6 = ir2 * ir2 * ir2;
+          gg = ir6 * (ir6 - (R)0.5) / rsq;
+        }
         fx += gg * dx;
         fy += gg * dy;
         fz += gg * dz;
@@ -1535,8 +1546,8 @@ __device__ __forceinline__ void real_pair(R xi, R yi, R zi, R qi, R xj, R yj, R 
                                           R hLx, R hLy, R rcc2, R alpha, R alpha2, R c2, R &fx, R &fy,
                                           R &fz) {
   R dx = xi - xj, dy = yi - yj, dz = zi - zj;
-  if (dx > hLx) dx -= Lx; else if (dx <= -hLx) dx += Lx;
-  if (dy > hLy) dy -= Ly; else if (dy <= -hLy) dy += Ly;
+  dx = image(dx, Lx, hLx);
+  dy = image(dy, Ly, hLy);
   const R rsq = dx * dx + dy * dy + dz * dz;
   if (rsq < rcc2) {
     const R rr = sqrt(rsq);
@@ -1549,7 +1560,9 @@ __device__ __forceinline__ void real_pair(R xi, R yi, R zi, R qi, R xj, R yj, R 
 
 // FP32 form: erfc by Abramowitz & Stegun 7.1.26, t = 1/(1 + p x),
 // erfc(x) = t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-x^2), absolute
-// error <= 1.5e-7, which shares exp(-a^2 r^2) with the second term.
+// error <= 1.5e-7, which shares exp(-a^2 r^2) with the second term.  1/r,
+// 1/(1 + p x) and exp2 are the hardware's single-instruction forms
+// (v_rsq_f32, v_rcp_f32, v_exp_f32: about 1 ulp each); 1/r^2 = (1/r)^2.
 template <>
 __device__ __forceinline__ void real_pair<float>(float xi, float yi, float zi, float qi, float xj,
                                                  float yj, float zj, float qj, float Lx, float Ly,
@@ -1557,17 +1570,18 @@ __device__ __forceinline__ void real_pair<float>(float xi, float yi, float zi, f
                                                  float alpha2, float c2, float &fx, float &fy,
                                                  float &fz) {
   float dx = xi - xj, dy = yi - yj, dz = zi - zj;
-  if (dx > hLx) dx -= Lx; else if (dx <= -hLx) dx += Lx;
-  if (dy > hLy) dy -= Ly; else if (dy <= -hLy) dy += Ly;
+  dx = image(dx, Lx, hLx);
+  dy = image(dy, Ly, hLy);
   const float rsq = dx * dx + dy * dy + dz * dz;
   if (rsq < rcc2) {
-    const float rr = sqrtf(rsq);
-    const float e = expf(-alpha2 * rsq);
-    const float t = 1.0f / (1.0f + 0.3275911f * alpha * rr);
+    const float ir = __builtin_amdgcn_rsqf(rsq);
+    const float rr = rsq * ir;
+    const float e = __builtin_amdgcn_exp2f(-(alpha2 * 1.44269504f) * rsq);
+    const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * alpha * rr);
     const float poly =
         t * (0.254829592f +
              t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
-    const float g = qi * qj / rsq * e * (poly / rr + c2);
+    const float g = qi * qj * (ir * ir) * e * (poly * ir + c2);
     fx += g * dx;
     fy += g * dy;
     fz += g * dz;
@@ -1717,8 +1731,8 @@ __device__ __forceinline__ void target_tile(const V pi, const V *cand, bool wrap
       R dx = pi.x - pj[u].x, dy = pi.y - pj[u].y;
       const R dz = pi.z - pj[u].z;
       if (wrap) {
-        if (dx > hLx) dx -= Lx; else if (dx <= -hLx) dx += Lx;
-        if (dy > hLy) dy -= Ly; else if (dy <= -hLy) dy += Ly;
+        dx = image(dx, Lx, hLx);
+        dy = image(dy, Ly, hLy);
       }
       in[u] = k < nc && k != self && dx * dx + dy * dy + dz * dz < rin2;
     }

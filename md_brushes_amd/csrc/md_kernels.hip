@@ -124,6 +124,47 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Value of v in lane l (l uniform over the wavefront).
+__device__ __forceinline__ float lane_value(float v, int l) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ __forceinline__ double lane_value(double v, int l) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Lane moves of the DPP modifier (quad permutes, row mirrors, row
+// broadcasts); lanes of rows outside row_mask RM read 0.
+template <int CTRL, int RM>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __builtin_bit_cast(float,
+                            __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, RM, 0xf, false));
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ double dpp_move(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, RM, 0xf, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, RM, 0xf, false);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Sum over the 64 lanes in a fixed order, every lane taking the total: pair
+// and quad sums (quad permutes), the 8 and 16 lanes of a row (half-row and
+// row mirrors), then rows 0+1 and 2+3 (lane 15 broadcast into rows 1, 3)
+// and all four (lane 31 broadcast into rows 2, 3); lane 63 holds the total.
+// DPP moves stay in the vector ALU, where a shuffle takes an LDS round trip.
+template <typename T> __device__ __forceinline__ T wave_total(T v) {
+  v += dpp_move<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
+  v += dpp_move<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
+  v += dpp_move<0x141, 0xf>(v);  // row_half_mirror
+  v += dpp_move<0x140, 0xf>(v);  // row_mirror
+  v += dpp_move<0x142, 0xa>(v);  // row_bcast:15
+  v += dpp_move<0x143, 0xc>(v);  // row_bcast:31
+  return lane_value(v, 63);
+}
+
 // Orders LDS accesses of the lanes of one wavefront (LDS operations of a
 // wavefront complete in order; the fences keep the compiler from moving
 // memory accesses across).
@@ -1701,7 +1742,7 @@ __device__ __forceinline__ bool interior_cell(int a, int b, const int nc[3]) {
 // Force on one target from a staged tile of nc candidates (candidate `self`
 // is the target itself), by one wavefront: the candidates inside r_c are
 // compacted in candidate order into hit[], the lanes stride over them, and
-// a fixed shuffle tree sums the lanes.  wrap = false for a cell whose 27
+// a fixed DPP tree sums the lanes (wave_total).  wrap = false for a cell whose 27
 // stencil cells are all direct neighbours in x and y (with three or more
 // cells along each, edge >= r_c): no in-range pair then needs the minimum
 // image (|dx| < r_c <= L/3), and no out-of-range pair comes within r_c
@@ -1751,11 +1792,9 @@ __device__ __forceinline__ void target_tile(const V pi, const V *cand, bool wrap
                  alpha2, c2, fx, fy, fz);
   }
   wave_sync();  // hit[] is refilled by the next target
-  for (int o = 32; o > 0; o >>= 1) {
-    fx += __shfl_xor(fx, o, 64);
-    fy += __shfl_xor(fy, o, 64);
-    fz += __shfl_xor(fz, o, 64);
-  }
+  fx = wave_total(fx);
+  fy = wave_total(fy);
+  fz = wave_total(fz);
 }
 
 // The sweep form for engines with few charges: a work item is a chunk of
@@ -1945,10 +1984,10 @@ __device__ __forceinline__ void real_cells_block(const Rep *__restrict__ reps, W
         wave_sync();
         for (int tg = 0; tg < nt; tg++) {
           V pi;
-          pi.x = __shfl(pt.x, tg, 64);
-          pi.y = __shfl(pt.y, tg, 64);
-          pi.z = __shfl(pt.z, tg, 64);
-          pi.w = __shfl(pt.w, tg, 64);
+          pi.x = lane_value(pt.x, tg);
+          pi.y = lane_value(pt.y, tg);
+          pi.z = lane_value(pt.z, tg);
+          pi.w = lane_value(pt.w, tg);
           R fx, fy, fz;
           target_tile<R>(pi, cand, wrap, nc, self0 + tg - base, hit, Lx, Ly, hLx, hLy, rcc2, alpha,
                          alpha2, c2, fx, fy, fz);

@@ -1581,14 +1581,17 @@ __global__ __launch_bounds__(kBlock) void k_coul_store(const Rep *__restrict__ r
 
 // erfc-screened Coulomb pair force of real_space (compt_acc.f90:1034-1036)
 // without the lb/Beta factor: q_i q_j / r^2 (erfc(a r)/r + 2a/sqrt(pi)
-// exp(-a^2 r^2)) r_ij, for r < r_c.
+// exp(-a^2 r^2)) r_ij, for r < r_c; wrap applies the x,y minimum image
+// (false for an image-shifted j).
 template <typename R>
-__device__ __forceinline__ void real_pair(R xi, R yi, R zi, R qi, R xj, R yj, R zj, R qj, R Lx, R Ly,
-                                          R hLx, R hLy, R rcc2, R alpha, R alpha2, R c2, R &fx, R &fy,
-                                          R &fz) {
+__device__ __forceinline__ void real_pair(R xi, R yi, R zi, R qi, R xj, R yj, R zj, R qj, bool wrap,
+                                          R Lx, R Ly, R hLx, R hLy, R rcc2, R alpha, R alpha2, R c2,
+                                          R &fx, R &fy, R &fz) {
   R dx = xi - xj, dy = yi - yj, dz = zi - zj;
-  dx = image(dx, Lx, hLx);
-  dy = image(dy, Ly, hLy);
+  if (wrap) {
+    dx = image(dx, Lx, hLx);
+    dy = image(dy, Ly, hLy);
+  }
   const R rsq = dx * dx + dy * dy + dz * dz;
   if (rsq < rcc2) {
     const R rr = sqrt(rsq);
@@ -1606,13 +1609,15 @@ __device__ __forceinline__ void real_pair(R xi, R yi, R zi, R qi, R xj, R yj, R 
 // (v_rsq_f32, v_rcp_f32, v_exp_f32: about 1 ulp each); 1/r^2 = (1/r)^2.
 template <>
 __device__ __forceinline__ void real_pair<float>(float xi, float yi, float zi, float qi, float xj,
-                                                 float yj, float zj, float qj, float Lx, float Ly,
-                                                 float hLx, float hLy, float rcc2, float alpha,
-                                                 float alpha2, float c2, float &fx, float &fy,
-                                                 float &fz) {
+                                                 float yj, float zj, float qj, bool wrap, float Lx,
+                                                 float Ly, float hLx, float hLy, float rcc2,
+                                                 float alpha, float alpha2, float c2, float &fx,
+                                                 float &fy, float &fz) {
   float dx = xi - xj, dy = yi - yj, dz = zi - zj;
-  dx = image(dx, Lx, hLx);
-  dy = image(dy, Ly, hLy);
+  if (wrap) {
+    dx = image(dx, Lx, hLx);
+    dy = image(dy, Ly, hLy);
+  }
   const float rsq = dx * dx + dy * dy + dz * dz;
   if (rsq < rcc2) {
     const float ir = __builtin_amdgcn_rsqf(rsq);
@@ -1734,20 +1739,34 @@ __global__ __launch_bounds__(kBlock) void k_sort_q(const Rep *__restrict__ reps,
 template <typename R> struct RealTile { static constexpr int N = 256; };
 template <> struct RealTile<float> { static constexpr int N = 512; };
 
-// Whether the 27 stencil cells of cell (a, b) are all direct x,y neighbours.
-__device__ __forceinline__ bool interior_cell(int a, int b, const int nc[3]) {
-  return nc[0] >= 3 && nc[1] >= 3 && a >= 1 && a <= nc[0] - 2 && b >= 1 && b <= nc[1] - 2;
+// Periodic image of stencil slot l's members for a target in cell (a, b),
+// with three or more cells of edge >= r_c along x and y: the x, y offset
+// that places them beside the target.  Each stencil cell is then a distinct
+// direct neighbour, and the shifted copy gives every pair within r_c its
+// minimum-image separation (|dx| < r_c <= L/3; any other image is at least
+// L - 2e >= e >= r_c away for cell edge e <= L/3), while a pair the minimum
+// image puts at r_c or beyond stays there.  The sweep stages shifted
+// candidates and tests them without the minimum image.
+template <typename R>
+__device__ __forceinline__ void slot_shift(int l, int a, int b, const int nc[3], R Lx, R Ly, R &sx,
+                                           R &sy) {
+  const int p = a + l / 9 - 1, q = b + (l / 3) % 3 - 1;
+  sx = p < 0 ? -Lx : (p >= nc[0] ? Lx : (R)0);
+  sy = q < 0 ? -Ly : (q >= nc[1] ? Ly : (R)0);
 }
+
+// Whether the sweep stages image-shifted candidates (slot_shift) for a
+// replica: three or more real-space cells along x and y.
+__device__ __forceinline__ bool shifted_stencil(const int nc[3]) { return nc[0] >= 3 && nc[1] >= 3; }
 
 // Force on one target from a staged tile of nc candidates (candidate `self`
 // is the target itself), by one wavefront: the candidates inside r_c are
 // compacted in candidate order into hit[], the lanes stride over them, and
-// a fixed DPP tree sums the lanes (wave_total).  wrap = false for a cell whose 27
-// stencil cells are all direct neighbours in x and y (with three or more
-// cells along each, edge >= r_c): no in-range pair then needs the minimum
-// image (|dx| < r_c <= L/3), and no out-of-range pair comes within r_c
-// through it (|dx| <= 2 L/3), so the compaction test skips it.  Both forms of the sweep use it with
-// the same tiles, so a target's sum does not depend on the form.
+// a fixed DPP tree sums the lanes (wave_total).  wrap = false for a tile
+// staged image-shifted (slot_shift): the tests and pair forces then take the
+// separations as they stand; wrap = true (fewer than three cells along x or
+// y) applies the minimum image.  Both forms of the sweep use it with the
+// same tiles, so a target's sum does not depend on the form.
 template <typename R, typename V>
 __device__ __forceinline__ void target_tile(const V pi, const V *cand, bool wrap, int nc, int self,
                                             short *hit, R Lx, R Ly, R hLx, R hLy, R rcc2, R alpha,
@@ -1788,7 +1807,7 @@ __device__ __forceinline__ void target_tile(const V pi, const V *cand, bool wrap
   fx = fy = fz = 0;
   for (int h = lane; h < nh; h += 64) {
     const V pj = cand[hit[h]];
-    real_pair<R>(pi.x, pi.y, pi.z, pi.w, pj.x, pj.y, pj.z, pj.w, Lx, Ly, hLx, hLy, rcc2, alpha,
+    real_pair<R>(pi.x, pi.y, pi.z, pi.w, pj.x, pj.y, pj.z, pj.w, wrap, Lx, Ly, hLx, hLy, rcc2, alpha,
                  alpha2, c2, fx, fy, fz);
   }
   wave_sync();  // hit[] is refilled by the next target
@@ -1817,6 +1836,7 @@ __device__ void real_cells_wg(const Rep &S, int b0, int nb, V *cand, short *hit_
   short *hit = hit_all + w * TN;
   const V *ps = (const V *)S.pos3s;
   const R Lx = (R)S.Lx, Ly = (R)S.Ly, hLx = (R)S.hLx, hLy = (R)S.hLy, rcc2 = (R)S.rcc2;
+  const bool wrap = !shifted_stencil(S.nc3);  // else the tiles are staged image-shifted
   const R alpha = (R)S.alpha, alpha2 = (R)S.alpha2, c2 = (R)S.c2;
   // the workgroup's items are b0 + j nb; this launch takes those with
   // j = part mod nparts, kBlock at a time, the non-empty ones compacted in order
@@ -1842,7 +1862,7 @@ __device__ void real_cells_wg(const Rep &S, int b0, int nb, V *cand, short *hit_
       const int it = work[q];
       const int cell = it % ncl, t0 = (it / ncl) * kRealChunkWG;
       const int nt = min(S.scnt3[cell] - t0, kRealChunkWG);
-      const bool wrap = !interior_cell(cell % S.nc3[0], (cell / S.nc3[0]) % S.nc3[1], S.nc3);
+      const int ca = cell % S.nc3[0], cb = (cell / S.nc3[0]) % S.nc3[1];
       const int total = stencil_offsets(cell, S.nc3, S.scnt3, offs, cids);
       // stencil slot 13 is the cell itself: candidate offs[13] + t0 + tg is target tg
       const int self0 = offs[13] + t0;
@@ -1852,7 +1872,14 @@ __device__ void real_cells_wg(const Rep &S, int b0, int nb, V *cand, short *hit_
         const int nc = min(TN, total - base);
         for (int k = threadIdx.x; k < nc; k += kBlock) {
           const int t = base + k, l = stencil_slot(offs, t);
-          cand[k] = ps[(size_t)cids[l] * S.ccap3 + (t - offs[l])];
+          V c = ps[(size_t)cids[l] * S.ccap3 + (t - offs[l])];
+          if (!wrap) {
+            R sx, sy;
+            slot_shift(l, ca, cb, S.nc3, Lx, Ly, sx, sy);
+            c.x += sx;
+            c.y += sy;
+          }
+          cand[k] = c;
         }
         __syncthreads();
         for (int tg = w; tg < nt; tg += NW) {
@@ -1882,9 +1909,9 @@ __device__ void real_cells_wg(const Rep &S, int b0, int nb, V *cand, short *hit_
 // whose items are too few to fill the chip, take real_cells_wg instead).  The members of the
 // cell's 27 neighbour cells (reference stencil order, ascending within a
 // cell) are staged in the wavefront's LDS tile from the cell-sorted position
-// copy; for each target the candidates inside r_c are compacted first, the
-// 64 lanes stride over those and a shuffle reduction (fixed order) sums the
-// lanes; lane tg keeps target tg's
+// copy, image-shifted (slot_shift); for each target the candidates inside
+// r_c are compacted first, the 64 lanes stride over those and a DPP tree
+// (fixed order) sums the lanes; lane tg keeps target tg's
 // sum over the tiles and the lanes store their targets together.
 // Wavefronts stride over the items and skip the empty ones 64 at a time.
 template <typename R>
@@ -1916,6 +1943,7 @@ __device__ __forceinline__ void real_cells_block(const Rep *__restrict__ reps, W
   short *hit = hit_all + w * TN;
   const V *ps = (const V *)S.pos3s;
   const R Lx = (R)S.Lx, Ly = (R)S.Ly, hLx = (R)S.hLx, hLy = (R)S.hLy, rcc2 = (R)S.rcc2;
+  const bool wrap = !shifted_stencil(S.nc3);  // else the tiles are staged image-shifted
   const R alpha = (R)S.alpha, alpha2 = (R)S.alpha2, c2 = (R)S.c2;
   if constexpr (WG) {
     real_cells_wg<R>(S, gb - wm.blk[r], wm.blk[r + 1] - wm.blk[r], cand_all, hit_all, part, nparts);
@@ -1940,7 +1968,6 @@ __device__ __forceinline__ void real_cells_block(const Rep *__restrict__ reps, W
       const int nt = min(S.scnt3[cell] - t0, kRealChunk);
       const int ca = cell % S.nc3[0], cb = (cell / S.nc3[0]) % S.nc3[1],
                 cc = cell / (S.nc3[0] * S.nc3[1]);
-      const bool wrap = !interior_cell(ca, cb, S.nc3);
       // stencil cell of lane l < 27, its count and exclusive offset
       int cl = -1, cn = 0;
       if (lane < 27) {
@@ -1965,6 +1992,7 @@ __device__ __forceinline__ void real_cells_block(const Rep *__restrict__ reps, W
         // stage the tile: TN/64 candidates per lane, loads issued together
         constexpr int U = TN / 64;
         size_t src[U];
+        int sl[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
           const int t = base + u * 64 + lane;
@@ -1977,10 +2005,20 @@ __device__ __forceinline__ void real_cells_block(const Rep *__restrict__ reps, W
           }
           const int lc = __shfl(cl, l, 64), lx = __shfl(excl, l, 64);
           src[u] = (size_t)lc * S.ccap3 + (t - lx);
+          sl[u] = l;
         }
 #pragma unroll
         for (int u = 0; u < U; u++)
-          if (u * 64 + lane < nc) cand[u * 64 + lane] = ps[src[u]];
+          if (u * 64 + lane < nc) {
+            V c = ps[src[u]];
+            if (!wrap) {
+              R sx, sy;
+              slot_shift(sl[u], ca, cb, S.nc3, Lx, Ly, sx, sy);
+              c.x += sx;
+              c.y += sy;
+            }
+            cand[u * 64 + lane] = c;
+          }
         wave_sync();
         for (int tg = 0; tg < nt; tg++) {
           V pi;

@@ -1738,6 +1738,8 @@ __global__ __launch_bounds__(kBlock) void k_sort_q(const Rep *__restrict__ reps,
 // Candidates staged in LDS per pass of a k_real_cells wavefront.
 template <typename R> struct RealTile { static constexpr int N = 256; };
 template <> struct RealTile<float> { static constexpr int N = 512; };
+static_assert(RealTile<float>::N % kRealRound == 0 && RealTile<double>::N % kRealRound == 0,
+              "tiles are padded to whole compaction rounds");
 
 // Periodic image of stencil slot l's members for a target in cell (a, b),
 // with three or more cells of edge >= r_c along x and y: the x, y offset
@@ -1759,10 +1761,20 @@ __device__ __forceinline__ void slot_shift(int l, int a, int b, const int nc[3],
 // replica: three or more real-space cells along x and y.
 __device__ __forceinline__ bool shifted_stencil(const int nc[3]) { return nc[0] >= 3 && nc[1] >= 3; }
 
+// Sentinel of the real-space tiles past their last candidate: far beyond
+// r_c of any target (its squared distance stays finite), charge 0.
+template <typename V> __device__ __forceinline__ V tile_pad() {
+  using R = decltype(V{}.x);
+  constexpr R far = sizeof(R) == 4 ? (R)1e18 : (R)1e150;
+  return V{far, far, far, (R)0};
+}
+
 // Force on one target from a staged tile of nc candidates (candidate `self`
 // is the target itself), by one wavefront: the candidates inside r_c are
 // compacted in candidate order into hit[], the lanes stride over them, and
-// a fixed DPP tree sums the lanes (wave_total).  wrap = false for a tile
+// a fixed DPP tree sums the lanes (wave_total).  The tile is padded with
+// far-away sentinels (tile_pad) up to the next multiple of kRealRound
+// candidates.  wrap = false for a tile
 // staged image-shifted (slot_shift): the tests and pair forces then take the
 // separations as they stand; wrap = true (fewer than three cells along x or
 // y) applies the minimum image.  Both forms of the sweep use it with the
@@ -1778,12 +1790,11 @@ __device__ __forceinline__ void target_tile(const V pi, const V *cand, bool wrap
   int nh = 0;
   // four rounds of 64 candidates at a time: their LDS loads are issued
   // together; the hits are appended round by round, in candidate order
-  constexpr int U = 4;
-  const unsigned long long below = (1ull << lane) - 1ull;
-  for (int k0 = 0; k0 < nc; k0 += 64 * U) {
+  constexpr int U = kRealRound / 64;
+  for (int k0 = 0; k0 < nc; k0 += kRealRound) {
     V pj[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) pj[u] = cand[min(k0 + u * 64 + lane, nc - 1)];
+    for (int u = 0; u < U; u++) pj[u] = cand[k0 + u * 64 + lane];
     bool in[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
@@ -1794,12 +1805,15 @@ __device__ __forceinline__ void target_tile(const V pi, const V *cand, bool wrap
         dx = image(dx, Lx, hLx);
         dy = image(dy, Ly, hLy);
       }
-      in[u] = k < nc && k != self && dx * dx + dy * dy + dz * dz < rin2;
+      in[u] = k != self && dx * dx + dy * dy + dz * dz < rin2;
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const unsigned long long mk = __ballot(in[u]);
-      if (in[u]) hit[nh + __popcll(mk & below)] = (short)(k0 + u * 64 + lane);
+      const unsigned long long mk = __builtin_amdgcn_ballot_w64(in[u]);
+      // hits of lower lanes: v_mbcnt over the ballot
+      const int below = __builtin_amdgcn_mbcnt_hi((unsigned)(mk >> 32),
+                                                  __builtin_amdgcn_mbcnt_lo((unsigned)mk, 0u));
+      if (in[u]) hit[nh + below] = (short)(k0 + u * 64 + lane);
       nh += __popcll(mk);
     }
   }
@@ -1870,14 +1884,18 @@ __device__ void real_cells_wg(const Rep &S, int b0, int nb, V *cand, short *hit_
       if (threadIdx.x < 3 * kRealChunkWG) accs[threadIdx.x / 3][threadIdx.x % 3] = 0.0;
       for (int base = 0; base < total; base += TN) {
         const int nc = min(TN, total - base);
-        for (int k = threadIdx.x; k < nc; k += kBlock) {
-          const int t = base + k, l = stencil_slot(offs, t);
-          V c = ps[(size_t)cids[l] * S.ccap3 + (t - offs[l])];
-          if (!wrap) {
-            R sx, sy;
-            slot_shift(l, ca, cb, S.nc3, Lx, Ly, sx, sy);
-            c.x += sx;
-            c.y += sy;
+        const int ncp = (nc + kRealRound - 1) / kRealRound * kRealRound;
+        for (int k = threadIdx.x; k < ncp; k += kBlock) {
+          V c = tile_pad<V>();
+          if (k < nc) {
+            const int t = base + k, l = stencil_slot(offs, t);
+            c = ps[(size_t)cids[l] * S.ccap3 + (t - offs[l])];
+            if (!wrap) {
+              R sx, sy;
+              slot_shift(l, ca, cb, S.nc3, Lx, Ly, sx, sy);
+              c.x += sx;
+              c.y += sy;
+            }
           }
           cand[k] = c;
         }
@@ -2008,17 +2026,19 @@ __device__ __forceinline__ void real_cells_block(const Rep *__restrict__ reps, W
           sl[u] = l;
         }
 #pragma unroll
-        for (int u = 0; u < U; u++)
+        for (int u = 0; u < U; u++) {  // the whole tile: candidates, then sentinels
+          V c = tile_pad<V>();
           if (u * 64 + lane < nc) {
-            V c = ps[src[u]];
+            c = ps[src[u]];
             if (!wrap) {
               R sx, sy;
               slot_shift(sl[u], ca, cb, S.nc3, Lx, Ly, sx, sy);
               c.x += sx;
               c.y += sy;
             }
-            cand[u * 64 + lane] = c;
           }
+          cand[u * 64 + lane] = c;
+        }
         wave_sync();
         for (int tg = 0; tg < nt; tg++) {
           V pi;

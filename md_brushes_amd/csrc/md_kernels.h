@@ -36,6 +36,8 @@ struct Plan {
   unsigned long long *lq_nx = nullptr;  // the queue of the other position buffer's steps
   int lq_cap = 0;
   int grid_bin = 1, grid_build = 1;
+  int lists_grid = 0;       // k_lists grid (0: the two list kernels)
+  unsigned *bar = nullptr;  // k_lists grid barrier counter
   double *vpart = nullptr;  // partial sums of rescale_velocity
   bool fp32 = false;
   bool real_wg = false;     // real-space sweep form of the engine (workgroup per item)
@@ -55,6 +57,7 @@ void launch_short(const Plan &P, hipStream_t st, int mode, int step, const doubl
 void launch_kick_drift(const Plan &P, hipStream_t st, int store_v = 1);
 // clear_queue: empty the rebuild queue afterwards (list updates outside the
 // replayed step cycle, where the next kick does it)
+int lists_grid();
 void launch_lists(const Plan &P, hipStream_t st, int inc_step, hipEvent_t *ev = nullptr,
                   bool clear_queue = false);
 void launch_real_list(const Plan &P, hipStream_t st);

@@ -595,8 +595,8 @@ __device__ __forceinline__ int stencil_cell(int l, int a, int b, int c, const in
 // Binning pass (compt_acc.f90:219-225): cell of each particle, appended to
 // the cell's members (order fixed by the rank pass).  Also clears the
 // other count buffer at the particle's previous cell, for the next build.
-__global__ __launch_bounds__(kBlock) void k_bin_lj(const Rep *__restrict__ reps,
-                                                   unsigned long long *lq, int cap) {
+__device__ __forceinline__ void bin_lj_pass(const Rep *__restrict__ reps, unsigned long long *lq,
+                                            int cap) {
   const unsigned long long head = *lq;
   const int ne = min((int)(head >> kQShift), cap);
   const int total = (int)(head & ((1ull << kQShift) - 1));
@@ -631,6 +631,11 @@ __global__ __launch_bounds__(kBlock) void k_bin_lj(const Rep *__restrict__ reps,
       if (slot >= S.ctrl->max_cell1) atomicMax(&S.ctrl->max_cell1, slot + 1);
     }
   }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bin_lj(const Rep *__restrict__ reps,
+                                                   unsigned long long *lq, int cap) {
+  bin_lj_pass(reps, lq, cap);
 }
 
 // Shared 27-cell candidate set of one cell: offs[l] = first candidate of
@@ -678,11 +683,8 @@ __device__ __forceinline__ int stencil_slot(const int *offs, int t) {
 // Pairs: i != j and r^2 < rvl^2, r^2 rounded as rij_and_rr rounds it.  The
 // list is ELL: slot k of particle i at k*NN + i.  The last kernel of a step
 // also advances every replica's step counter.
-__global__ __launch_bounds__(kBlock) void k_build_lj(const Rep *__restrict__ reps, int nrep,
-                                                     unsigned long long *lq, int cap,
-                                                     int inc_step) {
-  if (inc_step && blockIdx.x == 0)
-    for (int r = threadIdx.x; r < nrep; r += blockDim.x) reps[r].ctrl->step++;
+__device__ __forceinline__ void build_lj_pass(const Rep *__restrict__ reps, unsigned long long *lq,
+                                              int cap, int (*stage_all)[kLJStage]) {
   const unsigned long long head = *lq;
   const int ne = min((int)(head >> kQShift), cap);
   const int total = (int)(head & ((1ull << kQShift) - 1));
@@ -691,7 +693,6 @@ __global__ __launch_bounds__(kBlock) void k_build_lj(const Rep *__restrict__ rep
   const unsigned long long gmask = half ? 0xffffffff00000000ull : 0xffffffffull;
   const unsigned long long below = gmask & ((1ull << lane) - 1ull);
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  __shared__ int stage_all[kBlock / 32][kLJStage];
   int *stage = stage_all[threadIdx.x >> 5];
   const int stride = gridDim.x * blockDim.x / 32;
   for (int t0 = 2 * wave; t0 < total; t0 += stride) {
@@ -789,6 +790,92 @@ __global__ __launch_bounds__(kBlock) void k_build_lj(const Rep *__restrict__ rep
       if (n > S.ctrl->max_nb1) atomicMax(&S.ctrl->max_nb1, n);
     }
   }
+}
+
+__global__ __launch_bounds__(kBlock) void k_build_lj(const Rep *__restrict__ reps, int nrep,
+                                                     unsigned long long *lq, int cap,
+                                                     int inc_step) {
+  if (inc_step && blockIdx.x == 0)
+    for (int r = threadIdx.x; r < nrep; r += blockDim.x) reps[r].ctrl->step++;
+  __shared__ int stage_all[kBlock / 32][kLJStage];
+  build_lj_pass(reps, lq, cap, stage_all);
+}
+
+// Barrier over the n workgroups of a launch that are all resident at once.
+// bar[0] counts this round's arrivals, bar[1] numbers the rounds: a
+// workgroup reads the round (acquire, so before its own arrival), arrives,
+// and waits for the round to change; the n-th arrival resets the count and
+// opens the next round.  So launches of any grid size share the counters,
+// and no workgroup leaves before all n have arrived.  Every thread's
+// device-scope release fence before the arrival and acquire fence after the
+// wait order the passes' global memory accesses across the XCDs.  The wait
+// is bounded (kSpinMax polls, a few milliseconds): a workgroup that gives up
+// marks the counters broken (bar[2]), and later launches skip the barrier
+// and report MD_ESYNC instead of waiting.
+constexpr int kSpinMax = 1 << 13;
+__device__ bool grid_barrier(unsigned *bar, unsigned n) {
+  __shared__ int ok;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned round = __hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned a = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    int spins = 0;
+    if (a == n - 1) {
+      __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(bar + 1, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      while (__hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == round &&
+             spins < kSpinMax) {
+        __builtin_amdgcn_s_sleep(1);
+        spins++;
+      }
+    }
+    ok = spins < kSpinMax;
+    if (!ok) __hip_atomic_store(bar + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return ok;
+}
+
+// Both passes of the LJ list update in one launch (the replayed step's list
+// kernel): an empty queue returns at once; otherwise the binning pass, a
+// grid barrier, then the fill pass, over the same queue as k_bin_lj and
+// k_build_lj.  The grid fits on the chip at once (lists_grid); a barrier that times out records MD_ESYNC on the queued
+// replicas, whose lists are then stale (the engine reports the error).
+__global__ __launch_bounds__(kBlock) void k_lists(const Rep *__restrict__ reps, int nrep,
+                                                  unsigned long long *lq, int cap, int inc_step,
+                                                  unsigned *bar) {
+  if (inc_step && blockIdx.x == 0)
+    for (int r = threadIdx.x; r < nrep; r += blockDim.x) reps[r].ctrl->step++;
+  const unsigned long long head = *lq;
+  if ((head & ((1ull << kQShift) - 1)) == 0) return;
+  const bool broken = __hip_atomic_load(bar + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+  if (!broken) bin_lj_pass(reps, lq, cap);
+  if (broken || !grid_barrier(bar, gridDim.x)) {
+    if (threadIdx.x == 0) {
+      const int ne = min((int)(head >> kQShift), cap);
+      for (int e = 0; e < ne; e++) set_error(reps[lq_rep(lq)[e]].ctrl, MD_ESYNC, blockIdx.x, -1);
+    }
+    return;
+  }
+  __shared__ int stage_all[kBlock / 32][kLJStage];
+  build_lj_pass(reps, lq, cap, stage_all);
+}
+
+// Largest power-of-two grid of k_lists that is resident at once using at
+// most half of the chip's workgroup slots for it (at most 1024), or 0.
+int lists_grid() {
+  int dev = 0, cus = 0, occ = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_lists, kBlock, 0) != hipSuccess) return 0;
+  const long cap = std::min<long>(1024, (long)cus * occ / 2);
+  if (cap < 1) return 0;
+  int g = 1;
+  while (2L * g <= cap) g *= 2;
+  return g;
 }
 
 // Empties a rebuild queue (after a list update outside the step cycle, whose
@@ -2381,6 +2468,12 @@ void launch_kick_drift(const Plan &P, hipStream_t st, int store_v) {
 // update_verlet_list's LJ rebuild (compt_acc.f90:717-720); every kernel
 // returns at once for replicas whose rebuild flag is clear.
 void launch_lists(const Plan &P, hipStream_t st, int inc_step, hipEvent_t *ev, bool clear_queue) {
+  if (!ev && P.lists_grid > 0 && P.bar) {  // one launch (timed per pass: two)
+    hipLaunchKernelGGL(k_lists, dim3(P.lists_grid), dim3(kBlock), 0, st, P.reps, P.nrep, P.lq,
+                       P.lq_cap, inc_step, P.bar);
+    if (clear_queue) hipLaunchKernelGGL(k_lq_clear, dim3(1), dim3(1), 0, st, P.lq);
+    return;
+  }
   if (ev) (void)hipEventRecord(ev[0], st);
   hipLaunchKernelGGL(k_bin_lj, dim3(P.grid_bin), dim3(kBlock), 0, st, P.reps, P.lq, P.lq_cap);
   if (ev) (void)hipEventRecord(ev[1], st);

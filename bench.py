@@ -118,11 +118,16 @@ COPY_BYTES = 256 << 20  # calibration copy for the HBM byte counters
 def pmc_workload(seed):
     """Kernels of the default brush (time_kernels launches each kernel of a
     step and of a Coulomb evaluation), then one device copy of COPY_BYTES,
-    whose counter readings give the scale from counter units to bytes."""
+    whose counter readings give the scale from counter units to bytes.  The
+    pre-heat is shortened to 500 steps: counter collection serialises every
+    dispatch, and the bytes per launch do not depend on how far the brush
+    has relaxed."""
     import torch
 
-    state = prepare(seed, 2000)
+    state = prepare(seed, 500)
+    print("pmc workload: pre-heat done", flush=True)
     p, e, _, _ = time_engine(MD_FP32, state, 100, 10, seed)
+    print("pmc workload: FP32 steps done", flush=True)
     e.time_kernels(10)
     e.close()
     a = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device="cuda").fill_(1.0)

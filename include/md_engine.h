@@ -37,8 +37,7 @@ enum {
   MD_EOVERFLOW = 5,     /* neighbour-list or cell capacity exceeded          */
   MD_EHIP = 6,          /* HIP runtime error                                 */
   MD_ENOMEM = 7,        /* device allocation failed                          */
-  MD_EIO = 8,           /* parameter file could not be read                  */
-  MD_ESYNC = 9          /* list update's workgroups not all resident (grid barrier timed out) */
+  MD_EIO = 8            /* parameter file could not be read                  */
 };
 
 enum { MD_FP64 = 0, MD_FP32 = 1 };

@@ -13,7 +13,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmd_engine.so")
 
-MD_OK, MD_EINVAL, MD_EBOND, MD_ECELL, MD_ENONFINITE, MD_EOVERFLOW, MD_EHIP, MD_ENOMEM, MD_EIO, MD_ESYNC = range(10)
+MD_OK, MD_EINVAL, MD_EBOND, MD_ECELL, MD_ENONFINITE, MD_EOVERFLOW, MD_EHIP, MD_ENOMEM, MD_EIO = range(9)
 MD_FP64, MD_FP32 = 0, 1
 MD_NOISE_NONE, MD_NOISE_ETA, MD_NOISE_PHILOX = 0, 1, 2
 MD_MAX_ARM = 16

@@ -28,6 +28,9 @@ constexpr int kHistBins = 500;  // SizeHist of the z-density profile (glob_var.f
 // Per-replica control block, written by kernels.
 struct Ctrl {
   double dr_max1, dr_max2;        // accumulated (compt_acc.f90:16,41)
+  double pend_d1, pend_d2;        // a replayed step's maximum displacements, applied
+                                  // to dr_max1/2 by the step's list kernel (pend != 0)
+  int pend;
   int flag1, flag2;               // rebuild LJ / real list after this step
   int gen1, gen2;                 // rebuild generations (count-buffer parity)
   int err;                        // first error code (MD_E*)
@@ -37,7 +40,6 @@ struct Ctrl {
   int max_nb1, max_nb2;           // largest list length seen at last build
   int max_cell1, max_cell2;       // largest cell occupancy seen at last build
   int step;                       // step counter of graph-replayed steps
-  int pad;
 };
 
 // Per-replica descriptor (read-only in kernels).

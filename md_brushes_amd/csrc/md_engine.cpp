@@ -682,8 +682,6 @@ struct md_engine {
   int maps_cap = 0;
   unsigned long long *d_lq = nullptr;  // LJ rebuild queues: the batch's, then one per replica
   size_t lq_words = 0;
-  unsigned *d_bar = nullptr;       // grid barrier counter of the one-launch list update
-  int lists_g = -1;                // its largest grid (0: two list kernels)
   double *vpart = nullptr;
   size_t vpart_cap = 0;
   int *h_steps = nullptr;          // pinned, one per replica
@@ -707,7 +705,6 @@ struct md_engine {
     if (d_reps) (void)hipFree(d_reps);
     if (d_maps) (void)hipFree(d_maps);
     if (d_lq) (void)hipFree(d_lq);
-    if (d_bar) (void)hipFree(d_bar);
     if (vpart) (void)hipFree(vpart);
     if (h_steps) (void)hipHostFree(h_steps);
     if (st2) (void)hipStreamSynchronize(st2);
@@ -876,28 +873,11 @@ int md_engine::rebuild() {
     if (hipMalloc(&d_lq, lq_words * 8) != hipSuccess) return fail(MD_ENOMEM, "rebuild queue");
   }
   e = hipMemset(d_lq, 0, lq_words * 8);
-  if (e == hipSuccess && !d_bar) {
-    e = hipMalloc(&d_bar, 256);
-    if (e == hipSuccess) e = hipMemset(d_bar, 0, 256);
-  }
   if (e == hipSuccess) e = null_stream_done();
   if (e != hipSuccess) return hip_fail(e, "rebuild queue");
-  if (lists_g < 0) {
-    // MD_FUSED_LISTS=1: the list update as one launch with a grid barrier
-    // (k_lists; the same lists).  Off by default: at the default brush the
-    // cycle ran at 33.7k steps/s with it against 43.3k with the two list
-    // kernels (the barrier's device-scope cache maintenance in every
-    // wavefront costs more than the launch it saves).
-    const char *fu = std::getenv("MD_FUSED_LISTS");
-    lists_g = fu && fu[0] == '1' ? mdk::lists_grid() : 0;
-  }
-  auto grids = [&](Plan &P, long nn) {
+  auto grids = [](Plan &P, long nn) {
     P.grid_bin = (int)std::max(1L, std::min<long>(cdiv(nn, mdk::kBlock), 2048));
     P.grid_build = (int)std::max(1L, std::min<long>(cdiv(nn, 8), 2048));
-    int g = 1;  // one-launch grid: a power of two, at most lists_g
-    while (g < std::max(P.grid_bin, P.grid_build) && 2 * g <= lists_g) g *= 2;
-    P.lists_grid = lists_g > 0 ? g : 0;
-    P.bar = d_bar;
   };
   auto fill = [&](Plan &P, Rep *reps, int nrep, const int *maps, const int *totals) {
     P.reps = reps;
@@ -988,10 +968,6 @@ int md_engine::check_one(const Sys *s, const Ctrl &c, std::string &msg) const {
                   c.err_step);
     msg = buf;
     return MD_ENONFINITE;
-  }
-  if (c.err == MD_ESYNC) {
-    msg = "list update: its workgroups were not all resident (grid barrier timed out)";
-    return MD_ESYNC;
   }
   if (c.ovf) {
     std::snprintf(buf, sizeof buf,
@@ -1087,7 +1063,7 @@ int md_engine::launch_compute_force(const Plan &P, Sys *s, int step, int noise, 
 // one kick-drift (run_cycles) that puts its first step's positions in place.
 static void launch_cycle(md_engine *e, int q) {
   const int mode = mdk::SR_HELD | mdk::SR_FIELD | mdk::SR_ANCHOR | mdk::SR_NOISE_PHILOX |
-                   mdk::SR_KICK | mdk::SR_DRMAX | mdk::SR_DRIFT;
+                   mdk::SR_KICK | mdk::SR_DRMAX | mdk::SR_DRIFT | mdk::SR_BIN;
   bool any_coulomb = false, any_preheat = false;
   for (Sys *s : e->sys) {
     any_coulomb |= s->coulomb;
@@ -1101,7 +1077,7 @@ static void launch_cycle(md_engine *e, int q) {
     }
     if (e->fp32) mdk::launch_short<float>(P, e->st, mode, -1, nullptr, nullptr);
     else mdk::launch_short<double>(P, e->st, mode, -1, nullptr, nullptr);
-    mdk::launch_lists(P, e->st, 1);
+    mdk::launch_build(P, e->st, 1);  // the short-range kernel binned for a rebuild
     // pre-heat rescale between steps (its step is a multiple of 20, so of
     // multistep when that divides 20; otherwise checked after every step);
     // it redoes the next step's drift with the rescaled velocities

@@ -14,9 +14,12 @@ enum {
   SR_ANCHOR = 16,        // zero the force on anchors
   SR_KICK = 32,          // second half kick of new_position
   SR_DRMAX = 64,         // dr_max bookkeeping and rebuild flags
-  SR_DRIFT = 128         // with SR_KICK, for replayed steps: v holds v(n-1) (the
+  SR_DRIFT = 128,        // with SR_KICK, for replayed steps: v holds v(n-1) (the
                          // first half kick is redone), and the next step's
                          // kick and drift go to the other position buffer
+  SR_BIN = 256           // with SR_DRMAX: every workgroup decides and bins its
+                         // particles for a rebuild (the step's list kernel is
+                         // then k_build_lj alone, which does the bookkeeping)
 };
 
 // Everything a launch needs: device descriptors and the block maps.
@@ -36,8 +39,6 @@ struct Plan {
   unsigned long long *lq_nx = nullptr;  // the queue of the other position buffer's steps
   int lq_cap = 0;
   int grid_bin = 1, grid_build = 1;
-  int lists_grid = 0;       // k_lists grid (0: the two list kernels)
-  unsigned *bar = nullptr;  // k_lists grid barrier counter
   double *vpart = nullptr;  // partial sums of rescale_velocity
   bool fp32 = false;
   bool real_wg = false;     // real-space sweep form of the engine (workgroup per item)
@@ -57,7 +58,7 @@ void launch_short(const Plan &P, hipStream_t st, int mode, int step, const doubl
 void launch_kick_drift(const Plan &P, hipStream_t st, int store_v = 1);
 // clear_queue: empty the rebuild queue afterwards (list updates outside the
 // replayed step cycle, where the next kick does it)
-int lists_grid();
+void launch_build(const Plan &P, hipStream_t st, int inc_step);
 void launch_lists(const Plan &P, hipStream_t st, int inc_step, hipEvent_t *ev = nullptr,
                   bool clear_queue = false);
 void launch_real_list(const Plan &P, hipStream_t st);

@@ -68,20 +68,25 @@ __device__ __forceinline__ int wave_append(int *ctr) {
 // the range [start, start + NN) each takes in the global particle index the
 // list kernels stride over.  One 64-bit head packs (entries << 40) | total,
 // so that one atomic hands out the entry and its range in the same order;
-// a spare 64-bit word, then rep[cap] and start[cap].
+// a spare 64-bit word, then rep[cap] and start[cap].  A rep word also
+// carries the cell-count buffer (bit 30) the replica's particles are binned
+// into, so that the list kernels need not read the rebuild generation.
 constexpr int kQShift = 40;
+constexpr int kQBuf = 1 << 30;
 __device__ __forceinline__ int *lq_rep(unsigned long long *lq) { return (int *)(lq + 2); }
 __device__ __forceinline__ int *lq_start(unsigned long long *lq, int cap) {
   return (int *)(lq + 2) + cap;
 }
-__device__ __forceinline__ void lq_push(unsigned long long *lq, int cap, int r, int nn) {
+__device__ __forceinline__ void lq_push(unsigned long long *lq, int cap, int r, int nn, int buf) {
   const unsigned long long old = atomicAdd(lq, (1ull << kQShift) + (unsigned long long)nn);
   const int e = (int)(old >> kQShift);
   if (e < cap) {
-    lq_rep(lq)[e] = r;
+    lq_rep(lq)[e] = r | (buf ? kQBuf : 0);
     lq_start(lq, cap)[e] = (int)(old & ((1ull << kQShift) - 1));
   }
 }
+__device__ __forceinline__ int lq_entry_rep(int w) { return w & (kQBuf - 1); }
+__device__ __forceinline__ int lq_entry_buf(int w) { return (w & kQBuf) ? 1 : 0; }
 // Entry of global particle index g: the largest e with start[e] <= g.
 __device__ __forceinline__ int lq_find(const int *start, int n, int g) {
   int lo = 0, hi = n - 1;
@@ -290,6 +295,8 @@ __device__ __forceinline__ void noise_normals(const Rep &S, int i, int step, dou
   n2 = r2 * cospi(2 * u3);
 }
 
+__device__ __forceinline__ void bin_lj_particle(const Rep *Sp, int i, int buf);
+
 // Compute_Force without the Coulomb evaluation (compt_acc.f90:847-861):
 // held Coulomb force + lj_force (compt_acc.f90:895-977) + fene_force
 // (compt_acc.f90:865-892) + noise + field, anchors zeroed; optionally the
@@ -457,7 +464,12 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
   } else {  // wave 7
     // the next step's rebuild queue (its list kernels of two steps back are done)
     if ((mode & SR_DRIFT) && blockIdx.x == 0 && lane == 0) *lq_nx = 0ull;
-    if ((mode & SR_DRMAX) && blk == 0) {  // this step's displacements, all groups
+    // SR_BIN: every workgroup takes the decision (the same inputs, the same
+    // result) and bins its own particles; the replica's first workgroup
+    // queues the rebuild and leaves the bookkeeping to the list kernel, so
+    // that no workgroup of this launch writes what the others read
+    const bool every = (mode & SR_BIN) != 0;
+    if ((mode & SR_DRMAX) && (every || blk == 0)) {  // this step's displacements, all groups
       double d1 = 0, d2 = 0;
       const double2 *dp = S.dpart + (size_t)S.par * S.ngrp;
       for (int g = lane; g < S.ngrp; g += 64) {
@@ -467,8 +479,22 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
       }
       d1 = wave_max(d1);
       d2 = wave_max(d2);
-      if (lane == 0) {
-        Ctrl *c = S.ctrl;
+      Ctrl *c = S.ctrl;
+      if (every) {
+        const double a1 = c->dr_max1 + d1, a2 = c->dr_max2 + d2;
+        const int f1 = a1 > S.skin1;
+        const int f2 = (a2 > S.skin2) && S.real_verlet != 0;
+        const int buf = (c->gen1 + 1) & 1;  // the next generation's count buffer
+        if (f1) bin_lj_particle(&S, live ? i : -1, buf);
+        if (blk == 0 && lane == 0) {
+          c->pend_d1 = d1;
+          c->pend_d2 = d2;
+          c->pend = 1;
+          c->flag1 = f1;
+          c->flag2 = f2;
+          if (f1) lq_push(lq, lq_cap, r, S.NN, buf);
+        }
+      } else if (lane == 0) {
         c->dr_max1 = c->dr_max1 + d1;
         c->dr_max2 = c->dr_max2 + d2;
         int f1 = c->dr_max1 > S.skin1;
@@ -476,7 +502,7 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
         if (f1) {
           c->dr_max1 = 0;
           c->gen1++;
-          lq_push(lq, lq_cap, r, S.NN);
+          lq_push(lq, lq_cap, r, S.NN, c->gen1 & 1);
         }
         if (f2) { c->dr_max2 = 0; c->gen2++; }
         c->flag1 = f1;
@@ -557,7 +583,7 @@ __global__ void k_set_flags(const Rep *__restrict__ reps, int nrep, int clear,
     }
     c->flag1 = 1;
     c->gen1++;
-    lq_push(lq, cap, r, reps[r].NN);
+    lq_push(lq, cap, r, reps[r].NN, c->gen1 & 1);
     c->flag2 = reps[r].real_verlet != 0 && reps[r].Nq > 0;
     if (c->flag2) c->gen2++;
   }
@@ -595,6 +621,36 @@ __device__ __forceinline__ int stencil_cell(int l, int a, int b, int c, const in
 // Binning pass (compt_acc.f90:219-225): cell of each particle, appended to
 // the cell's members (order fixed by the rank pass).  Also clears the
 // other count buffer at the particle's previous cell, for the next build.
+// Particle i of replica *Sp (i < 0: none) into count buffer buf; every lane
+// of the wavefront calls it.  Also clears the other buffer at the
+// particle's previous cell.
+__device__ __forceinline__ void bin_lj_particle(const Rep *Sp, int i, int buf) {
+  int *ctr = nullptr;
+  int cell = -1;
+  if (i >= 0) {
+    const Rep &S = *Sp;
+    const int ncell = S.nc1[0] * S.nc1[1] * S.nc1[2];
+    const int old = S.cell1_of[i];
+    if (old >= 0) S.cnt1[(size_t)(buf ^ 1) * ncell + old] = 0;
+    int a, b, cz;
+    if (cell_of(S.pos[i], S.nc1, S.rcel1, S.hLx, S.hLy, a, b, cz)) {
+      cell = (cz * S.nc1[1] + b) * S.nc1[0] + a;
+      ctr = S.cnt1 + (size_t)buf * ncell + cell;
+    } else {
+      set_error(S.ctrl, MD_ECELL, i, -1);
+    }
+    S.cell1_of[i] = cell;
+  }
+  const int slot = wave_append(ctr);
+  if (ctr) {
+    const Rep &S = *Sp;
+    if (slot < S.ccap1) S.mem1[(size_t)cell * S.ccap1 + slot] = i;
+    else atomicOr(&S.ctrl->ovf, 1);
+    // the largest occupancy seen (read first: one address for every particle)
+    if (slot >= S.ctrl->max_cell1) atomicMax(&S.ctrl->max_cell1, slot + 1);
+  }
+}
+
 __device__ __forceinline__ void bin_lj_pass(const Rep *__restrict__ reps, unsigned long long *lq,
                                             int cap) {
   const unsigned long long head = *lq;
@@ -603,33 +659,15 @@ __device__ __forceinline__ void bin_lj_pass(const Rep *__restrict__ reps, unsign
   const int *qrep = lq_rep(lq), *qst = lq_start(lq, cap);
   for (int base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
     const int g = base + threadIdx.x;
-    int *ctr = nullptr;
-    int e = 0, i = 0, cell = -1;
+    const Rep *Sp = reps;
+    int i = -1, buf = 0;
     if (g < total) {
-      e = lq_find(qst, ne, g);
-      const Rep &S = reps[qrep[e]];
+      const int e = lq_find(qst, ne, g);
+      Sp = reps + lq_entry_rep(qrep[e]);
+      buf = lq_entry_buf(qrep[e]);
       i = g - qst[e];
-      const int ncell = S.nc1[0] * S.nc1[1] * S.nc1[2];
-      const int par = S.ctrl->gen1 & 1;
-      const int old = S.cell1_of[i];
-      if (old >= 0) S.cnt1[(size_t)(par ^ 1) * ncell + old] = 0;
-      int a, b, cz;
-      if (cell_of(S.pos[i], S.nc1, S.rcel1, S.hLx, S.hLy, a, b, cz)) {
-        cell = (cz * S.nc1[1] + b) * S.nc1[0] + a;
-        ctr = S.cnt1 + (size_t)par * ncell + cell;
-      } else {
-        set_error(S.ctrl, MD_ECELL, i, -1);
-      }
-      S.cell1_of[i] = cell;
     }
-    const int slot = wave_append(ctr);
-    if (ctr) {
-      const Rep &S = reps[qrep[e]];
-      if (slot < S.ccap1) S.mem1[(size_t)cell * S.ccap1 + slot] = i;
-      else atomicOr(&S.ctrl->ovf, 1);
-      // the largest occupancy seen (read first: one address for every particle)
-      if (slot >= S.ctrl->max_cell1) atomicMax(&S.ctrl->max_cell1, slot + 1);
-    }
+    bin_lj_particle(Sp, i, buf);
   }
 }
 
@@ -699,11 +737,12 @@ __device__ __forceinline__ void build_lj_pass(const Rep *__restrict__ reps, unsi
     const int gi = t0 + half;
     bool live = gi < total;
     const Rep *Sp = reps;
-    int i = 0, a = 0, b = 0, cz = 0;
+    int i = 0, a = 0, b = 0, cz = 0, buf = 0;
     double4 pi = make_double4(0, 0, 0, 0);
     if (live) {
       const int e = lq_find(qst, ne, gi);
-      Sp = reps + qrep[e];
+      Sp = reps + lq_entry_rep(qrep[e]);
+      buf = lq_entry_buf(qrep[e]);
       i = gi - qst[e];
       pi = Sp->pos[i];
       const int cell = Sp->cell1_of[i];
@@ -720,7 +759,7 @@ __device__ __forceinline__ void build_lj_pass(const Rep *__restrict__ reps, unsi
     if (live && g < 27) {
       cl = stencil_cell(g, a, b, cz, S.nc1);
       const int ncell = S.nc1[0] * S.nc1[1] * S.nc1[2];
-      if (cl >= 0) cn = min(S.cnt1[(size_t)(S.ctrl->gen1 & 1) * ncell + cl], S.ccap1);
+      if (cl >= 0) cn = min(S.cnt1[(size_t)buf * ncell + cl], S.ccap1);
     }
     int incl = cn;
     for (int o = 1; o < 32; o <<= 1) {
@@ -796,86 +835,27 @@ __global__ __launch_bounds__(kBlock) void k_build_lj(const Rep *__restrict__ rep
                                                      unsigned long long *lq, int cap,
                                                      int inc_step) {
   if (inc_step && blockIdx.x == 0)
-    for (int r = threadIdx.x; r < nrep; r += blockDim.x) reps[r].ctrl->step++;
-  __shared__ int stage_all[kBlock / 32][kLJStage];
-  build_lj_pass(reps, lq, cap, stage_all);
-}
-
-// Barrier over the n workgroups of a launch that are all resident at once.
-// bar[0] counts this round's arrivals, bar[1] numbers the rounds: a
-// workgroup reads the round (acquire, so before its own arrival), arrives,
-// and waits for the round to change; the n-th arrival resets the count and
-// opens the next round.  So launches of any grid size share the counters,
-// and no workgroup leaves before all n have arrived.  Every thread's
-// device-scope release fence before the arrival and acquire fence after the
-// wait order the passes' global memory accesses across the XCDs.  The wait
-// is bounded (kSpinMax polls, a few milliseconds): a workgroup that gives up
-// marks the counters broken (bar[2]), and later launches skip the barrier
-// and report MD_ESYNC instead of waiting.
-constexpr int kSpinMax = 1 << 13;
-__device__ bool grid_barrier(unsigned *bar, unsigned n) {
-  __shared__ int ok;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned round = __hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned a = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    int spins = 0;
-    if (a == n - 1) {
-      __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(bar + 1, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      while (__hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == round &&
-             spins < kSpinMax) {
-        __builtin_amdgcn_s_sleep(1);
-        spins++;
+    for (int r = threadIdx.x; r < nrep; r += blockDim.x) {
+      Ctrl *c = reps[r].ctrl;
+      c->step++;
+      if (c->pend) {  // the bookkeeping of an SR_BIN step (its flags are set)
+        if (c->flag1) {
+          c->dr_max1 = 0;
+          c->gen1++;
+        } else {
+          c->dr_max1 = c->dr_max1 + c->pend_d1;
+        }
+        if (c->flag2) {
+          c->dr_max2 = 0;
+          c->gen2++;
+        } else {
+          c->dr_max2 = c->dr_max2 + c->pend_d2;
+        }
+        c->pend = 0;
       }
     }
-    ok = spins < kSpinMax;
-    if (!ok) __hip_atomic_store(bar + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  return ok;
-}
-
-// Both passes of the LJ list update in one launch (the replayed step's list
-// kernel): an empty queue returns at once; otherwise the binning pass, a
-// grid barrier, then the fill pass, over the same queue as k_bin_lj and
-// k_build_lj.  The grid fits on the chip at once (lists_grid); a barrier that times out records MD_ESYNC on the queued
-// replicas, whose lists are then stale (the engine reports the error).
-__global__ __launch_bounds__(kBlock) void k_lists(const Rep *__restrict__ reps, int nrep,
-                                                  unsigned long long *lq, int cap, int inc_step,
-                                                  unsigned *bar) {
-  if (inc_step && blockIdx.x == 0)
-    for (int r = threadIdx.x; r < nrep; r += blockDim.x) reps[r].ctrl->step++;
-  const unsigned long long head = *lq;
-  if ((head & ((1ull << kQShift) - 1)) == 0) return;
-  const bool broken = __hip_atomic_load(bar + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
-  if (!broken) bin_lj_pass(reps, lq, cap);
-  if (broken || !grid_barrier(bar, gridDim.x)) {
-    if (threadIdx.x == 0) {
-      const int ne = min((int)(head >> kQShift), cap);
-      for (int e = 0; e < ne; e++) set_error(reps[lq_rep(lq)[e]].ctrl, MD_ESYNC, blockIdx.x, -1);
-    }
-    return;
-  }
   __shared__ int stage_all[kBlock / 32][kLJStage];
   build_lj_pass(reps, lq, cap, stage_all);
-}
-
-// Largest power-of-two grid of k_lists that is resident at once using at
-// most half of the chip's workgroup slots for it (at most 1024), or 0.
-int lists_grid() {
-  int dev = 0, cus = 0, occ = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return 0;
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_lists, kBlock, 0) != hipSuccess) return 0;
-  const long cap = std::min<long>(1024, (long)cus * occ / 2);
-  if (cap < 1) return 0;
-  int g = 1;
-  while (2L * g <= cap) g *= 2;
-  return g;
 }
 
 // Empties a rebuild queue (after a list update outside the step cycle, whose
@@ -2467,13 +2447,12 @@ void launch_kick_drift(const Plan &P, hipStream_t st, int store_v) {
 
 // update_verlet_list's LJ rebuild (compt_acc.f90:717-720); every kernel
 // returns at once for replicas whose rebuild flag is clear.
+void launch_build(const Plan &P, hipStream_t st, int inc_step) {
+  hipLaunchKernelGGL(k_build_lj, dim3(P.grid_build), dim3(kBlock), 0, st, P.reps, P.nrep, P.lq,
+                     P.lq_cap, inc_step);
+}
+
 void launch_lists(const Plan &P, hipStream_t st, int inc_step, hipEvent_t *ev, bool clear_queue) {
-  if (!ev && P.lists_grid > 0 && P.bar) {  // one launch (timed per pass: two)
-    hipLaunchKernelGGL(k_lists, dim3(P.lists_grid), dim3(kBlock), 0, st, P.reps, P.nrep, P.lq,
-                       P.lq_cap, inc_step, P.bar);
-    if (clear_queue) hipLaunchKernelGGL(k_lq_clear, dim3(1), dim3(1), 0, st, P.lq);
-    return;
-  }
   if (ev) (void)hipEventRecord(ev[0], st);
   hipLaunchKernelGGL(k_bin_lj, dim3(P.grid_bin), dim3(kBlock), 0, st, P.reps, P.lq, P.lq_cap);
   if (ev) (void)hipEventRecord(ev[1], st);

@@ -40,24 +40,6 @@ def test_alone_equals_mixed():
     assert np.array_equal(pa, pm)
 
 
-def test_one_launch_list_update_batch(monkeypatch):
-    """One-launch list updates (MD_FUSED_LISTS=1) whose grids differ between
-    the single-replica and the batch launches share one barrier: the same
-    replica states as the two list kernels."""
-    print("[batch] one-launch list update", flush=True)
-    out = []
-    for flag in ("1", "0"):
-        monkeypatch.setenv("MD_FUSED_LISTS", flag)
-        b = Batch(MD_FP32)
-        s0 = b.add(*_req(2, 25, 1, 1, 0.9, 12))
-        b.run(3)
-        s1 = b.add(*_req(4, 60, 4, 3, 0.1, 13))
-        b.run(5)
-        out.append([b.download(s0)[0], b.download(s1)[0]])
-        b.close()
-    assert all(np.array_equal(x, y) for x, y in zip(*out))
-
-
 def test_batch_replica_equals_single_engine():
     print("[batch] vs single engine fp64", flush=True)
     p, pos, vel = _req(3, 30, 1, 3, 0.3, 21, MD_FP64)

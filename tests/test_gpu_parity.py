@@ -361,23 +361,6 @@ def test_concurrent_real_space_matches_one_stream(prec, monkeypatch):
         assert np.array_equal(x, y)
 
 
-def test_one_launch_list_update(monkeypatch):
-    """The list update as one launch with a grid barrier (MD_FUSED_LISTS=1)
-    builds the lists of the two list kernels: the same states after a run
-    with several rebuilds."""
-    print("[one-launch list update] default fp32", flush=True)
-    out = []
-    for flag in ("1", "0"):
-        monkeypatch.setenv("MD_FUSED_LISTS", flag)
-        p, pos, vel, e = _engine("default", MD_FP32)
-        e.compute_force(0, MD_NOISE_PHILOX)
-        e.run(1, 40)
-        out.append(e.download())
-        e.close()
-    for x, y in zip(out[0], out[1]):
-        assert np.array_equal(x, y)
-
-
 @pytest.mark.parametrize("sweep", ["wave", "wg"])
 @pytest.mark.parametrize("name", NAMES)
 def test_real_space_sweep_forms(name, sweep, monkeypatch):

@@ -383,7 +383,7 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
     if (!S.tcnt) return fail(MD_ENOMEM, "spreading tiles");
     // x and y passes fused per z-plane when both planes fit in LDS
     // two regions of max(K2/2 * K1, Kh * K2) points (the FFT passes alternate)
-    plane_bytes = 2 * std::max((size_t)(K2 / 2) * K1, (size_t)S.Kh * K2) * csz;
+    plane_bytes = 2 * mdk::plane_region(K1, K2, S.Kh) * csz;
     lds_plane = plane_bytes;
     // MD_FFT_PLANES=0 keeps the separate x and y passes (for comparison)
     const char *fp = std::getenv("MD_FFT_PLANES");

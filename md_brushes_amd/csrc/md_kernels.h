@@ -22,6 +22,16 @@ enum {
                          // then k_build_lj alone, which does the bookkeeping)
 };
 
+// Complex points in each of the two LDS regions of a fused FFT plane
+// kernel: the x lines (K2/2 rows of K1) or the y lines (Kh lines of K2,
+// laid out K2 + 1 apart so that the transposing accesses, which step from
+// line to line, fall in different LDS banks).
+__host__ __device__ inline int plane_line_stride(int K2) { return K2 + 1; }
+__host__ __device__ inline size_t plane_region(int K1, int K2, int Kh) {
+  const size_t a = (size_t)(K2 / 2) * K1, b = (size_t)Kh * plane_line_stride(K2);
+  return a > b ? a : b;
+}
+
 // Everything a launch needs: device descriptors and the block maps.
 struct Plan {
   Rep *reps = nullptr;

@@ -1186,13 +1186,13 @@ __device__ __forceinline__ void dft_reg(C *v) {
 // them (input stride n/R, output stride Ns) after the twiddles of the block.
 template <typename C, int SGN, int R>
 __device__ __forceinline__ void stockham_pass(const C *src, C *dst, int ln, int nl, int Ns,
-                                              const C *__restrict__ tw) {
+                                              const C *__restrict__ tw, int ls) {
   const int n = 1 << ln, nr = n / R;
   const int lnr = ln - (R == 8 ? 3 : R == 4 ? 2 : 1);
   for (int t = threadIdx.x; t < nl * nr; t += blockDim.x) {
     const int line = t >> lnr, j = t & (nr - 1);
     const int k = j & (Ns - 1);
-    const C *s = src + (line << ln) + j;
+    const C *s = src + line * ls + j;
     C v[R];
 #pragma unroll
     for (int r = 0; r < R; r++) v[r] = s[r * nr];
@@ -1206,27 +1206,28 @@ __device__ __forceinline__ void stockham_pass(const C *src, C *dst, int ln, int 
       }
     }
     dft_reg<C, SGN, R>(v);
-    C *d = dst + (line << ln) + (j - k) * R + k;
+    C *d = dst + line * ls + (j - k) * R + k;
 #pragma unroll
     for (int r = 0; r < R; r++) d[r * Ns] = v[r];
   }
 }
 
 template <typename C, int SGN>
-__device__ C *fft_lines(C *a, C *b, int ln, int nl, const C *__restrict__ tw) {
+__device__ C *fft_lines(C *a, C *b, int ln, int nl, const C *__restrict__ tw, int ls = 0) {
+  if (ls == 0) ls = 1 << ln;  // line start stride (points)
   int rem = ln, Ns = 1;
   C *src = a, *dst = b;
   while (rem > 0) {
     if (rem >= 3) {
-      stockham_pass<C, SGN, 8>(src, dst, ln, nl, Ns, tw);
+      stockham_pass<C, SGN, 8>(src, dst, ln, nl, Ns, tw, ls);
       Ns <<= 3;
       rem -= 3;
     } else if (rem == 2) {
-      stockham_pass<C, SGN, 4>(src, dst, ln, nl, Ns, tw);
+      stockham_pass<C, SGN, 4>(src, dst, ln, nl, Ns, tw, ls);
       Ns <<= 2;
       rem -= 2;
     } else {
-      stockham_pass<C, SGN, 2>(src, dst, ln, nl, Ns, tw);
+      stockham_pass<C, SGN, 2>(src, dst, ln, nl, Ns, tw, ls);
       Ns <<= 1;
       rem -= 1;
     }
@@ -1361,8 +1362,8 @@ __global__ __launch_bounds__(kPlaneBlock) void k_plane_fwd(const Rep *__restrict
   const Rep &S = reps[r];
   const int z = blockIdx.x - wm.blk[r];
   const int K1 = S.K[0], K2 = S.K[1], Kh = S.Kh, l1 = S.lK[0], l2 = S.lK[1];
-  const int reg = max((K2 / 2) * K1, Kh * K2);
-  C *ra = (C *)smem, *rb = ra + reg;
+  const int KP = plane_line_stride(K2);
+  C *ra = (C *)smem, *rb = ra + plane_region(K1, K2, Kh);
   const R *q = (const R *)S.meshr + (size_t)z * K2 * K1;
   for (int t = threadIdx.x; t < (K2 / 2) * K1; t += blockDim.x) {
     const int l = t / K1, x = t - l * K1;
@@ -1382,15 +1383,15 @@ __global__ __launch_bounds__(kPlaneBlock) void k_plane_fwd(const Rep *__restrict
     a.y = (R)0.5 * (zk.y - zm.y);
     b.x = (R)0.5 * (zk.y + zm.y);
     b.y = (R)0.5 * (zm.x - zk.x);
-    P[k * K2 + 2 * l] = a;
-    P[k * K2 + 2 * l + 1] = b;
+    P[k * KP + 2 * l] = a;
+    P[k * KP + 2 * l + 1] = b;
   }
   __syncthreads();
-  const C *Y = fft_lines<C, -1>(P, (C *)A, l2, Kh, (const C *)S.tw[1]);
+  const C *Y = fft_lines<C, -1>(P, (C *)A, l2, Kh, (const C *)S.tw[1], KP);
   C *spec = (C *)S.spec + (size_t)z * K2 * Kh;
   for (int t = threadIdx.x; t < K2 * Kh; t += blockDim.x) {
     const int y = t / Kh, k = t - y * Kh;
-    spec[t] = Y[k * K2 + y];
+    spec[t] = Y[k * KP + y];
   }
 }
 
@@ -1405,25 +1406,25 @@ __global__ __launch_bounds__(kPlaneBlock) void k_plane_inv(const Rep *__restrict
   const Rep &S = reps[r];
   const int z = blockIdx.x - wm.blk[r];
   const int K1 = S.K[0], K2 = S.K[1], Kh = S.Kh, l1 = S.lK[0], l2 = S.lK[1];
-  const int reg = max((K2 / 2) * K1, Kh * K2);
-  C *ra = (C *)smem, *rb = ra + reg;
+  const int KP = plane_line_stride(K2);
+  C *ra = (C *)smem, *rb = ra + plane_region(K1, K2, Kh);
   const C *spec = (const C *)S.spec + (size_t)z * K2 * Kh;
   for (int t = threadIdx.x; t < K2 * Kh; t += blockDim.x) {
     const int y = t / Kh, k = t - y * Kh;
-    ra[k * K2 + y] = spec[t];
+    ra[k * KP + y] = spec[t];
   }
   __syncthreads();
-  const C *P = fft_lines<C, +1>(ra, rb, l2, Kh, (const C *)S.tw[1]);
+  const C *P = fft_lines<C, +1>(ra, rb, l2, Kh, (const C *)S.tw[1], KP);
   C *A = P == ra ? rb : ra;
   for (int t = threadIdx.x; t < (K2 / 2) * K1; t += blockDim.x) {
     const int l = t / K1, k = t - l * K1;
     C a, b, v;
     if (k < Kh) {
-      a = P[k * K2 + 2 * l];
-      b = P[k * K2 + 2 * l + 1];
+      a = P[k * KP + 2 * l];
+      b = P[k * KP + 2 * l + 1];
     } else {
-      a = P[(K1 - k) * K2 + 2 * l];
-      b = P[(K1 - k) * K2 + 2 * l + 1];
+      a = P[(K1 - k) * KP + 2 * l];
+      b = P[(K1 - k) * KP + 2 * l + 1];
       a.y = -a.y;
       b.y = -b.y;
     }

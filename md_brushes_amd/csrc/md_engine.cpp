@@ -463,8 +463,8 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
     S.part = dalloc<double>(4 * (size_t)S.nqblk);
     if (!S.fsp || !S.part || !S.freal) return fail(MD_ENOMEM, "spme scratch");
     lds_x = 2 * (size_t)S.px_tile * K1 * csz;
-    lds_y = 2 * (size_t)S.py_tile * K2 * csz;
-    lds_z = 2 * (size_t)S.pz_tile * K3 * csz;
+    lds_y = 2 * (size_t)S.py_tile * mdk::plane_line_stride(K2) * csz;
+    lds_z = 2 * (size_t)S.pz_tile * mdk::zline_stride(K3) * csz;
   }
   nb[0] = cdiv(NN, mdk::kBlock);
   nb[1] = coulomb ? cdiv(Nq, mdk::kBlock) : 0;
@@ -809,7 +809,7 @@ int md_engine::rebuild() {
     const int want_pz = zl ? std::atoi(zl) : tune_pz;
     if (want_pz > 0) x->h.pz_tile = std::max(1, std::min(want_pz, 16384 / (x->h.K[2] * csz)));
     x->nb[5] = cdiv((long)x->h.K[1] * x->h.Kh, x->h.pz_tile);
-    x->lds_z = 2 * (size_t)x->h.pz_tile * x->h.K[2] * csz;
+    x->lds_z = 2 * (size_t)x->h.pz_tile * mdk::zline_stride(x->h.K[2]) * csz;
     // fused planes where they fit, unless tuned off
     Rep &S = x->h;
     const char *fp = std::getenv("MD_FFT_PLANES");  // =0 keeps separate x and y passes

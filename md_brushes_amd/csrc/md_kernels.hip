@@ -1331,8 +1331,8 @@ __global__ __launch_bounds__(kBlock) void k_ffty(const Rep *__restrict__ reps, W
   extern __shared__ char smem[];
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
-  const int K2 = S.K[1], ln = S.lK[1], Kh = S.Kh, T = S.py_tile;
-  C *ba = (C *)smem, *bb = ba + (size_t)T * K2;
+  const int K2 = S.K[1], ln = S.lK[1], Kh = S.Kh, T = S.py_tile, KP = plane_line_stride(K2);
+  C *ba = (C *)smem, *bb = ba + (size_t)T * KP;
   const int tiles = (Kh + T - 1) / T;
   const int b = blockIdx.x - wm.blk[r];
   const int z = b / tiles, x0 = (b - z * tiles) * T;
@@ -1340,20 +1340,20 @@ __global__ __launch_bounds__(kBlock) void k_ffty(const Rep *__restrict__ reps, W
   C *spec = (C *)S.spec + (size_t)z * K2 * Kh + x0;
   for (int t = threadIdx.x; t < nl * K2; t += blockDim.x) {
     const int y = t / nl, l = t - y * nl;
-    ba[l * K2 + y] = spec[(size_t)y * Kh + l];
+    ba[l * KP + y] = spec[(size_t)y * Kh + l];
   }
   __syncthreads();
-  const C *buf = fft_lines<C, SGN>(ba, bb, ln, nl, (const C *)S.tw[1]);
+  const C *buf = fft_lines<C, SGN>(ba, bb, ln, nl, (const C *)S.tw[1], KP);
   for (int t = threadIdx.x; t < nl * K2; t += blockDim.x) {
     const int y = t / nl, l = t - y * nl;
-    spec[(size_t)y * Kh + l] = buf[l * K2 + y];
+    spec[(size_t)y * Kh + l] = buf[l * KP + y];
   }
 }
 
 // Forward x and y passes of one z-plane in LDS: pairs of real rows -> one
 // complex FFT of length K1 each, split into two half spectra stored
 // line-major by kx, then the y FFT of each kx line.  Two LDS regions of
-// max(K2/2 * K1, Kh * K2) points: the passes alternate between them.
+// plane_region points: the passes alternate between them.
 template <typename R>
 __global__ __launch_bounds__(kPlaneBlock) void k_plane_fwd(const Rep *__restrict__ reps, WMap wm) {
   using C = typename CT<R>::C;
@@ -1452,8 +1452,8 @@ __global__ __launch_bounds__(kBlock) void k_fftz_conv(const Rep *__restrict__ re
   extern __shared__ char smem[];
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
-  const int K3 = S.K[2], ln = S.lK[2], T = S.pz_tile;
-  C *ba = (C *)smem, *bb = ba + (size_t)T * K3;
+  const int K3 = S.K[2], ln = S.lK[2], T = S.pz_tile, KS = zline_stride(K3);
+  C *ba = (C *)smem, *bb = ba + (size_t)T * KS;
   const int f0 = (blockIdx.x - wm.blk[r]) * T;
   const size_t zs = (size_t)S.K[1] * S.Kh;
   const int nl = min(T, (int)zs - f0);
@@ -1461,23 +1461,23 @@ __global__ __launch_bounds__(kBlock) void k_fftz_conv(const Rep *__restrict__ re
   const R *bc = (const R *)S.bc + f0;
   for (int t = threadIdx.x; t < nl * K3; t += blockDim.x) {
     const int z = t / nl, l = t - z * nl;
-    ba[l * K3 + z] = spec[z * zs + l];
+    ba[l * KS + z] = spec[z * zs + l];
   }
   __syncthreads();
-  C *buf = fft_lines<C, -1>(ba, bb, ln, nl, (const C *)S.tw[2]);
+  C *buf = fft_lines<C, -1>(ba, bb, ln, nl, (const C *)S.tw[2], KS);
   for (int t = threadIdx.x; t < nl * K3; t += blockDim.x) {
     const int z = t / nl, l = t - z * nl;
     const R g = bc[z * zs + l];
-    C v = buf[l * K3 + z];
+    C v = buf[l * KS + z];
     v.x *= g;
     v.y *= g;
-    buf[l * K3 + z] = v;
+    buf[l * KS + z] = v;
   }
   __syncthreads();
-  const C *out = fft_lines<C, +1>(buf, buf == ba ? bb : ba, ln, nl, (const C *)S.tw[2]);
+  const C *out = fft_lines<C, +1>(buf, buf == ba ? bb : ba, ln, nl, (const C *)S.tw[2], KS);
   for (int t = threadIdx.x; t < nl * K3; t += blockDim.x) {
     const int z = t / nl, l = t - z * nl;
-    spec[z * zs + l] = out[l * K3 + z];
+    spec[z * zs + l] = out[l * KS + z];
   }
 }
 

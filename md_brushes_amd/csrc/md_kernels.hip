@@ -354,21 +354,14 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
     }
     R fx = 0, fy = 0, fz = 0;
     const int *nb = S.nb1 + ip;
-    for (int k = g; k < n; k += G) {
-      const int j = nb[(size_t)k * S.NN];
-      R xj, yj, zj, qj;
-      if constexpr (sizeof(R) == 4) {
-        float4 p = S.pos32[j];
-        xj = p.x; yj = p.y; zj = p.z; qj = p.w;
-      } else {
-        double4 p = S.pos[j];
-        xj = p.x; yj = p.y; zj = p.z; qj = p.w;
-      }
-      R dx = xi - xj, dy = yi - yj, dz = zi - zj;
+    using V = typename std::conditional<sizeof(R) == 4, float4, double4>::type;
+    const V *pv = (const V *)(sizeof(R) == 4 ? (const void *)S.pos32 : (const void *)S.pos);
+    auto pair = [&](const V pj) {
+      R dx = xi - (R)pj.x, dy = yi - (R)pj.y, dz = zi - (R)pj.z;
       dx = image(dx, Lx, hLx);
       dy = image(dy, Ly, hLy);
       const R rsq = dx * dx + dy * dy + dz * dz;
-      const int m = (int)(qa + (sizeof(R) == 4 ? fabsf((float)qj) : fabs((double)qj)));
+      const int m = (int)(qa + fabs((R)pj.w));  // |q| are small integers: exact
       R cut, s2;
       if (m == 6) { cut = rcl3; s2 = sig2; }
       else if (m < 3) { cut = rcl2; s2 = (R)1; }
@@ -386,12 +379,28 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
         fy += gg * dy;
         fz += gg * dz;
       }
+    };
+    // slots g, g + G, ... in order; four at a time, their index and position
+    // loads issued together (the same sums as one at a time)
+    for (int k = g; k < n; k += 4 * G) {
+      int j[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) j[u] = k + u * G < n ? nb[(size_t)(k + u * G) * S.NN] : -1;
+      V pj[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (j[u] >= 0) pj[u] = pv[j[u]];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (j[u] >= 0) pair(pj[u]);
     }
-    for (int o = 1; o < G; o <<= 1) {  // fixed tree over the lanes of the particle
-      fx += __shfl_xor(fx, o, G);
-      fy += __shfl_xor(fy, o, G);
-      fz += __shfl_xor(fz, o, G);
-    }
+    static_assert(G == 4, "the lanes of a particle are a quad of the DPP tree");
+    fx += dpp_move<0xb1, 0xf>(fx);  // fixed tree over the particle's lanes: xor 1, then xor 2
+    fy += dpp_move<0xb1, 0xf>(fy);
+    fz += dpp_move<0xb1, 0xf>(fz);
+    fx += dpp_move<0x4e, 0xf>(fx);
+    fy += dpp_move<0x4e, 0xf>(fy);
+    fz += dpp_move<0x4e, 0xf>(fz);
     if (g == 0) {
       lj[0][pl] = fx * (R)48;
       lj[1][pl] = fy * (R)48;

@@ -426,18 +426,29 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
       const int b0 = S.bstart[i], b1 = S.bstart[i + 1];
       const R R02 = (R)S.R0_2, kR = (R)S.kR02;
       R fx = 0, fy = 0, fz = 0;
-      for (int b = b0; b < b1; b++) {
-        const int nn = S.bpart[b];
-        double dxd, dyd, dzd;
-        const double rs = rsq_image(p, S.pos[nn], S.Lx, S.Ly, S.hLx, S.hLy, dxd, dyd, dzd);
-        if (rs > S.R0_2) {
-          set_error(S.ctrl, MD_EBOND, i < nn ? i : nn, step);
-          continue;
+      // bonds in order, four at a time: the partner and position loads issued together
+      for (int b = b0; b < b1; b += 4) {
+        int nn[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) nn[u] = b + u < b1 ? S.bpart[b + u] : -1;
+        double4 pj[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          if (nn[u] >= 0) pj[u] = S.pos[nn[u]];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          if (nn[u] < 0) continue;
+          double dxd, dyd, dzd;
+          const double rs = rsq_image(p, pj[u], S.Lx, S.Ly, S.hLx, S.hLy, dxd, dyd, dzd);
+          if (rs > S.R0_2) {
+            set_error(S.ctrl, MD_EBOND, i < nn[u] ? i : nn[u], step);
+            continue;
+          }
+          const R gg = -kR / (R02 - (R)rs);
+          fx += gg * (R)dxd;
+          fy += gg * (R)dyd;
+          fz += gg * (R)dzd;
         }
-        const R gg = -kR / (R02 - (R)rs);
-        fx += gg * (R)dxd;
-        fy += gg * (R)dyd;
-        fz += gg * (R)dzd;
       }
       fb[0][lane] = fx;
       fb[1][lane] = fy;
@@ -2374,7 +2385,11 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) 
   hipLaunchKernelGGL(k_bin_q, dim3(P.chg.total), dim3(kBlock), 0, st, P.reps, P.chg);
   mark();
   if (P.cell3.total)
-    hipLaunchKernelGGL(k_sort_q<R>, dim3(P.cell3.total), dim3(kBlock), P.lds_sort3, st, P.reps, P.cell3);
+    // many cells (batches): one wavefront per cell, whose bitonic stages'
+    // barriers stay inside it (306 -> 112 us for the 256-replica stream);
+    // few cells: four (11 against 14 us at the default brush)
+    hipLaunchKernelGGL(k_sort_q<R>, dim3(P.cell3.total), dim3(P.cell3.total > 4096 ? 64 : kBlock),
+                       P.lds_sort3, st, P.reps, P.cell3);
   mark();
   // the sweep's workgroups ride along with the spreading and gather launches
   // (half of their items in each), unless its launches are timed or forked

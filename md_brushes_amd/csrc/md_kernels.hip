@@ -310,7 +310,8 @@ __device__ __forceinline__ void bin_lj_particle(const Rep *Sp, int i, int buf);
 // takes slots g, g + kSRLanes, ..., a fixed tree over the lanes), wavefront
 // 4 draws the noise, 5 the wall and FENE forces (bond order), 6 loads the
 // particle's state and then adds everything up, kicks, drifts and stores;
-// wavefront 7 takes the rebuild decision of the replica's first workgroup.
+// wavefront 7 takes the rebuild decision (the replica's first workgroup, or
+// under SR_BIN every workgroup, which then bins its own particles).
 template <typename R>
 __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict__ reps, WMap wm,
                                                           int mode, int step_arg,

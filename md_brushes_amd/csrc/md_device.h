@@ -21,6 +21,7 @@ constexpr int kSRBlock = 512;     // its workgroup (LJ on 4 wavefronts, the per-
 constexpr int kSRParticles = 64;  // parts on the others) takes 64 particles
 constexpr int kRealChunk = 4;     // target charges per wavefront work item of the real-space sweep
 constexpr int kRealChunkWG = 16;  // ... per workgroup work item (replicas with dense cells)
+constexpr int kGatherLanes = 4;   // lanes per charge of the reciprocal-force gather
 constexpr int kRealRound = 256;   // real-space candidates tested per compaction round (4 per lane)
 constexpr int kLJStage = 256;     // LJ fill pass: candidates staged in order per target
 constexpr int kHistBins = 500;  // SizeHist of the z-density profile (glob_var.f90:59)

@@ -457,7 +457,7 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
     // z pass lines per workgroup: set at rebuild with the engine's size
     S.pz_tile = std::max(1, std::min(4, budget / (K3 * csz)));
     pz_big = std::max(1, std::min(16, budget / (K3 * csz)));
-    S.nqblk = cdiv((long)Nq, mdk::kBlock);  // gather blocks (one lane per charge)
+    S.nqblk = cdiv((long)Nq, mdk::kBlock / mdk::kGatherLanes);  // gather blocks
     S.fsp = dalloc<double>(3 * (size_t)Nq);
     S.freal = dalloc<char>(3 * (size_t)Nq * (fp32 ? 4 : 8));
     S.part = dalloc<double>(4 * (size_t)S.nqblk);

@@ -1508,7 +1508,7 @@ __global__ __launch_bounds__(kBlock) void k_fftz_conv(const Rep *__restrict__ re
 // p points along x, rows summed z outer, y inner.  The order is a template
 // parameter so that the weights stay in registers.
 template <typename R, int O>
-__device__ __forceinline__ void gather_one(const Rep &S, double4 p, R &f1, R &f2, R &f3) {
+__device__ __forceinline__ void gather_one(const Rep &S, double4 p, int cz, R &f1, R &f2, R &f3) {
   R wx[kMaxOrder], wy[kMaxOrder], wz[kMaxOrder], dx[kMaxOrder], dy[kMaxOrder], dz[kMaxOrder];
   const int K1 = S.K[0], K2 = S.K[1], K3 = S.K[2];
   const int bx = wrapk(spline<R, O>(p.x, S.kg[0], O, wx, dx), K1);
@@ -1525,6 +1525,7 @@ __device__ __forceinline__ void gather_one(const Rep &S, double4 p, R &f1, R &f2
   f1 = f2 = f3 = 0;
 #pragma unroll
   for (int c = 0; c < O; c++) {
+    if (c % kGatherLanes != cz) continue;  // this lane's z planes
     int z = bz + c;
     z -= z >= K3 ? K3 : 0;
 #pragma unroll
@@ -1546,33 +1547,46 @@ __device__ __forceinline__ void gather_one(const Rep &S, double4 p, R &f1, R &f2
   }
 }
 
-// Reciprocal force of each charge (one lane per charge), plus per-block
-// sums of the force (for ZeroForce, compt_acc.f90:1470-1492) and of
-// q z lb/Beta (the slab dipole M_z, compt_acc.f90:1041-1044), in double.
+// Reciprocal force of each charge (kGatherLanes lanes per charge, each
+// summing the z planes c = its lane mod kGatherLanes, then a fixed DPP tree
+// over the quad), plus per-block sums of the force (for ZeroForce,
+// compt_acc.f90:1470-1492) and of q z lb/Beta (the slab dipole M_z,
+// compt_acc.f90:1041-1044), in double.
 template <typename R>
 __device__ __forceinline__ void gather_block(const Rep *__restrict__ reps, WMap wm, int gb) {
   const int r = find_rep(wm, gb);
   const Rep &S = reps[r];
   const int blk = gb - wm.blk[r];
-  const int m = blk * kBlock + threadIdx.x;
+  static_assert(kGatherLanes == 4, "a charge's lanes are a quad of the DPP tree");
+  const int cz = threadIdx.x & (kGatherLanes - 1);
+  const int m = blk * (kBlock / kGatherLanes) + threadIdx.x / kGatherLanes;
   const bool live = m < S.Nq;
   {  // the tile lists are consumed: clear them for the next evaluation
     const int ntile = S.ntl[0] * S.ntl[1] * S.ntl[2] * kTileSub;
     const int nth = (wm.blk[r + 1] - wm.blk[r]) * kBlock;
-    for (int k = m; k < ntile; k += nth) S.tcnt[k] = 0;
+    for (int k = blk * kBlock + threadIdx.x; k < ntile; k += nth) S.tcnt[k] = 0;
   }
   double f[4] = {0, 0, 0, 0};
+  double4 p = make_double4(0, 0, 0, 0);
+  R f1 = 0, f2 = 0, f3 = 0;
   if (live) {
-    const double4 p = S.pos[S.charge[m]];
-    R f1, f2, f3;
+    p = S.pos[S.charge[m]];
     switch (S.order) {
-      case 3: gather_one<R, 3>(S, p, f1, f2, f3); break;
-      case 4: gather_one<R, 4>(S, p, f1, f2, f3); break;
-      case 5: gather_one<R, 5>(S, p, f1, f2, f3); break;
-      case 6: gather_one<R, 6>(S, p, f1, f2, f3); break;
-      case 7: gather_one<R, 7>(S, p, f1, f2, f3); break;
-      default: gather_one<R, 8>(S, p, f1, f2, f3); break;
+      case 3: gather_one<R, 3>(S, p, cz, f1, f2, f3); break;
+      case 4: gather_one<R, 4>(S, p, cz, f1, f2, f3); break;
+      case 5: gather_one<R, 5>(S, p, cz, f1, f2, f3); break;
+      case 6: gather_one<R, 6>(S, p, cz, f1, f2, f3); break;
+      case 7: gather_one<R, 7>(S, p, cz, f1, f2, f3); break;
+      default: gather_one<R, 8>(S, p, cz, f1, f2, f3); break;
     }
+  }
+  f1 += dpp_move<0xb1, 0xf>(f1);  // the quad's planes: xor 1, then xor 2
+  f2 += dpp_move<0xb1, 0xf>(f2);
+  f3 += dpp_move<0xb1, 0xf>(f3);
+  f1 += dpp_move<0x4e, 0xf>(f1);
+  f2 += dpp_move<0x4e, 0xf>(f2);
+  f3 += dpp_move<0x4e, 0xf>(f3);
+  if (live && cz == 0) {
     f[0] = -(double)f1 * S.fnorm[0];
     f[1] = -(double)f2 * S.fnorm[1];
     f[2] = -(double)f3 * S.fnorm[2];
@@ -2455,7 +2469,7 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) 
       hipLaunchKernelGGL((k_real_cells<R, false>), dim3(P.cell3c.total), dim3(kBlock), 0, st, P.reps,
                          P.cell3c);
   }
-  hipLaunchKernelGGL(k_coul_store<R>, dim3(P.chgG.total), dim3(kBlock), 0, st, P.reps, P.chgG, out);
+  hipLaunchKernelGGL(k_coul_store<R>, dim3(P.chg.total), dim3(kBlock), 0, st, P.reps, P.chg, out);
   mark();
 }
 

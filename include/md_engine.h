@@ -138,7 +138,9 @@ int md_download_state(md_engine *e, double *pos, double *vel, double *acc,
 int md_download_coulomb(md_engine *e, double *acc_c);
 
 /* Replaces lj_verlet_list (compt_acc.f90:185-314) and, when real_verlet
- * is set, real_verlet_list (compt_acc.f90:491-626). */
+ * is set, real_verlet_list (compt_acc.f90:491-626).  The real-space list's
+ * capacities are sized at the first call (a host scan of the positions);
+ * the step loop never needs that list, so uploads skip the scan. */
 int md_build_lists(md_engine *e);
 
 /* Verlet lists as (i,j) pairs, 1-based, sorted by (i,j).  Returns the pair

@@ -131,6 +131,7 @@ std::string hip_msg(hipError_t e, const char *what) {
 struct Sys {
   md_params p{};
   bool fp32 = false, coulomb = false, uploaded = false;
+  bool real_list_wanted = false;  // md_build_lists asked for the reference-form real list
   Rep h{};
   std::vector<int> charge, flags, bstart, bpart;  // host topology
   std::vector<void *> allocs;
@@ -561,7 +562,10 @@ int Sys::size_lists(const double *pos) {
   for (int i = 0; i < NN; i++) all[i] = i;
   int mc1, mn1, mc2 = 0, mn2 = 0, mc3 = 0, mn3 = 0;
   scan(all, h.nc1, p.rvl_eff, mc1, mn1);
-  if (h.real_verlet) scan(charge, h.nc2, p.rvc, mc2, mn2);
+  // the reference-form real-space list is built only on request
+  // (md_build_lists): its neighbour scan, the costly part of an upload, waits
+  // for the first request
+  if (h.real_verlet && real_list_wanted) scan(charge, h.nc2, p.rvc, mc2, mn2);
   if (h.real_cells) {
     scan(charge, h.nc3, 0.0, mc3, mn3);
     (void)occ3;
@@ -1267,10 +1271,32 @@ extern "C" int md_build_lists(md_engine *e) {
   ENGINE_CHECK(e);
   FIRST(e);
   STATE_CHECK(e);
-  mdk::launch_set_flags(P, e->st, 0);
-  mdk::launch_lists(P, e->st, 0);
-  mdk::launch_real_list(P, e->st);
-  mdk::launch_set_flags(P, e->st, 1);
+  if (s->h.real_verlet && !s->real_list_wanted) {  // size the real-space list first
+    int rc = e->sync();
+    if (rc) return rc;
+    const int NN = s->p.NN;
+    std::vector<double4> p4(NN);
+    hipError_t err = hipMemcpy(p4.data(), s->h.pos, NN * sizeof(double4), hipMemcpyDeviceToHost);
+    if (err != hipSuccess) return e->hip_fail(err, "positions");
+    std::vector<double> pos(4 * (size_t)NN);
+    for (int i = 0; i < NN; i++) {
+      pos[i] = p4[i].x;
+      pos[NN + i] = p4[i].y;
+      pos[2 * (size_t)NN + i] = p4[i].z;
+      pos[3 * (size_t)NN + i] = p4[i].w;
+    }
+    s->real_list_wanted = true;
+    rc = s->size_lists(pos.data());
+    if (rc) return e->fail(rc, s->err);
+    e->dirty = true;
+    rc = e->rebuild();
+    if (rc) return rc;
+  }
+  const Plan &Q = e->single[0];
+  mdk::launch_set_flags(Q, e->st, 0);
+  mdk::launch_lists(Q, e->st, 0);
+  mdk::launch_real_list(Q, e->st);
+  mdk::launch_set_flags(Q, e->st, 1);
   return e->check_all();
 }
 

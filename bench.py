@@ -261,7 +261,8 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (replicas shard "
-              "across GPUs with no communication; see DESIGN.md)", file=sys.stderr)
+              "across GPUs as independent engines, one process per GPU, with no communication)",
+              file=sys.stderr)
         sys.exit(2)
 
     state = prepare(args.seed, args.equil)

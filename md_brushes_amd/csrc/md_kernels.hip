@@ -1468,7 +1468,7 @@ __global__ __launch_bounds__(kPlaneBlock) void k_plane_inv(const Rep *__restrict
 // forward FFT along z, multiply by BC, inverse FFT along z; tiles of Tz
 // adjacent (ky, kx) lines, contiguous in the half-spectrum layout.
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_fftz_conv(const Rep *__restrict__ reps, WMap wm) {
+__global__ __launch_bounds__(kPlaneBlock) void k_fftz_conv(const Rep *__restrict__ reps, WMap wm) {
   using C = typename CT<R>::C;
   extern __shared__ char smem[];
   const int r = find_rep(wm, blockIdx.x);
@@ -2440,7 +2440,8 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) 
   if (P.fy.total)
     hipLaunchKernelGGL((k_ffty<R, -1>), dim3(P.fy.total), dim3(kBlock), P.lds_y, st, P.reps, P.fy);
   mark();
-  hipLaunchKernelGGL(k_fftz_conv<R>, dim3(P.fz.total), dim3(kBlock), P.lds_z, st, P.reps, P.fz);
+  // 512 lanes: the strided line loads and stores spread over more wavefronts
+  hipLaunchKernelGGL(k_fftz_conv<R>, dim3(P.fz.total), dim3(kPlaneBlock), P.lds_z, st, P.reps, P.fz);
   mark();
   if (P.fy.total)
     hipLaunchKernelGGL((k_ffty<R, +1>), dim3(P.fy.total), dim3(kBlock), P.lds_y, st, P.reps, P.fy);

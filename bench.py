@@ -26,40 +26,34 @@ from md_brushes_amd import (MD_FP32, MD_FP64, MD_NOISE_PHILOX, Engine, config,  
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 
 
-def kernel_bytes(p, name):
-    """Algorithmic (compulsory) bytes of one launch: each array the kernel
-    must touch, once.  Neighbour-list storage is excluded."""
+def step_bytes(p):
+    """Algorithmic bytes of one step of the FP32 force path (compulsory
+    traffic, each array touched once per pass; stored neighbour lists
+    excluded): per particle and step 160 B -- the integrator reads x, v
+    (FP64, 48 B) and the force (FP32, 12 B), writes x, v (48 B) and the FP32
+    position copy (12 B); the short-range pass reads the FP32 position and
+    charge (16 B) and the held Coulomb force (12 B) and writes the force
+    (12 B) -- plus, once per Coulomb evaluation (every multistep steps),
+    72 B per charge and 16 B per mesh point and per half-spectrum point.
+    The FP64 path: 200 B per particle, 144 B per charge, 32 B per point."""
     NN, Nq = p.NN, p.Nq
     K1, K2, K3 = p.ng
     M, Mh = K1 * K2 * K3, (K1 // 2 + 1) * K2 * K3
-    f32 = p.precision == MD_FP32
-    r = 4 if f32 else 8
+    if p.precision == MD_FP32:
+        return 160 * NN, 72 * Nq + 16 * M + 16 * Mh
+    return 200 * NN, 144 * Nq + 32 * M + 32 * Mh
+
+
+def kernel_bytes(p, name):
+    """Algorithmic bytes of one launch of a step's kernel: the short-range
+    kernel carries the whole per-particle part of the step (it kicks, drifts
+    and stores the next positions), the Coulomb chain the per-evaluation
+    part."""
+    per_particle, per_coulomb = step_bytes(p)
     if name == "short_range":
-        # replayed form: pos (32) [+ pos32 16], held Coulomb 3r, v read+write
-        # 48, acc read+write 48, flags 4, list length 4, bond offsets 4, the
-        # next step's positions written 32 [+ 16]; bonded partners 2*4*N_bond
-        per = 2 * (32 + (16 if f32 else 0)) + 3 * r + 48 + 48 + 12
-        return NN * per + 8 * p.N_bond
-    if name == "kick_drift":
-        return NN * (24 + 48 + 64 + (16 if f32 else 0))
-    if name == "real":
-        # sweep: cell-sorted position (4r), member index, pair sums written
-        # (3r); store: reciprocal force 24, pair sums 3r, charge index 4,
-        # position 32, held force written 3r
-        return Nq * (4 * r + 4 + 3 * r + 24 + 3 * r + 4 + 32 + 3 * r)
-    if name == "spread":
-        # bin members, charge index, position of each charge; mesh written once
-        return Nq * (4 + 4 + 32) + r * M
-    if name == "gather":
-        return Nq * (4 + 32 + 24) + r * M
-    if name in ("fft_x_fwd",):
-        return r * M + 2 * r * Mh
-    if name in ("fft_x_inv",):
-        return 2 * r * Mh + r * M
-    if name in ("fft_y_fwd", "fft_y_inv"):
-        return 2 * 2 * r * Mh
-    if name == "fft_z_conv":
-        return 2 * 2 * r * Mh + r * Mh
+        return per_particle
+    if name == "coulomb":
+        return per_coulomb
     return 0
 
 
@@ -67,7 +61,8 @@ def roofline(ms, plist, multistep):
     """HBM roofline of the kernel with the largest time per step (Coulomb
     kernels run every multistep steps; list kernels only on rebuilds; the
     kick-drift kernel once per run, replayed steps drift in short_range):
-    algorithmic bytes of one launch over all replicas / its mean duration."""
+    algorithmic bytes of one launch over all replicas / its mean duration
+    (HIP events on the engine's stream)."""
     per_step = {k: v * (1.0 / multistep if k in Engine.COULOMB else 1.0)
                 for k, v in ms.items() if k not in Engine.LISTS and k not in Engine.RUN_START}
     dom = max(per_step, key=per_step.get)
@@ -76,6 +71,25 @@ def roofline(ms, plist, multistep):
     return {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 5), "traffic": None, "bytes_per_launch": b,
             "avg_launch_us": round(ms[dom] * 1e3, 3)}
+
+
+def roofline_chain(ms, plist):
+    """The Coulomb evaluation as a whole (spreading, FFTs, gather, real
+    space, store): its algorithmic bytes over the sum of its kernels' mean
+    durations."""
+    t = sum(ms[k] for k in Engine.COULOMB) * 1e-3
+    b = sum(kernel_bytes(p, "coulomb") for p in plist)
+    return {"bound": "hbm", "achieved": round(b / t / 1e9, 2), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": round(b / t / HBM_PEAK, 5), "bytes_per_evaluation": b, "us_per_evaluation": round(t * 1e6, 2)}
+
+
+def roofline_step(p, ms_per_step):
+    """The whole step: algorithmic bytes per step over the timed wall time
+    per step."""
+    a, c = step_bytes(p)
+    b = a + c / p.multistep
+    return {"bound": "hbm", "achieved": round(b / (ms_per_step * 1e-3) / 1e9, 2), "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s", "frac": round(b / (ms_per_step * 1e-3) / HBM_PEAK, 5), "bytes_per_step": int(b)}
 
 
 def prepare(seed, equil_steps):
@@ -97,6 +111,13 @@ def prepare(seed, equil_steps):
     return pos, vel, acc
 
 
+def aligned_warmup(warmup, multistep):
+    """Warm-up steps rounded up to whole Coulomb cycles, so that the timed
+    steps start at a step n with n % multistep == 1: whole cycles of them are
+    graph-replayed (md_run issues steps before a cycle boundary one by one)."""
+    return max(multistep, -(-warmup // multistep) * multistep)
+
+
 def time_engine(prec, state, steps, warmup, seed):
     p = config("default", prec, seed=seed)
     e = Engine(p)
@@ -105,9 +126,10 @@ def time_engine(prec, state, steps, warmup, seed):
     e.build_lists()
     e.compute_force(0, MD_NOISE_PHILOX)
     e.tuned = e.autotune(3)  # launch forms only: results do not depend on them
-    e.run(1, warmup)
+    w = aligned_warmup(warmup, p.multistep)
+    e.run(1, w)
     t0 = time.perf_counter()
-    e.run(1 + warmup, steps)  # ends with a stream synchronize
+    e.run(1 + w, steps)  # ends with a stream synchronize
     dt = time.perf_counter() - t0
     return p, e, steps / dt, dt
 
@@ -274,6 +296,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "warmup_run": aligned_warmup(args.warmup, p32.multistep),
         "ms_per_step": round(1e3 * dt32 / args.steps, 5),
         "higher_is_better": True,
         "scaling": "weak",
@@ -310,6 +333,8 @@ def main():
     out["config"]["launch_forms"] = e32.tuned
     out["roofline"] = roofline(ms, [p32], p32.multistep)
     out["roofline"]["traffic"] = traffic_from_profile(out["roofline"]["kernel"])
+    out["roofline_coulomb"] = roofline_chain(ms, [p32])
+    out["roofline_step"] = roofline_step(p32, out["ms_per_step"])
     out["kernel_us"] = {k: round(v * 1e3, 3) for k, v in ms.items()}
     e32.close()
 
@@ -341,6 +366,7 @@ def main():
         rb["replicas"] = len(reqs)
         rb["particles"] = int(sum(r.params.NN for r in reqs))
         out["roofline_batch"] = rb
+        out["roofline_batch_coulomb"] = roofline_chain(msb, [r.params for r in reqs])
         out["kernel_us_batch"] = {k: round(v * 1e3, 3) for k, v in msb.items()}
 
     if not args.no_cpu:

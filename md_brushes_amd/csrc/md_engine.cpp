@@ -335,10 +335,14 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   if (!cc || !S.flags || !S.charge || !S.bstart || !S.bpart || !S.ctrl || !d_eta || !d_out1 ||
       !d_out2 || !d_hist || !d_hsum || !d_mol)
     return fail(MD_ENOMEM, "topology arrays");
-  (void)hipMemcpy(S.flags, flags.data(), nn * sizeof(int), hipMemcpyHostToDevice);
-  if (Nq) (void)hipMemcpy(S.charge, charge.data(), Nq * sizeof(int), hipMemcpyHostToDevice);
-  (void)hipMemcpy(S.bstart, bstart.data(), (nn + 1) * sizeof(int), hipMemcpyHostToDevice);
-  (void)hipMemcpy(S.bpart, bpart.data(), bpart.size() * sizeof(int), hipMemcpyHostToDevice);
+  hipError_t cerr = hipMemcpy(S.flags, flags.data(), nn * sizeof(int), hipMemcpyHostToDevice);
+  if (cerr == hipSuccess && Nq)
+    cerr = hipMemcpy(S.charge, charge.data(), Nq * sizeof(int), hipMemcpyHostToDevice);
+  if (cerr == hipSuccess)
+    cerr = hipMemcpy(S.bstart, bstart.data(), (nn + 1) * sizeof(int), hipMemcpyHostToDevice);
+  if (cerr == hipSuccess)
+    cerr = hipMemcpy(S.bpart, bpart.data(), bpart.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (cerr != hipSuccess) return fail(MD_EHIP, hip_msg(cerr, "topology upload"));
   S.cnt1 = dalloc<int>(2 * (size_t)ncell1);  // two count buffers, used in turn
   S.cell1_of = dalloc<int>(NN);
   if (S.cell1_of) (void)hipMemset(S.cell1_of, 0xff, NN * sizeof(int));  // no cell yet
@@ -1188,9 +1192,11 @@ static int download_sys(md_engine *e, Sys *s, double *pos, double *vel, double *
   const int NN = s->p.NN;
   int rc = e->sync();
   if (rc) return rc;
+  hipError_t err = hipSuccess;
   if (pos) {
     std::vector<double4> p4(NN);
-    (void)hipMemcpy(p4.data(), s->h.pos, NN * sizeof(double4), hipMemcpyDeviceToHost);
+    err = hipMemcpy(p4.data(), s->h.pos, NN * sizeof(double4), hipMemcpyDeviceToHost);
+    if (err != hipSuccess) return e->hip_fail(err, "positions");
     for (int i = 0; i < NN; i++) {
       pos[i] = p4[i].x;
       pos[NN + i] = p4[i].y;
@@ -1198,10 +1204,11 @@ static int download_sys(md_engine *e, Sys *s, double *pos, double *vel, double *
       pos[3 * (size_t)NN + i] = p4[i].w;
     }
   }
-  if (vel) (void)hipMemcpy(vel, s->h.vx, 3 * sizeof(double) * NN, hipMemcpyDeviceToHost);
-  if (acc) (void)hipMemcpy(acc, s->h.ax, 3 * sizeof(double) * NN, hipMemcpyDeviceToHost);
+  if (vel) err = hipMemcpy(vel, s->h.vx, 3 * sizeof(double) * NN, hipMemcpyDeviceToHost);
+  if (err == hipSuccess && acc) err = hipMemcpy(acc, s->h.ax, 3 * sizeof(double) * NN, hipMemcpyDeviceToHost);
+  if (err != hipSuccess) return e->hip_fail(err, "velocities / forces");
   Ctrl cc;
-  hipError_t err = hipMemcpy(&cc, s->h.ctrl, sizeof cc, hipMemcpyDeviceToHost);
+  err = hipMemcpy(&cc, s->h.ctrl, sizeof cc, hipMemcpyDeviceToHost);
   if (err != hipSuccess) return e->hip_fail(err, "control block");
   if (c) *c = cc;
   std::string msg;
@@ -1257,13 +1264,15 @@ extern "C" int md_download_coulomb(md_engine *e, double *acc_c) {
   int rc = e->sync();
   if (rc) return rc;
   const size_t n = 3 * (size_t)s->p.NN;
+  hipError_t err;
   if (s->fp32) {
     std::vector<float> f(n);
-    (void)hipMemcpy(f.data(), s->h.ccx, n * 4, hipMemcpyDeviceToHost);
+    err = hipMemcpy(f.data(), s->h.ccx, n * 4, hipMemcpyDeviceToHost);
     for (size_t k = 0; k < n; k++) acc_c[k] = f[k];
   } else {
-    (void)hipMemcpy(acc_c, s->h.ccx, n * 8, hipMemcpyDeviceToHost);
+    err = hipMemcpy(acc_c, s->h.ccx, n * 8, hipMemcpyDeviceToHost);
   }
+  if (err != hipSuccess) return e->hip_fail(err, "held Coulomb force");
   return e->check_all();
 }
 
@@ -1314,8 +1323,10 @@ extern "C" int md_download_list(md_engine *e, int32_t which, int32_t *pairs, int
     return MD_OK;
   }
   std::vector<int> cnt(n), nb((size_t)n * lcap);
-  (void)hipMemcpy(cnt.data(), lj ? s->h.nbn1 : s->h.nbn2, n * sizeof(int), hipMemcpyDeviceToHost);
-  (void)hipMemcpy(nb.data(), lj ? s->h.nb1 : s->h.nb2, nb.size() * sizeof(int), hipMemcpyDeviceToHost);
+  hipError_t err = hipMemcpy(cnt.data(), lj ? s->h.nbn1 : s->h.nbn2, n * sizeof(int), hipMemcpyDeviceToHost);
+  if (err == hipSuccess)
+    err = hipMemcpy(nb.data(), lj ? s->h.nb1 : s->h.nb2, nb.size() * sizeof(int), hipMemcpyDeviceToHost);
+  if (err != hipSuccess) return e->hip_fail(err, "lists");
   std::vector<std::pair<int, int>> pr;
   for (int t = 0; t < n; t++) {
     const int i = lj ? t : s->charge[t];
@@ -1434,8 +1445,10 @@ extern "C" int md_force_parts(md_engine *e, double *f_short, double *f_coul) {
     else mdk::launch_coulomb<double>(P, e->st, s->d_out2);
   }
   int rc = e->check_all();
-  if (f_short) (void)hipMemcpy(f_short, s->d_out1, n * sizeof(double), hipMemcpyDeviceToHost);
-  if (f_coul) (void)hipMemcpy(f_coul, s->d_out2, n * sizeof(double), hipMemcpyDeviceToHost);
+  hipError_t err = hipSuccess;
+  if (f_short) err = hipMemcpy(f_short, s->d_out1, n * sizeof(double), hipMemcpyDeviceToHost);
+  if (err == hipSuccess && f_coul) err = hipMemcpy(f_coul, s->d_out2, n * sizeof(double), hipMemcpyDeviceToHost);
+  if (err != hipSuccess) return e->hip_fail(err, "force parts");
   return rc;
 }
 
@@ -1870,8 +1883,10 @@ extern "C" int md_batch_observe(md_engine *e, int32_t slot, double *height, doub
   const int nb = cdiv(s->p.NN, mdk::kBlock);
   std::vector<double> hs(nb);
   std::vector<int> hist(mdk::kHistBins);
-  (void)hipMemcpy(hs.data(), s->d_hsum, nb * sizeof(double), hipMemcpyDeviceToHost);
-  (void)hipMemcpy(hist.data(), s->d_hist, hist.size() * sizeof(int), hipMemcpyDeviceToHost);
+  hipError_t err = hipMemcpy(hs.data(), s->d_hsum, nb * sizeof(double), hipMemcpyDeviceToHost);
+  if (err == hipSuccess)
+    err = hipMemcpy(hist.data(), s->d_hist, hist.size() * sizeof(int), hipMemcpyDeviceToHost);
+  if (err != hipSuccess) return e->hip_fail(err, "observables");
   double sum = 0;
   for (double v : hs) sum += v;
   if (height) *height = sum / s->p.Npe;
@@ -1968,10 +1983,12 @@ struct StateHead {
   int64_t NN, fp32, ncell1, ccap1, nbcap1;
 };
 
+size_t state_bytes(size_t NN, size_t rs, size_t ncell1, size_t ccap1, size_t nbcap1) {
+  return sizeof(StateHead) + NN * 10 * 8 + 3 * NN * rs + NN * 4 + nbcap1 * NN * 4 + 2 * ncell1 * 4 +
+         ncell1 * ccap1 * 4 + NN * 4 + sizeof(Ctrl);
+}
 size_t state_bytes(const Sys *s) {
-  const size_t NN = s->p.NN, rs = s->fp32 ? 4 : 8;
-  return sizeof(StateHead) + NN * 10 * 8 + 3 * NN * rs + NN * 4 + (size_t)s->h.nbcap1 * NN * 4 +
-         2 * (size_t)s->ncell1 * 4 + (size_t)s->ncell1 * s->h.ccap1 * 4 + NN * 4 + sizeof(Ctrl);
+  return state_bytes(s->p.NN, s->fp32 ? 4 : 8, s->ncell1, s->h.ccap1, s->h.nbcap1);
 }
 }  // namespace
 
@@ -2024,18 +2041,24 @@ extern "C" int md_state_load(md_engine *e, const void *buf, int64_t len) {
   ENGINE_CHECK(e && buf);
   FIRST(e);
   StateHead hd;
-  if ((size_t)len < sizeof hd) return e->fail(MD_EINVAL, "state too short");
+  if (len < (int64_t)sizeof hd) return e->fail(MD_EINVAL, "state too short");
   std::memcpy(&hd, buf, sizeof hd);
   if (std::memcmp(hd.magic, "MDSTATE1", 8) || hd.NN != s->p.NN || hd.fp32 != (int64_t)s->fp32 ||
       hd.ncell1 != s->ncell1)
     return e->fail(MD_EINVAL, "state of another system or precision");
+  // capacities as size_lists can produce them (cell members at most 128, a
+  // list at most every other particle, rounded to 8); the whole buffer is
+  // checked before anything of the engine changes
+  if (hd.ccap1 <= 0 || hd.ccap1 > 128 || hd.nbcap1 <= 0 || hd.nbcap1 > hd.NN + 8)
+    return e->fail(MD_EINVAL, "state with invalid list capacities");
   const size_t NN = s->p.NN, rs = s->fp32 ? 4 : 8;
+  if ((uint64_t)len < state_bytes(NN, rs, (size_t)hd.ncell1, (size_t)hd.ccap1, (size_t)hd.nbcap1))
+    return e->fail(MD_EINVAL, "state too short");
   const char *in = (const char *)buf + sizeof hd;
   int rc = upload_sys(e, s, (const double *)in, (const double *)in + 4 * NN, (const double *)in + 7 * NN);
   if (rc) return rc;
   rc = s->set_list_caps((int)hd.ccap1, (int)hd.nbcap1);
   if (rc) return e->fail(rc, s->err);
-  if ((size_t)len < state_bytes(s)) return e->fail(MD_EINVAL, "state too short");
   in += NN * 10 * 8;
   auto put = [&](void *dst, size_t n) {
     hipError_t err = hipMemcpy(dst, in, n, hipMemcpyHostToDevice);

@@ -24,6 +24,7 @@ program main
     if (prm%qq /= 0) call error_analysis
     step = 0
     call compute_force
+    call write_acc
   else
     call continue_read_data(i)
     if (read_state()) then  ! exact resume: the saved state, lists and forces
@@ -33,8 +34,14 @@ program main
       call lj_verlet_list
       if (prm%qq /= 0) call error_analysis
       call rescale_velocity
-      step = i
+      ! the reference's step counter is still 0 here (main.f90:58 runs before
+      ! any loop sets it), so this force includes a fresh Coulomb evaluation
+      ! that the following steps hold until their first multiple of multistep
+      step = 0
       call compute_force
+      call write_pos
+      call write_vel
+      call write_acc
     end if
   end if
 

@@ -275,6 +275,38 @@ contains
     deallocate(phi)
   end subroutine histogram
 
+  ! write_pos, write_vel, write_acc (in_output.f90:2428-2452, 2482-2507,
+  ! 2551-2576): the current state as text, 4F17.6 / 3F17.6
+  subroutine write_pos
+    integer :: i
+    call fetch_state
+    open(30, file='./data/pos.txt')
+      do i = 1, NN
+        write(30,'(4F17.6)') pos(i,1), pos(i,2), pos(i,3), pos(i,4)
+      end do
+    close(30)
+  end subroutine write_pos
+
+  subroutine write_vel
+    integer :: i
+    call fetch_state
+    open(32, file='./data/vel.txt')
+      do i = 1, NN
+        write(32,'(3F17.6)') vel(i,1), vel(i,2), vel(i,3)
+      end do
+    close(32)
+  end subroutine write_vel
+
+  subroutine write_acc
+    integer :: i
+    call fetch_state
+    open(37, file='./data/acc.txt')
+      do i = 1, NN
+        write(37,'(3F17.6)') acc(i,1), acc(i,2), acc(i,3)
+      end do
+    close(37)
+  end subroutine write_acc
+
   ! write_pos1 / write_vel1 (in_output.f90:2473-2478, 2529-2546): text
   ! checkpoint and the restart flag file.
   subroutine write_checkpoint(j)

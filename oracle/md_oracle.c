@@ -15,7 +15,8 @@
  * Pinned by tests/test_oracle.py against: closed forms, brute-force pair
  * enumeration, finite differences of the closed-form potentials, numpy.fft,
  * the B-spline recursion (compt_acc.f90:1667-1687), standard Ewald
- * (compt_acc.f90:1052-1135), the NaCl Madelung constant, Random123's
+ * (compt_acc.f90:1052-1135), the exact 2D-periodic Ewald sum (which the
+ * vacuum-stretched 3D sum plus the slab correction must reach), Random123's
  * published Philox4x32-10 known-answer vectors and the integrator's
  * closed-form single-particle solution.
  */

@@ -26,6 +26,28 @@ def _params():
     return p
 
 
+def _host_params(raw):
+    """The run's parameters as the host sets them: its mesh rounding and seed."""
+    p = derive(raw)
+    for d in range(3):
+        k = 4
+        while k < p.ng[d]:
+            k *= 2
+        p.ovr_ng[d] = k
+    p.seed = 20261203
+    return derive(p)
+
+
+def _run_host(d, raw, env=None):
+    write_param_files(raw, d / "system_data.txt", d / "force_data.txt", d / "q_arm.txt")
+    (d / "data").mkdir(exist_ok=True)
+    r = subprocess.run([EXE], cwd=d, capture_output=True, text=True, timeout=120,
+                       env=dict(os.environ, **(env or {})))
+    print(r.stdout[-1500:], r.stderr[-1500:])
+    assert r.returncode == 0 and "Finished!" in r.stdout
+    return r.stdout
+
+
 def test_fortran_host_matches_c_abi(tmp_path):
     assert os.path.exists(EXE), "build the Fortran host first"
     raw = _params()
@@ -45,14 +67,7 @@ def test_fortran_host_matches_c_abi(tmp_path):
     assert rms.shape == (1, 6) and rms[0, 0] == 0 and 0 < rms[0, 1] < 1e-2 and rms[0, 2] > 0
     assert "Rms force of coulomb force is:" in r.stdout
     # the same run through the C-ABI: the host's mesh rounding, seed and flow
-    p = derive(raw)
-    for d in range(3):
-        k = 4
-        while k < p.ng[d]:
-            k *= 2
-        p.ovr_ng[d] = k
-    p.seed = 20261203
-    p = derive(p)
+    p = _host_params(raw)
     pos = initialize_position(p, p.seed)
     vel = initialize_velocity(p, p.seed + 1)
     e = Engine(p)
@@ -124,3 +139,57 @@ def test_fortran_exact_resume(tmp_path):
             assert np.array_equal(x, y), k
         o += n
     assert np.array_equal(a[o:], b[o:])  # control block
+
+
+def test_fortran_text_restart(tmp_path):
+    """A restart from the text checkpoint (no MD_EXACT_RESUME) follows the
+    reference's restart (main.f90:40-64): positions and velocities read at
+    six decimals (in_output.f90:605-618), lists rebuilt, velocities
+    rescaled, then a force whose Coulomb part is evaluated afresh (the step
+    counter is still 0 there, compt_acc.f90:841) and held by the following
+    steps; the run continues at the checkpoint step + 1.  The same sequence
+    through the C-ABI reproduces the host's force and trajectory, and its
+    held Coulomb force equals the oracle's."""
+    import sys
+    sys.path.insert(0, os.path.dirname(__file__))
+    import orc
+    from systems import heights_ref
+    assert os.path.exists(EXE), "build the Fortran host first"
+    raw = _params()
+    _apply(raw, [("StepNum", 10), ("DeltaStep3", 50)])
+    print("[fortran text restart] steps 1..50", flush=True)
+    _run_host(tmp_path, raw)  # checkpoint at step 50
+    pos_txt = np.loadtxt(tmp_path / "data" / "pos1.txt")
+    vel_txt = np.loadtxt(tmp_path / "data" / "vel1.txt")
+    _apply(raw, [("StepNum", 40)])
+    print("[fortran text restart] restart, steps 51..80", flush=True)
+    out = _run_host(tmp_path, raw)
+    assert "exact resume" not in out
+    acc_host = np.loadtxt(tmp_path / "data" / "acc.txt")
+    vel_host = np.loadtxt(tmp_path / "data" / "vel.txt")
+    h = np.loadtxt(tmp_path / "data" / "height.txt")
+    assert list(h[-3:, 0]) == [60, 70, 80]
+
+    p = _host_params(raw)
+    e = Engine(p)
+    e.upload(pos_txt, vel_txt)
+    e.build_lists()
+    e.rescale_velocity()
+    e.compute_force(0, MD_NOISE_PHILOX)
+    pos0, vel0, acc0, _ = e.download()
+    fc = e.coulomb()
+    assert np.abs(vel0 - vel_host).max() < 1e-6
+    assert np.abs(acc0 - acc_host).max() < 1e-5
+    o = orc.Oracle(p)
+    o.set_state(pos_txt)
+    o.build_lists()
+    ref = o.coulomb()
+    charged = np.abs(pos_txt[:, 3]) > 0
+    assert np.abs(fc[charged]).max() > 1e-2  # the held Coulomb force is there
+    assert orc.rel_rms(fc, ref) < 1e-10
+    e.run(51, 30)
+    pg = e.download()[0]
+    ref_h = heights_ref(pg, p)
+    for c, k in enumerate(["h_avg", "hl_max", "hl_end", "hl_avg", "hs_max", "hs_end", "hs_branch", "hs_avg"]):
+        assert abs(h[-1, 1 + c] - ref_h[k]) < 1e-5, k
+    e.close()

@@ -464,3 +464,16 @@ def test_exact_resume(prec):
     with pytest.raises(MDError):
         e4.load_state(state)
     e4.close()
+    # a truncated state (a checkpoint file cut short) and one with corrupt
+    # list capacities are refused before the engine changes
+    e5 = Engine(p)
+    e5.upload(pos, vel)
+    before = e5.download()
+    bad = bytearray(state)
+    bad[40:48] = np.int64(1 << 40).tobytes()  # nbcap1
+    for st in (state[: len(state) // 2], state[:-1], bytes(bad)):
+        with pytest.raises(MDError):
+            e5.load_state(st)
+    for x, y in zip(before, e5.download()):
+        assert np.array_equal(x, y)
+    e5.close()

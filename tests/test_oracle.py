@@ -1,7 +1,9 @@
 """Pins of the FP64 oracle against facts it does not compute itself:
 published known answers, closed forms, brute force, numpy, finite
-differences of the textbook potentials, and standard Ewald.  (The reference
-ships no tests or golden vectors; see tests/golden/README.md.)
+differences of the textbook potentials, standard Ewald and the 2D-periodic
+Ewald sum.  The reference ships no tests, fixtures or golden vectors, and it
+cannot be built here (it needs FFTW3, absent from the image), so every pin is
+one of these independent facts.
 """
 import math
 
@@ -389,6 +391,80 @@ def test_error_analysis_rmsf_falls_with_mesh():
     assert out[1] < out[0] / 4, out
 
 
+# ------------------------------------------------------- slab correction
+
+def _slab_system(z_empty, tol=5.5, ng=(32, 32, 128), order=6):
+    """A 10 x 10 x 10 slab (x, y periodic) with Z_empty * Lx of vacuum
+    above it.  The pair-loop branch of the Ewald rule applies (r_c = L/2,
+    alpha = tol / r_c)."""
+    p = _base()
+    _apply(p, [("Nga", 0), ("Ngl", 4), ("Nml", 3), ("man_l", 1), ("man_s", 1), ("qqi", 1.0),
+               ("ion_ratio", 1.0), ("Lz", 10.0), ("Z_empty", z_empty), ("sigmag", 0.04), ("EF", 0.0),
+               ("ovr_ng", ng), ("ordr", (order,) * 3), ("tol", tol)])
+    return orc.derive(p)
+
+
+def _slab_charges(p):
+    """16 charges of +-1 with a large dipole along z (positive ones low,
+    negative ones high), every other particle uncharged."""
+    from md_brushes_amd import build_topology
+    charge = build_topology(p)[0] - 1
+    rng = np.random.default_rng(1)
+    pos = np.zeros((p.NN, 4))
+    pos[:, 0] = rng.uniform(-p.Lx / 2, p.Lx / 2, p.NN)
+    pos[:, 1] = rng.uniform(-p.Ly / 2, p.Ly / 2, p.NN)
+    pos[:, 2] = rng.uniform(1, 9, p.NN)
+    ch = charge[:16]
+    q = np.array([1.0] * 8 + [-1.0] * 8)
+    pos[ch, 3] = q
+    pos[ch[:8], 2] = rng.uniform(1, 4, 8)
+    pos[ch[8:], 2] = rng.uniform(6, 9, 8)
+    return pos, ch, q
+
+
+@pytest.fixture(scope="module")
+def ew2d_reference():
+    import ewald2d
+    p = _slab_system(3.0)
+    pos, ch, q = _slab_charges(p)
+    return pos, ch, q, ewald2d.forces(pos[ch, :3], q, p.Lx, p.Ly, p.alpha, 20, p.lb / p.Beta)
+
+
+@pytest.mark.parametrize("z_empty,tol", [(2.0, 1e-6), (4.0, 1e-8)])
+def test_slab_correction_matches_2d_ewald(ew2d_reference, z_empty, tol):
+    """Standard Ewald in the vacuum-stretched box plus real space and the
+    Yeh-Berkowitz correction (compt_acc.f90:1040-1048) equals the exact
+    2D-periodic Ewald sum (tests/ewald2d.py, an independent formula) up to
+    exp(-2 pi gap / L); a dropped, sign-flipped or wrong-volume slab term
+    misses it by more than 1%."""
+    pos, ch, q, f2d = ew2d_reference
+    p = _slab_system(z_empty)
+    o = orc.Oracle(p)
+    o.set_state(pos)
+    kmax = 2 * p.alpha * 6.0  # exp(-kmax^2 / 4 alpha^2) = exp(-36)
+    K = [math.ceil(kmax * L / (2 * math.pi)) for L in (p.Lx, p.Ly, p.gdim[2])]
+    f = o.standard_ewald(p.alpha, K) + o.real_space()
+    other = np.setdiff1d(np.arange(p.NN), ch)
+    assert np.all(f[other] == 0)
+    assert orc.rel_rms(f[ch], f2d) < tol
+    c1 = p.lb / p.Beta
+    slab = np.zeros_like(f2d)
+    slab[:, 2] = -4 * math.pi / (p.Lx * p.Ly * p.gdim[2]) * q * c1 * np.sum(q * pos[ch, 2])
+    for wrong in (f[ch] - slab, f[ch] - 2 * slab, f[ch] - slab + slab * p.Zfac):
+        assert orc.rel_rms(wrong, f2d) > 1e-2
+
+
+def test_spme_slab_matches_2d_ewald(ew2d_reference):
+    """The production Coulomb force (SPME + real space + slab) at order 6,
+    mesh spacing 0.31, alpha = 0.8 reaches the 2D-periodic sum to its mesh
+    error."""
+    pos, ch, q, f2d = ew2d_reference
+    p = _slab_system(3.0, tol=4.0)
+    o = orc.Oracle(p)
+    o.set_state(pos)
+    assert orc.rel_rms(o.coulomb()[ch], f2d) < 2e-5
+
+
 # ------------------------------------------------------------ integrator
 
 def _neutral_chains():
@@ -426,25 +502,38 @@ def test_energy_conserved_without_friction():
 
 
 def test_friction_decay_closed_form():
-    # a free particle's velocity obeys v_{n+1} = v_n (1 - xi dt/2)/(1 + xi dt/2)
-    p = _neutral_chains()
-    p.xi = 1.0
+    """A particle that feels no force keeps v_{n+1} = v_n (1 - xi dt/2) /
+    (1 + xi dt/2) (init_update.f90:581,606 with F = 0)."""
+    # charged chains with their counterions and lB = 0: a counterion has no
+    # bond and, away from every other particle and the walls, no force
+    p = _base()
+    _apply(p, [("Nga", 0), ("Ngl", 9), ("Nml", 20), ("man_l", 1), ("qqi", 0.0), ("Lz", 30.0),
+               ("sigmag", 0.05), ("lb", 0.0), ("EF", 0.0), ("ovr_ng", (16, 16, 64)),
+               ("ovr_alpha", 0.5)])
+    p = orc.derive(p)
     pos = initialize_position(p, 3)
-    pos[:, 2] += 0.0
     vel = np.zeros((p.NN, 3))
-    o = orc.Oracle(p)
-    # one particle far from every other one and from the walls
     far = p.NN - 1
-    pos[far, :3] = [0.0, 0.0, 15.0]
+    assert far >= p.Npe  # an ion
+    grid = [(x, y, z) for z in (8.0, 12.0, 16.0, 20.0, 24.0)
+            for x in np.linspace(-p.Lx / 2, p.Lx / 2, 9)[:-1]
+            for y in np.linspace(-p.Ly / 2, p.Ly / 2, 9)[:-1]]
+
+    def clear(c):
+        d = pos[:far, :3] - np.array(c)
+        d[:, :2] -= np.array([p.Lx, p.Ly]) * np.round(d[:, :2] / np.array([p.Lx, p.Ly]))
+        return np.sqrt((d ** 2).sum(1)).min()
+    best = max(grid, key=clear)
+    assert clear(best) > p.rcl_s  # beyond every LJ cut (the largest size class)
+    pos[far, :3] = best
     vel[far] = [0.3, -0.2, 0.1]
+    o = orc.Oracle(p)
     o.set_state(pos, vel)
     o.build_lists()
-    o.compute_force(0)
-    acc0 = o.state()[2][far]
-    if np.abs(acc0).max() > 0:
-        pytest.skip("particle not isolated")
+    assert o.compute_force(0) == 0
+    assert np.all(o.state()[2][far] == 0)
     for s in range(1, 6):
-        o.new_position(s)
+        assert o.new_position(s) == 0
     v = o.state()[1][far]
     c = (1 - p.xi * p.dt / 2) / (1 + p.xi * p.dt / 2)
     assert np.allclose(v, np.array([0.3, -0.2, 0.1]) * c ** 5, rtol=1e-13)

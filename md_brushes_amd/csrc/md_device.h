@@ -47,6 +47,10 @@ struct Ctrl {
 struct Rep {
   // sizes
   int NN, Nq, N_anchor, real_verlet;
+  // the real-space force walks the reference-form list (real_verlet and
+  // fewer than three list cells along x or y, where the reference's stencil
+  // lists a neighbour cell's pairs more than once); otherwise the cell sweep
+  int real_list;
   int Npe;                         // polymer beads are particles 0..Npe-1
   int Nga, arm, Nma, Ngl, Nml;     // stars (first), then linear chains
   int preheat_end;                 // batch replicas: steps 1..preheat_end rescale

@@ -31,8 +31,9 @@ using mdk::WMap;
 namespace {
 
 constexpr double kPi = 3.141592653589793;
-// part, chg, chgG, fx, fy, fz, cell1, cell1w, cell2, cell3, cell3c, brick, plane, partG
-constexpr int kMaps = 14;
+// part, chg, chgG, fx, fy, fz, cell1, cell1w, cell2, cell3, cell3c, brick, plane, partG,
+// rlst, cell2s
+constexpr int kMaps = 16;
 // LDS a workgroup may take on this device (both planes of a fused FFT
 // plane kernel must fit)
 size_t lds_limit() {
@@ -179,7 +180,8 @@ struct Sys {
   }
   int setup(const md_params *pin, bool want_fp32);
   int size_lists(const double *pos);
-  int set_list_caps(int ccap1, int nbcap1);  // LJ cell and list capacities as given
+  // LJ cell and list capacities, and the walked real-space list's, as given
+  int set_list_caps(int ccap1, int nbcap1, int nbcap2);
   // descriptor whose canonical positions are buffer q
   Rep table(int q) const {
     Rep t = h;
@@ -284,13 +286,22 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
     S.rcel2[d] = S.real_verlet ? L3[d] / S.nc2[d] : L3[d];
   }
   if (S.nc1[0] < 1 || S.nc1[1] < 1 || S.nc1[2] < 1) return fail(MD_EINVAL, "box smaller than rvl");
+  if (S.real_verlet && (S.nc2[0] < 1 || S.nc2[1] < 1 || S.nc2[2] < 1))
+    return fail(MD_EINVAL, "box smaller than the real-space list radius");
   ncell1 = S.nc1[0] * S.nc1[1] * S.nc1[2];
   ncell2 = S.real_verlet ? S.nc2[0] * S.nc2[1] * S.nc2[2] : 0;
+  // with fewer than three list cells along x or y the reference's stencil
+  // lists a neighbour cell's pairs twice (or three times with one cell) and
+  // its real-space sum counts every repeat: such replicas keep the
+  // reference-form list, rebuilt at the reference's rebuild steps, and walk
+  // it instead of sweeping cells
+  S.real_list = S.real_verlet && (S.nc2[0] <= 2 || S.nc2[1] <= 2);
+  if (S.real_list) real_list_wanted = true;
   // real-space sweep cells: edge >= r_c (one cell when the box is shorter);
   // with fewer than three cells along x or y the stencil visits each once
   S.real_cells = 0;
   ncell3 = 0;
-  if (coulomb) {
+  if (coulomb && !S.real_list) {
     for (int d = 0; d < 3; d++) {
       S.nc3[d] = std::max(1, (int)(L3[d] / p.rcc));
       S.rcel3[d] = L3[d] / S.nc3[d];
@@ -485,6 +496,8 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   nb[11] = coulomb ? nbin : 0;
   nb[12] = coulomb && S.plane_fused ? S.K[2] : 0;
   nb[13] = cdiv(NN, mdk::kSRParticles);
+  nb[14] = S.real_list ? cdiv(Nq, mdk::kBlock) : 0;
+  nb[15] = S.real_list ? ncell2 : 0;
   S.mem1 = S.mem2 = S.mem3 = nullptr;
   S.pos3s = nullptr;
   S.nb1 = S.nb2 = nullptr;
@@ -495,8 +508,14 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
 
 // Sizes cell and list capacities from a configuration: twice the largest
 // occupancy and list length found on the host, plus headroom.
-int Sys::set_list_caps(int ccap1, int nbcap1) {
+int Sys::set_list_caps(int ccap1, int nbcap1, int nbcap2) {
   if (ccap1 <= 0 || nbcap1 <= 0) return fail(MD_EINVAL, "list capacities");
+  if (h.real_list && nbcap2 != h.nbcap2) {
+    if (h.nb2) dfree(h.nb2, (size_t)h.Nq * h.nbcap2 * 4);
+    h.nb2 = dalloc<int>((size_t)h.Nq * nbcap2);
+    h.nbcap2 = nbcap2;
+    if (!h.nb2) return fail(MD_ENOMEM, "real-space list");
+  }
   if (ccap1 != h.ccap1) {
     if (h.mem1) dfree(h.mem1, (size_t)ncell1 * h.ccap1 * 4);
     h.mem1 = dalloc<int>((size_t)ncell1 * ccap1);
@@ -761,6 +780,11 @@ struct md_engine {
   int error_analysis(Sys *s, const Plan &P, double *out, double *acc_se);
   int build_graph(int q);
   int run_cycles(int ncycles);
+  bool any_real_list() const {
+    for (const Sys *x : sys)
+      if (x->h.real_list) return true;
+    return false;
+  }
   int index_of(const Sys *s) const {
     for (size_t k = 0; k < sys.size(); k++)
       if (sys[k] == s) return (int)k;
@@ -798,7 +822,7 @@ int md_engine::rebuild() {
   if (rsw && !std::strcmp(rsw, "wg")) wg = 1;
   for (Sys *x : sys) {
     x->h.real_wg = wg;
-    x->nb[10] = wg ? x->nb10_wg : x->nb10_wave;
+    x->nb[10] = wg ? x->nb10_wg : x->nb10_wave;  // 0 where the list is walked
   }
   const bool reps_host_wg = wg != 0;  // one form for every replica of the engine
   // z pass: few lines per workgroup (32-byte row pieces) while the engine's
@@ -890,8 +914,9 @@ int md_engine::rebuild() {
   auto fill = [&](Plan &P, Rep *reps, int nrep, const int *maps, const int *totals) {
     P.reps = reps;
     P.nrep = nrep;
-    WMap *w[kMaps] = {&P.part,  &P.chg,   &P.chgG,  &P.fx,    &P.fy,    &P.fz,   &P.cell1,
-                      &P.cell1w, &P.cell2, &P.cell3, &P.cell3c, &P.brick, &P.plane, &P.partG};
+    WMap *w[kMaps] = {&P.part,  &P.chg,   &P.chgG,  &P.fx,    &P.fy,    &P.fz,
+                      &P.cell1, &P.cell1w, &P.cell2, &P.cell3, &P.cell3c, &P.brick,
+                      &P.plane, &P.partG,  &P.rlst,  &P.cell2s};
     for (int k = 0; k < kMaps; k++) *w[k] = WMap{maps + k * (nrep + 1), nrep, totals[k]};
     P.vpart = vpart;
     P.fp32 = fp32;
@@ -1077,6 +1102,7 @@ static void launch_cycle(md_engine *e, int q) {
     any_coulomb |= s->coulomb;
     any_preheat |= s->h.preheat_end > 0;
   }
+  const bool any_rl = e->any_real_list();
   for (int k = 1; k <= e->multistep; k++) {
     const Plan &P = e->planp[q ^ ((k - 1) & 1)];
     if (any_coulomb && k == e->multistep) {
@@ -1086,6 +1112,7 @@ static void launch_cycle(md_engine *e, int q) {
     if (e->fp32) mdk::launch_short<float>(P, e->st, mode, -1, nullptr, nullptr);
     else mdk::launch_short<double>(P, e->st, mode, -1, nullptr, nullptr);
     mdk::launch_build(P, e->st, 1);  // the short-range kernel binned for a rebuild
+    if (any_rl) mdk::launch_real_list(P, e->st, false);
     // pre-heat rescale between steps (its step is a multiple of 20, so of
     // multistep when that divides 20; otherwise checked after every step);
     // it redoes the next step's drift with the rescaled velocities
@@ -1304,7 +1331,7 @@ extern "C" int md_build_lists(md_engine *e) {
   const Plan &Q = e->single[0];
   mdk::launch_set_flags(Q, e->st, 0);
   mdk::launch_lists(Q, e->st, 0);
-  mdk::launch_real_list(Q, e->st);
+  mdk::launch_real_list(Q, e->st, true);
   mdk::launch_set_flags(Q, e->st, 1);
   return e->check_all();
 }
@@ -1362,6 +1389,7 @@ extern "C" int md_update_verlet_list(md_engine *e) {
   FIRST(e);
   STATE_CHECK(e);
   mdk::launch_lists(P, e->st, 0);
+  if (s->h.real_list) mdk::launch_real_list(P, e->st, false);
   mdk::launch_set_flags(P, e->st, 1);
   hipError_t err = hipGetLastError();
   return err == hipSuccess ? MD_OK : e->hip_fail(err, "update_verlet_list");
@@ -1775,6 +1803,7 @@ extern "C" int md_batch_add(md_engine *e, const md_params *p, const double *pos,
     const Plan &P = e->single[e->index_of(s)];
     mdk::launch_set_flags(P, e->st, 0);
     mdk::launch_lists(P, e->st, 0);
+    if (s->h.real_list) mdk::launch_real_list(P, e->st, false);
     mdk::launch_set_flags(P, e->st, 1);
     rc = e->launch_compute_force(P, s, 0, MD_NOISE_PHILOX, nullptr, false);
   }
@@ -1979,16 +2008,24 @@ extern "C" int md_batch_heights(md_engine *e, int32_t slot, double *out) {
 // checkpoint and rescales the velocities, main.f90:40-64).
 namespace {
 struct StateHead {
-  char magic[8];  // "MDSTATE1"
+  char magic[8];  // "MDSTATE2"
   int64_t NN, fp32, ncell1, ccap1, nbcap1;
+  int64_t Nq, ncell2, nbcap2;  // the walked reference-form real-space list (0: none)
 };
 
-size_t state_bytes(size_t NN, size_t rs, size_t ncell1, size_t ccap1, size_t nbcap1) {
+// header, pos/vel/acc, held Coulomb force, LJ list lengths and slots, the
+// two LJ count buffers, LJ cell members, each particle's LJ cell, [real-space
+// list lengths and slots, its two count buffers,] control block
+size_t state_bytes(size_t NN, size_t rs, size_t ncell1, size_t ccap1, size_t nbcap1, size_t Nq,
+                   size_t ncell2, size_t nbcap2) {
   return sizeof(StateHead) + NN * 10 * 8 + 3 * NN * rs + NN * 4 + nbcap1 * NN * 4 + 2 * ncell1 * 4 +
-         ncell1 * ccap1 * 4 + NN * 4 + sizeof(Ctrl);
+         ncell1 * ccap1 * 4 + NN * 4 + (nbcap2 ? Nq * 4 + nbcap2 * Nq * 4 + 2 * ncell2 * 4 : 0) +
+         sizeof(Ctrl);
 }
 size_t state_bytes(const Sys *s) {
-  return state_bytes(s->p.NN, s->fp32 ? 4 : 8, s->ncell1, s->h.ccap1, s->h.nbcap1);
+  const bool rl = s->h.real_list != 0;
+  return state_bytes(s->p.NN, s->fp32 ? 4 : 8, s->ncell1, s->h.ccap1, s->h.nbcap1, rl ? s->h.Nq : 0,
+                     rl ? s->ncell2 : 0, rl ? s->h.nbcap2 : 0);
 }
 }  // namespace
 
@@ -2009,12 +2046,16 @@ extern "C" int md_state_save(md_engine *e, void *buf, int64_t cap) {
   const size_t NN = s->p.NN, rs = s->fp32 ? 4 : 8;
   char *o = (char *)buf;
   StateHead hd{};
-  std::memcpy(hd.magic, "MDSTATE1", 8);
+  std::memcpy(hd.magic, "MDSTATE2", 8);
   hd.NN = (int64_t)NN;
   hd.fp32 = s->fp32;
   hd.ncell1 = s->ncell1;
   hd.ccap1 = s->h.ccap1;
   hd.nbcap1 = s->h.nbcap1;
+  const bool rl = s->h.real_list != 0;
+  hd.Nq = rl ? s->h.Nq : 0;
+  hd.ncell2 = rl ? s->ncell2 : 0;
+  hd.nbcap2 = rl ? s->h.nbcap2 : 0;
   std::memcpy(o, &hd, sizeof hd);
   o += sizeof hd;
   Ctrl c{};
@@ -2032,6 +2073,12 @@ extern "C" int md_state_save(md_engine *e, void *buf, int64_t cap) {
   if (err == hipSuccess) err = get(s->h.cnt1, 2 * (size_t)s->ncell1 * 4);
   if (err == hipSuccess) err = get(s->h.mem1, (size_t)s->ncell1 * s->h.ccap1 * 4);
   if (err == hipSuccess) err = get(s->h.cell1_of, NN * 4);
+  if (rl) {
+    if (err == hipSuccess) err = get(s->h.nbn2, (size_t)s->h.Nq * 4);
+    if (err == hipSuccess) err = get(s->h.nb2, (size_t)s->h.nbcap2 * s->h.Nq * 4);
+    if (err == hipSuccess) err = get(s->h.cnt2[0], (size_t)s->ncell2 * 4);
+    if (err == hipSuccess) err = get(s->h.cnt2[1], (size_t)s->ncell2 * 4);
+  }
   if (err != hipSuccess) return e->hip_fail(err, "state save");
   std::memcpy(o, &c, sizeof c);
   return MD_OK;
@@ -2043,21 +2090,24 @@ extern "C" int md_state_load(md_engine *e, const void *buf, int64_t len) {
   StateHead hd;
   if (len < (int64_t)sizeof hd) return e->fail(MD_EINVAL, "state too short");
   std::memcpy(&hd, buf, sizeof hd);
-  if (std::memcmp(hd.magic, "MDSTATE1", 8) || hd.NN != s->p.NN || hd.fp32 != (int64_t)s->fp32 ||
-      hd.ncell1 != s->ncell1)
+  const bool rl = s->h.real_list != 0;
+  if (std::memcmp(hd.magic, "MDSTATE2", 8) || hd.NN != s->p.NN || hd.fp32 != (int64_t)s->fp32 ||
+      hd.ncell1 != s->ncell1 || hd.Nq != (rl ? s->h.Nq : 0) || hd.ncell2 != (rl ? s->ncell2 : 0))
     return e->fail(MD_EINVAL, "state of another system or precision");
   // capacities as size_lists can produce them (cell members at most 128, a
   // list at most every other particle, rounded to 8); the whole buffer is
   // checked before anything of the engine changes
-  if (hd.ccap1 <= 0 || hd.ccap1 > 128 || hd.nbcap1 <= 0 || hd.nbcap1 > hd.NN + 8)
+  if (hd.ccap1 <= 0 || hd.ccap1 > 128 || hd.nbcap1 <= 0 || hd.nbcap1 > hd.NN + 8 ||
+      (rl && (hd.nbcap2 <= 0 || hd.nbcap2 > hd.Nq + 64)) || (!rl && hd.nbcap2 != 0))
     return e->fail(MD_EINVAL, "state with invalid list capacities");
   const size_t NN = s->p.NN, rs = s->fp32 ? 4 : 8;
-  if ((uint64_t)len < state_bytes(NN, rs, (size_t)hd.ncell1, (size_t)hd.ccap1, (size_t)hd.nbcap1))
+  if ((uint64_t)len < state_bytes(NN, rs, (size_t)hd.ncell1, (size_t)hd.ccap1, (size_t)hd.nbcap1,
+                                  (size_t)hd.Nq, (size_t)hd.ncell2, (size_t)hd.nbcap2))
     return e->fail(MD_EINVAL, "state too short");
   const char *in = (const char *)buf + sizeof hd;
   int rc = upload_sys(e, s, (const double *)in, (const double *)in + 4 * NN, (const double *)in + 7 * NN);
   if (rc) return rc;
-  rc = s->set_list_caps((int)hd.ccap1, (int)hd.nbcap1);
+  rc = s->set_list_caps((int)hd.ccap1, (int)hd.nbcap1, (int)hd.nbcap2);
   if (rc) return e->fail(rc, s->err);
   in += NN * 10 * 8;
   auto put = [&](void *dst, size_t n) {
@@ -2071,6 +2121,12 @@ extern "C" int md_state_load(md_engine *e, const void *buf, int64_t len) {
   if (err == hipSuccess) err = put(s->h.cnt1, 2 * (size_t)s->ncell1 * 4);
   if (err == hipSuccess) err = put(s->h.mem1, (size_t)s->ncell1 * s->h.ccap1 * 4);
   if (err == hipSuccess) err = put(s->h.cell1_of, NN * 4);
+  if (rl) {
+    if (err == hipSuccess) err = put(s->h.nbn2, (size_t)s->h.Nq * 4);
+    if (err == hipSuccess) err = put(s->h.nb2, (size_t)s->h.nbcap2 * s->h.Nq * 4);
+    if (err == hipSuccess) err = put(s->h.cnt2[0], (size_t)s->ncell2 * 4);
+    if (err == hipSuccess) err = put(s->h.cnt2[1], (size_t)s->ncell2 * 4);
+  }
   if (err == hipSuccess) err = hipMemcpy(s->h.ctrl, in, sizeof(Ctrl), hipMemcpyHostToDevice);
   if (err == hipSuccess) err = null_stream_done();
   if (err != hipSuccess) return e->hip_fail(err, "state load");

@@ -44,6 +44,9 @@ struct Plan {
   // ... mesh bricks (one wavefront each), z-planes
   WMap part{}, chg{}, chgG{}, fx{}, fy{}, fz{}, cell1{}, cell1w{}, cell2{}, cell3{}, cell3c{},
       brick{}, plane{}, partG{};  // partG: short-range workgroups of kSRParticles
+  // charges and reference-form list cells of the replicas whose real-space
+  // force walks that list (Rep::real_list)
+  WMap rlst{}, cell2s{};
   size_t lds_x = 0, lds_y = 0, lds_z = 0, lds_sort = 0, lds_sort3 = 0, lds_plane = 0;
   // LJ rebuild queue (replicas flagged this step) and the fixed grids of the
   // list kernels, which stride over the queued replicas' particles
@@ -73,7 +76,7 @@ void launch_kick_drift(const Plan &P, hipStream_t st, int store_v = 1);
 void launch_build(const Plan &P, hipStream_t st, int inc_step);
 void launch_lists(const Plan &P, hipStream_t st, int inc_step, hipEvent_t *ev = nullptr,
                   bool clear_queue = false);
-void launch_real_list(const Plan &P, hipStream_t st);
+void launch_real_list(const Plan &P, hipStream_t st, bool all);
 void launch_set_flags(const Plan &P, hipStream_t st, int clear);
 // gated: only batch replicas whose pre-heat rule is due (main.f90:67-75);
 // redrift: between replayed steps, whose next drift already ran with the

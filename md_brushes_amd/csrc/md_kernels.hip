@@ -621,12 +621,18 @@ __device__ __forceinline__ bool cell_of(double4 p, const int nc[3], const double
 
 // Neighbour cell l (0..26) of cell (a,b,c) in the loop order of
 // lj_verlet_list (compt_acc.f90:232-255): x outer, y, z inner; x,y periodic,
-// z open (-1 when outside).
+// z open (-1 when outside).  The reference wraps a neighbour index by one
+// period, so with two cells along x (or y) both dp = -1 and dp = +1 name the
+// other cell and its members are listed twice, and with one cell all three
+// name it (compt_acc.f90:232-248, 545-558): the lists keep those repeats
+// (kOnce = false).  The real-space sweep visits every cell once (kOnce).
+template <bool kOnce>
 __device__ __forceinline__ int stencil_cell(int l, int a, int b, int c, const int nc[3]) {
   const int dp = l / 9 - 1, dq = (l / 3) % 3 - 1;
-  // fewer than three cells along a periodic axis: each cell once
-  if ((nc[0] == 1 && dp != 0) || (nc[0] == 2 && dp < 0)) return -1;
-  if ((nc[1] == 1 && dq != 0) || (nc[1] == 2 && dq < 0)) return -1;
+  if (kOnce) {  // fewer than three cells along a periodic axis: each cell once
+    if ((nc[0] == 1 && dp != 0) || (nc[0] == 2 && dp < 0)) return -1;
+    if ((nc[1] == 1 && dq != 0) || (nc[1] == 2 && dq < 0)) return -1;
+  }
   int p = a + dp, q = b + dq, z = c + l % 3 - 1;
   if (z < 0 || z >= nc[2]) return -1;
   p = p >= nc[0] ? p - nc[0] : (p < 0 ? p + nc[0] : p);
@@ -707,7 +713,7 @@ __device__ __forceinline__ int stencil_offsets(int cell, const int nc[3], const 
     const int a = cell % nc[0], b = (cell / nc[0]) % nc[1], c = cell / (nc[0] * nc[1]);
     int cnt = 0;
     if (lane < 27) {
-      const int cl = stencil_cell(lane, a, b, c, nc);
+      const int cl = stencil_cell<true>(lane, a, b, c, nc);
       cids[lane] = cl;
       cnt = cl >= 0 ? scnt[cl] : 0;
     }
@@ -778,7 +784,7 @@ __device__ __forceinline__ void build_lj_pass(const Rep *__restrict__ reps, unsi
     // stencil cell of lane g (< 27), its count, exclusive offsets in the group
     int cl = -1, cn = 0;
     if (live && g < 27) {
-      cl = stencil_cell(g, a, b, cz, S.nc1);
+      cl = stencil_cell<false>(g, a, b, cz, S.nc1);
       const int ncell = S.nc1[0] * S.nc1[1] * S.nc1[2];
       if (cl >= 0) cn = min(S.cnt1[(size_t)buf * ncell + cl], S.ccap1);
     }
@@ -886,8 +892,10 @@ __global__ void k_lq_clear(unsigned long long *lq) { *lq = 0ull; }
 // The reference-form real-space list, real_verlet_list
 // (compt_acc.f90:491-626): cells of edge >= r_c + skin over the charges,
 // binned, sorted, then one thread per charge fills its full list.  Built on
-// request (md_build_lists); the step loop finds real-space pairs with the
-// cell sweep of k_real_cells instead.
+// request (md_build_lists) for every replica that keeps a list, and in the
+// step loop at the reference's rebuild steps for the replicas whose
+// real-space force walks it (Rep::real_list); the others find real-space
+// pairs with the cell sweep of k_real_cells instead.
 __global__ __launch_bounds__(kBlock) void k_bin_rl(const Rep *__restrict__ reps, WMap wm) {
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
@@ -964,7 +972,7 @@ __global__ __launch_bounds__(kBlock) void k_build_rl(const Rep *__restrict__ rep
   if (!cell_of(pi, S.nc2, S.rcel2, S.hLx, S.hLy, a, b, cz)) return;
   int n = 0;
   for (int l = 0; l < 27; l++) {
-    const int cell = stencil_cell(l, a, b, cz, S.nc2);
+    const int cell = stencil_cell<false>(l, a, b, cz, S.nc2);
     if (cell < 0) continue;
     const int cn = min(cnt[cell], S.ccap2);
     const int *cm = S.mem2 + (size_t)cell * S.ccap2;
@@ -1737,6 +1745,46 @@ __device__ __forceinline__ void real_pair<float>(float xi, float yi, float zi, f
   }
 }
 
+// real_space over the reference-form list (compt_acc.f90:1012-1038) for
+// replicas whose list cells number fewer than three along x or y
+// (Rep::real_list), where the reference lists a neighbour cell's pairs more
+// than once and sums each repeat: one thread per charge slot walks its list
+// in order and keeps the pairs within r_c.  The lb/Beta factor and the slab
+// term are applied by the Coulomb store.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_real_list(const Rep *__restrict__ reps, WMap wm) {
+  const int r = find_rep(wm, blockIdx.x);
+  const Rep &S = reps[r];
+  const int t = (blockIdx.x - wm.blk[r]) * kBlock + threadIdx.x;
+  if (t >= S.Nq) return;
+  const int i = S.charge[t];
+  const int n = min(S.nbn2[t], S.nbcap2);
+  const R Lx = (R)S.Lx, Ly = (R)S.Ly, hLx = (R)S.hLx, hLy = (R)S.hLy, rcc2 = (R)S.rcc2,
+          alpha = (R)S.alpha, alpha2 = (R)S.alpha2, c2 = (R)S.c2;
+  R xi, yi, zi, qi;
+  if constexpr (sizeof(R) == 4) {
+    const float4 p = S.pos32[i];
+    xi = p.x; yi = p.y; zi = p.z; qi = p.w;
+  } else {
+    const double4 p = S.pos[i];
+    xi = p.x; yi = p.y; zi = p.z; qi = p.w;
+  }
+  R fx = 0, fy = 0, fz = 0;
+  for (int k = 0; k < n; k++) {
+    const int j = S.nb2[(size_t)k * S.Nq + t];
+    if constexpr (sizeof(R) == 4) {
+      const float4 p = S.pos32[j];
+      real_pair<R>(xi, yi, zi, qi, p.x, p.y, p.z, p.w, true, Lx, Ly, hLx, hLy, rcc2, alpha, alpha2, c2,
+                   fx, fy, fz);
+    } else {
+      const double4 p = S.pos[j];
+      real_pair<R>(xi, yi, zi, qi, p.x, p.y, p.z, p.w, true, Lx, Ly, hLx, hLy, rcc2, alpha, alpha2, c2,
+                   fx, fy, fz);
+    }
+  }
+  store_real<R>(S, t, fx, fy, fz);
+}
+
 // Charges into the lists of the spreading tiles that hold their spline
 // points (one to eight) and into the cells of the real-space sweep.
 __global__ __launch_bounds__(kBlock) void k_bin_q(const Rep *__restrict__ reps, WMap wm) {
@@ -2092,7 +2140,7 @@ __device__ __forceinline__ void real_cells_block(const Rep *__restrict__ reps, W
       // stencil cell of lane l < 27, its count and exclusive offset
       int cl = -1, cn = 0;
       if (lane < 27) {
-        cl = stencil_cell(lane, ca, cb, cc, S.nc3);
+        cl = stencil_cell<true>(lane, ca, cb, cc, S.nc3);
         cn = cl >= 0 ? S.scnt3[cl] : 0;
       }
       int incl = cn;
@@ -2470,6 +2518,8 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) 
       hipLaunchKernelGGL((k_real_cells<R, false>), dim3(P.cell3c.total), dim3(kBlock), 0, st, P.reps,
                          P.cell3c);
   }
+  if (P.rlst.total)  // replicas whose real-space force walks the reference-form list
+    hipLaunchKernelGGL(k_real_list<R>, dim3(P.rlst.total), dim3(kBlock), 0, st, P.reps, P.rlst);
   hipLaunchKernelGGL(k_coul_store<R>, dim3(P.chg.total), dim3(kBlock), 0, st, P.reps, P.chg, out);
   mark();
 }
@@ -2504,12 +2554,14 @@ void launch_lists(const Plan &P, hipStream_t st, int inc_step, hipEvent_t *ev, b
   if (clear_queue) hipLaunchKernelGGL(k_lq_clear, dim3(1), dim3(1), 0, st, P.lq);
 }
 
-// The reference-form real-space list (real_verlet_list), on request.
-void launch_real_list(const Plan &P, hipStream_t st) {
-  if (!P.cell2.total) return;
-  hipLaunchKernelGGL(k_bin_rl, dim3(P.chg.total), dim3(kBlock), 0, st, P.reps, P.chg);
-  hipLaunchKernelGGL(k_sort_rl, dim3(P.cell2.total), dim3(kBlock), P.lds_sort, st, P.reps, P.cell2);
-  hipLaunchKernelGGL(k_build_rl, dim3(P.chg.total), dim3(kBlock), 0, st, P.reps, P.chg);
+// The reference-form real-space list (real_verlet_list): every replica
+// that keeps one (all), or the replicas whose real-space force walks it.
+void launch_real_list(const Plan &P, hipStream_t st, bool all) {
+  const WMap &q = all ? P.chg : P.rlst, &c = all ? P.cell2 : P.cell2s;
+  if (!c.total) return;
+  hipLaunchKernelGGL(k_bin_rl, dim3(q.total), dim3(kBlock), 0, st, P.reps, q);
+  hipLaunchKernelGGL(k_sort_rl, dim3(c.total), dim3(kBlock), P.lds_sort, st, P.reps, c);
+  hipLaunchKernelGGL(k_build_rl, dim3(q.total), dim3(kBlock), 0, st, P.reps, q);
 }
 
 void launch_set_flags(const Plan &P, hipStream_t st, int clear) {

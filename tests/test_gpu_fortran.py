@@ -62,14 +62,24 @@ def test_fortran_host_matches_c_abi(tmp_path):
     # linear hl_max, hl_end, hl_avg, star hs_max, hs_end, hs_branch, hs_avg
     assert h.shape == (10, 9) and np.all(np.isfinite(h))
     assert np.all(h[:, 2] >= h[:, 4]) and np.all(h[:, 5] >= h[:, 8])  # a chain's top above its mean
-    # error_analysis at the start (main.f90:30-33 of the reference), on the device
+    # error_analysis at the start (main.f90:30-33 of the reference), on the
+    # device, equals the oracle's at the same configuration (this box has two
+    # real-space list cells along x and y, so the production force sums the
+    # reference's repeated pairs and its rmsf is large: the reference's own)
     rms = np.loadtxt(tmp_path / "data" / "rms_force.txt", ndmin=2)
-    assert rms.shape == (1, 6) and rms[0, 0] == 0 and 0 < rms[0, 1] < 1e-2 and rms[0, 2] > 0
+    assert rms.shape == (1, 6) and rms[0, 0] == 0 and rms[0, 2] > 0
     assert "Rms force of coulomb force is:" in r.stdout
+    import sys
+    sys.path.insert(0, os.path.dirname(__file__))
+    import orc
     # the same run through the C-ABI: the host's mesh rounding, seed and flow
     p = _host_params(raw)
     pos = initialize_position(p, p.seed)
     vel = initialize_velocity(p, p.seed + 1)
+    o = orc.Oracle(p)
+    o.set_state(pos, vel)
+    o.build_lists()
+    assert abs(rms[0, 1] - o.error_analysis()[0]["rmsf"]) < 2e-6
     e = Engine(p)
     e.upload(pos, np.zeros_like(vel))
     e.build_lists()
@@ -79,8 +89,6 @@ def test_fortran_host_matches_c_abi(tmp_path):
     e.run(41, 60)
     pg, vg, _, _ = e.download()
     assert abs(pg[: p.Npe, 2].mean() - h[-1, 1]) < 1e-5
-    import sys
-    sys.path.insert(0, os.path.dirname(__file__))
     from systems import heights_ref
     ref = heights_ref(pg, p)
     for c, k in enumerate(["h_avg", "hl_max", "hl_end", "hl_avg", "hs_max", "hs_end", "hs_branch", "hs_avg"]):
@@ -120,12 +128,16 @@ def test_fortran_exact_resume(tmp_path):
     assert a.size == b.size
     # md_state_save's layout: header, pos, vel, acc, held Coulomb force, list
     # lengths, ELL list, the two cell-count buffers, cell members as binned,
-    # each particle's cell, control block
-    NN, fp32, ncell1, ccap1, nbcap1 = (int(x) for x in np.frombuffer(a[:48].tobytes(), np.int64)[1:6])
+    # each particle's cell, [the walked real-space list: lengths, slots, the
+    # two count buffers,] control block
+    NN, fp32, ncell1, ccap1, nbcap1, Nq, ncell2, nbcap2 = (
+        int(x) for x in np.frombuffer(a[:72].tobytes(), np.int64)[1:9])
     rs = 4 if fp32 else 8
     sizes = [32 * NN, 24 * NN, 24 * NN, 3 * NN * rs, 4 * NN, 4 * nbcap1 * NN, 8 * ncell1,
              4 * ncell1 * ccap1, 4 * NN]
-    o = 48
+    if nbcap2:
+        sizes += [4 * Nq, 4 * nbcap2 * Nq, 8 * ncell2]
+    o = 72
     for k, n in enumerate(sizes):
         x, y = a[o:o + n], b[o:o + n]
         if k == 7:  # members in binning order: the order of the atomics is free

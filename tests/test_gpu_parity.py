@@ -477,3 +477,94 @@ def test_exact_resume(prec):
     for x, y in zip(before, e5.download()):
         assert np.array_equal(x, y)
     e5.close()
+
+
+def _small_box(prec=MD_FP64, rsk=0.2):
+    """A 2x2-chain brush (Lx = 28.3) whose real-space list cells number two
+    along x and y: the reference's stencil (compt_acc.f90:538-560) then lists
+    the other cell's pairs twice and real_space sums both.  A thin skin
+    (rsk = 0.2) makes the real-space list rebuild every few steps."""
+    from md_brushes_amd import derive
+    from md_brushes_amd.configs import replica
+    p = replica(2, 50, 1, 3, 0.5, 20261203, prec)
+    p.rsk = rsk
+    p = derive(p)
+    assert p.real_verlet == 1 and int(p.Lx / p.rvc) == 2
+    from md_brushes_amd import initialize_position, initialize_velocity
+    return p, initialize_position(p, 31), initialize_velocity(p, 32)
+
+
+def test_small_box_reference_lists_and_forces():
+    """Lists with the reference's repeated pairs are bit-exact; the FP64
+    Coulomb force (which walks that list) and the FP32 gate match the
+    oracle."""
+    print("[small box] lists and forces", flush=True)
+    p, pos, vel = _small_box()
+    e = Engine(p)
+    e.upload(pos, vel)
+    e.build_lists()
+    o = orc.Oracle(p)
+    o.set_state(pos, vel)
+    o.build_lists()
+    for which in (0, 1):
+        g = e.lists(which)
+        r = o.lists(which)
+        r = r[np.lexsort((r[:, 1], r[:, 0]))]
+        assert np.array_equal(g, r), which
+    r = o.lists(1)
+    assert len(np.unique(r, axis=0)) < len(r)  # the repeats are there
+    fs, fc = e.force_parts()
+    assert orc.rel_rms(fs, o.lj_force() + o.fene_force()[0]) < 1e-11
+    ref_c = o.coulomb()
+    assert orc.rel_rms(fc, ref_c) < 1e-10
+    e.close()
+    q = p.copy()
+    q.precision = MD_FP32
+    e = Engine(q)
+    e.upload(pos, vel)
+    e.build_lists()
+    fs, fc = e.force_parts()
+    err = orc.rel_rms(fs + fc, o.lj_force() + o.fene_force()[0] + ref_c)
+    print(f"fp32 relative force rms {err:.3e}")
+    assert err < 1e-4
+    e.close()
+
+
+def test_small_box_trajectory():
+    """Steps with injected noise (stepwise) and a Philox run (graph-replayed
+    cycles) follow the oracle through several real-space list rebuilds and
+    Coulomb evaluations."""
+    print("[small box] trajectory", flush=True)
+    p, pos, vel = _small_box()
+    noise = etas(p.NN, 41, seed=3)
+    e = Engine(p)
+    e.upload(pos, vel)
+    e.build_lists()
+    o = orc.Oracle(p)
+    o.set_state(pos, vel)
+    o.build_lists()
+    assert o.compute_force(0, MD_NOISE_ETA, noise[0]) == 0
+    e.compute_force(0, MD_NOISE_ETA, noise[0])
+    for s in range(1, 41):
+        assert o.new_position(s, MD_NOISE_ETA, noise[s]) == 0
+        assert o.update_verlet_list() == 0
+        e.new_position(s, MD_NOISE_ETA, noise[s])
+        e.update_verlet_list()
+    pg, vg, ag, dg = e.download()
+    po, vo, ao, _, do = o.state()
+    assert np.abs(pg - po).max() < 1e-8
+    assert orc.rel_rms(ag, ao) < 1e-8
+    e.close()
+    e = Engine(p)
+    e.upload(pos, vel)
+    e.build_lists()
+    o = orc.Oracle(p)
+    o.set_state(pos, vel)
+    o.build_lists()
+    assert o.compute_force(0, MD_NOISE_PHILOX) == 0
+    e.compute_force(0, MD_NOISE_PHILOX)
+    assert o.run(1, 40) == 0
+    e.run(1, 40)
+    pg = e.download()[0]
+    assert np.abs(pg - o.state()[0]).max() < 1e-8
+    e.close()

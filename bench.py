@@ -111,11 +111,16 @@ def prepare(seed, equil_steps):
     return pos, vel, acc
 
 
+WARMUP_MIN = 100  # the first replays of a freshly instantiated cycle graph run slower
+
+
 def aligned_warmup(warmup, multistep):
-    """Warm-up steps rounded up to whole Coulomb cycles, so that the timed
-    steps start at a step n with n % multistep == 1: whole cycles of them are
-    graph-replayed (md_run issues steps before a cycle boundary one by one)."""
-    return max(multistep, -(-warmup // multistep) * multistep)
+    """Warm-up steps: at least WARMUP_MIN, rounded up to whole Coulomb cycles,
+    so that the timed steps start at a step n with n % multistep == 1 and
+    whole cycles of them are graph-replayed (md_run issues steps before a
+    cycle boundary one by one)."""
+    w = max(warmup, WARMUP_MIN)
+    return -(-w // multistep) * multistep
 
 
 def time_engine(prec, state, steps, warmup, seed):

@@ -496,7 +496,7 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   nb[11] = coulomb ? nbin : 0;
   nb[12] = coulomb && S.plane_fused ? S.K[2] : 0;
   nb[13] = cdiv(NN, mdk::kSRParticles);
-  nb[14] = S.real_list ? cdiv(Nq, mdk::kBlock) : 0;
+  nb[14] = S.real_list ? cdiv(Nq, mdk::kBlock / 64) : 0;  // a wavefront per charge
   nb[15] = S.real_list ? ncell2 : 0;
   S.mem1 = S.mem2 = S.mem3 = nullptr;
   S.pos3s = nullptr;
@@ -1130,6 +1130,8 @@ int md_engine::build_graph(int q) {
   if (e != hipSuccess) return hip_fail(e, "end capture");
   e = hipGraphInstantiate(&gexec[q], graph[q], nullptr, nullptr, 0);
   if (e != hipSuccess) return hip_fail(e, "instantiate");
+  e = hipGraphUpload(gexec[q], st);  // device-side set-up before the first replay
+  if (e != hipSuccess) return hip_fail(e, "graph upload");
   return MD_OK;
 }
 

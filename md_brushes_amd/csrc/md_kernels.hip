@@ -901,19 +901,20 @@ __global__ __launch_bounds__(kBlock) void k_bin_rl(const Rep *__restrict__ reps,
   const Rep &S = reps[r];
   Ctrl *c = S.ctrl;
   if (!c->flag2) return;
-  const int t = (blockIdx.x - wm.blk[r]) * kBlock + threadIdx.x;
-  if (t >= S.Nq) return;
-  const int i = S.charge[t];
-  int a, b, cz;
-  if (!cell_of(S.pos[i], S.nc2, S.rcel2, S.hLx, S.hLy, a, b, cz)) {
-    set_error(c, MD_ECELL, i, -1);
-    return;
+  const int nblk = wm.blk[r + 1] - wm.blk[r];
+  for (int t = (blockIdx.x - wm.blk[r]) * kBlock + threadIdx.x; t < S.Nq; t += nblk * kBlock) {
+    const int i = S.charge[t];
+    int a, b, cz;
+    if (!cell_of(S.pos[i], S.nc2, S.rcel2, S.hLx, S.hLy, a, b, cz)) {
+      set_error(c, MD_ECELL, i, -1);
+      continue;
+    }
+    const int cell = (cz * S.nc2[1] + b) * S.nc2[0] + a;
+    int *cnt = S.cnt2[c->gen2 & 1];
+    const int slot = atomicAdd(&cnt[cell], 1);
+    if (slot < S.ccap2) S.mem2[(size_t)cell * S.ccap2 + slot] = t;
+    else atomicOr(&c->ovf, 2);
   }
-  const int cell = (cz * S.nc2[1] + b) * S.nc2[0] + a;
-  int *cnt = S.cnt2[c->gen2 & 1];
-  const int slot = atomicAdd(&cnt[cell], 1);
-  if (slot < S.ccap2) S.mem2[(size_t)cell * S.ccap2 + slot] = t;
-  else atomicOr(&c->ovf, 2);
 }
 
 // Bitonic sort of one cell's members in LDS (one workgroup per cell).
@@ -951,44 +952,59 @@ __global__ __launch_bounds__(kBlock) void k_sort_rl(const Rep *__restrict__ reps
   sort_cell_lds(S.mem2 + (size_t)cell * S.ccap2, n, sbuf);
 }
 
+// Fill pass: a wavefront per charge slot t (the replica's wavefronts stride
+// over its charges).  The candidates are the members of the 27 stencil cells
+// in the reference's loop order, ascending within a cell (k_sort_rl); 64 are
+// tested at a time and the hits appended in that order with a ballot.
 __global__ __launch_bounds__(kBlock) void k_build_rl(const Rep *__restrict__ reps, WMap wm) {
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
   Ctrl *c = S.ctrl;
   if (!c->flag2) return;
   const int blk = blockIdx.x - wm.blk[r];
-  const int t = blk * kBlock + threadIdx.x;
+  const int nblk = wm.blk[r + 1] - wm.blk[r];
   const int ncell = S.nc2[0] * S.nc2[1] * S.nc2[2];
   const int buf = c->gen2 & 1;
-  {  // clear the other count buffer for the next build
-    const int nblk = wm.blk[r + 1] - wm.blk[r];
-    for (int k = blk * kBlock + threadIdx.x; k < ncell; k += nblk * kBlock) S.cnt2[buf ^ 1][k] = 0;
-  }
-  if (t >= S.Nq) return;
-  const int i = S.charge[t];
+  // clear the other count buffer for the next build
+  for (int k = blk * kBlock + threadIdx.x; k < ncell; k += nblk * kBlock) S.cnt2[buf ^ 1][k] = 0;
   const int *cnt = S.cnt2[buf];
-  const double4 pi = S.pos[i];
-  int a, b, cz;
-  if (!cell_of(pi, S.nc2, S.rcel2, S.hLx, S.hLy, a, b, cz)) return;
-  int n = 0;
-  for (int l = 0; l < 27; l++) {
-    const int cell = stencil_cell<false>(l, a, b, cz, S.nc2);
-    if (cell < 0) continue;
-    const int cn = min(cnt[cell], S.ccap2);
-    const int *cm = S.mem2 + (size_t)cell * S.ccap2;
-    for (int u = 0; u < cn; u++) {
-      const int j = S.charge[cm[u]];
-      if (j == i) continue;
-      double dx, dy, dz;
-      if (rsq_image(pi, S.pos[j], S.Lx, S.Ly, S.hLx, S.hLy, dx, dy, dz) < S.rv2sq) {
-        if (n < S.nbcap2) S.nb2[(size_t)n * S.Nq + t] = j;
-        n++;
+  const int lane = threadIdx.x & 63;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const int nwave = nblk * (kBlock / 64);
+  for (int t = blk * (kBlock / 64) + (threadIdx.x >> 6); t < S.Nq; t += nwave) {
+    const int i = S.charge[t];
+    const double4 pi = S.pos[i];
+    int a, b, cz;
+    if (!cell_of(pi, S.nc2, S.rcel2, S.hLx, S.hLy, a, b, cz)) continue;
+    int n = 0;
+    for (int l = 0; l < 27; l++) {
+      const int cell = stencil_cell<false>(l, a, b, cz, S.nc2);
+      if (cell < 0) continue;
+      const int cn = min(cnt[cell], S.ccap2);
+      const int *cm = S.mem2 + (size_t)cell * S.ccap2;
+      for (int u0 = 0; u0 < cn; u0 += 64) {
+        const int u = u0 + lane;
+        bool hit = false;
+        int j = 0;
+        if (u < cn) {
+          j = S.charge[cm[u]];
+          double dx, dy, dz;
+          hit = j != i && rsq_image(pi, S.pos[j], S.Lx, S.Ly, S.hLx, S.hLy, dx, dy, dz) < S.rv2sq;
+        }
+        const unsigned long long mask = __ballot(hit);
+        if (hit) {
+          const int slot = n + __popcll(mask & below);
+          if (slot < S.nbcap2) S.nb2[(size_t)slot * S.Nq + t] = j;
+        }
+        n += __popcll(mask);
       }
     }
+    if (lane == 0) {
+      S.nbn2[t] = n;
+      if (n > S.nbcap2) atomicOr(&c->ovf, 8);
+      if (n > c->max_nb2) atomicMax(&c->max_nb2, n);
+    }
   }
-  S.nbn2[t] = n;
-  if (n > S.nbcap2) atomicOr(&c->ovf, 8);
-  if (n > c->max_nb2) atomicMax(&c->max_nb2, n);
 }
 
 // ------------------------------------------------------------- SPME
@@ -1748,41 +1764,48 @@ __device__ __forceinline__ void real_pair<float>(float xi, float yi, float zi, f
 // real_space over the reference-form list (compt_acc.f90:1012-1038) for
 // replicas whose list cells number fewer than three along x or y
 // (Rep::real_list), where the reference lists a neighbour cell's pairs more
-// than once and sums each repeat: one thread per charge slot walks its list
-// in order and keeps the pairs within r_c.  The lb/Beta factor and the slab
-// term are applied by the Coulomb store.
+// than once and sums each repeat: a wavefront per charge slot (the replica's
+// wavefronts stride over its charges), its lanes taking every 64th entry of
+// the list and keeping the pairs within r_c, summed by a fixed DPP tree.
+// The lb/Beta factor and the slab term are applied by the Coulomb store.
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_real_list(const Rep *__restrict__ reps, WMap wm) {
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
-  const int t = (blockIdx.x - wm.blk[r]) * kBlock + threadIdx.x;
-  if (t >= S.Nq) return;
-  const int i = S.charge[t];
-  const int n = min(S.nbn2[t], S.nbcap2);
+  const int blk = blockIdx.x - wm.blk[r];
+  const int nwave = (wm.blk[r + 1] - wm.blk[r]) * (kBlock / 64);
+  const int lane = threadIdx.x & 63;
   const R Lx = (R)S.Lx, Ly = (R)S.Ly, hLx = (R)S.hLx, hLy = (R)S.hLy, rcc2 = (R)S.rcc2,
           alpha = (R)S.alpha, alpha2 = (R)S.alpha2, c2 = (R)S.c2;
-  R xi, yi, zi, qi;
-  if constexpr (sizeof(R) == 4) {
-    const float4 p = S.pos32[i];
-    xi = p.x; yi = p.y; zi = p.z; qi = p.w;
-  } else {
-    const double4 p = S.pos[i];
-    xi = p.x; yi = p.y; zi = p.z; qi = p.w;
-  }
-  R fx = 0, fy = 0, fz = 0;
-  for (int k = 0; k < n; k++) {
-    const int j = S.nb2[(size_t)k * S.Nq + t];
+  for (int t = blk * (kBlock / 64) + (threadIdx.x >> 6); t < S.Nq; t += nwave) {
+    const int i = S.charge[t];
+    const int n = min(S.nbn2[t], S.nbcap2);
+    R xi, yi, zi, qi;
     if constexpr (sizeof(R) == 4) {
-      const float4 p = S.pos32[j];
-      real_pair<R>(xi, yi, zi, qi, p.x, p.y, p.z, p.w, true, Lx, Ly, hLx, hLy, rcc2, alpha, alpha2, c2,
-                   fx, fy, fz);
+      const float4 p = S.pos32[i];
+      xi = p.x; yi = p.y; zi = p.z; qi = p.w;
     } else {
-      const double4 p = S.pos[j];
-      real_pair<R>(xi, yi, zi, qi, p.x, p.y, p.z, p.w, true, Lx, Ly, hLx, hLy, rcc2, alpha, alpha2, c2,
-                   fx, fy, fz);
+      const double4 p = S.pos[i];
+      xi = p.x; yi = p.y; zi = p.z; qi = p.w;
     }
+    R fx = 0, fy = 0, fz = 0;
+    for (int k = lane; k < n; k += 64) {
+      const int j = S.nb2[(size_t)k * S.Nq + t];
+      if constexpr (sizeof(R) == 4) {
+        const float4 p = S.pos32[j];
+        real_pair<R>(xi, yi, zi, qi, p.x, p.y, p.z, p.w, true, Lx, Ly, hLx, hLy, rcc2, alpha, alpha2,
+                     c2, fx, fy, fz);
+      } else {
+        const double4 p = S.pos[j];
+        real_pair<R>(xi, yi, zi, qi, p.x, p.y, p.z, p.w, true, Lx, Ly, hLx, hLy, rcc2, alpha, alpha2,
+                     c2, fx, fy, fz);
+      }
+    }
+    fx = wave_total(fx);
+    fy = wave_total(fy);
+    fz = wave_total(fz);
+    if (lane == 0) store_real<R>(S, t, fx, fy, fz);
   }
-  store_real<R>(S, t, fx, fy, fz);
 }
 
 // Charges into the lists of the spreading tiles that hold their spline

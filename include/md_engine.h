@@ -273,6 +273,22 @@ int md_batch_error_analysis(md_engine *e, int32_t slot, double *out,
 int md_heights(md_engine *e, double *out);
 int md_batch_heights(md_engine *e, int32_t slot, double *out);
 
+/* Energies and pair virials at the current state, FP64 sums (pair and bond
+ * arithmetic in the engine's precision).  The reference computes only the
+ * kinetic energy (height, in_output.f90:1307-1311); the potentials are those
+ * whose negative gradients are its forces: WCA 4((s/r)^12 - (s/r)^6) + 1 per
+ * pair within the class cut (compt_acc.f90:922-952), wall 0.4/3 ((s/d)^9 -
+ * (s/d)^3) (:955-975), FENE -k R0^2/2 ln(1 - r^2/R0^2) (:865-892), real
+ * space lB/Beta q_i q_j erfc(a r)/r (:997-1038), reciprocal lB/Beta sum Q U /
+ * (2 K1 K2 K3) (:1368-1398), slab 2 pi lB/Beta (sum q z)^2 / V (:1040-1048).
+ * out[16]: WCA, wall, FENE, real space, reciprocal, slab, Ewald self term,
+ * Coulomb (the four before), field -E_z sum q z, potential total, kinetic
+ * sum |v|^2/2, kinetic / (NN - N_anchor) (the reference's kenetic_energy),
+ * pair virials sum r.F of WCA, FENE and real space, 0.  The held Coulomb
+ * force and the trajectory are unchanged. */
+int md_energies(md_engine *e, double *out);
+int md_batch_energies(md_engine *e, int32_t slot, double *out);
+
 /* Exact resume (no reference counterpart: the reference restarts from its
  * six-decimal text checkpoint and rescales the velocities, main.f90:40-64).
  * md_state_save writes the engine's complete state between steps

@@ -104,6 +104,8 @@ def load_library(path=None):
         "md_autotune": ([c_void_p, _I, ip], _I),
         "md_error_analysis": ([c_void_p, dp, dp], _I),
         "md_heights": ([c_void_p, dp], _I),
+        "md_energies": ([c_void_p, dp], _I),
+        "md_batch_energies": ([c_void_p, _I, dp], _I),
         "md_state_size": ([c_void_p, POINTER(c_int64)], _I),
         "md_state_save": ([c_void_p, c_void_p, c_int64], _I),
         "md_state_load": ([c_void_p, c_void_p, c_int64], _I),
@@ -311,6 +313,18 @@ class Engine:
         self._check(self.lib.md_heights(self.h, _dp(out)), "md_heights")
         return dict(zip(self.HEIGHTS, out))
 
+    ENERGIES = ["lj", "wall", "fene", "real", "recip", "slab", "self", "coulomb", "field", "potential",
+                "kinetic", "kinetic_per_particle", "virial_lj", "virial_fene", "virial_real"]
+
+    def energies(self):
+        """Energies and pair virials at the current state (md_energies), FP64
+        sums: WCA, wall, FENE, real space, reciprocal, slab, Ewald self,
+        Coulomb, field, potential, kinetic, kinetic per mobile particle (the
+        reference's kenetic_energy, in_output.f90:1307-1311), pair virials."""
+        out = np.zeros(16)
+        self._check(self.lib.md_energies(self.h, _dp(out)), "md_energies")
+        return dict(zip(self.ENERGIES, out))
+
     def error_analysis(self):
         """error_analysis (compt_acc.f90:729-812) on the device: (summary,
         standard-Ewald reference Coulomb force (NN,3))."""
@@ -405,6 +419,12 @@ class Batch:
         self._check(self.lib.md_batch_error_analysis(self.h, slot, _dp(out), _dp(f)),
                     "md_batch_error_analysis")
         return _ewald_summary(out), f.reshape(3, -1).T.copy()
+
+    def energies(self, slot):
+        """Engine.energies of one replica."""
+        out = np.zeros(16)
+        self._check(self.lib.md_batch_energies(self.h, slot, _dp(out)), "md_batch_energies")
+        return dict(zip(Engine.ENERGIES, out))
 
     def observe(self, slot):
         """(brush height, z-density histogram of the polymer beads)."""

@@ -152,6 +152,8 @@ struct Sys {
   int *d_hist = nullptr;     // observables: z histogram of the chains
   double *d_hsum = nullptr;  // observables: per-block sums of z
   double *d_mol = nullptr;   // observables: per-molecule heights
+  double *d_epart = nullptr; // energies: per-block partial sums
+  int en_blocks[3] = {0, 0, 0};  // energy kernels' blocks: particles, charges, mesh
   std::string err;
 
   ~Sys() {
@@ -338,13 +340,17 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   d_hist = dalloc<int>(mdk::kHistBins);
   d_hsum = dalloc<double>(cdiv(NN, mdk::kBlock));  // one per particle block of k_observe
   d_mol = dalloc<double>(4 * (size_t)std::max(1, p.Nga + p.Ngl));  // per-molecule heights
+  en_blocks[0] = cdiv(NN, mdk::kBlock);
+  en_blocks[1] = coulomb ? cdiv(Nq, mdk::kBlock) : 0;
+  en_blocks[2] = coulomb ? (int)std::min<long>(256, cdiv((long)p.ng[0] * p.ng[1] * p.ng[2], mdk::kBlock)) : 0;
+  d_epart = dalloc<double>((size_t)en_blocks[0] * mdk::kEnShortTerms + 2 * (size_t)en_blocks[1] + en_blocks[2]);
   S.Nga = p.Nga;
   S.arm = p.arm;
   S.Nma = p.Nma;
   S.Ngl = p.Ngl;
   S.Nml = p.Nml;
   if (!cc || !S.flags || !S.charge || !S.bstart || !S.bpart || !S.ctrl || !d_eta || !d_out1 ||
-      !d_out2 || !d_hist || !d_hsum || !d_mol)
+      !d_out2 || !d_hist || !d_hsum || !d_mol || !d_epart)
     return fail(MD_ENOMEM, "topology arrays");
   hipError_t cerr = hipMemcpy(S.flags, flags.data(), nn * sizeof(int), hipMemcpyHostToDevice);
   if (cerr == hipSuccess && Nq)
@@ -1979,6 +1985,82 @@ static int heights(md_engine *e, Sys *s, const Plan &P, double *out) {
   out[6] = p.Nga > 0 ? hs_branch / p.Nga : 0;
   out[7] = p.Nga > 0 ? hs_avg / (p.Nga * (p.arm * p.Nma + 1)) : 0;
   return MD_OK;
+}
+
+// Energies and pair virials of one replica at its current state (the
+// Coulomb evaluation is rerun at these positions; the held Coulomb force and
+// the trajectory are unchanged).  out[16]: WCA, wall, FENE, real space,
+// reciprocal, slab, Ewald self term, Coulomb (real + reciprocal + slab +
+// self), field, potential total, kinetic sum |v|^2/2, the reference's
+// kenetic_energy (kinetic / (NN - N_anchor), in_output.f90:1307-1311), the
+// pair virials sum r.F of WCA, FENE and real space, 0.
+static int energies(md_engine *e, Sys *s, const Plan &P, double *out) {
+  if (!out) return e->fail(MD_EINVAL, "out is null");
+  int rc = e->check_sys(s);
+  if (rc) return rc;
+  if (s->coulomb) {
+    if (s->fp32) mdk::launch_coulomb<float>(P, e->st, s->d_out2);
+    else mdk::launch_coulomb<double>(P, e->st, s->d_out2);
+  }
+  const int *nb = s->en_blocks;
+  if (s->fp32) mdk::launch_energies<float>(P, e->st, s->d_epart, nb[0], nb[1], nb[2]);
+  else mdk::launch_energies<double>(P, e->st, s->d_epart, nb[0], nb[1], nb[2]);
+  rc = e->check_sys(s);
+  if (rc) return rc;
+  const size_t n = (size_t)nb[0] * mdk::kEnShortTerms + 2 * (size_t)nb[1] + nb[2];
+  std::vector<double> h(n);
+  hipError_t err = hipMemcpy(h.data(), s->d_epart, n * sizeof(double), hipMemcpyDeviceToHost);
+  if (err != hipSuccess) return e->hip_fail(err, "energies");
+  double t[mdk::kEnShortTerms] = {0}, re = 0, rw = 0, qu = 0;
+  for (int b = 0; b < nb[0]; b++)
+    for (int k = 0; k < mdk::kEnShortTerms; k++) t[k] += h[(size_t)b * mdk::kEnShortTerms + k];
+  const double *h2 = h.data() + (size_t)nb[0] * mdk::kEnShortTerms;
+  for (int b = 0; b < nb[1]; b++) {
+    re += h2[2 * b];
+    rw += h2[2 * b + 1];
+  }
+  for (int b = 0; b < nb[2]; b++) qu += h2[2 * (size_t)nb[1] + b];
+  const md_params &p = s->p;
+  const double c1 = p.lb / p.Beta;
+  std::memset(out, 0, 16 * sizeof(double));
+  out[0] = t[0];
+  out[1] = t[1];
+  out[2] = t[2];
+  if (s->coulomb) {
+    const double M = (double)p.ng[0] * p.ng[1] * p.ng[2];
+    out[3] = c1 * re;
+    out[4] = c1 * qu / (2 * M);
+    out[5] = s->h.slab / 2 * c1 * t[4] * t[4];
+    out[6] = -c1 * p.alpha / std::sqrt(kPi) * t[5];
+    out[14] = c1 * rw;
+  }
+  out[7] = out[3] + out[4] + out[5] + out[6];
+  out[8] = t[6];
+  out[9] = out[0] + out[1] + out[2] + out[7] + out[8];
+  out[10] = t[3];
+  out[11] = p.NN > p.N_anchor ? t[3] / (p.NN - p.N_anchor) : 0;
+  out[12] = t[7];
+  out[13] = t[8];
+  return MD_OK;
+}
+
+extern "C" int md_energies(md_engine *e, double *out) {
+  ENGINE_CHECK(e && out);
+  FIRST(e);
+  STATE_CHECK(e);
+  return energies(e, s, P, out);
+}
+
+extern "C" int md_batch_energies(md_engine *e, int32_t slot, double *out) {
+  ENGINE_CHECK(e && out);
+  auto it = e->slots.find(slot);
+  if (it == e->slots.end()) return e->fail(MD_EINVAL, "no such slot");
+  if (e->dirty) {
+    int rc = e->rebuild();
+    if (rc) return rc;
+  }
+  Sys *s = it->second;
+  return energies(e, s, e->single[e->index_of(s)], out);
 }
 
 extern "C" int md_heights(md_engine *e, double *out) {

@@ -83,6 +83,10 @@ void launch_set_flags(const Plan &P, hipStream_t st, int clear);
 // velocities before the rescale: redo it with the rescaled ones
 void launch_rescale(const Plan &P, hipStream_t st, int gated = 0, int redrift = 0);
 void launch_observe(const Plan &P, hipStream_t st, int *hist, double *hsum);
+// one replica (P.nrep == 1): energies into per-block partial sums
+constexpr int kEnShortTerms = 10;
+template <typename R>
+void launch_energies(const Plan &P, hipStream_t st, double *part, int nb_short, int nb_q, int nb_mesh);
 // one replica (P.nrep == 1): per-molecule heights, 4 doubles per molecule
 void launch_heights(const Plan &P, hipStream_t st, double *mol);
 

@@ -2443,6 +2443,183 @@ __global__ __launch_bounds__(kBlock) void k_heights(const Rep *__restrict__ reps
   }
 }
 
+// ------------------------------------------------------------- energies
+
+// Energies and pair virials of one replica (reps[0]) at its current state,
+// in FP64 sums, with the pair and bond arithmetic in the engine's precision
+// R (as its forces).  The reference computes only the kinetic energy
+// (in_output.f90:1307-1311); the potentials are those whose negative
+// gradients are its forces: WCA 4((s/r)^12 - (s/r)^6) + 1 within the class
+// cut (compt_acc.f90:922-952), the wall 0.4/3 ((s/d)^9 - (s/d)^3) where its
+// force acts (:955-975), FENE -k R0^2/2 ln(1 - r^2/R0^2) (:865-892), the
+// screened real-space pair q_i q_j erfc(a r)/r (:997-1038), the reciprocal
+// energy sum Q U / (2 K1 K2 K3) (:1368-1398) and the slab term.  Every
+// kernel writes per-block partial sums that the host adds in block order.
+constexpr int kEnShort = 10;  // LJ, wall, FENE, kinetic, sum qz, sum q^2, field, W_lj, W_fene, 0
+
+template <typename R>
+__device__ __forceinline__ void block_partials(double (&v)[kEnShort], int nt, double *out) {
+  __shared__ double red[kEnShort][kBlock / 64];
+  for (int t = 0; t < nt; t++) {
+    const double x = wave_sum(v[t]);
+    if ((threadIdx.x & 63) == 0) red[t][threadIdx.x >> 6] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < nt) {
+    double s = 0;
+    for (int w = 0; w < kBlock / 64; w++) s += red[threadIdx.x][w];
+    out[(size_t)blockIdx.x * nt + threadIdx.x] = s;
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_energy_short(const Rep *__restrict__ reps, double *part) {
+  const Rep &S = reps[0];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double v[kEnShort] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (i < S.NN) {
+    const double4 p = S.pos[i];
+    // WCA over the full list, half per entry
+    const R hLx = (R)S.hLx, hLy = (R)S.hLy, Lx = (R)S.Lx, Ly = (R)S.Ly;
+    R xi, yi, zi, qa;
+    if constexpr (sizeof(R) == 4) {
+      const float4 q = S.pos32[i];
+      xi = q.x; yi = q.y; zi = q.z; qa = fabsf(q.w);
+    } else {
+      xi = p.x; yi = p.y; zi = p.z; qa = fabs(p.w);
+    }
+    const int n = min(S.nbn1[i], S.nbcap1);
+    for (int k = 0; k < n; k++) {
+      const int j = S.nb1[(size_t)k * S.NN + i];
+      R xj, yj, zj, qj;
+      if constexpr (sizeof(R) == 4) {
+        const float4 q = S.pos32[j];
+        xj = q.x; yj = q.y; zj = q.z; qj = q.w;
+      } else {
+        const double4 q = S.pos[j];
+        xj = q.x; yj = q.y; zj = q.z; qj = q.w;
+      }
+      R dx = image(xi - xj, Lx, hLx), dy = image(yi - yj, Ly, hLy), dz = zi - zj;
+      const R rsq = dx * dx + dy * dy + dz * dz;
+      const int m = (int)(qa + fabs(qj));
+      R cut, s2;
+      if (m == 6) { cut = (R)S.rcl3; s2 = (R)S.sigma2; }
+      else if (m < 3) { cut = (R)S.rcl2; s2 = (R)1; }
+      else { cut = (R)S.rcl4; s2 = (R)S.sigma3; }
+      if (rsq < cut) {
+        const R ir2 = s2 / rsq, ir6 = ir2 * ir2 * ir2;
+        v[0] += 0.5 * (double)((R)4 * (ir6 * ir6 - ir6) + (R)1);
+        v[7] += 0.5 * (double)((R)48 * ir6 * (ir6 - (R)0.5));
+      }
+    }
+    // wall (FP64, as the force)
+    const double z = p.z, Lz = S.Lz;
+    if (!(fabs(z) < 0.01 || fabs(z - Lz) < 0.01)) {
+      const bool big = p.w == 3;
+      const double ss = big ? S.sigma_s : 1.0, rw = big ? S.wall_rs : S.wall_r;
+      double dl = -1;
+      if (z < rw) dl = z;
+      else if (Lz - z < rw) dl = Lz - z;
+      if (dl > 0) {
+        const double t = ss / dl, t3 = t * t * t;
+        v[1] = S.wall_c / 3 * (t3 * t3 * t3 - t3);
+      }
+    }
+    // FENE over the bonded partners, half per bond end
+    for (int b = S.bstart[i]; b < S.bstart[i + 1]; b++) {
+      double dxd, dyd, dzd;
+      const double rs = rsq_image(p, S.pos[S.bpart[b]], S.Lx, S.Ly, S.hLx, S.hLy, dxd, dyd, dzd);
+      const R R02 = (R)S.R0_2, kR = (R)S.kR02, r2 = (R)rs;
+      v[2] += 0.5 * (double)(-(R)0.5 * kR * log((R)1 - r2 / R02));
+      v[8] += 0.5 * (double)(-kR / (R02 - r2) * r2);
+    }
+    const double vx = S.vx[i], vy = S.vy[i], vz = S.vz[i];
+    v[3] = (vx * vx + vy * vy + vz * vz) / 2;
+    if (S.flags[i] & 2) {
+      v[4] = p.w * p.z;
+      v[5] = p.w * p.w;
+    }
+    v[6] = -S.EF * p.w * p.z;
+  }
+  block_partials<R>(v, kEnShort, part);
+}
+
+// Screened real-space pair energy without lB/Beta, added with its pair
+// virial r.F into (e, w), for r < r_c; FP32 with the erfc of the FP32 force.
+template <typename R>
+__device__ __forceinline__ void real_pair_energy(R dx, R dy, R dz, R qq, R rcc2, R alpha, R alpha2, R c2,
+                                                 double f, double &e, double &w) {
+  const R rsq = dx * dx + dy * dy + dz * dz;
+  if (!(rsq < rcc2)) return;
+  R ec;
+  const R rr = sqrt(rsq);
+  if constexpr (sizeof(R) == 4) {
+    const float t = 1.0f / (1.0f + 0.3275911f * alpha * rr);
+    const float poly =
+        t * (0.254829592f +
+             t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
+    ec = poly * expf(-alpha2 * rsq);
+  } else {
+    ec = erfc(alpha * rr);
+  }
+  e += f * (double)(qq * ec / rr);
+  w += f * (double)(qq * (ec / rr + c2 * exp(-alpha2 * rsq)));
+}
+
+// One thread per charge slot: the sweep cells of the last Coulomb evaluation
+// (every other charge within r_c once, half per pair), or the walked
+// reference-form list (half per entry, repeats included).
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_energy_real(const Rep *__restrict__ reps, double *part) {
+  using V = typename std::conditional<sizeof(R) == 4, float4, double4>::type;
+  const Rep &S = reps[0];
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  double v[kEnShort] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (t < S.Nq) {
+    const int i = S.charge[t];
+    const V *pv = (const V *)(sizeof(R) == 4 ? (const void *)S.pos32 : (const void *)S.pos);
+    const V pi = pv[i];
+    const R Lx = (R)S.Lx, Ly = (R)S.Ly, hLx = (R)S.hLx, hLy = (R)S.hLy, rcc2 = (R)S.rcc2,
+            alpha = (R)S.alpha, alpha2 = (R)S.alpha2, c2 = (R)S.c2;
+    auto pair = [&](const V pj) {
+      const R dx = image((R)pi.x - (R)pj.x, Lx, hLx), dy = image((R)pi.y - (R)pj.y, Ly, hLy),
+              dz = (R)pi.z - (R)pj.z;
+      real_pair_energy<R>(dx, dy, dz, (R)pi.w * (R)pj.w, rcc2, alpha, alpha2, c2, 0.5, v[0], v[1]);
+    };
+    if (S.real_list) {
+      const int n = min(S.nbn2[t], S.nbcap2);
+      for (int k = 0; k < n; k++) pair(pv[S.nb2[(size_t)k * S.Nq + t]]);
+    } else if (S.real_cells) {
+      int a, b, c;
+      if (cell_of(S.pos[i], S.nc3, S.rcel3, S.hLx, S.hLy, a, b, c)) {
+        for (int l = 0; l < 27; l++) {
+          const int cl = stencil_cell<true>(l, a, b, c, S.nc3);
+          if (cl < 0) continue;
+          const int cn = min(S.scnt3[cl], S.ccap3);
+          const int *mm = S.mem3 + (size_t)cl * S.ccap3;
+          const V *ps = (const V *)S.pos3s + (size_t)cl * S.ccap3;
+          for (int k = 0; k < cn; k++)
+            if (mm[k] != t) pair(ps[k]);
+        }
+      }
+    }
+  }
+  block_partials<R>(v, 2, part);
+}
+
+// sum Q U over the mesh of the last Coulomb evaluation (charge mesh and the
+// potential mesh of its inverse FFT).
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_energy_mesh(const Rep *__restrict__ reps, double *part) {
+  const Rep &S = reps[0];
+  const size_t M = (size_t)S.K[0] * S.K[1] * S.K[2];
+  const R *q = (const R *)S.meshr, *u = (const R *)S.meshu;
+  double v[kEnShort] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (size_t g = (size_t)blockIdx.x * kBlock + threadIdx.x; g < M; g += (size_t)gridDim.x * kBlock)
+    v[0] += (double)q[g] * (double)u[g];
+  block_partials<R>(v, 1, part);
+}
+
 // --------------------------------------------------------- launchers
 
 // One replica (P.nrep == 1); Npe is read from the descriptor.
@@ -2600,6 +2777,21 @@ void launch_rescale(const Plan &P, hipStream_t st, int gated, int redrift) {
   hipLaunchKernelGGL(k_vscale, dim3(P.part.total), dim3(kBlock), 0, st, P.reps, P.part, P.vpart,
                      gated, redrift);
 }
+
+// Energies of one replica (P.nrep == 1) into per-block partials: nb_short
+// blocks of kEnShort sums, then nb_q blocks of 2 (real space), then nb_mesh
+// blocks of 1 (mesh); the Coulomb evaluation must have run at the current
+// positions (cells sorted, meshes filled).
+template <typename R>
+void launch_energies(const Plan &P, hipStream_t st, double *part, int nb_short, int nb_q, int nb_mesh) {
+  hipLaunchKernelGGL(k_energy_short<R>, dim3(nb_short), dim3(kBlock), 0, st, P.reps, part);
+  double *p2 = part + (size_t)nb_short * kEnShort;
+  if (nb_q) hipLaunchKernelGGL(k_energy_real<R>, dim3(nb_q), dim3(kBlock), 0, st, P.reps, p2);
+  if (nb_mesh)
+    hipLaunchKernelGGL(k_energy_mesh<R>, dim3(nb_mesh), dim3(kBlock), 0, st, P.reps, p2 + (size_t)nb_q * 2);
+}
+template void launch_energies<float>(const Plan &, hipStream_t, double *, int, int, int);
+template void launch_energies<double>(const Plan &, hipStream_t, double *, int, int, int);
 
 template void launch_coulomb<float>(const Plan &, hipStream_t, double *, hipEvent_t *);
 template void launch_coulomb<double>(const Plan &, hipStream_t, double *, hipEvent_t *);

@@ -967,6 +967,139 @@ void orc_error_analysis(orc_sys *s, double out[8], double *acc_se, double *acc_c
   free(fc);
 }
 
+/* ------------------------------------------------------------ energies */
+
+/* The reference computes only the kinetic energy (height,
+ * in_output.f90:1307-1311); the potentials below are the ones whose negative
+ * gradients are the reference's forces, with the same lists, cuts, classes
+ * and constants as orc_lj_force, orc_fene_force, orc_real_space and
+ * orc_spme:
+ *   out[0]  WCA: 4 ((s/r)^12 - (s/r)^6) + 1 per pair within the class cut
+ *           (compt_acc.f90:922-952; the +1 puts the energy at 0 at
+ *           r = 2^(1/6) s), half per entry of the full list
+ *   out[1]  wall: 0.4/3 ((s/d)^9 - (s/d)^3) on the wall the force acts from
+ *           (compt_acc.f90:955-975)
+ *   out[2]  FENE: -k R0^2 / 2 ln(1 - r^2/R0^2) per bond (compt_acc.f90:865-892)
+ *   out[3]  real space: lB/Beta q_i q_j erfc(a r)/r per pair within r_c
+ *           (compt_acc.f90:997-1038; a full-list entry counts half)
+ *   out[4]  reciprocal: lB/Beta / (2 K1 K2 K3) sum_mesh Q U, with U the
+ *           unnormalised inverse FFT of BC times the FFT of Q
+ *           (compt_acc.f90:1368-1398, whose gradient is IFrc :1402-1467)
+ *   out[5]  slab: 2 pi lB/Beta (sum q z)^2 / (Lx Ly Lz Zfac)
+ *           (the potential of the correction at compt_acc.f90:1040-1048)
+ *   out[6]  Ewald self term: -lB/Beta a/sqrt(pi) sum q^2
+ *   out[7]  out[3] + out[4] + out[5] + out[6]
+ *   out[8]  field: -E_z sum q z (compt_acc.f90:855)
+ *   out[9]  potential total: out[0] + out[1] + out[2] + out[7] + out[8]
+ *   out[10] kinetic: sum |v|^2 / 2
+ *   out[11] the reference's kenetic_energy: out[10] / (NN - N_anchor)
+ *   out[12] LJ pair virial sum r.F, out[13] FENE, out[14] real space
+ *   out[15] 0
+ * The reference's ZeroForce (mean reciprocal force removed) has no energy
+ * counterpart. */
+void orc_energies(orc_sys *s, double *out) {
+  md_params *p = &s->p;
+  int NN = s->NN;
+  double Lx = p->Lx, Ly = p->Ly, Lz = p->Lz;
+  double hLx = Lx / 2, nhLx = -Lx / 2, hLy = Ly / 2, nhLy = -Ly / 2;
+  double rcl2 = p->rcl * p->rcl, rcl3 = p->rcl_s * p->rcl_s, rcl4 = p->rcl_sp * p->rcl_sp;
+  double sigma2 = p->sigma_s * p->sigma_s, sigma3 = p->sigma_sp * p->sigma_sp;
+  memset(out, 0, sizeof(double) * 16);
+  for (int64_t k = 0; k < s->npair1; k++) {
+    int i = s->l1i[k], j = s->l1j[k];
+    double rij[3] = {X(i) - X(j), Y(i) - Y(j), Z(i) - Z(j)};
+    int m = (int)(fabs(QC(i)) + fabs(QC(j)));
+    if (rij[0] > hLx) rij[0] = rij[0] - Lx;
+    else if (rij[0] <= nhLx) rij[0] = rij[0] + Lx;
+    if (rij[1] > hLy) rij[1] = rij[1] - Ly;
+    else if (rij[1] <= nhLy) rij[1] = rij[1] + Ly;
+    double rsqr = rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2];
+    double inv_r2 = 0;
+    int hit = 0;
+    if (m == 6) {
+      if (rsqr < rcl3) { inv_r2 = sigma2 / rsqr; hit = 1; }
+    } else if (m < 3) {
+      if (rsqr < rcl2) { inv_r2 = 1 / rsqr; hit = 1; }
+    } else {
+      if (rsqr < rcl4) { inv_r2 = sigma3 / rsqr; hit = 1; }
+    }
+    if (hit) {
+      double inv_r6 = inv_r2 * inv_r2 * inv_r2;
+      out[0] += 0.5 * (4 * (inv_r6 * inv_r6 - inv_r6) + 1);
+      out[12] += 0.5 * 48 * inv_r6 * (inv_r6 - 0.5);
+    }
+  }
+  const double c04 = (double)0.4f;
+  const double c5 = (double)(float)pow(5.0, (double)(1.0f / 6.0f));
+  for (int i = 0; i < NN; i++) {
+    double z = Z(i);
+    if (fabs(z) < 0.01 || fabs(z - Lz) < 0.01) continue;
+    double ss = QC(i) != 3 ? 1.0 : p->sigma_s;
+    double rw = (QC(i) != 3 ? p->rcl : p->rcl_s) / c5, dl = -1;
+    if (z < rw) dl = z;
+    else if ((Lz - z) < rw) dl = Lz - z;
+    if (dl > 0) out[1] += c04 / 3 * (pow(ss / dl, 9) - pow(ss / dl, 3));
+  }
+  for (int b = 0; b < s->N_bond; b++) {
+    int j = s->fene[b], k = s->fene[s->N_bond + b];
+    double r[3];
+    double rsqr = rij_rr(s, j, k, r);
+    out[2] += -0.5 * p->kFENE * p->R0_2 * log(1 - rsqr / p->R0_2);
+    out[13] += -p->kFENE * p->R0_2 / (p->R0_2 - rsqr) * rsqr;
+  }
+  if (p->qq != 0 && s->Nq > 0) {
+    double c1 = p->lb / p->Beta, c2 = 2 * p->alpha / sqrt(PI);
+    double alpha2 = p->alpha * p->alpha, rcc2 = p->rcc * p->rcc;
+    if (p->real_verlet == 0) {
+      for (int m = 0; m < s->Nq - 1; m++)
+        for (int n = m + 1; n < s->Nq; n++) {
+          int i = s->charge[m], j = s->charge[n];
+          double rij[3];
+          double rsqr = rij_rr(s, i, j, rij);
+          if (rsqr < rcc2) {
+            double rr = sqrt(rsqr);
+            out[3] += c1 * QC(i) * QC(j) * erfc(p->alpha * rr) / rr;
+            out[14] += c1 * QC(i) * QC(j) * (erfc(p->alpha * rr) / rr + c2 * exp(-alpha2 * rsqr));
+          }
+        }
+    } else {
+      for (int64_t k = 0; k < s->npair2; k++) {
+        int i = s->l2i[k], j = s->l2j[k];
+        double rij[3];
+        double rsqr = rij_rr(s, i, j, rij);
+        if (rsqr < rcc2) {
+          double rr = sqrt(rsqr);
+          out[3] += 0.5 * c1 * QC(i) * QC(j) * erfc(p->alpha * rr) / rr;
+          out[14] += 0.5 * c1 * QC(i) * QC(j) * (erfc(p->alpha * rr) / rr + c2 * exp(-alpha2 * rsqr));
+        }
+      }
+    }
+    double *tmp = calloc((size_t)NN * 3, sizeof(double));
+    orc_spme(s, tmp);
+    free(tmp);
+    size_t M = (size_t)p->ng[0] * p->ng[1] * p->ng[2];
+    double qu = 0;
+    for (size_t g = 0; g < M; g++) qu += creal(s->Q[g]) * creal(s->U[g]);
+    out[4] = c1 * qu / (2.0 * (double)M);
+    double qz = 0, q2 = 0;
+    for (int m = 0; m < s->Nq; m++) {
+      int i = s->charge[m];
+      qz += QC(i) * Z(i);
+      q2 += QC(i) * QC(i);
+    }
+    out[5] = 2 * PI * c1 * qz * qz / (Lx * Ly * Lz * p->Zfac);
+    out[6] = -c1 * p->alpha / sqrt(PI) * q2;
+  }
+  out[7] = out[3] + out[4] + out[5] + out[6];
+  for (int i = 0; i < NN; i++) out[8] += -p->EF * QC(i) * Z(i);
+  out[9] = out[0] + out[1] + out[2] + out[7] + out[8];
+  if (s->vel) {
+    for (int i = 0; i < NN; i++)
+      for (int d = 0; d < 3; d++) out[10] += s->vel[(size_t)d * NN + i] * s->vel[(size_t)d * NN + i] / 2.;
+  }
+  out[11] = NN > s->N_anchor ? out[10] / (NN - s->N_anchor) : 0;
+}
+
 /* ------------------------------------------------------------- noise */
 
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2],

@@ -48,6 +48,7 @@ def lib():
         "orc_update_verlet_list": ([c_void_p], c_int32),
         "orc_rescale_velocity": ([c_void_p], None),
         "orc_run": ([c_void_p, c_int32, c_int32, c_int32], c_int32),
+        "orc_energies": ([c_void_p, dp], None),
         "orc_fft3d": ([c_int32, c_int32, c_int32, dp, c_int32], None),
         "orc_bspln_coeffs": ([c_int32, dp, dp], None),
         "orc_bspln": ([c_double, c_int32], c_double),
@@ -170,6 +171,15 @@ class Oracle:
 
     def run(self, step0, n, preheat=False):
         return lib().orc_run(self.h, step0, n, int(preheat))
+
+    ENERGIES = ["lj", "wall", "fene", "real", "recip", "slab", "self", "coulomb", "field", "potential",
+                "kinetic", "kinetic_per_particle", "virial_lj", "virial_fene", "virial_real"]
+
+    def energies(self):
+        """Energies and pair virials at the current state (dict)."""
+        out = np.zeros(16)
+        lib().orc_energies(self.h, dp(out))
+        return dict(zip(self.ENERGIES, out))
 
 
 def philox(ctr, key):

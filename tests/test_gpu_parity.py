@@ -568,3 +568,57 @@ def test_small_box_trajectory():
     pg = e.download()[0]
     assert np.abs(pg - o.state()[0]).max() < 1e-8
     e.close()
+
+
+ENERGY_TERMS = ["lj", "wall", "fene", "real", "recip", "slab", "self", "field", "kinetic", "virial_lj",
+                "virial_fene", "virial_real"]
+
+
+@pytest.mark.parametrize("name", ["small", "default", "small_box"])
+def test_energies_match_oracle(name):
+    """md_energies (FP64 sums on the device) equals the oracle's energies
+    term by term: FP64 engine within 1e-9 relative, FP32 engine within 1e-4
+    (the error printed per term)."""
+    print(f"[energies] {name}", flush=True)
+    if name == "small_box":
+        p, pos, vel = _small_box()
+    else:
+        p, pos, vel = system(name, MD_FP64)
+    o = orc.Oracle(p)
+    o.set_state(pos, vel)
+    o.build_lists()
+    ref = o.energies()
+    for prec, tol in ((MD_FP64, 1e-9), (MD_FP32, 1e-4)):
+        q = p.copy()
+        q.precision = prec
+        e = Engine(q)
+        e.upload(pos, vel)
+        e.build_lists()
+        got = e.energies()
+        e.close()
+        errs = {}
+        for k in ENERGY_TERMS:
+            scale = max(abs(ref[k]), 1e-6 * abs(ref["potential"]), 1e-12)
+            errs[k] = abs(got[k] - ref[k]) / scale
+        print(("fp64" if prec == MD_FP64 else "fp32"), {k: f"{v:.1e}" for k, v in errs.items()})
+        for k, v in errs.items():
+            assert v < tol, (k, got[k], ref[k])
+        assert abs(got["coulomb"] - (got["real"] + got["recip"] + got["slab"] + got["self"])) < 1e-9 * abs(
+            got["coulomb"]) + 1e-12
+
+
+def test_energies_leave_trajectory():
+    """An energy evaluation between runs leaves the trajectory bit for bit."""
+    print("[energies] trajectory unchanged", flush=True)
+    out = []
+    for probe in (False, True):
+        p, pos, vel, e = _engine("default", MD_FP32)
+        e.compute_force(0, MD_NOISE_PHILOX)
+        e.run(1, 20)
+        if probe:
+            e.energies()
+        e.run(21, 20)
+        out.append(e.download())
+        e.close()
+    for x, y in zip(out[0], out[1]):
+        assert np.array_equal(x, y)

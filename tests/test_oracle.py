@@ -553,3 +553,112 @@ def test_rescale_sets_temperature():
     moving = np.setdiff1d(np.arange(p.NN), anchors)
     v2 = np.sum(v[moving] ** 2) / (p.NN - p.N_anchor - 1)
     assert abs(v2 - 3 / p.Beta) < 1e-10
+
+
+# -------------------------------------------------------------- energies
+
+def _energy_box():
+    """_charged_box with ions in the walls' range (as the gradient test of
+    the short-range forces)."""
+    p = _charged_box()
+    pos = initialize_position(p, 9)
+    pos[p.NN - 1, 2] = 1.5
+    pos[p.Npe, 2] = 0.8
+    pos[p.Npe + 1, 2] = p.Lz - 0.8
+    return p, pos
+
+
+def test_energy_gradients_are_forces():
+    """Each energy's negative gradient (central differences, lists held) is
+    the oracle's force of the same term: WCA + wall -> lj_force, FENE ->
+    fene_force, real space + reciprocal + slab -> the Coulomb force (whose
+    reciprocal part has its mean removed, ZeroForce: below 1e-3 of the
+    force here), field -> q E_z."""
+    p, pos = _energy_box()
+    p = p.copy()
+    p.EF = 0.7
+    o = orc.Oracle(p)
+    o.set_state(pos)
+    o.build_lists()
+    f_lj = o.lj_force()
+    f_fene = o.fene_force()[0]
+    f_c = o.coulomb()
+    h = 1e-6
+    rng = np.random.default_rng(2)
+    picks = list(rng.choice(p.NN, 10, replace=False)) + [p.NN - 1, p.Npe, p.Npe + 1, 5]
+    scale_c = np.sqrt(np.mean(f_c ** 2))
+    for i in picks:
+        for dim in range(3):
+            pp, pm = pos.copy(), pos.copy()
+            pp[i, dim] += h
+            pm[i, dim] -= h
+            o.set_state(pp)
+            ep = o.energies()
+            o.set_state(pm)
+            em = o.energies()
+            g = {k: -(ep[k] - em[k]) / (2 * h) for k in ep}
+            assert abs(g["lj"] + g["wall"] - f_lj[i, dim]) < 1e-5 * max(1.0, abs(f_lj[i, dim])), (i, dim)
+            assert abs(g["fene"] - f_fene[i, dim]) < 1e-5 * max(1.0, abs(f_fene[i, dim])), (i, dim)
+            gc = g["real"] + g["recip"] + g["slab"]
+            assert abs(gc - f_c[i, dim]) < 1e-3 * scale_c, (i, dim, gc, f_c[i, dim])
+            assert g["self"] == 0
+            fe = p.EF * pos[i, 3] if dim == 2 else 0.0
+            assert abs(g["field"] - fe) < 1e-6
+    o.set_state(pos)
+
+
+def test_coulomb_energy_matches_2d_ewald():
+    """real space + reciprocal + slab + self of the production set-up equals
+    the 2D-periodic Ewald energy (tests/ewald2d.py); without the slab term,
+    or with its sign flipped, it does not."""
+    import ewald2d
+    p = _slab_system(3.0, tol=4.0)
+    pos, ch, q = _slab_charges(p)
+    o = orc.Oracle(p)
+    o.set_state(pos)
+    e = o.energies()
+    ref = ewald2d.energy(pos[ch, :3], q, p.Lx, p.Ly, p.alpha, 20, p.lb / p.Beta)
+    assert abs(e["coulomb"] - ref) < 2e-6 * abs(ref), (e["coulomb"], ref)
+    assert abs(e["coulomb"] - e["slab"] - ref) > 1e-3 * abs(ref)
+    assert abs(e["coulomb"] - 2 * e["slab"] - ref) > 1e-3 * abs(ref)
+
+
+def test_pair_virials_from_scaling():
+    """The pair virials sum r.F over pairs: minus the derivative of the pair
+    energies when every position and the box scale by lambda (alpha and r_c
+    held)."""
+    p0, pos = _energy_box()
+    p0 = p0.copy()
+    p0.ovr_rcc = 6.0
+
+    def energies(lam):
+        p = p0.copy()
+        p.sigmag = p0.sigmag / lam ** 2
+        p.Lz = p0.Lz * lam
+        p = orc.derive(p)
+        o = orc.Oracle(p)
+        x = pos.copy()
+        x[:, :3] *= lam
+        o.set_state(x)
+        o.build_lists()
+        return o.energies()
+
+    e0 = energies(1.0)
+    d = 1e-6
+    ep, em = energies(1 + d), energies(1 - d)
+    for term, vir in (("lj", "virial_lj"), ("fene", "virial_fene"), ("real", "virial_real")):
+        dedl = (ep[term] - em[term]) / (2 * d)
+        assert abs(-dedl - e0[vir]) < 1e-5 * max(1.0, abs(e0[vir])), (term, -dedl, e0[vir])
+
+
+def test_kinetic_energy_definition():
+    """kenetic_energy of height (in_output.f90:1307-1311): sum |v|^2/2 over
+    NN - N_anchor."""
+    p = config("small")
+    pos = initialize_position(p, 3)
+    vel = initialize_velocity(p, 4)
+    o = orc.Oracle(p)
+    o.set_state(pos, vel)
+    e = o.energies()
+    assert abs(e["kinetic"] - 0.5 * np.sum(vel ** 2)) < 1e-10 * e["kinetic"]
+    assert abs(e["kinetic_per_particle"] - e["kinetic"] / (p.NN - p.N_anchor)) < 1e-15

@@ -25,6 +25,7 @@ constexpr int kGatherLanes = 4;   // lanes per charge of the reciprocal-force ga
 constexpr int kRealRound = 256;   // real-space candidates tested per compaction round (4 per lane)
 constexpr int kLJStage = 256;     // LJ fill pass: candidates staged in order per target
 constexpr int kHistBins = 500;  // SizeHist of the z-density profile (glob_var.f90:59)
+constexpr int kMaxFac = 24;     // FFT passes of a mesh length that is not a power of two
 
 // Per-replica control block, written by kernels.
 struct Ctrl {
@@ -68,11 +69,15 @@ struct Rep {
   double kR02, R0_2;               // kFENE*R0_2, R0_2
   // Coulomb (compt_acc.f90:980-1049, 1213-1246)
   double c1, alpha, alpha2, c2, rcc2, slab;  // lb/Beta, 2a/sqrt(pi), 4pi/V
-  int K[3], order;
+  int K[3], order;                 // mesh; the spline order when equal on all axes, else 0
+  int ord[3];                      // spline order per axis, ordr(1:3) (compt_acc.f90:1232-1234)
   double kg[3];                    // K_d / gdim_d
   double fnorm[3];                 // K_d/gdim_d/(K1 K2 K3)
   int Kh;                          // K1/2+1
-  int lK[3];                       // log2 K_d
+  int lK[3];                       // log2 K_d (power-of-two axes)
+  int pw2[3];                      // 1: K_d is a power of two (radix 8/4/2 passes)
+  int nfac[3];                     // other lengths: one Stockham pass per factor
+  int fac[3][kMaxFac];             // (radix 2, 3, 4, 5, 8 butterflies; any other prime by a DFT)
   int px_tile, py_tile, pz_tile;   // FFT tile sizes (row pairs, lines, lines)
   // Verlet lists (compt_acc.f90:185-314, 491-626)
   double rv1sq, rv2sq;
@@ -119,11 +124,10 @@ struct Rep {
   int *nbn2, *nb2;                 // real list: per charge slot m
   void *meshr;                     // charge mesh Q (real type of the path) K3*K2*K1
   // spreading tiles: each tile lists the charges with spline points in it
-  int tsz[3], ntl[3];              // tile edge (mesh points) and tiles per axis
+  int tsz[3], trem[3], ntl[3];     // tiles per axis: the first trem of edge tsz + 1, then tsz
   int capt;                        // capacity of each of a tile's kTileSub sub-lists
   int *tcnt, *tmem;                // counts; particle indices (tile, sub, slot)
-  int plane_fused;                 // x and y FFT passes fused per z-plane; the
-                                   // spectrum is then kept in bit-reversed y order
+  int plane_fused;                 // x and y FFT passes fused per z-plane
   void *spec;                      // half spectrum K3*K2*Kh complex
   void *meshu;                     // real potential mesh K3*K2*K1
   void *bc;                        // influence function K3*K2*Kh real
@@ -133,6 +137,19 @@ struct Rep {
   int nqblk;                       // number of charge blocks
   Ctrl *ctrl;
 };
+
+// Spreading tiles along axis d: ntl tiles, the first trem of edge tsz + 1
+// and the rest of edge tsz (every edge >= the spline order, at most kTile).
+__host__ __device__ inline int tile_start(const Rep &S, int d, int t) {
+  return t * S.tsz[d] + (t < S.trem[d] ? t : S.trem[d]);
+}
+__host__ __device__ inline int tile_edge(const Rep &S, int d, int t) {
+  return S.tsz[d] + (t < S.trem[d] ? 1 : 0);
+}
+__host__ __device__ inline int tile_of(const Rep &S, int d, int b) {
+  const int big = S.trem[d] * (S.tsz[d] + 1);
+  return b < big ? b / (S.tsz[d] + 1) : S.trem[d] + (b - big) / S.tsz[d];
+}
 
 // Work map: block ranges per replica.
 struct WMap {

@@ -205,11 +205,10 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   if (NN <= 0 || p.Lx <= 0 || p.dt <= 0 || p.multistep <= 0) return fail(MD_EINVAL, "bad sizes");
   if (coulomb) {
     for (int d = 0; d < 3; d++) {
-      if (!is_pow2(p.ng[d]))
-        return fail(MD_EINVAL, "SPME mesh sizes must be powers of two >= 4 (set ovr_ng)");
-      if (p.ordr[d] != p.ordr[0] || p.ordr[d] < 3 || p.ordr[d] > mdk::kMaxOrder)
-        return fail(MD_EINVAL, "SPME order must be equal on all axes and in 3..8");
-      if (p.ng[d] < p.ordr[d]) return fail(MD_EINVAL, "SPME mesh smaller than the spline order");
+      if (p.ordr[d] < 3 || p.ordr[d] > mdk::kMaxOrder)
+        return fail(MD_EINVAL, "SPME order must be in 3..8 on every axis");
+      if (p.ng[d] < p.ordr[d] || p.ng[d] < 2) return fail(MD_EINVAL, "SPME mesh smaller than the spline order");
+      if (p.ng[d] > (1 << 14)) return fail(MD_EINVAL, "SPME mesh larger than 16384 along an axis");
     }
   }
   // topology (compt_acc.f90:317-372, 433-488)
@@ -379,11 +378,27 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
     const int K1 = p.ng[0], K2 = p.ng[1], K3 = p.ng[2];
     for (int d = 0; d < 3; d++) {
       S.K[d] = p.ng[d];
-      S.lK[d] = ilog2(p.ng[d]);
+      S.pw2[d] = is_pow2(p.ng[d]);
+      S.lK[d] = S.pw2[d] ? ilog2(p.ng[d]) : 0;
+      // other lengths: passes of radix 8, 4, 2 for the power-of-two part,
+      // then 5, 3 and the remaining primes
+      S.nfac[d] = 0;
+      if (!S.pw2[d]) {
+        int n = p.ng[d];
+        auto push = [&](int f) { S.fac[d][S.nfac[d]++] = f; };
+        while (n % 8 == 0) { push(8); n /= 8; }
+        while (n % 4 == 0) { push(4); n /= 4; }
+        while (n % 2 == 0) { push(2); n /= 2; }
+        while (n % 5 == 0) { push(5); n /= 5; }
+        while (n % 3 == 0) { push(3); n /= 3; }
+        for (int f = 7; n > 1; f += 2)
+          while (n % f == 0) { push(f); n /= f; }
+      }
       S.kg[d] = p.ng[d] / p.gdim[d];
       S.fnorm[d] = p.ng[d] / p.gdim[d] / ((double)K1 * K2 * K3);
+      S.ord[d] = p.ordr[d];
     }
-    S.order = p.ordr[0];
+    S.order = (p.ordr[1] == p.ordr[0] && p.ordr[2] == p.ordr[0]) ? p.ordr[0] : 0;
     S.Kh = K1 / 2 + 1;
 
     const size_t M = (size_t)K1 * K2 * K3, Mh = (size_t)K3 * K2 * S.Kh;
@@ -391,12 +406,14 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
     S.spec = dalloc<char>(Mh * csz);
     S.meshu = dalloc<char>(M * rs);
     if (!S.meshr || !S.spec || !S.meshu) return fail(MD_ENOMEM, "mesh");
-    // spreading tiles: edge >= order, so a charge's points fall in at most
-    // two tiles along each axis
+    // spreading tiles: ceil(K / kTile) per axis of near-equal edges (K / ntl
+    // or one more), every edge >= order (>= 8 from K >= 16 up, K itself
+    // below), so a charge's points fall in at most two tiles along each axis
     nbin = 1;
     for (int d = 0; d < 3; d++) {
-      S.tsz[d] = std::min(mdk::kTile, p.ng[d]);
-      S.ntl[d] = p.ng[d] / S.tsz[d];
+      S.ntl[d] = cdiv(p.ng[d], mdk::kTile);
+      S.tsz[d] = p.ng[d] / S.ntl[d];
+      S.trem[d] = p.ng[d] % S.ntl[d];
       nbin *= S.ntl[d];
     }
     S.tcnt = dalloc<int>((size_t)nbin * mdk::kTileSub);
@@ -414,15 +431,14 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
     S.plane_fused = can_fuse;
     if (!S.plane_fused) lds_plane = 0;
     // influence function of PME_BC (compt_acc.f90:1557-1598), half spectrum
-    const int o = S.order;
     char key[256];
-    std::snprintf(key, sizeof key, "bc %d %d %d %d %a %a %a %a %d %d", K1, K2, K3, o, p.alpha,
-                  p.gdim[0], p.gdim[1], p.gdim[2], (int)fp32, 0);
+    std::snprintf(key, sizeof key, "bc %d %d %d %d %d %d %a %a %a %a %d %d", K1, K2, K3, p.ordr[0],
+                  p.ordr[1], p.ordr[2], p.alpha, p.gdim[0], p.gdim[1], p.gdim[2], (int)fp32, 0);
     auto bc = shared_table(key, [&]() {
       std::vector<double> b2[3], mv[3], ev[3];
       for (int d = 0; d < 3; d++) {
         const int K = p.ng[d];
-        b2[d] = bmod2(K, o);
+        b2[d] = bmod2(K, p.ordr[d]);
         mv[d].resize(K);
         ev[d].resize(K);
         for (int k = 0; k < K; k++) {
@@ -474,7 +490,7 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
     }
     // FFT tiles: about 32 KB of LDS per workgroup
     const int budget = 16384;  // bytes per LDS buffer (the FFT passes use two)
-    S.px_tile = std::max(1, std::min(K2 * K3 / 2, budget / (K1 * csz)));
+    S.px_tile = std::max(1, std::min((K2 * K3 + 1) / 2, budget / (K1 * csz)));
     S.py_tile = std::max(1, std::min(S.Kh, budget / (K2 * csz)));
     // z pass lines per workgroup: set at rebuild with the engine's size
     S.pz_tile = std::max(1, std::min(4, budget / (K3 * csz)));
@@ -491,7 +507,7 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   nb[0] = cdiv(NN, mdk::kBlock);
   nb[1] = coulomb ? cdiv(Nq, mdk::kBlock) : 0;
   nb[2] = coulomb ? S.nqblk : 0;
-  nb[3] = coulomb && !S.plane_fused ? cdiv((long)S.K[1] * S.K[2] / 2, S.px_tile) : 0;
+  nb[3] = coulomb && !S.plane_fused ? cdiv(((long)S.K[1] * S.K[2] + 1) / 2, S.px_tile) : 0;
   nb[4] = coulomb && !S.plane_fused ? S.K[2] * cdiv(S.Kh, S.py_tile) : 0;
   nb[5] = coulomb ? cdiv((long)S.K[1] * S.Kh, S.pz_tile) : 0;
   // cell kernels stride over the cells with a grid of modest size
@@ -630,10 +646,10 @@ int Sys::size_lists(const double *pos) {
       const double c3[3] = {X[i], Y[i], Z[i]};
       int t0[3], nt[3];
       for (int d = 0; d < 3; d++) {
-        int b = ((int)std::ceil(h.kg[d] * c3[d]) - h.order) % h.K[d];
+        int b = ((int)std::ceil(h.kg[d] * c3[d]) - h.ord[d]) % h.K[d];
         b += b < 0 ? h.K[d] : 0;
-        t0[d] = b / h.tsz[d];
-        nt[d] = ((b + h.order - 1) % h.K[d]) / h.tsz[d] != t0[d] ? 2 : 1;
+        t0[d] = mdk::tile_of(h, d, b);
+        nt[d] = mdk::tile_of(h, d, (b + h.ord[d] - 1) % h.K[d]) != t0[d] ? 2 : 1;
       }
       for (int c = 0; c < nt[2]; c++)
         for (int b = 0; b < nt[1]; b++)
@@ -853,7 +869,7 @@ int md_engine::rebuild() {
     const char *fp = std::getenv("MD_FFT_PLANES");  // =0 keeps separate x and y passes
     S.plane_fused = x->can_fuse && tune_fuse != 0 && !(fp && fp[0] == '0');
     x->lds_plane = S.plane_fused ? x->plane_bytes : 0;
-    x->nb[3] = !S.plane_fused ? cdiv((long)S.K[1] * S.K[2] / 2, S.px_tile) : 0;
+    x->nb[3] = !S.plane_fused ? cdiv(((long)S.K[1] * S.K[2] + 1) / 2, S.px_tile) : 0;
     x->nb[4] = !S.plane_fused ? S.K[2] * cdiv(S.Kh, S.py_tile) : 0;
     x->nb[12] = S.plane_fused ? S.K[2] : 0;
   }

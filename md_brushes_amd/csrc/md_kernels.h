@@ -29,8 +29,9 @@ enum {
 __host__ __device__ inline int plane_line_stride(int K2) { return K2 + 1; }
 // ... and of the z pass's lines (K3 points, K3 + 1 apart), for the same reason.
 __host__ __device__ inline int zline_stride(int K3) { return K3 + 1; }
+// (rows are paired as one complex row; an odd K2 leaves one row alone)
 __host__ __device__ inline size_t plane_region(int K1, int K2, int Kh) {
-  const size_t a = (size_t)(K2 / 2) * K1, b = (size_t)Kh * plane_line_stride(K2);
+  const size_t a = (size_t)((K2 + 1) / 2) * K1, b = (size_t)Kh * plane_line_stride(K2);
   return a > b ? a : b;
 }
 

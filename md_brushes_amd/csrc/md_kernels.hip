@@ -1109,6 +1109,40 @@ __device__ __forceinline__ void tile_add(unsigned long long *acc, double4 p, con
   }
 }
 
+// The same with an order per axis (ordr(1:3) unequal), loops not unrolled.
+template <typename R>
+__device__ __forceinline__ void tile_add_axes(unsigned long long *acc, double4 p, const double *kg,
+                                              const int *K, const int *o, int x0, int y0, int z0,
+                                              const int *E) {
+  R wx[kMaxOrder], wy[kMaxOrder], wz[kMaxOrder], d[kMaxOrder];
+  const int bx = wrapk(spline<R, 0>(p.x, kg[0], o[0], wx, d), K[0]);
+  const int by = wrapk(spline<R, 0>(p.y, kg[1], o[1], wy, d), K[1]);
+  const int bz = wrapk(spline<R, 0>(p.z, kg[2], o[2], wz, d), K[2]);
+  const R q = (R)p.w;
+  for (int c = 0; c < o[2]; c++) {
+    int gz = bz + c;
+    gz -= gz >= K[2] ? K[2] : 0;
+    const int lz = gz - z0;
+    if ((unsigned)lz >= (unsigned)E[2]) continue;
+    for (int b = 0; b < o[1]; b++) {
+      int gy = by + b;
+      gy -= gy >= K[1] ? K[1] : 0;
+      const int ly = gy - y0;
+      if ((unsigned)ly >= (unsigned)E[1]) continue;
+      unsigned long long *row = acc + (lz * E[1] + ly) * E[0];
+      for (int a = 0; a < o[0]; a++) {
+        int gx = bx + a;
+        gx -= gx >= K[0] ? K[0] : 0;
+        const int lx = gx - x0;
+        if ((unsigned)lx < (unsigned)E[0]) {
+          const double v = (double)(wz[c] * ((wx[a] * q) * wy[b]));
+          atomicAdd(row + lx, (unsigned long long)(long long)rint(v * kQScale));
+        }
+      }
+    }
+  }
+}
+
 // MapCharges (compt_acc.f90:1324-1365) by mesh tiles: one workgroup per
 // tile accumulates the charges listed for it (those with spline points in
 // the tile) in LDS, in 2^-40 fixed point, so the sums do not depend on the
@@ -1120,10 +1154,10 @@ __device__ __forceinline__ void spread_block(const Rep *__restrict__ reps, WMap 
   const Rep &S = reps[r];
   const int tile = gb - wm.blk[r];
   const int o = S.order;
-  const int E[3] = {S.tsz[0], S.tsz[1], S.tsz[2]};
+  const int tx = tile % S.ntl[0], ty = (tile / S.ntl[0]) % S.ntl[1], tz = tile / (S.ntl[0] * S.ntl[1]);
+  const int E[3] = {tile_edge(S, 0, tx), tile_edge(S, 1, ty), tile_edge(S, 2, tz)};
   const int K[3] = {S.K[0], S.K[1], S.K[2]};
-  const int x0 = (tile % S.ntl[0]) * E[0], y0 = ((tile / S.ntl[0]) % S.ntl[1]) * E[1],
-            z0 = tile / (S.ntl[0] * S.ntl[1]) * E[2];
+  const int x0 = tile_start(S, 0, tx), y0 = tile_start(S, 1, ty), z0 = tile_start(S, 2, tz);
   const int npt = E[0] * E[1] * E[2];
   for (int k = threadIdx.x; k < npt; k += blockDim.x) acc[k] = 0ull;
   __syncthreads();
@@ -1149,7 +1183,8 @@ __device__ __forceinline__ void spread_block(const Rep *__restrict__ reps, WMap 
       case 5: tile_add<R, 5>(acc, p, S.kg, K, x0, y0, z0, E); break;
       case 6: tile_add<R, 6>(acc, p, S.kg, K, x0, y0, z0, E); break;
       case 7: tile_add<R, 7>(acc, p, S.kg, K, x0, y0, z0, E); break;
-      default: tile_add<R, 8>(acc, p, S.kg, K, x0, y0, z0, E); break;
+      case 8: tile_add<R, 8>(acc, p, S.kg, K, x0, y0, z0, E); break;
+      default: tile_add_axes<R>(acc, p, S.kg, K, S.ord, x0, y0, z0, E); break;
     }
   }
   __syncthreads();
@@ -1202,7 +1237,41 @@ __device__ __forceinline__ void dft4(C &a0, C &a1, C &a2, C &a3) {
 
 template <typename C, int SGN, int R>
 __device__ __forceinline__ void dft_reg(C *v) {
-  if constexpr (R == 2) {
+  using T = decltype(v[0].x);
+  if constexpr (R == 3) {  // X1,2 = v0 - (v1 + v2)/2 +- (SGN i) sqrt(3)/2 (v1 - v2)
+    const C s = cadd(v[1], v[2]);
+    C m = v[0], d = rot90<C, SGN>(csub(v[1], v[2]));
+    m.x -= (T)0.5 * s.x;
+    m.y -= (T)0.5 * s.y;
+    const T c = (T)0.86602540378443864676;
+    d.x *= c;
+    d.y *= c;
+    v[0] = cadd(v[0], s);
+    v[1] = cadd(m, d);
+    v[2] = csub(m, d);
+  } else if constexpr (R == 5) {
+    // with w = exp(SGN 2 pi i / 5): X1,4 = v0 + c1 s1 + c2 s2 +- (SGN i)(n1 d1 + n2 d2),
+    // X2,3 = v0 + c2 s1 + c1 s2 +- (SGN i)(n2 d1 - n1 d2); s1,d1 = v1 +- v4, s2,d2 = v2 +- v3
+    const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
+    const T n1 = (T)0.95105651629515357212, n2 = (T)0.58778525229247312917;
+    const C s1 = cadd(v[1], v[4]), d1 = csub(v[1], v[4]), s2 = cadd(v[2], v[3]), d2 = csub(v[2], v[3]);
+    C a1, a2, b1, b2;
+    a1.x = v[0].x + c1 * s1.x + c2 * s2.x;
+    a1.y = v[0].y + c1 * s1.y + c2 * s2.y;
+    a2.x = v[0].x + c2 * s1.x + c1 * s2.x;
+    a2.y = v[0].y + c2 * s1.y + c1 * s2.y;
+    b1.x = n1 * d1.x + n2 * d2.x;
+    b1.y = n1 * d1.y + n2 * d2.y;
+    b2.x = n2 * d1.x - n1 * d2.x;
+    b2.y = n2 * d1.y - n1 * d2.y;
+    b1 = rot90<C, SGN>(b1);
+    b2 = rot90<C, SGN>(b2);
+    v[0] = cadd(v[0], cadd(s1, s2));
+    v[1] = cadd(a1, b1);
+    v[4] = csub(a1, b1);
+    v[2] = cadd(a2, b2);
+    v[3] = csub(a2, b2);
+  } else if constexpr (R == 2) {
     const C a = v[0];
     v[0] = cadd(a, v[1]);
     v[1] = csub(a, v[1]);
@@ -1284,6 +1353,94 @@ __device__ C *fft_lines(C *a, C *b, int ln, int nl, const C *__restrict__ tw, in
   return src;
 }
 
+// Lines of any length n = fac[0] fac[1] ... : one Stockham pass per factor
+// (the same autosort indexing with divisions in place of shifts), radix 2,
+// 3, 4, 5 and 8 as butterflies in registers; any other factor (a prime) by
+// a pass that gives each thread one output, a DFT of the factor's inputs.
+template <typename C, int SGN, int R>
+__device__ __forceinline__ void stockham_pass_n(const C *src, C *dst, int n, int nl, int Ns,
+                                                const C *__restrict__ tw, int ls) {
+  const int nr = n / R, q = n / (Ns * R);
+  for (int t = threadIdx.x; t < nl * nr; t += blockDim.x) {
+    const int line = t / nr, j = t - line * nr;
+    const int k = j % Ns;
+    const C *s = src + line * ls + j;
+    C v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) v[r] = s[r * nr];
+    if (Ns > 1) {
+#pragma unroll
+      for (int r = 1; r < R; r++) {
+        C w = tw[r * k * q];
+        if (SGN > 0) w.y = -w.y;
+        v[r] = cmul(v[r], w);
+      }
+    }
+    dft_reg<C, SGN, R>(v);
+    C *d = dst + line * ls + (j - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; r++) d[r * Ns] = v[r];
+  }
+}
+
+template <typename C, int SGN>
+__device__ __forceinline__ void dft_pass_n(const C *src, C *dst, int n, int nl, int Ns,
+                                           const C *__restrict__ tw, int ls, int R) {
+  const int nr = n / R, q = n / (Ns * R);
+  for (int t = threadIdx.x; t < nl * n; t += blockDim.x) {
+    const int line = t / n, o = t - line * n;
+    const int sidx = o / nr, j = o - sidx * nr;  // output sidx of butterfly j
+    const int k = j % Ns;
+    const int step = (k + sidx * Ns) * q;  // < n: the twiddle and DFT exponent of input 1
+    const C *sl = src + line * ls + j;
+    C acc;
+    acc.x = 0;
+    acc.y = 0;
+    int idx = 0;
+    for (int r = 0; r < R; r++) {
+      C w = tw[idx];
+      if (SGN > 0) w.y = -w.y;
+      acc = cadd(acc, cmul(sl[r * nr], w));
+      idx += step;
+      idx -= idx >= n ? n : 0;
+    }
+    dst[line * ls + (j - k) * R + k + sidx * Ns] = acc;
+  }
+}
+
+template <typename C, int SGN>
+__device__ C *fft_lines_mixed(C *a, C *b, int n, int nl, const C *__restrict__ tw, int ls,
+                              const int *fac, int nfac) {
+  if (ls == 0) ls = n;
+  int Ns = 1;
+  C *src = a, *dst = b;
+  for (int f = 0; f < nfac; f++) {
+    const int R = fac[f];
+    switch (R) {
+      case 2: stockham_pass_n<C, SGN, 2>(src, dst, n, nl, Ns, tw, ls); break;
+      case 3: stockham_pass_n<C, SGN, 3>(src, dst, n, nl, Ns, tw, ls); break;
+      case 4: stockham_pass_n<C, SGN, 4>(src, dst, n, nl, Ns, tw, ls); break;
+      case 5: stockham_pass_n<C, SGN, 5>(src, dst, n, nl, Ns, tw, ls); break;
+      case 8: stockham_pass_n<C, SGN, 8>(src, dst, n, nl, Ns, tw, ls); break;
+      default: dft_pass_n<C, SGN>(src, dst, n, nl, Ns, tw, ls, R); break;
+    }
+    Ns *= R;
+    __syncthreads();
+    C *x = src;
+    src = dst;
+    dst = x;
+  }
+  return src;
+}
+
+// The lines of mesh axis d: power-of-two lengths take fft_lines.
+template <typename C, int SGN>
+__device__ __forceinline__ C *fft_axis(const Rep &S, int d, C *a, C *b, int nl, int ls = 0) {
+  const C *tw = (const C *)S.tw[d];
+  if (S.pw2[d]) return fft_lines<C, SGN>(a, b, S.lK[d], nl, tw, ls);
+  return fft_lines_mixed<C, SGN>(a, b, S.K[d], nl, tw, ls, S.fac[d], S.nfac[d]);
+}
+
 // Forward x pass: pairs of real mesh rows as one complex FFT of length K1,
 // split into two half spectra.
 template <typename R>
@@ -1292,8 +1449,8 @@ __global__ __launch_bounds__(kBlock) void k_fftx_fwd(const Rep *__restrict__ rep
   extern __shared__ char smem[];
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
-  const int K1 = S.K[0], ln = S.lK[0], Kh = S.Kh;
-  const int npairs = S.K[1] * S.K[2] / 2;
+  const int K1 = S.K[0], Kh = S.Kh;
+  const int nrow = S.K[1] * S.K[2], npairs = (nrow + 1) / 2;  // an odd count leaves a row alone
   const int P = S.px_tile;
   C *ba = (C *)smem, *bb = ba + (size_t)P * K1;
   const int pair0 = (blockIdx.x - wm.blk[r]) * P;
@@ -1304,15 +1461,15 @@ __global__ __launch_bounds__(kBlock) void k_fftx_fwd(const Rep *__restrict__ rep
     const size_t row = 2 * (size_t)(pair0 + l);
     C v;
     v.x = q[row * K1 + x];
-    v.y = q[(row + 1) * K1 + x];
+    v.y = row + 1 < (size_t)nrow ? q[(row + 1) * K1 + x] : (R)0;
     ba[l * K1 + x] = v;
   }
   __syncthreads();
-  const C *buf = fft_lines<C, -1>(ba, bb, ln, nl, (const C *)S.tw[0]);
+  const C *buf = fft_axis<C, -1>(S, 0, ba, bb, nl);
   C *spec = (C *)S.spec;
   for (int t = threadIdx.x; t < nl * Kh; t += blockDim.x) {
     const int l = t / Kh, k = t - l * Kh;
-    const C zk = buf[l * K1 + k], zm = buf[l * K1 + ((K1 - k) & (K1 - 1))];
+    const C zk = buf[l * K1 + k], zm = buf[l * K1 + (k == 0 ? 0 : K1 - k)];
     C A, B;  // A = (Z_k + conj Z_-k)/2, B = (Z_k - conj Z_-k)/(2i)
     A.x = (R)0.5 * (zk.x + zm.x);
     A.y = (R)0.5 * (zk.y - zm.y);
@@ -1320,7 +1477,7 @@ __global__ __launch_bounds__(kBlock) void k_fftx_fwd(const Rep *__restrict__ rep
     B.y = (R)0.5 * (zm.x - zk.x);
     const size_t row = 2 * (size_t)(pair0 + l);
     spec[row * Kh + k] = A;
-    spec[(row + 1) * Kh + k] = B;
+    if (row + 1 < (size_t)nrow) spec[(row + 1) * Kh + k] = B;
   }
 }
 
@@ -1332,8 +1489,8 @@ __global__ __launch_bounds__(kBlock) void k_fftx_inv(const Rep *__restrict__ rep
   extern __shared__ char smem[];
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
-  const int K1 = S.K[0], ln = S.lK[0], Kh = S.Kh;
-  const int npairs = S.K[1] * S.K[2] / 2;
+  const int K1 = S.K[0], Kh = S.Kh;
+  const int nrow = S.K[1] * S.K[2], npairs = (nrow + 1) / 2;
   const int P = S.px_tile;
   C *ba = (C *)smem, *bb = ba + (size_t)P * K1;
   const int pair0 = (blockIdx.x - wm.blk[r]) * P;
@@ -1342,13 +1499,15 @@ __global__ __launch_bounds__(kBlock) void k_fftx_inv(const Rep *__restrict__ rep
   for (int t = threadIdx.x; t < nl * K1; t += blockDim.x) {
     const int l = t / K1, k = t - l * K1;
     const size_t row = 2 * (size_t)(pair0 + l);
+    const bool two = row + 1 < (size_t)nrow;
     C A, B, z;
+    B.x = B.y = (R)0;
     if (k < Kh) {
       A = spec[row * Kh + k];
-      B = spec[(row + 1) * Kh + k];
+      if (two) B = spec[(row + 1) * Kh + k];
     } else {
       A = spec[row * Kh + (K1 - k)];
-      B = spec[(row + 1) * Kh + (K1 - k)];
+      if (two) B = spec[(row + 1) * Kh + (K1 - k)];
       A.y = -A.y;
       B.y = -B.y;
     }
@@ -1357,14 +1516,14 @@ __global__ __launch_bounds__(kBlock) void k_fftx_inv(const Rep *__restrict__ rep
     ba[l * K1 + k] = z;
   }
   __syncthreads();
-  const C *buf = fft_lines<C, +1>(ba, bb, ln, nl, (const C *)S.tw[0]);
+  const C *buf = fft_axis<C, +1>(S, 0, ba, bb, nl);
   R *u = (R *)S.meshu;
   for (int t = threadIdx.x; t < nl * K1; t += blockDim.x) {
     const int l = t / K1, x = t - l * K1;
     const C v = buf[l * K1 + x];
     const size_t row = 2 * (size_t)(pair0 + l);
     u[row * K1 + x] = v.x;
-    u[(row + 1) * K1 + x] = v.y;
+    if (row + 1 < (size_t)nrow) u[(row + 1) * K1 + x] = v.y;
   }
 }
 
@@ -1376,7 +1535,7 @@ __global__ __launch_bounds__(kBlock) void k_ffty(const Rep *__restrict__ reps, W
   extern __shared__ char smem[];
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
-  const int K2 = S.K[1], ln = S.lK[1], Kh = S.Kh, T = S.py_tile, KP = plane_line_stride(K2);
+  const int K2 = S.K[1], Kh = S.Kh, T = S.py_tile, KP = plane_line_stride(K2);
   C *ba = (C *)smem, *bb = ba + (size_t)T * KP;
   const int tiles = (Kh + T - 1) / T;
   const int b = blockIdx.x - wm.blk[r];
@@ -1388,7 +1547,7 @@ __global__ __launch_bounds__(kBlock) void k_ffty(const Rep *__restrict__ reps, W
     ba[l * KP + y] = spec[(size_t)y * Kh + l];
   }
   __syncthreads();
-  const C *buf = fft_lines<C, SGN>(ba, bb, ln, nl, (const C *)S.tw[1], KP);
+  const C *buf = fft_axis<C, SGN>(S, 1, ba, bb, nl, KP);
   for (int t = threadIdx.x; t < nl * K2; t += blockDim.x) {
     const int y = t / nl, l = t - y * nl;
     spec[(size_t)y * Kh + l] = buf[l * KP + y];
@@ -1406,33 +1565,33 @@ __global__ __launch_bounds__(kPlaneBlock) void k_plane_fwd(const Rep *__restrict
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
   const int z = blockIdx.x - wm.blk[r];
-  const int K1 = S.K[0], K2 = S.K[1], Kh = S.Kh, l1 = S.lK[0], l2 = S.lK[1];
+  const int K1 = S.K[0], K2 = S.K[1], Kh = S.Kh, H = (K2 + 1) / 2;  // row pairs (odd K2: one alone)
   const int KP = plane_line_stride(K2);
   C *ra = (C *)smem, *rb = ra + plane_region(K1, K2, Kh);
   const R *q = (const R *)S.meshr + (size_t)z * K2 * K1;
-  for (int t = threadIdx.x; t < (K2 / 2) * K1; t += blockDim.x) {
+  for (int t = threadIdx.x; t < H * K1; t += blockDim.x) {
     const int l = t / K1, x = t - l * K1;
     C v;
     v.x = q[(size_t)(2 * l) * K1 + x];
-    v.y = q[(size_t)(2 * l + 1) * K1 + x];
+    v.y = 2 * l + 1 < K2 ? q[(size_t)(2 * l + 1) * K1 + x] : (R)0;
     ra[l * K1 + x] = v;
   }
   __syncthreads();
-  const C *A = fft_lines<C, -1>(ra, rb, l1, K2 / 2, (const C *)S.tw[0]);
+  const C *A = fft_axis<C, -1>(S, 0, ra, rb, H);
   C *P = A == ra ? rb : ra;
-  for (int t = threadIdx.x; t < (K2 / 2) * Kh; t += blockDim.x) {
+  for (int t = threadIdx.x; t < H * Kh; t += blockDim.x) {
     const int l = t / Kh, k = t - l * Kh;
-    const C zk = A[l * K1 + k], zm = A[l * K1 + ((K1 - k) & (K1 - 1))];
+    const C zk = A[l * K1 + k], zm = A[l * K1 + (k == 0 ? 0 : K1 - k)];
     C a, b;
     a.x = (R)0.5 * (zk.x + zm.x);
     a.y = (R)0.5 * (zk.y - zm.y);
     b.x = (R)0.5 * (zk.y + zm.y);
     b.y = (R)0.5 * (zm.x - zk.x);
     P[k * KP + 2 * l] = a;
-    P[k * KP + 2 * l + 1] = b;
+    if (2 * l + 1 < K2) P[k * KP + 2 * l + 1] = b;
   }
   __syncthreads();
-  const C *Y = fft_lines<C, -1>(P, (C *)A, l2, Kh, (const C *)S.tw[1], KP);
+  const C *Y = fft_axis<C, -1>(S, 1, P, (C *)A, Kh, KP);
   C *spec = (C *)S.spec + (size_t)z * K2 * Kh;
   for (int t = threadIdx.x; t < K2 * Kh; t += blockDim.x) {
     const int y = t / Kh, k = t - y * Kh;
@@ -1450,7 +1609,7 @@ __global__ __launch_bounds__(kPlaneBlock) void k_plane_inv(const Rep *__restrict
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
   const int z = blockIdx.x - wm.blk[r];
-  const int K1 = S.K[0], K2 = S.K[1], Kh = S.Kh, l1 = S.lK[0], l2 = S.lK[1];
+  const int K1 = S.K[0], K2 = S.K[1], Kh = S.Kh, H = (K2 + 1) / 2;
   const int KP = plane_line_stride(K2);
   C *ra = (C *)smem, *rb = ra + plane_region(K1, K2, Kh);
   const C *spec = (const C *)S.spec + (size_t)z * K2 * Kh;
@@ -1459,17 +1618,19 @@ __global__ __launch_bounds__(kPlaneBlock) void k_plane_inv(const Rep *__restrict
     ra[k * KP + y] = spec[t];
   }
   __syncthreads();
-  const C *P = fft_lines<C, +1>(ra, rb, l2, Kh, (const C *)S.tw[1], KP);
+  const C *P = fft_axis<C, +1>(S, 1, ra, rb, Kh, KP);
   C *A = P == ra ? rb : ra;
-  for (int t = threadIdx.x; t < (K2 / 2) * K1; t += blockDim.x) {
+  for (int t = threadIdx.x; t < H * K1; t += blockDim.x) {
     const int l = t / K1, k = t - l * K1;
+    const bool two = 2 * l + 1 < K2;
     C a, b, v;
+    b.x = b.y = (R)0;
     if (k < Kh) {
       a = P[k * KP + 2 * l];
-      b = P[k * KP + 2 * l + 1];
+      if (two) b = P[k * KP + 2 * l + 1];
     } else {
       a = P[(K1 - k) * KP + 2 * l];
-      b = P[(K1 - k) * KP + 2 * l + 1];
+      if (two) b = P[(K1 - k) * KP + 2 * l + 1];
       a.y = -a.y;
       b.y = -b.y;
     }
@@ -1478,13 +1639,13 @@ __global__ __launch_bounds__(kPlaneBlock) void k_plane_inv(const Rep *__restrict
     A[l * K1 + k] = v;
   }
   __syncthreads();
-  const C *X = fft_lines<C, +1>(A, (C *)P, l1, K2 / 2, (const C *)S.tw[0]);
+  const C *X = fft_axis<C, +1>(S, 0, A, (C *)P, H);
   R *u = (R *)S.meshu + (size_t)z * K2 * K1;
-  for (int t = threadIdx.x; t < (K2 / 2) * K1; t += blockDim.x) {
+  for (int t = threadIdx.x; t < H * K1; t += blockDim.x) {
     const int l = t / K1, x = t - l * K1;
     const C v = X[l * K1 + x];
     u[(size_t)(2 * l) * K1 + x] = v.x;
-    u[(size_t)(2 * l + 1) * K1 + x] = v.y;
+    if (2 * l + 1 < K2) u[(size_t)(2 * l + 1) * K1 + x] = v.y;
   }
 }
 
@@ -1497,7 +1658,7 @@ __global__ __launch_bounds__(kPlaneBlock) void k_fftz_conv(const Rep *__restrict
   extern __shared__ char smem[];
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
-  const int K3 = S.K[2], ln = S.lK[2], T = S.pz_tile, KS = zline_stride(K3);
+  const int K3 = S.K[2], T = S.pz_tile, KS = zline_stride(K3);
   C *ba = (C *)smem, *bb = ba + (size_t)T * KS;
   const int f0 = (blockIdx.x - wm.blk[r]) * T;
   const size_t zs = (size_t)S.K[1] * S.Kh;
@@ -1509,7 +1670,7 @@ __global__ __launch_bounds__(kPlaneBlock) void k_fftz_conv(const Rep *__restrict
     ba[l * KS + z] = spec[z * zs + l];
   }
   __syncthreads();
-  C *buf = fft_lines<C, -1>(ba, bb, ln, nl, (const C *)S.tw[2], KS);
+  C *buf = fft_axis<C, -1>(S, 2, ba, bb, nl, KS);
   for (int t = threadIdx.x; t < nl * K3; t += blockDim.x) {
     const int z = t / nl, l = t - z * nl;
     const R g = bc[z * zs + l];
@@ -1519,7 +1680,7 @@ __global__ __launch_bounds__(kPlaneBlock) void k_fftz_conv(const Rep *__restrict
     buf[l * KS + z] = v;
   }
   __syncthreads();
-  const C *out = fft_lines<C, +1>(buf, buf == ba ? bb : ba, ln, nl, (const C *)S.tw[2], KS);
+  const C *out = fft_axis<C, +1>(S, 2, buf, buf == ba ? bb : ba, nl, KS);
   for (int t = threadIdx.x; t < nl * K3; t += blockDim.x) {
     const int z = t / nl, l = t - z * nl;
     spec[z * zs + l] = out[l * KS + z];
@@ -1571,6 +1732,39 @@ __device__ __forceinline__ void gather_one(const Rep &S, double4 p, int cz, R &f
   }
 }
 
+// The same with an order per axis (ordr(1:3) unequal), loops not unrolled.
+template <typename R>
+__device__ __forceinline__ void gather_axes(const Rep &S, double4 p, int cz, R &f1, R &f2, R &f3) {
+  R wx[kMaxOrder], wy[kMaxOrder], wz[kMaxOrder], dx[kMaxOrder], dy[kMaxOrder], dz[kMaxOrder];
+  const int K1 = S.K[0], K2 = S.K[1], K3 = S.K[2];
+  const int bx = wrapk(spline<R, 0>(p.x, S.kg[0], S.ord[0], wx, dx), K1);
+  const int by = wrapk(spline<R, 0>(p.y, S.kg[1], S.ord[1], wy, dy), K2);
+  const int bz = wrapk(spline<R, 0>(p.z, S.kg[2], S.ord[2], wz, dz), K3);
+  const R q = (R)p.w;
+  const R *mesh = (const R *)S.meshu;
+  f1 = f2 = f3 = 0;
+  for (int c = cz; c < S.ord[2]; c += kGatherLanes) {  // this lane's z planes
+    int z = bz + c;
+    z -= z >= K3 ? K3 : 0;
+    for (int b = 0; b < S.ord[1]; b++) {
+      int y = by + b;
+      y -= y >= K2 ? K2 : 0;
+      const R *row = mesh + ((size_t)z * K2 + y) * K1;
+      R s1 = 0, s2 = 0;
+      for (int a = 0; a < S.ord[0]; a++) {
+        int x = bx + a;
+        x -= x >= K1 ? K1 : 0;
+        const R v = row[x];
+        s1 += (dx[a] * q) * v;
+        s2 += (wx[a] * q) * v;
+      }
+      f1 += wz[c] * wy[b] * s1;
+      f2 += wz[c] * dy[b] * s2;
+      f3 += dz[c] * wy[b] * s2;
+    }
+  }
+}
+
 // Reciprocal force of each charge (kGatherLanes lanes per charge, each
 // summing the z planes c = its lane mod kGatherLanes, then a fixed DPP tree
 // over the quad), plus per-block sums of the force (for ZeroForce,
@@ -1601,7 +1795,8 @@ __device__ __forceinline__ void gather_block(const Rep *__restrict__ reps, WMap 
       case 5: gather_one<R, 5>(S, p, cz, f1, f2, f3); break;
       case 6: gather_one<R, 6>(S, p, cz, f1, f2, f3); break;
       case 7: gather_one<R, 7>(S, p, cz, f1, f2, f3); break;
-      default: gather_one<R, 8>(S, p, cz, f1, f2, f3); break;
+      case 8: gather_one<R, 8>(S, p, cz, f1, f2, f3); break;
+      default: gather_axes<R>(S, p, cz, f1, f2, f3); break;
     }
   }
   f1 += dpp_move<0xb1, 0xf>(f1);  // the quad's planes: xor 1, then xor 2
@@ -1823,15 +2018,15 @@ __global__ __launch_bounds__(kBlock) void k_bin_q(const Rep *__restrict__ reps, 
   }
   int t0[3] = {0, 0, 0}, nt[3] = {0, 0, 0};
   if (live) {
-    const int o = S.order;
     const double c3[3] = {p.x, p.y, p.z};
 #pragma unroll
     for (int d = 0; d < 3; d++) {
+      const int o = S.ord[d];
       const int b = spline_base(c3[d], S.kg[d], o, S.K[d]);
-      t0[d] = b / S.tsz[d];
+      t0[d] = tile_of(S, d, b);
       int e = b + o - 1;
       e -= e >= S.K[d] ? S.K[d] : 0;
-      nt[d] = e / S.tsz[d] != t0[d] ? 2 : 1;
+      nt[d] = tile_of(S, d, e) != t0[d] ? 2 : 1;
     }
   }
   // nine appends (up to eight tiles, the sweep cell): the runs first, then

@@ -62,26 +62,11 @@ contains
 
   ! initialize_force_parameters (compt_acc.f90:85-135): the engine builds
   ! bond, anchor and charge lists, B-spline tables and the influence
-  ! function.  Mesh sizes that are not powers of two are rounded up to the
-  ! next power of two (finer mesh).
+  ! function for the mesh and per-axis spline orders of force_data.txt and
+  ! the reference's automatic mesh rule (compt_acc.f90:375-430).
   subroutine initialize_force_parameters
-    integer :: d, k
     character(len=16) :: env
     integer :: lenv, st
-    if (prm%qq /= 0) then
-      do d = 1, 3
-        if (prm%ovr_ng(d) == 0) then
-          k = 4
-          do while (k < prm%ng(d))
-            k = 2 * k
-          end do
-          prm%ovr_ng(d) = k
-        end if
-      end do
-      if (prm%ordr(2) /= prm%ordr(1)) prm%ordr(2) = prm%ordr(1)
-      if (prm%ordr(3) /= prm%ordr(1)) prm%ordr(3) = prm%ordr(1)
-      call check(md_derive_params(prm), 'derive')
-    end if
     prm%precision = MD_FP64
     call get_environment_variable('MD_PRECISION', env, lenv, st)
     if (st == 0 .and. env(1:4) == 'fp32') prm%precision = MD_FP32

@@ -333,41 +333,45 @@ void orc_pme_bc_axis(int ng, int ord, double gdim, double alpha, double *b2,
 
 /* ------------------------------------------------------------------- FFT */
 
-static void fft1d(double complex *a, int n, int sign, double complex *tmp) {
-  if ((n & (n - 1)) == 0) { /* iterative radix-2 */
-    int j = 0;
-    for (int i = 1; i < n; i++) {
-      int bit = n >> 1;
-      for (; j & bit; bit >>= 1) j ^= bit;
-      j ^= bit;
-      if (i < j) {
-        double complex t = a[i];
-        a[i] = a[j];
-        a[j] = t;
-      }
-    }
-    for (int len = 2; len <= n; len <<= 1) {
-      for (int i = 0; i < n; i += len)
-        for (int k = 0; k < len / 2; k++) {
-          double ang = sign * 2 * PI * k / len;
-          double complex w = cos(ang) + I * sin(ang);
-          double complex u = a[i + k], v = a[i + k + len / 2] * w;
-          a[i + k] = u + v;
-          a[i + k + len / 2] = u - v;
-        }
-    }
-  } else { /* direct DFT for other lengths */
-    for (int k = 0; k < n; k++) {
-      double complex s = 0;
-      for (int j = 0; j < n; j++) {
-        long jk = ((long)j * k) % n;
-        double ang = sign * 2 * PI * (double)jk / n;
-        s += a[j] * (cos(ang) + I * sin(ang));
-      }
-      tmp[k] = s;
-    }
-    memcpy(a, tmp, sizeof(double complex) * n);
+/* Unnormalised DFT of one line by mixed-radix decimation in time: the p
+ * sub-transforms of length n/p (p the smallest prime factor of n) over the
+ * inputs r, r + p, ..., then X[k + m q] = sum_r w^(r (k + m q) N/n) Y_r[k].
+ * w[k] = exp(sign 2 pi i k / N) is the table of the whole line's length N,
+ * computed once per axis; tmp holds n points per level. */
+static int smallest_factor(int n) {
+  if (n % 2 == 0) return 2;
+  for (int f = 3; f * f <= n; f += 2)
+    if (n % f == 0) return f;
+  return n;
+}
+
+static void fft_rec(const double complex *x, int xs, double complex *X, int n, int N,
+                    const double complex *w, double complex *tmp) {
+  if (n == 1) {
+    X[0] = x[0];
+    return;
   }
+  int p = smallest_factor(n), m = n / p, wstep = N / n;
+  for (int r = 0; r < p; r++) fft_rec(x + (size_t)r * xs, xs * p, X + (size_t)r * m, m, N, w, tmp + n);
+  for (int q = 0; q < p; q++)
+    for (int k = 0; k < m; k++) {
+      double complex s = 0;
+      for (int r = 0; r < p; r++) {
+        long e = ((long)r * (k + (long)m * q)) % n;
+        s += X[(size_t)r * m + k] * w[e * wstep];
+      }
+      tmp[k + (size_t)m * q] = s;
+    }
+  memcpy(X, tmp, sizeof(double complex) * n);
+}
+
+static double complex *twiddles(int n, int sign) {
+  double complex *w = malloc(sizeof(double complex) * n);
+  for (int k = 0; k < n; k++) {
+    double ang = sign * 2 * PI * (double)k / n;
+    w[k] = cos(ang) + I * sin(ang);
+  }
+  return w;
 }
 
 /* Unnormalised 3D DFT, FFTW sign convention (fftw3.f90:12-13: forward -1,
@@ -377,26 +381,34 @@ static void fft3d_c(double complex *a, int n1, int n2, int n3, int sign) {
   int nmax = n1 > n2 ? n1 : n2;
   nmax = nmax > n3 ? nmax : n3;
   double complex *line = malloc(sizeof(double complex) * nmax);
-  double complex *tmp = malloc(sizeof(double complex) * nmax);
+  double complex *out = malloc(sizeof(double complex) * nmax);
+  /* scratch: n points per recursion level, at most 2n in all */
+  double complex *tmp = malloc(sizeof(double complex) * 2 * nmax);
+  double complex *w1 = twiddles(n1, sign), *w2 = twiddles(n2, sign), *w3 = twiddles(n3, sign);
   for (int k = 0; k < n3; k++)
     for (int j = 0; j < n2; j++) {
       double complex *r = a + ((size_t)k * n2 + j) * n1;
-      fft1d(r, n1, sign, tmp);
+      fft_rec(r, 1, out, n1, n1, w1, tmp);
+      memcpy(r, out, sizeof(double complex) * n1);
     }
   for (int k = 0; k < n3; k++)
     for (int i = 0; i < n1; i++) {
       for (int j = 0; j < n2; j++) line[j] = a[((size_t)k * n2 + j) * n1 + i];
-      fft1d(line, n2, sign, tmp);
-      for (int j = 0; j < n2; j++) a[((size_t)k * n2 + j) * n1 + i] = line[j];
+      fft_rec(line, 1, out, n2, n2, w2, tmp);
+      for (int j = 0; j < n2; j++) a[((size_t)k * n2 + j) * n1 + i] = out[j];
     }
   for (int j = 0; j < n2; j++)
     for (int i = 0; i < n1; i++) {
       for (int k = 0; k < n3; k++) line[k] = a[((size_t)k * n2 + j) * n1 + i];
-      fft1d(line, n3, sign, tmp);
-      for (int k = 0; k < n3; k++) a[((size_t)k * n2 + j) * n1 + i] = line[k];
+      fft_rec(line, 1, out, n3, n3, w3, tmp);
+      for (int k = 0; k < n3; k++) a[((size_t)k * n2 + j) * n1 + i] = out[k];
     }
   free(line);
+  free(out);
   free(tmp);
+  free(w1);
+  free(w2);
+  free(w3);
 }
 
 void orc_fft3d(int n1, int n2, int n3, double *re_im, int sign) {

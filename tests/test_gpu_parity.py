@@ -622,3 +622,97 @@ def test_energies_leave_trajectory():
         e.close()
     for x, y in zip(out[0], out[1]):
         assert np.array_equal(x, y)
+
+
+def _mesh_variant(name, ng, ordr, **kw):
+    """A named system with the SPME mesh and per-axis orders given."""
+    p, pos, vel = system(name, MD_FP64)
+    q = p.copy()
+    for d in range(3):
+        q.ovr_ng[d] = ng[d]
+        q.ordr[d] = ordr[d]
+    for k, v in kw.items():
+        setattr(q, k, v)
+    from md_brushes_amd import derive
+    return derive(q), pos, vel
+
+
+MESHES = {
+    # odd sizes on every axis (an odd row count, a lone row per plane)
+    "odd_45x45x135_p5": ("small", (45, 45, 135), (5, 5, 5)),
+    # prime factors 7, 11, 73 (the reference's rule gives lengths like 219 = 3 x 73)
+    "primes_21x22x146_p4": ("small", (21, 22, 146), (4, 4, 4)),
+    # an order per axis (force_data.txt holds three, compt_acc.f90:1232-1234)
+    "orders_32x30x128_p456": ("small", (32, 30, 128), (4, 5, 6)),
+    "orders_36x40x120_p635": ("small", (36, 40, 120), (6, 3, 5)),
+}
+
+
+@pytest.mark.parametrize("case", sorted(MESHES))
+@pytest.mark.parametrize("planes", ["1", "0"])
+def test_spme_any_mesh_and_orders(case, planes, monkeypatch):
+    """Meshes of any length (mixed-radix Stockham passes: radix 2, 3, 4, 5,
+    8 butterflies and a DFT pass for other primes) and unequal orders per
+    axis: FP64 Coulomb forces within 1e-10 of the oracle, with the fused
+    x/y planes and with the separate passes; FP32 within 1e-4."""
+    print(f"[spme mesh] {case} planes={planes}", flush=True)
+    name, ng, ordr = MESHES[case]
+    monkeypatch.setenv("MD_FFT_PLANES", planes)
+    p, pos, vel = _mesh_variant(name, ng, ordr)
+    o = orc.Oracle(p)
+    o.set_state(pos, vel)
+    o.build_lists()
+    ref = o.coulomb()
+    for prec, tol in ((MD_FP64, 1e-10), (MD_FP32, 1e-4)):
+        q = p.copy()
+        q.precision = prec
+        e = Engine(q)
+        e.upload(pos, vel)
+        e.build_lists()
+        fc = e.force_parts()[1]
+        e.close()
+        err = orc.rel_rms(fc, ref)
+        print(f"  prec={prec} rel rms {err:.2e}")
+        assert err < tol
+
+
+@pytest.mark.parametrize("prec", [MD_FP64, MD_FP32])
+def test_reference_ewald_setup(prec):
+    """The default brush with force_data.txt's own Ewald set-up (tol 1.9,
+    tau_rf 800, order 6, no overrides): the reference's automatic rule gives
+    alpha 0.2079, r_c 9.139 and the 45 x 45 x 162 mesh (compt_acc.f90:375-430),
+    which the engine runs as it stands.  Forces against the oracle: FP64
+    within 1e-10, FP32 within 1e-4; a Philox run follows the oracle."""
+    print(f"[reference set-up] default 45x45x162 order 6 prec={prec}", flush=True)
+    p, pos, vel = system("default", MD_FP64)
+    q = p.copy()
+    q.ovr_ng[0] = q.ovr_ng[1] = q.ovr_ng[2] = 0
+    q.ovr_alpha = 0.0
+    q.ovr_rcc = 0.0
+    q.tol, q.tau_rf = 1.9, 800.0
+    q.ordr[0] = q.ordr[1] = q.ordr[2] = 6
+    from md_brushes_amd import derive
+    q = derive(q)
+    assert list(q.ng) == [45, 45, 162] and abs(q.alpha - 0.2079) < 5e-5 and q.real_verlet == 1
+    o = orc.Oracle(q)
+    o.set_state(pos, vel)
+    o.build_lists()
+    ref_s = o.lj_force() + o.fene_force()[0]
+    ref_c = o.coulomb()
+    q.precision = prec
+    e = Engine(q)
+    e.upload(pos, vel)
+    e.build_lists()
+    fs, fc = e.force_parts()
+    if prec == MD_FP64:
+        assert orc.rel_rms(fc, ref_c) < 1e-10
+        e.compute_force(0, MD_NOISE_PHILOX)
+        assert o.compute_force(0, MD_NOISE_PHILOX) == 0
+        e.run(1, 20)
+        assert o.run(1, 20) == 0
+        assert np.abs(e.download()[0] - o.state()[0]).max() < 1e-8
+    else:
+        err = orc.rel_rms(fs + fc, ref_s + ref_c)
+        print(f"fp32 relative force rms {err:.3e}")
+        assert err < 1e-4
+    e.close()

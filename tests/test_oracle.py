@@ -127,7 +127,7 @@ def test_cubic_bspline_closed_form():
 
 # ------------------------------------------------------------------- FFT
 
-@pytest.mark.parametrize("shape", [(8, 4, 16), (6, 5, 9), (32, 2, 4)])
+@pytest.mark.parametrize("shape", [(8, 4, 16), (6, 5, 9), (32, 2, 4), (45, 7, 22), (73, 3, 30), (1, 11, 12)])
 @pytest.mark.parametrize("sign", [-1, 1])
 def test_fft_matches_numpy(shape, sign):
     n1, n2, n3 = shape

@@ -263,6 +263,19 @@ int md_batch_observe(md_engine *e, int32_t slot, double *height,
 int md_batch_error_analysis(md_engine *e, int32_t slot, double *out,
                             double *acc_se);
 
+/* md_batch_error_analysis that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Its buffers live in a static device scratch array of 64 MiB
+ * per GPU, held from library load to process end; a replica whose check
+ * needs more is refused with MD_EINVAL.  The k-vectors are the whole box
+ * with zero weight outside the reference's ellipsoid, so the results equal
+ * md_batch_error_analysis's up to the grouping of the sums.  Calls that
+ * overlap on one GPU share its scratch and race.  The engine must have run
+ * a cycle since its last membership change (MD_EINVAL otherwise). */
+int md_batch_error_analysis_noalloc(md_engine *e, int32_t slot, double *out,
+                                    double *acc_se);
+
 /* The height analysis of write_height (in_output.f90:1017-1079, columns 2-9
  * of height.txt, :1342-1344) from the resident positions: out[8] = h_avg
  * (polymer beads), hl_max, hl_end, hl_avg (linear chains: mean over chains

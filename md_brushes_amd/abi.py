@@ -111,6 +111,7 @@ def load_library(path=None):
         "md_state_load": ([c_void_p, c_void_p, c_int64], _I),
         "md_batch_heights": ([c_void_p, _I, dp], _I),
         "md_batch_error_analysis": ([c_void_p, _I, dp, dp], _I),
+        "md_batch_error_analysis_noalloc": ([c_void_p, _I, dp, dp], _I),
         "md_destroy": ([c_void_p], None),
         "md_batch_create": ([POINTER(c_void_p), _I, _I], _I),
         "md_batch_add": ([c_void_p, P, dp, dp, ip], _I),

@@ -800,6 +800,7 @@ struct md_engine {
   int add(const md_params *p, int *slot);
   int launch_compute_force(const Plan &P, Sys *s, int step, int noise, const double *eta, bool in_step);
   int error_analysis(Sys *s, const Plan &P, double *out, double *acc_se);
+  int error_analysis_noalloc(Sys *s, const Plan &P, double *out, double *acc_se);
   int build_graph(int q);
   int run_cycles(int ncycles);
   bool any_real_list() const {
@@ -1797,6 +1798,105 @@ int md_engine::error_analysis(Sys *s, const Plan &P, double *out, double *acc_se
   return check_sys(s);
 }
 
+// error_analysis without any allocation (for callers that cannot allocate:
+// graph capture, tight serving loops): the same reference force, with its
+// buffers carved from the library's per-GPU device scratch
+// (mdk::ewald_scratch, kEwaldScratchBytes), the k-vectors as the whole box
+// with zero weight outside the ellipsoid (the same sums up to their
+// grouping), fill kernels in place of memsets, and the rmsf sums and sum q^2
+// on the device.  Sizes beyond the scratch are refused.  Calls that overlap
+// on one GPU share its scratch and race; every buffer is written before it
+// is read, so consecutive calls need no clearing.
+int md_engine::error_analysis_noalloc(Sys *s, const Plan &P, double *out, double *acc_se) {
+  if (!out) return fail(MD_EINVAL, "out is null");
+  if (!s->coulomb) return fail(MD_EINVAL, "error_analysis needs a charged system");
+  char *scr = (char *)mdk::ewald_scratch();
+  if (!scr) return fail(MD_EHIP, "device scratch address");
+  int rc = check_sys(s);
+  if (rc) return rc;
+  const md_params &p = s->p;
+  const int NN = p.NN, Nq = s->h.Nq;
+  double alpha, rcc, rvc;
+  int k[3], rv;
+  mdk::ewald_rule(&p, 5.0, 10.0, false, &alpha, &rcc, &rvc, k, &rv);
+  int K[3];
+  for (int d = 0; d < 3; d++) K[d] = (int)std::ceil((float)(int)std::ceil((float)k[d] * 3.0f) / 3.0f);
+  const double Ze = p.Zfac, c1 = 2 * kPi / p.Lx, c2 = 2 * kPi / p.Ly, c3 = 2 * kPi / p.Lz / Ze;
+  const double pref = 4 * kPi / (p.Lx * p.Ly * p.Lz * Ze) * p.lb / p.Beta;
+  mdk::EwaldArgs a;
+  a.Nq = Nq;
+  a.NN = NN;
+  const long nbox = (long)(K[2] + 1) * (2 * K[0] + 1) * (2 * K[1] + 1);
+  if (nbox > (1L << 28)) return fail(MD_EINVAL, "error_analysis k-space too large for the device scratch");
+  a.nk = (int)nbox;
+  const int bq = cdiv(Nq, 256), bk = cdiv(a.nk, 256);
+  a.ms = std::max(1, std::min(16, cdiv(1024, bk)));
+  a.ks = std::max(1, std::min(bk, cdiv(1024, bq)));
+  a.rs = std::max(1, std::min(bq, cdiv(1024, bq)));
+  a.charge = s->h.charge;
+  a.pos = s->h.pos;
+  a.Lx = p.Lx;
+  a.Ly = p.Ly;
+  a.hLx = p.Lx / 2;
+  a.hLy = p.Ly / 2;
+  a.alpha = alpha;
+  a.alpha2 = alpha * alpha;
+  a.c2 = 2 * alpha / std::sqrt(kPi);
+  a.rcc2 = rcc * rcc;
+  a.c1 = p.lb / p.Beta;
+  a.slab = 4 * kPi / p.Lx / p.Ly / p.Lz / Ze;
+  const size_t nq = std::max(Nq, 1), nk = std::max(a.nk, 1);
+  size_t used = 0;
+  auto carve = [&](size_t n) {
+    char *x = scr + used;
+    used += (n + 255) / 256 * 256;
+    return x;
+  };
+  a.cq = (double4 *)carve(32 * nq);
+  double4 *kvd = (double4 *)carve(32 * nk);
+  a.kv = kvd;
+  a.sp = (double2 *)carve(16 * nk * a.ms);
+  a.sk = (double2 *)carve(16 * nk);
+  a.rp = (double *)carve(24 * nq * a.ks);
+  a.qp = (double *)carve(24 * nq * a.rs);
+  a.mz = (double *)carve(8);
+  double *stats = (double *)carve(3 * 8);
+  if (used > mdk::kEwaldScratchBytes)
+    return fail(MD_EINVAL, "error_analysis needs " + std::to_string(used) + " bytes of device scratch, more than " +
+                               std::to_string(mdk::kEwaldScratchBytes));
+  a.out = s->d_out1;
+  const size_t n3 = 3 * (size_t)NN;
+  mdk::launch_ewald_box(a.nk, kvd, st, K[0], K[1], K[2], c1, c2, c3, pref, alpha * alpha);
+  mdk::launch_zero(s->d_out1, n3, st);
+  mdk::launch_zero(s->d_out2, n3, st);
+  mdk::launch_ewald(a, st);
+  if (fp32) mdk::launch_coulomb<float>(P, st, s->d_out2);
+  else mdk::launch_coulomb<double>(P, st, s->d_out2);
+  mdk::launch_ewald_stats(s->d_out1, s->d_out2, n3, a, stats, st);
+  double h[3];
+  hipError_t err = hipMemcpyAsync(h, stats, sizeof h, hipMemcpyDeviceToHost, st);
+  if (err == hipSuccess && acc_se) err = hipMemcpyAsync(acc_se, s->d_out1, n3 * sizeof(double), hipMemcpyDeviceToHost, st);
+  if (err == hipSuccess) err = hipStreamSynchronize(st);
+  if (err == hipSuccess) err = hipGetLastError();
+  if (err != hipSuccess) return hip_fail(err, "error_analysis");
+  // production estimates (compt_acc.f90:746-751), as error_analysis computes them
+  const double third = (double)(1.0f / 3.0f);
+  const double ex = std::exp(-p.tol * p.tol);
+  const double f_r = 2 * std::fabs(p.qq) * std::sqrt(h[2] / p.rcc / p.Lx / p.Ly / p.Lz) * ex;
+  const float kprod = std::pow((float)(p.ng[0] * p.ng[1] * p.ng[2]), 1.0f / 3.0f);
+  const double f_k = std::fabs(p.qq) * p.alpha / std::pow(p.Lx * p.Ly * p.Lz, third) / kPi *
+                     std::sqrt(8 * h[2] / (double)kprod) * ex;
+  out[0] = h[1] > 0 ? std::sqrt(h[0] / h[1]) : 0;
+  out[1] = f_r;
+  out[2] = f_k;
+  out[3] = alpha;
+  out[4] = rcc;
+  out[5] = K[0];
+  out[6] = K[1];
+  out[7] = K[2];
+  return check_sys(s);
+}
+
 extern "C" int md_error_analysis(md_engine *e, double *out, double *acc_se) {
   ENGINE_CHECK(e && out);
   FIRST(e);
@@ -1946,6 +2046,17 @@ extern "C" int md_batch_observe(md_engine *e, int32_t slot, double *height, doub
   if (profile)
     for (int k = 0; k < mdk::kHistBins; k++) profile[k] = hist[k];
   return MD_OK;
+}
+
+extern "C" int md_batch_error_analysis_noalloc(md_engine *e, int32_t slot, double *out,
+                                               double *acc_se) {
+  ENGINE_CHECK(e && out);
+  auto it = e->slots.find(slot);
+  if (it == e->slots.end()) return e->fail(MD_EINVAL, "no such slot");
+  // a membership change re-uploads descriptors (an allocation): run a cycle first
+  if (e->dirty) return e->fail(MD_EINVAL, "replicas changed since the last run: run a cycle first");
+  Sys *s = it->second;
+  return e->error_analysis_noalloc(s, e->single[e->index_of(s)], out, acc_se);
 }
 
 extern "C" int md_batch_error_analysis(md_engine *e, int32_t slot, double *out, double *acc_se) {

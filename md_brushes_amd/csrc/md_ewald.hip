@@ -14,6 +14,8 @@
 // minimum image) runs as an all-pairs sweep, a thread per target charge.
 // Each pass splits its inner range into slices whose partial sums are added
 // in slice order, so the result does not depend on scheduling.
+#include <mutex>
+
 #include "md_kernels.h"
 
 namespace mdk {
@@ -21,6 +23,11 @@ namespace mdk {
 namespace {
 
 constexpr int kEB = 256;  // threads per workgroup and LDS stage of every pass
+
+// Device scratch of the allocation-free error analysis: every GPU holds its
+// own copy from library load to process end.  In host code its name is not a
+// device address; ewald_scratch() returns the address on the current device.
+__device__ __align__(256) char scratch[kEwaldScratchBytes];
 
 __device__ __forceinline__ int slice_begin(int n, int j, int ns) {
   return (int)((long long)n * j / ns);
@@ -178,7 +185,88 @@ __global__ __launch_bounds__(kEB) void k_ew_final(EwaldArgs a) {
   a.out[2 * (size_t)a.NN + i] = f[2];
 }
 
+// The k-vectors of Standard_Ewald as the whole box r = 0..K3, p = -K1..K1,
+// q = -K2..K2 in the reference's loop order (compt_acc.f90:650-668), e_k = 0
+// outside the ellipsoid (kcut > 1, in default real) and at k = 0, so the
+// passes need no compaction (a vector of weight 0 adds nothing).
+__global__ __launch_bounds__(kEB) void k_ew_kbox(int nk, double4 *kv, int K1, int K2, int K3, double c1,
+                                                 double c2, double c3, double pref, double alpha2) {
+  const int k = blockIdx.x * kEB + threadIdx.x;
+  if (k >= nk) return;
+  const int w2 = 2 * K2 + 1, w12 = (2 * K1 + 1) * w2;
+  const int r = k / w12, pp = (k - r * w12) / w2 - K1, q = k % w2 - K2;
+  const float fa = 1.0f * pp / K1, fb = 1.0f * q / K2, fc = 1.0f * r / K3;
+  const float kcut = fa * fa + fb * fb + fc * fc;
+  const double kx = pp * c1, ky = q * c2, kz = r * c3;
+  const double ksqr = kx * kx + ky * ky + kz * kz;
+  double ek = 0;
+  if (!(kcut > 1 || kcut == 0)) ek = (r == 0 ? 1 : 2) * pref * exp(-ksqr / 4 / alpha2) / ksqr;
+  kv[k] = make_double4(kx, ky, kz, ek);
+}
+
+__global__ __launch_bounds__(kEB) void k_ew_zero(double *x, size_t n) {
+  for (size_t k = (size_t)blockIdx.x * kEB + threadIdx.x; k < n; k += (size_t)gridDim.x * kEB) x[k] = 0;
+}
+
+// rmsf sums over every particle and sum q^2 over the charges, one
+// workgroup in a fixed order: out[0] = sum |F_se - F_c|^2, out[1] =
+// sum |F_se|^2, out[2] = sum q^2.
+__global__ __launch_bounds__(kEB) void k_ew_stats(const double *fse, const double *fc, size_t n3,
+                                                  EwaldArgs a, double *out) {
+  __shared__ double red[3][kEB];
+  double s1 = 0, s2 = 0, s3 = 0;
+  for (size_t k = threadIdx.x; k < n3; k += kEB) {
+    const double d = fse[k] - fc[k];
+    s1 += d * d;
+    s2 += fse[k] * fse[k];
+  }
+  for (int m = threadIdx.x; m < a.Nq; m += kEB) s3 += a.cq[m].w * a.cq[m].w;
+  red[0][threadIdx.x] = s1;
+  red[1][threadIdx.x] = s2;
+  red[2][threadIdx.x] = s3;
+  __syncthreads();
+  for (int w = kEB / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int t = 0; t < 3; t++) red[t][threadIdx.x] += red[t][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) out[threadIdx.x] = red[threadIdx.x][0];
+}
+
 }  // namespace
+
+void *ewald_scratch() {
+  // the address of `scratch` on each device, taken once per device with that
+  // device current (fixed table: no allocation)
+  constexpr int kMaxDev = 64;
+  static std::mutex mu;
+  static void *ptr[kMaxDev] = {nullptr};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDev) return nullptr;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptr[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+    ptr[dev] = p;
+  }
+  return ptr[dev];
+}
+
+void launch_ewald_box(int nk, double4 *kv, hipStream_t st, int K1, int K2, int K3, double c1, double c2,
+                      double c3, double pref, double alpha2) {
+  hipLaunchKernelGGL(k_ew_kbox, dim3((nk + kEB - 1) / kEB), dim3(kEB), 0, st, nk, kv, K1, K2, K3, c1, c2,
+                     c3, pref, alpha2);
+}
+
+void launch_zero(double *x, size_t n, hipStream_t st) {
+  const int nb = (int)std::min<size_t>(1024, (n + kEB - 1) / kEB);
+  hipLaunchKernelGGL(k_ew_zero, dim3(nb > 0 ? nb : 1), dim3(kEB), 0, st, x, n);
+}
+
+void launch_ewald_stats(const double *fse, const double *fc, size_t n3, const EwaldArgs &a, double *out,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(k_ew_stats, dim3(1), dim3(kEB), 0, st, fse, fc, n3, a, out);
+}
 
 void launch_ewald(const EwaldArgs &a, hipStream_t st) {
   const int bq = (a.Nq + kEB - 1) / kEB, bk = (a.nk + kEB - 1) / kEB;

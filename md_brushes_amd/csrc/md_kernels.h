@@ -109,6 +109,19 @@ struct EwaldArgs {
 };
 void launch_ewald(const EwaldArgs &a, hipStream_t st);
 
+// The allocation-free form: its buffers are carved from a file-scope device
+// scratch array of kEwaldScratchBytes (one per GPU); ewald_scratch() gives
+// its address on the current device (null on failure).  k-vectors as the
+// whole box with zero weight outside the ellipsoid (launch_ewald_box), a fill
+// kernel in place of memsets, and the rmsf sums on the device.
+constexpr size_t kEwaldScratchBytes = (size_t)64 << 20;
+void *ewald_scratch();
+void launch_ewald_box(int nk, double4 *kv, hipStream_t st, int K1, int K2, int K3, double c1, double c2,
+                      double c3, double pref, double alpha2);
+void launch_zero(double *x, size_t n, hipStream_t st);
+void launch_ewald_stats(const double *fse, const double *fc, size_t n3, const EwaldArgs &a, double *out,
+                        hipStream_t st);
+
 // Initialize_ewald_parameters (compt_acc.f90:387-414) at (tol, tau_rf):
 // alpha, r_c, r_v, the k-ranges before the mesh factor 3, and whether the
 // real space keeps a Verlet list; ovr_alpha / ovr_rcc apply when use_ovr.

@@ -251,6 +251,13 @@ int md_batch_download(md_engine *e, int32_t slot, double *pos, double *vel,
  * replica count. */
 int md_batch_info(md_engine *e, int32_t slot, int64_t *bytes, int32_t *nrep);
 
+/* Device bytes that md_batch_add would allocate for a replica with these
+ * parameters and positions pos(NN,4) (the same sizing, lists sized from the
+ * configuration; the read-only tables shared by replicas of one mesh not
+ * counted), computed without touching the device: what a request loop
+ * admits against its HBM budget.  No reference counterpart. */
+int md_replica_bytes(const md_params *p, const double *pos, int64_t *bytes);
+
 /* Observables computed on the device: brush height (mean z of the polymer
  * beads, in_output.f90:1017-1020) and the z-density histogram of the polymer
  * beads, nbins = 500 bins of Lz/500 (in_output.f90:1878-1886). */

@@ -2,9 +2,9 @@
 polyelectrolyte brushes (the hot path of wangshaoyun/MD_Brushes)."""
 from .abi import (Batch, MD_FP32, MD_FP64, MD_NOISE_ETA, MD_NOISE_NONE, MD_NOISE_PHILOX, Engine, MDError,
                   MdParams, build_topology, derive, initialize_position, initialize_velocity,
-                  load_library)
+                  load_library, replica_bytes)
 from .configs import config, reference_defaults, replica
 
 __all__ = ["Batch", "Engine", "MDError", "MdParams", "MD_FP32", "MD_FP64", "MD_NOISE_ETA", "MD_NOISE_NONE",
            "MD_NOISE_PHILOX", "build_topology", "config", "derive", "initialize_position",
-           "initialize_velocity", "load_library", "reference_defaults", "replica"]
+           "initialize_velocity", "load_library", "reference_defaults", "replica", "replica_bytes"]

@@ -120,6 +120,7 @@ def load_library(path=None):
         "md_batch_status": ([c_void_p, _I, ip, c_char_p, _I], _I),
         "md_batch_download": ([c_void_p, _I, dp, dp, dp], _I),
         "md_batch_info": ([c_void_p, _I, POINTER(c_int64), ip], _I),
+        "md_replica_bytes": ([P, dp, POINTER(c_int64)], _I),
         "md_batch_observe": ([c_void_p, _I, dp, dp, _I], _I),
     }
     for name, (args, res) in sig.items():
@@ -164,6 +165,15 @@ def build_topology(p):
         raise MDError(rc, "md_build_topology")
     nb = p.N_bond
     return charge[: p.Nq], fene[: 2 * nb].reshape(2, nb).T.copy(), anchor[: p.N_anchor]
+
+
+def replica_bytes(p, pos):
+    """Device bytes md_batch_add would allocate for this replica (no device call)."""
+    b = c_int64()
+    rc = load_library().md_replica_bytes(byref(p), _dp(_colmajor(pos)), byref(b))
+    if rc:
+        raise MDError(rc, "md_replica_bytes")
+    return b.value
 
 
 def initialize_position(p, seed):

@@ -153,6 +153,9 @@ struct Sys {
   double *d_hsum = nullptr;  // observables: per-block sums of z
   double *d_mol = nullptr;   // observables: per-molecule heights
   double *d_epart = nullptr; // energies: per-block partial sums
+  // dry run: allocations are only counted (no device call), for the bytes a
+  // replica would take before it is admitted
+  bool dry = false;
   int en_blocks[3] = {0, 0, 0};  // energy kernels' blocks: particles, charges, mesh
   std::string err;
 
@@ -165,6 +168,10 @@ struct Sys {
   }
   template <typename T> T *dalloc(size_t n) {
     if (n == 0) n = 1;
+    if (dry) {  // a placeholder address that is never dereferenced
+      bytes += n * sizeof(T);
+      return reinterpret_cast<T *>(alignof(double4));
+    }
     void *ptr = nullptr;
     if (hipMalloc(&ptr, n * sizeof(T)) != hipSuccess) return nullptr;
     (void)hipMemset(ptr, 0, n * sizeof(T));
@@ -173,6 +180,10 @@ struct Sys {
     return (T *)ptr;
   }
   void dfree(void *ptr, size_t nbytes) {
+    if (dry) {
+      bytes -= nbytes;
+      return;
+    }
     auto it = std::find(allocs.begin(), allocs.end(), ptr);
     if (it != allocs.end()) {
       (void)hipFree(ptr);
@@ -351,17 +362,17 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   if (!cc || !S.flags || !S.charge || !S.bstart || !S.bpart || !S.ctrl || !d_eta || !d_out1 ||
       !d_out2 || !d_hist || !d_hsum || !d_mol || !d_epart)
     return fail(MD_ENOMEM, "topology arrays");
-  hipError_t cerr = hipMemcpy(S.flags, flags.data(), nn * sizeof(int), hipMemcpyHostToDevice);
-  if (cerr == hipSuccess && Nq)
+  hipError_t cerr = dry ? hipSuccess : hipMemcpy(S.flags, flags.data(), nn * sizeof(int), hipMemcpyHostToDevice);
+  if (cerr == hipSuccess && Nq && !dry)
     cerr = hipMemcpy(S.charge, charge.data(), Nq * sizeof(int), hipMemcpyHostToDevice);
-  if (cerr == hipSuccess)
+  if (cerr == hipSuccess && !dry)
     cerr = hipMemcpy(S.bstart, bstart.data(), (nn + 1) * sizeof(int), hipMemcpyHostToDevice);
-  if (cerr == hipSuccess)
+  if (cerr == hipSuccess && !dry)
     cerr = hipMemcpy(S.bpart, bpart.data(), bpart.size() * sizeof(int), hipMemcpyHostToDevice);
   if (cerr != hipSuccess) return fail(MD_EHIP, hip_msg(cerr, "topology upload"));
   S.cnt1 = dalloc<int>(2 * (size_t)ncell1);  // two count buffers, used in turn
   S.cell1_of = dalloc<int>(NN);
-  if (S.cell1_of) (void)hipMemset(S.cell1_of, 0xff, NN * sizeof(int));  // no cell yet
+  if (S.cell1_of && !dry) (void)hipMemset(S.cell1_of, 0xff, NN * sizeof(int));  // no cell yet
   S.cnt2[0] = dalloc<int>(std::max(ncell2, 1));
   S.cnt2[1] = dalloc<int>(std::max(ncell2, 1));
   S.cnt3 = dalloc<int>(std::max(ncell3, 1));
@@ -427,10 +438,13 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
     // MD_FFT_PLANES=0 keeps the separate x and y passes (for comparison)
     const char *fp = std::getenv("MD_FFT_PLANES");
     // (at most 64 KB: larger planes leave one workgroup per CU and lose)
-    can_fuse = lds_plane <= std::min<size_t>(65536, lds_limit()) && !(fp && fp[0] == '0');
+    can_fuse = lds_plane <= std::min<size_t>(65536, dry ? 65536 : lds_limit()) && !(fp && fp[0] == '0');
     S.plane_fused = can_fuse;
     if (!S.plane_fused) lds_plane = 0;
     // influence function of PME_BC (compt_acc.f90:1557-1598), half spectrum
+    // (shared between replicas and not counted in a replica's bytes: a dry
+    // run skips them)
+    if (!dry) {
     char key[256];
     std::snprintf(key, sizeof key, "bc %d %d %d %d %d %d %a %a %a %a %d %d", K1, K2, K3, p.ordr[0],
                   p.ordr[1], p.ordr[2], p.alpha, p.gdim[0], p.gdim[1], p.gdim[2], (int)fp32, 0);
@@ -488,6 +502,7 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
       tables.push_back(tw);
       S.tw[d] = tw->ptr;
     }
+    }
     // FFT tiles: about 32 KB of LDS per workgroup
     const int budget = 16384;  // bytes per LDS buffer (the FFT passes use two)
     S.px_tile = std::max(1, std::min((K2 * K3 + 1) / 2, budget / (K1 * csz)));
@@ -524,7 +539,7 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   S.pos3s = nullptr;
   S.nb1 = S.nb2 = nullptr;
   S.ccap1 = S.ccap2 = S.ccap3 = S.nbcap1 = S.nbcap2 = 0;
-  if (null_stream_done() != hipSuccess) return fail(MD_EHIP, "set-up copies");
+  if (!dry && null_stream_done() != hipSuccess) return fail(MD_EHIP, "set-up copies");
   return MD_OK;
 }
 
@@ -701,7 +716,7 @@ int Sys::size_lists(const double *pos) {
   int np = 1;
   while (np < h.ccap2) np <<= 1;
   lds_sort = h.real_verlet ? (size_t)np * sizeof(int) : 0;
-  if (null_stream_done() != hipSuccess) return fail(MD_EHIP, "list buffers");
+  if (!dry && null_stream_done() != hipSuccess) return fail(MD_EHIP, "list buffers");
   return MD_OK;
 }
 
@@ -2000,6 +2015,16 @@ extern "C" int md_batch_download(md_engine *e, int32_t slot, double *pos, double
   auto it = e->slots.find(slot);
   if (it == e->slots.end()) return e->fail(MD_EINVAL, "no such slot");
   return download_sys(e, it->second, pos, vel, acc, nullptr);
+}
+
+extern "C" int md_replica_bytes(const md_params *p, const double *pos, int64_t *bytes) {
+  if (!p || !pos || !bytes) return MD_EINVAL;
+  Sys s;
+  s.dry = true;  // every allocation counted, none made; no device call
+  int rc = s.setup(p, p->precision == MD_FP32);
+  if (rc == MD_OK) rc = s.size_lists(pos);
+  *bytes = rc == MD_OK ? (int64_t)s.bytes : 0;
+  return rc;
 }
 
 extern "C" int md_batch_info(md_engine *e, int32_t slot, int64_t *bytes, int32_t *nrep) {

@@ -17,7 +17,8 @@ import time
 
 import numpy as np
 
-from .abi import MD_FP32, MD_FP64, MD_NOISE_PHILOX, Batch, Engine, initialize_position, initialize_velocity
+from .abi import (MD_FP32, MD_FP64, MD_NOISE_PHILOX, Batch, Engine, initialize_position, initialize_velocity,
+                  replica_bytes)
 from .configs import replica
 
 
@@ -26,6 +27,7 @@ class Request:
         self.rid, self.arrival, self.params, self.nsteps = rid, arrival, params, nsteps
         self.pos, self.vel, self.desc = pos, vel, desc
         self.slot = None
+        self.need = None
         self.start_step = 1
         self.done_steps = 0
         self.t_admit = self.t_done = None
@@ -60,13 +62,20 @@ def make_stream(n, nsteps, rate, seed, precision=MD_FP32, max_side=8, preheat=No
     return out
 
 
-def footprint(p):
-    """Device bytes of one replica, from its sizes (state, lists, mesh)."""
-    K = p.ng[0] * p.ng[1] * p.ng[2]
-    return int(p.NN * 900 + p.Nq * 2500 + K * 40 + (1 << 20))
+def footprint(req):
+    """Device bytes the replica will hold: md_replica_bytes, the engine's own
+    sizing of its buffers for these parameters and this configuration,
+    computed on the host."""
+    if getattr(req, "need", None) is None:
+        req.need = replica_bytes(req.params, req.pos)
+    return req.need
 
 
 class Server:
+    """Continuous batching: a request is admitted at a cycle boundary when a
+    slot is free and its device bytes fit the HBM budget beside those of the
+    running replicas; otherwise it waits in arrival order."""
+
     def __init__(self, precision=MD_FP32, multistep=10, hbm_budget=64 << 30, max_slots=256, chunk_cycles=10):
         self.batch = Batch(precision, multistep)
         self.multistep = multistep
@@ -75,16 +84,22 @@ class Server:
         self.chunk = chunk_cycles
         self.active = []
         self.used = 0
+        self.peak_used = 0
+        self.peak_active = 0
+        self.queued = 0  # admissions that had to wait for memory or a slot
 
     def admit(self, req, now):
-        need = footprint(req.params)
+        need = footprint(req)
+        if need > self.budget:
+            raise ValueError(f"request {req.rid} needs {need} device bytes, more than the budget {self.budget}")
         if len(self.active) >= self.max_slots or self.used + need > self.budget:
             return False
         req.slot = self.batch.add(req.params, req.pos, req.vel)
         req.t_admit = now
-        req.need = need
         self.used += need
+        self.peak_used = max(self.peak_used, self.used)
         self.active.append(req)
+        self.peak_active = max(self.peak_active, len(self.active))
         return True
 
     def serve(self, requests, clock=time.perf_counter):
@@ -97,6 +112,9 @@ class Server:
             now = clock() - t0
             while pending and pending[0].arrival <= now:
                 if not self.admit(pending[0], now):
+                    if not getattr(pending[0], "waited", False):
+                        pending[0].waited = True
+                        self.queued += 1
                     break
                 pending.pop(0)
             if not self.active:

@@ -203,3 +203,40 @@ def test_heights_match_definitions(name):
     b.close()
     for k, v in ref.items():
         assert abs(hb[k] - v) <= 1e-12 * max(1.0, abs(v)), (k, hb[k], v)
+
+
+def test_replica_bytes_match_allocation():
+    """md_replica_bytes (host-side sizing) equals the bytes md_batch_add
+    allocates, for linear and star replicas and a small box that walks the
+    reference-form real-space list."""
+    print("[serve] replica bytes", flush=True)
+    from md_brushes_amd import replica_bytes
+    b = Batch(MD_FP32)
+    for args in ((2, 30, 1, 3, 0.5, 61), (5, 70, 3, 1, 0.2, 62), (8, 100, 1, 3, 0.5, 63)):
+        p, pos, vel = _req(*args)
+        want = replica_bytes(p, pos)
+        sl = b.add(p, pos, vel)
+        got, _ = b.info(sl)
+        assert got == want, (args, got, want)
+    b.close()
+
+
+def test_admission_queues_on_device_bytes():
+    """With an HBM budget that holds only a few replicas, requests that
+    arrive together wait for memory (admitted as others finish), the bytes
+    in use never exceed the budget, and every result equals the request
+    run alone."""
+    print("[serve] tiny HBM budget", flush=True)
+    from md_brushes_amd.serve import footprint
+    reqs = make_stream(10, 100, rate=1e6, seed=9, max_side=4)
+    need = sorted(footprint(r) for r in reqs)
+    budget = need[-1] + need[-2] + need[-3]
+    srv = Server(hbm_budget=budget, chunk_cycles=2)
+    done, wall = srv.serve(reqs)
+    assert len(done) == 10 and all(r.result["status"] == 0 for r in done)
+    assert srv.queued > 0 and srv.peak_used <= budget and srv.peak_active < 10
+    assert max(r.t_admit for r in done) > min(r.t_done for r in done)  # admitted after a completion
+    for r in done:
+        h, prof, _, _ = run_alone(r)
+        assert h == r.result["height"]
+        assert np.array_equal(prof, r.result["profile"])

@@ -173,3 +173,17 @@ def test_initial_configuration_is_seeded():
     p = config("small")
     assert np.array_equal(initialize_position(p, 3), initialize_position(p, 3))
     assert not np.array_equal(initialize_position(p, 3), initialize_position(p, 4))
+
+
+def test_replica_bytes_without_device():
+    """md_replica_bytes sizes a replica on the host (no device call): more
+    particles take more bytes, FP32 state less than FP64."""
+    from md_brushes_amd import MD_FP32, MD_FP64, initialize_position, replica, replica_bytes
+    sizes = {}
+    for side in (2, 4):
+        for prec in (MD_FP32, MD_FP64):
+            p = replica(side, 40, 1, 3, 0.5, 7, prec)
+            sizes[side, prec] = replica_bytes(p, initialize_position(p, 7))
+    assert all(v > 0 for v in sizes.values())
+    assert sizes[4, MD_FP32] > sizes[2, MD_FP32] and sizes[4, MD_FP64] > sizes[2, MD_FP64]
+    assert sizes[4, MD_FP32] < sizes[4, MD_FP64]

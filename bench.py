@@ -140,6 +140,7 @@ def time_engine(prec, state, steps, warmup, seed):
 
 
 COPY_BYTES = 256 << 20  # calibration copy for the HBM byte counters
+SPOT_EVERY = 8  # accuracy spot check of every 8th served request
 
 
 def pmc_workload(seed):
@@ -272,7 +273,7 @@ def main():
     ap.add_argument("--serve", type=int, default=256, help="requests in the serving leg (0: skip)")
     ap.add_argument("--serve-steps", type=int, default=10000)
     ap.add_argument("--serve-rate", type=float, default=128.0)
-    ap.add_argument("--serve-seq", type=int, default=16)
+    ap.add_argument("--serve-seq", type=int, default=256, help="requests in the sequential FP64 comparator")
     ap.add_argument("--pmc-workload", action="store_true",
                     help="short run for hardware-counter passes: a few hundred steps of the default brush "
                          "and a plain device copy of known size (counter calibration); prints no JSON")
@@ -347,15 +348,27 @@ def main():
         from md_brushes_amd.serve import Server, make_stream, sequential_fp64, summarize
 
         reqs = make_stream(args.serve, args.serve_steps, rate=args.serve_rate, seed=5)
-        done, wall = Server().serve(reqs)
+        srv = Server(spot_check_every=SPOT_EVERY)
+        done, wall = srv.serve(reqs)
         sv = summarize(done, wall)
+        sv["admission"] = {"hbm_budget_bytes": srv.budget, "peak_bytes": srv.peak_used,
+                           "peak_replicas": srv.peak_active, "queued": srv.queued}
         nseq = min(len(reqs), args.serve_seq)
-        sps_seq, _ = sequential_fp64(reqs[:nseq])
+        sps_seq, _, rms64 = sequential_fp64(reqs[:nseq], spot_check_every=SPOT_EVERY)
         sv["sequential_fp64_steps_per_s"] = round(sps_seq, 1)
         sv["speedup_vs_sequential"] = round(sv["aggregate_steps_per_s"] / sps_seq, 2)
+        rms32 = [r.result["rmsf"] for r in done if r.result and "rmsf" in r.result]
+        # error_analysis (rmsf of the Coulomb force against standard Ewald) of
+        # every SPOT_EVERY-th request at its end: served FP32 beside the FP64 runs
+        sv["accuracy_rmsf"] = {"sample": len(rms32),
+                               "served_fp32_median": float(np.median(rms32)) if rms32 else None,
+                               "served_fp32_max": float(np.max(rms32)) if rms32 else None,
+                               "sequential_fp64_median": float(np.median(rms64)) if rms64 else None,
+                               "sequential_fp64_max": float(np.max(rms64)) if rms64 else None}
         sv["stream"] = (f"{args.serve} requests, Poisson rate {args.serve_rate}/s, {args.serve_steps} steps "
                         f"each, chains 2x2..8x8, N 20..100, linear/3-/4-arm stars, salt valence 1/3, "
-                        f"Ez in [0,1], FP32 forces; sequential comparator on the first {nseq}")
+                        f"Ez in [0,1], FP32 forces; sequential comparator on {nseq}; admission on "
+                        f"md_replica_bytes against the budget")
         out["serving"] = sv
         out["replica_latency_p50_s"] = sv["latency_p50_s"]
         out["replica_latency_p99_s"] = sv["latency_p99_s"]

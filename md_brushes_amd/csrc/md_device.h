@@ -106,6 +106,7 @@ struct Rep {
   int *charge;                     // Nq: particle index of charge m
   int *bstart;                     // NN+1 CSR of bonded partners
   int *bpart;
+  int4 *bond4;                     // each particle's first four partners, bond order (-1: none)
   // LJ cells: two count buffers used in turn (gen1 & 1 selects the one a
   // build fills; it clears the other at its particles' previous cells),
   // members as binned, each particle's cell (-1: none)

@@ -156,6 +156,7 @@ struct Sys {
   // dry run: allocations are only counted (no device call), for the bytes a
   // replica would take before it is admitted
   bool dry = false;
+  int step_dev = -1;  // the control block's step counter when known on the host (md_run), else -1
   int en_blocks[3] = {0, 0, 0};  // energy kernels' blocks: particles, charges, mesh
   std::string err;
 
@@ -343,6 +344,7 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   S.charge = dalloc<int>(std::max(Nq, 1));
   S.bstart = dalloc<int>(nn + 1);
   S.bpart = dalloc<int>(bpart.size());
+  S.bond4 = dalloc<int4>(nn);
   S.ctrl = dalloc<Ctrl>(1);
   d_eta = dalloc<double>(3 * nn);
   d_out1 = dalloc<double>(3 * nn);
@@ -359,7 +361,7 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   S.Nma = p.Nma;
   S.Ngl = p.Ngl;
   S.Nml = p.Nml;
-  if (!cc || !S.flags || !S.charge || !S.bstart || !S.bpart || !S.ctrl || !d_eta || !d_out1 ||
+  if (!cc || !S.flags || !S.charge || !S.bstart || !S.bpart || !S.bond4 || !S.ctrl || !d_eta || !d_out1 ||
       !d_out2 || !d_hist || !d_hsum || !d_mol || !d_epart)
     return fail(MD_ENOMEM, "topology arrays");
   hipError_t cerr = dry ? hipSuccess : hipMemcpy(S.flags, flags.data(), nn * sizeof(int), hipMemcpyHostToDevice);
@@ -369,6 +371,15 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
     cerr = hipMemcpy(S.bstart, bstart.data(), (nn + 1) * sizeof(int), hipMemcpyHostToDevice);
   if (cerr == hipSuccess && !dry)
     cerr = hipMemcpy(S.bpart, bpart.data(), bpart.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (cerr == hipSuccess && !dry) {
+    std::vector<int4> b4(nn);
+    for (int i = 0; i < NN; i++) {
+      int v[4] = {-1, -1, -1, -1};
+      for (int k = 0; k < 4 && bstart[i] + k < bstart[i + 1]; k++) v[k] = bpart[bstart[i] + k];
+      b4[i] = make_int4(v[0], v[1], v[2], v[3]);
+    }
+    cerr = hipMemcpy(S.bond4, b4.data(), nn * sizeof(int4), hipMemcpyHostToDevice);
+  }
   if (cerr != hipSuccess) return fail(MD_EHIP, hip_msg(cerr, "topology upload"));
   S.cnt1 = dalloc<int>(2 * (size_t)ncell1);  // two count buffers, used in turn
   S.cell1_of = dalloc<int>(NN);
@@ -433,7 +444,8 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
     if (!S.tcnt) return fail(MD_ENOMEM, "spreading tiles");
     // x and y passes fused per z-plane when both planes fit in LDS
     // two regions of max(K2/2 * K1, Kh * K2) points (the FFT passes alternate)
-    plane_bytes = 2 * mdk::plane_region(K1, K2, S.Kh) * csz;
+    // (plus the x and y twiddle tables, staged in LDS)
+    plane_bytes = (2 * mdk::plane_region(K1, K2, S.Kh) + K1 + K2) * csz;
     lds_plane = plane_bytes;
     // MD_FFT_PLANES=0 keeps the separate x and y passes (for comparison)
     const char *fp = std::getenv("MD_FFT_PLANES");
@@ -515,9 +527,9 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
     S.freal = dalloc<char>(3 * (size_t)Nq * (fp32 ? 4 : 8));
     S.part = dalloc<double>(4 * (size_t)S.nqblk);
     if (!S.fsp || !S.part || !S.freal) return fail(MD_ENOMEM, "spme scratch");
-    lds_x = 2 * (size_t)S.px_tile * K1 * csz;
-    lds_y = 2 * (size_t)S.py_tile * mdk::plane_line_stride(K2) * csz;
-    lds_z = 2 * (size_t)S.pz_tile * mdk::zline_stride(K3) * csz;
+    lds_x = (2 * (size_t)S.px_tile * K1 + K1) * csz;
+    lds_y = (2 * (size_t)S.py_tile * mdk::plane_line_stride(K2) + K2) * csz;
+    lds_z = (2 * (size_t)S.pz_tile * mdk::zline_stride(K3) + K3) * csz;
   }
   nb[0] = cdiv(NN, mdk::kBlock);
   nb[1] = coulomb ? cdiv(Nq, mdk::kBlock) : 0;
@@ -749,6 +761,7 @@ struct md_engine {
   double *vpart = nullptr;
   size_t vpart_cap = 0;
   int *h_steps = nullptr;          // pinned, one per replica
+  Ctrl *h_ctrl = nullptr;          // pinned, one per replica: control blocks read back
   int steps_cap = 0;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};  // cycle graphs by the buffer of their first step
   hipGraph_t graph[2] = {nullptr, nullptr};
@@ -771,6 +784,7 @@ struct md_engine {
     if (d_lq) (void)hipFree(d_lq);
     if (vpart) (void)hipFree(vpart);
     if (h_steps) (void)hipHostFree(h_steps);
+    if (h_ctrl) (void)hipHostFree(h_ctrl);
     if (st2) (void)hipStreamSynchronize(st2);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
@@ -879,7 +893,7 @@ int md_engine::rebuild() {
     const int want_pz = zl ? std::atoi(zl) : tune_pz;
     if (want_pz > 0) x->h.pz_tile = std::max(1, std::min(want_pz, 16384 / (x->h.K[2] * csz)));
     x->nb[5] = cdiv((long)x->h.K[1] * x->h.Kh, x->h.pz_tile);
-    x->lds_z = 2 * (size_t)x->h.pz_tile * mdk::zline_stride(x->h.K[2]) * csz;
+    x->lds_z = (2 * (size_t)x->h.pz_tile * mdk::zline_stride(x->h.K[2]) + x->h.K[2]) * csz;
     // fused planes where they fit, unless tuned off
     Rep &S = x->h;
     const char *fp = std::getenv("MD_FFT_PLANES");  // =0 keeps separate x and y passes
@@ -930,8 +944,11 @@ int md_engine::rebuild() {
   }
   if (R > steps_cap) {
     if (h_steps) (void)hipHostFree(h_steps);
+    if (h_ctrl) (void)hipHostFree(h_ctrl);
+    h_ctrl = nullptr;
     steps_cap = std::max(2 * R, 8);
-    if (hipHostMalloc((void **)&h_steps, steps_cap * sizeof(int)) != hipSuccess)
+    if (hipHostMalloc((void **)&h_steps, steps_cap * sizeof(int)) != hipSuccess ||
+        hipHostMalloc((void **)&h_ctrl, steps_cap * sizeof(Ctrl)) != hipSuccess)
       return fail(MD_ENOMEM, "pinned");
   }
   // rebuild queues, one set per position buffer: 2 header words + rep/start
@@ -1064,14 +1081,26 @@ int md_engine::check_sys(const Sys *s) {
 }
 
 int md_engine::check_all() {
+  // every control block into pinned memory behind the queued work, one wait
+  const bool pinned = h_ctrl && (int)sys.size() <= steps_cap && !dirty;
+  if (pinned) {
+    for (size_t k = 0; k < sys.size(); k++) {
+      hipError_t e = hipMemcpyAsync(&h_ctrl[k], sys[k]->h.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st);
+      if (e != hipSuccess) return hip_fail(e, "control block");
+    }
+  }
   int rc = sync();
   if (rc) return rc;
-  for (Sys *s : sys) {
+  for (size_t k = 0; k < sys.size(); k++) {
     Ctrl c;
-    hipError_t e = hipMemcpy(&c, s->h.ctrl, sizeof c, hipMemcpyDeviceToHost);
-    if (e != hipSuccess) return hip_fail(e, "control block");
+    if (pinned) {
+      c = h_ctrl[k];
+    } else {
+      hipError_t e = hipMemcpy(&c, sys[k]->h.ctrl, sizeof c, hipMemcpyDeviceToHost);
+      if (e != hipSuccess) return hip_fail(e, "control block");
+    }
     std::string msg;
-    rc = check_one(s, c, msg);
+    rc = check_one(sys[k], c, msg);
     if (rc) return fail(rc, msg);
   }
   return MD_OK;
@@ -1188,8 +1217,9 @@ int md_engine::run_cycles(int ncycles) {
   const bool eager = ng && ng[0] == '1';
   const int q0 = cur;
   if (!eager) {
+    // both starting buffers: a run of even cycles starts on q0, but it ends
+    // with the other buffer current, where the next run starts
     for (int q = 0; q < 2; q++) {
-      if (q != q0 && multistep % 2 == 0) continue;  // even cycles all start on q0
       int rc = build_graph(q);
       if (rc) return rc;
     }
@@ -1252,6 +1282,7 @@ static int upload_sys(md_engine *e, Sys *s, const double *pos, const double *vel
   if (err == hipSuccess) err = null_stream_done();
   if (err != hipSuccess) return e->hip_fail(err, "upload");
   s->uploaded = true;
+  s->step_dev = -1;
   return MD_OK;
 }
 
@@ -1484,14 +1515,20 @@ extern "C" int md_run(md_engine *e, int32_t step0, int32_t nsteps, int32_t prehe
     if (rc) return rc;
   }
   if (end - step >= ms) {
-    e->h_steps[0] = step;  // the replayed steps read their numbers here
-    hipError_t err = hipMemcpyAsync(&s->h.ctrl->step, e->h_steps, sizeof(int),
-                                    hipMemcpyHostToDevice, e->st);
-    if (err != hipSuccess) return e->hip_fail(err, "step upload");
+    // the replayed steps read their numbers from the control block, which
+    // the cycles advance: uploaded unless it already holds this step
+    if (s->step_dev != step) {
+      e->h_steps[0] = step;
+      hipError_t err = hipMemcpyAsync(&s->h.ctrl->step, e->h_steps, sizeof(int),
+                                      hipMemcpyHostToDevice, e->st);
+      if (err != hipSuccess) return e->hip_fail(err, "step upload");
+    }
     const int nc = (end - step) / ms;
+    s->step_dev = -1;
     int rc = e->run_cycles(nc);
     if (rc) return rc;
     step += nc * ms;
+    s->step_dev = step;
   }
   while (step < end) {
     int rc = one(step++);
@@ -2364,6 +2401,7 @@ extern "C" int md_state_load(md_engine *e, const void *buf, int64_t len) {
     if (err == hipSuccess) err = put(s->h.cnt2[1], (size_t)s->ncell2 * 4);
   }
   if (err == hipSuccess) err = hipMemcpy(s->h.ctrl, in, sizeof(Ctrl), hipMemcpyHostToDevice);
+  s->step_dev = -1;
   if (err == hipSuccess) err = null_stream_done();
   if (err != hipSuccess) return e->hip_fail(err, "state load");
   return e->rebuild();

@@ -382,8 +382,29 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
       }
     };
     // slots g, g + G, ... in order; four at a time, their index and position
-    // loads issued together (the same sums as one at a time)
-    for (int k = g; k < n; k += 4 * G) {
+    // loads issued together (the same sums as one at a time).  The first
+    // four slots' indices are loaded beside the list length (slots past the
+    // length are within the list's capacity and left unused), so a short
+    // list costs two dependent loads: indices, then positions.
+    {
+      int jp[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int k = g + u * G;
+        jp[u] = lv && k < S.nbcap1 ? nb[(size_t)k * S.NN] : 0;
+      }
+      int j[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) j[u] = g + u * G < n ? jp[u] : -1;
+      V pj[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (j[u] >= 0) pj[u] = pv[j[u]];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (j[u] >= 0) pair(pj[u]);
+    }
+    for (int k = g + 4 * G; k < n; k += 4 * G) {
       int j[4];
 #pragma unroll
       for (int u = 0; u < 4; u++) j[u] = k + u * G < n ? nb[(size_t)(k + u * G) * S.NN] : -1;
@@ -423,15 +444,27 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
     }
   } else if (w == 5) {  // wall and FENE over the bonded partners of i
     if (live) {
+      // the first four partners (bond order) come with the position; a
+      // fifth and more (branch points of stars with five or more arms) from
+      // the CSR lists
       const double4 p = S.pos[i];
-      const int b0 = S.bstart[i], b1 = S.bstart[i + 1];
+      const int4 b4 = S.bond4[i];
       const R R02 = (R)S.R0_2, kR = (R)S.kR02;
       R fx = 0, fy = 0, fz = 0;
+      const int b1 = b4.w >= 0 ? S.bstart[i + 1] : 0;  // past the fourth: CSR (else no load)
       // bonds in order, four at a time: the partner and position loads issued together
-      for (int b = b0; b < b1; b += 4) {
+      for (int b = 0; b == 0 || (b1 > 0 && S.bstart[i] + b < b1); b += 4) {
         int nn[4];
+        if (b == 0) {
+          nn[0] = b4.x;
+          nn[1] = b4.y;
+          nn[2] = b4.z;
+          nn[3] = b4.w;
+        } else {
+          const int b0 = S.bstart[i] + b;
 #pragma unroll
-        for (int u = 0; u < 4; u++) nn[u] = b + u < b1 ? S.bpart[b + u] : -1;
+          for (int u = 0; u < 4; u++) nn[u] = b0 + u < b1 ? S.bpart[b0 + u] : -1;
+        }
         double4 pj[4];
 #pragma unroll
         for (int u = 0; u < 4; u++)
@@ -1433,12 +1466,20 @@ __device__ C *fft_lines_mixed(C *a, C *b, int n, int nl, const C *__restrict__ t
   return src;
 }
 
-// The lines of mesh axis d: power-of-two lengths take fft_lines.
+// The lines of mesh axis d: power-of-two lengths take fft_lines.  tw is
+// the axis's twiddle table staged in LDS (stage_twiddles).
 template <typename C, int SGN>
-__device__ __forceinline__ C *fft_axis(const Rep &S, int d, C *a, C *b, int nl, int ls = 0) {
-  const C *tw = (const C *)S.tw[d];
+__device__ __forceinline__ C *fft_axis(const Rep &S, int d, const C *tw, C *a, C *b, int nl, int ls = 0) {
   if (S.pw2[d]) return fft_lines<C, SGN>(a, b, S.lK[d], nl, tw, ls);
   return fft_lines_mixed<C, SGN>(a, b, S.K[d], nl, tw, ls, S.fac[d], S.nfac[d]);
+}
+
+// Copies axis d's twiddle table (K_d complex) into LDS; the kernel's barrier
+// after its data loads covers it.
+template <typename C>
+__device__ __forceinline__ void stage_twiddles(const Rep &S, int d, C *dst) {
+  const C *tw = (const C *)S.tw[d];
+  for (int k = threadIdx.x; k < S.K[d]; k += blockDim.x) dst[k] = tw[k];
 }
 
 // Forward x pass: pairs of real mesh rows as one complex FFT of length K1,
@@ -1452,10 +1493,11 @@ __global__ __launch_bounds__(kBlock) void k_fftx_fwd(const Rep *__restrict__ rep
   const int K1 = S.K[0], Kh = S.Kh;
   const int nrow = S.K[1] * S.K[2], npairs = (nrow + 1) / 2;  // an odd count leaves a row alone
   const int P = S.px_tile;
-  C *ba = (C *)smem, *bb = ba + (size_t)P * K1;
+  C *ba = (C *)smem, *bb = ba + (size_t)P * K1, *twx = bb + (size_t)P * K1;
   const int pair0 = (blockIdx.x - wm.blk[r]) * P;
   const int nl = min(P, npairs - pair0);
   const R *q = (const R *)S.meshr;
+  stage_twiddles(S, 0, twx);
   for (int t = threadIdx.x; t < nl * K1; t += blockDim.x) {
     const int l = t / K1, x = t - l * K1;
     const size_t row = 2 * (size_t)(pair0 + l);
@@ -1465,7 +1507,7 @@ __global__ __launch_bounds__(kBlock) void k_fftx_fwd(const Rep *__restrict__ rep
     ba[l * K1 + x] = v;
   }
   __syncthreads();
-  const C *buf = fft_axis<C, -1>(S, 0, ba, bb, nl);
+  const C *buf = fft_axis<C, -1>(S, 0, twx, ba, bb, nl);
   C *spec = (C *)S.spec;
   for (int t = threadIdx.x; t < nl * Kh; t += blockDim.x) {
     const int l = t / Kh, k = t - l * Kh;
@@ -1492,10 +1534,11 @@ __global__ __launch_bounds__(kBlock) void k_fftx_inv(const Rep *__restrict__ rep
   const int K1 = S.K[0], Kh = S.Kh;
   const int nrow = S.K[1] * S.K[2], npairs = (nrow + 1) / 2;
   const int P = S.px_tile;
-  C *ba = (C *)smem, *bb = ba + (size_t)P * K1;
+  C *ba = (C *)smem, *bb = ba + (size_t)P * K1, *twx = bb + (size_t)P * K1;
   const int pair0 = (blockIdx.x - wm.blk[r]) * P;
   const int nl = min(P, npairs - pair0);
   const C *spec = (const C *)S.spec;
+  stage_twiddles(S, 0, twx);
   for (int t = threadIdx.x; t < nl * K1; t += blockDim.x) {
     const int l = t / K1, k = t - l * K1;
     const size_t row = 2 * (size_t)(pair0 + l);
@@ -1516,7 +1559,7 @@ __global__ __launch_bounds__(kBlock) void k_fftx_inv(const Rep *__restrict__ rep
     ba[l * K1 + k] = z;
   }
   __syncthreads();
-  const C *buf = fft_axis<C, +1>(S, 0, ba, bb, nl);
+  const C *buf = fft_axis<C, +1>(S, 0, twx, ba, bb, nl);
   R *u = (R *)S.meshu;
   for (int t = threadIdx.x; t < nl * K1; t += blockDim.x) {
     const int l = t / K1, x = t - l * K1;
@@ -1536,7 +1579,8 @@ __global__ __launch_bounds__(kBlock) void k_ffty(const Rep *__restrict__ reps, W
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
   const int K2 = S.K[1], Kh = S.Kh, T = S.py_tile, KP = plane_line_stride(K2);
-  C *ba = (C *)smem, *bb = ba + (size_t)T * KP;
+  C *ba = (C *)smem, *bb = ba + (size_t)T * KP, *twy = bb + (size_t)T * KP;
+  stage_twiddles(S, 1, twy);
   const int tiles = (Kh + T - 1) / T;
   const int b = blockIdx.x - wm.blk[r];
   const int z = b / tiles, x0 = (b - z * tiles) * T;
@@ -1547,7 +1591,7 @@ __global__ __launch_bounds__(kBlock) void k_ffty(const Rep *__restrict__ reps, W
     ba[l * KP + y] = spec[(size_t)y * Kh + l];
   }
   __syncthreads();
-  const C *buf = fft_axis<C, SGN>(S, 1, ba, bb, nl, KP);
+  const C *buf = fft_axis<C, SGN>(S, 1, twy, ba, bb, nl, KP);
   for (int t = threadIdx.x; t < nl * K2; t += blockDim.x) {
     const int y = t / nl, l = t - y * nl;
     spec[(size_t)y * Kh + l] = buf[l * KP + y];
@@ -1568,6 +1612,9 @@ __global__ __launch_bounds__(kPlaneBlock) void k_plane_fwd(const Rep *__restrict
   const int K1 = S.K[0], K2 = S.K[1], Kh = S.Kh, H = (K2 + 1) / 2;  // row pairs (odd K2: one alone)
   const int KP = plane_line_stride(K2);
   C *ra = (C *)smem, *rb = ra + plane_region(K1, K2, Kh);
+  C *twx = rb + plane_region(K1, K2, Kh), *twy = twx + K1;
+  stage_twiddles(S, 0, twx);
+  stage_twiddles(S, 1, twy);
   const R *q = (const R *)S.meshr + (size_t)z * K2 * K1;
   for (int t = threadIdx.x; t < H * K1; t += blockDim.x) {
     const int l = t / K1, x = t - l * K1;
@@ -1577,7 +1624,7 @@ __global__ __launch_bounds__(kPlaneBlock) void k_plane_fwd(const Rep *__restrict
     ra[l * K1 + x] = v;
   }
   __syncthreads();
-  const C *A = fft_axis<C, -1>(S, 0, ra, rb, H);
+  const C *A = fft_axis<C, -1>(S, 0, twx, ra, rb, H);
   C *P = A == ra ? rb : ra;
   for (int t = threadIdx.x; t < H * Kh; t += blockDim.x) {
     const int l = t / Kh, k = t - l * Kh;
@@ -1591,7 +1638,7 @@ __global__ __launch_bounds__(kPlaneBlock) void k_plane_fwd(const Rep *__restrict
     if (2 * l + 1 < K2) P[k * KP + 2 * l + 1] = b;
   }
   __syncthreads();
-  const C *Y = fft_axis<C, -1>(S, 1, P, (C *)A, Kh, KP);
+  const C *Y = fft_axis<C, -1>(S, 1, twy, P, (C *)A, Kh, KP);
   C *spec = (C *)S.spec + (size_t)z * K2 * Kh;
   for (int t = threadIdx.x; t < K2 * Kh; t += blockDim.x) {
     const int y = t / Kh, k = t - y * Kh;
@@ -1612,13 +1659,16 @@ __global__ __launch_bounds__(kPlaneBlock) void k_plane_inv(const Rep *__restrict
   const int K1 = S.K[0], K2 = S.K[1], Kh = S.Kh, H = (K2 + 1) / 2;
   const int KP = plane_line_stride(K2);
   C *ra = (C *)smem, *rb = ra + plane_region(K1, K2, Kh);
+  C *twx = rb + plane_region(K1, K2, Kh), *twy = twx + K1;
+  stage_twiddles(S, 0, twx);
+  stage_twiddles(S, 1, twy);
   const C *spec = (const C *)S.spec + (size_t)z * K2 * Kh;
   for (int t = threadIdx.x; t < K2 * Kh; t += blockDim.x) {
     const int y = t / Kh, k = t - y * Kh;
     ra[k * KP + y] = spec[t];
   }
   __syncthreads();
-  const C *P = fft_axis<C, +1>(S, 1, ra, rb, Kh, KP);
+  const C *P = fft_axis<C, +1>(S, 1, twy, ra, rb, Kh, KP);
   C *A = P == ra ? rb : ra;
   for (int t = threadIdx.x; t < H * K1; t += blockDim.x) {
     const int l = t / K1, k = t - l * K1;
@@ -1639,7 +1689,7 @@ __global__ __launch_bounds__(kPlaneBlock) void k_plane_inv(const Rep *__restrict
     A[l * K1 + k] = v;
   }
   __syncthreads();
-  const C *X = fft_axis<C, +1>(S, 0, A, (C *)P, H);
+  const C *X = fft_axis<C, +1>(S, 0, twx, A, (C *)P, H);
   R *u = (R *)S.meshu + (size_t)z * K2 * K1;
   for (int t = threadIdx.x; t < H * K1; t += blockDim.x) {
     const int l = t / K1, x = t - l * K1;
@@ -1659,7 +1709,8 @@ __global__ __launch_bounds__(kPlaneBlock) void k_fftz_conv(const Rep *__restrict
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
   const int K3 = S.K[2], T = S.pz_tile, KS = zline_stride(K3);
-  C *ba = (C *)smem, *bb = ba + (size_t)T * KS;
+  C *ba = (C *)smem, *bb = ba + (size_t)T * KS, *twz = bb + (size_t)T * KS;
+  stage_twiddles(S, 2, twz);
   const int f0 = (blockIdx.x - wm.blk[r]) * T;
   const size_t zs = (size_t)S.K[1] * S.Kh;
   const int nl = min(T, (int)zs - f0);
@@ -1670,7 +1721,7 @@ __global__ __launch_bounds__(kPlaneBlock) void k_fftz_conv(const Rep *__restrict
     ba[l * KS + z] = spec[z * zs + l];
   }
   __syncthreads();
-  C *buf = fft_axis<C, -1>(S, 2, ba, bb, nl, KS);
+  C *buf = fft_axis<C, -1>(S, 2, twz, ba, bb, nl, KS);
   for (int t = threadIdx.x; t < nl * K3; t += blockDim.x) {
     const int z = t / nl, l = t - z * nl;
     const R g = bc[z * zs + l];
@@ -1680,7 +1731,7 @@ __global__ __launch_bounds__(kPlaneBlock) void k_fftz_conv(const Rep *__restrict
     buf[l * KS + z] = v;
   }
   __syncthreads();
-  const C *out = fft_axis<C, +1>(S, 2, buf, buf == ba ? bb : ba, nl, KS);
+  const C *out = fft_axis<C, +1>(S, 2, twz, buf, buf == ba ? bb : ba, nl, KS);
   for (int t = threadIdx.x; t < nl * K3; t += blockDim.x) {
     const int z = t / nl, l = t - z * nl;
     spec[z * zs + l] = out[l * KS + z];

@@ -76,8 +76,12 @@ class Server:
     slot is free and its device bytes fit the HBM budget beside those of the
     running replicas; otherwise it waits in arrival order."""
 
-    def __init__(self, precision=MD_FP32, multistep=10, hbm_budget=64 << 30, max_slots=256, chunk_cycles=10):
+    def __init__(self, precision=MD_FP32, multistep=10, hbm_budget=64 << 30, max_slots=256, chunk_cycles=10,
+                 spot_check_every=0):
         self.batch = Batch(precision, multistep)
+        # accuracy spot check (error_analysis on the device) of every n-th
+        # request as it finishes: its rmsf against standard Ewald
+        self.spot_every = spot_check_every
         self.multistep = multistep
         self.budget = hbm_budget
         self.max_slots = max_slots
@@ -133,6 +137,8 @@ class Server:
                 if rc == 0:
                     h, prof = self.batch.observe(r.slot)
                     r.result = {"height": h, "profile": prof, "status": 0}
+                    if self.spot_every and r.rid % self.spot_every == 0:
+                        r.result["rmsf"] = self.batch.error_analysis(r.slot)[0]["rmsf"]
                 else:
                     r.result = {"status": rc, "message": msg}
                 self.batch.remove(r.slot)
@@ -166,10 +172,13 @@ def run_alone(req, precision=MD_FP32, multistep=10):
     return h, prof, pos, vel
 
 
-def sequential_fp64(requests, multistep=10):
+def sequential_fp64(requests, multistep=10, spot_check_every=0):
     """Comparator: each request alone on the plain FP64 path, one after the
-    other; returns total steps / total seconds."""
+    other; returns total steps / total seconds, the total seconds, and the
+    rmsf of error_analysis at the end of every spot_check_every-th request
+    (outside the timing)."""
     total_steps, total_t = 0, 0.0
+    rmsf = []
     for r in requests:
         p = r.params.copy()
         p.precision = MD_FP64
@@ -182,10 +191,12 @@ def sequential_fp64(requests, multistep=10):
         if pre:
             e.run(1, pre, preheat=True)
         e.run(1 + pre, r.nsteps - pre)
-        e.close()
         total_t += time.perf_counter() - t0
         total_steps += r.nsteps
-    return total_steps / total_t, total_t
+        if spot_check_every and r.rid % spot_check_every == 0:
+            rmsf.append(e.error_analysis()[0]["rmsf"])
+        e.close()
+    return total_steps / total_t, total_t, rmsf
 
 
 def summarize(done, wall):
@@ -236,7 +247,7 @@ def main():
     done, wall = srv.serve(reqs)
     out = summarize(done, wall)
     n = len(reqs) if args.sequential < 0 else args.sequential
-    sps, tseq = sequential_fp64(reqs[:n])
+    sps, tseq, _ = sequential_fp64(reqs[:n])
     out["sequential_fp64_steps_per_s"] = round(sps, 1)
     out["speedup_vs_sequential"] = round(out["aggregate_steps_per_s"] / sps, 2)
     print(json.dumps(out))

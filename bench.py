@@ -6,8 +6,9 @@ SPME mesh 64x64x256 order 4, Coulomb every 10 steps, dt = 0.005; one step =
 new_position + update_verlet_list (main.f90:92-94) with Philox noise, all
 state resident in HBM.  Headline: FP32-force steps/s, reported beside the
 plain FP64 path and the FP32-vs-FP64 relative force RMS on the same
-configuration.  The CPU baseline is the FP64 oracle (a serial C restatement
-of the reference) timed on this host for a bounded sample.
+configuration.  The CPU baseline is the FP64 oracle (a C restatement of the
+reference, -O3) timed on this host for a bounded sample, on one thread and
+with OpenMP on the host's threads.
 """
 import argparse
 import json
@@ -236,14 +237,18 @@ def traffic_from_profile(kernel):
     return None if k is None else k.get("bytes_per_launch")
 
 
-def cpu_baseline(state, seed, budget_s):
-    """The FP64 oracle (serial C) on this host: whole multistep cycles until
-    about budget_s seconds have passed."""
+def cpu_baseline(state, seed, budget_s, threads=1):
+    """The FP64 oracle (a C restatement of the reference, -O3, OpenMP over
+    its particle, pair-list, mesh and FFT-line loops) on this host with the
+    given thread count: whole multistep cycles until about budget_s seconds
+    have passed."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import orc  # the oracle: baseline only
 
+    L = orc.fast_lib()
+    L.orc_set_threads(threads)
     p = config("default", MD_FP64, seed=seed)
-    o = orc.Oracle(p)
+    o = orc.Oracle(p, L)
     pos, vel, acc = state
     o.set_state(pos, vel, acc)
     o.build_lists()
@@ -260,6 +265,14 @@ def cpu_baseline(state, seed, budget_s):
     return steps / el, steps, el
 
 
+def host_threads():
+    """Threads the host offers this process (OMP_NUM_THREADS where set)."""
+    try:
+        return max(1, int(os.environ.get("OMP_NUM_THREADS", "")))
+    except ValueError:
+        return max(1, len(os.sched_getaffinity(0)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -267,7 +280,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--equil", type=int, default=2000)
     ap.add_argument("--seed", type=int, default=20261203)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--serve", type=int, default=256, help="requests in the serving leg (0: skip)")
@@ -388,11 +401,17 @@ def main():
         out["kernel_us_batch"] = {k: round(v * 1e3, 3) for k, v in msb.items()}
 
     if not args.no_cpu:
-        sps_cpu, n, el = cpu_baseline(state, args.seed, args.cpu_seconds)
+        sps_cpu, n, el = cpu_baseline(state, args.seed, args.cpu_seconds, 1)
         out["cpu_baseline"] = {"value": round(sps_cpu, 3), "unit": "steps/s", "cores": 1, "kind": "port",
-                               "sample": f"{n} steps of the same workload (FP64 oracle, serial C, -O2) "
-                                         f"in {el:.1f} s"}
+                               "sample": f"{n} steps of the same workload (FP64 oracle, C, -O3 "
+                                         f"-march=x86-64-v3, one thread) in {el:.1f} s"}
+        nt = host_threads()
+        sps_omp, n2, el2 = cpu_baseline(state, args.seed, args.cpu_seconds, nt)
+        out["cpu_baseline_omp"] = {"value": round(sps_omp, 3), "unit": "steps/s", "cores": nt, "kind": "port",
+                                   "sample": f"{n2} steps of the same workload (the same oracle, OpenMP "
+                                             f"over {nt} threads) in {el2:.1f} s"}
         out["speedup_vs_cpu"] = round(sps32 / sps_cpu, 1)
+        out["speedup_vs_cpu_omp"] = round(sps32 / sps_omp, 1)
     print(json.dumps(out), flush=True)
 
 

@@ -26,6 +26,44 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Built with -fopenmp (liborc_fast.so, the all-cores CPU baseline) the hot
+ * loops below run in parallel without changing any result: list loops are
+ * split at particle boundaries (each particle's sum keeps its order), the
+ * charge spreading is split by mesh planes (each point's sum keeps the charge
+ * order), and the FFT, gather, noise and integrator loops are independent
+ * per line or particle.  The checker build (liborc.so) is serial. */
+#ifdef _OPENMP
+#define ORC_NT omp_get_max_threads()
+#define ORC_TID omp_get_thread_num()
+#else
+#define ORC_NT 1
+#define ORC_TID 0
+#endif
+
+/* Thread t's share [b[t], b[t+1]) of a pair list grouped by its first index:
+ * equal shares moved forward to the next change of that index. */
+static void row_chunks(const int32_t *li, int64_t n, int nt, int64_t *b) {
+  b[0] = 0;
+  for (int t = 1; t < nt; t++) {
+    int64_t k = n * t / nt;
+    if (k < b[t - 1]) k = b[t - 1];
+    while (k > 0 && k < n && li[k] == li[k - 1]) k++;
+    b[t] = k;
+  }
+  b[nt] = n;
+}
+
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n > 0 ? n : 1);
+#else
+  (void)n;
+#endif
+}
 
 #define PI 3.141592653589793
 #define MAXO 12
@@ -380,23 +418,28 @@ static double complex *twiddles(int n, int sign) {
 static void fft3d_c(double complex *a, int n1, int n2, int n3, int sign) {
   int nmax = n1 > n2 ? n1 : n2;
   nmax = nmax > n3 ? nmax : n3;
+  double complex *w1 = twiddles(n1, sign), *w2 = twiddles(n2, sign), *w3 = twiddles(n3, sign);
+#pragma omp parallel
+  {
   double complex *line = malloc(sizeof(double complex) * nmax);
   double complex *out = malloc(sizeof(double complex) * nmax);
   /* scratch: n points per recursion level, at most 2n in all */
   double complex *tmp = malloc(sizeof(double complex) * 2 * nmax);
-  double complex *w1 = twiddles(n1, sign), *w2 = twiddles(n2, sign), *w3 = twiddles(n3, sign);
+#pragma omp for
   for (int k = 0; k < n3; k++)
     for (int j = 0; j < n2; j++) {
       double complex *r = a + ((size_t)k * n2 + j) * n1;
       fft_rec(r, 1, out, n1, n1, w1, tmp);
       memcpy(r, out, sizeof(double complex) * n1);
     }
+#pragma omp for
   for (int k = 0; k < n3; k++)
     for (int i = 0; i < n1; i++) {
       for (int j = 0; j < n2; j++) line[j] = a[((size_t)k * n2 + j) * n1 + i];
       fft_rec(line, 1, out, n2, n2, w2, tmp);
       for (int j = 0; j < n2; j++) a[((size_t)k * n2 + j) * n1 + i] = out[j];
     }
+#pragma omp for
   for (int j = 0; j < n2; j++)
     for (int i = 0; i < n1; i++) {
       for (int k = 0; k < n3; k++) line[k] = a[((size_t)k * n2 + j) * n1 + i];
@@ -406,6 +449,7 @@ static void fft3d_c(double complex *a, int n1, int n2, int n3, int sign) {
   free(line);
   free(out);
   free(tmp);
+  }
   free(w1);
   free(w2);
   free(w3);
@@ -555,20 +599,25 @@ static int64_t build_list(orc_sys *s, int n, const int32_t *idx, double rv,
     cl[m] = hoc[h];
     hoc[h] = m;
   }
-  int64_t k = 0;
+  /* two passes over the targets m: the hits of each (count), then the
+   * pairs at each target's offset (fill); targets in order, so the list is
+   * the reference's whether or not the passes run in parallel */
+  int64_t *off = calloc((size_t)n + 2, sizeof(int64_t));
   for (int pass = 0; pass < 2; pass++) {
     if (pass == 1) {
-      if (k > *cap) {
+      for (int m = 1; m <= n; m++) off[m + 1] += off[m];
+      if (off[n + 1] > *cap) {
         free(*li); free(*lj);
-        *cap = k + 1;
+        *cap = off[n + 1] + 1;
         *li = malloc(sizeof(int32_t) * *cap);
         *lj = malloc(sizeof(int32_t) * *cap);
       }
-      k = 0;
     }
+#pragma omp parallel for schedule(dynamic, 64)
     for (int m = 1; m <= n; m++) {
       int i = idx ? idx[m - 1] : m - 1;
       int a = ci[3 * m], b = ci[3 * m + 1], c = ci[3 * m + 2];
+      int64_t k = pass == 1 ? off[m] : 0;
       for (int l = -1; l <= 1; l++) {
         int pp = a + l >= nc1 ? a + l - nc1 : (a + l < 0 ? a + l + nc1 : a + l);
         for (int mm = -1; mm <= 1; mm++) {
@@ -593,8 +642,11 @@ static int64_t build_list(orc_sys *s, int n, const int32_t *idx, double rv,
           }
         }
       }
+      if (pass == 0) off[m + 1] = k;
     }
   }
+  int64_t k = off[n + 1];
+  free(off);
   free(hoc); free(cl); free(ci);
   return k;
 }
@@ -635,7 +687,11 @@ int orc_lj_force(orc_sys *s, double *f) {
   double rcl2 = p->rcl * p->rcl, rcl3 = p->rcl_s * p->rcl_s, rcl4 = p->rcl_sp * p->rcl_sp;
   double sigma2 = p->sigma_s * p->sigma_s, sigma3 = p->sigma_sp * p->sigma_sp;
   memset(f, 0, sizeof(double) * 3 * NN);
-  for (int64_t k = 0; k < s->npair1; k++) {
+  int64_t *rb = malloc(sizeof(int64_t) * (ORC_NT + 1));
+  row_chunks(s->l1i, s->npair1, ORC_NT, rb);
+#pragma omp parallel
+  {
+  for (int64_t k = rb[ORC_TID]; k < rb[ORC_TID + 1]; k++) {
     int i = s->l1i[k], j = s->l1j[k];
     double rij[3] = {X(i) - X(j), Y(i) - Y(j), Z(i) - Z(j)};
     int m = (int)(fabs(QC(i)) + fabs(QC(j)));
@@ -659,6 +715,9 @@ int orc_lj_force(orc_sys *s, double *f) {
         f[(size_t)d * NN + i] = f[(size_t)d * NN + i] + inv_r6 * (inv_r6 - 0.5) * rij[d] / rsqr;
     }
   }
+  }
+  free(rb);
+#pragma omp parallel for
   for (int i = 0; i < 3 * NN; i++) f[i] = f[i] * 48;
   /* wall (compt_acc.f90:955-975); 0.4 and 5**(1./6) are default-real */
   const double c04 = (double)0.4f;
@@ -738,7 +797,11 @@ void orc_real_space(orc_sys *s, double *f) {
       }
     }
   } else {
-    for (int64_t k = 0; k < s->npair2; k++) {
+    int64_t *rb = malloc(sizeof(int64_t) * (ORC_NT + 1));
+    row_chunks(s->l2i, s->npair2, ORC_NT, rb);
+#pragma omp parallel
+    {
+    for (int64_t k = rb[ORC_TID]; k < rb[ORC_TID + 1]; k++) {
       int i = s->l2i[k], j = s->l2j[k];
       double rij[3] = {X(i) - X(j), Y(i) - Y(j), Z(i) - Z(j)};
       if (rij[0] > Lx / 2) rij[0] = rij[0] - Lx;
@@ -755,6 +818,8 @@ void orc_real_space(orc_sys *s, double *f) {
         }
       }
     }
+    }
+    free(rb);
   }
   for (int m = 0; m < s->Nq; m++) {
     int i = s->charge[m];
@@ -784,6 +849,7 @@ void orc_spme(orc_sys *s, double *f) {
   }
   /* splcof (compt_acc.f90:1250-1321) */
   for (int h = 0; h < 3; h++) {
+#pragma omp parallel for
     for (int m = 0; m < Nq; m++) {
       int i = s->charge[m];
       double uu = p->ng[h] / p->gdim[h] * s->pos[(size_t)h * NN + i];
@@ -805,23 +871,31 @@ void orc_spme(orc_sys *s, double *f) {
   }
   /* MapCharges (compt_acc.f90:1324-1365) */
   memset(s->Q, 0, sizeof(double complex) * M);
+#pragma omp parallel
+  {
+  const int z0 = (int)((long)n3 * ORC_TID / ORC_NT), z1 = (int)((long)n3 * (ORC_TID + 1) / ORC_NT);
   for (int m = 0; m < Nq; m++)
-    for (int c = 0; c < o[2]; c++)
+    for (int c = 0; c < o[2]; c++) {
+      const int zc = map[2][m * o[2] + c];
+      if (zc < z0 || zc >= z1) continue;  /* this thread's planes */
       for (int b = 0; b < o[1]; b++)
         for (int a = 0; a < o[0]; a++) {
-          size_t g = ((size_t)map[2][m * o[2] + c] * n2 + map[1][m * o[1] + b]) * n1 +
-                     map[0][m * o[0] + a];
+          size_t g = ((size_t)zc * n2 + map[1][m * o[1] + b]) * n1 + map[0][m * o[0] + a];
           s->Q[g] += SP[2][m * o[2] + c] * (SP[0][m * o[0] + a] * SP[1][m * o[1] + b]);
         }
+    }
+  }
   /* pmeOrthoConvBC (compt_acc.f90:1368-1398) */
   memcpy(s->U, s->Q, sizeof(double complex) * M);
   fft3d_c(s->U, n1, n2, n3, -1);
   s->U[0] = 0;
+#pragma omp parallel for
   for (size_t g = 0; g < M; g++) s->U[g] = s->U[g] * s->BC[g];
   fft3d_c(s->U, n1, n2, n3, +1);
   /* IFrc (compt_acc.f90:1402-1467) */
   double *af = calloc((size_t)Nq * 3, sizeof(double));
   double nn = (double)(n1 * n2 * n3);
+#pragma omp parallel for
   for (int m = 0; m < Nq; m++) {
     double complex f1 = 0, f2 = 0, f3 = 0;
     for (int c = 0; c < o[2]; c++)
@@ -1168,14 +1242,17 @@ int orc_compute_force(orc_sys *s, int32_t step, int32_t noise,
   }
   memcpy(s->acc, s->acc_c, sizeof(double) * 3 * NN);
   orc_lj_force(s, tmp);
+#pragma omp parallel for
   for (int i = 0; i < 3 * NN; i++) s->acc[i] = s->acc[i] + tmp[i];
   int rc = orc_fene_force(s, tmp);
   if (rc != MD_OK) {
     free(tmp);
     return rc;
   }
+#pragma omp parallel for
   for (int i = 0; i < 3 * NN; i++) s->acc[i] = s->acc[i] + tmp[i];
   double sig = sqrt(2.0 / p->Beta / p->dt * p->xi);
+#pragma omp parallel for
   for (int i = 0; i < NN; i++) {
     double n[3] = {0, 0, 0};
     if (noise == MD_NOISE_ETA) {
@@ -1202,9 +1279,11 @@ int orc_new_position(orc_sys *s, int32_t step, int32_t noise,
   md_params *p = &s->p;
   int NN = s->NN;
   double dt = p->dt, xi = p->xi;
+#pragma omp parallel for
   for (int i = 0; i < 3 * NN; i++)
     s->vel[i] = s->acc[i] * dt / 2 + (1 - xi * dt / 2) * s->vel[i];
   double dmax1 = 0, dmax2 = 0;
+#pragma omp parallel for reduction(max : dmax1, dmax2)
   for (int i = 0; i < NN; i++) {
     double dri[3];
     for (int d = 0; d < 3; d++) {
@@ -1225,6 +1304,7 @@ int orc_new_position(orc_sys *s, int32_t step, int32_t noise,
     else if (Y(k) <= -p->Ly / 2) Y(k) = Y(k) + p->Ly;
   }
   int rc = orc_compute_force(s, step, noise, eta);
+#pragma omp parallel for
   for (int i = 0; i < 3 * NN; i++)
     s->vel[i] = (s->vel[i] + s->acc[i] * dt / 2.) / (1 + xi * dt / 2.);
   return rc;

@@ -54,6 +54,7 @@ int orc_update_verlet_list(orc_sys *s);
 void orc_rescale_velocity(orc_sys *s);
 int orc_run(orc_sys *s, int32_t step0, int32_t nsteps, int32_t preheat);
 void orc_energies(orc_sys *s, double *out);  /* out[16], see md_oracle.c */
+void orc_set_threads(int n);                 /* OpenMP build: threads of the hot loops */
 
 /* building blocks exposed for tests */
 void orc_fft3d(int n1, int n2, int n3, double *re_im, int sign);

@@ -662,3 +662,25 @@ def test_kinetic_energy_definition():
     e = o.energies()
     assert abs(e["kinetic"] - 0.5 * np.sum(vel ** 2)) < 1e-10 * e["kinetic"]
     assert abs(e["kinetic_per_particle"] - e["kinetic"] / (p.NN - p.N_anchor)) < 1e-15
+
+
+def test_fast_oracle_equals_checker():
+    """The CPU-baseline build of the oracle (-O3, OpenMP) gives the checker's
+    results bit for bit, with one thread and with four."""
+    p, pos, vel = system("small")
+    ref = orc.Oracle(p)
+    ref.set_state(pos, vel)
+    ref.build_lists()
+    assert ref.compute_force(0, 2) == 0
+    assert ref.run(1, 25) == 0
+    want = ref.state()
+    for nt in (1, 4):
+        orc.fast_lib().orc_set_threads(nt)
+        o = orc.Oracle(p, orc.fast_lib())
+        o.set_state(pos, vel)
+        o.build_lists()
+        assert o.compute_force(0, 2) == 0
+        assert o.run(1, 25) == 0
+        got = o.state()
+        for a, b in zip(got, want):
+            assert np.array_equal(a, b), nt

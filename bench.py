@@ -112,6 +112,72 @@ def prepare(seed, equil_steps):
     return pos, vel, acc
 
 
+OBS_STEPS, OBS_EVERY, OBS_BLOCKS = 5000, 50, 10  # observable runs: length, sampling, blocks
+ENERGY_TERMS = ("lj", "wall", "fene", "coulomb", "field", "potential", "kinetic", "virial_lj",
+                "virial_real")
+
+
+def observables(state, seed):
+    """FP32 forces against FP64 per observable, on the equilibrated default
+    brush.  Energies and virials at one configuration (relative difference
+    per term), then two runs from the same state with the same Philox noise
+    stream, FP64 and FP32, sampled every OBS_EVERY steps: brush height (mean
+    z of the polymer beads, in_output.f90:1017-1020), potential energy and
+    the beads' z-density profile in 25 bins; each mean's standard error from
+    OBS_BLOCKS block averages, and the FP32 - FP64 difference in units of
+    the combined error (the runs decorrelate, so this is a statistical
+    comparison)."""
+    pos, vel, acc = state
+    en = {}
+    for prec in (MD_FP64, MD_FP32):
+        e = Engine(config("default", prec, seed=seed))
+        e.upload(pos, vel, acc)
+        e.build_lists()
+        en[prec] = e.energies()
+        e.close()
+    same = {k: float(abs(en[MD_FP32][k] - en[MD_FP64][k]) / max(abs(en[MD_FP64][k]), 1e-300))
+            for k in ENERGY_TERMS}
+    series = {}
+    for prec in (MD_FP64, MD_FP32):
+        p = config("default", prec, seed=seed)
+        e = Engine(p)
+        e.upload(pos, vel, acc)
+        e.build_lists()
+        e.compute_force(0, MD_NOISE_PHILOX)
+        edges = np.linspace(0, p.Lz, 26)
+        h, prof, pot = [], [], []
+        for s in range(1, OBS_STEPS + 1, OBS_EVERY):
+            e.run(s, OBS_EVERY)
+            z = e.download()[0][: p.Npe, 2]
+            h.append(z.mean())
+            prof.append(np.histogram(z, edges)[0])
+            pot.append(e.energies()["potential"])
+        e.close()
+        series[prec] = (np.array(h), np.array(prof, float), np.array(pot))
+
+    def mean_se(x):
+        m = np.array([b.mean(axis=0) for b in np.array_split(x, OBS_BLOCKS)])
+        return x.mean(axis=0), m.std(axis=0, ddof=1) / np.sqrt(OBS_BLOCKS)
+
+    out = {"energies_same_state_rel_diff": same,
+           "runs": f"{OBS_STEPS} steps each from the bench state, sampled every {OBS_EVERY}, "
+                   f"{OBS_BLOCKS} blocks"}
+    for k, name in ((0, "h_avg"), (2, "potential")):
+        m64, s64 = mean_se(series[MD_FP64][k])
+        m32, s32 = mean_se(series[MD_FP32][k])
+        se = float(np.hypot(s64, s32))
+        out[name] = {"fp64": float(m64), "fp32": float(m32), "se": se,
+                     "z": float((m32 - m64) / se) if se > 0 else 0.0}
+    m64, s64 = mean_se(series[MD_FP64][1])
+    m32, s32 = mean_se(series[MD_FP32][1])
+    se = np.hypot(s64, s32)
+    sel = se > 0
+    zz = (m32[sel] - m64[sel]) / se[sel]
+    out["z_profile"] = {"bins": int(sel.sum()), "chi2_per_bin": float(np.mean(zz ** 2)) if sel.any() else 0.0,
+                        "max_abs_z": float(np.abs(zz).max()) if sel.any() else 0.0}
+    return out
+
+
 WARMUP_MIN = 100  # the first replays of a freshly instantiated cycle graph run slower
 
 
@@ -265,6 +331,18 @@ def cpu_baseline(state, seed, budget_s, threads=1):
     return steps / el, steps, el
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def host_threads():
     """Threads the host offers this process (OMP_NUM_THREADS where set)."""
     try:
@@ -283,6 +361,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-obs", action="store_true", help="skip the FP32-vs-FP64 observable runs")
     ap.add_argument("--serve", type=int, default=256, help="requests in the serving leg (0: skip)")
     ap.add_argument("--serve-steps", type=int, default=10000)
     ap.add_argument("--serve-rate", type=float, default=128.0)
@@ -341,6 +420,8 @@ def main():
     out["fp32_force_rel_rms"] = float(np.sqrt(((f32 - f64) ** 2).sum() / (f64 ** 2).sum()))
     e32b.close()
     e64.close()
+    if not args.no_obs:
+        out["fp32_observables"] = observables(state, args.seed)
     if not args.no_fp64:
         n64 = min(args.steps, 4000)
         _, e64t, sps64, _ = time_engine(MD_FP64, state, n64, args.warmup, args.seed)
@@ -403,6 +484,7 @@ def main():
     if not args.no_cpu:
         sps_cpu, n, el = cpu_baseline(state, args.seed, args.cpu_seconds, 1)
         out["cpu_baseline"] = {"value": round(sps_cpu, 3), "unit": "steps/s", "cores": 1, "kind": "port",
+                               "cpu": cpu_model(), "host_cpus": os.cpu_count(),
                                "sample": f"{n} steps of the same workload (FP64 oracle, C, -O3 "
                                          f"-march=x86-64-v3, one thread) in {el:.1f} s"}
         nt = host_threads()

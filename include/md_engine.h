@@ -191,18 +191,20 @@ int md_error_analysis(md_engine *e, double *out, double *acc_se);
 
 /* Mean duration in milliseconds of each kernel of a step, timed with HIP
  * events on the engine's stream over `iters` launches (no reference
- * counterpart: measurement only; short_range in the form md_run's replayed
- * steps use (second half kick, then the next step's half kick and drift into
- * the other position buffer) over 8 back-to-back launches per iteration;
- * kick_drift, the stepwise form that md_run issues once per run, as the
- * difference between it followed by those launches and the launches alone).
+ * counterpart: measurement only; short_range is the launch md_run's
+ * replayed steps issue (list bookkeeping, second half kick, then the next
+ * step's half kick and drift into the other position buffer), on a step
+ * without a list rebuild, over 8 back-to-back launches per iteration, each
+ * reading the buffer the one before wrote; kick_drift, the form that starts
+ * a run, as the difference between it followed by those launches and the
+ * launches alone).
  * ms[15]: kick_drift, short_range, bin_q, sort_q, spread, fft_x_fwd,
  * fft_y_fwd, fft_z_conv, fft_y_inv, fft_x_inv, gather, real, bin_lj,
  * sort_lj, build_lj (with fused FFT planes, fft_x_* time the plane kernels
  * and fft_y_* are empty; sort_lj is empty, its rank pass is part of
- * build_lj).  Positions, velocities and forces are restored
- * after each iteration; the lists are rebuilt and the held Coulomb force is
- * recomputed at the current positions. */
+ * build_lj).  Positions, velocities, forces, the list bookkeeping and the
+ * rebuild queues are restored after each iteration; the lists are rebuilt
+ * and the held Coulomb force is recomputed at the current positions. */
 int md_time_kernels(md_engine *e, int32_t iters, double *ms, int32_t n);
 
 /* Chooses the engine's launch forms by timing its Coulomb evaluation under

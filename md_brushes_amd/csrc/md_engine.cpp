@@ -1584,62 +1584,117 @@ extern "C" int md_time_kernels(md_engine *e, int32_t iters, double *ms, int32_t 
     (void)hipEventElapsedTime(&t, ev[a], ev[b]);
     return (double)t;
   };
-  // positions, velocities and forces, which the timed kick-drift and
-  // short-range launches move: restored after every iteration, before the
-  // Coulomb and list kernels run at the restored positions
-  std::vector<char *> snap(e->sys.size(), nullptr);
-  for (size_t k = 0; k < snap.size(); k++) {
-    const size_t NN = e->sys[k]->p.NN;
-    if (hipMalloc((void **)&snap[k], NN * (32 + 16 + 24 + 24)) != hipSuccess) snap[k] = nullptr;
-    if (!snap[k]) {
-      for (char *x : snap)
-        if (x) (void)hipFree(x);
-      for (auto &x : ev) (void)hipEventDestroy(x);
+  // What the timed kick-drift and short-range launches write: both position
+  // buffers, velocities, forces, the group displacement maxima, the control
+  // block and the rebuild queues; saved once and restored after every
+  // iteration of the short-range timing, which runs first, all iterations
+  // together, so that no list kernel advances the control block's rebuild
+  // generation in between (a restored generation behind the count buffers
+  // would make the next binning append to a buffer it did not clear).  The
+  // timed launches start from a copy of the control block whose accumulated
+  // displacements are -infinity, so that none of them can take a rebuild (a
+  // rebuild step bins into the next list generation): they time the
+  // cycle's launch on a step without a rebuild, the same instructions.
+  const int q0 = e->cur;
+  struct Part {
+    void *dev;
+    size_t n;
+  };
+  auto parts = [&](Sys *x) {
+    const size_t NN = x->p.NN;
+    std::vector<Part> v = {{x->posb[0], 32 * NN},  {x->posb[1], 32 * NN},
+                           {x->pos32b[0], 16 * NN}, {x->pos32b[1], 16 * NN},
+                           {x->h.vx, 24 * NN},      {x->h.ax, 24 * NN},
+                           {x->h.dpart, 2 * sizeof(double2) * (size_t)x->h.ngrp},
+                           {x->h.ctrl, sizeof(Ctrl)}};
+    return v;
+  };
+  auto rnd = [](size_t n) { return (n + 255) & ~(size_t)255; };
+  const size_t R = e->sys.size();
+  std::vector<char *> snap(R + 1, nullptr);  // per replica: its parts, then its timing control block
+  std::vector<char *> tctrl(R, nullptr);
+  auto release = [&]() {
+    for (char *x : snap)
+      if (x) (void)hipFree(x);
+    for (auto &x : ev) (void)hipEventDestroy(x);
+  };
+  for (size_t k = 0; k <= R; k++) {
+    size_t bytes = e->lq_words * 8;
+    if (k < R) {
+      bytes = 0;
+      for (const Part &pt : parts(e->sys[k])) bytes += rnd(pt.n);
+      bytes += rnd(sizeof(Ctrl));
+    }
+    if (hipMalloc((void **)&snap[k], bytes) != hipSuccess) {
+      snap[k] = nullptr;
+      release();
       return e->fail(MD_ENOMEM, "time_kernels snapshot");
     }
   }
-  auto keep = [&](bool restore) {
-    for (size_t k = 0; k < snap.size(); k++) {
-      Sys *x = e->sys[k];
-      const size_t NN = x->p.NN;
-      void *dev[4] = {x->h.pos, x->h.pos32, x->h.vx, x->h.ax};
-      const size_t nb[4] = {32 * NN, 16 * NN, 24 * NN, 24 * NN};
+  auto keep = [&](bool restore, bool timing_ctrl) {
+    for (size_t k = 0; k < R; k++) {
       char *h = snap[k];
-      for (int j = 0; j < 4; j++) {
-        if (dev[j]) {
-          if (restore) (void)hipMemcpyAsync(dev[j], h, nb[j], hipMemcpyDeviceToDevice, e->st);
-          else (void)hipMemcpyAsync(h, dev[j], nb[j], hipMemcpyDeviceToDevice, e->st);
+      for (const Part &pt : parts(e->sys[k])) {
+        if (pt.dev) {
+          const void *src = restore ? (const void *)h : pt.dev;
+          if (restore && timing_ctrl && pt.dev == e->sys[k]->h.ctrl) src = tctrl[k];
+          (void)hipMemcpyAsync(restore ? pt.dev : (void *)h, src, pt.n, hipMemcpyDeviceToDevice,
+                               e->st);
         }
-        h += nb[j];
+        h += rnd(pt.n);
       }
     }
+    char *lq = snap[R];
+    if (restore) (void)hipMemcpyAsync(e->d_lq, lq, e->lq_words * 8, hipMemcpyDeviceToDevice, e->st);
+    else (void)hipMemcpyAsync(lq, e->d_lq, e->lq_words * 8, hipMemcpyDeviceToDevice, e->st);
   };
+  keep(false, false);
+  for (size_t k = 0; k < R; k++) {  // the control blocks the timed launches start from
+    size_t off = 0;
+    for (const Part &pt : parts(e->sys[k])) off += rnd(pt.n);
+    tctrl[k] = snap[k] + off;
+    Ctrl c;
+    hipError_t err = hipMemcpyAsync(&c, e->sys[k]->h.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, e->st);
+    if (err == hipSuccess) err = hipStreamSynchronize(e->st);
+    c.dr_max1 = c.dr_max2 = -HUGE_VAL;
+    c.pend = 0;
+    if (err == hipSuccess) err = hipMemcpy(tctrl[k], &c, sizeof(Ctrl), hipMemcpyHostToDevice);
+    if (err != hipSuccess) {
+      release();
+      return e->hip_fail(err, "time_kernels control block");
+    }
+  }
+  // the launch of a replayed step (launch_cycle): list bookkeeping, the
+  // second kick, the next step's kick and drift into the other buffer
   const int mode = mdk::SR_HELD | mdk::SR_FIELD | mdk::SR_ANCHOR | mdk::SR_NOISE_PHILOX |
-                   mdk::SR_KICK | mdk::SR_DRIFT;
+                   mdk::SR_KICK | mdk::SR_DRMAX | mdk::SR_DRIFT | mdk::SR_BIN;
   for (int it = 0; it < iters; it++) {
-    // the short-range kernel writes the other position buffer: timed over
-    // back-to-back launches, so that the per-launch mean excludes the event
-    // overhead; the kick-drift as the difference between [kick, shorts] and
-    // [shorts]
+    // timed over back-to-back launches, so that the per-launch mean
+    // excludes the event overhead; each launch reads the buffer the one
+    // before wrote, as in the cycle; the kick-drift that starts a run as
+    // the difference between [kick, shorts] and [shorts]
     constexpr int kRepeat = 8;
-    auto shorts = [&]() {
+    auto shorts = [&](int k0) {
       for (int k = 0; k < kRepeat; k++) {
-        if (e->fp32) mdk::launch_short<float>(B, e->st, mode, 0, nullptr, nullptr);
-        else mdk::launch_short<double>(B, e->st, mode, 0, nullptr, nullptr);
+        const Plan &Pk = e->planp[q0 ^ ((k0 + k) & 1)];
+        if (e->fp32) mdk::launch_short<float>(Pk, e->st, mode, -1, nullptr, nullptr);
+        else mdk::launch_short<double>(Pk, e->st, mode, -1, nullptr, nullptr);
       }
     };
-    keep(false);
+    keep(true, true);
     (void)hipEventRecord(ev[0], e->st);
-    mdk::launch_kick_drift(B, e->st);
-    shorts();
+    mdk::launch_kick_drift(e->planp[q0], e->st, 0);
+    shorts(0);
     (void)hipEventRecord(ev[1], e->st);
-    shorts();
+    shorts(kRepeat);
     (void)hipEventRecord(ev[2], e->st);
-    keep(true);
     const double both = span(0, 1), sh = span(1, 2);
     acc[0] += std::max(0.0, both - sh);
     acc[1] += sh / kRepeat;
-  if (any_coulomb) {
+  }
+  keep(true, false);  // the state as it was; the Coulomb and list kernels move no particle
+  for (int it = 0; it < iters; it++) {
+    if (any_coulomb) {
       if (e->fp32) mdk::launch_coulomb<float>(B, e->st, nullptr, ev);
       else mdk::launch_coulomb<double>(B, e->st, nullptr, ev);
       for (int k = 0; k < 10; k++) acc[2 + k] += span(k, k + 1);

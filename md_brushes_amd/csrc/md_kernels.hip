@@ -524,18 +524,32 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
     // that no workgroup of this launch writes what the others read
     const bool every = (mode & SR_BIN) != 0;
     if ((mode & SR_DRMAX) && (every || blk == 0)) {  // this step's displacements, all groups
+      Ctrl *c = S.ctrl;
+      // the accumulated maxima load beside the group maxima, and the group
+      // maxima eight per lane at a time: the decision is one memory round
+      // trip, shorter than the LJ sums the barrier waits for (no workgroup
+      // of this launch writes either)
+      const double m1 = c->dr_max1, m2 = c->dr_max2;
       double d1 = 0, d2 = 0;
       const double2 *dp = S.dpart + (size_t)S.par * S.ngrp;
-      for (int g = lane; g < S.ngrp; g += 64) {
-        const double2 v = dp[g];
-        d1 = fmax(d1, v.x);
-        d2 = fmax(d2, v.y);
+      constexpr int U = 8;
+      for (int g0 = 0; g0 < S.ngrp; g0 += 64 * U) {
+        double2 v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int g = g0 + u * 64 + lane;
+          v[u] = g < S.ngrp ? dp[g] : make_double2(0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          d1 = fmax(d1, v[u].x);
+          d2 = fmax(d2, v[u].y);
+        }
       }
       d1 = wave_max(d1);
       d2 = wave_max(d2);
-      Ctrl *c = S.ctrl;
       if (every) {
-        const double a1 = c->dr_max1 + d1, a2 = c->dr_max2 + d2;
+        const double a1 = m1 + d1, a2 = m2 + d2;
         const int f1 = a1 > S.skin1;
         const int f2 = (a2 > S.skin2) && S.real_verlet != 0;
         const int buf = (c->gen1 + 1) & 1;  // the next generation's count buffer
@@ -549,8 +563,8 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
           if (f1) lq_push(lq, lq_cap, r, S.NN, buf);
         }
       } else if (lane == 0) {
-        c->dr_max1 = c->dr_max1 + d1;
-        c->dr_max2 = c->dr_max2 + d2;
+        c->dr_max1 = m1 + d1;
+        c->dr_max2 = m2 + d2;
         int f1 = c->dr_max1 > S.skin1;
         int f2 = (c->dr_max2 > S.skin2) && S.real_verlet != 0;
         if (f1) {

@@ -21,8 +21,8 @@ sys.path.insert(0, ROOT)
 
 import numpy as np  # noqa: E402
 
-from md_brushes_amd import (MD_FP32, MD_FP64, MD_NOISE_PHILOX, Engine, config,  # noqa: E402
-                            initialize_position, initialize_velocity)
+from md_brushes_amd import (MD_FP32, MD_FP64, MD_NOISE_PHILOX, Engine, autotune_geometry,  # noqa: E402
+                            config, initialize_position, initialize_velocity)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 
@@ -178,6 +178,16 @@ def observables(state, seed):
     return out
 
 
+TUNE_CYCLES = 50  # cycles timed per autotuner candidate
+
+
+def geometry(p):
+    """The engine geometry of derived parameters: spreading tile edges (0 =
+    16), LJ skin (0 = the reference's 0.5), real-space cell scale (0 = 1)."""
+    return {"tile": list(p.ovr_tile), "skin": p.ovr_skin, "cell_scale": p.ovr_cell_scale,
+            "rvl": round(p.rvl_eff, 4)}
+
+
 WARMUP_MIN = 100  # the first replays of a freshly instantiated cycle graph run slower
 
 
@@ -190,8 +200,8 @@ def aligned_warmup(warmup, multistep):
     return -(-w // multistep) * multistep
 
 
-def time_engine(prec, state, steps, warmup, seed):
-    p = config("default", prec, seed=seed)
+def time_engine(prec, state, steps, warmup, seed, p=None):
+    p = p if p is not None else config("default", prec, seed=seed)
     e = Engine(p)
     pos, vel, acc = state
     e.upload(pos, vel, acc)
@@ -386,7 +396,10 @@ def main():
         sys.exit(2)
 
     state = prepare(args.seed, args.equil)
-    p32, e32, sps32, dt32 = time_engine(MD_FP32, state, args.steps, args.warmup, args.seed)
+    # engine geometry by the autotuner: spreading tiles only (results unchanged)
+    geo32, tuner32 = autotune_geometry(config("default", MD_FP32, seed=args.seed), state[0], state[1],
+                                       cycles=TUNE_CYCLES)
+    p32, e32, sps32, dt32 = time_engine(MD_FP32, state, args.steps, args.warmup, args.seed, geo32)
     out = {
         "metric": "steps/s at 21,333 particles (FP32 vs FP64, force err<=1e-4); replica p50/p99",
         "value": round(sps32, 2),
@@ -404,7 +417,8 @@ def main():
         "config": {"workload": "default brush 8x8 linear N=100 + trivalent salt, NN=21396, Nq=21332, "
                                "SPME 64x64x256 p=4, multistep 10",
                    "NN": p32.NN, "Nq": p32.Nq, "mesh": list(p32.ng), "order": p32.ordr[0],
-                   "alpha": round(p32.alpha, 5), "r_c": round(p32.rcc, 4), "multistep": p32.multistep},
+                   "alpha": round(p32.alpha, 5), "r_c": round(p32.rcc, 4), "multistep": p32.multistep,
+                   "geometry": geometry(geo32)},
     }
     # accuracy gate on one configuration: FP32 forces against FP64 forces
     pos, vel, acc = state
@@ -424,7 +438,9 @@ def main():
         out["fp32_observables"] = observables(state, args.seed)
     if not args.no_fp64:
         n64 = min(args.steps, 4000)
-        _, e64t, sps64, _ = time_engine(MD_FP64, state, n64, args.warmup, args.seed)
+        geo64, _ = autotune_geometry(config("default", MD_FP64, seed=args.seed), state[0], state[1],
+                                     cycles=TUNE_CYCLES)
+        _, e64t, sps64, _ = time_engine(MD_FP64, state, n64, args.warmup, args.seed, geo64)
         out["fp64_steps_per_s"] = round(sps64, 2)
         e64t.close()
 
@@ -437,6 +453,14 @@ def main():
     out["roofline_step"] = roofline_step(p32, out["ms_per_step"])
     out["kernel_us"] = {k: round(v * 1e3, 3) for k, v in ms.items()}
     e32.close()
+    # the autotuner with the list knobs too (Verlet skin, sweep cell size): the
+    # same pairs in another summation order, so reported beside the headline
+    geo_l, tuner_l = autotune_geometry(config("default", MD_FP32, seed=args.seed), state[0], state[1],
+                                       cycles=TUNE_CYCLES, lists=True)
+    out["tuned_lists"] = dict(geometry(geo_l), tuner_steps_per_s=round(tuner_l, 1),
+                              tuner_steps_per_s_tiles_only=round(tuner32, 1),
+                              note="md_autotune_geometry rates over 50 cycles after 10 warm-up cycles; "
+                                   "the list knobs change the summation order (forces to rounding)")
 
     if args.serve > 0:
         from md_brushes_amd.serve import Server, make_stream, sequential_fp64, summarize

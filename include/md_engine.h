@@ -89,6 +89,11 @@ typedef struct md_params {
   int32_t precision;                   /* MD_FP64 or MD_FP32                     */
   int32_t pad3;
   uint64_t seed;                       /* key of the Philox noise stream         */
+  /* engine geometry (0 = the reference's; md_autotune_geometry chooses) */
+  double ovr_skin;        /* LJ list: rvl_eff = rcl_s + 2 ovr_skin (reference rcl_s + 1)  */
+  double ovr_cell_scale;  /* real-space sweep cells of edge >= ovr_cell_scale * r_c (>= 1) */
+  int32_t ovr_tile[3];    /* spreading tile edge per axis, 4..16 mesh points (16)         */
+  int32_t pad4;
 } md_params;
 
 typedef struct md_engine md_engine;
@@ -217,6 +222,22 @@ int md_time_kernels(md_engine *e, int32_t iters, double *ms, int32_t n);
  * Leaves the state as it was; the held Coulomb force is recomputed for the
  * current positions. */
 int md_autotune(md_engine *e, int32_t iters, int32_t *choice);
+
+enum { MD_TUNE_LISTS = 1 };
+
+/* Engine geometry for a system (no reference counterpart: tuning only),
+ * chosen by running it from pos(NN,4), vel(NN,3) in fresh engines: the
+ * spreading tile edges ovr_tile (results unchanged) and, with flags &
+ * MD_TUNE_LISTS, the LJ Verlet skin ovr_skin and the real-space sweep's
+ * cell scale ovr_cell_scale (the same pairs within every cut in another
+ * order: forces change by rounding; the reference's skin is 0.5).  Each
+ * candidate runs `cycles` multistep cycles after ten warm-up cycles; knobs
+ * are tuned one after the other and a candidate must beat the incumbent by
+ * 1% (3% for the list knobs).  *best receives p with the chosen overrides,
+ * derived; *steps_per_s (may be null) its measured rate. */
+int md_autotune_geometry(const md_params *p, const double *pos, const double *vel,
+                         int32_t cycles, int32_t flags, md_params *best,
+                         double *steps_per_s);
 
 /* ---- Replica batches (no reference counterpart: the reference runs one
  * system per process).  One engine holds many independent replicas of

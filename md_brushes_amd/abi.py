@@ -45,6 +45,7 @@ class MdParams(ctypes.Structure):
         ("Lx", _D), ("Ly", _D), ("Zfac", _D), ("sigma_sp", _D), ("rcl_s", _D), ("rcl_sp", _D),
         ("rvl_eff", _D), ("alpha", _D), ("rcc", _D), ("rvc", _D), ("gdim", _D * 3),
         ("precision", _I), ("pad3", _I), ("seed", c_uint64),
+        ("ovr_skin", _D), ("ovr_cell_scale", _D), ("ovr_tile", _I * 3), ("pad4", _I),
     ]
 
     def copy(self):
@@ -121,6 +122,7 @@ def load_library(path=None):
         "md_batch_download": ([c_void_p, _I, dp, dp, dp], _I),
         "md_batch_info": ([c_void_p, _I, POINTER(c_int64), ip], _I),
         "md_replica_bytes": ([P, dp, POINTER(c_int64)], _I),
+        "md_autotune_geometry": ([P, dp, dp, _I, _I, P, dp], _I),
         "md_batch_observe": ([c_void_p, _I, dp, dp, _I], _I),
     }
     for name, (args, res) in sig.items():
@@ -174,6 +176,23 @@ def replica_bytes(p, pos):
     if rc:
         raise MDError(rc, "md_replica_bytes")
     return b.value
+
+
+MD_TUNE_LISTS = 1
+
+
+def autotune_geometry(p, pos, vel, cycles=50, lists=False):
+    """Engine geometry chosen by running the system (md_autotune_geometry):
+    spreading tile edges (results unchanged) and, with lists=True, the LJ
+    Verlet skin and the real-space sweep's cell scale (forces change by
+    rounding).  Returns (derived params with the chosen overrides, steps/s)."""
+    best = MdParams()
+    sps = c_double()
+    rc = load_library().md_autotune_geometry(byref(p), _dp(_colmajor(pos)), _dp(_colmajor(vel)), cycles,
+                                             MD_TUNE_LISTS if lists else 0, byref(best), byref(sps))
+    if rc:
+        raise MDError(rc, "md_autotune_geometry")
+    return best, sps.value
 
 
 def initialize_position(p, seed):

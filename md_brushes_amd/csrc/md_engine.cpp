@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -316,7 +317,9 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   ncell3 = 0;
   if (coulomb && !S.real_list) {
     for (int d = 0; d < 3; d++) {
-      S.nc3[d] = std::max(1, (int)(L3[d] / p.rcc));
+      // cells of edge >= r_c (times the caller's cell scale, >= 1)
+      const double cs = p.ovr_cell_scale > 1 ? p.ovr_cell_scale : 1.0;
+      S.nc3[d] = std::max(1, (int)(L3[d] / (p.rcc * cs)));
       S.rcel3[d] = L3[d] / S.nc3[d];
     }
     S.real_cells = 1;
@@ -428,12 +431,15 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
     S.spec = dalloc<char>(Mh * csz);
     S.meshu = dalloc<char>(M * rs);
     if (!S.meshr || !S.spec || !S.meshu) return fail(MD_ENOMEM, "mesh");
-    // spreading tiles: ceil(K / kTile) per axis of near-equal edges (K / ntl
-    // or one more), every edge >= order (>= 8 from K >= 16 up, K itself
-    // below), so a charge's points fall in at most two tiles along each axis
+    // spreading tiles: ceil(K / edge) per axis of near-equal edges (K / ntl
+    // or one more), edge kTile or the caller's (4..kTile), every edge >=
+    // the axis's order (fewer tiles where it would not be), so a charge's
+    // points fall in at most two tiles along each axis
     nbin = 1;
     for (int d = 0; d < 3; d++) {
-      S.ntl[d] = cdiv(p.ng[d], mdk::kTile);
+      const int edge = p.ovr_tile[d] > 0 ? std::min(std::max(p.ovr_tile[d], 4), mdk::kTile) : mdk::kTile;
+      S.ntl[d] = cdiv(p.ng[d], edge);
+      if (p.ng[d] / S.ntl[d] < S.ord[d]) S.ntl[d] = std::max(1, p.ng[d] / S.ord[d]);
       S.tsz[d] = p.ng[d] / S.ntl[d];
       S.trem[d] = p.ng[d] % S.ntl[d];
       nbin *= S.ntl[d];
@@ -1780,6 +1786,86 @@ extern "C" int md_autotune(md_engine *e, int32_t iters, int32_t *choice) {
     choice[3] = e->tune_ride;
   }
   return rc;
+}
+
+// Chooses the engine geometry of a system by running it: the spreading
+// tile edges (the spread sums are exact fixed point, so results do not
+// change) and, with flags & MD_TUNE_LISTS, the LJ Verlet skin and the
+// real-space sweep's cell scale (the same pairs within every cut, but the
+// lists and sweep cells order the sums differently: forces change by
+// rounding).  Each candidate is a fresh engine started from (pos, vel):
+// lists, step-0 forces, warm-up cycles (graph capture and first replays),
+// then `cycles` multistep cycles timed on the host around md_run, which ends
+// with a stream synchronise.  The knobs are tuned one after the other, each
+// keeping the best so far; a candidate replaces the incumbent only when it
+// is faster by kMargin (results-changing knobs by kMarginLists), so timing
+// noise does not move the reference's choice.
+extern "C" int md_autotune_geometry(const md_params *p, const double *pos, const double *vel,
+                                    int32_t cycles, int32_t flags, md_params *best,
+                                    double *steps_per_s) {
+  if (!p || !pos || !vel || !best || cycles <= 0) return MD_EINVAL;
+  constexpr double kMargin = 1.01, kMarginLists = 1.03;
+  constexpr int kWarmCycles = 10;
+  auto rate = [&](const md_params &cand, double &sps) -> int {
+    md_params d = cand;
+    int rc = md_derive_params(&d);
+    if (rc) return rc;
+    md_engine *e = nullptr;
+    rc = md_create(&e, &d);
+    if (!rc) rc = md_upload_state(e, pos, vel, nullptr);
+    if (!rc) rc = md_build_lists(e);
+    if (!rc) rc = md_compute_force(e, 0, MD_NOISE_PHILOX, nullptr);
+    const int ms = std::max(1, d.multistep);
+    if (!rc) rc = md_run(e, 1, kWarmCycles * ms, 0);
+    if (!rc) {
+      const auto t0 = std::chrono::steady_clock::now();
+      rc = md_run(e, 1 + kWarmCycles * ms, cycles * ms, 0);
+      const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      sps = dt > 0 ? cycles * ms / dt : 0;
+    }
+    md_destroy(e);
+    return rc;
+  };
+  md_params cur = *p;
+  double cur_sps = 0;
+  int rc = rate(cur, cur_sps);
+  if (rc) return rc;
+  auto consider = [&](const md_params &cand, double margin) {
+    double sps = 0;
+    if (rate(cand, sps) == MD_OK && sps > cur_sps * margin) {
+      cur = cand;
+      cur_sps = sps;
+    }
+  };
+  // tile edges: 16 or 8 points along x and y together, then along z
+  const int start_tile[3] = {p->ovr_tile[0], p->ovr_tile[1], p->ovr_tile[2]};
+  for (int e : {16, 8}) {
+    md_params c = cur;
+    c.ovr_tile[0] = c.ovr_tile[1] = e;
+    if (e != (start_tile[0] ? start_tile[0] : 16)) consider(c, kMargin);
+  }
+  for (int e : {16, 8}) {
+    md_params c = cur;
+    c.ovr_tile[2] = e;
+    if (e != (start_tile[2] ? start_tile[2] : 16)) consider(c, kMargin);
+  }
+  if (flags & MD_TUNE_LISTS) {
+    for (double skin : {0.75, 1.0}) {  // the reference's skin is 0.5 (rvl = rcl_s + 1)
+      md_params c = cur;
+      c.ovr_skin = skin;
+      consider(c, kMarginLists);
+    }
+    for (double scale : {1.25, 1.5}) {
+      md_params c = cur;
+      c.ovr_cell_scale = scale;
+      consider(c, kMarginLists);
+    }
+  }
+  rc = md_derive_params(&cur);
+  if (rc) return rc;
+  *best = cur;
+  if (steps_per_s) *steps_per_s = cur_sps;
+  return MD_OK;
 }
 
 // error_analysis (compt_acc.f90:729-812) for one replica.  The production

@@ -196,7 +196,8 @@ extern "C" int md_derive_params(md_params *p) {
   p->sigma_sp = (p->sigma + p->sigma_s) / 2;
   p->rcl_s = (double)two6 * p->sigma_s;
   p->rcl_sp = (double)two6 * p->sigma_sp;
-  p->rvl_eff = p->rcl_s + 1;
+  // LJ list radius rcl_s + 1 (compt_acc.f90:162), or the skin a caller chose
+  p->rvl_eff = p->rcl_s + (p->ovr_skin > 0 ? 2 * p->ovr_skin : 1);
   p->gdim[0] = p->Lx;
   p->gdim[1] = p->Ly;
   p->gdim[2] = p->Lz * p->Zfac;

@@ -24,6 +24,8 @@ module md_api
     real(c_double) :: Lx, Ly, Zfac, sigma_sp, rcl_s, rcl_sp, rvl_eff, alpha, rcc, rvc, gdim(3)
     integer(c_int32_t) :: precision, pad3
     integer(c_int64_t) :: seed
+    real(c_double) :: ovr_skin, ovr_cell_scale
+    integer(c_int32_t) :: ovr_tile(3), pad4
   end type md_params
 
   interface

@@ -171,7 +171,9 @@ int orc_derive(md_params *p) {
     p->rcl_s = (double)c6 * p->sigma_s;
     p->rcl_sp = (double)c6 * p->sigma_sp;
   }
-  p->rvl_eff = p->rcl_s + 1;
+  /* the LJ list radius rcl_s + 1 (compt_acc.f90:162); a caller's skin
+     (ovr_skin, an engine option) widens it the same way */
+  p->rvl_eff = p->rcl_s + (p->ovr_skin > 0 ? 2 * p->ovr_skin : 1);
   if (p->qq != 0) {
     orc_ewald_rule(p, p->tol, p->tau_rf);
   } else {

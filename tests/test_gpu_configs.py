@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import orc
-from md_brushes_amd import MD_FP32, MD_FP64, MD_NOISE_PHILOX, Engine
+from md_brushes_amd import MD_FP32, MD_FP64, MD_NOISE_PHILOX, Engine, autotune_geometry
 from md_brushes_amd.serve import Server, make_stream, run_alone, summarize
 from systems import system
 
@@ -60,6 +60,28 @@ def test_mixed_large_brush():
     x = e.download()[0]
     e.close()
     assert np.all(np.isfinite(x))
+
+
+def test_mixed_large_brush_autotuned():
+    """The autotuner picks the spreading tiles, the Verlet skin and the
+    real-space cell size for the mixed large brush (after 500 pre-heat
+    steps); at its choice the lists are bit-exact with the oracle's at the
+    same parameters, FP64 forces match, and the FP32 gate holds."""
+    print("[config] mixed large brush, autotuned geometry", flush=True)
+    p, pos, vel = system("mixed")
+    e = Engine(p)
+    e.upload(pos, vel)
+    e.build_lists()
+    e.compute_force(0, MD_NOISE_PHILOX)
+    e.run(1, 500, preheat=True)
+    x, v = e.download()[:2]
+    e.close()
+    best, sps = autotune_geometry(p, x, v, cycles=10, lists=True)
+    print(f"  tile {list(best.ovr_tile)} skin {best.ovr_skin} cell {best.ovr_cell_scale}: {sps:.0f} steps/s")
+    assert sps > 0
+    es, ec, e32 = _parity(best, x, v)
+    print(f"  fp64 short {es:.1e} coulomb {ec:.1e}; fp32 {e32:.2e}")
+    assert es < 1e-11 and ec < 1e-10 and e32 < 1e-4
 
 
 @pytest.mark.parametrize("ez", [0.0, 0.25, 0.5, 0.75, 1.0])

@@ -65,6 +65,8 @@ def _random_params(rng):
         p.ovr_alpha = float(rng.uniform(0.2, 1.0))
     if rng.random() < 0.3:
         p.ovr_ng[0], p.ovr_ng[1], p.ovr_ng[2] = 16, 32, 64
+    if rng.random() < 0.3:
+        p.ovr_skin = float(rng.uniform(0.3, 1.5))
     return p
 
 
@@ -74,6 +76,18 @@ def test_derivation_matches_oracle():
     for p in cases:
         a, b = derive(p), orc.derive(p)
         assert a.as_dict() == b.as_dict()
+
+
+def test_skin_override_widens_the_list_radius():
+    """rvl_eff = rcl_s + 1 by the reference's rule (compt_acc.f90:162); a
+    caller's skin gives rcl_s + 2 skin in the engine and in the oracle."""
+    p = reference_defaults()
+    a = derive(p)
+    assert a.rvl_eff == a.rcl_s + 1
+    q = p.copy()
+    q.ovr_skin = 0.8
+    b, c = derive(q), orc.derive(q)
+    assert b.rvl_eff == b.rcl_s + 1.6 and c.rvl_eff == b.rvl_eff
 
 
 def test_topology_matches_oracle():

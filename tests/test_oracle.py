@@ -167,8 +167,16 @@ def _brute_pairs(pos, items, rv, Lx, Ly):
     return np.array(sorted(out), np.int32).reshape(-1, 2)
 
 
-def test_lists_match_brute_force():
+@pytest.mark.parametrize("skin", [0.0, 0.9])
+def test_lists_match_brute_force(skin):
+    """Both lists against O(N^2) enumeration, at the reference's LJ radius
+    and at a wider skin (ovr_skin: rvl_eff = rcl_s + 2 skin)."""
     p = _charged_box()
+    if skin:
+        p = p.copy()
+        p.ovr_skin = skin
+        p = orc.derive(p)
+        assert abs(p.rvl_eff - (p.rcl_s + 2 * skin)) < 1e-15
     assert p.real_verlet == 1
     pos = initialize_position(p, 5)
     o = orc.Oracle(p)

@@ -432,6 +432,10 @@ def main():
     fs32, fc32 = e32b.force_parts()
     f64, f32 = fs64 + fc64, fs32 + fc32
     out["fp32_force_rel_rms"] = float(np.sqrt(((f32 - f64) ** 2).sum() / (f64 ** 2).sum()))
+    # the Coulomb force against standard Ewald (error_analysis, on the device):
+    # the SPME set-up's own accuracy, FP64 and FP32
+    out["coulomb_rmsf_vs_ewald"] = {"fp64": e64.error_analysis()[0]["rmsf"],
+                                    "fp32": e32b.error_analysis()[0]["rmsf"]}
     e32b.close()
     e64.close()
     if not args.no_obs:

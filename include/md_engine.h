@@ -217,8 +217,9 @@ int md_time_kernels(md_engine *e, int32_t iters, double *ms, int32_t n);
  * separate x/y passes, z-pass lines per workgroup, a wavefront or a
  * workgroup per real-space work item, the real-space sweep in a launch of
  * its own or inside the spreading and gather launches.  The forms give the
- * same results.  choice[4] (may be null) receives fused (0/1), z lines,
- * workgroup (0/1), shared launches (0/1).
+ * same results.  choice[5] (may be null) receives fused (0/1), z lines,
+ * workgroup (0/1), shared launches (0/1), the workgroup form's grid cap
+ * (1024 or 2048 workgroups).
  * Leaves the state as it was; the held Coulomb force is recomputed for the
  * current positions. */
 int md_autotune(md_engine *e, int32_t iters, int32_t *choice);

@@ -143,7 +143,8 @@ struct Sys {
   int nb[kMaps] = {0};
   size_t lds_x = 0, lds_y = 0, lds_z = 0, lds_sort = 0, lds_sort3 = 0, lds_plane = 0;
   int nbin = 0;  // spreading tiles
-  int nb10_wave = 0, nb10_wg = 0;  // real-space sweep grids of the two forms
+  int nb10_wave = 0, nb10_wg = 0;  // real-space sweep grids of the two forms (the
+                                   // workgroup form's before the engine's cap)
   int pz_big = 1;                  // z-pass lines per workgroup in large engines
   bool can_fuse = false;           // both planes of a fused FFT kernel fit in LDS
   size_t plane_bytes = 0;
@@ -705,10 +706,10 @@ int Sys::size_lists(const double *pos) {
   lds_sort3 = h.real_cells ? (size_t)np3 * sizeof(int) : 0;
   // about one non-empty item per wavefront, or per workgroup (the form is
   // chosen for the whole engine at rebuild)
-  nb10_wg = h.real_cells ? std::max(1, std::min(cdiv(h.Nq, mdk::kRealChunkWG), 1024)) : 0;
+  nb10_wg = h.real_cells ? std::max(1, cdiv(h.Nq, mdk::kRealChunkWG)) : 0;
   nb10_wave =
       h.real_cells ? std::max(1, std::min(cdiv(h.Nq, mdk::kRealChunk * (mdk::kBlock / 64)), 4096)) : 0;
-  nb[10] = h.real_wg ? nb10_wg : nb10_wave;
+  nb[10] = h.real_wg ? std::min(nb10_wg, 1024) : nb10_wave;  // the engine's rebuild sets its cap
   if (ccap1 != h.ccap1) {
     if (h.mem1) dfree(h.mem1, (size_t)ncell1 * h.ccap1 * 4);
     h.mem1 = dalloc<int>((size_t)ncell1 * ccap1);
@@ -779,6 +780,7 @@ struct md_engine {
   int tune_pz = 0;                 // z-pass lines per workgroup
   int tune_fuse = -1;              // fused FFT planes where they fit: 0 no, 1 yes
   int tune_ride = -1;              // real-space sweep inside the spreading / gather launches
+  int tune_rgrid = 1024;           // workgroups of the real-space sweep's workgroup form (cap)
   std::string err;
 
   ~md_engine() {
@@ -878,9 +880,14 @@ int md_engine::rebuild() {
   const char *rsw = std::getenv("MD_REAL_SWEEP");
   if (rsw && !std::strcmp(rsw, "wave")) wg = 0;
   if (rsw && !std::strcmp(rsw, "wg")) wg = 1;
+  // the workgroup form's grid: about one non-empty item per workgroup, at
+  // most tune_rgrid workgroups (MD_REAL_GRID forces a cap); the items a
+  // workgroup takes do not change any sum
+  const char *rg = std::getenv("MD_REAL_GRID");
+  const int rcap = rg ? std::max(1, std::atoi(rg)) : tune_rgrid;
   for (Sys *x : sys) {
     x->h.real_wg = wg;
-    x->nb[10] = wg ? x->nb10_wg : x->nb10_wave;  // 0 where the list is walked
+    x->nb[10] = wg ? std::min(x->nb10_wg, rcap) : x->nb10_wave;  // 0 where the list is walked
   }
   const bool reps_host_wg = wg != 0;  // one form for every replica of the engine
   // z pass: few lines per workgroup (32-byte row pieces) while the engine's
@@ -1758,7 +1765,7 @@ extern "C" int md_autotune(md_engine *e, int32_t iters, int32_t *choice) {
     std::vector<int> options;
   };
   Knob knobs[] = {{&e->tune_fuse, {0, 1}}, {&e->tune_pz, {2, 4, 8, 16}}, {&e->tune_real, {0, 1}},
-                  {&e->tune_ride, {0, 1}}};
+                  {&e->tune_ride, {0, 1}}, {&e->tune_rgrid, {1024, 2048}}};
   for (Knob &k : knobs) {
     int best = *k.field;
     double best_ms = 1e30;
@@ -1784,6 +1791,7 @@ extern "C" int md_autotune(md_engine *e, int32_t iters, int32_t *choice) {
     choice[1] = e->tune_pz;
     choice[2] = e->tune_real;
     choice[3] = e->tune_ride;
+    choice[4] = e->tune_rgrid;
   }
   return rc;
 }

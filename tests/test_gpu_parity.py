@@ -311,13 +311,14 @@ def test_fft_plane_paths_agree(prec, monkeypatch):
 @pytest.mark.parametrize("prec", [MD_FP64, MD_FP32])
 def test_launch_forms_do_not_change_results(prec, monkeypatch):
     """Every launch form the autotuner may choose (fused planes or x/y
-    passes, z-pass lines per workgroup, sweep form) gives bitwise the same
-    Coulomb forces, and autotuning keeps them."""
+    passes, z-pass lines per workgroup, sweep form, the sweep's workgroup
+    count) gives bitwise the same Coulomb forces, and autotuning keeps
+    them."""
     print(f"[launch forms] default prec={prec}", flush=True)
     p, pos, vel = system("default", prec)
 
     def coulomb(env):
-        for k in ("MD_FFT_PLANES", "MD_Z_LINES", "MD_REAL_SWEEP", "MD_RIDE"):
+        for k in ("MD_FFT_PLANES", "MD_Z_LINES", "MD_REAL_SWEEP", "MD_RIDE", "MD_REAL_GRID"):
             monkeypatch.delenv(k, raising=False)
         for k, v in env.items():
             monkeypatch.setenv(k, v)
@@ -331,14 +332,15 @@ def test_launch_forms_do_not_change_results(prec, monkeypatch):
     e.close()
     for env in ({"MD_FFT_PLANES": "0", "MD_Z_LINES": "2"}, {"MD_Z_LINES": "16", "MD_REAL_SWEEP": "wave"},
                 {"MD_FFT_PLANES": "0", "MD_Z_LINES": "8", "MD_REAL_SWEEP": "wg"},
-                {"MD_RIDE": "0", "MD_REAL_SWEEP": "wg"}, {"MD_RIDE": "1", "MD_REAL_SWEEP": "wave"}):
+                {"MD_RIDE": "0", "MD_REAL_SWEEP": "wg"}, {"MD_RIDE": "1", "MD_REAL_SWEEP": "wave"},
+                {"MD_REAL_GRID": "2048"}, {"MD_REAL_GRID": "300", "MD_RIDE": "0"}):
         e, f = coulomb(env)
         e.close()
         assert np.array_equal(f, ref), env
     e, _ = coulomb({})
     choice = e.autotune(2)
     assert choice["z_lines"] in (2, 4, 8, 16) and choice["fused_planes"] in (0, 1)
-    assert choice["real_shared_launch"] in (0, 1)
+    assert choice["real_shared_launch"] in (0, 1) and choice["real_grid"] in (1024, 2048)
     assert np.array_equal(e.force_parts()[1], ref)
     e.close()
 
@@ -411,16 +413,18 @@ def test_error_analysis_matches_oracle():
 
 
 def test_error_analysis_default_fp32():
-    """At the default brush the FP32 engine's Coulomb force is within 1e-3
-    of the standard-Ewald reference, whose device value equals the oracle's;
-    the check leaves the trajectory as it would have been."""
+    """At the default brush the FP32 engine's Coulomb force is within 5e-4
+    of the standard-Ewald reference (the FP64 oracle measures 4.03e-4 there:
+    the 64x64x256 order-4 mesh's discretisation error), whose device value
+    equals the oracle's; the check leaves the trajectory as it would have
+    been."""
     print("[error_analysis] default fp32", flush=True)
     p, pos, vel, e = _engine("default", MD_FP32)
     g, fse = e.error_analysis()
     o = _oracle("default")
     r, fse_o, _ = o.error_analysis()
     assert orc.rel_rms(fse, fse_o) < 1e-10
-    assert g["rmsf"] < 1e-3
+    assert g["rmsf"] < 5e-4
     # the FP32 figure differs from the FP64 oracle's by the FP32 rounding only
     assert abs(g["rmsf"] - r["rmsf"]) < 0.1 * r["rmsf"]
     e2 = _engine("default", MD_FP32)[3]

@@ -16,6 +16,8 @@ constexpr int kMaxOrder = 8;
 constexpr int kTile = 16;         // spreading tile edge (mesh points)
 constexpr int kTileSub = 8;       // sub-lists per spreading tile (spreads the appends)
 constexpr int kPlaneBlock = 512;  // threads of the fused FFT plane kernels
+constexpr int kPlaneSpreadPer = 16;  // plane points per thread of k_plane_spread (K1 K2 <= 16 kPlaneBlock)
+constexpr int kPlaneSpreadTable = 272;  // LDS bytes of k_plane_spread's sub-list table, after the twiddles
 constexpr int kSRLanes = 4;       // lanes per particle in the LJ pass of the short-range kernel
 constexpr int kSRBlock = 512;     // its workgroup (LJ on 4 wavefronts, the per-particle
 constexpr int kSRParticles = 64;  // parts on the others) takes 64 particles
@@ -127,7 +129,9 @@ struct Rep {
   // spreading tiles: each tile lists the charges with spline points in it
   int tsz[3], trem[3], ntl[3];     // tiles per axis: the first trem of edge tsz + 1, then tsz
   int capt;                        // capacity of each of a tile's kTileSub sub-lists
-  int *tcnt, *tmem;                // counts; particle indices (tile, sub, slot)
+  int *tcnt, *tmem;                // counts; particle indices (list, sub, slot): a list per
+                                   // spreading tile, or per first z index (plane spreading)
+  int nlist;                       // counts to clear after a gather: max(tiles, K3) x kTileSub
   int plane_fused;                 // x and y FFT passes fused per z-plane
   void *spec;                      // half spectrum K3*K2*Kh complex
   void *meshu;                     // real potential mesh K3*K2*K1

@@ -142,7 +142,8 @@ struct Sys {
   int ncell1 = 0, ncell2 = 0, ncell3 = 0;
   int nb[kMaps] = {0};
   size_t lds_x = 0, lds_y = 0, lds_z = 0, lds_sort = 0, lds_sort3 = 0, lds_plane = 0;
-  int nbin = 0;  // spreading tiles
+  int nbin = 0;   // spreading tiles
+  int nlists = 0; // spreading lists: one per tile or one per z index, whichever is more
   int nb10_wave = 0, nb10_wg = 0;  // real-space sweep grids of the two forms (the
                                    // workgroup form's before the engine's cap)
   int pz_big = 1;                  // z-pass lines per workgroup in large engines
@@ -445,14 +446,18 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
       S.trem[d] = p.ng[d] % S.ntl[d];
       nbin *= S.ntl[d];
     }
-    S.tcnt = dalloc<int>((size_t)nbin * mdk::kTileSub);
+    // the same lists serve plane spreading, one per first z index
+    nlists = std::max(nbin, p.ng[2]);
+    S.nlist = nlists * mdk::kTileSub;
+    S.tcnt = dalloc<int>((size_t)nlists * mdk::kTileSub);
     S.tmem = nullptr;
     S.capt = 0;
     if (!S.tcnt) return fail(MD_ENOMEM, "spreading tiles");
     // x and y passes fused per z-plane when both planes fit in LDS
     // two regions of max(K2/2 * K1, Kh * K2) points (the FFT passes alternate)
-    // (plus the x and y twiddle tables, staged in LDS)
-    plane_bytes = (2 * mdk::plane_region(K1, K2, S.Kh) + K1 + K2) * csz;
+    // (plus the x and y twiddle tables, staged in LDS, and plane
+    // spreading's sub-list table)
+    plane_bytes = (2 * mdk::plane_region(K1, K2, S.Kh) + K1 + K2) * csz + mdk::kPlaneSpreadTable;
     lds_plane = plane_bytes;
     // MD_FFT_PLANES=0 keeps the separate x and y passes (for comparison)
     const char *fp = std::getenv("MD_FFT_PLANES");
@@ -673,6 +678,7 @@ int Sys::size_lists(const double *pos) {
   }
   if (coulomb) {  // spreading tile lists
     std::vector<int> cnt((size_t)nbin * mdk::kTileSub, 0);
+    std::vector<int> cntp((size_t)h.K[2] * mdk::kTileSub, 0);  // plane lists: first z index
     int mb = 0;
     for (int m = 0; m < (int)charge.size(); m++) {
       const int i = charge[m];
@@ -684,6 +690,7 @@ int Sys::size_lists(const double *pos) {
         b += b < 0 ? h.K[d] : 0;
         t0[d] = mdk::tile_of(h, d, b);
         nt[d] = mdk::tile_of(h, d, (b + h.ord[d] - 1) % h.K[d]) != t0[d] ? 2 : 1;
+        if (d == 2) mb = std::max(mb, ++cntp[(size_t)b * mdk::kTileSub + sub]);
       }
       for (int c = 0; c < nt[2]; c++)
         for (int b = 0; b < nt[1]; b++)
@@ -695,8 +702,8 @@ int Sys::size_lists(const double *pos) {
     }
     const int capt = round8(std::min(h.Nq, 2 * mb + 32));
     if (capt != h.capt) {
-      if (h.tmem) dfree(h.tmem, (size_t)nbin * mdk::kTileSub * h.capt * 4);
-      h.tmem = dalloc<int>((size_t)nbin * mdk::kTileSub * capt);
+      if (h.tmem) dfree(h.tmem, (size_t)nlists * mdk::kTileSub * h.capt * 4);
+      h.tmem = dalloc<int>((size_t)nlists * mdk::kTileSub * capt);
       h.capt = capt;
       if (!h.tmem) return fail(MD_ENOMEM, "spreading tiles");
     }
@@ -781,6 +788,7 @@ struct md_engine {
   int tune_fuse = -1;              // fused FFT planes where they fit: 0 no, 1 yes
   int tune_ride = -1;              // real-space sweep inside the spreading / gather launches
   int tune_rgrid = 1024;           // workgroups of the real-space sweep's workgroup form (cap)
+  int tune_pspread = 1;            // spreading inside the forward plane kernel: 0 no, 1 yes
   std::string err;
 
   ~md_engine() {
@@ -916,6 +924,21 @@ int md_engine::rebuild() {
     x->nb[4] = !S.plane_fused ? S.K[2] * cdiv(S.Kh, S.py_tile) : 0;
     x->nb[12] = S.plane_fused ? S.K[2] : 0;
   }
+  // spreading inside the forward plane kernel: when every replica with
+  // Coulomb forces has fused planes of at most kPlaneSpreadPer points per
+  // lane, and it is asked for (tuned; MD_PLANE_SPREAD=0/1 forces); the
+  // mesh is the same either way
+  bool ps_ok = false;
+  for (Sys *x : sys) {
+    if (!x->coulomb) continue;
+    ps_ok = true;
+    if (!x->h.plane_fused || x->h.K[0] * x->h.K[1] > mdk::kPlaneSpreadPer * mdk::kPlaneBlock) {
+      ps_ok = false;
+      break;
+    }
+  }
+  const char *ps = std::getenv("MD_PLANE_SPREAD");
+  const bool pspread = ps_ok && (ps ? ps[0] == '1' : tune_pspread > 0);
   std::vector<Rep> hr(R);
   for (int q = 0; q < 2; q++) {
     for (int r = 0; r < R; r++) hr[r] = sys[r]->table(q);
@@ -993,6 +1016,7 @@ int md_engine::rebuild() {
     // latency-bound (one system), not in batches (MD_RIDE=0 / =1 forces)
     const char *rd = std::getenv("MD_RIDE");
     P.ride = rd ? rd[0] != '0' : (tune_ride >= 0 ? tune_ride != 0 : !batch);
+    P.pspread = pspread;
     // MD_TWO_STREAMS=1: the real-space sweep beside the SPME kernels on a
     // second stream (the same results; slower in graph replay at the
     // default brush, so off unless asked for)
@@ -1765,7 +1789,7 @@ extern "C" int md_autotune(md_engine *e, int32_t iters, int32_t *choice) {
     std::vector<int> options;
   };
   Knob knobs[] = {{&e->tune_fuse, {0, 1}}, {&e->tune_pz, {2, 4, 8, 16}}, {&e->tune_real, {0, 1}},
-                  {&e->tune_ride, {0, 1}}, {&e->tune_rgrid, {1024, 2048}}};
+                  {&e->tune_ride, {0, 1}}, {&e->tune_rgrid, {1024, 2048}}, {&e->tune_pspread, {0, 1}}};
   for (Knob &k : knobs) {
     int best = *k.field;
     double best_ms = 1e30;
@@ -1792,6 +1816,7 @@ extern "C" int md_autotune(md_engine *e, int32_t iters, int32_t *choice) {
     choice[2] = e->tune_real;
     choice[3] = e->tune_ride;
     choice[4] = e->tune_rgrid;
+    choice[5] = e->tune_pspread;
   }
   return rc;
 }

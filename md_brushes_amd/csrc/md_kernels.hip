@@ -1062,6 +1062,11 @@ __global__ __launch_bounds__(kBlock) void k_build_rl(const Rep *__restrict__ rep
 // O > 0 fixes the order at compile time (all loops unroll); O = 0 takes p_.
 template <typename R, int O = 0>
 __device__ __forceinline__ int spline(double coord, double kg, int p_, R *w, R *dw) {
+  // evaluated as written (no fused multiply-adds): the weights are then the
+  // same bits in every kernel that inlines this, whatever the surrounding
+  // code lets the compiler contract (the tile and the plane spreading must
+  // give the same mesh)
+#pragma clang fp contract(off)
   const int p = O > 0 ? O : p_;
   const double uu = kg * coord;
   const double uuc = ceil(uu);
@@ -1616,28 +1621,13 @@ __global__ __launch_bounds__(kBlock) void k_ffty(const Rep *__restrict__ reps, W
 // complex FFT of length K1 each, split into two half spectra stored
 // line-major by kx, then the y FFT of each kx line.  Two LDS regions of
 // plane_region points: the passes alternate between them.
-template <typename R>
-__global__ __launch_bounds__(kPlaneBlock) void k_plane_fwd(const Rep *__restrict__ reps, WMap wm) {
-  using C = typename CT<R>::C;
-  extern __shared__ char smem[];
-  const int r = find_rep(wm, blockIdx.x);
-  const Rep &S = reps[r];
-  const int z = blockIdx.x - wm.blk[r];
-  const int K1 = S.K[0], K2 = S.K[1], Kh = S.Kh, H = (K2 + 1) / 2;  // row pairs (odd K2: one alone)
+// The forward x and y passes of z-plane z whose rows are in ra (pairs of
+// rows as complex lines), with the twiddles staged: the spectrum plane.
+template <typename R, typename C>
+__device__ __forceinline__ void plane_fwd_lds(const Rep &S, int z, C *ra, C *rb, const C *twx,
+                                              const C *twy) {
+  const int K1 = S.K[0], K2 = S.K[1], Kh = S.Kh, H = (K2 + 1) / 2;
   const int KP = plane_line_stride(K2);
-  C *ra = (C *)smem, *rb = ra + plane_region(K1, K2, Kh);
-  C *twx = rb + plane_region(K1, K2, Kh), *twy = twx + K1;
-  stage_twiddles(S, 0, twx);
-  stage_twiddles(S, 1, twy);
-  const R *q = (const R *)S.meshr + (size_t)z * K2 * K1;
-  for (int t = threadIdx.x; t < H * K1; t += blockDim.x) {
-    const int l = t / K1, x = t - l * K1;
-    C v;
-    v.x = q[(size_t)(2 * l) * K1 + x];
-    v.y = 2 * l + 1 < K2 ? q[(size_t)(2 * l + 1) * K1 + x] : (R)0;
-    ra[l * K1 + x] = v;
-  }
-  __syncthreads();
   const C *A = fft_axis<C, -1>(S, 0, twx, ra, rb, H);
   C *P = A == ra ? rb : ra;
   for (int t = threadIdx.x; t < H * Kh; t += blockDim.x) {
@@ -1658,6 +1648,173 @@ __global__ __launch_bounds__(kPlaneBlock) void k_plane_fwd(const Rep *__restrict
     const int y = t / Kh, k = t - y * Kh;
     spec[t] = Y[k * KP + y];
   }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kPlaneBlock) void k_plane_fwd(const Rep *__restrict__ reps, WMap wm) {
+  using C = typename CT<R>::C;
+  extern __shared__ char smem[];
+  const int r = find_rep(wm, blockIdx.x);
+  const Rep &S = reps[r];
+  const int z = blockIdx.x - wm.blk[r];
+  const int K1 = S.K[0], K2 = S.K[1], Kh = S.Kh, H = (K2 + 1) / 2;  // row pairs (odd K2: one alone)
+  C *ra = (C *)smem, *rb = ra + plane_region(K1, K2, Kh);
+  C *twx = rb + plane_region(K1, K2, Kh), *twy = twx + K1;
+  stage_twiddles(S, 0, twx);
+  stage_twiddles(S, 1, twy);
+  const R *q = (const R *)S.meshr + (size_t)z * K2 * K1;
+  for (int t = threadIdx.x; t < H * K1; t += blockDim.x) {
+    const int l = t / K1, x = t - l * K1;
+    C v;
+    v.x = q[(size_t)(2 * l) * K1 + x];
+    v.y = 2 * l + 1 < K2 ? q[(size_t)(2 * l + 1) * K1 + x] : (R)0;
+    ra[l * K1 + x] = v;
+  }
+  __syncthreads();
+  plane_fwd_lds<R>(S, z, ra, rb, twx, twy);
+}
+
+// One charge's products in z-plane z (weight c = z - its first z index)
+// into the plane's LDS accumulator (int64 fixed point, K1 x K2, x fastest):
+// the same products, in the same arithmetic, as tile_add's for that plane.
+template <typename R, int O>
+__device__ __forceinline__ void plane_add(unsigned long long *acc, double4 p, const double *kg,
+                                          const int *K, int z) {
+  R wx[kMaxOrder], wy[kMaxOrder], wz[kMaxOrder], d[kMaxOrder];
+  const int bx = wrapk(spline<R, O>(p.x, kg[0], O, wx, d), K[0]);
+  const int by = wrapk(spline<R, O>(p.y, kg[1], O, wy, d), K[1]);
+  const int bz = wrapk(spline<R, O>(p.z, kg[2], O, wz, d), K[2]);
+  const int c = wrapk(z - bz, K[2]);
+  if (c >= O) return;
+  const R q = (R)p.w;
+#pragma unroll
+  for (int b = 0; b < O; b++) {
+    int gy = by + b;
+    gy -= gy >= K[1] ? K[1] : 0;
+    unsigned long long *row = acc + gy * K[0];
+#pragma unroll
+    for (int a = 0; a < O; a++) {
+      int gx = bx + a;
+      gx -= gx >= K[0] ? K[0] : 0;
+      const double v = (double)(wz[c] * ((wx[a] * q) * wy[b]));
+      atomicAdd(row + gx, (unsigned long long)(long long)rint(v * kQScale));
+    }
+  }
+}
+
+// The same with an order per axis (as tile_add_axes).
+template <typename R>
+__device__ __forceinline__ void plane_add_axes(unsigned long long *acc, double4 p, const double *kg,
+                                               const int *K, const int *o, int z) {
+  R wx[kMaxOrder], wy[kMaxOrder], wz[kMaxOrder], d[kMaxOrder];
+  const int bx = wrapk(spline<R, 0>(p.x, kg[0], o[0], wx, d), K[0]);
+  const int by = wrapk(spline<R, 0>(p.y, kg[1], o[1], wy, d), K[1]);
+  const int bz = wrapk(spline<R, 0>(p.z, kg[2], o[2], wz, d), K[2]);
+  const int c = wrapk(z - bz, K[2]);
+  if (c >= o[2]) return;
+  const R q = (R)p.w;
+  for (int b = 0; b < o[1]; b++) {
+    int gy = by + b;
+    gy -= gy >= K[1] ? K[1] : 0;
+    unsigned long long *row = acc + gy * K[0];
+    for (int a = 0; a < o[0]; a++) {
+      int gx = bx + a;
+      gx -= gx >= K[0] ? K[0] : 0;
+      const double v = (double)(wz[c] * ((wx[a] * q) * wy[b]));
+      atomicAdd(row + gx, (unsigned long long)(long long)rint(v * kQScale));
+    }
+  }
+}
+
+// MapCharges (compt_acc.f90:1324-1365) fused with the forward x and y
+// passes, one workgroup per z-plane: the charges whose first z index is
+// one of the order's planes up to z (k_bin_q's plane lists) add their
+// products in this plane to an LDS accumulator in 2^-40 fixed point, which
+// then becomes the plane's rows, exactly the values the tile spreading
+// writes, so the spectrum is bitwise that of k_spread + k_plane_fwd.  The
+// accumulator shares the two FFT regions (it is read out into registers
+// before the rows are written); the charge mesh is also stored, for the
+// mesh energy.
+template <typename R>
+__global__ __launch_bounds__(kPlaneBlock) void k_plane_spread(const Rep *__restrict__ reps, WMap wm) {
+  using C = typename CT<R>::C;
+  extern __shared__ char smem[];
+  const int r = find_rep(wm, blockIdx.x);
+  const Rep &S = reps[r];
+  const int z = blockIdx.x - wm.blk[r];
+  const int K1 = S.K[0], K2 = S.K[1], Kh = S.Kh, H = (K2 + 1) / 2;
+  C *ra = (C *)smem, *rb = ra + plane_region(K1, K2, Kh);
+  C *twx = rb + plane_region(K1, K2, Kh), *twy = twx + K1;
+  unsigned long long *acc = (unsigned long long *)smem;  // K1 K2 words within ra and rb
+  const int npt = K1 * K2;
+  for (int k = threadIdx.x; k < npt; k += blockDim.x) acc[k] = 0ull;
+  stage_twiddles(S, 0, twx);
+  stage_twiddles(S, 1, twy);
+  __syncthreads();
+  // the lists of first z indices z - c, c = 0..order-1, each of kTileSub
+  // sub-lists: sub-list j = c kTileSub + u, its exclusive offset off[j]
+  // (wavefront 0: a lane per sub-list, then a scan)
+  // (the table follows the twiddles in the dynamic LDS: the kernel has no
+  // static LDS, which would shift the dynamic base off the 8-byte alignment
+  // of the accumulator's 64-bit atomics)
+  static_assert(kMaxOrder * kTileSub <= 64, "a lane per sub-list");
+  static_assert((kMaxOrder * kTileSub + 1) * sizeof(int) <= kPlaneSpreadTable, "the table's LDS");
+  int *off = (int *)(twy + K2);
+  const int oz = S.ord[2], K[3] = {S.K[0], S.K[1], S.K[2]}, nsub = oz * kTileSub;
+  if (threadIdx.x < 64) {
+    const int j = threadIdx.x;
+    const int c = j / kTileSub, u = j - c * kTileSub;
+    const int cnt = j < nsub ? min(S.tcnt[wrapk(z - c, S.K[2]) * kTileSub + u], S.capt) : 0;
+    int incl = cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int v = __shfl_up(incl, o, 64);
+      if (j >= o) incl += v;
+    }
+    if (j < nsub) off[j] = incl - cnt;
+    if (j == nsub - 1) off[nsub] = incl;
+  }
+  __syncthreads();
+  const int n = off[nsub];
+  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    int j = 0;  // the sub-list holding entry t: the largest j with off[j] <= t
+    for (int st = 32; st > 0; st >>= 1)
+      if (j + st < nsub && off[j + st] <= t) j += st;
+    const int c = j / kTileSub, u = j - c * kTileSub, lz = wrapk(z - c, S.K[2]);
+    const double4 p = S.pos[S.tmem[((size_t)lz * kTileSub + u) * S.capt + (t - off[j])]];
+    switch (S.order) {
+      case 3: plane_add<R, 3>(acc, p, S.kg, K, z); break;
+      case 4: plane_add<R, 4>(acc, p, S.kg, K, z); break;
+      case 5: plane_add<R, 5>(acc, p, S.kg, K, z); break;
+      case 6: plane_add<R, 6>(acc, p, S.kg, K, z); break;
+      case 7: plane_add<R, 7>(acc, p, S.kg, K, z); break;
+      case 8: plane_add<R, 8>(acc, p, S.kg, K, z); break;
+      default: plane_add_axes<R>(acc, p, S.kg, K, S.ord, z); break;
+    }
+  }
+  __syncthreads();
+  // read the accumulator out (every word before any row is written: they
+  // share the memory), store the charge mesh plane, then the rows as pairs
+  R val[kPlaneSpreadPer];
+#pragma unroll
+  for (int k = 0; k < kPlaneSpreadPer; k++) {
+    const int t = threadIdx.x + k * kPlaneBlock;
+    val[k] = t < npt ? (R)((double)(long long)acc[t] * kQInv) : (R)0;
+  }
+  __syncthreads();
+  R *qm = (R *)S.meshr + (size_t)z * K2 * K1;
+#pragma unroll
+  for (int k = 0; k < kPlaneSpreadPer; k++) {
+    const int t = threadIdx.x + k * kPlaneBlock;
+    if (t < npt) {
+      qm[t] = val[k];
+      const int y = t / K1, x = t - y * K1;
+      R *row = (R *)&ra[(y >> 1) * K1 + x];
+      row[y & 1] = val[k];
+      if ((K2 & 1) && y == K2 - 1) row[1] = (R)0;  // a lone last row
+    }
+  }
+  __syncthreads();
+  plane_fwd_lds<R>(S, z, ra, rb, twx, twy);
 }
 
 // Inverse y and x passes of one z-plane: inverse y FFT of each kx line,
@@ -1844,8 +2001,8 @@ __device__ __forceinline__ void gather_block(const Rep *__restrict__ reps, WMap 
   const int cz = threadIdx.x & (kGatherLanes - 1);
   const int m = blk * (kBlock / kGatherLanes) + threadIdx.x / kGatherLanes;
   const bool live = m < S.Nq;
-  {  // the tile lists are consumed: clear them for the next evaluation
-    const int ntile = S.ntl[0] * S.ntl[1] * S.ntl[2] * kTileSub;
+  {  // the tile (or plane) lists are consumed: clear them for the next evaluation
+    const int ntile = S.nlist;
     const int nth = (wm.blk[r + 1] - wm.blk[r]) * kBlock;
     for (int k = blk * kBlock + threadIdx.x; k < ntile; k += nth) S.tcnt[k] = 0;
   }
@@ -2070,7 +2227,7 @@ __global__ __launch_bounds__(kBlock) void k_real_list(const Rep *__restrict__ re
 
 // Charges into the lists of the spreading tiles that hold their spline
 // points (one to eight) and into the cells of the real-space sweep.
-__global__ __launch_bounds__(kBlock) void k_bin_q(const Rep *__restrict__ reps, WMap wm) {
+__global__ __launch_bounds__(kBlock) void k_bin_q(const Rep *__restrict__ reps, WMap wm, int planes) {
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
   const int m = (blockIdx.x - wm.blk[r]) * kBlock + threadIdx.x;
@@ -2081,7 +2238,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_q(const Rep *__restrict__ reps, 
     i = S.charge[m];
     p = S.pos[i];
   }
-  int t0[3] = {0, 0, 0}, nt[3] = {0, 0, 0};
+  int t0[3] = {0, 0, 0}, nt[3] = {0, 0, 0}, bz = 0;
   if (live) {
     const double c3[3] = {p.x, p.y, p.z};
 #pragma unroll
@@ -2092,7 +2249,10 @@ __global__ __launch_bounds__(kBlock) void k_bin_q(const Rep *__restrict__ reps, 
       int e = b + o - 1;
       e -= e >= S.K[d] ? S.K[d] : 0;
       nt[d] = tile_of(S, d, e) != t0[d] ? 2 : 1;
+      if (d == 2) bz = b;
     }
+    // plane spreading (k_plane_spread): one list per first z index instead
+    if (planes) nt[0] = nt[1] = nt[2] = 0;
   }
   // nine appends (up to eight tiles, the sweep cell): the runs first, then
   // every atomic issued before any result is used, so that their latencies
@@ -2109,6 +2269,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_q(const Rep *__restrict__ reps, 
       key[k] = ((tz * S.ntl[1] + ty) * S.ntl[0] + tx) * kTileSub + ((m >> 3) & (kTileSub - 1));
     }
   }
+  if (live && planes) key[0] = bz * kTileSub + ((m >> 3) & (kTileSub - 1));
   key[8] = -1;
   if (live && S.real_cells) {
     int a, b, c;
@@ -2315,6 +2476,12 @@ __device__ void real_cells_wg(const Rep &S, int b0, int nb, V *cand, short *hit_
       // stencil slot 13 is the cell itself: candidate offs[13] + t0 + tg is target tg
       const int self0 = offs[13] + t0;
       const V *tp = ps + (size_t)cell * S.ccap3 + t0;
+      // the item's targets, lane tg of every wavefront holding target tg:
+      // one load per item, read out per target (readlane), instead of a
+      // dependent global load before each target's tile
+      static_assert(kRealChunkWG <= 64, "an item's targets fit one wavefront's lanes");
+      V pt{};
+      if (lane < nt) pt = tp[lane];
       if (threadIdx.x < 3 * kRealChunkWG) accs[threadIdx.x / 3][threadIdx.x % 3] = 0.0;
       for (int base = 0; base < total; base += TN) {
         const int nc = min(TN, total - base);
@@ -2335,8 +2502,13 @@ __device__ void real_cells_wg(const Rep &S, int b0, int nb, V *cand, short *hit_
         }
         __syncthreads();
         for (int tg = w; tg < nt; tg += NW) {
+          V pi;
+          pi.x = lane_value(pt.x, tg);
+          pi.y = lane_value(pt.y, tg);
+          pi.z = lane_value(pt.z, tg);
+          pi.w = lane_value(pt.w, tg);
           R fx, fy, fz;
-          target_tile<R>(tp[tg], cand, wrap, nc, self0 + tg - base, hit, Lx, Ly, hLx, hLy, rcc2, alpha,
+          target_tile<R>(pi, cand, wrap, nc, self0 + tg - base, hit, Lx, Ly, hLx, hLy, rcc2, alpha,
                          alpha2, c2, fx, fy, fz);
           if (lane == 0) {
             accs[tg][0] += (double)fx;
@@ -2521,10 +2693,10 @@ __global__ __launch_bounds__(kBlock) void k_spread_real(const Rep *__restrict__ 
 
 template <typename R, bool WG>
 __global__ __launch_bounds__(kBlock) void k_gather_real(const Rep *__restrict__ reps, WMap wg,
-                                                        WMap wr) {
+                                                        WMap wr, int nparts) {
   __shared__ __align__(16) char lds[RealLDS<R>::bytes];
   if ((int)blockIdx.x < wg.total) gather_block<R>(reps, wg, blockIdx.x);
-  else real_cells_block<R, WG>(reps, wr, (int)blockIdx.x - wg.total, lds, 1, 2);
+  else real_cells_block<R, WG>(reps, wr, (int)blockIdx.x - wg.total, lds, nparts - 1, nparts);
 }
 
 // ---------------------------------------------------- rescale_velocity
@@ -2905,7 +3077,7 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) 
   // are sorted and joins before the store (timed launches stay in order)
   const bool fork = P.st2 && !ev && P.cell3.total;
   mark();
-  hipLaunchKernelGGL(k_bin_q, dim3(P.chg.total), dim3(kBlock), 0, st, P.reps, P.chg);
+  hipLaunchKernelGGL(k_bin_q, dim3(P.chg.total), dim3(kBlock), 0, st, P.reps, P.chg, P.pspread ? 1 : 0);
   mark();
   if (P.cell3.total)
     // many cells (batches): one wavefront per cell, whose bitonic stages'
@@ -2929,7 +3101,10 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) 
                          P.cell3c);
     (void)hipEventRecord(P.ev_join, P.st2);
   }
-  if (ride && P.real_wg)
+  // plane spreading: the spreading is the forward plane kernel's first
+  // part, and the whole sweep rides in the gather launch
+  if (P.pspread) {
+  } else if (ride && P.real_wg)
     hipLaunchKernelGGL((k_spread_real<R, true>), dim3(P.brick.total + nr), dim3(kBlock), 0, st, P.reps,
                        P.brick, P.cell3c);
   else if (ride)
@@ -2939,7 +3114,10 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) 
     hipLaunchKernelGGL(k_spread<R>, dim3(P.brick.total), dim3(kBlock), 0, st, P.reps, P.brick);
   mark();
   // replicas with fused planes take k_plane_*, the others the x and y passes
-  if (P.plane.total)
+  if (P.plane.total && P.pspread)
+    hipLaunchKernelGGL(k_plane_spread<R>, dim3(P.plane.total), dim3(kPlaneBlock), P.lds_plane, st,
+                       P.reps, P.plane);
+  else if (P.plane.total)
     hipLaunchKernelGGL(k_plane_fwd<R>, dim3(P.plane.total), dim3(kPlaneBlock), P.lds_plane, st, P.reps,
                        P.plane);
   if (P.fx.total)
@@ -2960,12 +3138,13 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) 
   if (P.fx.total)
     hipLaunchKernelGGL(k_fftx_inv<R>, dim3(P.fx.total), dim3(kBlock), P.lds_x, st, P.reps, P.fx);
   mark();
+  const int nparts = P.pspread ? 1 : 2;  // the gather launch's share of the sweep's items
   if (ride && P.real_wg)
     hipLaunchKernelGGL((k_gather_real<R, true>), dim3(P.chgG.total + nr), dim3(kBlock), 0, st, P.reps,
-                       P.chgG, P.cell3c);
+                       P.chgG, P.cell3c, nparts);
   else if (ride)
     hipLaunchKernelGGL((k_gather_real<R, false>), dim3(P.chgG.total + nr), dim3(kBlock), 0, st, P.reps,
-                       P.chgG, P.cell3c);
+                       P.chgG, P.cell3c, nparts);
   else
     hipLaunchKernelGGL(k_gather<R>, dim3(P.chgG.total), dim3(kBlock), 0, st, P.reps, P.chgG);
   mark();

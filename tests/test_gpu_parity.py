@@ -312,13 +312,13 @@ def test_fft_plane_paths_agree(prec, monkeypatch):
 def test_launch_forms_do_not_change_results(prec, monkeypatch):
     """Every launch form the autotuner may choose (fused planes or x/y
     passes, z-pass lines per workgroup, sweep form, the sweep's workgroup
-    count) gives bitwise the same Coulomb forces, and autotuning keeps
-    them."""
+    count, spreading inside the forward plane kernel) gives bitwise the same
+    Coulomb forces, and autotuning keeps them."""
     print(f"[launch forms] default prec={prec}", flush=True)
     p, pos, vel = system("default", prec)
 
     def coulomb(env):
-        for k in ("MD_FFT_PLANES", "MD_Z_LINES", "MD_REAL_SWEEP", "MD_RIDE", "MD_REAL_GRID"):
+        for k in ("MD_FFT_PLANES", "MD_Z_LINES", "MD_REAL_SWEEP", "MD_RIDE", "MD_REAL_GRID", "MD_PLANE_SPREAD"):
             monkeypatch.delenv(k, raising=False)
         for k, v in env.items():
             monkeypatch.setenv(k, v)
@@ -333,7 +333,9 @@ def test_launch_forms_do_not_change_results(prec, monkeypatch):
     for env in ({"MD_FFT_PLANES": "0", "MD_Z_LINES": "2"}, {"MD_Z_LINES": "16", "MD_REAL_SWEEP": "wave"},
                 {"MD_FFT_PLANES": "0", "MD_Z_LINES": "8", "MD_REAL_SWEEP": "wg"},
                 {"MD_RIDE": "0", "MD_REAL_SWEEP": "wg"}, {"MD_RIDE": "1", "MD_REAL_SWEEP": "wave"},
-                {"MD_REAL_GRID": "2048"}, {"MD_REAL_GRID": "300", "MD_RIDE": "0"}):
+                {"MD_REAL_GRID": "2048"}, {"MD_REAL_GRID": "300", "MD_RIDE": "0"},
+                {"MD_PLANE_SPREAD": "1"}, {"MD_PLANE_SPREAD": "1", "MD_RIDE": "0", "MD_REAL_SWEEP": "wave"},
+                {"MD_PLANE_SPREAD": "1", "MD_FFT_PLANES": "0"}):
         e, f = coulomb(env)
         e.close()
         assert np.array_equal(f, ref), env
@@ -341,6 +343,7 @@ def test_launch_forms_do_not_change_results(prec, monkeypatch):
     choice = e.autotune(2)
     assert choice["z_lines"] in (2, 4, 8, 16) and choice["fused_planes"] in (0, 1)
     assert choice["real_shared_launch"] in (0, 1) and choice["real_grid"] in (1024, 2048)
+    assert choice["plane_spreading"] in (0, 1)
     assert np.array_equal(e.force_parts()[1], ref)
     e.close()
 
@@ -678,6 +681,30 @@ def test_spme_any_mesh_and_orders(case, planes, monkeypatch):
         err = orc.rel_rms(fc, ref)
         print(f"  prec={prec} rel rms {err:.2e}")
         assert err < tol
+
+
+@pytest.mark.parametrize("case", sorted(MESHES))
+def test_plane_spreading_equals_tiles(case, monkeypatch):
+    """Spreading inside the forward plane kernel (a workgroup per z-plane
+    over the charges of the order's first z indices) gives bitwise the
+    Coulomb forces and energies of the tile spreading, on odd lengths,
+    prime factors and an order per axis."""
+    print(f"[plane spreading] {case}", flush=True)
+    name, ng, ordr = MESHES[case]
+    p, pos, vel = _mesh_variant(name, ng, ordr)
+    for prec in (MD_FP64, MD_FP32):
+        q = p.copy()
+        q.precision = prec
+        out = []
+        for flag in ("0", "1"):
+            monkeypatch.setenv("MD_PLANE_SPREAD", flag)
+            e = Engine(q)
+            e.upload(pos, vel)
+            e.build_lists()
+            out.append((e.force_parts()[1], e.energies()))
+            e.close()
+        assert np.array_equal(out[0][0], out[1][0]), prec
+        assert out[0][1] == out[1][1], prec
 
 
 @pytest.mark.parametrize("prec", [MD_FP64, MD_FP32])

@@ -1736,12 +1736,12 @@ __device__ __forceinline__ void plane_add_axes(unsigned long long *acc, double4 
 // before the rows are written); the charge mesh is also stored, for the
 // mesh energy.
 template <typename R>
-__global__ __launch_bounds__(kPlaneBlock) void k_plane_spread(const Rep *__restrict__ reps, WMap wm) {
+__device__ __forceinline__ void plane_spread_block(const Rep *__restrict__ reps, WMap wm, int gb,
+                                                   char *smem) {
   using C = typename CT<R>::C;
-  extern __shared__ char smem[];
-  const int r = find_rep(wm, blockIdx.x);
+  const int r = find_rep(wm, gb);
   const Rep &S = reps[r];
-  const int z = blockIdx.x - wm.blk[r];
+  const int z = gb - wm.blk[r];
   const int K1 = S.K[0], K2 = S.K[1], Kh = S.Kh, H = (K2 + 1) / 2;
   C *ra = (C *)smem, *rb = ra + plane_region(K1, K2, Kh);
   C *twx = rb + plane_region(K1, K2, Kh), *twy = twx + K1;
@@ -1817,6 +1817,12 @@ __global__ __launch_bounds__(kPlaneBlock) void k_plane_spread(const Rep *__restr
   plane_fwd_lds<R>(S, z, ra, rb, twx, twy);
 }
 
+template <typename R>
+__global__ __launch_bounds__(kPlaneBlock) void k_plane_spread(const Rep *__restrict__ reps, WMap wm) {
+  extern __shared__ char smem[];
+  plane_spread_block<R>(reps, wm, blockIdx.x, smem);
+}
+
 // Inverse y and x passes of one z-plane: inverse y FFT of each kx line,
 // pairs of rows Z = A + iB rebuilt over the full x range (A, B Hermitian),
 // inverse x FFT, real and imaginary parts are the two rows.
@@ -1874,15 +1880,14 @@ __global__ __launch_bounds__(kPlaneBlock) void k_plane_inv(const Rep *__restrict
 // forward FFT along z, multiply by BC, inverse FFT along z; tiles of Tz
 // adjacent (ky, kx) lines, contiguous in the half-spectrum layout.
 template <typename R>
-__global__ __launch_bounds__(kPlaneBlock) void k_fftz_conv(const Rep *__restrict__ reps, WMap wm) {
+__device__ __forceinline__ void fftz_block(const Rep *__restrict__ reps, WMap wm, int gb, char *smem) {
   using C = typename CT<R>::C;
-  extern __shared__ char smem[];
-  const int r = find_rep(wm, blockIdx.x);
+  const int r = find_rep(wm, gb);
   const Rep &S = reps[r];
   const int K3 = S.K[2], T = S.pz_tile, KS = zline_stride(K3);
   C *ba = (C *)smem, *bb = ba + (size_t)T * KS, *twz = bb + (size_t)T * KS;
   stage_twiddles(S, 2, twz);
-  const int f0 = (blockIdx.x - wm.blk[r]) * T;
+  const int f0 = (gb - wm.blk[r]) * T;
   const size_t zs = (size_t)S.K[1] * S.Kh;
   const int nl = min(T, (int)zs - f0);
   C *spec = (C *)S.spec + f0;
@@ -1907,6 +1912,12 @@ __global__ __launch_bounds__(kPlaneBlock) void k_fftz_conv(const Rep *__restrict
     const int z = t / nl, l = t - z * nl;
     spec[z * zs + l] = out[l * KS + z];
   }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kPlaneBlock) void k_fftz_conv(const Rep *__restrict__ reps, WMap wm) {
+  extern __shared__ char smem[];
+  fftz_block<R>(reps, wm, blockIdx.x, smem);
 }
 
 // One charge's reciprocal force from the potential mesh (IFrc,
@@ -2336,6 +2347,25 @@ template <> struct RealTile<float> { static constexpr int N = 512; };
 static_assert(RealTile<float>::N % kRealRound == 0 && RealTile<double>::N % kRealRound == 0,
               "tiles are padded to whole compaction rounds");
 
+// LDS layout of the sweep's workgroup form (real_cells_wg) in workgroups of
+// BS lanes: the candidate tile, a hit list per wavefront, the item's target
+// sums (double), the stencil offsets and cells, the work list, the
+// wavefronts' counts.
+template <typename R, int BS>
+struct RealWG {
+  using V = typename std::conditional<sizeof(R) == 4, float4, double4>::type;
+  static constexpr int TN = RealTile<R>::N, NW = BS / 64;
+  static constexpr size_t cand = 0;
+  static constexpr size_t hits = cand + (size_t)TN * sizeof(V);
+  static constexpr size_t accs = hits + (size_t)NW * TN * sizeof(short);
+  static constexpr size_t offs = accs + (size_t)kRealChunkWG * 3 * sizeof(double);
+  static constexpr size_t cids = offs + 28 * sizeof(int);
+  static constexpr size_t work = cids + 28 * sizeof(int);
+  static constexpr size_t wcnt = work + (size_t)BS * sizeof(int);
+  static constexpr size_t bytes = (wcnt + (size_t)NW * sizeof(int) + 15) / 16 * 16;
+  static_assert(accs % 8 == 0, "the target sums are doubles");
+};
+
 // Periodic image of stencil slot l's members for a target in cell (a, b),
 // with three or more cells of edge >= r_c along x and y: the x, y offset
 // that places them beside the target.  Each stencil cell is then a distinct
@@ -2432,17 +2462,22 @@ __device__ __forceinline__ void target_tile(const V pi, const V *cand, bool wrap
 // (target_tile each) and sum them over the tiles in LDS (double), in the
 // same order as the wavefront form.  Workgroups stride over the items
 // (chunk-major, as in the wavefront form) and skip the empty ones in bulk.
-template <typename R, typename V>
-__device__ void real_cells_wg(const Rep &S, int b0, int nb, V *cand, short *hit_all, int part,
-                              int nparts) {
-  constexpr int TN = RealTile<R>::N, NW = kBlock / 64;
+template <typename R, typename V, int BS>
+__device__ void real_cells_wg(const Rep &S, int b0, int nb, char *lds, int part, int nparts) {
+  constexpr int TN = RealTile<R>::N, NW = BS / 64;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nch = (S.ccap3 + kRealChunkWG - 1) / kRealChunkWG;
   const int ncl = S.nc3[0] * S.nc3[1] * S.nc3[2], nitem = ncl * nch;
-  __shared__ int offs[28], cids[27];
-  __shared__ int work[kBlock], wcnt[NW];
-  __shared__ double accs[kRealChunkWG][3];
-  short *hit = hit_all + w * TN;
+  // the workgroup's LDS (RealWG<R, BS>): the candidate tile, a hit list per
+  // wavefront, the target sums, the stencil table and the work list; no
+  // static LDS, so that kernels sharing a launch keep their dynamic LDS
+  // where they expect it
+  using L = RealWG<R, BS>;
+  V *cand = (V *)(lds + L::cand);
+  short *hit = (short *)(lds + L::hits) + w * TN;
+  double (*accs)[3] = (double(*)[3])(lds + L::accs);
+  int *offs = (int *)(lds + L::offs), *cids = (int *)(lds + L::cids);
+  int *work = (int *)(lds + L::work), *wcnt = (int *)(lds + L::wcnt);
   const V *ps = (const V *)S.pos3s;
   const R Lx = (R)S.Lx, Ly = (R)S.Ly, hLx = (R)S.hLx, hLy = (R)S.hLy, rcc2 = (R)S.rcc2;
   const bool wrap = !shifted_stencil(S.nc3);  // else the tiles are staged image-shifted
@@ -2450,7 +2485,7 @@ __device__ void real_cells_wg(const Rep &S, int b0, int nb, V *cand, short *hit_
   // the workgroup's items are b0 + j nb; this launch takes those with
   // j = part mod nparts, kBlock at a time, the non-empty ones compacted in order
   const int step = nb * nparts;
-  for (int it0 = b0 + part * nb; it0 < nitem; it0 += step * kBlock) {
+  for (int it0 = b0 + part * nb; it0 < nitem; it0 += step * BS) {
     const int item = it0 + threadIdx.x * step;
     bool ne = false;
     if (item < nitem) {
@@ -2486,7 +2521,7 @@ __device__ void real_cells_wg(const Rep &S, int b0, int nb, V *cand, short *hit_
       for (int base = 0; base < total; base += TN) {
         const int nc = min(TN, total - base);
         const int ncp = (nc + kRealRound - 1) / kRealRound * kRealRound;
-        for (int k = threadIdx.x; k < ncp; k += kBlock) {
+        for (int k = threadIdx.x; k < ncp; k += BS) {
           V c = tile_pad<V>();
           if (k < nc) {
             const int t = base + k, l = stencil_slot(offs, t);
@@ -2539,18 +2574,22 @@ __device__ void real_cells_wg(const Rep &S, int b0, int nb, V *cand, short *hit_
 // sum over the tiles and the lanes store their targets together.
 // Wavefronts stride over the items and skip the empty ones 64 at a time.
 template <typename R>
-struct RealLDS {  // LDS of a real-space workgroup: candidate tiles, then hit lists
+struct RealLDS {  // LDS of a real-space workgroup of kBlock lanes, either form:
+                  // the wavefront form's candidate tiles and hit lists
   using V = typename std::conditional<sizeof(R) == 4, float4, double4>::type;
   static constexpr int NW = kBlock / 64, TN = RealTile<R>::N;
-  static constexpr size_t bytes = (size_t)NW * TN * (sizeof(V) + sizeof(short));
+  static constexpr size_t wave = (size_t)NW * TN * (sizeof(V) + sizeof(short));
+  static constexpr size_t bytes = wave > RealWG<R, kBlock>::bytes ? wave : RealWG<R, kBlock>::bytes;
 };
 
-// Real-space workgroup gb of the sweep's block map (its LDS at lds).
-template <typename R, bool WG>
+// Real-space workgroup gb of the sweep's block map (its LDS at lds: at
+// least RealLDS<R>::bytes, or RealWG<R, BS>::bytes for the workgroup form).
+template <typename R, bool WG, int BS = kBlock>
 __device__ __forceinline__ void real_cells_block(const Rep *__restrict__ reps, WMap wm, int gb,
                                                  char *lds, int part = 0, int nparts = 1) {
   using V = typename RealLDS<R>::V;
-  constexpr int TN = RealTile<R>::N, NW = kBlock / 64;
+  static_assert(WG || BS == kBlock, "the wavefront form runs in kBlock workgroups");
+  constexpr int TN = RealTile<R>::N, NW = BS / 64;
   const int r = find_rep(wm, gb);
   const Rep &S = reps[r];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -2570,7 +2609,7 @@ __device__ __forceinline__ void real_cells_block(const Rep *__restrict__ reps, W
   const bool wrap = !shifted_stencil(S.nc3);  // else the tiles are staged image-shifted
   const R alpha = (R)S.alpha, alpha2 = (R)S.alpha2, c2 = (R)S.c2;
   if constexpr (WG) {
-    real_cells_wg<R>(S, gb - wm.blk[r], wm.blk[r + 1] - wm.blk[r], cand_all, hit_all, part, nparts);
+    real_cells_wg<R, V, BS>(S, gb - wm.blk[r], wm.blk[r + 1] - wm.blk[r], lds, part, nparts);
     return;
   }
   // the wavefront's items are gw + j nwv; this launch takes j = part mod nparts
@@ -2697,6 +2736,26 @@ __global__ __launch_bounds__(kBlock) void k_gather_real(const Rep *__restrict__ 
   __shared__ __align__(16) char lds[RealLDS<R>::bytes];
   if ((int)blockIdx.x < wg.total) gather_block<R>(reps, wg, blockIdx.x);
   else real_cells_block<R, WG>(reps, wr, (int)blockIdx.x - wg.total, lds, nparts - 1, nparts);
+}
+
+// With plane spreading the sweep's workgroup form (in workgroups of
+// kPlaneBlock lanes, its LDS in the launch's dynamic LDS) also rides in the
+// plane-spreading launch (part 0 of 3) and the z-pass launch (part 1); the
+// gather launch takes part 2.  Each of these workgroups runs one kind only.
+template <typename R>
+__global__ __launch_bounds__(kPlaneBlock) void k_plane_spread_real(const Rep *__restrict__ reps,
+                                                                   WMap wp, WMap wr) {
+  extern __shared__ char smem[];
+  if ((int)blockIdx.x < wp.total) plane_spread_block<R>(reps, wp, blockIdx.x, smem);
+  else real_cells_block<R, true, kPlaneBlock>(reps, wr, (int)blockIdx.x - wp.total, smem, 0, 3);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kPlaneBlock) void k_fftz_real(const Rep *__restrict__ reps, WMap wz,
+                                                           WMap wr) {
+  extern __shared__ char smem[];
+  if ((int)blockIdx.x < wz.total) fftz_block<R>(reps, wz, blockIdx.x, smem);
+  else real_cells_block<R, true, kPlaneBlock>(reps, wr, (int)blockIdx.x - wz.total, smem, 1, 3);
 }
 
 // ---------------------------------------------------- rescale_velocity
@@ -3113,8 +3172,15 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) 
   else
     hipLaunchKernelGGL(k_spread<R>, dim3(P.brick.total), dim3(kBlock), 0, st, P.reps, P.brick);
   mark();
-  // replicas with fused planes take k_plane_*, the others the x and y passes
-  if (P.plane.total && P.pspread)
+  // replicas with fused planes take k_plane_*, the others the x and y passes;
+  // with plane spreading the workgroup-form sweep rides a third each in the
+  // plane, z-pass and gather launches
+  const bool ride3 = ride && P.pspread && P.real_wg;
+  const size_t real512 = RealWG<R, kPlaneBlock>::bytes;
+  if (P.plane.total && ride3)
+    hipLaunchKernelGGL(k_plane_spread_real<R>, dim3(P.plane.total + nr), dim3(kPlaneBlock),
+                       P.lds_plane > real512 ? P.lds_plane : real512, st, P.reps, P.plane, P.cell3c);
+  else if (P.plane.total && P.pspread)
     hipLaunchKernelGGL(k_plane_spread<R>, dim3(P.plane.total), dim3(kPlaneBlock), P.lds_plane, st,
                        P.reps, P.plane);
   else if (P.plane.total)
@@ -3127,7 +3193,11 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) 
     hipLaunchKernelGGL((k_ffty<R, -1>), dim3(P.fy.total), dim3(kBlock), P.lds_y, st, P.reps, P.fy);
   mark();
   // 512 lanes: the strided line loads and stores spread over more wavefronts
-  hipLaunchKernelGGL(k_fftz_conv<R>, dim3(P.fz.total), dim3(kPlaneBlock), P.lds_z, st, P.reps, P.fz);
+  if (ride3)
+    hipLaunchKernelGGL(k_fftz_real<R>, dim3(P.fz.total + nr), dim3(kPlaneBlock),
+                       P.lds_z > real512 ? P.lds_z : real512, st, P.reps, P.fz, P.cell3c);
+  else
+    hipLaunchKernelGGL(k_fftz_conv<R>, dim3(P.fz.total), dim3(kPlaneBlock), P.lds_z, st, P.reps, P.fz);
   mark();
   if (P.fy.total)
     hipLaunchKernelGGL((k_ffty<R, +1>), dim3(P.fy.total), dim3(kBlock), P.lds_y, st, P.reps, P.fy);
@@ -3138,7 +3208,7 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) 
   if (P.fx.total)
     hipLaunchKernelGGL(k_fftx_inv<R>, dim3(P.fx.total), dim3(kBlock), P.lds_x, st, P.reps, P.fx);
   mark();
-  const int nparts = P.pspread ? 1 : 2;  // the gather launch's share of the sweep's items
+  const int nparts = ride3 ? 3 : (P.pspread ? 1 : 2);  // the gather launch takes the last part
   if (ride && P.real_wg)
     hipLaunchKernelGGL((k_gather_real<R, true>), dim3(P.chgG.total + nr), dim3(kBlock), 0, st, P.reps,
                        P.chgG, P.cell3c, nparts);

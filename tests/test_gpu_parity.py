@@ -335,7 +335,8 @@ def test_launch_forms_do_not_change_results(prec, monkeypatch):
                 {"MD_RIDE": "0", "MD_REAL_SWEEP": "wg"}, {"MD_RIDE": "1", "MD_REAL_SWEEP": "wave"},
                 {"MD_REAL_GRID": "2048"}, {"MD_REAL_GRID": "300", "MD_RIDE": "0"},
                 {"MD_PLANE_SPREAD": "1"}, {"MD_PLANE_SPREAD": "1", "MD_RIDE": "0", "MD_REAL_SWEEP": "wave"},
-                {"MD_PLANE_SPREAD": "1", "MD_FFT_PLANES": "0"}):
+                {"MD_PLANE_SPREAD": "1", "MD_REAL_SWEEP": "wave"}, {"MD_PLANE_SPREAD": "1", "MD_REAL_GRID": "300"},
+                {"MD_PLANE_SPREAD": "1", "MD_FFT_PLANES": "0"}, {"MD_PLANE_SPREAD": "0"}):
         e, f = coulomb(env)
         e.close()
         assert np.array_equal(f, ref), env

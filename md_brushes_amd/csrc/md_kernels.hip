@@ -2324,13 +2324,21 @@ __global__ __launch_bounds__(kBlock) void k_sort_q(const Rep *__restrict__ reps,
   const int n = min(raw, S.ccap3);
   extern __shared__ int sbuf[];
   int *mem = S.mem3 + (size_t)cell * S.ccap3;
-  sort_cell_lds(mem, n, sbuf);
-  // cell-sorted copy of the positions (the same thread wrote mem[k])
+  // each member's place in ascending order is the number of members below
+  // it (the members are distinct charges): one pass over the cell's members
+  // in LDS (broadcast reads), then the member and its position go straight
+  // to their sorted slots
+  for (int k = threadIdx.x; k < n; k += blockDim.x) sbuf[k] = mem[k];
+  __syncthreads();
   V *ps = (V *)S.pos3s + (size_t)cell * S.ccap3;
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
-    const int i = S.charge[mem[k]];
-    if constexpr (sizeof(R) == 4) ps[k] = S.pos32[i];
-    else ps[k] = S.pos[i];
+    const int m = sbuf[k];
+    int rank = 0;
+    for (int j = 0; j < n; j++) rank += sbuf[j] < m;
+    mem[rank] = m;
+    const int i = S.charge[m];
+    if constexpr (sizeof(R) == 4) ps[rank] = S.pos32[i];
+    else ps[rank] = S.pos[i];
   }
   if (threadIdx.x == 0) {
     S.scnt3[cell] = n;

@@ -246,6 +246,7 @@ def pmc_workload(seed):
 KERNEL_SYMBOLS = {  # kernel names of time_kernels -> device symbols (FP32 path)
     "kick_drift": "k_kick_drift", "short_range": "k_short_range<float>", "bin_q": "k_bin_q",
     "sort_q": "k_sort_q<float>", "spread": "k_spread<float>", "fft_x_fwd": "k_plane_fwd<float>",
+    "plane_spread": "k_plane_spread<float>",
     "fft_z_conv": "k_fftz_conv<float>", "fft_x_inv": "k_plane_inv<float>", "gather": "k_gather<float>",
     "real": ("k_real_cells<float", "k_coul_store<float>"), "bin_lj": "k_bin_lj", "build_lj": "k_build_lj"}
 
@@ -372,6 +373,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-obs", action="store_true", help="skip the FP32-vs-FP64 observable runs")
+    ap.add_argument("--no-geometry", action="store_true",
+                    help="the reference's geometry without the geometry autotuner (profiling runs)")
+    ap.add_argument("--no-ewald", action="store_true", help="skip the standard-Ewald accuracy check")
     ap.add_argument("--serve", type=int, default=256, help="requests in the serving leg (0: skip)")
     ap.add_argument("--serve-steps", type=int, default=10000)
     ap.add_argument("--serve-rate", type=float, default=128.0)
@@ -397,8 +401,9 @@ def main():
 
     state = prepare(args.seed, args.equil)
     # engine geometry by the autotuner: spreading tiles only (results unchanged)
-    geo32, tuner32 = autotune_geometry(config("default", MD_FP32, seed=args.seed), state[0], state[1],
-                                       cycles=TUNE_CYCLES)
+    geo32, tuner32 = config("default", MD_FP32, seed=args.seed), None
+    if not args.no_geometry:
+        geo32, tuner32 = autotune_geometry(geo32, state[0], state[1], cycles=TUNE_CYCLES)
     p32, e32, sps32, dt32 = time_engine(MD_FP32, state, args.steps, args.warmup, args.seed, geo32)
     out = {
         "metric": "steps/s at 21,333 particles (FP32 vs FP64, force err<=1e-4); replica p50/p99",
@@ -434,16 +439,18 @@ def main():
     out["fp32_force_rel_rms"] = float(np.sqrt(((f32 - f64) ** 2).sum() / (f64 ** 2).sum()))
     # the Coulomb force against standard Ewald (error_analysis, on the device):
     # the SPME set-up's own accuracy, FP64 and FP32
-    out["coulomb_rmsf_vs_ewald"] = {"fp64": e64.error_analysis()[0]["rmsf"],
-                                    "fp32": e32b.error_analysis()[0]["rmsf"]}
+    if not args.no_ewald:
+        out["coulomb_rmsf_vs_ewald"] = {"fp64": e64.error_analysis()[0]["rmsf"],
+                                        "fp32": e32b.error_analysis()[0]["rmsf"]}
     e32b.close()
     e64.close()
     if not args.no_obs:
         out["fp32_observables"] = observables(state, args.seed)
     if not args.no_fp64:
         n64 = min(args.steps, 4000)
-        geo64, _ = autotune_geometry(config("default", MD_FP64, seed=args.seed), state[0], state[1],
-                                     cycles=TUNE_CYCLES)
+        geo64 = config("default", MD_FP64, seed=args.seed)
+        if not args.no_geometry:
+            geo64, _ = autotune_geometry(geo64, state[0], state[1], cycles=TUNE_CYCLES)
         _, e64t, sps64, _ = time_engine(MD_FP64, state, n64, args.warmup, args.seed, geo64)
         out["fp64_steps_per_s"] = round(sps64, 2)
         e64t.close()
@@ -459,12 +466,13 @@ def main():
     e32.close()
     # the autotuner with the list knobs too (Verlet skin, sweep cell size): the
     # same pairs in another summation order, so reported beside the headline
-    geo_l, tuner_l = autotune_geometry(config("default", MD_FP32, seed=args.seed), state[0], state[1],
-                                       cycles=TUNE_CYCLES, lists=True)
-    out["tuned_lists"] = dict(geometry(geo_l), tuner_steps_per_s=round(tuner_l, 1),
-                              tuner_steps_per_s_tiles_only=round(tuner32, 1),
-                              note="md_autotune_geometry rates over 50 cycles after 10 warm-up cycles; "
-                                   "the list knobs change the summation order (forces to rounding)")
+    if not args.no_geometry:
+        geo_l, tuner_l = autotune_geometry(config("default", MD_FP32, seed=args.seed), state[0], state[1],
+                                           cycles=TUNE_CYCLES, lists=True)
+        out["tuned_lists"] = dict(geometry(geo_l), tuner_steps_per_s=round(tuner_l, 1),
+                                  tuner_steps_per_s_tiles_only=round(tuner32, 1),
+                                  note="md_autotune_geometry rates over 50 cycles after 10 warm-up cycles; "
+                                       "the list knobs change the summation order (forces to rounding)")
 
     if args.serve > 0:
         from md_brushes_amd.serve import Server, make_stream, sequential_fp64, summarize

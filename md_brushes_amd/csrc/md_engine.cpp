@@ -789,6 +789,7 @@ struct md_engine {
   int tune_ride = -1;              // real-space sweep inside the spreading / gather launches
   int tune_rgrid = 1024;           // workgroups of the real-space sweep's workgroup form (cap)
   int tune_pspread = 1;            // spreading inside the forward plane kernel: 0 no, 1 yes
+  int tune_rparts = 3;             // with plane spreading: launches the real-space sweep rides in
   std::string err;
 
   ~md_engine() {
@@ -1017,6 +1018,8 @@ int md_engine::rebuild() {
     const char *rd = std::getenv("MD_RIDE");
     P.ride = rd ? rd[0] != '0' : (tune_ride >= 0 ? tune_ride != 0 : !batch);
     P.pspread = pspread;
+    const char *rp = std::getenv("MD_RIDE_PARTS");  // forces the plane-spreading ride split
+    P.ride_parts = std::max(1, std::min(3, rp ? std::atoi(rp) : tune_rparts));
     // MD_TWO_STREAMS=1: the real-space sweep beside the SPME kernels on a
     // second stream (the same results; slower in graph replay at the
     // default brush, so off unless asked for)
@@ -1789,7 +1792,8 @@ extern "C" int md_autotune(md_engine *e, int32_t iters, int32_t *choice) {
     std::vector<int> options;
   };
   Knob knobs[] = {{&e->tune_fuse, {0, 1}}, {&e->tune_pz, {2, 4, 8, 16}}, {&e->tune_real, {0, 1}},
-                  {&e->tune_ride, {0, 1}}, {&e->tune_rgrid, {1024, 2048}}, {&e->tune_pspread, {0, 1}}};
+                  {&e->tune_ride, {0, 1}}, {&e->tune_rgrid, {1024, 2048}}, {&e->tune_pspread, {0, 1}},
+                  {&e->tune_rparts, {1, 2, 3}}};
   for (Knob &k : knobs) {
     int best = *k.field;
     double best_ms = 1e30;
@@ -1817,6 +1821,7 @@ extern "C" int md_autotune(md_engine *e, int32_t iters, int32_t *choice) {
     choice[3] = e->tune_ride;
     choice[4] = e->tune_rgrid;
     choice[5] = e->tune_pspread;
+    choice[6] = e->tune_rparts;
   }
   return rc;
 }

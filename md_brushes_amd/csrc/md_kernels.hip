@@ -2747,23 +2747,24 @@ __global__ __launch_bounds__(kBlock) void k_gather_real(const Rep *__restrict__ 
 }
 
 // With plane spreading the sweep's workgroup form (in workgroups of
-// kPlaneBlock lanes, its LDS in the launch's dynamic LDS) also rides in the
-// plane-spreading launch (part 0 of 3) and the z-pass launch (part 1); the
-// gather launch takes part 2.  Each of these workgroups runs one kind only.
+// kPlaneBlock lanes, its LDS in the launch's dynamic LDS) can also ride in
+// the plane-spreading launch (part 0) and the z-pass launch (part 1 of 3);
+// the gather launch takes the last part.  Each of these workgroups runs
+// one kind only.
 template <typename R>
 __global__ __launch_bounds__(kPlaneBlock) void k_plane_spread_real(const Rep *__restrict__ reps,
-                                                                   WMap wp, WMap wr) {
+                                                                   WMap wp, WMap wr, int nparts) {
   extern __shared__ char smem[];
   if ((int)blockIdx.x < wp.total) plane_spread_block<R>(reps, wp, blockIdx.x, smem);
-  else real_cells_block<R, true, kPlaneBlock>(reps, wr, (int)blockIdx.x - wp.total, smem, 0, 3);
+  else real_cells_block<R, true, kPlaneBlock>(reps, wr, (int)blockIdx.x - wp.total, smem, 0, nparts);
 }
 
 template <typename R>
 __global__ __launch_bounds__(kPlaneBlock) void k_fftz_real(const Rep *__restrict__ reps, WMap wz,
-                                                           WMap wr) {
+                                                           WMap wr, int nparts) {
   extern __shared__ char smem[];
   if ((int)blockIdx.x < wz.total) fftz_block<R>(reps, wz, blockIdx.x, smem);
-  else real_cells_block<R, true, kPlaneBlock>(reps, wr, (int)blockIdx.x - wz.total, smem, 1, 3);
+  else real_cells_block<R, true, kPlaneBlock>(reps, wr, (int)blockIdx.x - wz.total, smem, 1, nparts);
 }
 
 // ---------------------------------------------------- rescale_velocity
@@ -3181,13 +3182,15 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) 
     hipLaunchKernelGGL(k_spread<R>, dim3(P.brick.total), dim3(kBlock), 0, st, P.reps, P.brick);
   mark();
   // replicas with fused planes take k_plane_*, the others the x and y passes;
-  // with plane spreading the workgroup-form sweep rides a third each in the
-  // plane, z-pass and gather launches
-  const bool ride3 = ride && P.pspread && P.real_wg;
+  // with plane spreading the workgroup-form sweep's items are dealt to
+  // P.ride_parts launches: the gather's alone (1), the plane and the gather
+  // launches (2), or the plane, z-pass and gather launches (3)
+  const int rparts = ride && P.pspread && P.real_wg ? P.ride_parts : (ride && P.pspread ? 1 : 0);
   const size_t real512 = RealWG<R, kPlaneBlock>::bytes;
-  if (P.plane.total && ride3)
+  if (P.plane.total && rparts >= 2)
     hipLaunchKernelGGL(k_plane_spread_real<R>, dim3(P.plane.total + nr), dim3(kPlaneBlock),
-                       P.lds_plane > real512 ? P.lds_plane : real512, st, P.reps, P.plane, P.cell3c);
+                       P.lds_plane > real512 ? P.lds_plane : real512, st, P.reps, P.plane, P.cell3c,
+                       rparts);
   else if (P.plane.total && P.pspread)
     hipLaunchKernelGGL(k_plane_spread<R>, dim3(P.plane.total), dim3(kPlaneBlock), P.lds_plane, st,
                        P.reps, P.plane);
@@ -3201,9 +3204,9 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) 
     hipLaunchKernelGGL((k_ffty<R, -1>), dim3(P.fy.total), dim3(kBlock), P.lds_y, st, P.reps, P.fy);
   mark();
   // 512 lanes: the strided line loads and stores spread over more wavefronts
-  if (ride3)
+  if (rparts == 3)
     hipLaunchKernelGGL(k_fftz_real<R>, dim3(P.fz.total + nr), dim3(kPlaneBlock),
-                       P.lds_z > real512 ? P.lds_z : real512, st, P.reps, P.fz, P.cell3c);
+                       P.lds_z > real512 ? P.lds_z : real512, st, P.reps, P.fz, P.cell3c, rparts);
   else
     hipLaunchKernelGGL(k_fftz_conv<R>, dim3(P.fz.total), dim3(kPlaneBlock), P.lds_z, st, P.reps, P.fz);
   mark();
@@ -3216,7 +3219,7 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) 
   if (P.fx.total)
     hipLaunchKernelGGL(k_fftx_inv<R>, dim3(P.fx.total), dim3(kBlock), P.lds_x, st, P.reps, P.fx);
   mark();
-  const int nparts = ride3 ? 3 : (P.pspread ? 1 : 2);  // the gather launch takes the last part
+  const int nparts = P.pspread ? rparts : 2;  // the gather launch takes the last part
   if (ride && P.real_wg)
     hipLaunchKernelGGL((k_gather_real<R, true>), dim3(P.chgG.total + nr), dim3(kBlock), 0, st, P.reps,
                        P.chgG, P.cell3c, nparts);

@@ -318,7 +318,8 @@ def test_launch_forms_do_not_change_results(prec, monkeypatch):
     p, pos, vel = system("default", prec)
 
     def coulomb(env):
-        for k in ("MD_FFT_PLANES", "MD_Z_LINES", "MD_REAL_SWEEP", "MD_RIDE", "MD_REAL_GRID", "MD_PLANE_SPREAD"):
+        for k in ("MD_FFT_PLANES", "MD_Z_LINES", "MD_REAL_SWEEP", "MD_RIDE", "MD_REAL_GRID", "MD_PLANE_SPREAD",
+                  "MD_RIDE_PARTS"):
             monkeypatch.delenv(k, raising=False)
         for k, v in env.items():
             monkeypatch.setenv(k, v)
@@ -336,7 +337,8 @@ def test_launch_forms_do_not_change_results(prec, monkeypatch):
                 {"MD_REAL_GRID": "2048"}, {"MD_REAL_GRID": "300", "MD_RIDE": "0"},
                 {"MD_PLANE_SPREAD": "1"}, {"MD_PLANE_SPREAD": "1", "MD_RIDE": "0", "MD_REAL_SWEEP": "wave"},
                 {"MD_PLANE_SPREAD": "1", "MD_REAL_SWEEP": "wave"}, {"MD_PLANE_SPREAD": "1", "MD_REAL_GRID": "300"},
-                {"MD_PLANE_SPREAD": "1", "MD_FFT_PLANES": "0"}, {"MD_PLANE_SPREAD": "0"}):
+                {"MD_PLANE_SPREAD": "1", "MD_FFT_PLANES": "0"}, {"MD_PLANE_SPREAD": "0"},
+                {"MD_PLANE_SPREAD": "1", "MD_RIDE_PARTS": "1"}, {"MD_PLANE_SPREAD": "1", "MD_RIDE_PARTS": "2"}):
         e, f = coulomb(env)
         e.close()
         assert np.array_equal(f, ref), env
@@ -344,7 +346,7 @@ def test_launch_forms_do_not_change_results(prec, monkeypatch):
     choice = e.autotune(2)
     assert choice["z_lines"] in (2, 4, 8, 16) and choice["fused_planes"] in (0, 1)
     assert choice["real_shared_launch"] in (0, 1) and choice["real_grid"] in (1024, 2048)
-    assert choice["plane_spreading"] in (0, 1)
+    assert choice["plane_spreading"] in (0, 1) and choice["ride_parts"] in (1, 2, 3)
     assert np.array_equal(e.force_parts()[1], ref)
     e.close()
 

@@ -174,13 +174,23 @@ def test_initial_configuration_is_valid(name):
     assert np.allclose(np.sqrt((d ** 2).sum(1)), p.R_bond)
     assert abs(pos[:, 3].sum()) < 1e-12  # neutral
     assert np.all(pos[charge[: p.Nq_PE] - 1, 3] == p.qq)
-    # no two beads closer than the rejection radius of the growth
-    x = pos[:, :3]
-    dd = x[:, None] - x[None]
-    dd[..., 0] -= p.Lx * np.round(dd[..., 0] / p.Lx)
-    dd[..., 1] -= p.Ly * np.round(dd[..., 1] / p.Ly)
-    r2 = (dd ** 2).sum(-1) + np.eye(p.NN) * 10
-    assert r2.min() >= 0.7 - 1e-12
+    # no two beads closer than the rejection radius of the growth: every pair
+    # within the radius, found by a periodic k-d tree in x and y (z open: its
+    # period is larger than any distance), then the minimum image squared
+    # distance of each such pair as the growth computes it
+    from scipy.spatial import cKDTree
+    x = pos[:, :3].copy()
+    x[:, 0] = np.mod(x[:, 0] + p.Lx / 2, p.Lx)
+    x[:, 1] = np.mod(x[:, 1] + p.Ly / 2, p.Ly)
+    x[:, 0][x[:, 0] >= p.Lx] = 0.0  # a rounding of mod(-tiny, L) to L
+    x[:, 1][x[:, 1] >= p.Ly] = 0.0
+    box = [p.Lx, p.Ly, 4 * (p.Lz + 1)]
+    pairs = cKDTree(x, boxsize=box).query_pairs(np.sqrt(0.7) + 1e-6, output_type="ndarray")
+    if len(pairs):
+        dd = pos[pairs[:, 0], :3] - pos[pairs[:, 1], :3]
+        dd[:, 0] -= p.Lx * np.round(dd[:, 0] / p.Lx)
+        dd[:, 1] -= p.Ly * np.round(dd[:, 1] / p.Ly)
+        assert (dd ** 2).sum(1).min() >= 0.7 - 1e-12
 
 
 def test_initial_configuration_is_seeded():

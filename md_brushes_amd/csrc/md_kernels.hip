@@ -120,10 +120,6 @@ __device__ __forceinline__ double rsq_image(double4 a, double4 b, double Lx, dou
   return dx * dx + dy * dy + dz * dz;
 }
 
-__device__ __forceinline__ double wave_max(double v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  return v;
-}
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
@@ -170,6 +166,21 @@ template <typename T> __device__ __forceinline__ T wave_total(T v) {
   return lane_value(v, 63);
 }
 
+// Maximum over the 64 lanes of non-negative values (displacements), every
+// lane taking it: the DPP moves of wave_total with fmax (lanes a move
+// leaves out read 0, which no maximum of non-negative values takes; fmax
+// passes over NaN).  The maximum does not depend on the order of the
+// moves.
+__device__ __forceinline__ double wave_max_nonneg(double v) {
+  v = fmax(v, dpp_move<0xb1, 0xf>(v));
+  v = fmax(v, dpp_move<0x4e, 0xf>(v));
+  v = fmax(v, dpp_move<0x141, 0xf>(v));
+  v = fmax(v, dpp_move<0x140, 0xf>(v));
+  v = fmax(v, dpp_move<0x142, 0xa>(v));
+  v = fmax(v, dpp_move<0x143, 0xc>(v));
+  return lane_value(v, 63);
+}
+
 // Orders LDS accesses of the lanes of one wavefront (LDS operations of a
 // wavefront complete in order; the fences keep the compiler from moving
 // memory accesses across).
@@ -182,14 +193,16 @@ __device__ __forceinline__ void wave_sync() {
 // ------------------------------------------------------------- integrator
 
 // First half kick of new_position (init_update.f90:581-589), mass 1.
-__device__ __forceinline__ double half_kick(const Rep &S, double a, double v) {
+template <typename K>
+__device__ __forceinline__ double half_kick(const K &S, double a, double v) {
 #pragma clang fp contract(off)
   return a * S.dt / 2 + S.c1m * v;
 }
 
 // Drift and period_condition_pos (init_update.f90:594-601, 631-642): x += v dt,
 // x and y wrapped to (-L/2, L/2]; returns |v dt|.
-__device__ __forceinline__ double drift(const Rep &S, double4 &p, double vx, double vy, double vz) {
+template <typename K>
+__device__ __forceinline__ double drift(const K &S, double4 &p, double vx, double vy, double vz) {
 #pragma clang fp contract(off)
   const double dt = S.dt;
   const double dx = vx * dt, dy = vy * dt, dz = vz * dt;
@@ -206,9 +219,10 @@ __device__ __forceinline__ double drift(const Rep &S, double4 &p, double vx, dou
 
 // Maximum displacement of one wavefront's 64 consecutive particles (group g)
 // into slot q of the group maxima; every lane calls it.
-__device__ __forceinline__ void dmax_group(const Rep &S, int q, int g, double d1, double d2) {
-  d1 = wave_max(d1);
-  d2 = wave_max(d2);
+template <typename K>
+__device__ __forceinline__ void dmax_group(const K &S, int q, int g, double d1, double d2) {
+  d1 = wave_max_nonneg(d1);
+  d2 = wave_max_nonneg(d2);
   if ((threadIdx.x & 63) == 0 && g < S.ngrp) S.dpart[(size_t)q * S.ngrp + g] = make_double2(d1, d2);
 }
 
@@ -297,6 +311,55 @@ __device__ __forceinline__ void noise_normals(const Rep &S, int i, int step, dou
 
 __device__ __forceinline__ void bin_lj_particle(const Rep *Sp, int i, int buf);
 
+// What the state wavefront of k_short_range uses after the workgroup barrier:
+// the replica's step constants and pointers, read from the descriptor
+// before the barrier (a descriptor field first read after it costs that
+// wavefront a memory round trip of its own, on the kernel's critical path;
+// hold() keeps the compiler from moving the loads past the barrier).
+struct StepK {
+  double dt, c1m, c1p, sig, EF, Lx, Ly, hLx, hLy;
+  double *vx, *vy, *vz, *ax, *ay, *az;
+  double4 *pos_nx;
+  float4 *pos32_nx;
+  double2 *dpart;
+  Ctrl *ctrl;
+  int NN, ngrp, par;
+  __device__ __forceinline__ void hold() const {
+    asm volatile("" ::"s"(dt), "s"(c1m), "s"(c1p), "s"(sig), "s"(EF), "s"(Lx), "s"(Ly), "s"(hLx),
+                 "s"(hLy));
+    asm volatile("" ::"s"(vx), "s"(vy), "s"(vz), "s"(ax), "s"(ay), "s"(az), "s"(pos_nx),
+                 "s"(pos32_nx), "s"(dpart), "s"(ctrl));
+    asm volatile("" ::"s"(NN), "s"(ngrp), "s"(par));
+  }
+};
+
+__device__ __forceinline__ StepK step_constants(const Rep &S) {
+  StepK k;
+  k.dt = S.dt;
+  k.c1m = S.c1m;
+  k.c1p = S.c1p;
+  k.sig = S.sig;
+  k.EF = S.EF;
+  k.Lx = S.Lx;
+  k.Ly = S.Ly;
+  k.hLx = S.hLx;
+  k.hLy = S.hLy;
+  k.vx = S.vx;
+  k.vy = S.vy;
+  k.vz = S.vz;
+  k.ax = S.ax;
+  k.ay = S.ay;
+  k.az = S.az;
+  k.pos_nx = S.pos_nx;
+  k.pos32_nx = S.pos32_nx;
+  k.dpart = S.dpart;
+  k.ctrl = S.ctrl;
+  k.NN = S.NN;
+  k.ngrp = S.ngrp;
+  k.par = S.par;
+  return k;
+}
+
 // Compute_Force without the Coulomb evaluation (compt_acc.f90:847-861):
 // held Coulomb force + lj_force (compt_acc.f90:895-977) + fene_force
 // (compt_acc.f90:865-892) + noise + field, anchors zeroed; optionally the
@@ -335,6 +398,7 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
   double4 pi = make_double4(0, 0, 0, 0);
   double vx = 0, vy = 0, vz = 0, ax0 = 0, ay0 = 0, az0 = 0, cx = 0, cy = 0, cz = 0;
   int anchor = 0;
+  StepK K;
   if (w < 4) {  // WCA-LJ over the full list: class by |q_i|+|q_j| (compt_acc.f90:922-952)
     const int g = threadIdx.x & (G - 1), pl = threadIdx.x / G;
     const int ip = blk * NP + pl;
@@ -490,6 +554,7 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
       wzs[lane] = wall_force(S, p);
     }
   } else if (w == 6) {  // the particle's state, for after the barrier
+    K = step_constants(S);
     if (live) {
       pi = S.pos[i];
       anchor = S.flags[i] & 1;
@@ -515,6 +580,7 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
         }
       }
     }
+    K.hold();
   } else {  // wave 7
     // the next step's rebuild queue (its list kernels of two steps back are done)
     if ((mode & SR_DRIFT) && blockIdx.x == 0 && lane == 0) *lq_nx = 0ull;
@@ -525,11 +591,15 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
     const bool every = (mode & SR_BIN) != 0;
     if ((mode & SR_DRMAX) && (every || blk == 0)) {  // this step's displacements, all groups
       Ctrl *c = S.ctrl;
-      // the accumulated maxima load beside the group maxima, and the group
-      // maxima eight per lane at a time: the decision is one memory round
-      // trip, shorter than the LJ sums the barrier waits for (no workgroup
-      // of this launch writes either)
+      // the accumulated maxima, the rebuild generation and the skins load
+      // beside the group maxima, and the group maxima eight per lane at a
+      // time: the decision is one memory round trip, shorter than the LJ
+      // sums the barrier waits for (no workgroup of this launch writes any
+      // of them)
       const double m1 = c->dr_max1, m2 = c->dr_max2;
+      const int gen1 = c->gen1;
+      const double skin1 = S.skin1, skin2 = S.skin2;
+      const int rverlet = S.real_verlet;
       double d1 = 0, d2 = 0;
       const double2 *dp = S.dpart + (size_t)S.par * S.ngrp;
       constexpr int U = 8;
@@ -546,13 +616,13 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
           d2 = fmax(d2, v[u].y);
         }
       }
-      d1 = wave_max(d1);
-      d2 = wave_max(d2);
+      d1 = wave_max_nonneg(d1);
+      d2 = wave_max_nonneg(d2);
       if (every) {
         const double a1 = m1 + d1, a2 = m2 + d2;
-        const int f1 = a1 > S.skin1;
-        const int f2 = (a2 > S.skin2) && S.real_verlet != 0;
-        const int buf = (c->gen1 + 1) & 1;  // the next generation's count buffer
+        const int f1 = a1 > skin1;
+        const int f2 = (a2 > skin2) && rverlet != 0;
+        const int buf = (gen1 + 1) & 1;  // the next generation's count buffer
         if (f1) bin_lj_particle(&S, live ? i : -1, buf);
         if (blk == 0 && lane == 0) {
           c->pend_d1 = d1;
@@ -565,8 +635,8 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
       } else if (lane == 0) {
         c->dr_max1 = m1 + d1;
         c->dr_max2 = m2 + d2;
-        int f1 = c->dr_max1 > S.skin1;
-        int f2 = (c->dr_max2 > S.skin2) && S.real_verlet != 0;
+        int f1 = c->dr_max1 > skin1;
+        int f2 = (c->dr_max2 > skin2) && rverlet != 0;
         if (f1) {
           c->dr_max1 = 0;
           c->gen1++;
@@ -591,48 +661,48 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
       az += cz;
     }
     if (mode & (SR_NOISE_ETA | SR_NOISE_PHILOX)) {
-      ax += nzs[0][lane] * S.sig;
-      ay += nzs[1][lane] * S.sig;
-      az += nzs[2][lane] * S.sig;
+      ax += nzs[0][lane] * K.sig;
+      ay += nzs[1][lane] * K.sig;
+      az += nzs[2][lane] * K.sig;
     }
-    if (mode & SR_FIELD) az += pi.w * S.EF;
+    if (mode & SR_FIELD) az += pi.w * K.EF;
     if ((mode & SR_ANCHOR) && anchor) ax = ay = az = 0;
-    if (!(isfinite(ax) && isfinite(ay) && isfinite(az))) set_error(S.ctrl, MD_ENONFINITE, i, step);
+    if (!(isfinite(ax) && isfinite(ay) && isfinite(az))) set_error(K.ctrl, MD_ENONFINITE, i, step);
     if (out) {
       out[i] = ax;
-      out[(size_t)S.NN + i] = ay;
-      out[2 * (size_t)S.NN + i] = az;
+      out[(size_t)K.NN + i] = ay;
+      out[2 * (size_t)K.NN + i] = az;
     } else {
-      S.ax[i] = ax;
-      S.ay[i] = ay;
-      S.az[i] = az;
+      K.ax[i] = ax;
+      K.ay[i] = ay;
+      K.az[i] = az;
       if (mode & SR_KICK) {
 #pragma clang fp contract(off)
         if (mode & SR_DRIFT) {  // v holds v(n-1): redo this step's first half kick
-          vx = half_kick(S, ax0, vx);
-          vy = half_kick(S, ay0, vy);
-          vz = half_kick(S, az0, vz);
+          vx = half_kick(K, ax0, vx);
+          vy = half_kick(K, ay0, vy);
+          vz = half_kick(K, az0, vz);
         }
-        const double dt = S.dt;
-        vx = (vx + ax * dt / 2.) / S.c1p;
-        vy = (vy + ay * dt / 2.) / S.c1p;
-        vz = (vz + az * dt / 2.) / S.c1p;
-        S.vx[i] = vx;
-        S.vy[i] = vy;
-        S.vz[i] = vz;
+        const double dt = K.dt;
+        vx = (vx + ax * dt / 2.) / K.c1p;
+        vy = (vy + ay * dt / 2.) / K.c1p;
+        vz = (vz + az * dt / 2.) / K.c1p;
+        K.vx[i] = vx;
+        K.vy[i] = vy;
+        K.vz[i] = vz;
         if (mode & SR_DRIFT) {  // the next step's first half kick and drift, into the other buffer
           double4 p = pi;
           const double dr =
-              drift(S, p, half_kick(S, ax, vx), half_kick(S, ay, vy), half_kick(S, az, vz));
-          S.pos_nx[i] = p;
-          if (S.pos32_nx) S.pos32_nx[i] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
+              drift(K, p, half_kick(K, ax, vx), half_kick(K, ay, vy), half_kick(K, az, vz));
+          K.pos_nx[i] = p;
+          if (K.pos32_nx) K.pos32_nx[i] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
           d1 = dr;
           if (p.w != 0) d2 = dr;
         }
       }
     }
   }
-  if (mode & SR_DRIFT) dmax_group(S, S.par ^ 1, blk, d1, d2);
+  if (mode & SR_DRIFT) dmax_group(K, K.par ^ 1, blk, d1, d2);
 }
 
 // ------------------------------------------------------- Verlet lists
@@ -695,6 +765,7 @@ __device__ __forceinline__ int stencil_cell(int l, int a, int b, int c, const in
 // Binning pass (compt_acc.f90:219-225): cell of each particle, appended to
 // the cell's members (order fixed by the rank pass).  Also clears the
 // other count buffer at the particle's previous cell, for the next build.
+
 // Particle i of replica *Sp (i < 0: none) into count buffer buf; every lane
 // of the wavefront calls it.  Also clears the other buffer at the
 // particle's previous cell.
@@ -1742,7 +1813,7 @@ __device__ __forceinline__ void plane_spread_block(const Rep *__restrict__ reps,
   const int r = find_rep(wm, gb);
   const Rep &S = reps[r];
   const int z = gb - wm.blk[r];
-  const int K1 = S.K[0], K2 = S.K[1], Kh = S.Kh, H = (K2 + 1) / 2;
+  const int K1 = S.K[0], K2 = S.K[1], Kh = S.Kh;
   C *ra = (C *)smem, *rb = ra + plane_region(K1, K2, Kh);
   C *twx = rb + plane_region(K1, K2, Kh), *twy = twx + K1;
   unsigned long long *acc = (unsigned long long *)smem;  // K1 K2 words within ra and rb

@@ -28,6 +28,7 @@ constexpr int kRealRound = 256;   // real-space candidates tested per compaction
 constexpr int kLJStage = 256;     // LJ fill pass: candidates staged in order per target
 constexpr int kHistBins = 500;  // SizeHist of the z-density profile (glob_var.f90:59)
 constexpr int kMaxFac = 24;     // FFT passes of a mesh length that is not a power of two
+constexpr int kXCD = 8;         // MI355X accelerator dies: workgroup b of a launch runs on die b % 8
 
 // Per-replica control block, written by kernels.
 struct Ctrl {

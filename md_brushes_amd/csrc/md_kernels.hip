@@ -1950,6 +1950,21 @@ __global__ __launch_bounds__(kPlaneBlock) void k_plane_inv(const Rep *__restrict
 // z pass with the convolution of pmeOrthoConvBC (compt_acc.f90:1382-1396):
 // forward FFT along z, multiply by BC, inverse FFT along z; tiles of Tz
 // adjacent (ky, kx) lines, contiguous in the half-spectrum layout.
+// Die-grouped order of n work units: the workgroups of a launch are handed
+// to the kXCD dies in turn (workgroup b to die b % kXCD), and each die has
+// its own L2.  Local workgroups j, j + kXCD, ..., j + (G - 1) kXCD, which
+// share a die, take G consecutive units, so that units reading neighbouring
+// pieces of one cache line read it through one L2.  A permutation of
+// [0, n): whole runs of G kXCD workgroups are regrouped, the rest keep
+// their order.
+template <int G>
+__device__ __forceinline__ int die_grouped(int j, int n) {
+  constexpr int run = G * kXCD;
+  if (j >= n / run * run) return j;
+  const int s = j / kXCD, x = j % kXCD;
+  return (s / G) * run + x * G + s % G;
+}
+
 template <typename R>
 __device__ __forceinline__ void fftz_block(const Rep *__restrict__ reps, WMap wm, int gb, char *smem) {
   using C = typename CT<R>::C;
@@ -1958,7 +1973,10 @@ __device__ __forceinline__ void fftz_block(const Rep *__restrict__ reps, WMap wm
   const int K3 = S.K[2], T = S.pz_tile, KS = zline_stride(K3);
   C *ba = (C *)smem, *bb = ba + (size_t)T * KS, *twz = bb + (size_t)T * KS;
   stage_twiddles(S, 2, twz);
-  const int f0 = (gb - wm.blk[r]) * T;
+  // four line groups per die: 32 lines of 8-byte complex values (FP32
+  // path, 8 lines a group) are two whole 128-byte pieces of each z-row of
+  // the spectrum and one of the influence function
+  const int f0 = die_grouped<4>(gb - wm.blk[r], wm.blk[r + 1] - wm.blk[r]) * T;
   const size_t zs = (size_t)S.K[1] * S.Kh;
   const int nl = min(T, (int)zs - f0);
   C *spec = (C *)S.spec + f0;

@@ -190,6 +190,26 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Loads one word of each 64-byte line of a replica's descriptor with scalar
+// loads issued together, waiting once: the scalar data cache then holds the
+// whole descriptor, and the kernel's later reads of its fields (which the
+// compiler issues where they are used, each a wait of its own) hit it
+// instead of going to memory one after another.
+__device__ __forceinline__ void desc_prefetch(const Rep &S) {
+  static_assert(sizeof(Rep) <= 24 * 64, "the descriptor spans at most 24 lines");
+  constexpr int L = (int)((sizeof(Rep) + 63) / 64);
+  const unsigned *q = (const unsigned *)&S;
+  unsigned v[24];
+#pragma unroll
+  for (int k = 0; k < 24; k++) v[k] = k < L ? q[16 * k] : 0u;
+  // every word an operand of one statement: the loads are all in flight
+  // before the one wait
+  asm volatile("" ::"s"(v[0]), "s"(v[1]), "s"(v[2]), "s"(v[3]), "s"(v[4]), "s"(v[5]), "s"(v[6]),
+               "s"(v[7]), "s"(v[8]), "s"(v[9]), "s"(v[10]), "s"(v[11]), "s"(v[12]), "s"(v[13]),
+               "s"(v[14]), "s"(v[15]), "s"(v[16]), "s"(v[17]), "s"(v[18]), "s"(v[19]), "s"(v[20]),
+               "s"(v[21]), "s"(v[22]), "s"(v[23]));
+}
+
 // ------------------------------------------------------------- integrator
 
 // First half kick of new_position (init_update.f90:581-589), mass 1.
@@ -1572,6 +1592,56 @@ __device__ __forceinline__ void stage_twiddles(const Rep &S, int d, C *dst) {
   for (int k = threadIdx.x; k < S.K[d]; k += blockDim.x) dst[k] = tw[k];
 }
 
+// Stores st(t, ld(t)) for t = threadIdx.x, threadIdx.x + blockDim.x, ...
+// below n, in batches of U elements per thread whose loads are all issued
+// before the batch's first store: a global-to-LDS copy then waits one memory
+// round trip per batch, not one per element (a plain loop waits for each
+// load before its store).
+template <int U, typename Ld, typename St>
+__device__ __forceinline__ void copy_batched(int n, Ld ld, St st) {
+  using T = decltype(ld(0));
+  for (int t0 = threadIdx.x; t0 < n; t0 += U * (int)blockDim.x) {
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int t = t0 + u * (int)blockDim.x;
+      if (t < n) v[u] = ld(t);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int t = t0 + u * (int)blockDim.x;
+      if (t < n) st(t, v[u]);
+    }
+  }
+}
+
+// The x and y (or one axis') twiddle tables of a plane or line kernel,
+// loaded before the kernel's own copy so that both wait in the same memory
+// round trip: one element per thread where the table fits the workgroup,
+// else staged by stage_twiddles.  stage() stores what load() fetched.
+template <typename C>
+struct TwPair {
+  C a, b;
+  bool one;
+  __device__ __forceinline__ void load(const Rep &S, int da, int db) {
+    const int k = threadIdx.x;
+    one = S.K[da] <= (int)blockDim.x && (db < 0 || S.K[db] <= (int)blockDim.x);
+    a = b = C{};
+    if (one && k < S.K[da]) a = ((const C *)S.tw[da])[k];
+    if (one && db >= 0 && k < S.K[db]) b = ((const C *)S.tw[db])[k];
+  }
+  __device__ __forceinline__ void stage(const Rep &S, int da, C *dst_a, int db, C *dst_b) const {
+    const int k = threadIdx.x;
+    if (one) {
+      if (k < S.K[da]) dst_a[k] = a;
+      if (db >= 0 && k < S.K[db]) dst_b[k] = b;
+    } else {
+      stage_twiddles(S, da, dst_a);
+      if (db >= 0) stage_twiddles(S, db, dst_b);
+    }
+  }
+};
+
 // Forward x pass: pairs of real mesh rows as one complex FFT of length K1,
 // split into two half spectra.
 template <typename R>
@@ -1731,16 +1801,20 @@ __global__ __launch_bounds__(kPlaneBlock) void k_plane_fwd(const Rep *__restrict
   const int K1 = S.K[0], K2 = S.K[1], Kh = S.Kh, H = (K2 + 1) / 2;  // row pairs (odd K2: one alone)
   C *ra = (C *)smem, *rb = ra + plane_region(K1, K2, Kh);
   C *twx = rb + plane_region(K1, K2, Kh), *twy = twx + K1;
-  stage_twiddles(S, 0, twx);
-  stage_twiddles(S, 1, twy);
+  TwPair<C> tw;
+  tw.load(S, 0, 1);
   const R *q = (const R *)S.meshr + (size_t)z * K2 * K1;
-  for (int t = threadIdx.x; t < H * K1; t += blockDim.x) {
-    const int l = t / K1, x = t - l * K1;
-    C v;
-    v.x = q[(size_t)(2 * l) * K1 + x];
-    v.y = 2 * l + 1 < K2 ? q[(size_t)(2 * l + 1) * K1 + x] : (R)0;
-    ra[l * K1 + x] = v;
-  }
+  copy_batched<4>(
+      H * K1,
+      [&](int t) {
+        const int l = t / K1, x = t - l * K1;
+        C v;
+        v.x = q[(size_t)(2 * l) * K1 + x];
+        v.y = 2 * l + 1 < K2 ? q[(size_t)(2 * l + 1) * K1 + x] : (R)0;
+        return v;
+      },
+      [&](int t, C v) { ra[t] = v; });
+  tw.stage(S, 0, twx, 1, twy);
   __syncthreads();
   plane_fwd_lds<R>(S, z, ra, rb, twx, twy);
 }
@@ -1818,10 +1892,9 @@ __device__ __forceinline__ void plane_spread_block(const Rep *__restrict__ reps,
   C *twx = rb + plane_region(K1, K2, Kh), *twy = twx + K1;
   unsigned long long *acc = (unsigned long long *)smem;  // K1 K2 words within ra and rb
   const int npt = K1 * K2;
+  TwPair<C> tw;
+  tw.load(S, 0, 1);
   for (int k = threadIdx.x; k < npt; k += blockDim.x) acc[k] = 0ull;
-  stage_twiddles(S, 0, twx);
-  stage_twiddles(S, 1, twy);
-  __syncthreads();
   // the lists of first z indices z - c, c = 0..order-1, each of kTileSub
   // sub-lists: sub-list j = c kTileSub + u, its exclusive offset off[j]
   // (wavefront 0: a lane per sub-list, then a scan)
@@ -1844,7 +1917,8 @@ __device__ __forceinline__ void plane_spread_block(const Rep *__restrict__ reps,
     if (j < nsub) off[j] = incl - cnt;
     if (j == nsub - 1) off[nsub] = incl;
   }
-  __syncthreads();
+  tw.stage(S, 0, twx, 1, twy);
+  __syncthreads();  // the accumulator is clear, the table and twiddles staged
   const int n = off[nsub];
   for (int t = threadIdx.x; t < n; t += blockDim.x) {
     int j = 0;  // the sub-list holding entry t: the largest j with off[j] <= t
@@ -1908,13 +1982,16 @@ __global__ __launch_bounds__(kPlaneBlock) void k_plane_inv(const Rep *__restrict
   const int KP = plane_line_stride(K2);
   C *ra = (C *)smem, *rb = ra + plane_region(K1, K2, Kh);
   C *twx = rb + plane_region(K1, K2, Kh), *twy = twx + K1;
-  stage_twiddles(S, 0, twx);
-  stage_twiddles(S, 1, twy);
+  TwPair<C> tw;
+  tw.load(S, 0, 1);
   const C *spec = (const C *)S.spec + (size_t)z * K2 * Kh;
-  for (int t = threadIdx.x; t < K2 * Kh; t += blockDim.x) {
-    const int y = t / Kh, k = t - y * Kh;
-    ra[k * KP + y] = spec[t];
-  }
+  copy_batched<8>(
+      K2 * Kh, [&](int t) { return spec[t]; },
+      [&](int t, C v) {
+        const int y = t / Kh, k = t - y * Kh;
+        ra[k * KP + y] = v;
+      });
+  tw.stage(S, 0, twx, 1, twy);
   __syncthreads();
   const C *P = fft_axis<C, +1>(S, 1, twy, ra, rb, Kh, KP);
   C *A = P == ra ? rb : ra;
@@ -1972,7 +2049,8 @@ __device__ __forceinline__ void fftz_block(const Rep *__restrict__ reps, WMap wm
   const Rep &S = reps[r];
   const int K3 = S.K[2], T = S.pz_tile, KS = zline_stride(K3);
   C *ba = (C *)smem, *bb = ba + (size_t)T * KS, *twz = bb + (size_t)T * KS;
-  stage_twiddles(S, 2, twz);
+  TwPair<C> tw;
+  tw.load(S, 2, -1);
   // four line groups per die: 32 lines of 8-byte complex values (FP32
   // path, 8 lines a group) are two whole 128-byte pieces of each z-row of
   // the spectrum and one of the influence function
@@ -1981,25 +2059,83 @@ __device__ __forceinline__ void fftz_block(const Rep *__restrict__ reps, WMap wm
   const int nl = min(T, (int)zs - f0);
   C *spec = (C *)S.spec + f0;
   const R *bc = (const R *)S.bc + f0;
-  for (int t = threadIdx.x; t < nl * K3; t += blockDim.x) {
-    const int z = t / nl, l = t - z * nl;
-    ba[l * KS + z] = spec[z * zs + l];
+  // Thread-fixed lines when the group is whole and the workgroup holds a
+  // whole number of its rows (the usual case): a thread keeps line
+  // l = tid % nl and steps through z by blockDim / nl, with no division per
+  // element; its influence-function values load with the spectrum and stay
+  // in registers until the multiply (kZPer per thread at most).  Otherwise
+  // element t of the tile is (z, l) = (t / nl, t % nl).
+  constexpr int kZPer = 8;
+  const int zstep = (int)blockDim.x / nl;
+  const bool fixed = nl == T && zstep * nl == (int)blockDim.x && K3 <= kZPer * zstep;
+  R g[kZPer];
+  if (fixed) {
+    const int l = threadIdx.x % nl, z0 = threadIdx.x / nl;
+    C v[kZPer];
+#pragma unroll
+    for (int u = 0; u < kZPer; u++) {
+      const int z = z0 + u * zstep;
+      if (z < K3) {
+        v[u] = spec[(size_t)z * zs + l];
+        g[u] = bc[(size_t)z * zs + l];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kZPer; u++) {
+      const int z = z0 + u * zstep;
+      if (z < K3) ba[l * KS + z] = v[u];
+    }
+  } else {
+    copy_batched<8>(
+        nl * K3,
+        [&](int t) {
+          const int z = t / nl, l = t - z * nl;
+          return spec[z * zs + l];
+        },
+        [&](int t, C v) {
+          const int z = t / nl, l = t - z * nl;
+          ba[l * KS + z] = v;
+        });
   }
+  tw.stage(S, 2, twz, -1, nullptr);
   __syncthreads();
   C *buf = fft_axis<C, -1>(S, 2, twz, ba, bb, nl, KS);
-  for (int t = threadIdx.x; t < nl * K3; t += blockDim.x) {
-    const int z = t / nl, l = t - z * nl;
-    const R g = bc[z * zs + l];
-    C v = buf[l * KS + z];
-    v.x *= g;
-    v.y *= g;
-    buf[l * KS + z] = v;
+  if (fixed) {
+    const int l = threadIdx.x % nl, z0 = threadIdx.x / nl;
+#pragma unroll
+    for (int u = 0; u < kZPer; u++) {
+      const int z = z0 + u * zstep;
+      if (z < K3) {
+        C v = buf[l * KS + z];
+        v.x *= g[u];
+        v.y *= g[u];
+        buf[l * KS + z] = v;
+      }
+    }
+  } else {
+    for (int t = threadIdx.x; t < nl * K3; t += blockDim.x) {
+      const int z = t / nl, l = t - z * nl;
+      const R gg = bc[z * zs + l];
+      C v = buf[l * KS + z];
+      v.x *= gg;
+      v.y *= gg;
+      buf[l * KS + z] = v;
+    }
   }
   __syncthreads();
   const C *out = fft_axis<C, +1>(S, 2, twz, buf, buf == ba ? bb : ba, nl, KS);
-  for (int t = threadIdx.x; t < nl * K3; t += blockDim.x) {
-    const int z = t / nl, l = t - z * nl;
-    spec[z * zs + l] = out[l * KS + z];
+  if (fixed) {
+    const int l = threadIdx.x % nl, z0 = threadIdx.x / nl;
+#pragma unroll
+    for (int u = 0; u < kZPer; u++) {
+      const int z = z0 + u * zstep;
+      if (z < K3) spec[(size_t)z * zs + l] = out[l * KS + z];
+    }
+  } else {
+    for (int t = threadIdx.x; t < nl * K3; t += blockDim.x) {
+      const int z = t / nl, l = t - z * nl;
+      spec[z * zs + l] = out[l * KS + z];
+    }
   }
 }
 
@@ -2330,6 +2466,7 @@ __global__ __launch_bounds__(kBlock) void k_real_list(const Rep *__restrict__ re
 __global__ __launch_bounds__(kBlock) void k_bin_q(const Rep *__restrict__ reps, WMap wm, int planes) {
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
+  desc_prefetch(S);
   const int m = (blockIdx.x - wm.blk[r]) * kBlock + threadIdx.x;
   const bool live = m < S.Nq;
   int i = 0;
@@ -2339,7 +2476,9 @@ __global__ __launch_bounds__(kBlock) void k_bin_q(const Rep *__restrict__ reps, 
     p = S.pos[i];
   }
   int t0[3] = {0, 0, 0}, nt[3] = {0, 0, 0}, bz = 0;
-  if (live) {
+  if (live && planes) {  // plane spreading (k_plane_spread): one list per first z index
+    bz = spline_base(p.z, S.kg[2], S.ord[2], S.K[2]);
+  } else if (live) {  // the spreading tiles of the charge's spline points
     const double c3[3] = {p.x, p.y, p.z};
 #pragma unroll
     for (int d = 0; d < 3; d++) {
@@ -2349,10 +2488,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_q(const Rep *__restrict__ reps, 
       int e = b + o - 1;
       e -= e >= S.K[d] ? S.K[d] : 0;
       nt[d] = tile_of(S, d, e) != t0[d] ? 2 : 1;
-      if (d == 2) bz = b;
     }
-    // plane spreading (k_plane_spread): one list per first z index instead
-    if (planes) nt[0] = nt[1] = nt[2] = 0;
   }
   // nine appends (up to eight tiles, the sweep cell): the runs first, then
   // every atomic issued before any result is used, so that their latencies
@@ -2376,19 +2512,27 @@ __global__ __launch_bounds__(kBlock) void k_bin_q(const Rep *__restrict__ reps, 
     if (cell_of(p, S.nc3, S.rcel3, S.hLx, S.hLy, a, b, c)) key[8] = (c * S.nc3[1] + b) * S.nc3[0] + a;
     else set_error(S.ctrl, MD_ECELL, i, -1);
   }
+  // the appends in use: one plane list (plane spreading) or up to eight
+  // tiles, and the sweep cell; the keys of the others are empty in every
+  // lane (uniform: planes and real_cells are the launch's and the
+  // replica's), so their run detection and slot broadcast are skipped
+  const int nk = planes ? 1 : 8;
+  const bool cells = S.real_cells != 0;
   int *ctr[9];
   WaveRun run[9];
   int base[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) {
     ctr[k] = key[k] < 0 ? nullptr : (k < 8 ? S.tcnt : S.cnt3) + key[k];
-    run[k] = wave_run(ctr[k]);
+    run[k] = WaveRun{0, 0, false};
+    if (k < nk || (k == 8 && cells)) run[k] = wave_run(ctr[k]);
   }
 #pragma unroll
   for (int k = 0; k < 9; k++) base[k] = run[k].head && ctr[k] ? atomicAdd(ctr[k], run[k].n) : 0;
   const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int k = 0; k < 9; k++) {
+    if (!(k < nk || (k == 8 && cells))) continue;
     const int slot = __shfl(base[k], run[k].h, 64) + (lane - run[k].h);
     if (!ctr[k]) continue;
     if (k < 8) {

@@ -1671,6 +1671,12 @@ extern "C" int md_time_kernels(md_engine *e, int32_t iters, double *ms, int32_t 
       return e->fail(MD_ENOMEM, "time_kernels snapshot");
     }
   }
+  // the copies' first failure (a failed restore would leave the engine's
+  // state changed: the pass then reports it)
+  hipError_t copy_err = hipSuccess;
+  auto note = [&](hipError_t x) {
+    if (copy_err == hipSuccess) copy_err = x;
+  };
   auto keep = [&](bool restore, bool timing_ctrl) {
     for (size_t k = 0; k < R; k++) {
       char *h = snap[k];
@@ -1678,15 +1684,15 @@ extern "C" int md_time_kernels(md_engine *e, int32_t iters, double *ms, int32_t 
         if (pt.dev) {
           const void *src = restore ? (const void *)h : pt.dev;
           if (restore && timing_ctrl && pt.dev == e->sys[k]->h.ctrl) src = tctrl[k];
-          (void)hipMemcpyAsync(restore ? pt.dev : (void *)h, src, pt.n, hipMemcpyDeviceToDevice,
-                               e->st);
+          note(hipMemcpyAsync(restore ? pt.dev : (void *)h, src, pt.n, hipMemcpyDeviceToDevice,
+                              e->st));
         }
         h += rnd(pt.n);
       }
     }
     char *lq = snap[R];
-    if (restore) (void)hipMemcpyAsync(e->d_lq, lq, e->lq_words * 8, hipMemcpyDeviceToDevice, e->st);
-    else (void)hipMemcpyAsync(lq, e->d_lq, e->lq_words * 8, hipMemcpyDeviceToDevice, e->st);
+    if (restore) note(hipMemcpyAsync(e->d_lq, lq, e->lq_words * 8, hipMemcpyDeviceToDevice, e->st));
+    else note(hipMemcpyAsync(lq, e->d_lq, e->lq_words * 8, hipMemcpyDeviceToDevice, e->st));
   };
   keep(false, false);
   for (size_t k = 0; k < R; k++) {  // the control blocks the timed launches start from
@@ -1748,6 +1754,7 @@ extern "C" int md_time_kernels(md_engine *e, int32_t iters, double *ms, int32_t 
   for (char *x : snap) (void)hipFree(x);
   for (auto &x : ev) (void)hipEventDestroy(x);
   for (int k = 0; k < 15; k++) ms[k] = acc[k] / iters;
+  if (rc == MD_OK && copy_err != hipSuccess) return e->hip_fail(copy_err, "time_kernels state copies");
   return rc;
 }
 

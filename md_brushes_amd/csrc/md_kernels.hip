@@ -886,6 +886,100 @@ __device__ __forceinline__ int stencil_slot(const int *offs, int t) {
 // Pairs: i != j and r^2 < rvl^2, r^2 rounded as rij_and_rr rounds it.  The
 // list is ELL: slot k of particle i at k*NN + i.  The last kernel of a step
 // also advances every replica's step counter.
+// One target of the fill pass (below): particle i of replica S (live: a
+// target at all), into count generation buf; the 32 lanes of half-wave
+// `half` of the wavefront take it.
+__device__ __forceinline__ void fill_target(const Rep &S, int i, int buf, bool live, int g,
+                                            unsigned long long gmask, unsigned long long below,
+                                            int *stage) {
+  int a = 0, b = 0, cz = 0;
+  double4 pi = make_double4(0, 0, 0, 0);
+  if (live) {
+    pi = S.pos[i];
+    const int cell = S.cell1_of[i];
+    live = cell >= 0;
+    if (live) {
+      a = cell % S.nc1[0];
+      b = (cell / S.nc1[0]) % S.nc1[1];
+      cz = cell / (S.nc1[0] * S.nc1[1]);
+    }
+  }
+  // stencil cell of lane g (< 27), its count, exclusive offsets in the group
+  int cl = -1, cn = 0;
+  if (live && g < 27) {
+    cl = stencil_cell<false>(g, a, b, cz, S.nc1);
+    const int ncell = S.nc1[0] * S.nc1[1] * S.nc1[2];
+    if (cl >= 0) cn = min(S.cnt1[(size_t)buf * ncell + cl], S.ccap1);
+  }
+  int incl = cn;
+  for (int o = 1; o < 32; o <<= 1) {
+    const int v = __shfl_up(incl, o, 32);
+    if (g >= o) incl += v;
+  }
+  const int excl = incl - cn;
+  const int tot = __shfl(incl, 26, 32);
+  const int tmax = max(tot, __shfl_xor(tot, 32, 64));
+  const bool staged = tot <= kLJStage;
+  // stencil slot of candidate t: the largest l with excl_l <= t
+  auto slot_of = [&](int t) {
+    int l = 0;
+    for (int s = 16; s > 0; s >>= 1) {
+      const int c = l + s;
+      const int ex = __shfl(excl, c < 27 ? c : 26, 32);
+      if (c < 27 && ex <= t) l = c;
+    }
+    return l;
+  };
+  for (int k0 = 0; k0 < tmax; k0 += 32) {  // binned member -> its place in order
+    const int t = k0 + g;
+    const int l = slot_of(t);
+    const int lcell = __shfl(cl, l, 32), lex = __shfl(excl, l, 32), lcn = __shfl(cn, l, 32);
+    if (staged && t < tot) {
+      const int *m = S.mem1 + (size_t)lcell * S.ccap1;
+      const int j = m[t - lex];
+      int rk = 0;
+      for (int k = 0; k < lcn; k++) rk += m[k] < j;
+      stage[lex + rk] = j;
+    }
+  }
+  wave_sync();
+  int n = 0;
+  for (int k0 = 0; k0 < tmax; k0 += 32) {
+    const int t = k0 + g;
+    const int l = slot_of(t);
+    const int lcell = __shfl(cl, l, 32), lex = __shfl(excl, l, 32), lcn = __shfl(cn, l, 32);
+    bool hit = false;
+    int j = 0;
+    if (t < tot) {
+      if (staged) {
+        j = stage[t];
+      } else {  // the member of rank t - lex: the one with that many smaller members
+        const int *m = S.mem1 + (size_t)lcell * S.ccap1;
+        for (int k = 0; k < lcn; k++) {
+          const int v = m[k];
+          int rk = 0;
+          for (int k2 = 0; k2 < lcn; k2++) rk += m[k2] < v;
+          if (rk == t - lex) j = v;
+        }
+      }
+      double dx, dy, dz;
+      hit = j != i && rsq_image(pi, S.pos[j], S.Lx, S.Ly, S.hLx, S.hLy, dx, dy, dz) < S.rv1sq;
+    }
+    const unsigned long long mask = __ballot(hit);
+    if (hit) {
+      const int slot = n + __popcll(mask & below);
+      if (slot < S.nbcap1) S.nb1[(size_t)slot * S.NN + i] = j;
+    }
+    n += __popcll(mask & gmask);
+  }
+  wave_sync();  // the stage is refilled for the next target
+  if (live && g == 0) {
+    S.nbn1[i] = n;
+    if (n > S.nbcap1) atomicOr(&S.ctrl->ovf, 4);
+    if (n > S.ctrl->max_nb1) atomicMax(&S.ctrl->max_nb1, n);
+  }
+}
+
 __device__ __forceinline__ void build_lj_pass(const Rep *__restrict__ reps, unsigned long long *lq,
                                               int cap, int (*stage_all)[kLJStage]) {
   const unsigned long long head = *lq;
@@ -900,99 +994,23 @@ __device__ __forceinline__ void build_lj_pass(const Rep *__restrict__ reps, unsi
   const int stride = gridDim.x * blockDim.x / 32;
   for (int t0 = 2 * wave; t0 < total; t0 += stride) {
     const int gi = t0 + half;
-    bool live = gi < total;
-    const Rep *Sp = reps;
-    int i = 0, a = 0, b = 0, cz = 0, buf = 0;
-    double4 pi = make_double4(0, 0, 0, 0);
-    if (live) {
+    const bool in = gi < total;
+    int qe = 0, i = 0;
+    if (in) {
       const int e = lq_find(qst, ne, gi);
-      Sp = reps + lq_entry_rep(qrep[e]);
-      buf = lq_entry_buf(qrep[e]);
+      qe = qrep[e];
       i = gi - qst[e];
-      pi = Sp->pos[i];
-      const int cell = Sp->cell1_of[i];
-      live = cell >= 0;
-      if (live) {
-        a = cell % Sp->nc1[0];
-        b = (cell / Sp->nc1[0]) % Sp->nc1[1];
-        cz = cell / (Sp->nc1[0] * Sp->nc1[1]);
-      }
     }
-    const Rep &S = *Sp;
-    // stencil cell of lane g (< 27), its count, exclusive offsets in the group
-    int cl = -1, cn = 0;
-    if (live && g < 27) {
-      cl = stencil_cell<false>(g, a, b, cz, S.nc1);
-      const int ncell = S.nc1[0] * S.nc1[1] * S.nc1[2];
-      if (cl >= 0) cn = min(S.cnt1[(size_t)buf * ncell + cl], S.ccap1);
-    }
-    int incl = cn;
-    for (int o = 1; o < 32; o <<= 1) {
-      const int v = __shfl_up(incl, o, 32);
-      if (g >= o) incl += v;
-    }
-    const int excl = incl - cn;
-    const int tot = __shfl(incl, 26, 32);
-    const int tmax = max(tot, __shfl_xor(tot, 32, 64));
-    const bool staged = tot <= kLJStage;
-    // stencil slot of candidate t: the largest l with excl_l <= t
-    auto slot_of = [&](int t) {
-      int l = 0;
-      for (int s = 16; s > 0; s >>= 1) {
-        const int c = l + s;
-        const int ex = __shfl(excl, c < 27 ? c : 26, 32);
-        if (c < 27 && ex <= t) l = c;
-      }
-      return l;
-    };
-    for (int k0 = 0; k0 < tmax; k0 += 32) {  // binned member -> its place in order
-      const int t = k0 + g;
-      const int l = slot_of(t);
-      const int lcell = __shfl(cl, l, 32), lex = __shfl(excl, l, 32), lcn = __shfl(cn, l, 32);
-      if (staged && t < tot) {
-        const int *m = S.mem1 + (size_t)lcell * S.ccap1;
-        const int j = m[t - lex];
-        int rk = 0;
-        for (int k = 0; k < lcn; k++) rk += m[k] < j;
-        stage[lex + rk] = j;
-      }
-    }
-    wave_sync();
-    int n = 0;
-    for (int k0 = 0; k0 < tmax; k0 += 32) {
-      const int t = k0 + g;
-      const int l = slot_of(t);
-      const int lcell = __shfl(cl, l, 32), lex = __shfl(excl, l, 32), lcn = __shfl(cn, l, 32);
-      bool hit = false;
-      int j = 0;
-      if (t < tot) {
-        if (staged) {
-          j = stage[t];
-        } else {  // the member of rank t - lex: the one with that many smaller members
-          const int *m = S.mem1 + (size_t)lcell * S.ccap1;
-          for (int k = 0; k < lcn; k++) {
-            const int v = m[k];
-            int rk = 0;
-            for (int k2 = 0; k2 < lcn; k2++) rk += m[k2] < v;
-            if (rk == t - lex) j = v;
-          }
-        }
-        double dx, dy, dz;
-        hit = j != i && rsq_image(pi, S.pos[j], S.Lx, S.Ly, S.hLx, S.hLy, dx, dy, dz) < S.rv1sq;
-      }
-      const unsigned long long mask = __ballot(hit);
-      if (hit) {
-        const int slot = n + __popcll(mask & below);
-        if (slot < S.nbcap1) S.nb1[(size_t)slot * S.NN + i] = j;
-      }
-      n += __popcll(mask & gmask);
-    }
-    wave_sync();  // the stage is refilled for the next target
-    if (live && g == 0) {
-      S.nbn1[i] = n;
-      if (n > S.nbcap1) atomicOr(&S.ctrl->ovf, 4);
-      if (n > S.ctrl->max_nb1) atomicMax(&S.ctrl->max_nb1, n);
-    }
+    // the wavefront's two targets mostly belong to one queue entry (with one
+    // replica, always): then the descriptor is wave-uniform and its fields
+    // come by scalar loads instead of a vector load per field on each
+    // target's chain of dependent loads (lane 0 holds target t0 < total)
+    const int q0 = __builtin_amdgcn_readfirstlane(qe);
+    if (__ballot(in && qe != q0) == 0ull)
+      fill_target(reps[lq_entry_rep(q0)], i, lq_entry_buf(q0), in, g, gmask, below, stage);
+    else
+      fill_target(reps[lq_entry_rep(in ? qe : q0)], i, lq_entry_buf(in ? qe : q0), in, g, gmask,
+                  below, stage);
   }
 }
 

@@ -1014,7 +1014,7 @@ __device__ __forceinline__ void build_lj_pass(const Rep *__restrict__ reps, unsi
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_build_lj(const Rep *__restrict__ reps, int nrep,
+__global__ __launch_bounds__(kBlock, 8) void k_build_lj(const Rep *__restrict__ reps, int nrep,
                                                      unsigned long long *lq, int cap,
                                                      int inc_step) {
   if (inc_step && blockIdx.x == 0)

@@ -1900,7 +1900,7 @@ __device__ __forceinline__ void plane_add_axes(unsigned long long *acc, double4 
 // mesh energy.
 template <typename R>
 __device__ __forceinline__ void plane_spread_block(const Rep *__restrict__ reps, WMap wm, int gb,
-                                                   char *smem) {
+                                                   char *smem, int store_q) {
   using C = typename CT<R>::C;
   const int r = find_rep(wm, gb);
   const Rep &S = reps[r];
@@ -1969,7 +1969,7 @@ __device__ __forceinline__ void plane_spread_block(const Rep *__restrict__ reps,
   for (int k = 0; k < kPlaneSpreadPer; k++) {
     const int t = threadIdx.x + k * kPlaneBlock;
     if (t < npt) {
-      qm[t] = val[k];
+      if (store_q) qm[t] = val[k];
       const int y = t / K1, x = t - y * K1;
       R *row = (R *)&ra[(y >> 1) * K1 + x];
       row[y & 1] = val[k];
@@ -1981,9 +1981,10 @@ __device__ __forceinline__ void plane_spread_block(const Rep *__restrict__ reps,
 }
 
 template <typename R>
-__global__ __launch_bounds__(kPlaneBlock) void k_plane_spread(const Rep *__restrict__ reps, WMap wm) {
+__global__ __launch_bounds__(kPlaneBlock) void k_plane_spread(const Rep *__restrict__ reps, WMap wm,
+                                                              int store_q) {
   extern __shared__ char smem[];
-  plane_spread_block<R>(reps, wm, blockIdx.x, smem);
+  plane_spread_block<R>(reps, wm, blockIdx.x, smem, store_q);
 }
 
 // Inverse y and x passes of one z-plane: inverse y FFT of each kx line,
@@ -3004,9 +3005,10 @@ __global__ __launch_bounds__(kBlock) void k_gather_real(const Rep *__restrict__ 
 // one kind only.
 template <typename R>
 __global__ __launch_bounds__(kPlaneBlock) void k_plane_spread_real(const Rep *__restrict__ reps,
-                                                                   WMap wp, WMap wr, int nparts) {
+                                                                   WMap wp, WMap wr, int nparts,
+                                                                   int store_q) {
   extern __shared__ char smem[];
-  if ((int)blockIdx.x < wp.total) plane_spread_block<R>(reps, wp, blockIdx.x, smem);
+  if ((int)blockIdx.x < wp.total) plane_spread_block<R>(reps, wp, blockIdx.x, smem, store_q);
   else real_cells_block<R, true, kPlaneBlock>(reps, wr, (int)blockIdx.x - wp.total, smem, 0, nparts);
 }
 
@@ -3389,6 +3391,10 @@ void launch_heights(const Plan &P, hipStream_t st, double *mol) {
 template <typename R>
 void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) {
   int k = 0;
+  // plane spreading stores the charge mesh only for an evaluation whose
+  // forces go out (the energies' mesh term reads it); the held-force
+  // evaluation of the step loop keeps the mesh in LDS alone
+  const int store_q = out != nullptr;
   auto mark = [&]() {
     if (ev) (void)hipEventRecord(ev[k++], st);
   };
@@ -3441,10 +3447,10 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) 
   if (P.plane.total && rparts >= 2)
     hipLaunchKernelGGL(k_plane_spread_real<R>, dim3(P.plane.total + nr), dim3(kPlaneBlock),
                        P.lds_plane > real512 ? P.lds_plane : real512, st, P.reps, P.plane, P.cell3c,
-                       rparts);
+                       rparts, store_q);
   else if (P.plane.total && P.pspread)
     hipLaunchKernelGGL(k_plane_spread<R>, dim3(P.plane.total), dim3(kPlaneBlock), P.lds_plane, st,
-                       P.reps, P.plane);
+                       P.reps, P.plane, store_q);
   else if (P.plane.total)
     hipLaunchKernelGGL(k_plane_fwd<R>, dim3(P.plane.total), dim3(kPlaneBlock), P.lds_plane, st, P.reps,
                        P.plane);

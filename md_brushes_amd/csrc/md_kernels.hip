@@ -409,15 +409,21 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
   const Rep &S = reps[r];
   const int blk = blockIdx.x - wm.blk[r];
   const int step = step_arg >= 0 ? step_arg : S.ctrl->step;
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the wavefront index as a uniform value: each wavefront's part is then a
+  // scalar branch, and what wave 6 keeps across the barrier stays in scalar
+  // registers where it is uniform (70 VGPRs instead of 88: three
+  // workgroups per CU)
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int i = blk * NP + lane;  // the particle of this lane on waves 4..7
   const bool live = i < S.NN;
   __shared__ R lj[3][NP], fb[3][NP];
   __shared__ double wzs[NP], nzs[3][NP];
-  // wave 6's registers across the barrier
-  double4 pi = make_double4(0, 0, 0, 0);
-  double vx = 0, vy = 0, vz = 0, ax0 = 0, ay0 = 0, az0 = 0, cx = 0, cy = 0, cz = 0;
-  int anchor = 0;
+  // wave 6's registers across the barrier: assigned on its path only (each
+  // under the mode bit that later reads it), so that no other wavefront's
+  // branch holds them live
+  double4 pi;
+  double vx, vy, vz, ax0, ay0, az0, cx, cy, cz;
+  int anchor;
   StepK K;
   if (w < 4) {  // WCA-LJ over the full list: class by |q_i|+|q_j| (compt_acc.f90:922-952)
     const int g = threadIdx.x & (G - 1), pl = threadIdx.x / G;

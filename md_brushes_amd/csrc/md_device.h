@@ -107,6 +107,7 @@ struct Rep {
   void *ccx, *ccy, *ccz;           // held Coulomb force, real type of the path
   int *flags;                      // bit0 anchor, bit1 charged
   int *charge;                     // Nq: particle index of charge m
+  int *qslot;                      // NN: charge slot m of particle i (-1: uncharged)
   int *bstart;                     // NN+1 CSR of bonded partners
   int *bpart;
   int4 *bond4;                     // each particle's first four partners, bond order (-1: none)

@@ -347,6 +347,7 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   S.ccx = cc; S.ccy = cc + nn * rs; S.ccz = cc + 2 * nn * rs;
   S.flags = dalloc<int>(nn);
   S.charge = dalloc<int>(std::max(Nq, 1));
+  S.qslot = dalloc<int>(nn);
   S.bstart = dalloc<int>(nn + 1);
   S.bpart = dalloc<int>(bpart.size());
   S.bond4 = dalloc<int4>(nn);
@@ -366,12 +367,17 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   S.Nma = p.Nma;
   S.Ngl = p.Ngl;
   S.Nml = p.Nml;
-  if (!cc || !S.flags || !S.charge || !S.bstart || !S.bpart || !S.bond4 || !S.ctrl || !d_eta || !d_out1 ||
+  if (!cc || !S.flags || !S.charge || !S.qslot || !S.bstart || !S.bpart || !S.bond4 || !S.ctrl || !d_eta || !d_out1 ||
       !d_out2 || !d_hist || !d_hsum || !d_mol || !d_epart)
     return fail(MD_ENOMEM, "topology arrays");
   hipError_t cerr = dry ? hipSuccess : hipMemcpy(S.flags, flags.data(), nn * sizeof(int), hipMemcpyHostToDevice);
   if (cerr == hipSuccess && Nq && !dry)
     cerr = hipMemcpy(S.charge, charge.data(), Nq * sizeof(int), hipMemcpyHostToDevice);
+  if (cerr == hipSuccess && !dry) {
+    std::vector<int> qs(nn, -1);
+    for (int m = 0; m < Nq; m++) qs[charge[m]] = m;
+    cerr = hipMemcpy(S.qslot, qs.data(), nn * sizeof(int), hipMemcpyHostToDevice);
+  }
   if (cerr == hipSuccess && !dry)
     cerr = hipMemcpy(S.bstart, bstart.data(), (nn + 1) * sizeof(int), hipMemcpyHostToDevice);
   if (cerr == hipSuccess && !dry)
@@ -1212,12 +1218,16 @@ static void launch_cycle(md_engine *e, int q) {
   const bool any_rl = e->any_real_list();
   for (int k = 1; k <= e->multistep; k++) {
     const Plan &P = e->planp[q ^ ((k - 1) & 1)];
-    if (any_coulomb && k == e->multistep) {
-      if (e->fp32) mdk::launch_coulomb<float>(P, e->st, nullptr);
-      else mdk::launch_coulomb<double>(P, e->st, nullptr);
+    // the Coulomb step's short-range launch assembles and holds the force
+    // the evaluation leaves (SR_COUL) in place of a store kernel
+    const bool coul = any_coulomb && k == e->multistep;
+    if (coul) {
+      if (e->fp32) mdk::launch_coulomb<float>(P, e->st, nullptr, nullptr, false);
+      else mdk::launch_coulomb<double>(P, e->st, nullptr, nullptr, false);
     }
-    if (e->fp32) mdk::launch_short<float>(P, e->st, mode, -1, nullptr, nullptr);
-    else mdk::launch_short<double>(P, e->st, mode, -1, nullptr, nullptr);
+    const int mk = coul ? mode | mdk::SR_COUL : mode;
+    if (e->fp32) mdk::launch_short<float>(P, e->st, mk, -1, nullptr, nullptr);
+    else mdk::launch_short<double>(P, e->st, mk, -1, nullptr, nullptr);
     mdk::launch_build(P, e->st, 1);  // the short-range kernel binned for a rebuild
     if (any_rl) mdk::launch_real_list(P, e->st, false);
     // pre-heat rescale between steps (its step is a multiple of 20, so of

@@ -17,9 +17,11 @@ enum {
   SR_DRIFT = 128,        // with SR_KICK, for replayed steps: v holds v(n-1) (the
                          // first half kick is redone), and the next step's
                          // kick and drift go to the other position buffer
-  SR_BIN = 256           // with SR_DRMAX: every workgroup decides and bins its
+  SR_BIN = 256,          // with SR_DRMAX: every workgroup decides and bins its
                          // particles for a rebuild (the step's list kernel is
                          // then k_build_lj alone, which does the bookkeeping)
+  SR_COUL = 512          // with SR_HELD, after launch_coulomb(store = false):
+                         // assemble, hold and add the evaluation's Coulomb force
 };
 
 // Complex points in each of the two LDS regions of a fused FFT plane
@@ -68,7 +70,8 @@ struct Plan {
 };
 
 template <typename R>
-void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev = nullptr);
+void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev = nullptr,
+                    bool store = true);
 template <typename R>
 void launch_short(const Plan &P, hipStream_t st, int mode, int step, const double *eta,
                   double *out);

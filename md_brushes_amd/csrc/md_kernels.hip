@@ -380,12 +380,47 @@ __device__ __forceinline__ StepK step_constants(const Rep &S) {
   return k;
 }
 
+// The sums behind a Coulomb evaluation's store, by one wavefront: the
+// gather's block partials of the reciprocal force (3) and of q z lb/Beta
+// (M_z), lane l summing blocks l, l + 64, ... in order, then a fixed DPP
+// tree over the lanes (wave_total).  k_coul_store and the short-range
+// kernel of a Coulomb step (SR_COUL) both take their totals here, so the
+// held force is the same bits whichever stores it.
+__device__ __forceinline__ void coul_totals(const Rep &S, double tot[4]) {
+  const int lane = threadIdx.x & 63;
+  double v[4] = {0, 0, 0, 0};
+  for (int b = lane; b < S.nqblk; b += 64)
+#pragma unroll
+    for (int d = 0; d < 4; d++) v[d] += S.part[(size_t)b * 4 + d];
+#pragma unroll
+  for (int d = 0; d < 4; d++) tot[d] = wave_total(v[d]);
+}
+
+// acc_c of one charge: the reciprocal force with its mean removed, scaled by
+// lb/Beta (compt_acc.f90:1242-1244), plus the real-space pair force (scaled
+// here), minus the slab correction (compt_acc.f90:1045-1048).  Rounded
+// term by term (no contraction), so that every kernel that inlines it gets
+// the same bits.
+template <typename R>
+__device__ __forceinline__ void coul_value(const Rep &S, int m, double q, const double *tot,
+                                           double &ax, double &ay, double &az) {
+#pragma clang fp contract(off)
+  const R *f = (const R *)S.freal;
+  const double c1 = S.c1, inv = 1.0 / S.Nq;
+  ax = (S.fsp[m] - tot[0] * inv) * c1 + (double)f[m] * c1;
+  ay = (S.fsp[(size_t)S.Nq + m] - tot[1] * inv) * c1 + (double)f[(size_t)S.Nq + m] * c1;
+  az = (S.fsp[2 * (size_t)S.Nq + m] - tot[2] * inv) * c1 + (double)f[2 * (size_t)S.Nq + m] * c1 -
+       S.slab * q * tot[3];
+}
+
 // Compute_Force without the Coulomb evaluation (compt_acc.f90:847-861):
 // held Coulomb force + lj_force (compt_acc.f90:895-977) + fene_force
-// (compt_acc.f90:865-892) + noise + field, anchors zeroed; optionally the
-// second half kick of new_position (init_update.f90:606), for replayed
-// steps (SR_DRIFT) the next step's half kick and drift into the other
-// position buffer (init_update.f90:581-601, 631-642), and the dr_max
+// (compt_acc.f90:865-892) + noise + field, anchors zeroed (under SR_COUL
+// the held force is first assembled from the evaluation just done, as
+// k_coul_store would); optionally the second half kick of new_position
+// (init_update.f90:606), for replayed steps (SR_DRIFT) the next step's half
+// kick and drift into the other position buffer (init_update.f90:581-601,
+// 631-642), and the dr_max
 // bookkeeping of new_position / update_verlet_list (init_update.f90:600-601,
 // compt_acc.f90:717-724).  A workgroup takes 64 particles and gives its
 // wavefronts the independent parts, so that their memory latencies overlap:
@@ -581,6 +616,15 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
     }
   } else if (w == 6) {  // the particle's state, for after the barrier
     K = step_constants(S);
+    // SR_COUL: the Coulomb force of the step's evaluation is assembled here
+    // (as k_coul_store would, the same bits) and held for the next steps
+    const bool coul = (mode & SR_COUL) && S.nqblk > 0;
+    int m = -1;
+    double tot[4];
+    if (coul) {
+      if (live) m = S.qslot[i];
+      coul_totals(S, tot);
+    }
     if (live) {
       pi = S.pos[i];
       anchor = S.flags[i] & 1;
@@ -594,6 +638,17 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
           cy = ((const double *)S.ccy)[i];
           cz = ((const double *)S.ccz)[i];
         }
+      }
+      if (m >= 0) {
+        double hx, hy, hz;
+        coul_value<R>(S, m, pi.w, tot, hx, hy, hz);
+        const R rx = (R)hx, ry = (R)hy, rz = (R)hz;
+        ((R *)S.ccx)[i] = rx;
+        ((R *)S.ccy)[i] = ry;
+        ((R *)S.ccz)[i] = rz;
+        cx = (double)rx;
+        cy = (double)ry;
+        cz = (double)rz;
       }
       if (mode & SR_KICK) {
         vx = S.vx[i];
@@ -2315,17 +2370,11 @@ __global__ __launch_bounds__(kBlock) void k_gather(const Rep *__restrict__ reps,
   gather_block<R>(reps, wm, blockIdx.x);
 }
 
-// acc_c of one charge: the reciprocal force with its mean removed, scaled by
-// lb/Beta (compt_acc.f90:1242-1244), plus the real-space pair force (scaled
-// here), minus the slab correction (compt_acc.f90:1045-1048).
 template <typename R>
 __device__ __forceinline__ void store_coulomb(const Rep &S, int m, int i, double q, const double *tot,
-                                              R fx, R fy, R fz, double *out) {
-  const double c1 = S.c1, inv = 1.0 / S.Nq;
-  const double ax = (S.fsp[m] - tot[0] * inv) * c1 + (double)fx * c1;
-  const double ay = (S.fsp[(size_t)S.Nq + m] - tot[1] * inv) * c1 + (double)fy * c1;
-  const double az = (S.fsp[2 * (size_t)S.Nq + m] - tot[2] * inv) * c1 + (double)fz * c1 -
-                    S.slab * q * tot[3];
+                                              double *out) {
+  double ax, ay, az;
+  coul_value<R>(S, m, q, tot, ax, ay, az);
   if (out) {
     out[i] = ax;
     out[(size_t)S.NN + i] = ay;
@@ -2351,37 +2400,27 @@ __device__ __forceinline__ void store_real(const Rep &S, int m, R fx, R fy, R fz
 }
 
 // Joins the two halves of a Coulomb evaluation, which run concurrently
-// (SPME on one stream, the real-space sweep on another): each workgroup sums
-// the gather kernel's block partials (mean reciprocal force, 3, and M_z) --
-// each thread a fixed strided subset, then a fixed tree, the same bits in
-// every workgroup -- and stores the Coulomb force of its charges.
+// (SPME on one stream, the real-space sweep on another): each workgroup's
+// first wavefront sums the gather kernel's block partials (mean reciprocal
+// force, 3, and M_z; coul_totals, the same bits in every workgroup), and
+// the workgroup stores the Coulomb force of its charges.
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_coul_store(const Rep *__restrict__ reps, WMap wm,
                                                        double *__restrict__ out) {
   const int r = find_rep(wm, blockIdx.x);
   const Rep &S = reps[r];
-  __shared__ double red[4][kBlock / 64];
   __shared__ double tot[4];
-  double v[4] = {0, 0, 0, 0};
-  for (int b = threadIdx.x; b < S.nqblk; b += kBlock)
-    for (int d = 0; d < 4; d++) v[d] += S.part[(size_t)b * 4 + d];
-  for (int d = 0; d < 4; d++) {
-    const double x = wave_sum(v[d]);
-    if ((threadIdx.x & 63) == 0) red[d][threadIdx.x >> 6] = x;
-  }
-  __syncthreads();
-  if (threadIdx.x < 4) {
-    double t = 0;
-    for (int w = 0; w < kBlock / 64; w++) t += red[threadIdx.x][w];
-    tot[threadIdx.x] = t;
+  if (threadIdx.x < 64) {
+    double t[4];
+    coul_totals(S, t);
+    if (threadIdx.x == 0)
+      for (int d = 0; d < 4; d++) tot[d] = t[d];
   }
   __syncthreads();
   const int m = (blockIdx.x - wm.blk[r]) * kBlock + threadIdx.x;
   if (m >= S.Nq) return;
-  const R *f = (const R *)S.freal;
   const int i = S.charge[m];
-  store_coulomb<R>(S, m, i, S.pos[i].w, tot, f[m], f[(size_t)S.Nq + m], f[2 * (size_t)S.Nq + m],
-                   out);
+  store_coulomb<R>(S, m, i, S.pos[i].w, tot, out);
 }
 
 // erfc-screened Coulomb pair force of real_space (compt_acc.f90:1034-1036)
@@ -3394,8 +3433,10 @@ void launch_heights(const Plan &P, hipStream_t st, double *mol) {
 
 // With ev != null, records ev[k] before kernel k and ev[8] after the last
 // (kernel timing); never used under graph capture.
+// With store false the forces are left to the next short-range launch
+// (SR_COUL), which assembles and holds them.
 template <typename R>
-void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) {
+void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev, bool store) {
   int k = 0;
   // plane spreading stores the charge mesh only for an evaluation whose
   // forces go out (the energies' mesh term reads it); the held-force
@@ -3503,7 +3544,8 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev) 
   }
   if (P.rlst.total)  // replicas whose real-space force walks the reference-form list
     hipLaunchKernelGGL(k_real_list<R>, dim3(P.rlst.total), dim3(kBlock), 0, st, P.reps, P.rlst);
-  hipLaunchKernelGGL(k_coul_store<R>, dim3(P.chg.total), dim3(kBlock), 0, st, P.reps, P.chg, out);
+  if (store || out)
+    hipLaunchKernelGGL(k_coul_store<R>, dim3(P.chg.total), dim3(kBlock), 0, st, P.reps, P.chg, out);
   mark();
 }
 
@@ -3576,8 +3618,8 @@ void launch_energies(const Plan &P, hipStream_t st, double *part, int nb_short, 
 template void launch_energies<float>(const Plan &, hipStream_t, double *, int, int, int);
 template void launch_energies<double>(const Plan &, hipStream_t, double *, int, int, int);
 
-template void launch_coulomb<float>(const Plan &, hipStream_t, double *, hipEvent_t *);
-template void launch_coulomb<double>(const Plan &, hipStream_t, double *, hipEvent_t *);
+template void launch_coulomb<float>(const Plan &, hipStream_t, double *, hipEvent_t *, bool);
+template void launch_coulomb<double>(const Plan &, hipStream_t, double *, hipEvent_t *, bool);
 template void launch_short<float>(const Plan &, hipStream_t, int, int, const double *, double *);
 template void launch_short<double>(const Plan &, hipStream_t, int, int, const double *, double *);
 

@@ -207,7 +207,7 @@ def time_engine(prec, state, steps, warmup, seed, p=None):
     e.upload(pos, vel, acc)
     e.build_lists()
     e.compute_force(0, MD_NOISE_PHILOX)
-    e.tuned = e.autotune(3)  # launch forms only: results do not depend on them
+    e.tuned = e.autotune(10)  # launch forms only: results do not depend on them
     w = aligned_warmup(warmup, p.multistep)
     e.run(1, w)
     t0 = time.perf_counter()

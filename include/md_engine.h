@@ -262,8 +262,10 @@ int md_batch_add(md_engine *e, const md_params *p, const double *pos,
 int md_batch_remove(md_engine *e, int32_t slot);
 
 /* Queues ncycles * multistep steps (new_position + update_verlet_list,
- * Philox noise) of every replica as graph-replayed launch sequences and
- * returns without waiting: md_batch_add, _status, _download and _observe
+ * Philox noise) of every replica, the first cycle as a captured graph's
+ * replay and the others as launch sequences on the engine's stream
+ * (MD_GRAPH=1: every cycle replayed, MD_GRAPH=0: none), and returns
+ * without waiting: md_batch_add, _status, _download and _observe
  * wait for them, so host work between calls overlaps the device. */
 int md_batch_run(md_engine *e, int32_t ncycles);
 

@@ -1254,17 +1254,26 @@ int md_engine::build_graph(int q) {
 
 // ncycles cycles from the current state: the first step's kick-drift (its
 // velocities kept: the step's short-range kernel redoes the kick), then the
-// cycles, graph-replayed (MD_NO_GRAPH=1 issues the same launches on the
-// stream, for tracing tools that do not follow graph launches).  The last
-// step's positions become the current buffer.
+// cycles: the first as a replay of its captured graph, which one call
+// submits, the others issued on the stream launch by launch while it runs.
+// On this runtime a cycle issued on the stream leaves shorter gaps between
+// its dependent kernels than its graph replay (the default brush over
+// 2,000 steps: 62.7k against 61.2k steps/s; the 256-replica serving
+// stream, device-bound, the same either way), but the host, at about 3 us
+// per launch, can fall behind the device at the start of a run; the first
+// cycle's replay gives it the lead.  Over 20 steps (eight runs each): this
+// mix 58.6k-59.3k steps/s, every cycle on the stream 52.5k-60.7k, every
+// cycle replayed 55.9k-58.3k.  MD_GRAPH=1 replays every cycle, MD_GRAPH=0
+// issues every cycle on the stream (for tracing tools that do not follow
+// graph launches).  The last step's positions become the current buffer.
 int md_engine::run_cycles(int ncycles) {
   if (ncycles <= 0 || sys.empty()) return MD_OK;
   if (dirty) {
     int rc = rebuild();
     if (rc) return rc;
   }
-  const char *ng = std::getenv("MD_NO_GRAPH");
-  const bool eager = ng && ng[0] == '1';
+  const char *gr = std::getenv("MD_GRAPH");
+  const bool replay_all = gr && gr[0] == '1', eager = gr && gr[0] == '0';
   const int q0 = cur;
   if (!eager) {
     // both starting buffers: a run of even cycles starts on q0, but it ends
@@ -1277,7 +1286,7 @@ int md_engine::run_cycles(int ncycles) {
   mdk::launch_kick_drift(planp[q0], st, 0);
   for (int c = 0; c < ncycles; c++) {
     const int q = q0 ^ ((c * multistep) & 1);
-    if (eager) {
+    if (eager || (c > 0 && !replay_all)) {
       launch_cycle(this, q);
       continue;
     }
@@ -1795,13 +1804,18 @@ extern "C" int md_autotune(md_engine *e, int32_t iters, int32_t *choice) {
       else mdk::launch_coulomb<double>(B, e->st, nullptr);
     };
     one();  // warm
-    (void)hipEventRecord(a, e->st);
-    for (int k = 0; k < iters; k++) one();
-    (void)hipEventRecord(b, e->st);
-    (void)hipEventSynchronize(b);
-    float t = 0;
-    (void)hipEventElapsedTime(&t, a, b);
-    ms = t / iters;
+    // the best of three batches of `iters` evaluations: a batch that a
+    // host or device hiccup slowed does not decide the choice
+    ms = 1e30;
+    for (int rep = 0; rep < 3; rep++) {
+      (void)hipEventRecord(a, e->st);
+      for (int k = 0; k < iters; k++) one();
+      (void)hipEventRecord(b, e->st);
+      (void)hipEventSynchronize(b);
+      float t = 0;
+      (void)hipEventElapsedTime(&t, a, b);
+      ms = std::min(ms, (double)t / iters);
+    }
     return MD_OK;
   };
   struct Knob {

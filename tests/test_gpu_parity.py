@@ -108,8 +108,23 @@ def test_step_with_injected_noise(name):
     e.close()
 
 
-def test_graph_run_matches_stepwise():
-    print("[graph vs stepwise] default", flush=True)
+def _cycle_form(form, monkeypatch):
+    """How md_run issues its cycles: the first as a captured graph's replay
+    and the others launch by launch on the stream (the default), every
+    cycle replayed (MD_GRAPH=1), or every cycle on the stream
+    (MD_GRAPH=0)."""
+    monkeypatch.delenv("MD_GRAPH", raising=False)
+    if form != "mixed":
+        monkeypatch.setenv("MD_GRAPH", "1" if form == "graph" else "0")
+
+
+@pytest.mark.parametrize("cycles", ["mixed", "graph", "stream"])
+def test_graph_run_matches_stepwise(cycles, monkeypatch):
+    """Cycles (the first replayed as a captured graph and the others issued
+    on the stream, every cycle replayed, or every cycle on the stream)
+    equal the step-by-step calls."""
+    print(f"[{cycles} cycles vs stepwise] default", flush=True)
+    _cycle_form(cycles, monkeypatch)
     p, pos, vel, e1 = _engine("default")
     p, pos, vel, e2 = _engine("default")
     e1.compute_force(0, MD_NOISE_PHILOX)
@@ -127,12 +142,13 @@ def test_graph_run_matches_stepwise():
 
 
 def test_eager_cycles_match_graph(monkeypatch):
-    """The replayed cycle issued on the stream (MD_NO_GRAPH=1, the form
-    tracing tools see) equals graph replay, across runs of odd length."""
+    """The default mix of replayed and stream-issued cycles, every cycle
+    replayed as a captured graph (MD_GRAPH=1) and every cycle on the stream
+    (MD_GRAPH=0) give the same states, across runs of odd length."""
     print("[eager vs graph cycles] default fp32", flush=True)
     out = []
-    for flag in ("0", "1"):
-        monkeypatch.setenv("MD_NO_GRAPH", flag)
+    for form in ("mixed", "graph", "stream"):
+        _cycle_form(form, monkeypatch)
         p, pos, vel, e = _engine("default", MD_FP32)
         e.compute_force(0, MD_NOISE_PHILOX)
         e.run(1, 30)
@@ -140,15 +156,19 @@ def test_eager_cycles_match_graph(monkeypatch):
         e.run(48, 25)
         out.append(e.download())
         e.close()
-    for x, y in zip(out[0], out[1]):
-        assert np.array_equal(x, y)
+    for other in out[1:]:
+        for x, y in zip(out[0], other):
+            assert np.array_equal(x, y)
 
 
-def test_odd_multistep_run_matches_stepwise():
+@pytest.mark.parametrize("cycles", ["mixed", "graph", "stream"])
+def test_odd_multistep_run_matches_stepwise(cycles, monkeypatch):
     """With an odd multistep, consecutive replayed cycles start on
-    alternating position buffers (one graph per buffer): the run still
-    equals the stepwise sequence, Coulomb at the multiples of multistep."""
-    print("[odd multistep graph vs stepwise] small", flush=True)
+    alternating position buffers (one launch sequence or graph per buffer):
+    the run still equals the stepwise sequence, Coulomb at the multiples of
+    multistep."""
+    print(f"[odd multistep {cycles} cycles vs stepwise] small", flush=True)
+    _cycle_form(cycles, monkeypatch)
     from md_brushes_amd import config
     p = config("small")
     p.multistep = 5
@@ -174,11 +194,13 @@ def test_odd_multistep_run_matches_stepwise():
     e2.close()
 
 
-def test_preheat_graph_matches_stepwise():
-    """Pre-heat (main.f90:67-75) replayed in graph cycles with the gated
-    rescale kernels equals the step-by-step sequence with rescale_velocity
-    after every 20th step; the run starts off the cycle alignment."""
-    print("[preheat graph vs stepwise] small", flush=True)
+@pytest.mark.parametrize("cycles", ["mixed", "graph", "stream"])
+def test_preheat_graph_matches_stepwise(cycles, monkeypatch):
+    """Pre-heat (main.f90:67-75) replayed in cycles with the gated rescale
+    kernels equals the step-by-step sequence with rescale_velocity after
+    every 20th step; the run starts off the cycle alignment."""
+    print(f"[preheat {cycles} cycles vs stepwise] small", flush=True)
+    _cycle_form(cycles, monkeypatch)
     p, pos, vel, e1 = _engine("small")
     p, pos, vel, e2 = _engine("small")
     e1.compute_force(0, MD_NOISE_PHILOX)

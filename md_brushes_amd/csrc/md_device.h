@@ -107,7 +107,6 @@ struct Rep {
   void *ccx, *ccy, *ccz;           // held Coulomb force, real type of the path
   int *flags;                      // bit0 anchor, bit1 charged
   int *charge;                     // Nq: particle index of charge m
-  int *qslot;                      // NN: charge slot m of particle i (-1: uncharged)
   int *bstart;                     // NN+1 CSR of bonded partners
   int *bpart;
   int4 *bond4;                     // each particle's first four partners, bond order (-1: none)
@@ -122,9 +121,10 @@ struct Rep {
   int nc3[3];
   double rcel3[3];
   int ccap3;
-  int *cnt3, *mem3, *scnt3;
+  int *cnt3, *mem3, *scnt3;        // mem3: the cells' charges as particle indices
   void *pos3s;                     // positions in cell-sorted order (float4 / double4)
-  void *freal;                     // real-space pair sums per charge slot, 3 x Nq (real type)
+  void *freal;                     // real-space pair sums per particle, 3 x NN (real type;
+                                   // charged particles only)
   int *nbn1, *nb1;                 // LJ list: length per particle, ELL slots
   int *nbn2, *nb2;                 // real list: per charge slot m
   void *meshr;                     // charge mesh Q (real type of the path) K3*K2*K1
@@ -139,7 +139,8 @@ struct Rep {
   void *meshu;                     // real potential mesh K3*K2*K1
   void *bc;                        // influence function K3*K2*Kh real
   const void *tw[3];               // twiddles per axis (K_d/2 complex)
-  double *fsp;                     // Nq*3 reciprocal force before mean removal
+  double *fsp;                     // 3 x NN reciprocal force before mean removal (charged
+                                   // particles only)
   double *part;                    // block partial sums: 4 per charge block
   int nqblk;                       // number of charge blocks
   Ctrl *ctrl;

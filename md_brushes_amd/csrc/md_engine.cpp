@@ -347,7 +347,6 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   S.ccx = cc; S.ccy = cc + nn * rs; S.ccz = cc + 2 * nn * rs;
   S.flags = dalloc<int>(nn);
   S.charge = dalloc<int>(std::max(Nq, 1));
-  S.qslot = dalloc<int>(nn);
   S.bstart = dalloc<int>(nn + 1);
   S.bpart = dalloc<int>(bpart.size());
   S.bond4 = dalloc<int4>(nn);
@@ -367,17 +366,12 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
   S.Nma = p.Nma;
   S.Ngl = p.Ngl;
   S.Nml = p.Nml;
-  if (!cc || !S.flags || !S.charge || !S.qslot || !S.bstart || !S.bpart || !S.bond4 || !S.ctrl || !d_eta || !d_out1 ||
+  if (!cc || !S.flags || !S.charge || !S.bstart || !S.bpart || !S.bond4 || !S.ctrl || !d_eta || !d_out1 ||
       !d_out2 || !d_hist || !d_hsum || !d_mol || !d_epart)
     return fail(MD_ENOMEM, "topology arrays");
   hipError_t cerr = dry ? hipSuccess : hipMemcpy(S.flags, flags.data(), nn * sizeof(int), hipMemcpyHostToDevice);
   if (cerr == hipSuccess && Nq && !dry)
     cerr = hipMemcpy(S.charge, charge.data(), Nq * sizeof(int), hipMemcpyHostToDevice);
-  if (cerr == hipSuccess && !dry) {
-    std::vector<int> qs(nn, -1);
-    for (int m = 0; m < Nq; m++) qs[charge[m]] = m;
-    cerr = hipMemcpy(S.qslot, qs.data(), nn * sizeof(int), hipMemcpyHostToDevice);
-  }
   if (cerr == hipSuccess && !dry)
     cerr = hipMemcpy(S.bstart, bstart.data(), (nn + 1) * sizeof(int), hipMemcpyHostToDevice);
   if (cerr == hipSuccess && !dry)
@@ -541,8 +535,10 @@ int Sys::setup(const md_params *pin, bool want_fp32) {
     S.pz_tile = std::max(1, std::min(4, budget / (K3 * csz)));
     pz_big = std::max(1, std::min(16, budget / (K3 * csz)));
     S.nqblk = cdiv((long)Nq, mdk::kBlock / mdk::kGatherLanes);  // gather blocks
-    S.fsp = dalloc<double>(3 * (size_t)Nq);
-    S.freal = dalloc<char>(3 * (size_t)Nq * (fp32 ? 4 : 8));
+    // per particle (the charged ones written), so that the short-range
+    // kernel reads its particles' entries without the charge slot
+    S.fsp = dalloc<double>(3 * (size_t)NN);
+    S.freal = dalloc<char>(3 * (size_t)NN * (fp32 ? 4 : 8));
     S.part = dalloc<double>(4 * (size_t)S.nqblk);
     if (!S.fsp || !S.part || !S.freal) return fail(MD_ENOMEM, "spme scratch");
     lds_x = (2 * (size_t)S.px_tile * K1 + K1) * csz;

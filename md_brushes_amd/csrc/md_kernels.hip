@@ -402,15 +402,15 @@ __device__ __forceinline__ void coul_totals(const Rep &S, double tot[4]) {
 // term by term (no contraction), so that every kernel that inlines it gets
 // the same bits.
 template <typename R>
-__device__ __forceinline__ void coul_value(const Rep &S, int m, double q, const double *tot,
+__device__ __forceinline__ void coul_value(const Rep &S, int i, double q, const double *tot,
                                            double &ax, double &ay, double &az) {
 #pragma clang fp contract(off)
   const R *f = (const R *)S.freal;
+  const size_t NN = S.NN;
   const double c1 = S.c1, inv = 1.0 / S.Nq;
-  ax = (S.fsp[m] - tot[0] * inv) * c1 + (double)f[m] * c1;
-  ay = (S.fsp[(size_t)S.Nq + m] - tot[1] * inv) * c1 + (double)f[(size_t)S.Nq + m] * c1;
-  az = (S.fsp[2 * (size_t)S.Nq + m] - tot[2] * inv) * c1 + (double)f[2 * (size_t)S.Nq + m] * c1 -
-       S.slab * q * tot[3];
+  ax = (S.fsp[i] - tot[0] * inv) * c1 + (double)f[i] * c1;
+  ay = (S.fsp[NN + i] - tot[1] * inv) * c1 + (double)f[NN + i] * c1;
+  az = (S.fsp[2 * NN + i] - tot[2] * inv) * c1 + (double)f[2 * NN + i] * c1 - S.slab * q * tot[3];
 }
 
 // Compute_Force without the Coulomb evaluation (compt_acc.f90:847-861):
@@ -619,15 +619,12 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
     // SR_COUL: the Coulomb force of the step's evaluation is assembled here
     // (as k_coul_store would, the same bits) and held for the next steps
     const bool coul = (mode & SR_COUL) && S.nqblk > 0;
-    int m = -1;
     double tot[4];
-    if (coul) {
-      if (live) m = S.qslot[i];
-      coul_totals(S, tot);
-    }
+    if (coul) coul_totals(S, tot);
     if (live) {
       pi = S.pos[i];
-      anchor = S.flags[i] & 1;
+      const int fl = S.flags[i];
+      anchor = fl & 1;
       if (mode & SR_HELD) {
         if constexpr (sizeof(R) == 4) {
           cx = ((const float *)S.ccx)[i];
@@ -639,16 +636,21 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
           cz = ((const double *)S.ccz)[i];
         }
       }
-      if (m >= 0) {
+      if (coul) {
+        // the particle's entries load beside its state (one memory round
+        // trip); only a charged particle's are defined and used
         double hx, hy, hz;
-        coul_value<R>(S, m, pi.w, tot, hx, hy, hz);
+        coul_value<R>(S, i, pi.w, tot, hx, hy, hz);
         const R rx = (R)hx, ry = (R)hy, rz = (R)hz;
-        ((R *)S.ccx)[i] = rx;
-        ((R *)S.ccy)[i] = ry;
-        ((R *)S.ccz)[i] = rz;
-        cx = (double)rx;
-        cy = (double)ry;
-        cz = (double)rz;
+        const bool charged = (fl & 2) != 0;
+        if (charged) {
+          ((R *)S.ccx)[i] = rx;
+          ((R *)S.ccy)[i] = ry;
+          ((R *)S.ccz)[i] = rz;
+        }
+        cx = charged ? (double)rx : cx;
+        cy = charged ? (double)ry : cy;
+        cz = charged ? (double)rz : cz;
       }
       if (mode & SR_KICK) {
         vx = S.vx[i];
@@ -2325,8 +2327,10 @@ __device__ __forceinline__ void gather_block(const Rep *__restrict__ reps, WMap 
   double f[4] = {0, 0, 0, 0};
   double4 p = make_double4(0, 0, 0, 0);
   R f1 = 0, f2 = 0, f3 = 0;
+  int i = 0;
   if (live) {
-    p = S.pos[S.charge[m]];
+    i = S.charge[m];
+    p = S.pos[i];
     switch (S.order) {
       case 3: gather_one<R, 3>(S, p, cz, f1, f2, f3); break;
       case 4: gather_one<R, 4>(S, p, cz, f1, f2, f3); break;
@@ -2348,9 +2352,9 @@ __device__ __forceinline__ void gather_block(const Rep *__restrict__ reps, WMap 
     f[1] = -(double)f2 * S.fnorm[1];
     f[2] = -(double)f3 * S.fnorm[2];
     f[3] = p.w * p.z * S.c1;
-    S.fsp[m] = f[0];
-    S.fsp[(size_t)S.Nq + m] = f[1];
-    S.fsp[2 * (size_t)S.Nq + m] = f[2];
+    S.fsp[i] = f[0];
+    S.fsp[(size_t)S.NN + i] = f[1];
+    S.fsp[2 * (size_t)S.NN + i] = f[2];
   }
   __shared__ double red[4][kBlock / 64];
   for (int d = 0; d < 4; d++) {
@@ -2371,10 +2375,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(const Rep *__restrict__ reps,
 }
 
 template <typename R>
-__device__ __forceinline__ void store_coulomb(const Rep &S, int m, int i, double q, const double *tot,
+__device__ __forceinline__ void store_coulomb(const Rep &S, int i, double q, const double *tot,
                                               double *out) {
   double ax, ay, az;
-  coul_value<R>(S, m, q, tot, ax, ay, az);
+  coul_value<R>(S, i, q, tot, ax, ay, az);
   if (out) {
     out[i] = ax;
     out[(size_t)S.NN + i] = ay;
@@ -2390,13 +2394,13 @@ __device__ __forceinline__ void store_coulomb(const Rep &S, int m, int i, double
   }
 }
 
-// The real-space pair sums of charge slot m, before the lb/Beta factor.
+// The real-space pair sums of charged particle i, before the lb/Beta factor.
 template <typename R>
-__device__ __forceinline__ void store_real(const Rep &S, int m, R fx, R fy, R fz) {
+__device__ __forceinline__ void store_real(const Rep &S, int i, R fx, R fy, R fz) {
   R *f = (R *)S.freal;
-  f[m] = fx;
-  f[(size_t)S.Nq + m] = fy;
-  f[2 * (size_t)S.Nq + m] = fz;
+  f[i] = fx;
+  f[(size_t)S.NN + i] = fy;
+  f[2 * (size_t)S.NN + i] = fz;
 }
 
 // Joins the two halves of a Coulomb evaluation, which run concurrently
@@ -2420,7 +2424,7 @@ __global__ __launch_bounds__(kBlock) void k_coul_store(const Rep *__restrict__ r
   const int m = (blockIdx.x - wm.blk[r]) * kBlock + threadIdx.x;
   if (m >= S.Nq) return;
   const int i = S.charge[m];
-  store_coulomb<R>(S, m, i, S.pos[i].w, tot, out);
+  store_coulomb<R>(S, i, S.pos[i].w, tot, out);
 }
 
 // erfc-screened Coulomb pair force of real_space (compt_acc.f90:1034-1036)
@@ -2521,7 +2525,7 @@ __global__ __launch_bounds__(kBlock) void k_real_list(const Rep *__restrict__ re
     fx = wave_total(fx);
     fy = wave_total(fy);
     fz = wave_total(fz);
-    if (lane == 0) store_real<R>(S, t, fx, fy, fz);
+    if (lane == 0) store_real<R>(S, i, fx, fy, fz);
   }
 }
 
@@ -2603,7 +2607,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_q(const Rep *__restrict__ reps, 
       if (slot < S.capt) S.tmem[(size_t)key[k] * S.capt + slot] = i;
       else atomicOr(&S.ctrl->ovf, 32);
     } else {
-      if (slot < S.ccap3) S.mem3[(size_t)key[k] * S.ccap3 + slot] = m;
+      if (slot < S.ccap3) S.mem3[(size_t)key[k] * S.ccap3 + slot] = i;
       else atomicOr(&S.ctrl->ovf, 16);
     }
   }
@@ -2622,18 +2626,19 @@ __global__ __launch_bounds__(kBlock) void k_sort_q(const Rep *__restrict__ reps,
   extern __shared__ int sbuf[];
   int *mem = S.mem3 + (size_t)cell * S.ccap3;
   // each member's place in ascending order is the number of members below
-  // it (the members are distinct charges): one pass over the cell's members
+  // it (the members are distinct particles, whose order is the charge
+  // slots' order: md_build_topology lists the charges by ascending particle
+  // index): one pass over the cell's members
   // in LDS (broadcast reads), then the member and its position go straight
   // to their sorted slots
   for (int k = threadIdx.x; k < n; k += blockDim.x) sbuf[k] = mem[k];
   __syncthreads();
   V *ps = (V *)S.pos3s + (size_t)cell * S.ccap3;
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
-    const int m = sbuf[k];
+    const int i = sbuf[k];
     int rank = 0;
-    for (int j = 0; j < n; j++) rank += sbuf[j] < m;
-    mem[rank] = m;
-    const int i = S.charge[m];
+    for (int j = 0; j < n; j++) rank += sbuf[j] < i;
+    mem[rank] = i;
     if constexpr (sizeof(R) == 4) ps[rank] = S.pos32[i];
     else ps[rank] = S.pos[i];
   }
@@ -3397,7 +3402,7 @@ __global__ __launch_bounds__(kBlock) void k_energy_real(const Rep *__restrict__ 
           const int *mm = S.mem3 + (size_t)cl * S.ccap3;
           const V *ps = (const V *)S.pos3s + (size_t)cl * S.ccap3;
           for (int k = 0; k < cn; k++)
-            if (mm[k] != t) pair(ps[k]);
+            if (mm[k] != i) pair(ps[k]);  // the cells hold particle indices
         }
       }
     }

@@ -386,12 +386,29 @@ __device__ __forceinline__ StepK step_constants(const Rep &S) {
 // tree over the lanes (wave_total).  k_coul_store and the short-range
 // kernel of a Coulomb step (SR_COUL) both take their totals here, so the
 // held force is the same bits whichever stores it.
+template <int U>
 __device__ __forceinline__ void coul_totals(const Rep &S, double tot[4]) {
   const int lane = threadIdx.x & 63;
+  const double4 *pb = (const double4 *)S.part;
   double v[4] = {0, 0, 0, 0};
-  for (int b = lane; b < S.nqblk; b += 64)
+  // U blocks per lane at a time, their loads issued together (one memory
+  // round trip per 64 U blocks), added in block order
+  for (int b0 = lane; b0 < S.nqblk; b0 += 64 * U) {
+    double4 x[U];
 #pragma unroll
-    for (int d = 0; d < 4; d++) v[d] += S.part[(size_t)b * 4 + d];
+    for (int u = 0; u < U; u++) {
+      const int b = b0 + 64 * u;
+      x[u] = b < S.nqblk ? pb[b] : make_double4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (b0 + 64 * u < S.nqblk) {
+        v[0] += x[u].x;
+        v[1] += x[u].y;
+        v[2] += x[u].z;
+        v[3] += x[u].w;
+      }
+  }
 #pragma unroll
   for (int d = 0; d < 4; d++) tot[d] = wave_total(v[d]);
 }
@@ -401,16 +418,29 @@ __device__ __forceinline__ void coul_totals(const Rep &S, double tot[4]) {
 // here), minus the slab correction (compt_acc.f90:1045-1048).  Rounded
 // term by term (no contraction), so that every kernel that inlines it gets
 // the same bits.
+// A charged particle's entries of the two halves (reciprocal, real space).
+template <typename R> struct CoulParts {
+  double fs[3];
+  R fr[3];
+  __device__ __forceinline__ void load(const Rep &S, int i) {
+    const R *f = (const R *)S.freal;
+    const size_t NN = S.NN;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      fs[d] = S.fsp[d * NN + i];
+      fr[d] = f[d * NN + i];
+    }
+  }
+};
+
 template <typename R>
-__device__ __forceinline__ void coul_value(const Rep &S, int i, double q, const double *tot,
-                                           double &ax, double &ay, double &az) {
+__device__ __forceinline__ void coul_value(const Rep &S, const CoulParts<R> &c, double q,
+                                           const double *tot, double &ax, double &ay, double &az) {
 #pragma clang fp contract(off)
-  const R *f = (const R *)S.freal;
-  const size_t NN = S.NN;
   const double c1 = S.c1, inv = 1.0 / S.Nq;
-  ax = (S.fsp[i] - tot[0] * inv) * c1 + (double)f[i] * c1;
-  ay = (S.fsp[NN + i] - tot[1] * inv) * c1 + (double)f[NN + i] * c1;
-  az = (S.fsp[2 * NN + i] - tot[2] * inv) * c1 + (double)f[2 * NN + i] * c1 - S.slab * q * tot[3];
+  ax = (c.fs[0] - tot[0] * inv) * c1 + (double)c.fr[0] * c1;
+  ay = (c.fs[1] - tot[1] * inv) * c1 + (double)c.fr[1] * c1;
+  az = (c.fs[2] - tot[2] * inv) * c1 + (double)c.fr[2] * c1 - S.slab * q * tot[3];
 }
 
 // Compute_Force without the Coulomb evaluation (compt_acc.f90:847-861):
@@ -453,13 +483,17 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
   const bool live = i < S.NN;
   __shared__ R lj[3][NP], fb[3][NP];
   __shared__ double wzs[NP], nzs[3][NP];
+  __shared__ double ctot[4];  // SR_COUL: the Coulomb evaluation's totals (coul_totals)
   // wave 6's registers across the barrier: assigned on its path only (each
   // under the mode bit that later reads it), so that no other wavefront's
   // branch holds them live
   double4 pi;
   double vx, vy, vz, ax0, ay0, az0, cx, cy, cz;
   int anchor;
+  bool charged;
+  CoulParts<R> cp;
   StepK K;
+  const bool coul = (mode & SR_COUL) && S.nqblk > 0;  // uniform: the mode and the replica
   if (w < 4) {  // WCA-LJ over the full list: class by |q_i|+|q_j| (compt_acc.f90:922-952)
     const int g = threadIdx.x & (G - 1), pl = threadIdx.x / G;
     const int ip = blk * NP + pl;
@@ -567,6 +601,12 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
       nzs[1][lane] = n1;
       nzs[2][lane] = n2;
     }
+    if ((mode & SR_COUL) && S.nqblk > 0) {  // the totals for wavefront 6, after the barrier
+      double t[4];
+      coul_totals<4>(S, t);
+      if (lane == 0)
+        for (int d = 0; d < 4; d++) ctot[d] = t[d];
+    }
   } else if (w == 5) {  // wall and FENE over the bonded partners of i
     if (live) {
       // the first four partners (bond order) come with the position; a
@@ -618,13 +658,11 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
     K = step_constants(S);
     // SR_COUL: the Coulomb force of the step's evaluation is assembled here
     // (as k_coul_store would, the same bits) and held for the next steps
-    const bool coul = (mode & SR_COUL) && S.nqblk > 0;
-    double tot[4];
-    if (coul) coul_totals(S, tot);
     if (live) {
       pi = S.pos[i];
       const int fl = S.flags[i];
       anchor = fl & 1;
+      charged = (fl & 2) != 0;
       if (mode & SR_HELD) {
         if constexpr (sizeof(R) == 4) {
           cx = ((const float *)S.ccx)[i];
@@ -636,22 +674,9 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
           cz = ((const double *)S.ccz)[i];
         }
       }
-      if (coul) {
-        // the particle's entries load beside its state (one memory round
-        // trip); only a charged particle's are defined and used
-        double hx, hy, hz;
-        coul_value<R>(S, i, pi.w, tot, hx, hy, hz);
-        const R rx = (R)hx, ry = (R)hy, rz = (R)hz;
-        const bool charged = (fl & 2) != 0;
-        if (charged) {
-          ((R *)S.ccx)[i] = rx;
-          ((R *)S.ccy)[i] = ry;
-          ((R *)S.ccz)[i] = rz;
-        }
-        cx = charged ? (double)rx : cx;
-        cy = charged ? (double)ry : cy;
-        cz = charged ? (double)rz : cz;
-      }
+      // SR_COUL: the particle's entries load beside its state (one memory
+      // round trip; only a charged particle's are defined and used)
+      if (coul) cp.load(S, i);
       if (mode & SR_KICK) {
         vx = S.vx[i];
         vy = S.vy[i];
@@ -738,6 +763,17 @@ __global__ __launch_bounds__(kSRBlock) void k_short_range(const Rep *__restrict_
     const R fx = lj[0][lane] + fb[0][lane], fy = lj[1][lane] + fb[1][lane],
             fz = lj[2][lane] + fb[2][lane];
     double ax = (double)fx, ay = (double)fy, az = (double)fz + wzs[lane];
+    if (coul && charged) {  // this step's Coulomb force: held for the next steps and added
+      double hx, hy, hz;
+      coul_value<R>(S, cp, pi.w, ctot, hx, hy, hz);
+      const R rx = (R)hx, ry = (R)hy, rz = (R)hz;
+      ((R *)S.ccx)[i] = rx;
+      ((R *)S.ccy)[i] = ry;
+      ((R *)S.ccz)[i] = rz;
+      cx = (double)rx;
+      cy = (double)ry;
+      cz = (double)rz;
+    }
     if (mode & SR_HELD) {
       ax += cx;
       ay += cy;
@@ -2377,8 +2413,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(const Rep *__restrict__ reps,
 template <typename R>
 __device__ __forceinline__ void store_coulomb(const Rep &S, int i, double q, const double *tot,
                                               double *out) {
+  CoulParts<R> c;
+  c.load(S, i);
   double ax, ay, az;
-  coul_value<R>(S, i, q, tot, ax, ay, az);
+  coul_value<R>(S, c, q, tot, ax, ay, az);
   if (out) {
     out[i] = ax;
     out[(size_t)S.NN + i] = ay;
@@ -2416,7 +2454,7 @@ __global__ __launch_bounds__(kBlock) void k_coul_store(const Rep *__restrict__ r
   __shared__ double tot[4];
   if (threadIdx.x < 64) {
     double t[4];
-    coul_totals(S, t);
+    coul_totals<4>(S, t);
     if (threadIdx.x == 0)
       for (int d = 0; d < 4; d++) tot[d] = t[d];
   }

@@ -792,6 +792,7 @@ struct md_engine {
   int tune_rgrid = 1024;           // workgroups of the real-space sweep's workgroup form (cap)
   int tune_pspread = 1;            // spreading inside the forward plane kernel: 0 no, 1 yes
   int tune_rparts = 3;             // with plane spreading: launches the real-space sweep rides in
+  int tune_lead = 0;               // riding sweep workgroups dispatched ahead of a launch's own
   std::string err;
 
   ~md_engine() {
@@ -1022,6 +1023,8 @@ int md_engine::rebuild() {
     P.pspread = pspread;
     const char *rp = std::getenv("MD_RIDE_PARTS");  // forces the plane-spreading ride split
     P.ride_parts = std::max(1, std::min(3, rp ? std::atoi(rp) : tune_rparts));
+    const char *rl = std::getenv("MD_RIDE_LEAD");  // forces the riders dispatched first
+    P.ride_lead = rl ? std::max(0, std::atoi(rl)) : tune_lead;
     // MD_TWO_STREAMS=1: the real-space sweep beside the SPME kernels on a
     // second stream (the same results; slower in graph replay at the
     // default brush, so off unless asked for)
@@ -1820,7 +1823,7 @@ extern "C" int md_autotune(md_engine *e, int32_t iters, int32_t *choice) {
   };
   Knob knobs[] = {{&e->tune_fuse, {0, 1}}, {&e->tune_pz, {2, 4, 8, 16}}, {&e->tune_real, {0, 1}},
                   {&e->tune_ride, {0, 1}}, {&e->tune_rgrid, {1024, 2048}}, {&e->tune_pspread, {0, 1}},
-                  {&e->tune_rparts, {1, 2, 3}}};
+                  {&e->tune_rparts, {1, 2, 3}}, {&e->tune_lead, {0, 256, 512, 768}}};
   for (Knob &k : knobs) {
     int best = *k.field;
     double best_ms = 1e30;
@@ -1849,6 +1852,7 @@ extern "C" int md_autotune(md_engine *e, int32_t iters, int32_t *choice) {
     choice[4] = e->tune_rgrid;
     choice[5] = e->tune_pspread;
     choice[6] = e->tune_rparts;
+    choice[7] = e->tune_lead;
   }
   return rc;
 }

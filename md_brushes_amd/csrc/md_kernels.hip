@@ -3080,10 +3080,11 @@ __global__ __launch_bounds__(kBlock) void k_spread_real(const Rep *__restrict__ 
 
 template <typename R, bool WG>
 __global__ __launch_bounds__(kBlock) void k_gather_real(const Rep *__restrict__ reps, WMap wg,
-                                                        WMap wr, int nparts) {
+                                                        WMap wr, int nparts, int lead) {
   __shared__ __align__(16) char lds[RealLDS<R>::bytes];
-  if ((int)blockIdx.x < wg.total) gather_block<R>(reps, wg, blockIdx.x);
-  else real_cells_block<R, WG>(reps, wr, (int)blockIdx.x - wg.total, lds, nparts - 1, nparts);
+  const int b = blockIdx.x;  // riders 0..lead-1, the gather's workgroups, the other riders
+  if (b >= lead && b < lead + wg.total) gather_block<R>(reps, wg, b - lead);
+  else real_cells_block<R, WG>(reps, wr, b < lead ? b : b - wg.total, lds, nparts - 1, nparts);
 }
 
 // With plane spreading the sweep's workgroup form (in workgroups of
@@ -3094,10 +3095,11 @@ __global__ __launch_bounds__(kBlock) void k_gather_real(const Rep *__restrict__ 
 template <typename R>
 __global__ __launch_bounds__(kPlaneBlock) void k_plane_spread_real(const Rep *__restrict__ reps,
                                                                    WMap wp, WMap wr, int nparts,
-                                                                   int store_q) {
+                                                                   int store_q, int lead) {
   extern __shared__ char smem[];
-  if ((int)blockIdx.x < wp.total) plane_spread_block<R>(reps, wp, blockIdx.x, smem, store_q);
-  else real_cells_block<R, true, kPlaneBlock>(reps, wr, (int)blockIdx.x - wp.total, smem, 0, nparts);
+  const int b = blockIdx.x;  // riders 0..lead-1, the plane workgroups, the other riders
+  if (b >= lead && b < lead + wp.total) plane_spread_block<R>(reps, wp, b - lead, smem, store_q);
+  else real_cells_block<R, true, kPlaneBlock>(reps, wr, b < lead ? b : b - wp.total, smem, 0, nparts);
 }
 
 template <typename R>
@@ -3537,7 +3539,7 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev, 
   if (P.plane.total && rparts >= 2)
     hipLaunchKernelGGL(k_plane_spread_real<R>, dim3(P.plane.total + nr), dim3(kPlaneBlock),
                        P.lds_plane > real512 ? P.lds_plane : real512, st, P.reps, P.plane, P.cell3c,
-                       rparts, store_q);
+                       rparts, store_q, std::min(P.ride_lead, nr));
   else if (P.plane.total && P.pspread)
     hipLaunchKernelGGL(k_plane_spread<R>, dim3(P.plane.total), dim3(kPlaneBlock), P.lds_plane, st,
                        P.reps, P.plane, store_q);
@@ -3569,10 +3571,10 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev, 
   const int nparts = P.pspread ? rparts : 2;  // the gather launch takes the last part
   if (ride && P.real_wg)
     hipLaunchKernelGGL((k_gather_real<R, true>), dim3(P.chgG.total + nr), dim3(kBlock), 0, st, P.reps,
-                       P.chgG, P.cell3c, nparts);
+                       P.chgG, P.cell3c, nparts, std::min(P.ride_lead, nr));
   else if (ride)
     hipLaunchKernelGGL((k_gather_real<R, false>), dim3(P.chgG.total + nr), dim3(kBlock), 0, st, P.reps,
-                       P.chgG, P.cell3c, nparts);
+                       P.chgG, P.cell3c, nparts, std::min(P.ride_lead, nr));
   else
     hipLaunchKernelGGL(k_gather<R>, dim3(P.chgG.total), dim3(kBlock), 0, st, P.reps, P.chgG);
   mark();

@@ -219,13 +219,14 @@ int md_time_kernels(md_engine *e, int32_t iters, double *ms, int32_t n);
  * its own or inside the spreading and gather launches, the sweep's
  * workgroup count, the spreading as its own launch or inside the forward
  * plane FFT kernel (k_plane_spread), how many of the sweep's workgroups
- * riding in a launch are dispatched ahead of the launch's own.  The forms
- * give the same results.  choice[8] (may be null) receives fused (0/1), z
- * lines, workgroup (0/1), shared launches (0/1), the workgroup form's grid
- * cap (1024 or 2048 workgroups), spreading inside the forward plane kernel
- * (0/1), and with it the launches the sweep rides in (1: gather, 2: plane
- * and gather, 3: plane, z pass and gather), and the riders dispatched
- * first (0, 256, 512 or 768).
+ * riding in the plane and gather launches are dispatched ahead of the
+ * launch's own.  The forms give the same results.  choice[9] (may be null)
+ * receives fused (0/1), z lines, workgroup (0/1), shared launches (0/1),
+ * the workgroup form's grid cap (1024 or 2048 workgroups), spreading inside
+ * the forward plane kernel (0/1), and with it the launches the sweep rides
+ * in (1: gather, 2: plane and gather, 3: plane, z pass and gather), and the
+ * riders dispatched first in the plane launch (0, 256, 512 or 768) and in
+ * the gather launch (0, 512, 1024 or 65536: all).
  * Leaves the state as it was; the held Coulomb force is recomputed for the
  * current positions. */
 int md_autotune(md_engine *e, int32_t iters, int32_t *choice);

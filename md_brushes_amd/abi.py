@@ -299,10 +299,11 @@ class Engine:
         """Times the launch forms (fused FFT planes, z-pass lines per
         workgroup, real-space sweep form) and keeps the fastest; results do
         not depend on them.  Returns the choice."""
-        ch = (c_int32 * 8)()
+        ch = (c_int32 * 9)()
         self._check(self.lib.md_autotune(self.h, iters, ch), "md_autotune")
         return {"fused_planes": ch[0], "z_lines": ch[1], "real_workgroup": ch[2], "real_shared_launch": ch[3],
-                "real_grid": ch[4], "plane_spreading": ch[5], "ride_parts": ch[6], "ride_lead": ch[7]}
+                "real_grid": ch[4], "plane_spreading": ch[5], "ride_parts": ch[6], "ride_lead": ch[7],
+                "ride_lead_gather": ch[8]}
 
     KERNELS = ["kick_drift", "short_range", "bin_q", "sort_q", "spread", "fft_x_fwd", "fft_y_fwd",
                "fft_z_conv", "fft_y_inv", "fft_x_inv", "gather", "real", "bin_lj", "sort_lj", "build_lj"]
@@ -393,10 +394,11 @@ class Batch:
 
     def autotune(self, iters=5):
         """As Engine.autotune, over all replicas of the batch."""
-        ch = (c_int32 * 8)()
+        ch = (c_int32 * 9)()
         self._check(self.lib.md_autotune(self.h, iters, ch), "md_autotune")
         return {"fused_planes": ch[0], "z_lines": ch[1], "real_workgroup": ch[2], "real_shared_launch": ch[3],
-                "real_grid": ch[4], "plane_spreading": ch[5], "ride_parts": ch[6], "ride_lead": ch[7]}
+                "real_grid": ch[4], "plane_spreading": ch[5], "ride_parts": ch[6], "ride_lead": ch[7],
+                "ride_lead_gather": ch[8]}
 
     def time_kernels(self, iters=20):
         """Mean ms per launch of each kernel over all replicas (HIP events);

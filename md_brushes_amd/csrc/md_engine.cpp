@@ -792,7 +792,8 @@ struct md_engine {
   int tune_rgrid = 1024;           // workgroups of the real-space sweep's workgroup form (cap)
   int tune_pspread = 1;            // spreading inside the forward plane kernel: 0 no, 1 yes
   int tune_rparts = 3;             // with plane spreading: launches the real-space sweep rides in
-  int tune_lead = 0;               // riding sweep workgroups dispatched ahead of a launch's own
+  int tune_lead = 0;               // riding sweep workgroups dispatched ahead of the plane launch's own
+  int tune_lead_g = 0;             // ... ahead of the gather launch's own
   std::string err;
 
   ~md_engine() {
@@ -1025,6 +1026,8 @@ int md_engine::rebuild() {
     P.ride_parts = std::max(1, std::min(3, rp ? std::atoi(rp) : tune_rparts));
     const char *rl = std::getenv("MD_RIDE_LEAD");  // forces the riders dispatched first
     P.ride_lead = rl ? std::max(0, std::atoi(rl)) : tune_lead;
+    const char *rg2 = std::getenv("MD_RIDE_LEAD_G");
+    P.ride_lead_g = rg2 ? std::max(0, std::atoi(rg2)) : tune_lead_g;
     // MD_TWO_STREAMS=1: the real-space sweep beside the SPME kernels on a
     // second stream (the same results; slower in graph replay at the
     // default brush, so off unless asked for)
@@ -1823,7 +1826,8 @@ extern "C" int md_autotune(md_engine *e, int32_t iters, int32_t *choice) {
   };
   Knob knobs[] = {{&e->tune_fuse, {0, 1}}, {&e->tune_pz, {2, 4, 8, 16}}, {&e->tune_real, {0, 1}},
                   {&e->tune_ride, {0, 1}}, {&e->tune_rgrid, {1024, 2048}}, {&e->tune_pspread, {0, 1}},
-                  {&e->tune_rparts, {1, 2, 3}}, {&e->tune_lead, {0, 256, 512, 768}}};
+                  {&e->tune_rparts, {1, 2, 3}}, {&e->tune_lead, {0, 256, 512, 768}},
+                  {&e->tune_lead_g, {0, 512, 1024, 65536}}};
   for (Knob &k : knobs) {
     int best = *k.field;
     double best_ms = 1e30;
@@ -1853,6 +1857,7 @@ extern "C" int md_autotune(md_engine *e, int32_t iters, int32_t *choice) {
     choice[5] = e->tune_pspread;
     choice[6] = e->tune_rparts;
     choice[7] = e->tune_lead;
+    choice[8] = e->tune_lead_g;
   }
   return rc;
 }

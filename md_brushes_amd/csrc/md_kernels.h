@@ -63,7 +63,8 @@ struct Plan {
   bool ride = true;         // sweep workgroups inside the spreading and gather launches
   bool pspread = false;     // spreading inside the forward plane kernel (k_plane_spread)
   int ride_parts = 3;       // with plane spreading: launches the sweep's items ride in (1..3)
-  int ride_lead = 0;  // riding sweep workgroups dispatched before the launch's own
+  int ride_lead = 0;    // riding sweep workgroups dispatched before the plane launch's own
+  int ride_lead_g = 0;  // ... before the gather launch's own
   // second stream of a Coulomb evaluation (the real-space sweep runs there,
   // beside the SPME kernels) and its fork / join events; null: one stream
   hipStream_t st2 = nullptr;

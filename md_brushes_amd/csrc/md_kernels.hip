@@ -3571,10 +3571,10 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev, 
   const int nparts = P.pspread ? rparts : 2;  // the gather launch takes the last part
   if (ride && P.real_wg)
     hipLaunchKernelGGL((k_gather_real<R, true>), dim3(P.chgG.total + nr), dim3(kBlock), 0, st, P.reps,
-                       P.chgG, P.cell3c, nparts, std::min(P.ride_lead, nr));
+                       P.chgG, P.cell3c, nparts, std::min(P.ride_lead_g, nr));
   else if (ride)
     hipLaunchKernelGGL((k_gather_real<R, false>), dim3(P.chgG.total + nr), dim3(kBlock), 0, st, P.reps,
-                       P.chgG, P.cell3c, nparts, std::min(P.ride_lead, nr));
+                       P.chgG, P.cell3c, nparts, std::min(P.ride_lead_g, nr));
   else
     hipLaunchKernelGGL(k_gather<R>, dim3(P.chgG.total), dim3(kBlock), 0, st, P.reps, P.chgG);
   mark();

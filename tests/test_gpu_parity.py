@@ -342,7 +342,7 @@ def test_launch_forms_do_not_change_results(prec, monkeypatch):
 
     def coulomb(env):
         for k in ("MD_FFT_PLANES", "MD_Z_LINES", "MD_REAL_SWEEP", "MD_RIDE", "MD_REAL_GRID", "MD_PLANE_SPREAD",
-                  "MD_RIDE_PARTS", "MD_RIDE_LEAD"):
+                  "MD_RIDE_PARTS", "MD_RIDE_LEAD", "MD_RIDE_LEAD_G"):
             monkeypatch.delenv(k, raising=False)
         for k, v in env.items():
             monkeypatch.setenv(k, v)
@@ -364,7 +364,8 @@ def test_launch_forms_do_not_change_results(prec, monkeypatch):
                 {"MD_PLANE_SPREAD": "1", "MD_RIDE_PARTS": "1"}, {"MD_PLANE_SPREAD": "1", "MD_RIDE_PARTS": "2"},
                 {"MD_PLANE_SPREAD": "1", "MD_RIDE": "1", "MD_RIDE_PARTS": "2", "MD_RIDE_LEAD": "512"},
                 {"MD_PLANE_SPREAD": "1", "MD_RIDE": "1", "MD_REAL_SWEEP": "wave", "MD_RIDE_LEAD": "768"},
-                {"MD_PLANE_SPREAD": "0", "MD_RIDE": "1", "MD_RIDE_LEAD": "256"}):
+                {"MD_PLANE_SPREAD": "0", "MD_RIDE": "1", "MD_RIDE_LEAD": "256"},
+                {"MD_PLANE_SPREAD": "1", "MD_RIDE": "1", "MD_RIDE_LEAD": "512", "MD_RIDE_LEAD_G": "65536"}):
         e, f = coulomb(env)
         e.close()
         assert np.array_equal(f, ref), env
@@ -373,7 +374,7 @@ def test_launch_forms_do_not_change_results(prec, monkeypatch):
     assert choice["z_lines"] in (2, 4, 8, 16) and choice["fused_planes"] in (0, 1)
     assert choice["real_shared_launch"] in (0, 1) and choice["real_grid"] in (1024, 2048)
     assert choice["plane_spreading"] in (0, 1) and choice["ride_parts"] in (1, 2, 3)
-    assert choice["ride_lead"] in (0, 256, 512, 768)
+    assert choice["ride_lead"] in (0, 256, 512, 768) and choice["ride_lead_gather"] in (0, 512, 1024, 65536)
     assert np.array_equal(e.force_parts()[1], ref)
     e.close()
 

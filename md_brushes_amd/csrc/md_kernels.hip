@@ -3104,10 +3104,11 @@ __global__ __launch_bounds__(kPlaneBlock) void k_plane_spread_real(const Rep *__
 
 template <typename R>
 __global__ __launch_bounds__(kPlaneBlock) void k_fftz_real(const Rep *__restrict__ reps, WMap wz,
-                                                           WMap wr, int nparts) {
+                                                           WMap wr, int nparts, int lead) {
   extern __shared__ char smem[];
-  if ((int)blockIdx.x < wz.total) fftz_block<R>(reps, wz, blockIdx.x, smem);
-  else real_cells_block<R, true, kPlaneBlock>(reps, wr, (int)blockIdx.x - wz.total, smem, 1, nparts);
+  const int b = blockIdx.x;  // riders 0..lead-1, the z-pass workgroups, the other riders
+  if (b >= lead && b < lead + wz.total) fftz_block<R>(reps, wz, b - lead, smem);
+  else real_cells_block<R, true, kPlaneBlock>(reps, wr, b < lead ? b : b - wz.total, smem, 1, nparts);
 }
 
 // ---------------------------------------------------- rescale_velocity
@@ -3555,7 +3556,8 @@ void launch_coulomb(const Plan &P, hipStream_t st, double *out, hipEvent_t *ev, 
   // 512 lanes: the strided line loads and stores spread over more wavefronts
   if (rparts == 3)
     hipLaunchKernelGGL(k_fftz_real<R>, dim3(P.fz.total + nr), dim3(kPlaneBlock),
-                       P.lds_z > real512 ? P.lds_z : real512, st, P.reps, P.fz, P.cell3c, rparts);
+                       P.lds_z > real512 ? P.lds_z : real512, st, P.reps, P.fz, P.cell3c, rparts,
+                       std::min(P.ride_lead, nr));
   else
     hipLaunchKernelGGL(k_fftz_conv<R>, dim3(P.fz.total), dim3(kPlaneBlock), P.lds_z, st, P.reps, P.fz);
   mark();

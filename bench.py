@@ -244,7 +244,7 @@ def pmc_workload(seed):
 
 
 KERNEL_SYMBOLS = {  # kernel names of time_kernels -> device symbols (FP32 path)
-    "kick_drift": "k_kick_drift", "short_range": "k_short_range<float>", "bin_q": "k_bin_q",
+    "kick_drift": "k_kick_drift", "short_range": "k_short_range<float, true>", "bin_q": "k_bin_q",
     "sort_q": "k_sort_q<float>", "spread": "k_spread<float>", "fft_x_fwd": "k_plane_fwd<float>",
     "plane_spread": "k_plane_spread<float>",
     "fft_z_conv": "k_fftz_conv<float>", "fft_x_inv": "k_plane_inv<float>", "gather": "k_gather<float>",
